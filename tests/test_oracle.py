@@ -1,0 +1,297 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each check is chosen so a plausible slip (dropped term, wrong index, swapped
+operand, transposed matrix) fails something that is not the oracle's own
+formula retyped: hashlib for FIPS 202, the reference's Saber test vectors
+(SABER-GPU/include/tv.h), numpy brute force at full size, closed forms,
+round trips and tamper behaviour.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+FRODO, SABER = 0, 1
+
+
+# ---------------------------------------------------------------- FIPS 202
+# (inlen, outlen) pairs the two KEMs use, plus rate boundaries.
+SHAKE_LENS = [0, 1, 17, 18, 25, 32, 48, 135, 136, 137, 167, 168, 169, 992, 1088, 15632, 15768]
+OUT_LENS = [16, 24, 32, 48, 64, 136, 168, 768, 1952, 3744, 31232, 31360]
+
+
+@pytest.mark.parametrize("inlen", SHAKE_LENS)
+def test_fips202_vs_hashlib(oracle, inlen):
+    rng = np.random.default_rng(inlen)
+    d = rng.integers(0, 256, inlen, dtype=np.uint8).tobytes()
+    for outlen in OUT_LENS:
+        assert oracle.shake128(d, outlen) == hashlib.shake_128(d).digest(outlen)
+        assert oracle.shake256(d, outlen) == hashlib.shake_256(d).digest(outlen)
+    assert oracle.sha3_256(d) == hashlib.sha3_256(d).digest()
+    assert oracle.sha3_512(d) == hashlib.sha3_512(d).digest()
+
+
+def test_keccak_f1600_zero_state(oracle):
+    # Keccak team's published intermediate value: lane 0 after one
+    # permutation of the all-zero state.
+    st = oracle.keccak_f1600(np.zeros(25, np.uint64))
+    assert int(st[0]) == 0xF1258F7940E1DDE7
+
+
+# ---------------------------------------------------------------- PRNG
+def _py_xoshiro(seed, n):
+    M = 2**64 - 1
+    x = seed
+    s = []
+    for _ in range(4):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        s.append(z ^ (z >> 31))
+    rotl = lambda v, k: ((v << k) | (v >> (64 - k))) & M
+    out = bytearray()
+    while len(out) < n:
+        r = (rotl((s[1] * 5) & M, 7) * 9) & M
+        t = (s[1] << 17) & M
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t
+        s[3] = rotl(s[3], 45)
+        out += r.to_bytes(8, "little")
+    return bytes(out[:n])
+
+
+def test_prng_matches_python(oracle):
+    for seed in (0, 1, 0x44504B454D000000 + 2):
+        assert oracle.prng_bytes(seed, 203).tobytes() == _py_xoshiro(seed, 203)
+
+
+# ---------------------------------------------------------------- Saber vs tv.h
+@pytest.fixture(scope="module")
+def tv():
+    with open(os.path.join(GOLDEN, "saber_tv.json")) as f:
+        d = json.load(f)
+    return dict(
+        A=np.array(d["A"], np.uint16), sp=np.array(d["sp"], np.uint16),
+        pk=bytes.fromhex(d["pk"]), m=bytes(d["m_tv"]), sk=bytes.fromhex(d["sk_tv"]),
+        buf=bytes.fromhex(d["buf_tv"]),
+    )
+
+
+def test_saber_tv_genmatrix(oracle, tv):
+    assert np.array_equal(oracle.saber_gen_matrix(tv["pk"][960:992]), tv["A"])
+
+
+def test_saber_tv_message_and_gensecret(oracle, tv):
+    assert oracle.sha3_256(tv["buf"][:32]) == tv["m"]
+    kr = oracle.sha3_512(tv["m"] + oracle.sha3_256(tv["pk"]))
+    assert np.array_equal(oracle.saber_gen_secret(kr[32:]), tv["sp"])
+
+
+def test_saber_tv_secret_key_layout(oracle, tv):
+    sk, pk = tv["sk"], tv["pk"]
+    assert sk[1248:2240] == pk
+    assert sk[2240:2272] == hashlib.sha3_256(pk).digest()
+    s = oracle.saber_unpack13(sk[:1248]).astype(np.int64)
+    s = ((s + 4096) % 8192) - 4096
+    assert s.min() >= -4 and s.max() <= 4  # CBD(mu=8) support
+    assert oracle.saber_pack13(oracle.saber_unpack13(sk[:1248])) == sk[:1248]
+
+
+def test_saber_tv_keygen_relation(oracle, tv):
+    """pk.b = pack10((A^T s + h1) >> 3): pins transposition, rounding, packers."""
+    s = oracle.saber_unpack13(tv["sk"][:1248])
+    b = oracle.saber_matvec(tv["A"], s, transpose=True)
+    b = ((b.astype(np.uint32) + 4) >> 3).astype(np.uint16)
+    assert oracle.saber_pack10(b) == tv["pk"][:960]
+    # the non-transposed product must NOT match (guards a swapped index)
+    b2 = oracle.saber_matvec(tv["A"], s, transpose=False)
+    b2 = ((b2.astype(np.uint32) + 4) >> 3).astype(np.uint16)
+    assert oracle.saber_pack10(b2) != tv["pk"][:960]
+
+
+def test_saber_tv_encaps_decaps(oracle, tv):
+    pk = np.frombuffer(tv["pk"], np.uint8)[None]
+    sk = np.frombuffer(tv["sk"], np.uint8)[None]
+    ct, ss = oracle.enc(SABER, pk, np.frombuffer(tv["buf"][:32], np.uint8)[None])
+    # derived (not reference-held) cross-check values recorded at survey time
+    assert hashlib.sha3_256(ct.tobytes()).hexdigest().startswith("dd05a10fcf07f1b2")
+    assert ss.tobytes().hex().startswith("c842a1e43b3a3fe8")
+    assert (oracle.dec(SABER, ct, sk) == ss).all()
+
+
+def _negacyclic(a, b):
+    a = a.astype(np.int64); b = b.astype(np.int64)
+    full = np.convolve(a, b)  # length 511
+    res = full[:256].copy()
+    res[: 255] -= full[256:]
+    return (res % 65536).astype(np.uint16)
+
+
+def test_saber_polymul_vs_numpy(oracle):
+    rng = np.random.default_rng(7)
+    for _ in range(4):
+        a = rng.integers(0, 8192, 256).astype(np.uint16)
+        b = rng.integers(-4, 5, 256).astype(np.int64) % 65536
+        assert np.array_equal(oracle.saber_polymul(a, b.astype(np.uint16)), _negacyclic(a, b))
+
+
+def test_saber_packers_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    v = rng.integers(0, 8192, 768).astype(np.uint16)
+    b13 = oracle.saber_pack13(v)
+    bits = np.unpackbits(np.frombuffer(b13, np.uint8), bitorder="little").reshape(-1, 13)
+    assert np.array_equal((bits * (1 << np.arange(13))).sum(1), v)
+    v10 = v & 1023
+    b10 = oracle.saber_pack10(v10)
+    bits = np.unpackbits(np.frombuffer(b10, np.uint8), bitorder="little").reshape(-1, 10)
+    assert np.array_equal((bits * (1 << np.arange(10))).sum(1), v10)
+    assert np.array_equal(oracle.saber_unpack10(b10), v10)
+
+
+def test_saber_gensecret_cbd_closed_form(oracle):
+    seed = bytes(range(32))
+    buf = hashlib.shake_128(seed).digest(768)
+    exp = [bin(x & 15).count("1") - bin(x >> 4).count("1") for x in buf]
+    got = oracle.saber_gen_secret(seed).astype(np.int16)
+    assert got.tolist() == exp
+
+
+def test_saber_roundtrip_and_tamper(oracle):
+    n = 16
+    coins = oracle.prng_bytes(11, n * 96).reshape(n, 96)
+    pk, sk = oracle.keypair(SABER, coins)
+    assert (sk[:, 1248:2240] == pk).all()
+    for i in range(n):
+        assert sk[i, 2240:2272].tobytes() == hashlib.sha3_256(pk[i].tobytes()).digest()
+        assert sk[i, 2272:].tobytes() == coins[i, 64:].tobytes()
+        assert pk[i, 960:].tobytes() == hashlib.shake_128(coins[i, :32].tobytes()).digest(32)
+    m = oracle.prng_bytes(12, n * 32).reshape(n, 32)
+    ct, ss = oracle.enc(SABER, pk, m)
+    assert (oracle.dec(SABER, ct, sk) == ss).all()
+    bad = ct.copy(); bad[:, 5] ^= 1
+    ss_bad = oracle.dec(SABER, bad, sk)
+    for i in range(n):
+        z = sk[i, 2272:].tobytes()
+        exp = hashlib.sha3_256(z + hashlib.sha3_256(bad[i].tobytes()).digest()).digest()
+        assert ss_bad[i].tobytes() == exp
+
+
+# ---------------------------------------------------------------- Frodo
+CDF = [5638, 15915, 23689, 28571, 31116, 32217, 32613, 32731, 32760, 32766, 32767]
+
+
+def _frodo_sample_np(r):
+    r = np.asarray(r, np.int64)
+    t = r >> 1
+    mag = np.searchsorted(np.array(CDF[:10]), t, side="left")  # #{j: CDF[j] < t}
+    return np.where(r & 1, (-mag) % 65536, mag).astype(np.uint16)
+
+
+def test_frodo_sampler_exhaustive(oracle):
+    r = np.arange(65536, dtype=np.uint16)
+    got = oracle.frodo_sample(r)
+    assert np.array_equal(got, _frodo_sample_np(r))
+    signed = got.astype(np.int16)
+    assert signed.min() == -10 and signed.max() == 10
+    # the CDF table fixes the distribution: #{r : |e(r)| <= k} = 2 * (CDF[k] + 1)
+    hist = np.bincount(np.abs(signed.astype(np.int64)), minlength=11)
+    assert np.array_equal(np.cumsum(hist)[:10], 2 * (np.array(CDF[:10]) + 1))
+    assert hist.sum() == 65536
+
+
+def test_frodo_gen_a_row_vs_hashlib(oracle):
+    seed = bytes(range(16))
+    for i in (0, 1, 255, 256, 975):
+        exp = np.frombuffer(hashlib.shake_128(i.to_bytes(2, "little") + seed).digest(1952), "<u2")
+        assert np.array_equal(oracle.frodo_gen_a_row(seed, i), exp)
+
+
+def test_frodo_pack16_is_big_endian(oracle):
+    v = np.arange(0, 65536, 257, dtype=np.uint16)
+    assert oracle.frodo_pack16(v) == v.astype(">u2").tobytes()
+    assert np.array_equal(oracle.frodo_unpack16(v.astype(">u2").tobytes()), v)
+
+
+def test_frodo_encode_decode(oracle):
+    rng = np.random.default_rng(5)
+    for _ in range(8):
+        mu = rng.integers(0, 256, 24, dtype=np.uint8).tobytes()
+        enc = oracle.frodo_encode(mu)
+        bits = np.unpackbits(np.frombuffer(mu, np.uint8), bitorder="little").reshape(64, 3)
+        assert np.array_equal(enc >> 13, (bits * [1, 2, 4]).sum(1))
+        assert np.all(enc & 0x1FFF == 0)
+        noise = rng.integers(-4095, 4096, 64)
+        assert oracle.frodo_decode(((enc.astype(np.int64) + noise) % 65536).astype(np.uint16)) == mu
+
+
+def _frodo_a(seed):
+    return np.stack([np.frombuffer(hashlib.shake_128(i.to_bytes(2, "little") + seed).digest(1952), "<u2")
+                     for i in range(976)]).astype(np.int64)
+
+
+def test_frodo_matrix_products_vs_numpy(oracle):
+    rng = np.random.default_rng(9)
+    seed = rng.integers(0, 256, 16, dtype=np.uint8).tobytes()
+    A = _frodo_a(seed)
+    st = (rng.integers(-10, 11, (8, 976)) % 65536).astype(np.uint16)
+    e = (rng.integers(-10, 11, (976, 8)) % 65536).astype(np.uint16)
+    got = oracle.frodo_as_plus_e(st, e, seed).reshape(976, 8)
+    exp = (A @ st.T.astype(np.int64) + e) % 65536
+    assert np.array_equal(got, exp)
+    got2 = oracle.frodo_sa_plus_e(st, e.T.copy(), seed).reshape(8, 976)
+    exp2 = (st.astype(np.int64) @ A + e.T) % 65536
+    assert np.array_equal(got2, exp2)
+
+
+def test_frodo_keygen_vs_numpy_restatement(oracle):
+    coins = oracle.prng_bytes(21, 64)
+    pk, sk = oracle.keypair(FRODO, coins[None])
+    pk, sk = pk[0].tobytes(), sk[0].tobytes()
+    s, seed_se, z = coins[:24].tobytes(), coins[24:48].tobytes(), coins[48:].tobytes()
+    seed_a = hashlib.shake_256(z).digest(16)
+    r = np.frombuffer(hashlib.shake_256(b"\x5f" + seed_se).digest(31232), "<u2")
+    st = _frodo_sample_np(r[:7808]).reshape(8, 976)
+    e = _frodo_sample_np(r[7808:]).reshape(976, 8)
+    B = (_frodo_a(seed_a) @ st.T.astype(np.int64) + e) % 65536
+    assert pk == seed_a + B.astype(">u2").tobytes()
+    assert sk == s + pk + st.astype("<u2").tobytes() + hashlib.shake_256(pk).digest(24)
+
+
+def test_frodo_encaps_vs_numpy_restatement(oracle):
+    coins = oracle.prng_bytes(22, 64)
+    pk, _ = oracle.keypair(FRODO, coins[None])
+    mu = oracle.prng_bytes(23, 24)
+    ct, ss = oracle.enc(FRODO, pk, mu[None])
+    pkb = pk[0].tobytes()
+    pkh = hashlib.shake_256(pkb).digest(24)
+    g2 = hashlib.shake_256(pkh + mu.tobytes()).digest(48)
+    r = np.frombuffer(hashlib.shake_256(b"\x96" + g2[:24]).digest(31360), "<u2")
+    smp = _frodo_sample_np(r)
+    sp, ep, epp = smp[:7808].reshape(8, 976), smp[7808:15616].reshape(8, 976), smp[15616:].reshape(8, 8)
+    A = _frodo_a(pkb[:16])
+    Bp = (sp.astype(np.int64) @ A + ep) % 65536
+    B = np.frombuffer(pkb[16:], ">u2").reshape(976, 8).astype(np.int64)
+    bits = np.unpackbits(mu, bitorder="little").reshape(64, 3)
+    enc = ((bits * [1, 2, 4]).sum(1) << 13).reshape(8, 8)
+    C = (sp.astype(np.int64) @ B + epp + enc) % 65536
+    exp_ct = Bp.astype(">u2").tobytes() + C.astype(">u2").tobytes()
+    assert ct[0].tobytes() == exp_ct
+    assert ss[0].tobytes() == hashlib.shake_256(exp_ct + g2[24:]).digest(24)
+
+
+def test_frodo_roundtrip_and_tamper(oracle):
+    n = 3
+    coins = oracle.prng_bytes(31, n * 64).reshape(n, 64)
+    pk, sk = oracle.keypair(FRODO, coins, threads=3)
+    mu = oracle.prng_bytes(32, n * 24).reshape(n, 24)
+    ct, ss = oracle.enc(FRODO, pk, mu, threads=3)
+    assert (oracle.dec(FRODO, ct, sk, threads=3) == ss).all()
+    for pos in (0, 15700):  # one flip in B', one in C
+        bad = ct.copy(); bad[:, pos] ^= 0x40
+        ss_bad = oracle.dec(FRODO, bad, sk, threads=3)
+        for i in range(n):
+            exp = hashlib.shake_256(bad[i].tobytes() + sk[i, :24].tobytes()).digest(24)
+            assert ss_bad[i].tobytes() == exp
