@@ -1,0 +1,392 @@
+// Host engine and C ABI for the batched KEM kernels.
+//
+// Replaces the reference's host drivers (FrodoKEM976-GPU/src/cuda_kernel.cu:21-335,
+// SABER-GPU/src/cuda_kernel.cu:19-299), which allocate and free device memory
+// on every call, run on the legacy default stream with synchronous copies and
+// report nothing.  Here an engine owns one stream and a grow-only scratch
+// arena bounded by an HBM budget; batches larger than the budget allows are
+// cut into chunks; host pointers are staged through device buffers; every HIP
+// error becomes a return code.
+#include <hip/hip_runtime.h>
+#include <string.h>
+#include <sys/random.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/dpkem.h"
+#include "launch.hpp"
+
+using namespace dpkem;
+
+namespace {
+
+struct SchemeSizes {
+  size_t pk, sk, ct, ss, kcoins, ecoins;
+};
+
+const SchemeSizes kSizes[2] = {
+    {15632, 31296, 15744, 24, 64, 24},  // FrodoKEM-976-SHAKE
+    {992, 2304, 1088, 32, 96, 32},      // Saber
+};
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+struct Carver {
+  uint8_t* base;
+  size_t off = 0;
+  uint8_t* take(size_t bytes) {
+    uint8_t* p = base + off;
+    off += align_up(bytes);
+    return p;
+  }
+};
+
+}  // namespace
+
+struct dpkem_engine {
+  int device = 0;
+  size_t budget = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int form = DPKEM_FORM_BATCHED;
+  uint8_t* arena = nullptr;
+  size_t arena_bytes = 0;
+  char last_err[256] = {0};
+  std::mutex mu;
+};
+
+namespace {
+
+int fail_hip(dpkem_engine* e, hipError_t err, const char* what) {
+  if (err == hipSuccess) return DPKEM_OK;
+  if (e) snprintf(e->last_err, sizeof e->last_err, "%s: %s", what, hipGetErrorString(err));
+  return err == hipErrorOutOfMemory ? DPKEM_ENOMEM : DPKEM_EDEVICE;
+}
+
+#define HIP_TRY(e, call)                                  \
+  do {                                                    \
+    hipError_t _err = (call);                             \
+    if (_err != hipSuccess) return fail_hip(e, _err, #call); \
+  } while (0)
+
+int ensure_arena(dpkem_engine* e, size_t bytes) {
+  if (bytes <= e->arena_bytes) return DPKEM_OK;
+  if (bytes > e->budget) {
+    snprintf(e->last_err, sizeof e->last_err, "scratch of %zu B exceeds HBM budget %zu B", bytes, e->budget);
+    return DPKEM_EBUDGET;
+  }
+  if (e->arena) {
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    HIP_TRY(e, hipFree(e->arena));
+    e->arena = nullptr;
+    e->arena_bytes = 0;
+  }
+  HIP_TRY(e, hipMalloc(&e->arena, bytes));
+  e->arena_bytes = bytes;
+  return DPKEM_OK;
+}
+
+// Bytes of device scratch one instance needs for (scheme, op) in the current
+// form, excluding staged I/O.
+size_t scratch_per_inst(int scheme) {
+  return scheme == DPKEM_FRODO976 ? align_up(kFrodoScratchPerInst) : align_up(kSaberScratchPerInst);
+}
+
+enum Op { kKeygen = 0, kEnc = 1, kDec = 2 };
+
+// I/O byte sizes per instance: two outputs and two inputs (0 if unused).
+void io_sizes(int scheme, int op, size_t out[2], size_t in[2]) {
+  const SchemeSizes& s = kSizes[scheme];
+  if (op == kKeygen) {
+    out[0] = s.pk; out[1] = s.sk; in[0] = s.kcoins; in[1] = 0;
+  } else if (op == kEnc) {
+    out[0] = s.ct; out[1] = s.ss; in[0] = s.pk; in[1] = s.ecoins;
+  } else {
+    out[0] = s.ss; out[1] = 0; in[0] = s.ct; in[1] = s.sk;
+  }
+}
+
+// Launch one chunk with all pointers on the device.
+void launch_chunk(dpkem_engine* e, int scheme, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+                  const uint8_t* i1, uint8_t* scratch) {
+  Carver c{scratch};
+  hipStream_t st = e->stream;
+  if (scheme == DPKEM_SABER) {
+    uint8_t* araw = c.take((size_t)n * 3744);
+    uint8_t* sraw = c.take((size_t)n * 768);
+    uint8_t* mbuf = c.take((size_t)n * 32);
+    uint8_t* krbuf = c.take((size_t)n * 64);
+    uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
+    if (op == kKeygen) saber_keygen_launch(st, n, i0, o0, o1, araw, sraw);
+    else if (op == kEnc) saber_enc_launch(st, n, i0, i1, o0, o1, araw, sraw, mbuf, krbuf);
+    else saber_dec_launch(st, n, i0, i1, o0, araw, sraw, mbuf, krbuf, fail);
+  } else {
+    uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
+    uint32_t* e8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
+    uint32_t* epp8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 64));
+    uint8_t* kbuf = c.take((size_t)n * 64);
+    uint8_t* mubuf = c.take((size_t)n * 32);
+    uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
+    if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8);
+    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf);
+    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail);
+  }
+}
+
+// Upper bound on the arena bytes the chunk carver uses for n instances
+// (per-array alignment padding included).
+size_t chunk_scratch_bytes(int scheme, size_t n) { return n * scratch_per_inst(scheme) + 8 * kAlign; }
+
+int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+           const uint8_t* i1, int mem) {
+  if (!e || (scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER)) return DPKEM_EINVAL;
+  if (mem != DPKEM_MEM_HOST && mem != DPKEM_MEM_DEVICE) return DPKEM_EINVAL;
+  if (e->form != DPKEM_FORM_BATCHED) return DPKEM_EINVAL;  // plain form: see dpkem_plain_*
+  if (n == 0) return DPKEM_OK;
+  size_t out[2], in[2];
+  io_sizes(scheme, op, out, in);
+  uint8_t* outs[2] = {o0, o1};
+  const uint8_t* ins[2] = {i0, i1};
+  for (int k = 0; k < 2; ++k) {
+    if (out[k] && !outs[k]) return DPKEM_EINVAL;
+    if (in[k] && !ins[k]) return DPKEM_EINVAL;
+    if (mem == DPKEM_MEM_DEVICE && ((out[k] && ((uintptr_t)outs[k] & 7)) || (in[k] && ((uintptr_t)ins[k] & 7)))) {
+      snprintf(e->last_err, sizeof e->last_err, "device pointers must be 8-byte aligned");
+      return DPKEM_EINVAL;
+    }
+  }
+  std::lock_guard<std::mutex> lock(e->mu);
+  HIP_TRY(e, hipSetDevice(e->device));
+  const size_t io_per = mem == DPKEM_MEM_HOST ? align_up(out[0]) + align_up(out[1]) + align_up(in[0]) + align_up(in[1]) : 0;
+  const size_t per = scratch_per_inst(scheme) + io_per;
+  size_t chunk = std::min(n, (size_t)INT32_MAX);
+  while (chunk > 1 && chunk * per + 16 * kAlign > e->budget) chunk = (chunk + 1) / 2;
+  if (chunk * per + 16 * kAlign > e->budget) return DPKEM_EBUDGET;
+  int rc = ensure_arena(e, chunk * per + 16 * kAlign);
+  if (rc) return rc;
+  for (size_t done = 0; done < n; done += chunk) {
+    const int c = (int)std::min(chunk, n - done);
+    uint8_t* dout[2];
+    const uint8_t* din[2];
+    Carver io{e->arena + align_up(chunk_scratch_bytes(scheme, chunk))};
+    for (int k = 0; k < 2; ++k) {
+      if (mem == DPKEM_MEM_DEVICE) {
+        dout[k] = out[k] ? outs[k] + done * out[k] : nullptr;
+        din[k] = in[k] ? ins[k] + done * in[k] : nullptr;
+      } else {
+        dout[k] = out[k] ? io.take((size_t)c * out[k]) : nullptr;
+        uint8_t* d = in[k] ? io.take((size_t)c * in[k]) : nullptr;
+        if (d) HIP_TRY(e, hipMemcpyAsync(d, ins[k] + done * in[k], (size_t)c * in[k], hipMemcpyHostToDevice, e->stream));
+        din[k] = d;
+      }
+    }
+    launch_chunk(e, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
+    HIP_TRY(e, hipGetLastError());
+    if (mem == DPKEM_MEM_HOST) {
+      for (int k = 0; k < 2; ++k)
+        if (out[k])
+          HIP_TRY(e, hipMemcpyAsync(outs[k] + done * out[k], dout[k], (size_t)c * out[k], hipMemcpyDeviceToHost, e->stream));
+      HIP_TRY(e, hipStreamSynchronize(e->stream));
+    }
+  }
+  return DPKEM_OK;
+}
+
+// ---- randomness for the PQClean-shaped and reference-shaped wrappers
+std::mutex g_rng_mu;
+dpkem_randombytes_fn g_rng = nullptr;
+void* g_rng_ctx = nullptr;
+
+void os_randombytes(uint8_t* out, size_t len, void*) {
+  while (len > 0) {
+    ssize_t r = getrandom(out, len, 0);
+    if (r <= 0) continue;
+    out += r;
+    len -= (size_t)r;
+  }
+}
+
+void randombytes(uint8_t* out, size_t len) {
+  std::lock_guard<std::mutex> lock(g_rng_mu);
+  (g_rng ? g_rng : os_randombytes)(out, len, g_rng_ctx);
+}
+
+std::mutex g_default_mu;
+dpkem_engine* g_default = nullptr;
+
+dpkem_engine* default_engine() {
+  std::lock_guard<std::mutex> lock(g_default_mu);
+  if (!g_default && dpkem_engine_create(&g_default, 0, 0) != DPKEM_OK) g_default = nullptr;
+  return g_default;
+}
+
+int wrapped(int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1,
+            int form = DPKEM_FORM_BATCHED) {
+  dpkem_engine* e = default_engine();
+  if (!e) return DPKEM_EDEVICE;
+  const SchemeSizes& s = kSizes[scheme];
+  std::vector<uint8_t> coins;
+  if (op == kKeygen) {
+    coins.resize(n * s.kcoins);
+    randombytes(coins.data(), coins.size());
+    i0 = coins.data();
+  } else if (op == kEnc) {
+    coins.resize(n * s.ecoins);
+    randombytes(coins.data(), coins.size());
+    i1 = coins.data();
+  }
+  const int prev = e->form;
+  e->form = form;
+  const int rc = run_op(e, scheme, op, n, o0, o1, i0, i1, DPKEM_MEM_HOST);
+  e->form = prev;
+  return rc;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int dpkem_sizes(int scheme, size_t* pk, size_t* sk, size_t* ct, size_t* ss, size_t* kc, size_t* ec) {
+  if (scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER) return DPKEM_EINVAL;
+  const SchemeSizes& s = kSizes[scheme];
+  if (pk) *pk = s.pk;
+  if (sk) *sk = s.sk;
+  if (ct) *ct = s.ct;
+  if (ss) *ss = s.ss;
+  if (kc) *kc = s.kcoins;
+  if (ec) *ec = s.ecoins;
+  return DPKEM_OK;
+}
+
+int dpkem_engine_create(dpkem_engine** out, int device, size_t budget) {
+  if (!out) return DPKEM_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return DPKEM_EDEVICE;
+  dpkem_engine* e = new (std::nothrow) dpkem_engine();
+  if (!e) return DPKEM_ENOMEM;
+  e->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return DPKEM_EDEVICE;
+  }
+  e->stream = e->own_stream;
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+    budget = free_b / 4;
+  }
+  e->budget = budget;
+  *out = e;
+  return DPKEM_OK;
+}
+
+int dpkem_engine_destroy(dpkem_engine* e) {
+  if (!e) return DPKEM_EINVAL;
+  {
+    std::lock_guard<std::mutex> lock(e->mu);
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    if (e->arena) (void)hipFree(e->arena);
+    if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
+  }
+  delete e;
+  return DPKEM_OK;
+}
+
+int dpkem_engine_set_stream(dpkem_engine* e, void* s, int own) {
+  if (!e) return DPKEM_EINVAL;
+  std::lock_guard<std::mutex> lock(e->mu);
+  hipStream_t next = own ? e->own_stream : reinterpret_cast<hipStream_t>(s);
+  if (next == e->stream) return DPKEM_OK;
+  HIP_TRY(e, hipSetDevice(e->device));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));  // the arena may still be in use there
+  e->stream = next;
+  return DPKEM_OK;
+}
+
+int dpkem_engine_set_form(dpkem_engine* e, int form) {
+  if (!e || (form != DPKEM_FORM_BATCHED && form != DPKEM_FORM_PLAIN)) return DPKEM_EINVAL;
+  std::lock_guard<std::mutex> lock(e->mu);
+  e->form = form;
+  return DPKEM_OK;
+}
+
+int dpkem_engine_sync(dpkem_engine* e) {
+  if (!e) return DPKEM_EINVAL;
+  std::lock_guard<std::mutex> lock(e->mu);
+  HIP_TRY(e, hipSetDevice(e->device));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return DPKEM_OK;
+}
+
+const char* dpkem_engine_last_error(dpkem_engine* e) { return e ? e->last_err : "no engine"; }
+
+int dpkem_keypair(dpkem_engine* e, int scheme, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, int mem) {
+  return run_op(e, scheme, kKeygen, n, pk, sk, coins, nullptr, mem);
+}
+
+int dpkem_enc(dpkem_engine* e, int scheme, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk,
+              const uint8_t* coins, int mem) {
+  return run_op(e, scheme, kEnc, n, ct, ss, pk, coins, mem);
+}
+
+int dpkem_dec(dpkem_engine* e, int scheme, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, int mem) {
+  return run_op(e, scheme, kDec, n, ss, nullptr, ct, sk, mem);
+}
+
+int dpkem_set_randombytes(dpkem_randombytes_fn fn, void* ctx) {
+  std::lock_guard<std::mutex> lock(g_rng_mu);
+  g_rng = fn;
+  g_rng_ctx = ctx;
+  return DPKEM_OK;
+}
+
+int dpkem_frodo976_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
+  return wrapped(DPKEM_FRODO976, kKeygen, n, pk, sk, nullptr, nullptr);
+}
+int dpkem_frodo976_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  return wrapped(DPKEM_FRODO976, kEnc, n, ct, ss, pk, nullptr);
+}
+int dpkem_frodo976_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  return wrapped(DPKEM_FRODO976, kDec, n, ss, nullptr, ct, sk);
+}
+int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
+  return wrapped(DPKEM_SABER, kKeygen, n, pk, sk, nullptr, nullptr);
+}
+int dpkem_saber_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  return wrapped(DPKEM_SABER, kEnc, n, ct, ss, pk, nullptr);
+}
+int dpkem_saber_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  return wrapped(DPKEM_SABER, kDec, n, ss, nullptr, ct, sk);
+}
+
+int frodoKEMKeypair(uint8_t* h_pk, uint8_t* h_sk) {
+  if (!h_pk || !h_sk) return DPKEM_EINVAL;
+  return wrapped(DPKEM_FRODO976, kKeygen, DPKEM_REF_BATCH, h_pk, h_sk, nullptr, nullptr);
+}
+int frodoKEMEncap(uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  if (!ct || !ss || !pk) return DPKEM_EINVAL;
+  return wrapped(DPKEM_FRODO976, kEnc, DPKEM_REF_BATCH, ct, ss, pk, nullptr);
+}
+int frodoKEMDecap(uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  if (!ss || !ct || !sk) return DPKEM_EINVAL;
+  return wrapped(DPKEM_FRODO976, kDec, DPKEM_REF_BATCH, ss, nullptr, ct, sk);
+}
+int saber_enc(uint8_t mode, uint8_t* h_k, uint8_t* h_pk, uint8_t* h_c) {
+  if (!h_k || !h_pk || !h_c || mode > 1) return DPKEM_EINVAL;
+  return wrapped(DPKEM_SABER, kEnc, DPKEM_REF_BATCH, h_c, h_k, h_pk, nullptr,
+                 mode == 0 ? DPKEM_FORM_BATCHED : DPKEM_FORM_PLAIN);
+}
+int saber_dec(uint8_t mode, uint8_t* h_c, uint8_t* h_pk, uint8_t* h_sk, uint8_t* h_k) {
+  (void)h_pk;  // the public key is embedded in the secret key
+  if (!h_c || !h_sk || !h_k || mode > 1) return DPKEM_EINVAL;
+  return wrapped(DPKEM_SABER, kDec, DPKEM_REF_BATCH, h_k, nullptr, h_c, h_sk,
+                 mode == 0 ? DPKEM_FORM_BATCHED : DPKEM_FORM_PLAIN);
+}
+
+}  // extern "C"

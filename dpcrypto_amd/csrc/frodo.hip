@@ -1,0 +1,546 @@
+// FrodoKEM-976-SHAKE batched kernels for gfx950.
+//
+// Reference path replaced (benlwk/DPCrypto FrodoKEM976-GPU/):
+//   shake256_gpu (25-thread, shared memory)        src/shake.cu:495-604
+//   shake128_ser  A-row generation into HBM        src/shake.cu:619-645
+//   sample_n_gpu{,_small}                          src/sample_pack.cu:13-46
+//   as_plus_e_gpu / DoDP4Av5s / sa_plus_e_gpu      src/matrix_kernel.cu:13-241
+//   mul_add_sb_plus_e_gpu / mul_bs_gpu             src/matrix_kernel.cu:245-278
+//   pack / unpack / key_encode / key_decode        src/sample_pack.cu:55-198, src/encode_decode.cu
+//   copy_* / add / sub / ct_verify / ct_select     src/util.cu
+//   host drivers                                   src/cuda_kernel.cu:21-335
+//
+// Per op the engine runs
+//   pre    lane-per-instance SHAKE256 chains (pkh, G2, noise stream) with the
+//          CDF sampler fused into the squeeze; secrets and errors leave as int8;
+//   fused  one 1024-thread workgroup per instance; lane i squeezes row i of A
+//          (SHAKE128(LE16(i) || seedA)) in VGPRs and feeds it straight into
+//          v_mfma_i32_16x16x64_i8 as two byte limbs, so A never reaches memory:
+//            keygen  B  = A S + E       (reduction along the lane's own row)
+//            enc/dec B' = S' A + E'     (reduction across lanes: the row data is
+//                                        transposed by selector MFMAs first)
+//          followed by the epilogue (pack, V = S' B + E'', encode, compare);
+//   post   lane-per-instance SHAKE256 (pkh for keygen, ss for enc/dec).
+//
+// Limb split: a = 256 * hi + lo with hi = (int8)(a >> 8) and
+// lo = (int8)((a & 0xFF) ^ 0x80), so a == 256 hi + lo + 128 (mod 2^16);
+// the 128 * sum(S) term is added once per output.  With |S| <= 10 and 976
+// terms every i32 partial sum stays below 2^21 in magnitude.
+#include "common.hpp"
+#include "keccak.hpp"
+#include "sponge.hpp"
+
+namespace dpkem {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kFN = kFrodoN, kFNB = kFrodoNbar;
+constexpr int kFMat = kFN * kFNB;         // 7808 entries of S / E / S' / E' / B / B'
+constexpr int kFPkWords = kFrodoPk / 8;   // 1954
+constexpr int kFCtWords = kFrodoCt / 8;   // 1968
+constexpr int kFCtC1 = 2 * kFMat;         // 15616: offset of C in ct
+constexpr int kFSkS = kFrodoSec + kFrodoPk;  // 15656: offset of S^T in sk
+constexpr int kFSkPkh = kFrodoSk - kFrodoSec;  // 31272
+constexpr int kFUnits = kFN / 16;         // 61 MFMA K-units of 16 columns per row
+
+__constant__ static const uint16_t kFrodoCdf[10] = {5638, 15915, 23689, 28571, 31116,
+                                                    32217, 32613, 32731, 32760, 32766};
+
+// CDF sampler on one 16-bit draw (FrodoKEM976-GPU/src/sample_pack.cu:13-32):
+// magnitude = #{j < 10 : CDF[j] < r >> 1}, sign = r & 1.
+__device__ __forceinline__ int32_t frodo_sample(uint32_t r) {
+  const uint32_t t = (r >> 1) & 0x7FFF;
+  uint32_t e = 0;
+#pragma unroll
+  for (int j = 0; j < 10; ++j) e += ((uint32_t)kFrodoCdf[j] - t) >> 31;
+  const uint32_t s = r & 1;
+  return (int32_t)((e ^ (0u - s)) + s);
+}
+
+// Four samples from one squeezed word, packed as int8 bytes.
+__device__ __forceinline__ uint32_t sample4_i8(uint64_t w, uint64_t* le16) {
+  uint32_t out = 0;
+  uint64_t le = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int32_t e = frodo_sample((uint32_t)(w >> (16 * i)) & 0xFFFF);
+    out |= ((uint32_t)e & 0xFF) << (8 * i);
+    le |= (uint64_t)((uint32_t)e & 0xFFFF) << (16 * i);
+  }
+  if (le16) *le16 = le;
+  return out;
+}
+
+// SHAKE256(dom || seed[24 bytes]), squeezing OUTW words into f(word index, word).
+template <int OUTW, class F>
+__device__ __forceinline__ void shake256_seed_squeeze(uint64_t dom, const uint64_t seed[3], F&& f) {
+  uint64_t A[25];
+  keccak_zero(A);
+  A[0] = dom | (seed[0] << 8);
+  A[1] = (seed[0] >> 56) | (seed[1] << 8);
+  A[2] = (seed[1] >> 56) | (seed[2] << 8);
+  A[3] = (seed[2] >> 56) | (kDsShake << 8);
+  A[kRateShake256 - 1] ^= kPadEnd;
+  keccakf(A);
+  constexpr int NB = OUTW / kRateShake256, REM = OUTW % kRateShake256;
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int i = 0; i < kRateShake256; ++i) f(b * kRateShake256 + i, A[i]);
+    keccakf(A);
+  }
+#pragma unroll
+  for (int i = 0; i < REM; ++i) f(NB * kRateShake256 + i, A[i]);
+}
+
+// ----------------------------------------------------------------- keygen chains
+
+// seedA = SHAKE256(z, 16); S^T || E = sample(SHAKE256(0x5F || seedSE, 31232)).
+// Writes pk[0:16], sk[0:24] = s, sk[24:40] = seedA, sk S^T part (LE u16) and
+// the int8 copies of S^T ([k][j]) and E ([i][k]) for the fused kernel.
+__global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
+                                                       uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                       uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
+  const int inst = blockIdx.x * 64 + threadIdx.x;
+  if (inst >= n) return;
+  const uint8_t* c = coins + (size_t)inst * kFrodoKeygenCoins;
+  uint8_t* p = pk + (size_t)inst * kFrodoPk;
+  uint8_t* q = sk + (size_t)inst * kFrodoSk;
+  {
+    uint64_t z[2] = {ld64(c + 48), ld64(c + 56)}, A[25];
+    absorb_regs<kRateShake256, 2>(A, z, kDsShake);
+    st64(p, A[0]);
+    st64(p + 8, A[1]);
+    st64(q + 24, A[0]);
+    st64(q + 32, A[1]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) st64(q + 8 * i, ld64(c + 8 * i));
+  }
+  const uint64_t seed[3] = {ld64(c + 24), ld64(c + 32), ld64(c + 40)};
+  uint32_t* sd = s8 + (size_t)inst * (kFMat / 4);
+  uint32_t* ed = e8 + (size_t)inst * (kFMat / 4);
+  uint8_t* skS = q + kFSkS;
+  shake256_seed_squeeze<2 * kFMat / 4>(0x5F, seed, [&](int w, uint64_t v) {
+    if (w < kFMat / 4) {
+      uint64_t le;
+      sd[w] = sample4_i8(v, &le);
+      st64(skS + 8 * w, le);
+    } else {
+      ed[w - kFMat / 4] = sample4_i8(v, nullptr);
+    }
+  });
+}
+
+// pkh = SHAKE256(pk, 24); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
+__global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
+  const int inst = blockIdx.x * 64 + threadIdx.x;
+  if (inst >= n) return;
+  uint64_t A[25];
+  absorb_global<kRateShake256, kFPkWords>(A, pk + (size_t)inst * kFrodoPk, kDsShake);
+  uint8_t* q = sk + (size_t)inst * kFrodoSk + kFSkPkh;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) st64(q + 8 * i, A[i]);
+}
+
+// ----------------------------------------------------------------- enc / dec chains
+
+// enc: pkh = SHAKE256(pk); mu = coins.   dec: pkh from sk; mu = recovered mu'.
+// G2 = SHAKE256(pkh || mu, 48) = seedSE || k; S' || E' || E'' from
+// SHAKE256(0x96 || seedSE, 31360).  k goes to kbuf[0:24].
+template <bool DEC>
+__global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __restrict__ keys, size_t key_stride,
+                                                       const uint8_t* __restrict__ mu, size_t mu_stride,
+                                                       uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
+                                                       uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8) {
+  const int inst = blockIdx.x * 64 + threadIdx.x;
+  if (inst >= n) return;
+  const uint8_t* key = keys + (size_t)inst * key_stride;
+  uint64_t g2in[6], A[25];
+  if (DEC) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g2in[i] = ld64(key + kFSkPkh + 8 * i);
+  } else {
+    absorb_global<kRateShake256, kFPkWords>(A, key, kDsShake);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g2in[i] = A[i];
+  }
+  const uint8_t* m = mu + (size_t)inst * mu_stride;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) g2in[3 + i] = ld64(m + 8 * i);
+  absorb_regs<kRateShake256, 6>(A, g2in, kDsShake);
+  const uint64_t seed[3] = {A[0], A[1], A[2]};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) st64(kbuf + (size_t)inst * 64 + 8 * i, A[3 + i]);
+  uint32_t* sd = s8 + (size_t)inst * (kFMat / 4);
+  uint32_t* ed = e8 + (size_t)inst * (kFMat / 4);
+  uint32_t* pd = epp8 + (size_t)inst * 16;
+  shake256_seed_squeeze<(2 * kFMat + 64) / 4>(0x96, seed, [&](int w, uint64_t v) {
+    const uint32_t x = sample4_i8(v, nullptr);
+    if (w < kFMat / 4) sd[w] = x;
+    else if (w < 2 * kFMat / 4) ed[w - kFMat / 4] = x;
+    else pd[w - 2 * kFMat / 4] = x;
+  });
+}
+
+// ss = SHAKE256(ct || kbar, 24); for decaps kbar = fail ? s : k'.
+__global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ kbuf,
+                                               const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
+                                               uint8_t* __restrict__ ss) {
+  const int inst = blockIdx.x * 64 + threadIdx.x;
+  if (inst >= n) return;
+  const uint8_t* c = ct + (size_t)inst * kFrodoCt;
+  constexpr int RW = kRateShake256;
+  constexpr int NB = (kFCtWords + 3) / RW;           // 115 full blocks
+  constexpr int REM = kFCtWords - NB * RW;           // 13 ct words in the last block
+  static_assert(REM + 3 == RW - 1, "k and padding share the final block");
+  uint64_t A[25];
+  keccak_zero(A);
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int i = 0; i < RW; ++i) A[i] ^= ld64(c + 8 * (b * RW + i));
+    keccakf(A);
+  }
+#pragma unroll
+  for (int i = 0; i < REM; ++i) A[i] ^= ld64(c + 8 * (NB * RW + i));
+  uint64_t mask = 0;
+  if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const uint64_t k = ld64(kbuf + (size_t)inst * 64 + 8 * i);
+    const uint64_t s = sk ? ld64(sk + (size_t)inst * kFrodoSk + 8 * i) : 0;
+    A[REM + i] ^= (k & ~mask) | (s & mask);
+  }
+  A[RW - 1] ^= kDsShake ^ kPadEnd;
+  keccakf(A);
+  uint8_t* o = ss + (size_t)inst * kFrodoSs;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) st64(o + 8 * i, A[i]);
+}
+
+// decaps: W = C - B' S, mu' = Decode(W).  256 threads per instance.
+__global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
+                                                    uint8_t* __restrict__ mubuf) {
+  __shared__ uint32_t W[64];
+  const int inst = blockIdx.x, tid = threadIdx.x;
+  if (inst >= n) return;
+  if (tid < 64) W[tid] = 0;
+  __syncthreads();
+  const uint8_t* c = ct + (size_t)inst * kFrodoCt;
+  const uint8_t* sT = sk + (size_t)inst * kFrodoSk + kFSkS;
+  const int o = tid & 63, i = o >> 3, j = o & 7, slice = tid >> 6;  // 4 slices of 244
+  uint32_t acc = 0;
+  for (int k = slice * 244; k < slice * 244 + 244; ++k) {
+    const uint32_t bp = ((uint32_t)c[2 * (i * kFN + k)] << 8) | c[2 * (i * kFN + k) + 1];
+    const uint32_t s = sT[2 * (j * kFN + k)] | ((uint32_t)sT[2 * (j * kFN + k) + 1] << 8);
+    acc += bp * s;
+  }
+  atomicAdd(&W[o], acc);
+  __syncthreads();
+  if (tid < 8) {
+    uint32_t word = 0;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const int idx = 8 * tid + jj;
+      const uint32_t cv = ((uint32_t)c[kFCtC1 + 2 * idx] << 8) | c[kFCtC1 + 2 * idx + 1];
+      const uint32_t w = (cv - W[idx]) & 0xFFFF;
+      word |= (((w + (1u << 12)) >> 13) & 7u) << (3 * jj);
+    }
+    uint8_t* m = mubuf + (size_t)inst * 32 + 3 * tid;
+    m[0] = (uint8_t)word;
+    m[1] = (uint8_t)(word >> 8);
+    m[2] = (uint8_t)(word >> 16);
+  }
+}
+
+// ----------------------------------------------------------------- fused A generation + MFMA
+
+// Byte limbs of four u16 entries held in one 64-bit word.
+__device__ __forceinline__ void limb_split(uint64_t w, uint32_t& lo, uint32_t& hi) {
+  const uint32_t d0 = (uint32_t)w, d1 = (uint32_t)(w >> 32);
+  lo = ((d0 & 0xFFu) | ((d0 >> 8) & 0xFF00u) | ((d1 & 0xFFu) << 16) | ((d1 << 8) & 0xFF000000u)) ^ 0x80808080u;
+  hi = ((d0 >> 8) & 0xFFu) | ((d0 >> 16) & 0xFF00u) | ((d1 << 8) & 0xFF0000u) | (d1 & 0xFF000000u);
+}
+
+__device__ __forceinline__ void unit_limbs(const uint64_t w[4], v4i& lo, v4i& hi) {
+  uint32_t l[4], h[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) limb_split(w[m], l[m], h[m]);
+  lo = v4i{(int)l[0], (int)l[1], (int)l[2], (int)l[3]};
+  hi = v4i{(int)h[0], (int)h[1], (int)h[2], (int)h[3]};
+}
+
+// Row generation: each block of 21 words (84 entries of A) is cut into
+// MFMA units of 4 words (16 columns).  84 is not a multiple of 16, so up to
+// three words carry into the next block; the carry count follows the block
+// index mod 4 and is a template parameter so all register indices are static.
+template <int C, class UNIT>
+__device__ __forceinline__ void consume_block(const uint64_t A[25], uint64_t pend[3], UNIT&& unit) {
+  constexpr int TOTAL = kRateShake128 + C, NU = TOTAL / 4, CO = TOTAL % 4;
+#pragma unroll
+  for (int v = 0; v < NU; ++v) {
+    uint64_t w[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int idx = 4 * v + m;
+      w[m] = idx < C ? pend[idx] : A[idx - C];
+    }
+    unit(w);
+  }
+#pragma unroll
+  for (int m = 0; m < CO; ++m) pend[m] = A[kRateShake128 - CO + m];
+}
+
+template <class UNIT>
+__device__ __forceinline__ void gen_row(int row, uint64_t s0, uint64_t s1, UNIT&& unit) {
+  uint64_t A[25];
+  keccak_zero(A);
+  A[0] = (uint64_t)(row & 0xFFFF) | (s0 << 16);
+  A[1] = (s0 >> 48) | (s1 << 16);
+  A[2] = (s1 >> 48) | (kDsShake << 16);
+  A[kRateShake128 - 1] ^= kPadEnd;
+  uint64_t pend[3] = {0, 0, 0};
+  // 12 squeeze blocks cover 1008 >= 976 columns
+  for (int t = 0; t < 12; t += 4) {
+    keccakf(A);
+    consume_block<0>(A, pend, unit);
+    keccakf(A);
+    consume_block<1>(A, pend, unit);
+    keccakf(A);
+    consume_block<2>(A, pend, unit);
+    keccakf(A);
+    consume_block<3>(A, pend, unit);
+  }
+}
+
+constexpr int kSRow = 1008;  // LDS row stride of int8 S^T (63 units, zero beyond 976)
+
+// keygen: B = A S + E.  pk[16:] and sk[40:15656] receive B (big-endian u16).
+__global__ void __launch_bounds__(1024) frodo_as_fused(int n, uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                       const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8) {
+  __shared__ __align__(16) int8_t Sl[9 * kSRow];  // rows 0..7 = S^T, row 8 = zeros
+  __shared__ int colsum[8];
+  const int inst = blockIdx.x, tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  if (inst >= n) return;
+  const uint32_t* s = s8 + (size_t)inst * (kFMat / 4);
+  uint32_t* Sd = reinterpret_cast<uint32_t*>(Sl);
+  for (int x = tid; x < 9 * kSRow / 4; x += 1024) {
+    const int k = x / (kSRow / 4), j4 = x % (kSRow / 4);
+    Sd[x] = (k < 8 && j4 < kFN / 4) ? s[k * (kFN / 4) + j4] : 0u;
+  }
+  if (tid < 8) colsum[tid] = 0;
+  __syncthreads();
+  for (int x = tid; x < kFMat / 4; x += 1024) {
+    const int k = x / (kFN / 4), j4 = x % (kFN / 4);
+    const uint32_t v = Sd[k * (kSRow / 4) + j4];
+    const int sum = (int8_t)v + (int8_t)(v >> 8) + (int8_t)(v >> 16) + (int8_t)(v >> 24);
+    atomicAdd(&colsum[k], sum);
+  }
+  __syncthreads();
+
+  const uint8_t* p = pk + (size_t)inst * kFrodoPk;
+  const uint64_t s0 = ld64(p), s1 = ld64(p + 8);
+  const int q = lane >> 4, nn = lane & 15;
+  // B operand: lane (q, nn) carries S^T column 4r + nn%4 when nn/4 == q, else zeros
+  const bool mine = (nn >> 2) == q;
+  const int boff0 = mine ? (nn & 3) * kSRow : 8 * kSRow;
+  const int boff1 = mine ? (4 + (nn & 3)) * kSRow : 8 * kSRow;
+  v4i acc_lo0 = {0, 0, 0, 0}, acc_lo1 = {0, 0, 0, 0}, acc_hi0 = {0, 0, 0, 0}, acc_hi1 = {0, 0, 0, 0};
+  int u = 0;
+  gen_row(wave * 64 + lane, s0, s1, [&](const uint64_t w[4]) {
+    if (u < kFUnits) {
+      v4i lo, hi;
+      unit_limbs(w, lo, hi);
+      const v4i b0 = *reinterpret_cast<const v4i*>(Sl + boff0 + 16 * u);
+      const v4i b1 = *reinterpret_cast<const v4i*>(Sl + boff1 + 16 * u);
+      acc_lo0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(lo, b0, acc_lo0, 0, 0, 0);
+      acc_lo1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(lo, b1, acc_lo1, 0, 0, 0);
+      acc_hi0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(hi, b0, acc_hi0, 0, 0, 0);
+      acc_hi1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(hi, b1, acc_hi1, 0, 0, 0);
+    }
+    ++u;
+  });
+
+  // D layout: lane (g, nn), element i = C[4g + i][nn].  C row m comes from
+  // A-operand lane 16 * (nn / 4) + m; the column is 4r + nn % 4.
+  const int g = lane >> 4;
+  const int8_t* e = e8 + (size_t)inst * kFMat;
+  uint8_t* pb = pk + (size_t)inst * kFrodoPk + kFrodoSeedA;
+  uint8_t* sb = sk + (size_t)inst * kFrodoSk + kFrodoSec + kFrodoSeedA;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const v4i lo = r ? acc_lo1 : acc_lo0, hi = r ? acc_hi1 : acc_hi0;
+    const int col = 4 * r + (nn & 3);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int grow = wave * 64 + 16 * (nn >> 2) + 4 * g + i;
+      if (grow < kFN) {
+        const uint32_t v = (uint32_t)lo[i] + 256u * (uint32_t)hi[i] + 128u * (uint32_t)colsum[col] +
+                           (uint32_t)(int32_t)e[grow * kFNB + col];
+        const int idx = grow * kFNB + col;
+        const uint16_t be = (uint16_t)(((v & 0xFF) << 8) | ((v >> 8) & 0xFF));
+        *reinterpret_cast<uint16_t*>(pb + 2 * idx) = be;
+        *reinterpret_cast<uint16_t*>(sb + 2 * idx) = be;
+      }
+    }
+  }
+}
+
+// enc / dec: B' = S' A + E', then V = S' B + E'' and C = V + Encode(mu).
+// enc writes ct; dec compares with the received ct and raises fail[inst].
+template <bool DEC>
+__global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __restrict__ pkb, size_t pk_stride,
+                                                       const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8,
+                                                       const int8_t* __restrict__ epp8, const uint8_t* __restrict__ mu,
+                                                       size_t mu_stride, uint8_t* __restrict__ ct_out,
+                                                       const uint8_t* __restrict__ ct_in, uint32_t* __restrict__ fail) {
+  __shared__ __align__(16) uint32_t Bacc[kFMat];
+  __shared__ __align__(16) int8_t Sp[8 * 1024];  // S' rows, zero for i >= 976
+  __shared__ int rowsum[8];
+  __shared__ uint32_t V[64];
+  __shared__ uint32_t diff;
+  const int inst = blockIdx.x, tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  if (inst >= n) return;
+  const uint32_t* s = s8 + (size_t)inst * (kFMat / 4);
+  uint32_t* Sd = reinterpret_cast<uint32_t*>(Sp);
+  for (int x = tid; x < 8 * 256; x += 1024) {
+    const int k = x >> 8, i4 = x & 255;
+    Sd[x] = i4 < kFN / 4 ? s[k * (kFN / 4) + i4] : 0u;
+  }
+  for (int x = tid; x < kFMat; x += 1024) Bacc[x] = 0;
+  if (tid < 8) rowsum[tid] = 0;
+  if (tid < 64) V[tid] = 0;
+  if (tid == 0) diff = 0;
+  __syncthreads();
+  for (int x = tid; x < 8 * 256; x += 1024) {
+    const uint32_t v = Sd[x];
+    const int sum = (int8_t)v + (int8_t)(v >> 8) + (int8_t)(v >> 16) + (int8_t)(v >> 24);
+    atomicAdd(&rowsum[x >> 8], sum);
+  }
+
+  const uint8_t* pk = pkb + (size_t)inst * pk_stride;
+  const uint64_t s0 = ld64(pk), s1 = ld64(pk + 8);
+  const int g = lane >> 4, nn = lane & 15;
+  // product A operand: lane (g, m = nn) holds S'[m][wave rows 16 q + 4 g + i] in dword q, byte i
+  v4i sop = {0, 0, 0, 0};
+  if (nn < 8) {
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) sop[qq] = (int)Sd[nn * 256 + (wave * 64 + 16 * qq + 4 * g) / 4];
+  }
+  // transpose selectors: B operand of transpose q is one-hot(byte nn) on lanes of group q
+  const int onehot = 1 << (8 * (nn & 3));
+  v4i sel[4];
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+    for (int d = 0; d < 4; ++d) sel[qq][d] = (g == qq && d == (nn >> 2)) ? onehot : 0;
+  const v4i zero = {0, 0, 0, 0};
+
+  int u = 0;
+  gen_row(wave * 64 + lane, s0, s1, [&](const uint64_t w[4]) {
+    if (u < kFUnits) {
+      v4i lm[2];
+      unit_limbs(w, lm[0], lm[1]);
+      v4i prod[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // t_q: lane (g, nn) element i = limb of row 16 q + 4 g + i at column 16 u + nn
+        v4i packed;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const v4i t = __builtin_amdgcn_mfma_i32_16x16x64_i8(lm[h], sel[qq], zero, 0, 0, 0);
+          packed[qq] = (int)(((uint32_t)t[0] & 0xFF) | (((uint32_t)t[1] & 0xFF) << 8) |
+                             (((uint32_t)t[2] & 0xFF) << 16) | ((uint32_t)t[3] << 24));
+        }
+        prod[h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sop, packed, zero, 0, 0, 0);
+      }
+      const int col = 16 * u + nn;
+      if (g < 2 && col < kFN) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          atomicAdd(&Bacc[(4 * g + i) * kFN + col], (uint32_t)prod[0][i] + 256u * (uint32_t)prod[1][i]);
+      }
+    }
+    ++u;
+  });
+  __syncthreads();
+
+  // B' = acc + 128 rowsum + E'  (k-major, 8 x 976)
+  const int8_t* ep = e8 + (size_t)inst * kFMat;
+  uint32_t d = 0;
+  for (int x = tid; x < kFMat; x += 1024) {
+    const uint32_t v = (Bacc[x] + 128u * (uint32_t)rowsum[x / kFN] + (uint32_t)(int32_t)ep[x]) & 0xFFFF;
+    const uint16_t be = (uint16_t)((v << 8) | (v >> 8));
+    if (DEC) {
+      d |= (uint32_t)(*reinterpret_cast<const uint16_t*>(ct_in + (size_t)inst * kFrodoCt + 2 * x) ^ be);
+    } else {
+      *reinterpret_cast<uint16_t*>(ct_out + (size_t)inst * kFrodoCt + 2 * x) = be;
+    }
+  }
+  // V = S' B (B = pk.b, 976 x 8 big-endian): 64 outputs, 16 slices of 61 rows
+  {
+    const int o = tid & 63, i = o >> 3, j = o & 7, slice = tid >> 6;
+    uint32_t acc = 0;
+    for (int k = slice * 61; k < slice * 61 + 61; ++k) {
+      const uint8_t* bp = pk + kFrodoSeedA + 2 * (k * kFNB + j);
+      const uint32_t b = ((uint32_t)bp[0] << 8) | bp[1];
+      acc += (uint32_t)(int32_t)Sp[i * 1024 + k] * b;
+    }
+    atomicAdd(&V[o], acc);
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const uint8_t* m = mu + (size_t)inst * mu_stride + 3 * (tid >> 3);
+    const uint32_t t24 = m[0] | ((uint32_t)m[1] << 8) | ((uint32_t)m[2] << 16);
+    const uint32_t enc = ((t24 >> (3 * (tid & 7))) & 7u) << 13;
+    const uint32_t cv = (V[tid] + (uint32_t)(int32_t)epp8[(size_t)inst * 64 + tid] + enc) & 0xFFFF;
+    const uint16_t be = (uint16_t)((cv << 8) | (cv >> 8));
+    if (DEC) {
+      d |= (uint32_t)(*reinterpret_cast<const uint16_t*>(ct_in + (size_t)inst * kFrodoCt + kFCtC1 + 2 * tid) ^ be);
+    } else {
+      *reinterpret_cast<uint16_t*>(ct_out + (size_t)inst * kFrodoCt + kFCtC1 + 2 * tid) = be;
+    }
+  }
+  if (DEC) {
+    if (d) atomicOr(&diff, 1u);
+    __syncthreads();
+    if (tid == 0) fail[inst] = diff;
+  }
+}
+
+// ----------------------------------------------------------------- host launchers
+
+static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
+
+void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
+                         uint32_t* e8) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(frodo_keygen_pre, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, coins, pk, sk, s8, e8);
+  hipLaunchKernelGGL(frodo_as_fused, dim3(n), dim3(1024), 0, st, n, pk, sk, s8, reinterpret_cast<const int8_t*>(e8));
+  hipLaunchKernelGGL(frodo_keygen_post, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, pk, sk);
+}
+
+void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(frodo_encdec_pre<false>, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, pk, (size_t)kFrodoPk, coins,
+                     (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8);
+  hipLaunchKernelGGL(frodo_sa_fused<false>, dim3(n), dim3(1024), 0, st, n, pk, (size_t)kFrodoPk, s8,
+                     reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), coins,
+                     (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
+  hipLaunchKernelGGL(frodo_ss, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, kbuf, nullptr, nullptr, ss);
+}
+
+void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(frodo_dec_mu, dim3(n), dim3(256), 0, st, n, ct, sk, mubuf);
+  hipLaunchKernelGGL(frodo_encdec_pre<true>, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, sk, (size_t)kFrodoSk, mubuf,
+                     (size_t)32, kbuf, s8, e8, epp8);
+  hipLaunchKernelGGL(frodo_sa_fused<true>, dim3(n), dim3(1024), 0, st, n, sk + kFrodoSec, (size_t)kFrodoSk, s8,
+                     reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32,
+                     nullptr, ct, fail);
+  hipLaunchKernelGGL(frodo_ss, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, kbuf, sk, fail, ss);
+}
+
+}  // namespace dpkem

@@ -1,0 +1,28 @@
+// Host-side launcher declarations shared by the engine and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace dpkem {
+
+// ---- batched form (saber.hip, frodo.hip)
+void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
+                         uint8_t* sraw);
+void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
+                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf);
+void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
+                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail);
+
+void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
+                         uint32_t* e8);
+void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf);
+void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail);
+
+// Scratch bytes per instance for each (scheme, op) of the batched form.
+constexpr size_t kSaberScratchPerInst = 3744 + 768 + 32 + 64 + 16;
+constexpr size_t kFrodoScratchPerInst = 7808 + 7808 + 64 + 64 + 32 + 16;
+
+}  // namespace dpkem

@@ -1,0 +1,391 @@
+// Saber l=3 batched kernels for gfx950.
+//
+// Reference path replaced (benlwk/DPCrypto SABER-GPU/):
+//   hashing      sha3_256_gpu / sha3_512_gpu / GenSecret_gpu (<<<1,BATCH>>>)
+//                and the 25-thread shake128_gpu GenMatrix   src/poly.cu:897-1093
+//   products     DoDP4Av7 / DoDP4Av7_inner (+ operand repackers)  src/poly.cu:14-206
+//   rounding     post_process{,2,3}                          src/poly.cu:360-382
+//   packers      POLVECp2BS / BS2POLVECp / BS2POLVECq / POLT2BS / msg  src/pack.cu:8-211
+//   FO glue      copysk / verify_gpu / cmov_gpu              src/pack.cu:215-253
+//   drivers      saber_enc / saber_dec                       src/cuda_kernel.cu:19-299
+// Keygen, absent from the reference, follows the published scheme.
+//
+// Design: each op is three launches.
+//   hash1  lane-per-instance Keccak, split in two waves per 64 instances so
+//          GenMatrix (23 permutations) runs beside the message / secret chain;
+//   poly   one workgroup of 128 threads per instance: unpacking, CBD, the
+//          negacyclic products as LDS-staged Toeplitz dot products
+//          (v_dot2_u32_u16, 8 outputs per thread), rounding and packing;
+//   hash2  lane-per-instance closing hashes (H(pk) or H(ct) and K).
+#include "common.hpp"
+#include "keccak.hpp"
+#include "sponge.hpp"
+#include "saber_poly.hpp"
+
+namespace dpkem {
+
+constexpr int kSabGenMatrixWords = kSaberGenMatrixBytes / 8;  // 468
+constexpr int kSabGenSecretWords = kSaberGenSecretBytes / 8;  // 96
+
+// ----------------------------------------------------------------- hash1
+// blockDim = 128: wave 0 runs role 0 and wave 1 role 1 for the same 64 instances.
+
+__global__ void __launch_bounds__(128) sab_keygen_hash1(int n, const uint8_t* __restrict__ coins,
+                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                        uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
+  const int role = threadIdx.x >> 6;
+  const int inst = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (inst >= n) return;
+  const uint8_t* c = coins + (size_t)inst * kSaberKeygenCoins;
+  if (role == 0) {
+    uint64_t m[4], A[25];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m[i] = ld64(c + 8 * i);
+    absorb_regs<kRateShake128, 4>(A, m, kDsShake);  // seedA = SHAKE128(seedA_raw, 32)
+    uint8_t* p = pk + (size_t)inst * kSaberPk + kSaberPolyVecCompBytes;
+    uint8_t* q = sk + (size_t)inst * kSaberSk + kSaberPolyVecBytes + kSaberPolyVecCompBytes;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      m[i] = A[i];
+      st64(p + 8 * i, A[i]);
+      st64(q + 8 * i, A[i]);
+    }
+    shake128_regs_to_global<4, kSabGenMatrixWords>(araw + (size_t)inst * kSaberGenMatrixBytes, m);
+  } else {
+    uint64_t m[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m[i] = ld64(c + 32 + 8 * i);
+    shake128_regs_to_global<4, kSabGenSecretWords>(sraw + (size_t)inst * kSaberGenSecretBytes, m);
+    uint8_t* z = sk + (size_t)inst * kSaberSk + kSaberSk - 32;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) st64(z + 8 * i, ld64(c + 64 + 8 * i));
+  }
+}
+
+__global__ void __launch_bounds__(128) sab_enc_hash1(int n, const uint8_t* __restrict__ pk,
+                                                     const uint8_t* __restrict__ coins, uint8_t* __restrict__ araw,
+                                                     uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
+                                                     uint8_t* __restrict__ krbuf) {
+  const int role = threadIdx.x >> 6;
+  const int inst = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (inst >= n) return;
+  const uint8_t* p = pk + (size_t)inst * kSaberPk;
+  if (role == 0) {
+    uint64_t seed[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) seed[i] = ld64(p + kSaberPolyVecCompBytes + 8 * i);
+    shake128_regs_to_global<4, kSabGenMatrixWords>(araw + (size_t)inst * kSaberGenMatrixBytes, seed);
+  } else {
+    uint64_t raw[4], buf[8], kr[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) raw[i] = ld64(coins + (size_t)inst * kSaberEncCoins + 8 * i);
+    sha3_256_regs<4>(buf, raw);                 // m = H(m_raw)
+    sha3_256_global<kSaberPk / 8>(buf + 4, p);  // H(pk)
+    sha3_512_64(kr, buf);                       // kr = G(m || H(pk))
+#pragma unroll
+    for (int i = 0; i < 4; ++i) st64(mbuf + (size_t)inst * 32 + 8 * i, buf[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) st64(krbuf + (size_t)inst * 64 + 8 * i, kr[i]);
+    shake128_regs_to_global<4, kSabGenSecretWords>(sraw + (size_t)inst * kSaberGenSecretBytes, kr + 4);
+  }
+}
+
+__global__ void __launch_bounds__(128) sab_dec_hash1(int n, const uint8_t* __restrict__ sk,
+                                                     const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
+                                                     uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
+  const int role = threadIdx.x >> 6;
+  const int inst = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (inst >= n) return;
+  const uint8_t* s = sk + (size_t)inst * kSaberSk;
+  if (role == 0) {
+    uint64_t seed[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) seed[i] = ld64(s + kSaberPolyVecBytes + kSaberPolyVecCompBytes + 8 * i);
+    shake128_regs_to_global<4, kSabGenMatrixWords>(araw + (size_t)inst * kSaberGenMatrixBytes, seed);
+  } else {
+    uint64_t buf[8], kr[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      buf[i] = ld64(mbuf + (size_t)inst * 32 + 8 * i);      // m'
+      buf[4 + i] = ld64(s + kSaberSk - 64 + 8 * i);         // H(pk) from sk
+    }
+    sha3_512_64(kr, buf);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) st64(krbuf + (size_t)inst * 64 + 8 * i, kr[i]);
+    shake128_regs_to_global<4, kSabGenSecretWords>(sraw + (size_t)inst * kSaberGenSecretBytes, kr + 4);
+  }
+}
+
+// ----------------------------------------------------------------- hash2
+
+__global__ void __launch_bounds__(64) sab_keygen_hash2(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
+  const int inst = blockIdx.x * 64 + threadIdx.x;
+  if (inst >= n) return;
+  uint64_t h[4];
+  sha3_256_global<kSaberPk / 8>(h, pk + (size_t)inst * kSaberPk);
+  uint8_t* d = sk + (size_t)inst * kSaberSk + kSaberSk - 64;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) st64(d + 8 * i, h[i]);
+}
+
+// K = SHA3-256(kr[0:32] || H(ct)); for decaps kr[0:32] is replaced by z when
+// the re-encryption differs (constant-time select on the mismatch flag).
+__global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ krbuf,
+                                                const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
+                                                uint8_t* __restrict__ ss) {
+  const int inst = blockIdx.x * 64 + threadIdx.x;
+  if (inst >= n) return;
+  uint64_t m[8];
+  sha3_256_global<kSaberCt / 8>(m + 4, ct + (size_t)inst * kSaberCt);
+  uint64_t mask = 0;
+  if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint64_t k = ld64(krbuf + (size_t)inst * 64 + 8 * i);
+    uint64_t z = sk ? ld64(sk + (size_t)inst * kSaberSk + kSaberSk - 32 + 8 * i) : 0;
+    m[i] = (k & ~mask) | (z & mask);
+  }
+  uint64_t h[4];
+  sha3_256_regs<8>(h, m);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) st64(ss + (size_t)inst * kSaberSs + 8 * i, h[i]);
+}
+
+// ----------------------------------------------------------------- products
+
+enum SabMode { kSabKeygen = 0, kSabEnc = 1, kSabReenc = 2 };
+
+struct SabPolyArgs {
+  int n;
+  const uint8_t* araw;   // GenMatrix stream, 3744 B / instance
+  const uint8_t* sraw;   // GenSecret stream, 768 B / instance
+  const uint8_t* pk;     // keygen: out (b part); enc: in; reenc: pk inside sk
+  size_t pk_stride;      // bytes between instances' pk
+  const uint8_t* mbuf;   // enc / reenc: message, 32 B / instance
+  uint8_t* out;          // keygen: pk; enc: ct; reenc: unused
+  uint8_t* sk;           // keygen: sk out
+  const uint8_t* ct_in;  // reenc: received ct to compare with
+  uint32_t* fail;        // reenc: per-instance mismatch flag
+};
+
+struct SabPolyLds {
+  uint16_t T[12 * kToepStride];  // 9 A polynomials + 3 b polynomials
+  uint16_t S[3 * 256];
+  uint32_t stage[kSaberGenMatrixBytes / 4];
+  uint8_t outb[kSaberCt + 64];
+  uint32_t diff;
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
+  __shared__ __align__(16) SabPolyLds L;
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  if (inst >= a.n) return;
+
+  // 1. stage GenMatrix bytes, expand GenSecret with CBD (mu = 8)
+  const uint32_t* ar = reinterpret_cast<const uint32_t*>(a.araw + (size_t)inst * kSaberGenMatrixBytes);
+  for (int i = tid; i < kSaberGenMatrixBytes / 4; i += 128) L.stage[i] = ar[i];
+  const uint32_t* sr = reinterpret_cast<const uint32_t*>(a.sraw + (size_t)inst * kSaberGenSecretBytes);
+  for (int i = tid; i < kSaberGenSecretBytes / 4; i += 128) {
+    const uint32_t w = sr[i];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint32_t byte = (w >> (8 * b)) & 0xFF;
+      L.S[4 * i + b] = (uint16_t)(popc(byte & 0xF) - popc(byte >> 4));
+    }
+  }
+  if (tid == 0) L.diff = 0;
+  __syncthreads();
+
+  // 2. unpack A (9 polys, 288 groups of 8) into Toeplitz form; b (3 polys) from pk
+  const uint8_t* sb = reinterpret_cast<const uint8_t*>(L.stage);
+  for (int g = tid; g < 288; g += 128) {
+    uint16_t v[8];
+    unpack13_group(v, sb + 13 * g);
+    const int P = g >> 5, q0 = (g & 31) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) toep_put(L.T, P, q0 + i, v[i]);
+  }
+  if (MODE != kSabKeygen) {
+    const uint8_t* pkb = a.pk + (size_t)inst * a.pk_stride;
+    for (int g = tid; g < 192; g += 128) {
+      uint8_t bytes[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) bytes[i] = pkb[5 * g + i];
+      uint16_t v[4];
+      unpack10_group(v, bytes);
+      const int P = 9 + (g >> 6), q0 = (g & 63) * 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) toep_put(L.T, P, q0 + i, v[i]);
+    }
+  }
+  __syncthreads();
+
+  // 3. products: threads 0..95 -> b' row i = tid/32 (A or A^T); 96..127 -> v' = b . s'
+  const int t0 = 8 * (tid & 31);
+  const int row = tid >> 5;  // 0..2 matrix rows, 3 inner product
+  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (MODE != kSabKeygen || row < 3) {
+    for (int j = 0; j < 3; ++j) {
+      const int P = row < 3 ? (MODE == kSabKeygen ? j * 3 + row : row * 3 + j) : 9 + j;
+      toep_dot(acc, L.T, P, L.S + 256 * j, t0);
+    }
+  }
+
+  // 4. rounding + packing into the LDS output image
+  if (row < 3) {
+    // b' = (x + h1) >> 3 -> 10 bits, 8 coefficients -> 10 bytes
+    uint64_t lo = 0;
+    uint32_t hi = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const uint64_t v = ((acc[d] + 4u) >> 3) & 0x3FF;
+      const int bit = 10 * d;
+      if (bit < 64) lo |= v << bit;
+      if (bit + 10 > 64) hi |= (uint32_t)(bit >= 64 ? v << (bit - 64) : v >> (64 - bit));
+    }
+    uint8_t* o = L.outb + row * 320 + 10 * (tid & 31);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
+    o[8] = (uint8_t)hi;
+    o[9] = (uint8_t)(hi >> 8);
+  } else if (MODE != kSabKeygen) {
+    // c_m = (v' - m * 2^9 + h1) >> 6 -> 4 bits; byte g of m holds the bits of t0..t0+7
+    const uint32_t mbyte = a.mbuf[(size_t)inst * 32 + (tid & 31)];
+    uint32_t packed = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const uint32_t mb = (mbyte >> d) & 1;
+      const uint32_t c = ((acc[d] - (mb << 9) + 4u) >> 6) & 0xF;
+      packed |= c << (4 * d);
+    }
+    *reinterpret_cast<uint32_t*>(L.outb + kSaberPolyVecCompBytes + 4 * (tid & 31)) = packed;
+  }
+  __syncthreads();
+
+  // 5. emit
+  if (MODE == kSabKeygen) {
+    uint8_t* pk = a.out + (size_t)inst * kSaberPk;
+    uint8_t* sk = a.sk + (size_t)inst * kSaberSk;
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(L.outb);
+    for (int i = tid; i < kSaberPolyVecCompBytes / 4; i += 128) {
+      reinterpret_cast<uint32_t*>(pk)[i] = ob[i];
+      reinterpret_cast<uint32_t*>(sk + kSaberPolyVecBytes)[i] = ob[i];
+    }
+    // sk[0:1248] = POLVECq(s): 96 groups of 8 coefficients -> 13 bytes
+    if (tid < 96) {
+      uint64_t lo = 0, hi = 0;
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const uint64_t v = L.S[8 * tid + d] & 0x1FFF;
+        const int bit = 13 * d;
+        if (bit < 64) lo |= v << bit;
+        if (bit + 13 > 64) hi |= bit >= 64 ? v << (bit - 64) : v >> (64 - bit);
+      }
+      uint8_t* o = sk + 13 * tid;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
+#pragma unroll
+      for (int i = 0; i < 5; ++i) o[8 + i] = (uint8_t)(hi >> (8 * i));
+    }
+  } else if (MODE == kSabEnc) {
+    uint32_t* ct = reinterpret_cast<uint32_t*>(a.out + (size_t)inst * kSaberCt);
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(L.outb);
+    for (int i = tid; i < kSaberCt / 4; i += 128) ct[i] = ob[i];
+  } else {
+    const uint32_t* ct = reinterpret_cast<const uint32_t*>(a.ct_in + (size_t)inst * kSaberCt);
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(L.outb);
+    uint32_t d = 0;
+    for (int i = tid; i < kSaberCt / 4; i += 128) d |= ct[i] ^ ob[i];
+    if (d) atomicOr(&L.diff, 1u);
+    __syncthreads();
+    if (tid == 0) a.fail[inst] = L.diff;
+  }
+}
+
+// Decaps inner product v = b' . s and message recovery
+// m'_t = ((v + h2 - c_m * 2^6) >> 9) & 1.  Two instances per 64-thread block.
+struct SabInnerLds {
+  uint16_t T[2][3 * kToepStride];
+  uint16_t S[2][3 * 256];
+  uint8_t m[2][32];
+};
+
+__global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __restrict__ ct,
+                                                    const uint8_t* __restrict__ sk, uint8_t* __restrict__ mbuf) {
+  __shared__ __align__(16) SabInnerLds L;
+  const int half = threadIdx.x >> 5, t = threadIdx.x & 31;
+  const int inst = blockIdx.x * 2 + half;
+  const bool live = inst < n;
+  const uint8_t* c = ct + (size_t)(live ? inst : 0) * kSaberCt;
+  const uint8_t* s = sk + (size_t)(live ? inst : 0) * kSaberSk;
+  if (live) {
+    for (int g = t; g < 192; g += 32) {  // b' from ct (10-bit)
+      uint8_t bytes[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) bytes[i] = c[5 * g + i];
+      uint16_t v[4];
+      unpack10_group(v, bytes);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) toep_put(L.T[half], g >> 6, (g & 63) * 4 + i, v[i]);
+    }
+    for (int g = t; g < 96; g += 32) {  // s from sk (13-bit)
+      uint8_t bytes[13];
+#pragma unroll
+      for (int i = 0; i < 13; ++i) bytes[i] = s[13 * g + i];
+      uint16_t v[8];
+      unpack13_group(v, bytes);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) L.S[half][8 * g + i] = v[i];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int t0 = 8 * t;
+  for (int j = 0; j < 3; ++j) toep_dot(acc, L.T[half], j, L.S[half] + 256 * j, t0);
+  const uint32_t cm = *reinterpret_cast<const uint32_t*>(c + kSaberPolyVecCompBytes + 4 * t);
+  uint32_t byte = 0;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    const uint32_t cmd = (cm >> (4 * d)) & 0xF;
+    byte |= (((acc[d] + 228u - (cmd << 6)) >> 9) & 1u) << d;
+  }
+  mbuf[(size_t)inst * 32 + t] = (uint8_t)byte;
+}
+
+// ----------------------------------------------------------------- host launchers
+// (declared in engine.hpp; all run on the given stream)
+
+static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
+
+void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
+                         uint8_t* sraw) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(sab_keygen_hash1, dim3(blocks_for(n, 64)), dim3(128), 0, st, n, coins, pk, sk, araw, sraw);
+  SabPolyArgs a{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
+  hipLaunchKernelGGL(sab_poly<kSabKeygen>, dim3(n), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(sab_keygen_hash2, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, pk, sk);
+}
+
+void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
+                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(sab_enc_hash1, dim3(blocks_for(n, 64)), dim3(128), 0, st, n, pk, coins, araw, sraw, mbuf, krbuf);
+  SabPolyArgs a{n, araw, sraw, pk, (size_t)kSaberPk, mbuf, ct, nullptr, nullptr, nullptr};
+  hipLaunchKernelGGL(sab_poly<kSabEnc>, dim3(n), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(sab_hash2, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, krbuf, nullptr, nullptr, ss);
+}
+
+void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
+                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(sab_dec_inner, dim3(blocks_for(n, 2)), dim3(64), 0, st, n, ct, sk, mbuf);
+  hipLaunchKernelGGL(sab_dec_hash1, dim3(blocks_for(n, 64)), dim3(128), 0, st, n, sk, mbuf, araw, sraw, krbuf);
+  SabPolyArgs a{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
+  hipLaunchKernelGGL(sab_poly<kSabReenc>, dim3(n), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(sab_hash2, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, krbuf, sk, fail, ss);
+}
+
+}  // namespace dpkem

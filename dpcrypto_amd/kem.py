@@ -1,0 +1,204 @@
+"""Python binding of the dpkem C ABI (include/dpkem.h) over ctypes.
+
+The product path is the HIP library ``dpcrypto_amd/libdpkem.so``; this module
+only marshals pointers.  Arrays may be numpy ``uint8`` arrays (host memory)
+or torch ``uint8`` tensors on the engine's GPU (device memory, no copies).
+There is no CPU fallback: if the library is missing the import of the engine
+fails with an error that says how to build it.
+
+Mirrors the reference's batched entry points (FrodoKEM976-GPU/include/
+cuda_kernel.cuh:7-9, SABER-GPU/include/cuda_kernel.cuh:6-7) with explicit
+coins per instance.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdpkem.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "dpkem.h")
+
+FRODO976 = 0
+SABER = 1
+MEM_HOST = 0
+MEM_DEVICE = 1
+FORM_BATCHED = 0
+FORM_PLAIN = 1
+
+ERRORS = {
+    -1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EBUDGET", -5: "EBUSY", -6: "ENOTFOUND",
+}
+
+SIZES = {
+    FRODO976: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
+    SABER: dict(pk=992, sk=2304, ct=1088, ss=32, kcoins=96, ecoins=32),
+}
+
+_lib = None
+
+
+class DpkemError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"dpkem error {code} ({ERRORS.get(code, '?')}) {what}".strip())
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libdpkem.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `make -C dpcrypto_amd`")
+        lib = ctypes.CDLL(path)
+        vp, sz, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+        lib.dpkem_sizes.argtypes = [i] + [ctypes.POINTER(sz)] * 6
+        lib.dpkem_engine_create.argtypes = [ctypes.POINTER(vp), i, sz]
+        lib.dpkem_engine_destroy.argtypes = [vp]
+        lib.dpkem_engine_set_stream.argtypes = [vp, vp, i]
+        lib.dpkem_engine_set_form.argtypes = [vp, i]
+        lib.dpkem_engine_sync.argtypes = [vp]
+        lib.dpkem_engine_last_error.argtypes = [vp]
+        lib.dpkem_engine_last_error.restype = ctypes.c_char_p
+        lib.dpkem_keypair.argtypes = [vp, i, sz, vp, vp, vp, i]
+        lib.dpkem_enc.argtypes = [vp, i, sz, vp, vp, vp, vp, i]
+        lib.dpkem_dec.argtypes = [vp, i, sz, vp, vp, vp, i]
+        _lib = lib
+    return _lib
+
+
+def header_functions(path: str = HEADER_PATH):
+    """Names of the functions declared in include/dpkem.h."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(\w+)\s*\(", text, flags=re.M)))
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if _is_torch(x):
+        return ctypes.c_void_p(x.data_ptr())
+    return x.ctypes.data_as(ctypes.c_void_p)
+
+
+class Engine:
+    """One engine = one GPU, one HIP stream, one grow-only scratch arena."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, form: int = FORM_BATCHED):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.dpkem_engine_create(ctypes.byref(h), device, hbm_budget)
+        if rc:
+            raise DpkemError(rc, "engine_create")
+        self.h = h
+        self.device = device
+        if form != FORM_BATCHED:
+            self.set_form(form)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dpkem_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise DpkemError(rc, f"{what}: {self.lib.dpkem_engine_last_error(self.h).decode(errors='replace')}")
+
+    def set_form(self, form: int):
+        self._check(self.lib.dpkem_engine_set_form(self.h, form), "set_form")
+
+    def set_stream(self, stream_handle: int | None, own: bool = False):
+        """Bind to a HIP stream handle (0 = null stream), or back to the engine's own."""
+        rc = self.lib.dpkem_engine_set_stream(self.h, ctypes.c_void_p(stream_handle or 0), int(own))
+        self._check(rc, "set_stream")
+
+    def sync(self):
+        self._check(self.lib.dpkem_engine_sync(self.h), "sync")
+
+    # -------- array helpers
+    def _mem(self, *arrays):
+        kinds = {_is_torch(a) for a in arrays if a is not None}
+        if len(kinds) != 1:
+            raise TypeError("mix of host and device arrays")
+        if kinds.pop():
+            import torch
+            for a in arrays:
+                if a is not None and (not a.is_cuda or not a.is_contiguous() or a.dtype != torch.uint8):
+                    raise TypeError("device arrays must be contiguous uint8 CUDA tensors")
+            # order the engine's work after / before torch's on the same stream
+            self.set_stream(torch.cuda.current_stream(arrays[0].device).cuda_stream)
+            return MEM_DEVICE
+        return MEM_HOST
+
+    @staticmethod
+    def _empty_like_kind(ref, n, width):
+        if _is_torch(ref):
+            import torch
+            return torch.empty((n, width), dtype=torch.uint8, device=ref.device)
+        return np.empty((n, width), np.uint8)
+
+    @staticmethod
+    def _host(x, width):
+        if _is_torch(x):
+            return x
+        return np.ascontiguousarray(x, dtype=np.uint8).reshape(-1, width)
+
+    # -------- KEM
+    def keypair(self, scheme: int, coins):
+        s = SIZES[scheme]
+        coins = self._host(coins, s["kcoins"])
+        n = coins.shape[0]
+        pk = self._empty_like_kind(coins, n, s["pk"])
+        sk = self._empty_like_kind(coins, n, s["sk"])
+        mem = self._mem(coins)
+        self._check(self.lib.dpkem_keypair(self.h, scheme, n, _ptr(pk), _ptr(sk), _ptr(coins), mem), "keypair")
+        return pk, sk
+
+    def enc(self, scheme: int, pk, coins):
+        s = SIZES[scheme]
+        pk = self._host(pk, s["pk"]); coins = self._host(coins, s["ecoins"])
+        n = pk.shape[0]
+        if coins.shape[0] != n:
+            raise ValueError("pk and coins batch sizes differ")
+        ct = self._empty_like_kind(pk, n, s["ct"])
+        ss = self._empty_like_kind(pk, n, s["ss"])
+        mem = self._mem(pk, coins)
+        self._check(self.lib.dpkem_enc(self.h, scheme, n, _ptr(ct), _ptr(ss), _ptr(pk), _ptr(coins), mem), "enc")
+        return ct, ss
+
+    def dec(self, scheme: int, ct, sk):
+        s = SIZES[scheme]
+        ct = self._host(ct, s["ct"]); sk = self._host(sk, s["sk"])
+        n = ct.shape[0]
+        if sk.shape[0] != n:
+            raise ValueError("ct and sk batch sizes differ")
+        ss = self._empty_like_kind(ct, n, s["ss"])
+        mem = self._mem(ct, sk)
+        self._check(self.lib.dpkem_dec(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), mem), "dec")
+        return ss
+
+    # raw pointer entry points (device arrays preallocated by the caller)
+    def keypair_into(self, scheme, n, pk, sk, coins):
+        self._check(self.lib.dpkem_keypair(self.h, scheme, n, _ptr(pk), _ptr(sk), _ptr(coins), MEM_DEVICE), "keypair")
+
+    def enc_into(self, scheme, n, ct, ss, pk, coins):
+        self._check(self.lib.dpkem_enc(self.h, scheme, n, _ptr(ct), _ptr(ss), _ptr(pk), _ptr(coins), MEM_DEVICE), "enc")
+
+    def dec_into(self, scheme, n, ss, ct, sk):
+        self._check(self.lib.dpkem_dec(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), MEM_DEVICE), "dec")

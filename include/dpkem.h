@@ -1,0 +1,126 @@
+/*
+ * dpkem -- batched FrodoKEM-976-SHAKE and Saber (l=3) on AMD Instinct MI355X.
+ *
+ * C ABI of dpcrypto_amd/libdpkem.so.  Plain pointers and sizes only.
+ *
+ * All arrays are instance-major and contiguous (instance i of `pk` starts at
+ * pk + i * pk_bytes), exactly as the reference lays out its batches
+ * (FrodoKEM976-GPU/src/cuda_kernel.cu:30-46, SABER-GPU/src/cuda_kernel.cu:35-57).
+ * Wire formats are the published round-3 byte layouts.
+ *
+ * Return codes: 0 on success, a negative DPKEM_E* code on error; nothing
+ * aborts.  Decapsulation returns 0 on implicit rejection too (the shared
+ * secret is then derived from the secret rejection key, as the schemes
+ * specify).
+ */
+#ifndef DPKEM_H
+#define DPKEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- constants */
+#define DPKEM_FRODO976 0 /* FrodoKEM-976-SHAKE */
+#define DPKEM_SABER 1    /* Saber, l = 3 */
+
+#define DPKEM_MEM_HOST 0   /* pointers are host memory (pageable or pinned) */
+#define DPKEM_MEM_DEVICE 1 /* pointers are device memory on the engine's GPU */
+
+#define DPKEM_FORM_BATCHED 0 /* fused / MFMA / Toeplitz kernels (default) */
+#define DPKEM_FORM_PLAIN 1   /* one-workgroup-per-KEM multi-launch comparator */
+
+#define DPKEM_OK 0
+#define DPKEM_EINVAL -1    /* bad argument (NULL pointer, unknown scheme, n < 0) */
+#define DPKEM_ENOMEM -2    /* device or host allocation failed */
+#define DPKEM_EDEVICE -3   /* HIP runtime error */
+#define DPKEM_EBUDGET -4   /* request exceeds the engine's HBM budget */
+#define DPKEM_EBUSY -5     /* serving queue full (try again) */
+#define DPKEM_ENOTFOUND -6 /* unknown request / key handle */
+
+typedef struct dpkem_engine dpkem_engine;
+
+/* ---------------------------------------------------------------- sizes */
+/* Byte sizes per instance for `scheme`.  Any output pointer may be NULL.
+ * keygen_coins: Frodo 64 = s || seedSE || z ; Saber 96 = seedA_raw || seed_s || z.
+ * enc_coins:    Frodo 24 = mu ; Saber 32 = m_raw.
+ * Replaces the size macros of FrodoKEM976-GPU/include/api.h:7-10 and
+ * SABER-GPU/include/params.h:42-48. */
+int dpkem_sizes(int scheme, size_t *pk, size_t *sk, size_t *ct, size_t *ss, size_t *keygen_coins,
+                size_t *enc_coins);
+
+/* ---------------------------------------------------------------- engine */
+/* Bind an engine to HIP device `device` with an HBM budget for its scratch
+ * (0 = 25% of free device memory).  One engine owns one HIP stream. */
+int dpkem_engine_create(dpkem_engine **out, int device, size_t hbm_budget_bytes);
+int dpkem_engine_destroy(dpkem_engine *e);
+/* Run subsequent work on a caller-provided HIP stream (hipStream_t; NULL is
+ * the null stream) when own == 0, or on the engine's own stream when own != 0.
+ * Work already queued on the previous stream is waited for first. */
+int dpkem_engine_set_stream(dpkem_engine *e, void *hip_stream, int own);
+/* Select the kernel form used by subsequent calls (DPKEM_FORM_*). */
+int dpkem_engine_set_form(dpkem_engine *e, int form);
+/* Block until all work submitted on the engine's stream has finished. */
+int dpkem_engine_sync(dpkem_engine *e);
+/* Last HIP error string seen by this engine (static storage). */
+const char *dpkem_engine_last_error(dpkem_engine *e);
+
+/* ---------------------------------------------------------------- batched KEM
+ * Explicit-coins entry points over arrays of n instances.  `mem` says where
+ * the pointers live.  With DPKEM_MEM_DEVICE the call is asynchronous on the
+ * engine's stream (call dpkem_engine_sync before reading results); with
+ * DPKEM_MEM_HOST it stages through device scratch and returns when the
+ * outputs are in host memory.
+ *
+ * Replace the reference's hidden-randomness batch drivers
+ *   frodoKEMKeypair / frodoKEMEncap / frodoKEMDecap  FrodoKEM976-GPU/include/cuda_kernel.cuh:7-9
+ *   saber_enc / saber_dec                            SABER-GPU/include/cuda_kernel.cuh:6-7
+ * and the per-instance prototypes crypto_kem_keypair / enc / dec declared at
+ * FrodoKEM976-GPU/include/api.h:14-18. */
+int dpkem_keypair(dpkem_engine *e, int scheme, size_t n, uint8_t *pk, uint8_t *sk, const uint8_t *coins,
+                  int mem);
+int dpkem_enc(dpkem_engine *e, int scheme, size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk,
+              const uint8_t *coins, int mem);
+int dpkem_dec(dpkem_engine *e, int scheme, size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk,
+              int mem);
+
+/* ---------------------------------------------------------------- randomness
+ * PQClean-shaped wrappers draw coins from a randombytes callback.  The
+ * default reads the OS entropy source (getrandom). */
+typedef void (*dpkem_randombytes_fn)(uint8_t *out, size_t len, void *ctx);
+int dpkem_set_randombytes(dpkem_randombytes_fn fn, void *ctx);
+
+/* crypto_kem_{keypair,enc,dec} over arrays of n instances (host memory),
+ * on a process-wide default engine bound to device 0.
+ * Replace PQCLEAN_FRODOKEM976*_crypto_kem_* (FrodoKEM976-GPU/include/api.h:14-18). */
+int dpkem_frodo976_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
+int dpkem_frodo976_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int dpkem_frodo976_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
+int dpkem_saber_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int dpkem_saber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+
+/* ---------------------------------------------------------------- reference-shaped drivers
+ * Same signatures and batch size (768 instances, host arrays) as the
+ * reference's benchmark drivers, so a caller of those can link this library
+ * instead.  Randomness comes from the randombytes callback; unlike the
+ * reference, every instance uses its own inputs, and saber_dec returns the
+ * shared secrets in h_k.  `mode` 0 selects the batched form, 1 the plain form
+ * (the reference's DP / INT32 switch, SABER-GPU/main.cu:25-33).
+ *   frodoKEMKeypair/Encap/Decap  FrodoKEM976-GPU/include/cuda_kernel.cuh:7-9 (BLOCK = 768, :2)
+ *   saber_enc / saber_dec        SABER-GPU/include/cuda_kernel.cuh:6-7 (BATCH = 768, :1)
+ * Return 0 on success (the reference returns void). */
+#define DPKEM_REF_BATCH 768
+int frodoKEMKeypair(uint8_t *h_pk, uint8_t *h_sk);
+int frodoKEMEncap(uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int frodoKEMDecap(uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+int saber_enc(uint8_t mode, uint8_t *h_k, uint8_t *h_pk, uint8_t *h_c);
+int saber_dec(uint8_t mode, uint8_t *h_c, uint8_t *h_pk, uint8_t *h_sk, uint8_t *h_k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPKEM_H */
