@@ -91,7 +91,8 @@ int ensure_arena(dpkem_engine* e, size_t bytes) {
 
 // Bytes of device scratch one instance needs for (scheme, op) in the current
 // form, excluding staged I/O.
-size_t scratch_per_inst(int scheme) {
+size_t scratch_per_inst(int scheme, int form) {
+  if (form == DPKEM_FORM_PLAIN) return align_up(plain_scratch_per_inst(scheme));
   return scheme == DPKEM_FRODO976 ? align_up(kFrodoScratchPerInst) : align_up(kSaberScratchPerInst);
 }
 
@@ -114,6 +115,11 @@ void launch_chunk(dpkem_engine* e, int scheme, int op, int n, uint8_t* o0, uint8
                   const uint8_t* i1, uint8_t* scratch) {
   Carver c{scratch};
   hipStream_t st = e->stream;
+  if (e->form == DPKEM_FORM_PLAIN) {
+    if (scheme == DPKEM_SABER) plain_saber_launch(st, op, n, o0, o1, i0, i1, scratch);
+    else plain_frodo_launch(st, op, n, o0, o1, i0, i1, scratch);
+    return;
+  }
   if (scheme == DPKEM_SABER) {
     uint8_t* araw = c.take((size_t)n * 3744);
     uint8_t* sraw = c.take((size_t)n * 768);
@@ -138,13 +144,12 @@ void launch_chunk(dpkem_engine* e, int scheme, int op, int n, uint8_t* o0, uint8
 
 // Upper bound on the arena bytes the chunk carver uses for n instances
 // (per-array alignment padding included).
-size_t chunk_scratch_bytes(int scheme, size_t n) { return n * scratch_per_inst(scheme) + 8 * kAlign; }
+size_t chunk_scratch_bytes(int scheme, int form, size_t n) { return n * scratch_per_inst(scheme, form) + 16 * kAlign; }
 
 int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
            const uint8_t* i1, int mem) {
   if (!e || (scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER)) return DPKEM_EINVAL;
   if (mem != DPKEM_MEM_HOST && mem != DPKEM_MEM_DEVICE) return DPKEM_EINVAL;
-  if (e->form != DPKEM_FORM_BATCHED) return DPKEM_EINVAL;  // plain form: see dpkem_plain_*
   if (n == 0) return DPKEM_OK;
   size_t out[2], in[2];
   io_sizes(scheme, op, out, in);
@@ -161,17 +166,17 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
   std::lock_guard<std::mutex> lock(e->mu);
   HIP_TRY(e, hipSetDevice(e->device));
   const size_t io_per = mem == DPKEM_MEM_HOST ? align_up(out[0]) + align_up(out[1]) + align_up(in[0]) + align_up(in[1]) : 0;
-  const size_t per = scratch_per_inst(scheme) + io_per;
+  const size_t per = scratch_per_inst(scheme, e->form) + io_per;
   size_t chunk = std::min(n, (size_t)INT32_MAX);
-  while (chunk > 1 && chunk * per + 16 * kAlign > e->budget) chunk = (chunk + 1) / 2;
-  if (chunk * per + 16 * kAlign > e->budget) return DPKEM_EBUDGET;
-  int rc = ensure_arena(e, chunk * per + 16 * kAlign);
+  while (chunk > 1 && chunk * per + 24 * kAlign > e->budget) chunk = (chunk + 1) / 2;
+  if (chunk * per + 24 * kAlign > e->budget) return DPKEM_EBUDGET;
+  int rc = ensure_arena(e, chunk * per + 24 * kAlign);
   if (rc) return rc;
   for (size_t done = 0; done < n; done += chunk) {
     const int c = (int)std::min(chunk, n - done);
     uint8_t* dout[2];
     const uint8_t* din[2];
-    Carver io{e->arena + align_up(chunk_scratch_bytes(scheme, chunk))};
+    Carver io{e->arena + align_up(chunk_scratch_bytes(scheme, e->form, chunk))};
     for (int k = 0; k < 2; ++k) {
       if (mem == DPKEM_MEM_DEVICE) {
         dout[k] = out[k] ? outs[k] + done * out[k] : nullptr;

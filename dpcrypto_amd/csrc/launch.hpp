@@ -21,6 +21,14 @@ void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* c
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
                       uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail);
 
+// ---- plain comparator form (plain.hip); op 0 keygen, 1 enc, 2 dec with the
+// same output / input roles as the batched launchers.
+size_t plain_scratch_per_inst(int scheme);
+void plain_saber_launch(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1,
+                        uint8_t* scratch);
+void plain_frodo_launch(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1,
+                        uint8_t* scratch);
+
 // Scratch bytes per instance for each (scheme, op) of the batched form.
 constexpr size_t kSaberScratchPerInst = 3744 + 768 + 32 + 64 + 16;
 constexpr size_t kFrodoScratchPerInst = 7808 + 7808 + 64 + 64 + 32 + 16;

@@ -212,3 +212,31 @@ def test_reference_shaped_drivers(eng, oracle):
         assert (fsk[5, 24:24 + 15632] == fpk[5]).all()
     finally:
         lib.dpkem_set_randombytes(None, None)
+
+
+# ------------------------------------------------------------------ plain comparator form
+@pytest.fixture(scope="module")
+def plain_eng(torch_cuda):
+    import dpcrypto_amd as D
+    e = D.Engine(0, form=D.FORM_PLAIN)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("scheme,n", [(SABER, 70), (FRODO, 3)])
+def test_plain_form_matches_oracle(plain_eng, oracle, scheme, n):
+    import dpcrypto_amd as D
+    s = D.SIZES[scheme]
+    c = _coins(oracle, 800 + scheme, n, s["kcoins"])
+    m = _coins(oracle, 900 + scheme, n, s["ecoins"])
+    pk, sk = plain_eng.keypair(scheme, c)
+    opk, osk = oracle.keypair(scheme, c, threads=4)
+    _eq(pk, opk, "pk")
+    _eq(sk, osk, "sk")
+    ct, ss = plain_eng.enc(scheme, opk, m)
+    oct_, oss = oracle.enc(scheme, opk, m, threads=4)
+    _eq(ct, oct_, "ct")
+    _eq(ss, oss, "ss")
+    _eq(plain_eng.dec(scheme, oct_, osk), oss, "dec ss")
+    bad = oct_.copy(); bad[:, 7] ^= 2
+    _eq(plain_eng.dec(scheme, bad, osk), oracle.dec(scheme, bad, osk, threads=4), "rejected ss")
