@@ -17,6 +17,10 @@
 
 #include "../../include/dpkem.h"
 #include "launch.hpp"
+#include "prof.hpp"
+
+#include <map>
+#include <string>
 
 using namespace dpkem;
 
@@ -56,6 +60,9 @@ struct dpkem_engine {
   size_t arena_bytes = 0;
   char last_err[256] = {0};
   std::mutex mu;
+  bool profiling = false;
+  Profiler prof;
+  std::map<std::string, std::pair<long, double>> kstats;  // name -> (launches, total ms)
 };
 
 namespace {
@@ -188,7 +195,9 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
         din[k] = d;
       }
     }
+    tls_profiler = e->profiling ? &e->prof : nullptr;
     launch_chunk(e, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
+    tls_profiler = nullptr;
     HIP_TRY(e, hipGetLastError());
     if (mem == DPKEM_MEM_HOST) {
       for (int k = 0; k < 2; ++k)
@@ -298,6 +307,11 @@ int dpkem_engine_destroy(dpkem_engine* e) {
     (void)hipStreamSynchronize(e->stream);
     if (e->arena) (void)hipFree(e->arena);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
+    for (auto& r : e->prof.pending) {
+      (void)hipEventDestroy(r.start);
+      (void)hipEventDestroy(r.stop);
+    }
+    for (auto ev : e->prof.pool) (void)hipEventDestroy(ev);
   }
   delete e;
   return DPKEM_OK;
@@ -330,6 +344,43 @@ int dpkem_engine_sync(dpkem_engine* e) {
 }
 
 const char* dpkem_engine_last_error(dpkem_engine* e) { return e ? e->last_err : "no engine"; }
+
+int dpkem_engine_set_profiling(dpkem_engine* e, int on) {
+  if (!e) return DPKEM_EINVAL;
+  std::lock_guard<std::mutex> lock(e->mu);
+  e->profiling = on != 0;
+  return DPKEM_OK;
+}
+
+int dpkem_engine_kernel_stats(dpkem_engine* e, char* buf, size_t len, int reset) {
+  if (!e || !buf || len == 0) return DPKEM_EINVAL;
+  std::lock_guard<std::mutex> lock(e->mu);
+  HIP_TRY(e, hipSetDevice(e->device));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  for (auto& r : e->prof.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) {
+      auto& s = e->kstats[r.name];
+      s.first += 1;
+      s.second += ms;
+    }
+    e->prof.pool.push_back(r.start);
+    e->prof.pool.push_back(r.stop);
+  }
+  e->prof.pending.clear();
+  std::string out = "{";
+  for (auto& kv : e->kstats) {
+    char item[256];
+    snprintf(item, sizeof item, "%s\"%s\": [%ld, %.6f]", out.size() > 1 ? ", " : "", kv.first.c_str(),
+             kv.second.first, kv.second.second);
+    out += item;
+  }
+  out += "}";
+  if (reset) e->kstats.clear();
+  if (out.size() + 1 > len) return DPKEM_EINVAL;
+  memcpy(buf, out.c_str(), out.size() + 1);
+  return DPKEM_OK;
+}
 
 int dpkem_keypair(dpkem_engine* e, int scheme, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, int mem) {
   return run_op(e, scheme, kKeygen, n, pk, sk, coins, nullptr, mem);

@@ -29,6 +29,7 @@
 #include "common.hpp"
 #include "keccak.hpp"
 #include "sponge.hpp"
+#include "prof.hpp"
 
 namespace dpkem {
 
@@ -515,32 +516,32 @@ static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
                          uint32_t* e8) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(frodo_keygen_pre, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, coins, pk, sk, s8, e8);
-  hipLaunchKernelGGL(frodo_as_fused, dim3(n), dim3(1024), 0, st, n, pk, sk, s8, reinterpret_cast<const int8_t*>(e8));
-  hipLaunchKernelGGL(frodo_keygen_post, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, pk, sk);
+  DPKEM_LAUNCH("frodo_keygen_pre", frodo_keygen_pre, dim3(blocks_for(n, 64)), dim3(64), st, n, coins, pk, sk, s8, e8);
+  DPKEM_LAUNCH("frodo_as_fused", frodo_as_fused, dim3(n), dim3(1024), st, n, pk, sk, s8, reinterpret_cast<const int8_t*>(e8));
+  DPKEM_LAUNCH("frodo_keygen_post", frodo_keygen_post, dim3(blocks_for(n, 64)), dim3(64), st, n, pk, sk);
 }
 
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(frodo_encdec_pre<false>, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, pk, (size_t)kFrodoPk, coins,
+  DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 64)), dim3(64), st, n, pk, (size_t)kFrodoPk, coins,
                      (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8);
-  hipLaunchKernelGGL(frodo_sa_fused<false>, dim3(n), dim3(1024), 0, st, n, pk, (size_t)kFrodoPk, s8,
+  DPKEM_LAUNCH("frodo_sa_fused<false>", frodo_sa_fused<false>, dim3(n), dim3(1024), st, n, pk, (size_t)kFrodoPk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), coins,
                      (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
-  hipLaunchKernelGGL(frodo_ss, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, kbuf, nullptr, nullptr, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, kbuf, nullptr, nullptr, ss);
 }
 
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
                       uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(frodo_dec_mu, dim3(n), dim3(256), 0, st, n, ct, sk, mubuf);
-  hipLaunchKernelGGL(frodo_encdec_pre<true>, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, sk, (size_t)kFrodoSk, mubuf,
+  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, mubuf);
+  DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 64)), dim3(64), st, n, sk, (size_t)kFrodoSk, mubuf,
                      (size_t)32, kbuf, s8, e8, epp8);
-  hipLaunchKernelGGL(frodo_sa_fused<true>, dim3(n), dim3(1024), 0, st, n, sk + kFrodoSec, (size_t)kFrodoSk, s8,
+  DPKEM_LAUNCH("frodo_sa_fused<true>", frodo_sa_fused<true>, dim3(n), dim3(1024), st, n, sk + kFrodoSec, (size_t)kFrodoSk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32,
                      nullptr, ct, fail);
-  hipLaunchKernelGGL(frodo_ss, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, kbuf, sk, fail, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, kbuf, sk, fail, ss);
 }
 
 }  // namespace dpkem

@@ -14,6 +14,7 @@
 #include "keccak.hpp"
 #include "launch.hpp"
 #include "sponge.hpp"
+#include "prof.hpp"
 
 namespace dpkem {
 namespace plain {
@@ -403,8 +404,8 @@ __global__ void __launch_bounds__(64) fr_select(int n, const uint8_t* k, size_t 
 // ------------------------------------------------------------ sequencing
 
 using namespace plain;
-#define L64(kern, ...) hipLaunchKernelGGL(kern, dim3(lanes(n)), dim3(64), 0, st, n, __VA_ARGS__)
-#define LWG(kern, thr, ...) hipLaunchKernelGGL(kern, dim3(n), dim3(thr), 0, st, n, __VA_ARGS__)
+#define L64(kern, ...) DPKEM_LAUNCH("plain", kern, dim3(lanes(n)), dim3(64), st, n, __VA_ARGS__)
+#define LWG(kern, thr, ...) DPKEM_LAUNCH("plain", kern, dim3(n), dim3(thr), st, n, __VA_ARGS__)
 
 size_t plain_scratch_per_inst(int scheme) {
   if (scheme == 1) return 16384;  // Saber (see carve below)

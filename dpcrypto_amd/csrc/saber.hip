@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "keccak.hpp"
 #include "sponge.hpp"
+#include "prof.hpp"
 #include "saber_poly.hpp"
 
 namespace dpkem {
@@ -363,29 +364,29 @@ static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
                          uint8_t* sraw) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(sab_keygen_hash1, dim3(blocks_for(n, 64)), dim3(128), 0, st, n, coins, pk, sk, araw, sraw);
+  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, dim3(blocks_for(n, 64)), dim3(128), st, n, coins, pk, sk, araw, sraw);
   SabPolyArgs a{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  hipLaunchKernelGGL(sab_poly<kSabKeygen>, dim3(n), dim3(128), 0, st, a);
-  hipLaunchKernelGGL(sab_keygen_hash2, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, pk, sk);
+  DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), dim3(128), st, a);
+  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, dim3(blocks_for(n, 64)), dim3(64), st, n, pk, sk);
 }
 
 void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(sab_enc_hash1, dim3(blocks_for(n, 64)), dim3(128), 0, st, n, pk, coins, araw, sraw, mbuf, krbuf);
+  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, dim3(blocks_for(n, 64)), dim3(128), st, n, pk, coins, araw, sraw, mbuf, krbuf);
   SabPolyArgs a{n, araw, sraw, pk, (size_t)kSaberPk, mbuf, ct, nullptr, nullptr, nullptr};
-  hipLaunchKernelGGL(sab_poly<kSabEnc>, dim3(n), dim3(128), 0, st, a);
-  hipLaunchKernelGGL(sab_hash2, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, krbuf, nullptr, nullptr, ss);
+  DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), dim3(128), st, a);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, krbuf, nullptr, nullptr, ss);
 }
 
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
                       uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(sab_dec_inner, dim3(blocks_for(n, 2)), dim3(64), 0, st, n, ct, sk, mbuf);
-  hipLaunchKernelGGL(sab_dec_hash1, dim3(blocks_for(n, 64)), dim3(128), 0, st, n, sk, mbuf, araw, sraw, krbuf);
+  DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), dim3(64), st, n, ct, sk, mbuf);
+  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, dim3(blocks_for(n, 64)), dim3(128), st, n, sk, mbuf, araw, sraw, krbuf);
   SabPolyArgs a{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
-  hipLaunchKernelGGL(sab_poly<kSabReenc>, dim3(n), dim3(128), 0, st, a);
-  hipLaunchKernelGGL(sab_hash2, dim3(blocks_for(n, 64)), dim3(64), 0, st, n, ct, krbuf, sk, fail, ss);
+  DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), dim3(128), st, a);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, krbuf, sk, fail, ss);
 }
 
 }  // namespace dpkem

@@ -68,6 +68,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_keypair.argtypes = [vp, i, sz, vp, vp, vp, i]
         lib.dpkem_enc.argtypes = [vp, i, sz, vp, vp, vp, vp, i]
         lib.dpkem_dec.argtypes = [vp, i, sz, vp, vp, vp, i]
+        lib.dpkem_engine_set_profiling.argtypes = [vp, i]
+        lib.dpkem_engine_kernel_stats.argtypes = [vp, ctypes.c_char_p, sz, i]
         _lib = lib
     return _lib
 
@@ -130,6 +132,16 @@ class Engine:
 
     def sync(self):
         self._check(self.lib.dpkem_engine_sync(self.h), "sync")
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.dpkem_engine_set_profiling(self.h, int(on)), "set_profiling")
+
+    def kernel_stats(self, reset: bool = True) -> dict:
+        """{kernel: (launches, total_ms)} from HIP events on the engine's stream."""
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(self.lib.dpkem_engine_kernel_stats(self.h, buf, len(buf), int(reset)), "kernel_stats")
+        return {k: (v[0], v[1]) for k, v in json.loads(buf.value.decode()).items()}
 
     # -------- array helpers
     def _mem(self, *arrays):

@@ -67,6 +67,14 @@ int dpkem_engine_set_form(dpkem_engine *e, int form);
 int dpkem_engine_sync(dpkem_engine *e);
 /* Last HIP error string seen by this engine (static storage). */
 const char *dpkem_engine_last_error(dpkem_engine *e);
+/* Per-kernel timing: when on, every kernel launch is bracketed by HIP events
+ * on the engine's stream.  kernel_stats waits for the stream, then writes a
+ * JSON object {"kernel": [launches, total_ms], ...} into buf (reset != 0
+ * clears the counters).  Replaces the reference's cudaEvent pairs around
+ * whole drivers (FrodoKEM976-GPU/src/cuda_kernel.cu:55,89-92) and its
+ * MEAS_MV / MEAS_IP phase events (SABER-GPU/src/cuda_kernel.cu:77-125). */
+int dpkem_engine_set_profiling(dpkem_engine *e, int on);
+int dpkem_engine_kernel_stats(dpkem_engine *e, char *buf, size_t len, int reset);
 
 /* ---------------------------------------------------------------- batched KEM
  * Explicit-coins entry points over arrays of n instances.  `mem` says where
