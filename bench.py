@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Batched KEM throughput on one MI355X.
+
+Headline workload (the metric's single-GPU configuration): Saber l=3, a batch
+of 4096 instances, one step = keygen + encaps + decaps of every instance,
+inputs (coins) resident in HBM before the timed region, outputs left in HBM.
+value = KEM operations per second = 3 * 4096 * steps / seconds.
+
+Reported beside it in the same JSON line:
+  frodo976       FrodoKEM-976-SHAKE, batch 8192, keygen + encaps + decaps
+  plain          the one-workgroup-per-KEM comparator form on the same inputs
+  latency        p50 / p99 of the per-step wall time (ms)
+  roofline       dominant kernel of the headline workload, from HIP events on
+                 the engine's stream during the timed steps
+  cpu_baseline   the CPU oracle on the host cores, on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+VALU_PEAK = 256 * 4 * 16 * 2.4e9  # lane-ops / s: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
+KECCAK_LANE_OPS = 4176            # int32 VALU lane-ops per Keccak-f[1600] (gfx950 floor, bitop3)
+HBM_PEAK = 8.0e12
+
+# Algorithmic work per instance for each kernel: (Keccak permutations, MACs).
+# MACs run 2 per VALU lane-op (v_dot2) for Saber; Frodo's big products run on
+# MFMA and are not counted against the VALU roofline.
+KERNEL_WORK = {
+    "sab_keygen_hash1": (1 + 23 + 5, 0),
+    "sab_poly<kSabKeygen>": (0, 9 * 65536),
+    "sab_keygen_hash2": (8, 0),
+    "sab_enc_hash1": (23 + 1 + 8 + 1 + 5, 0),
+    "sab_poly<kSabEnc>": (0, 12 * 65536),
+    "sab_hash2": (9 + 1, 0),
+    "sab_dec_inner": (0, 3 * 65536),
+    "sab_dec_hash1": (23 + 1 + 5, 0),
+    "sab_poly<kSabReenc>": (0, 12 * 65536),
+    "frodo_keygen_pre": (1 + 230, 0),
+    "frodo_as_fused": (976 * 12, 0),
+    "frodo_keygen_post": (115, 0),
+    "frodo_encdec_pre<false>": (115 + 1 + 231, 0),
+    "frodo_encdec_pre<true>": (1 + 231, 0),
+    "frodo_sa_fused<false>": (976 * 12, 0),
+    "frodo_sa_fused<true>": (976 * 12, 0),
+    "frodo_ss": (116, 0),
+    "frodo_dec_mu": (0, 64 * 976),
+}
+
+
+def lane_ops(kernel: str, n: int) -> float:
+    perms, macs = KERNEL_WORK.get(kernel, (0, 0))
+    return n * (perms * KECCAK_LANE_OPS + macs / 2.0)
+
+
+def pct(vals, q):
+    v = sorted(vals)
+    if not v:
+        return None
+    k = (len(v) - 1) * q
+    lo = int(k)
+    hi = min(lo + 1, len(v) - 1)
+    return v[lo] + (v[hi] - v[lo]) * (k - lo)
+
+
+def run_workload(torch, eng, scheme, n, steps, warmup, seed, profile=False):
+    """Time `steps` keygen+enc+dec steps over n device-resident instances."""
+    import numpy as np
+    import dpcrypto_amd as D
+    s = D.SIZES[scheme]
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    kc = torch.from_numpy(rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)).to(dev)
+    ec = torch.from_numpy(rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)).to(dev)
+    pk = torch.empty((n, s["pk"]), dtype=torch.uint8, device=dev)
+    sk = torch.empty((n, s["sk"]), dtype=torch.uint8, device=dev)
+    ct = torch.empty((n, s["ct"]), dtype=torch.uint8, device=dev)
+    ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
+    ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    eng.set_stream(stream.cuda_stream)
+
+    def step():
+        eng.keypair_into(scheme, n, pk, sk, kc)
+        eng.enc_into(scheme, n, ct, ss, pk, ec)
+        eng.dec_into(scheme, n, ss2, ct, sk)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    eng.kernel_stats(reset=True)
+    eng.set_profiling(profile)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    stops = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        step()
+        stops[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    eng.set_profiling(False)
+    stats = eng.kernel_stats(reset=True) if profile else {}
+    per_step = [a.elapsed_time(b) for a, b in zip(starts, stops)]
+    ok = bool(torch.equal(ss, ss2))
+    return dict(wall_s=wall, per_step_ms=per_step, kernels=stats, roundtrip_ok=ok,
+                ops_per_s=3 * n * steps / wall)
+
+
+def roofline_from(stats, n):
+    """Dominant kernel by total time -> achieved VALU lane-ops/s vs peak."""
+    if not stats:
+        return None
+    name, (count, total_ms) = max(stats.items(), key=lambda kv: kv[1][1])
+    avg_s = total_ms / count / 1e3
+    work = lane_ops(name, n)
+    achieved = work / avg_s
+    return {
+        "kernel": name,
+        "bound": "valu",
+        "achieved": round(achieved / 1e12, 3),
+        "peak": round(VALU_PEAK / 1e12, 3),
+        "unit": "Tlane-op/s",
+        "frac": round(achieved / VALU_PEAK, 4),
+        "traffic": None,
+        "avg_launch_ms": round(avg_s * 1e3, 4),
+        "algorithmic_lane_ops_per_launch": work,
+        "share_of_step": round(total_ms / sum(v[1] for v in stats.values()), 3),
+    }
+
+
+def cpu_baseline(budget_s: float):
+    """The oracle (a scalar C port) on the host cores, bounded sample."""
+    import numpy as np
+    from oracle import pyoracle as O
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    n = threads * 8
+    rng = np.random.default_rng(1)
+    kc = rng.integers(0, 256, (n, 96), dtype=np.uint8)
+    ec = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    ops = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        pk, sk = O.keypair(1, kc, threads=threads)
+        ct, ss = O.enc(1, pk, ec, threads=threads)
+        O.dec(1, ct, sk, threads=threads)
+        ops += 3 * n
+    dt = time.perf_counter() - t0
+    return {"value": round(ops / dt, 1), "unit": "KEM ops/s", "cores": threads, "kind": "port",
+            "sample": f"Saber l=3 keygen+encaps+decaps, {n} instances per round, {ops // (3 * n)} rounds "
+                      f"({dt:.1f} s) on {threads} threads of the CPU oracle (oracle/, -O3 C)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--saber-batch", type=int, default=4096)
+    ap.add_argument("--frodo-batch", type=int, default=8192)
+    ap.add_argument("--frodo-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--skip-frodo", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: instances are independent, so N GPUs would run N disjoint "
+              "shards of the batch with no communication; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dpcrypto_amd as D
+
+    eng = D.Engine(0)
+    sab = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001, True)
+    result = {
+        "metric": "KEM ops/s (keygen+enc+dec) Frodo-976 & Saber at 1 GPU; p99 latency",
+        "value": round(sab["ops_per_s"], 1),
+        "unit": "KEM ops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sab["wall_s"] * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u32 integer (mod 2^13 / 2^16)",
+        "data": "synthetic seeded coins",
+        "config": {"workload": f"saber-l3 batch {args.saber_batch}: keygen+encaps+decaps per step",
+                   "global_batch": args.saber_batch, "parallelism": "instances (1 GPU)"},
+        "latency_ms": {"p50": round(pct(sab["per_step_ms"], 0.5), 4), "p99": round(pct(sab["per_step_ms"], 0.99), 4),
+                       "what": "per-step (batch) latency"},
+        "roundtrip_ok": sab["roundtrip_ok"],
+        "roofline": roofline_from(sab["kernels"], args.saber_batch),
+        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab["kernels"].items()},
+    }
+    if not args.skip_frodo:
+        fr = run_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002, True)
+        result["frodo976"] = {
+            "workload": f"frodokem-976-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
+            "ops_per_s": round(fr["ops_per_s"], 1),
+            "ms_per_step": round(fr["wall_s"] * 1e3 / args.frodo_steps, 3),
+            "steps": args.frodo_steps,
+            "roundtrip_ok": fr["roundtrip_ok"],
+            "roofline": roofline_from(fr["kernels"], args.frodo_batch),
+            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr["kernels"].items()},
+        }
+    if not args.skip_plain:
+        pe = D.Engine(0, form=D.FORM_PLAIN)
+        sp = run_workload(torch, pe, D.SABER, args.saber_batch, args.plain_steps, 1, 0x44504B454D000001)
+        plain = {"saber_ops_per_s": round(sp["ops_per_s"], 1),
+                 "saber_speedup_batched_over_plain": round(sab["ops_per_s"] / sp["ops_per_s"], 2)}
+        if not args.skip_frodo:
+            fp = run_workload(torch, pe, D.FRODO976, args.frodo_batch, 1, 0, 0x44504B454D000002)
+            plain["frodo976_ops_per_s"] = round(fp["ops_per_s"], 1)
+            plain["frodo976_speedup_batched_over_plain"] = round(fr["ops_per_s"] / fp["ops_per_s"], 2)
+        pe.close()
+        result["plain"] = plain
+    if args.cpu_seconds > 0:
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    eng.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
