@@ -181,7 +181,11 @@ def main():
     import torch
     import dpcrypto_amd as D
 
+    def progress(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     eng = D.Engine(0)
+    progress(f"saber batch {args.saber_batch}: {args.warmup} warmup + {args.steps} timed steps")
     sab = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001, True)
     result = {
         "metric": "KEM ops/s (keygen+enc+dec) Frodo-976 & Saber at 1 GPU; p99 latency",
@@ -204,7 +208,9 @@ def main():
         "roofline": roofline_from(sab["kernels"], args.saber_batch),
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab["kernels"].items()},
     }
+    progress(f"saber: {sab['ops_per_s']:.0f} ops/s")
     if not args.skip_frodo:
+        progress(f"frodo976 batch {args.frodo_batch}: 1 warmup + {args.frodo_steps} timed steps")
         fr = run_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002, True)
         result["frodo976"] = {
             "workload": f"frodokem-976-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
@@ -216,6 +222,7 @@ def main():
             "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr["kernels"].items()},
         }
     if not args.skip_plain:
+        progress("plain comparator forms")
         pe = D.Engine(0, form=D.FORM_PLAIN)
         sp = run_workload(torch, pe, D.SABER, args.saber_batch, args.plain_steps, 1, 0x44504B454D000001)
         plain = {"saber_ops_per_s": round(sp["ops_per_s"], 1),
@@ -227,6 +234,7 @@ def main():
         pe.close()
         result["plain"] = plain
     if args.cpu_seconds > 0:
+        progress(f"cpu baseline ({args.cpu_seconds:.0f} s)")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     eng.close()
     print(json.dumps(result), flush=True)
