@@ -11,8 +11,9 @@
 //   host drivers                                   src/cuda_kernel.cu:21-335
 //
 // Per op the engine runs
-//   pre    lane-per-instance SHAKE256 chains (pkh, G2, noise stream) with the
-//          CDF sampler fused into the squeeze; secrets and errors leave as int8;
+//   pre    SHAKE256 chains (pkh, G2, noise stream) on lane pairs (two lanes
+//          per state, keccak2.hpp) with the CDF sampler fused into the
+//          squeeze; secrets and errors leave as int8;
 //   fused  one 1024-thread workgroup per instance; lane i squeezes row i of A
 //          (SHAKE128(LE16(i) || seedA)) in VGPRs and feeds it straight into
 //          v_mfma_i32_16x16x64_i8 as two byte limbs, so A never reaches memory:
@@ -20,7 +21,7 @@
 //            enc/dec B' = S' A + E'     (reduction across lanes: the row data is
 //                                        transposed by selector MFMAs first)
 //          followed by the epilogue (pack, V = S' B + E'', encode, compare);
-//   post   lane-per-instance SHAKE256 (pkh for keygen, ss for enc/dec).
+//   post   lane-pair SHAKE256 (pkh for keygen, ss for enc/dec).
 //
 // Limb split: a = 256 * hi + lo with hi = (int8)(a >> 8) and
 // lo = (int8)((a & 0xFF) ^ 0x80), so a == 256 hi + lo + 128 (mod 2^16);
@@ -29,6 +30,7 @@
 #include "common.hpp"
 #include "keccak.hpp"
 #include "sponge.hpp"
+#include "keccak2.hpp"
 #include "prof.hpp"
 
 namespace dpkem {
@@ -58,39 +60,53 @@ __device__ __forceinline__ int32_t frodo_sample(uint32_t r) {
   return (int32_t)((e ^ (0u - s)) + s);
 }
 
-// Four samples from one squeezed word, packed as int8 bytes.
-__device__ __forceinline__ uint32_t sample4_i8(uint64_t w, uint64_t* le16) {
-  uint32_t out = 0;
-  uint64_t le = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int32_t e = frodo_sample((uint32_t)(w >> (16 * i)) & 0xFFFF);
-    out |= ((uint32_t)e & 0xFF) << (8 * i);
-    le |= (uint64_t)((uint32_t)e & 0xFFFF) << (16 * i);
-  }
-  if (le16) *le16 = le;
-  return out;
+// ----------------------------------------------------------------- chains (lane pairs)
+//
+// Serial SHAKE256 chains run on lane pairs (keccak2.hpp): lanes 2k / 2k+1
+// hold the even / odd bit halves of instance k's state, 32 instances per
+// 64-thread block.  Squeezed noise words are rebuilt as 32-bit halves, so
+// each lane samples two of the four entries of every word.
+
+using k2::absorb2_global;
+using k2::absorb2_halves;
+using k2::absorb2_regs;
+using k2::store2_words;
+
+#define FR_PAIR_SETUP                                   \
+  const int lane = threadIdx.x & 63;                    \
+  const uint32_t p = lane & 1;                          \
+  const int inst = blockIdx.x * 32 + (lane >> 1);       \
+  if (inst >= n) return;
+
+// Two samples from the 32-bit half of a squeezed word: int8 pair (low 16
+// bits) and the little-endian u16 pair (full 32 bits).
+__device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t* le) {
+  const int32_t e0 = frodo_sample(x & 0xFFFF), e1 = frodo_sample(x >> 16);
+  *le = ((uint32_t)e0 & 0xFFFF) | ((uint32_t)e1 << 16);
+  return ((uint32_t)e0 & 0xFF) | (((uint32_t)e1 & 0xFF) << 8);
 }
 
-// SHAKE256(dom || seed[24 bytes]), squeezing OUTW words into f(word index, word).
+// SHAKE256(dom || seed[24 bytes]) on a lane pair, squeezing OUTW words; f(w,
+// half) receives this lane's 32-bit half (low half for the even lane, high
+// for the odd lane) of every output word w.
 template <int OUTW, class F>
-__device__ __forceinline__ void shake256_seed_squeeze(uint64_t dom, const uint64_t seed[3], F&& f) {
-  uint64_t A[25];
-  keccak_zero(A);
-  A[0] = dom | (seed[0] << 8);
-  A[1] = (seed[0] >> 56) | (seed[1] << 8);
-  A[2] = (seed[1] >> 56) | (seed[2] << 8);
-  A[3] = (seed[2] >> 56) | (kDsShake << 8);
-  A[kRateShake256 - 1] ^= kPadEnd;
-  keccakf(A);
+__device__ __forceinline__ void shake256_seed_squeeze2(uint32_t p, uint64_t dom, const uint64_t seed[3], F&& f) {
+  const uint64_t m[4] = {dom | (seed[0] << 8), (seed[0] >> 56) | (seed[1] << 8), (seed[1] >> 56) | (seed[2] << 8),
+                         (seed[2] >> 56) | (kDsShake << 8)};
+  uint32_t a[25];
+  k2::zero2(a);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a[i] = k2::deint(m[i], p);
+  a[kRateShake256 - 1] ^= k2::half_const(kPadEnd, p);
+  k2::keccakf2(a, p);
   constexpr int NB = OUTW / kRateShake256, REM = OUTW % kRateShake256;
   for (int b = 0; b < NB; ++b) {
 #pragma unroll
-    for (int i = 0; i < kRateShake256; ++i) f(b * kRateShake256 + i, A[i]);
-    keccakf(A);
+    for (int i = 0; i < kRateShake256; ++i) f(b * kRateShake256 + i, k2::out_half(a[i], p));
+    k2::keccakf2(a, p);
   }
 #pragma unroll
-  for (int i = 0; i < REM; ++i) f(NB * kRateShake256 + i, A[i]);
+  for (int i = 0; i < REM; ++i) f(NB * kRateShake256 + i, k2::out_half(a[i], p));
 }
 
 // ----------------------------------------------------------------- keygen chains
@@ -101,45 +117,42 @@ __device__ __forceinline__ void shake256_seed_squeeze(uint64_t dom, const uint64
 __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                        uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
-  const int inst = blockIdx.x * 64 + threadIdx.x;
-  if (inst >= n) return;
+  FR_PAIR_SETUP
   const uint8_t* c = coins + (size_t)inst * kFrodoKeygenCoins;
-  uint8_t* p = pk + (size_t)inst * kFrodoPk;
+  uint8_t* pp = pk + (size_t)inst * kFrodoPk;
   uint8_t* q = sk + (size_t)inst * kFrodoSk;
   {
-    uint64_t z[2] = {ld64(c + 48), ld64(c + 56)}, A[25];
-    absorb_regs<kRateShake256, 2>(A, z, kDsShake);
-    st64(p, A[0]);
-    st64(p + 8, A[1]);
-    st64(q + 24, A[0]);
-    st64(q + 32, A[1]);
+    const uint64_t z[2] = {ld64(c + 48), ld64(c + 56)};
+    uint32_t a[25];
+    absorb2_regs<kRateShake256, 2>(a, p, z, kDsShake);
+    store2_words<2>(a, p, pp);
+    store2_words<2>(a, p, q + 24);
+    const uint32_t* sc = reinterpret_cast<const uint32_t*>(c);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) st64(q + 8 * i, ld64(c + 8 * i));
+    for (int i = 0; i < 3; ++i) reinterpret_cast<uint32_t*>(q)[2 * i + p] = sc[2 * i + p];
   }
   const uint64_t seed[3] = {ld64(c + 24), ld64(c + 32), ld64(c + 40)};
-  uint32_t* sd = s8 + (size_t)inst * (kFMat / 4);
-  uint32_t* ed = e8 + (size_t)inst * (kFMat / 4);
-  uint8_t* skS = q + kFSkS;
-  shake256_seed_squeeze<2 * kFMat / 4>(0x5F, seed, [&](int w, uint64_t v) {
+  uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (kFMat / 4)) + 2 * p;
+  uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (kFMat / 4)) + 2 * p;
+  uint8_t* skS = q + kFSkS + 4 * p;
+  shake256_seed_squeeze2<2 * kFMat / 4>(p, 0x5F, seed, [&](int w, uint32_t x) {
+    uint32_t le;
+    const uint32_t b = sample2(x, &le);
     if (w < kFMat / 4) {
-      uint64_t le;
-      sd[w] = sample4_i8(v, &le);
-      st64(skS + 8 * w, le);
+      *reinterpret_cast<uint16_t*>(sd + 4 * w) = (uint16_t)b;
+      *reinterpret_cast<uint32_t*>(skS + 8 * w) = le;
     } else {
-      ed[w - kFMat / 4] = sample4_i8(v, nullptr);
+      *reinterpret_cast<uint16_t*>(ed + 4 * (w - kFMat / 4)) = (uint16_t)b;
     }
   });
 }
 
 // pkh = SHAKE256(pk, 24); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
 __global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
-  const int inst = blockIdx.x * 64 + threadIdx.x;
-  if (inst >= n) return;
-  uint64_t A[25];
-  absorb_global<kRateShake256, kFPkWords>(A, pk + (size_t)inst * kFrodoPk, kDsShake);
-  uint8_t* q = sk + (size_t)inst * kFrodoSk + kFSkPkh;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) st64(q + 8 * i, A[i]);
+  FR_PAIR_SETUP
+  uint32_t a[25];
+  absorb2_global<kRateShake256, kFPkWords>(a, p, pk + (size_t)inst * kFrodoPk, kDsShake);
+  store2_words<3>(a, p, sk + (size_t)inst * kFrodoSk + kFSkPkh);
 }
 
 // ----------------------------------------------------------------- enc / dec chains
@@ -152,33 +165,32 @@ __global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __r
                                                        const uint8_t* __restrict__ mu, size_t mu_stride,
                                                        uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
                                                        uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8) {
-  const int inst = blockIdx.x * 64 + threadIdx.x;
-  if (inst >= n) return;
+  FR_PAIR_SETUP
   const uint8_t* key = keys + (size_t)inst * key_stride;
-  uint64_t g2in[6], A[25];
+  uint32_t a[25], g[6];
   if (DEC) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g2in[i] = ld64(key + kFSkPkh + 8 * i);
+    for (int i = 0; i < 3; ++i) g[i] = k2::deint(ld64(key + kFSkPkh + 8 * i), p);
   } else {
-    absorb_global<kRateShake256, kFPkWords>(A, key, kDsShake);
+    absorb2_global<kRateShake256, kFPkWords>(a, p, key, kDsShake);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g2in[i] = A[i];
+    for (int i = 0; i < 3; ++i) g[i] = a[i];
   }
   const uint8_t* m = mu + (size_t)inst * mu_stride;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) g2in[3 + i] = ld64(m + 8 * i);
-  absorb_regs<kRateShake256, 6>(A, g2in, kDsShake);
-  const uint64_t seed[3] = {A[0], A[1], A[2]};
-#pragma unroll
-  for (int i = 0; i < 3; ++i) st64(kbuf + (size_t)inst * 64 + 8 * i, A[3 + i]);
-  uint32_t* sd = s8 + (size_t)inst * (kFMat / 4);
-  uint32_t* ed = e8 + (size_t)inst * (kFMat / 4);
-  uint32_t* pd = epp8 + (size_t)inst * 16;
-  shake256_seed_squeeze<(2 * kFMat + 64) / 4>(0x96, seed, [&](int w, uint64_t v) {
-    const uint32_t x = sample4_i8(v, nullptr);
-    if (w < kFMat / 4) sd[w] = x;
-    else if (w < 2 * kFMat / 4) ed[w - kFMat / 4] = x;
-    else pd[w - 2 * kFMat / 4] = x;
+  for (int i = 0; i < 3; ++i) g[3 + i] = k2::deint(ld64(m + 8 * i), p);
+  absorb2_halves<kRateShake256, 6>(a, p, g, kDsShake);
+  store2_words<3>(a + 3, p, kbuf + (size_t)inst * 64);
+  const uint64_t seed[3] = {k2::out_full(a[0], p), k2::out_full(a[1], p), k2::out_full(a[2], p)};
+  uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (kFMat / 4)) + 2 * p;
+  uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (kFMat / 4)) + 2 * p;
+  uint8_t* pd = reinterpret_cast<uint8_t*>(epp8 + (size_t)inst * 16) + 2 * p;
+  shake256_seed_squeeze2<(2 * kFMat + 64) / 4>(p, 0x96, seed, [&](int w, uint32_t x) {
+    uint32_t le;
+    const uint16_t b = (uint16_t)sample2(x, &le);
+    if (w < kFMat / 4) *reinterpret_cast<uint16_t*>(sd + 4 * w) = b;
+    else if (w < 2 * kFMat / 4) *reinterpret_cast<uint16_t*>(ed + 4 * (w - kFMat / 4)) = b;
+    else *reinterpret_cast<uint16_t*>(pd + 4 * (w - 2 * kFMat / 4)) = b;
   });
 }
 
@@ -186,35 +198,32 @@ __global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __r
 __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ kbuf,
                                                const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
                                                uint8_t* __restrict__ ss) {
-  const int inst = blockIdx.x * 64 + threadIdx.x;
-  if (inst >= n) return;
+  FR_PAIR_SETUP
   const uint8_t* c = ct + (size_t)inst * kFrodoCt;
   constexpr int RW = kRateShake256;
   constexpr int NB = (kFCtWords + 3) / RW;           // 115 full blocks
   constexpr int REM = kFCtWords - NB * RW;           // 13 ct words in the last block
   static_assert(REM + 3 == RW - 1, "k and padding share the final block");
-  uint64_t A[25];
-  keccak_zero(A);
+  uint32_t a[25];
+  k2::zero2(a);
   for (int b = 0; b < NB; ++b) {
 #pragma unroll
-    for (int i = 0; i < RW; ++i) A[i] ^= ld64(c + 8 * (b * RW + i));
-    keccakf(A);
+    for (int i = 0; i < RW; ++i) a[i] ^= k2::deint(ld64(c + 8 * (b * RW + i)), p);
+    k2::keccakf2(a, p);
   }
 #pragma unroll
-  for (int i = 0; i < REM; ++i) A[i] ^= ld64(c + 8 * (NB * RW + i));
+  for (int i = 0; i < REM; ++i) a[i] ^= k2::deint(ld64(c + 8 * (NB * RW + i)), p);
   uint64_t mask = 0;
   if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const uint64_t k = ld64(kbuf + (size_t)inst * 64 + 8 * i);
     const uint64_t s = sk ? ld64(sk + (size_t)inst * kFrodoSk + 8 * i) : 0;
-    A[REM + i] ^= (k & ~mask) | (s & mask);
+    a[REM + i] ^= k2::deint((k & ~mask) | (s & mask), p);
   }
-  A[RW - 1] ^= kDsShake ^ kPadEnd;
-  keccakf(A);
-  uint8_t* o = ss + (size_t)inst * kFrodoSs;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) st64(o + 8 * i, A[i]);
+  a[RW - 1] ^= k2::half_const(kDsShake ^ kPadEnd, p);
+  k2::keccakf2(a, p);
+  store2_words<3>(a, p, ss + (size_t)inst * kFrodoSs);
 }
 
 // decaps: W = C - B' S, mu' = Decode(W).  256 threads per instance.
@@ -516,32 +525,32 @@ static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
                          uint32_t* e8) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("frodo_keygen_pre", frodo_keygen_pre, dim3(blocks_for(n, 64)), dim3(64), st, n, coins, pk, sk, s8, e8);
+  DPKEM_LAUNCH("frodo_keygen_pre", frodo_keygen_pre, dim3(blocks_for(n, 32)), dim3(64), st, n, coins, pk, sk, s8, e8);
   DPKEM_LAUNCH("frodo_as_fused", frodo_as_fused, dim3(n), dim3(1024), st, n, pk, sk, s8, reinterpret_cast<const int8_t*>(e8));
-  DPKEM_LAUNCH("frodo_keygen_post", frodo_keygen_post, dim3(blocks_for(n, 64)), dim3(64), st, n, pk, sk);
+  DPKEM_LAUNCH("frodo_keygen_post", frodo_keygen_post, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
 }
 
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 64)), dim3(64), st, n, pk, (size_t)kFrodoPk, coins,
+  DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, (size_t)kFrodoPk, coins,
                      (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8);
   DPKEM_LAUNCH("frodo_sa_fused<false>", frodo_sa_fused<false>, dim3(n), dim3(1024), st, n, pk, (size_t)kFrodoPk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), coins,
                      (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, kbuf, nullptr, nullptr, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, nullptr, ss);
 }
 
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
                       uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail) {
   if (n <= 0) return;
   DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, mubuf);
-  DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 64)), dim3(64), st, n, sk, (size_t)kFrodoSk, mubuf,
+  DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk, (size_t)kFrodoSk, mubuf,
                      (size_t)32, kbuf, s8, e8, epp8);
   DPKEM_LAUNCH("frodo_sa_fused<true>", frodo_sa_fused<true>, dim3(n), dim3(1024), st, n, sk + kFrodoSec, (size_t)kFrodoSk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32,
                      nullptr, ct, fail);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, kbuf, sk, fail, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, fail, ss);
 }
 
 }  // namespace dpkem

@@ -11,15 +11,17 @@
 // Keygen, absent from the reference, follows the published scheme.
 //
 // Design: each op is three launches.
-//   hash1  lane-per-instance Keccak, split in two waves per 64 instances so
-//          GenMatrix (23 permutations) runs beside the message / secret chain;
+//   hash1  lane-pair Keccak (two lanes per state, keccak2.hpp), split in two
+//          waves per 32 instances so GenMatrix (23 permutations) runs beside
+//          the message / secret chain;
 //   poly   one workgroup of 128 threads per instance: unpacking, CBD, the
 //          negacyclic products as LDS-staged Toeplitz dot products
 //          (v_dot2_u32_u16, 8 outputs per thread), rounding and packing;
-//   hash2  lane-per-instance closing hashes (H(pk) or H(ct) and K).
+//   hash2  lane-pair closing hashes (H(pk) or H(ct) and K).
 #include "common.hpp"
 #include "keccak.hpp"
 #include "sponge.hpp"
+#include "keccak2.hpp"
 #include "prof.hpp"
 #include "saber_poly.hpp"
 
@@ -29,37 +31,54 @@ constexpr int kSabGenMatrixWords = kSaberGenMatrixBytes / 8;  // 468
 constexpr int kSabGenSecretWords = kSaberGenSecretBytes / 8;  // 96
 
 // ----------------------------------------------------------------- hash1
-// blockDim = 128: wave 0 runs role 0 and wave 1 role 1 for the same 64 instances.
+// Lane-pair sponges (keccak2.hpp): lanes 2k / 2k+1 hold the even / odd bit
+// halves of instance k's state.  blockDim = 128: wave 0 runs role 0 and wave
+// 1 role 1 for the same 32 instances.  GenMatrix / GenSecret streams leave in
+// the sliced layout (half of word w at u32 index 2w + p).
+
+using k2::absorb2_global;
+using k2::absorb2_halves;
+using k2::absorb2_regs;
+using k2::squeeze2_sliced;
+using k2::store2_words;
+
+#define SAB_PAIR_SETUP(PER_BLOCK)                        \
+  const int lane = threadIdx.x & 63;                     \
+  const uint32_t p = lane & 1;                           \
+  const int inst = blockIdx.x * 32 + (lane >> 1);        \
+  if (inst >= n) return;
 
 __global__ void __launch_bounds__(128) sab_keygen_hash1(int n, const uint8_t* __restrict__ coins,
                                                         uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                         uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
   const int role = threadIdx.x >> 6;
-  const int inst = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (inst >= n) return;
+  SAB_PAIR_SETUP(32)
   const uint8_t* c = coins + (size_t)inst * kSaberKeygenCoins;
+  uint32_t a[25];
   if (role == 0) {
-    uint64_t m[4], A[25];
+    uint64_t m[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) m[i] = ld64(c + 8 * i);
-    absorb_regs<kRateShake128, 4>(A, m, kDsShake);  // seedA = SHAKE128(seedA_raw, 32)
-    uint8_t* p = pk + (size_t)inst * kSaberPk + kSaberPolyVecCompBytes;
-    uint8_t* q = sk + (size_t)inst * kSaberSk + kSaberPolyVecBytes + kSaberPolyVecCompBytes;
+    absorb2_regs<kRateShake128, 4>(a, p, m, kDsShake);  // seedA = SHAKE128(seedA_raw, 32)
+    uint32_t h[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      m[i] = A[i];
-      st64(p + 8 * i, A[i]);
-      st64(q + 8 * i, A[i]);
-    }
-    shake128_regs_to_global<4, kSabGenMatrixWords>(araw + (size_t)inst * kSaberGenMatrixBytes, m);
+    for (int i = 0; i < 4; ++i) h[i] = a[i];
+    store2_words<4>(h, p, pk + (size_t)inst * kSaberPk + kSaberPolyVecCompBytes);
+    store2_words<4>(h, p, sk + (size_t)inst * kSaberSk + kSaberPolyVecBytes + kSaberPolyVecCompBytes);
+    absorb2_halves<kRateShake128, 4>(a, p, h, kDsShake);  // GenMatrix = SHAKE128(seedA, 3744)
+    squeeze2_sliced<kRateShake128, kSabGenMatrixWords>(
+        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * kSaberGenMatrixBytes));
   } else {
     uint64_t m[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) m[i] = ld64(c + 32 + 8 * i);
-    shake128_regs_to_global<4, kSabGenSecretWords>(sraw + (size_t)inst * kSaberGenSecretBytes, m);
-    uint8_t* z = sk + (size_t)inst * kSaberSk + kSaberSk - 32;
+    absorb2_regs<kRateShake128, 4>(a, p, m, kDsShake);  // GenSecret stream
+    squeeze2_sliced<kRateShake128, kSabGenSecretWords>(
+        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * kSaberGenSecretBytes));
+    uint32_t* z = reinterpret_cast<uint32_t*>(sk + (size_t)inst * kSaberSk + kSaberSk - 32);
+    const uint32_t* zc = reinterpret_cast<const uint32_t*>(c + 64);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) st64(z + 8 * i, ld64(c + 64 + 8 * i));
+    for (int i = 0; i < 4; ++i) z[2 * i + p] = zc[2 * i + p];
   }
 }
 
@@ -68,26 +87,36 @@ __global__ void __launch_bounds__(128) sab_enc_hash1(int n, const uint8_t* __res
                                                      uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
                                                      uint8_t* __restrict__ krbuf) {
   const int role = threadIdx.x >> 6;
-  const int inst = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (inst >= n) return;
-  const uint8_t* p = pk + (size_t)inst * kSaberPk;
+  SAB_PAIR_SETUP(32)
+  const uint8_t* pkp = pk + (size_t)inst * kSaberPk;
+  uint32_t a[25];
   if (role == 0) {
     uint64_t seed[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) seed[i] = ld64(p + kSaberPolyVecCompBytes + 8 * i);
-    shake128_regs_to_global<4, kSabGenMatrixWords>(araw + (size_t)inst * kSaberGenMatrixBytes, seed);
+    for (int i = 0; i < 4; ++i) seed[i] = ld64(pkp + kSaberPolyVecCompBytes + 8 * i);
+    absorb2_regs<kRateShake128, 4>(a, p, seed, kDsShake);
+    squeeze2_sliced<kRateShake128, kSabGenMatrixWords>(
+        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * kSaberGenMatrixBytes));
   } else {
-    uint64_t raw[4], buf[8], kr[8];
+    uint64_t raw[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) raw[i] = ld64(coins + (size_t)inst * kSaberEncCoins + 8 * i);
-    sha3_256_regs<4>(buf, raw);                 // m = H(m_raw)
-    sha3_256_global<kSaberPk / 8>(buf + 4, p);  // H(pk)
-    sha3_512_64(kr, buf);                       // kr = G(m || H(pk))
+    uint32_t g[8];
+    absorb2_regs<kRateSha3_256, 4>(a, p, raw, kDsSha3);  // m = H(m_raw)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) st64(mbuf + (size_t)inst * 32 + 8 * i, buf[i]);
+    for (int i = 0; i < 4; ++i) g[i] = a[i];
+    absorb2_global<kRateSha3_256, kSaberPk / 8>(a, p, pkp, kDsSha3);  // H(pk)
 #pragma unroll
-    for (int i = 0; i < 8; ++i) st64(krbuf + (size_t)inst * 64 + 8 * i, kr[i]);
-    shake128_regs_to_global<4, kSabGenSecretWords>(sraw + (size_t)inst * kSaberGenSecretBytes, kr + 4);
+    for (int i = 0; i < 4; ++i) g[4 + i] = a[i];
+    store2_words<4>(g, p, mbuf + (size_t)inst * 32);
+    absorb2_halves<kRateSha3_512, 8>(a, p, g, kDsSha3);  // kr = G(m || H(pk))
+    uint32_t kr[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) kr[i] = a[i];
+    store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
+    absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);  // GenSecret(kr[32:64])
+    squeeze2_sliced<kRateShake128, kSabGenSecretWords>(
+        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * kSaberGenSecretBytes));
   }
 }
 
@@ -95,38 +124,41 @@ __global__ void __launch_bounds__(128) sab_dec_hash1(int n, const uint8_t* __res
                                                      const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
                                                      uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
   const int role = threadIdx.x >> 6;
-  const int inst = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (inst >= n) return;
+  SAB_PAIR_SETUP(32)
   const uint8_t* s = sk + (size_t)inst * kSaberSk;
+  uint32_t a[25];
   if (role == 0) {
     uint64_t seed[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) seed[i] = ld64(s + kSaberPolyVecBytes + kSaberPolyVecCompBytes + 8 * i);
-    shake128_regs_to_global<4, kSabGenMatrixWords>(araw + (size_t)inst * kSaberGenMatrixBytes, seed);
+    absorb2_regs<kRateShake128, 4>(a, p, seed, kDsShake);
+    squeeze2_sliced<kRateShake128, kSabGenMatrixWords>(
+        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * kSaberGenMatrixBytes));
   } else {
-    uint64_t buf[8], kr[8];
+    uint64_t buf[8];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      buf[i] = ld64(mbuf + (size_t)inst * 32 + 8 * i);      // m'
-      buf[4 + i] = ld64(s + kSaberSk - 64 + 8 * i);         // H(pk) from sk
+      buf[i] = ld64(mbuf + (size_t)inst * 32 + 8 * i);  // m'
+      buf[4 + i] = ld64(s + kSaberSk - 64 + 8 * i);     // H(pk) from sk
     }
-    sha3_512_64(kr, buf);
+    absorb2_regs<kRateSha3_512, 8>(a, p, buf, kDsSha3);
+    uint32_t kr[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) st64(krbuf + (size_t)inst * 64 + 8 * i, kr[i]);
-    shake128_regs_to_global<4, kSabGenSecretWords>(sraw + (size_t)inst * kSaberGenSecretBytes, kr + 4);
+    for (int i = 0; i < 8; ++i) kr[i] = a[i];
+    store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
+    absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
+    squeeze2_sliced<kRateShake128, kSabGenSecretWords>(
+        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * kSaberGenSecretBytes));
   }
 }
 
 // ----------------------------------------------------------------- hash2
 
 __global__ void __launch_bounds__(64) sab_keygen_hash2(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
-  const int inst = blockIdx.x * 64 + threadIdx.x;
-  if (inst >= n) return;
-  uint64_t h[4];
-  sha3_256_global<kSaberPk / 8>(h, pk + (size_t)inst * kSaberPk);
-  uint8_t* d = sk + (size_t)inst * kSaberSk + kSaberSk - 64;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) st64(d + 8 * i, h[i]);
+  SAB_PAIR_SETUP(32)
+  uint32_t a[25];
+  absorb2_global<kRateSha3_256, kSaberPk / 8>(a, p, pk + (size_t)inst * kSaberPk, kDsSha3);
+  store2_words<4>(a, p, sk + (size_t)inst * kSaberSk + kSaberSk - 64);
 }
 
 // K = SHA3-256(kr[0:32] || H(ct)); for decaps kr[0:32] is replaced by z when
@@ -134,22 +166,21 @@ __global__ void __launch_bounds__(64) sab_keygen_hash2(int n, const uint8_t* __r
 __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ krbuf,
                                                 const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
                                                 uint8_t* __restrict__ ss) {
-  const int inst = blockIdx.x * 64 + threadIdx.x;
-  if (inst >= n) return;
-  uint64_t m[8];
-  sha3_256_global<kSaberCt / 8>(m + 4, ct + (size_t)inst * kSaberCt);
+  SAB_PAIR_SETUP(32)
+  uint32_t a[25], m[8];
+  absorb2_global<kRateSha3_256, kSaberCt / 8>(a, p, ct + (size_t)inst * kSaberCt, kDsSha3);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) m[4 + i] = a[i];
   uint64_t mask = 0;
   if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    uint64_t k = ld64(krbuf + (size_t)inst * 64 + 8 * i);
-    uint64_t z = sk ? ld64(sk + (size_t)inst * kSaberSk + kSaberSk - 32 + 8 * i) : 0;
-    m[i] = (k & ~mask) | (z & mask);
+    const uint64_t k = ld64(krbuf + (size_t)inst * 64 + 8 * i);
+    const uint64_t z = sk ? ld64(sk + (size_t)inst * kSaberSk + kSaberSk - 32 + 8 * i) : 0;
+    m[i] = k2::deint((k & ~mask) | (z & mask), p);
   }
-  uint64_t h[4];
-  sha3_256_regs<8>(h, m);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) st64(ss + (size_t)inst * kSaberSs + 8 * i, h[i]);
+  absorb2_halves<kRateSha3_256, 8>(a, p, m, kDsSha3);
+  store2_words<4>(a, p, ss + (size_t)inst * kSaberSs);
 }
 
 // ----------------------------------------------------------------- products
@@ -172,7 +203,7 @@ struct SabPolyArgs {
 struct SabPolyLds {
   uint16_t T[12 * kToepStride];  // 9 A polynomials + 3 b polynomials
   uint16_t S[3 * 256];
-  uint32_t stage[kSaberGenMatrixBytes / 4];
+  alignas(8) uint32_t stage[kSaberGenMatrixBytes / 4];
   uint8_t outb[kSaberCt + 64];
   uint32_t diff;
 };
@@ -185,15 +216,17 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
   if (inst >= a.n) return;
 
   // 1. stage GenMatrix bytes, expand GenSecret with CBD (mu = 8)
+  // the hash kernels leave both streams in the lane-pair sliced layout
   const uint32_t* ar = reinterpret_cast<const uint32_t*>(a.araw + (size_t)inst * kSaberGenMatrixBytes);
-  for (int i = tid; i < kSaberGenMatrixBytes / 4; i += 128) L.stage[i] = ar[i];
+  for (int w = tid; w < kSabGenMatrixWords; w += 128)
+    reinterpret_cast<uint64_t*>(L.stage)[w] = k2::sliced_word(ar, w);
   const uint32_t* sr = reinterpret_cast<const uint32_t*>(a.sraw + (size_t)inst * kSaberGenSecretBytes);
-  for (int i = tid; i < kSaberGenSecretBytes / 4; i += 128) {
-    const uint32_t w = sr[i];
+  for (int w = tid; w < kSabGenSecretWords; w += 128) {
+    const uint64_t v = k2::sliced_word(sr, w);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const uint32_t byte = (w >> (8 * b)) & 0xFF;
-      L.S[4 * i + b] = (uint16_t)(popc(byte & 0xF) - popc(byte >> 4));
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t byte = (uint32_t)(v >> (8 * b)) & 0xFF;
+      L.S[8 * w + b] = (uint16_t)(popc(byte & 0xF) - popc(byte >> 4));
     }
   }
   if (tid == 0) L.diff = 0;
@@ -364,29 +397,29 @@ static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
                          uint8_t* sraw) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, dim3(blocks_for(n, 64)), dim3(128), st, n, coins, pk, sk, araw, sraw);
+  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, dim3(blocks_for(n, 32)), dim3(128), st, n, coins, pk, sk, araw, sraw);
   SabPolyArgs a{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
   DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), dim3(128), st, a);
-  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, dim3(blocks_for(n, 64)), dim3(64), st, n, pk, sk);
+  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
 }
 
 void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, dim3(blocks_for(n, 64)), dim3(128), st, n, pk, coins, araw, sraw, mbuf, krbuf);
+  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, dim3(blocks_for(n, 32)), dim3(128), st, n, pk, coins, araw, sraw, mbuf, krbuf);
   SabPolyArgs a{n, araw, sraw, pk, (size_t)kSaberPk, mbuf, ct, nullptr, nullptr, nullptr};
   DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), dim3(128), st, a);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, krbuf, nullptr, nullptr, ss);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, nullptr, nullptr, ss);
 }
 
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
                       uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail) {
   if (n <= 0) return;
   DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), dim3(64), st, n, ct, sk, mbuf);
-  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, dim3(blocks_for(n, 64)), dim3(128), st, n, sk, mbuf, araw, sraw, krbuf);
+  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, dim3(blocks_for(n, 32)), dim3(128), st, n, sk, mbuf, araw, sraw, krbuf);
   SabPolyArgs a{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
   DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), dim3(128), st, a);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 64)), dim3(64), st, n, ct, krbuf, sk, fail, ss);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail, ss);
 }
 
 }  // namespace dpkem

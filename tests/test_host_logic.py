@@ -1,0 +1,31 @@
+"""Host-side logic of the device code, compiled for the CPU.
+
+The lane-pair Keccak round template in dpcrypto_amd/csrc/keccak2.hpp is run
+in a two-lane host emulation (partner exchange = swapping the two lanes) and
+compared with the oracle's permutation; the Saber Toeplitz helpers in
+saber_poly.hpp are compared with the oracle's negacyclic product.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CXX = shutil.which("g++") or shutil.which("clang++")
+
+
+def _build_and_run(src, tmp_path, oracle):
+    exe = str(tmp_path / "prog")
+    odir = os.path.join(ROOT, "oracle")
+    subprocess.run([CXX, "-O1", "-std=c++17", os.path.join(HERE, "host", src), "-o", exe,
+                    f"-L{odir}", "-loracle", f"-Wl,-rpath,{odir}"], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    return out.returncode, out.stdout
+
+
+@pytest.mark.skipif(CXX is None, reason="no host C++ compiler")
+def test_lane_pair_keccak_matches_oracle(tmp_path, oracle):
+    rc, out = _build_and_run("k2_emul.cpp", tmp_path, oracle)
+    assert rc == 0 and out.strip() == "OK", out
