@@ -114,6 +114,56 @@ def run_workload(torch, eng, scheme, n, steps, warmup, seed, profile=False):
                 ops_per_s=3 * n * steps / wall)
 
 
+def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=60.0):
+    """Time `steps` keygen+enc+dec steps, each one replay of a graph-captured
+    plan (keygen -> enc -> dec chained per instance chunk); the plan's
+    (streams, chunks) schedule is picked by the autotuner first."""
+    import numpy as np
+    import dpcrypto_amd as D
+    from dpcrypto_amd import autotune
+    s = D.SIZES[scheme]
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    kc = torch.from_numpy(rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)).to(dev)
+    ec = torch.from_numpy(rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)).to(dev)
+    pk = torch.empty((n, s["pk"]), dtype=torch.uint8, device=dev)
+    sk = torch.empty((n, s["sk"]), dtype=torch.uint8, device=dev)
+    ct = torch.empty((n, s["ct"]), dtype=torch.uint8, device=dev)
+    ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
+    ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    eng.set_stream(stream.cuda_stream)
+
+    def build(ns, ch):
+        p = eng.plan(nstreams=ns, chunks=ch)
+        t0 = p.keypair(scheme, pk, sk, kc)
+        t1 = p.enc(scheme, ct, ss, pk, ec, after=t0)
+        p.dec(scheme, ss2, ct, sk, after=t1)
+        return p.build()
+
+    best, tried = autotune.tune(torch, eng, f"{'frodo976' if scheme == 0 else 'saber'}|kg-enc-dec|n={n}", build,
+                                use_cache=False, budget_s=tune_budget_s)
+    plan = build(*best)
+    for _ in range(warmup):
+        plan.launch()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    stops = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        plan.launch()
+        stops[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_step = [a.elapsed_time(b) for a, b in zip(starts, stops)]
+    ok = bool(torch.equal(ss, ss2))
+    plan.close()
+    return dict(wall_s=wall, per_step_ms=per_step, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
+                schedule={"nstreams": best[0], "chunks": best[1],
+                          "tuning_ms": {f"{a}x{b}": round(v, 4) for (a, b), v in tried.items()}})
+
+
 def roofline_from(stats, n):
     """Dominant kernel by total time -> achieved VALU lane-ops/s vs peak."""
     if not stats:
@@ -185,8 +235,11 @@ def main():
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     eng = D.Engine(0)
-    progress(f"saber batch {args.saber_batch}: {args.warmup} warmup + {args.steps} timed steps")
-    sab = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001, True)
+    progress(f"saber batch {args.saber_batch}: autotune, {args.warmup} warmup + {args.steps} timed graph steps")
+    sab = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001)
+    progress(f"saber: {sab['ops_per_s']:.0f} ops/s with schedule {sab['schedule']['nstreams']}x{sab['schedule']['chunks']}")
+    progress("saber eager pass with per-kernel HIP events")
+    sab_eager = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, 1, 0x44504B454D000001, True)
     result = {
         "metric": "KEM ops/s (keygen+enc+dec) Frodo-976 & Saber at 1 GPU; p99 latency",
         "value": round(sab["ops_per_s"], 1),
@@ -204,22 +257,28 @@ def main():
                    "global_batch": args.saber_batch, "parallelism": "instances (1 GPU)"},
         "latency_ms": {"p50": round(pct(sab["per_step_ms"], 0.5), 4), "p99": round(pct(sab["per_step_ms"], 0.99), 4),
                        "what": "per-step (batch) latency"},
-        "roundtrip_ok": sab["roundtrip_ok"],
-        "roofline": roofline_from(sab["kernels"], args.saber_batch),
-        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab["kernels"].items()},
+        "roundtrip_ok": sab["roundtrip_ok"] and sab_eager["roundtrip_ok"],
+        "schedule": sab["schedule"],
+        "eager_ops_per_s": round(sab_eager["ops_per_s"], 1),
+        "roofline": roofline_from(sab_eager["kernels"], args.saber_batch),
+        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
     }
-    progress(f"saber: {sab['ops_per_s']:.0f} ops/s")
     if not args.skip_frodo:
-        progress(f"frodo976 batch {args.frodo_batch}: 1 warmup + {args.frodo_steps} timed steps")
-        fr = run_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002, True)
+        progress(f"frodo976 batch {args.frodo_batch}: autotune, 1 warmup + {args.frodo_steps} timed graph steps")
+        fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002)
+        progress(f"frodo976: {fr['ops_per_s']:.0f} ops/s; eager pass with per-kernel HIP events")
+        fr_eager = run_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002, True)
         result["frodo976"] = {
             "workload": f"frodokem-976-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
             "ops_per_s": round(fr["ops_per_s"], 1),
             "ms_per_step": round(fr["wall_s"] * 1e3 / args.frodo_steps, 3),
+            "latency_ms": {"p50": round(pct(fr["per_step_ms"], 0.5), 3), "p99": round(pct(fr["per_step_ms"], 0.99), 3)},
             "steps": args.frodo_steps,
-            "roundtrip_ok": fr["roundtrip_ok"],
-            "roofline": roofline_from(fr["kernels"], args.frodo_batch),
-            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr["kernels"].items()},
+            "roundtrip_ok": fr["roundtrip_ok"] and fr_eager["roundtrip_ok"],
+            "schedule": fr["schedule"],
+            "eager_ops_per_s": round(fr_eager["ops_per_s"], 1),
+            "roofline": roofline_from(fr_eager["kernels"], args.frodo_batch),
+            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_eager["kernels"].items()},
         }
     if not args.skip_plain:
         progress("plain comparator forms")

@@ -15,12 +15,7 @@
 #include <mutex>
 #include <vector>
 
-#include "../../include/dpkem.h"
-#include "launch.hpp"
-#include "prof.hpp"
-
-#include <map>
-#include <string>
+#include "engine_internal.hpp"
 
 using namespace dpkem;
 
@@ -49,21 +44,6 @@ struct Carver {
 };
 
 }  // namespace
-
-struct dpkem_engine {
-  int device = 0;
-  size_t budget = 0;
-  hipStream_t own_stream = nullptr;
-  hipStream_t stream = nullptr;
-  int form = DPKEM_FORM_BATCHED;
-  uint8_t* arena = nullptr;
-  size_t arena_bytes = 0;
-  char last_err[256] = {0};
-  std::mutex mu;
-  bool profiling = false;
-  Profiler prof;
-  std::map<std::string, std::pair<long, double>> kstats;  // name -> (launches, total ms)
-};
 
 namespace {
 
@@ -96,14 +76,17 @@ int ensure_arena(dpkem_engine* e, size_t bytes) {
   return DPKEM_OK;
 }
 
-// Bytes of device scratch one instance needs for (scheme, op) in the current
-// form, excluding staged I/O.
+}  // namespace
+
+namespace dpkem {
+
+int record_hip_error(dpkem_engine* e, hipError_t err, const char* what) { return fail_hip(e, err, what); }
+
+// Bytes of device scratch one instance needs in the given form, excluding staged I/O.
 size_t scratch_per_inst(int scheme, int form) {
   if (form == DPKEM_FORM_PLAIN) return align_up(plain_scratch_per_inst(scheme));
   return scheme == DPKEM_FRODO976 ? align_up(kFrodoScratchPerInst) : align_up(kSaberScratchPerInst);
 }
-
-enum Op { kKeygen = 0, kEnc = 1, kDec = 2 };
 
 // I/O byte sizes per instance: two outputs and two inputs (0 if unused).
 void io_sizes(int scheme, int op, size_t out[2], size_t in[2]) {
@@ -118,11 +101,10 @@ void io_sizes(int scheme, int op, size_t out[2], size_t in[2]) {
 }
 
 // Launch one chunk with all pointers on the device.
-void launch_chunk(dpkem_engine* e, int scheme, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
-                  const uint8_t* i1, uint8_t* scratch) {
+void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
+                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch) {
   Carver c{scratch};
-  hipStream_t st = e->stream;
-  if (e->form == DPKEM_FORM_PLAIN) {
+  if (form == DPKEM_FORM_PLAIN) {
     if (scheme == DPKEM_SABER) plain_saber_launch(st, op, n, o0, o1, i0, i1, scratch);
     else plain_frodo_launch(st, op, n, o0, o1, i0, i1, scratch);
     return;
@@ -152,6 +134,10 @@ void launch_chunk(dpkem_engine* e, int scheme, int op, int n, uint8_t* o0, uint8
 // Upper bound on the arena bytes the chunk carver uses for n instances
 // (per-array alignment padding included).
 size_t chunk_scratch_bytes(int scheme, int form, size_t n) { return n * scratch_per_inst(scheme, form) + 16 * kAlign; }
+
+}  // namespace dpkem
+
+namespace {
 
 int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
            const uint8_t* i1, int mem) {
@@ -196,7 +182,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
       }
     }
     tls_profiler = e->profiling ? &e->prof : nullptr;
-    launch_chunk(e, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
+    launch_chunk_on(e->stream, e->form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
     tls_profiler = nullptr;
     HIP_TRY(e, hipGetLastError());
     if (mem == DPKEM_MEM_HOST) {

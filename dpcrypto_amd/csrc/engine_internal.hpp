@@ -1,0 +1,45 @@
+// Engine internals shared by engine.cpp (batched ops, C ABI), plan.cpp
+// (graph-captured task sets) and server.cpp (serving loop).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/dpkem.h"
+#include "launch.hpp"
+#include "prof.hpp"
+
+struct dpkem_engine {
+  int device = 0;
+  size_t budget = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int form = DPKEM_FORM_BATCHED;
+  uint8_t* arena = nullptr;
+  size_t arena_bytes = 0;
+  char last_err[256] = {0};
+  std::mutex mu;
+  bool profiling = false;
+  dpkem::Profiler prof;
+  std::map<std::string, std::pair<long, double>> kstats;  // name -> (launches, total ms)
+};
+
+
+namespace dpkem {
+
+enum Op { kKeygen = 0, kEnc = 1, kDec = 2 };
+
+// Per-instance scratch of the current form (bytes, 256-aligned).
+size_t scratch_per_inst(int scheme, int form);
+// Upper bound of the scratch one chunk of n instances carves.
+size_t chunk_scratch_bytes(int scheme, int form, size_t n);
+// I/O byte sizes per instance: outputs out[0..1], inputs in[0..1] (0 = unused).
+void io_sizes(int scheme, int op, size_t out[2], size_t in[2]);
+// Launch one chunk (all pointers on the device) on stream st using `scratch`.
+void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
+                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch);
+int record_hip_error(dpkem_engine* e, hipError_t err, const char* what);
+
+}  // namespace dpkem

@@ -1,0 +1,258 @@
+// Graph-captured task sets ("plans").
+//
+// The reference launches 15-31 kernels per op, one op at a time, on the
+// legacy default stream (FrodoKEM976-GPU/src/cuda_kernel.cu:64-84, 159-189,
+// 282-323; SABER-GPU/src/cuda_kernel.cu:68-133, 213-273).  A plan instead
+// records a whole set of KEM tasks on device-resident arrays -- possibly
+// different schemes and ops, with instance-wise dependencies such as
+// keygen -> encaps -> decaps -- splits each task into instance chunks, spreads
+// the chunks over several streams, and captures everything once into a HIP
+// graph.  Replaying the graph issues all launches at once; the serial hash
+// chains of one chunk (few waves, latency-bound) then run beside the
+// throughput-bound kernels of other chunks.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "engine_internal.hpp"
+
+using namespace dpkem;
+
+namespace {
+
+struct Task {
+  int scheme, op;
+  size_t n;
+  uint8_t *o0, *o1;
+  const uint8_t *i0, *i1;
+  int after;  // task index this one depends on, or -1
+};
+
+struct Sub {
+  int task;
+  size_t begin, count;
+  int stream;
+  size_t scratch_off;
+  hipEvent_t done = nullptr;
+  std::vector<int> deps;  // sub indices
+};
+
+}  // namespace
+
+struct dpkem_plan {
+  dpkem_engine* e = nullptr;
+  int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED;
+  std::vector<Task> tasks;
+  std::vector<Sub> subs;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  uint8_t* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void release(dpkem_plan* p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  for (auto& s : p->subs)
+    if (s.done) (void)hipEventDestroy(s.done);
+  for (auto ev : p->join) (void)hipEventDestroy(ev);
+  p->join.clear();
+  if (p->fork) (void)hipEventDestroy(p->fork);
+  p->fork = nullptr;
+  for (auto st : p->streams) (void)hipStreamDestroy(st);
+  p->streams.clear();
+  if (p->scratch) (void)hipFree(p->scratch);
+  p->scratch = nullptr;
+  p->subs.clear();
+}
+
+size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Split every task into chunks and assign streams and dependencies.
+void schedule(dpkem_plan* p) {
+  p->subs.clear();
+  std::vector<std::vector<int>> task_subs(p->tasks.size());
+  std::vector<int> chain_of(p->tasks.size(), -1);
+  int chains = 0;
+  size_t off = 0;
+  for (size_t t = 0; t < p->tasks.size(); ++t) {
+    const Task& T = p->tasks[t];
+    chain_of[t] = T.after >= 0 ? chain_of[T.after] : chains++;
+    const size_t k = std::max<size_t>(1, std::min<size_t>((size_t)p->chunks, T.n));
+    const size_t per = (T.n + k - 1) / k;
+    for (size_t c = 0, b = 0; b < T.n; ++c, b += per) {
+      Sub s;
+      s.task = (int)t;
+      s.begin = b;
+      s.count = std::min(per, T.n - b);
+      // chunk c of a dependency chain stays on one stream; chains rotate
+      s.stream = (int)((c + (size_t)chain_of[t] * k) % (size_t)p->nstreams);
+      s.scratch_off = off;
+      off += round_up(chunk_scratch_bytes(T.scheme, p->form, s.count));
+      if (T.after >= 0) {
+        const Task& A = p->tasks[T.after];
+        const auto& prev = task_subs[T.after];
+        if (A.n == T.n && prev.size() == (size_t)std::max<size_t>(1, std::min<size_t>((size_t)p->chunks, A.n)) &&
+            c < prev.size()) {
+          s.deps.push_back(prev[c]);  // instance-aligned chunk
+        } else {
+          s.deps = prev;  // join on the whole producer
+        }
+      }
+      task_subs[t].push_back((int)p->subs.size());
+      p->subs.push_back(s);
+    }
+  }
+  p->scratch_bytes = off;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpkem_plan_create(dpkem_engine* e, dpkem_plan** out, int nstreams, int chunks) {
+  if (!e || !out || nstreams < 1 || nstreams > 16 || chunks < 1 || chunks > 64) return DPKEM_EINVAL;
+  dpkem_plan* p = new (std::nothrow) dpkem_plan();
+  if (!p) return DPKEM_ENOMEM;
+  p->e = e;
+  p->nstreams = nstreams;
+  p->chunks = chunks;
+  p->form = e->form;
+  *out = p;
+  return DPKEM_OK;
+}
+
+int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+                   const uint8_t* i1, int after, int* task_id) {
+  if (!p || p->exec || (scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER) || op < 0 || op > 2 || n == 0 ||
+      n > (size_t)INT32_MAX)
+    return DPKEM_EINVAL;
+  if (after >= (int)p->tasks.size() || after < -1) return DPKEM_EINVAL;
+  size_t out[2], in[2];
+  io_sizes(scheme, op, out, in);
+  uint8_t* outs[2] = {o0, o1};
+  const uint8_t* ins[2] = {i0, i1};
+  for (int k = 0; k < 2; ++k) {
+    if ((out[k] && (!outs[k] || ((uintptr_t)outs[k] & 7))) || (in[k] && (!ins[k] || ((uintptr_t)ins[k] & 7))))
+      return DPKEM_EINVAL;
+  }
+  p->tasks.push_back({scheme, op, n, o0, o1, i0, i1, after});
+  if (task_id) *task_id = (int)p->tasks.size() - 1;
+  return DPKEM_OK;
+}
+
+int dpkem_plan_build(dpkem_plan* p) {
+  if (!p || p->tasks.empty()) return DPKEM_EINVAL;
+  dpkem_engine* e = p->e;
+  std::lock_guard<std::mutex> lock(e->mu);
+  release(p);
+  schedule(p);
+  if (p->scratch_bytes > e->budget) {
+    snprintf(e->last_err, sizeof e->last_err, "plan scratch %zu B exceeds HBM budget %zu B", p->scratch_bytes, e->budget);
+    return DPKEM_EBUDGET;
+  }
+  hipError_t err;
+#define PLAN_TRY(call)                                              \
+  do {                                                              \
+    err = (call);                                                   \
+    if (err != hipSuccess) {                                        \
+      int rc = record_hip_error(e, err, #call);                     \
+      release(p);                                                   \
+      return rc;                                                    \
+    }                                                               \
+  } while (0)
+  PLAN_TRY(hipSetDevice(e->device));
+  PLAN_TRY(hipMalloc(&p->scratch, std::max<size_t>(p->scratch_bytes, 256)));
+  p->streams.resize(p->nstreams, nullptr);
+  for (auto& st : p->streams) PLAN_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  PLAN_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+  p->join.resize(p->nstreams, nullptr);
+  for (auto& ev : p->join) PLAN_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  for (auto& s : p->subs) PLAN_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+
+  // capture: origin -> fork -> per-stream sub-task chains -> join -> origin
+  hipStream_t origin = p->streams[0];
+  hipStream_t cap;
+  PLAN_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) {
+    (void)hipStreamDestroy(cap);
+    int rc = record_hip_error(e, err, "hipStreamBeginCapture");
+    release(p);
+    return rc;
+  }
+  bool ok = hipEventRecord(p->fork, cap) == hipSuccess;
+  for (auto st : p->streams) ok = ok && hipStreamWaitEvent(st, p->fork, 0) == hipSuccess;
+  Profiler* saved = tls_profiler;
+  tls_profiler = nullptr;  // no timing events inside the graph
+  for (auto& s : p->subs) {
+    const Task& T = p->tasks[s.task];
+    hipStream_t st = p->streams[s.stream];
+    for (int d : s.deps)
+      if (p->subs[d].stream != s.stream) ok = ok && hipStreamWaitEvent(st, p->subs[d].done, 0) == hipSuccess;
+    size_t out[2], in[2];
+    io_sizes(T.scheme, T.op, out, in);
+    uint8_t* o0 = out[0] ? T.o0 + s.begin * out[0] : nullptr;
+    uint8_t* o1 = out[1] ? T.o1 + s.begin * out[1] : nullptr;
+    const uint8_t* i0 = in[0] ? T.i0 + s.begin * in[0] : nullptr;
+    const uint8_t* i1 = in[1] ? T.i1 + s.begin * in[1] : nullptr;
+    launch_chunk_on(st, p->form, T.scheme, T.op, (int)s.count, o0, o1, i0, i1, p->scratch + s.scratch_off);
+    ok = ok && hipGetLastError() == hipSuccess;
+    ok = ok && hipEventRecord(s.done, st) == hipSuccess;
+  }
+  tls_profiler = saved;
+  for (int k = 0; k < p->nstreams; ++k) {
+    ok = ok && hipEventRecord(p->join[k], p->streams[k]) == hipSuccess;
+    ok = ok && hipStreamWaitEvent(cap, p->join[k], 0) == hipSuccess;
+  }
+  err = hipStreamEndCapture(cap, &p->graph);
+  (void)hipStreamDestroy(cap);
+  (void)origin;
+  if (err != hipSuccess || !ok) {
+    int rc = record_hip_error(e, err != hipSuccess ? err : hipErrorUnknown, "graph capture");
+    release(p);
+    return rc;
+  }
+  PLAN_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+#undef PLAN_TRY
+  return DPKEM_OK;
+}
+
+int dpkem_plan_launch(dpkem_plan* p) {
+  if (!p || !p->exec) return DPKEM_EINVAL;
+  dpkem_engine* e = p->e;
+  std::lock_guard<std::mutex> lock(e->mu);
+  hipError_t err = hipSetDevice(e->device);
+  if (err == hipSuccess) err = hipGraphLaunch(p->exec, e->stream);
+  return err == hipSuccess ? DPKEM_OK : record_hip_error(e, err, "hipGraphLaunch");
+}
+
+int dpkem_plan_info(dpkem_plan* p, size_t* subtasks, size_t* scratch_bytes) {
+  if (!p) return DPKEM_EINVAL;
+  if (subtasks) *subtasks = p->subs.size();
+  if (scratch_bytes) *scratch_bytes = p->scratch_bytes;
+  return DPKEM_OK;
+}
+
+int dpkem_plan_destroy(dpkem_plan* p) {
+  if (!p) return DPKEM_EINVAL;
+  {
+    std::lock_guard<std::mutex> lock(p->e->mu);
+    (void)hipSetDevice(p->e->device);
+    (void)hipStreamSynchronize(p->e->stream);
+    release(p);
+  }
+  delete p;
+  return DPKEM_OK;
+}
+
+}  // extern "C"

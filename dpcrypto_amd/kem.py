@@ -69,6 +69,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_enc.argtypes = [vp, i, sz, vp, vp, vp, vp, i]
         lib.dpkem_dec.argtypes = [vp, i, sz, vp, vp, vp, i]
         lib.dpkem_engine_set_profiling.argtypes = [vp, i]
+        lib.dpkem_plan_create.argtypes = [vp, ctypes.POINTER(vp), i, i]
+        lib.dpkem_plan_add.argtypes = [vp, i, i, sz, vp, vp, vp, vp, i, ctypes.POINTER(i)]
+        lib.dpkem_plan_build.argtypes = [vp]
+        lib.dpkem_plan_launch.argtypes = [vp]
+        lib.dpkem_plan_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        lib.dpkem_plan_destroy.argtypes = [vp]
         lib.dpkem_engine_kernel_stats.argtypes = [vp, ctypes.c_char_p, sz, i]
         _lib = lib
     return _lib
@@ -205,6 +211,9 @@ class Engine:
         self._check(self.lib.dpkem_dec(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), mem), "dec")
         return ss
 
+    def plan(self, nstreams: int = 1, chunks: int = 1) -> "Plan":
+        return Plan(self, nstreams, chunks)
+
     # raw pointer entry points (device arrays preallocated by the caller)
     def keypair_into(self, scheme, n, pk, sk, coins):
         self._check(self.lib.dpkem_keypair(self.h, scheme, n, _ptr(pk), _ptr(sk), _ptr(coins), MEM_DEVICE), "keypair")
@@ -214,3 +223,59 @@ class Engine:
 
     def dec_into(self, scheme, n, ss, ct, sk):
         self._check(self.lib.dpkem_dec(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), MEM_DEVICE), "dec")
+
+
+OP_KEYGEN, OP_ENC, OP_DEC = 0, 1, 2
+
+
+class Plan:
+    """A graph-captured set of KEM tasks over device arrays (dpkem_plan_*)."""
+
+    def __init__(self, engine: Engine, nstreams: int = 1, chunks: int = 1):
+        self.eng = engine
+        self.lib = engine.lib
+        h = ctypes.c_void_p()
+        engine._check(self.lib.dpkem_plan_create(engine.h, ctypes.byref(h), nstreams, chunks), "plan_create")
+        self.h = h
+        self._keep = []  # arrays referenced by the graph stay alive with the plan
+
+    def add(self, scheme, op, n, o0=None, o1=None, i0=None, i1=None, after=-1) -> int:
+        tid = ctypes.c_int(-1)
+        self._keep += [a for a in (o0, o1, i0, i1) if a is not None]
+        rc = self.lib.dpkem_plan_add(self.h, scheme, op, n, _ptr(o0), _ptr(o1), _ptr(i0), _ptr(i1), after,
+                                     ctypes.byref(tid))
+        self.eng._check(rc, "plan_add")
+        return tid.value
+
+    def keypair(self, scheme, pk, sk, coins, after=-1):
+        return self.add(scheme, OP_KEYGEN, pk.shape[0], pk, sk, coins, None, after)
+
+    def enc(self, scheme, ct, ss, pk, coins, after=-1):
+        return self.add(scheme, OP_ENC, ct.shape[0], ct, ss, pk, coins, after)
+
+    def dec(self, scheme, ss, ct, sk, after=-1):
+        return self.add(scheme, OP_DEC, ss.shape[0], ss, None, ct, sk, after)
+
+    def build(self):
+        self.eng._check(self.lib.dpkem_plan_build(self.h), "plan_build")
+        return self
+
+    def launch(self):
+        self.eng._check(self.lib.dpkem_plan_launch(self.h), "plan_launch")
+
+    def info(self):
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        self.lib.dpkem_plan_info(self.h, ctypes.byref(a), ctypes.byref(b))
+        return {"subtasks": a.value, "scratch_bytes": b.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dpkem_plan_destroy(self.h)
+            self.h = None
+            self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

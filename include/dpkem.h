@@ -95,6 +95,29 @@ int dpkem_enc(dpkem_engine *e, int scheme, size_t n, uint8_t *ct, uint8_t *ss, c
 int dpkem_dec(dpkem_engine *e, int scheme, size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk,
               int mem);
 
+/* ---------------------------------------------------------------- plans (graph-captured task sets)
+ * A plan records KEM tasks over device-resident arrays, optionally chained
+ * instance-wise (task `after` produces what this task consumes, e.g.
+ * keygen -> enc -> dec), splits each task into `chunks` instance ranges,
+ * spreads them over `nstreams` internal streams and captures everything
+ * into one HIP graph; dpkem_plan_launch replays it on the engine's stream.
+ * Output / input roles per op are those of dpkem_keypair / enc / dec:
+ *   KEYGEN: o0 = pk, o1 = sk, i0 = coins        ENC: o0 = ct, o1 = ss, i0 = pk, i1 = coins
+ *   DEC:    o0 = ss, i0 = ct, i1 = sk
+ * Replaces the reference's strictly sequential per-op launch sequences
+ * (FrodoKEM976-GPU/src/cuda_kernel.cu:64-84,159-189,282-323). */
+#define DPKEM_OP_KEYGEN 0
+#define DPKEM_OP_ENC 1
+#define DPKEM_OP_DEC 2
+typedef struct dpkem_plan dpkem_plan;
+int dpkem_plan_create(dpkem_engine *e, dpkem_plan **out, int nstreams, int chunks);
+int dpkem_plan_add(dpkem_plan *p, int scheme, int op, size_t n, uint8_t *o0, uint8_t *o1, const uint8_t *i0,
+                   const uint8_t *i1, int after, int *task_id);
+int dpkem_plan_build(dpkem_plan *p);
+int dpkem_plan_launch(dpkem_plan *p);
+int dpkem_plan_info(dpkem_plan *p, size_t *subtasks, size_t *scratch_bytes);
+int dpkem_plan_destroy(dpkem_plan *p);
+
 /* ---------------------------------------------------------------- randomness
  * PQClean-shaped wrappers draw coins from a randombytes callback.  The
  * default reads the OS entropy source (getrandom). */

@@ -1,0 +1,117 @@
+"""Graph-captured plans (dpkem_plan_*): results equal the oracle on every replay.
+
+A plan captures chained keygen -> encaps -> decaps tasks (split into instance
+chunks over several streams) or a mix of independent Frodo / Saber tasks
+into one HIP graph.  Each test replays the graph several times, clearing the
+outputs and running unrelated eager work between replays, and checks every
+replay byte for byte against the oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FRODO, SABER = 0, 1
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def eng(torch_cuda):
+    import dpcrypto_amd as D
+    e = D.Engine(0)
+    yield e
+    e.close()
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("scheme,n,nstreams,chunks", [(SABER, 300, 4, 4), (SABER, 97, 2, 3), (FRODO, 10, 3, 3)])
+def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, scheme, n, nstreams, chunks):
+    import dpcrypto_amd as D
+    torch = torch_cuda
+    s = D.SIZES[scheme]
+    kc = oracle.prng_bytes(1000 + n, n * s["kcoins"]).reshape(n, -1)
+    ec = oracle.prng_bytes(2000 + n, n * s["ecoins"]).reshape(n, -1)
+    opk, osk = oracle.keypair(scheme, kc, threads=4)
+    oct_, oss = oracle.enc(scheme, opk, ec, threads=4)
+    dkc, dec_ = _dev(torch, kc), _dev(torch, ec)
+    pk = torch.empty((n, s["pk"]), dtype=torch.uint8, device="cuda")
+    sk = torch.empty((n, s["sk"]), dtype=torch.uint8, device="cuda")
+    ct = torch.empty((n, s["ct"]), dtype=torch.uint8, device="cuda")
+    ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
+    ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    plan = eng.plan(nstreams=nstreams, chunks=chunks)
+    t0 = plan.keypair(scheme, pk, sk, dkc)
+    t1 = plan.enc(scheme, ct, ss, pk, dec_, after=t0)
+    plan.dec(scheme, ss2, ct, sk, after=t1)
+    plan.build()
+    assert plan.info()["subtasks"] == 3 * min(chunks, n)
+    for rep in range(3):
+        for t in (pk, sk, ct, ss, ss2):
+            t.zero_()
+        scratch = torch.randn(1 << 20, device="cuda").sum()  # unrelated eager work
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        assert float(scratch) == float(scratch)
+        assert np.array_equal(pk.cpu().numpy(), opk), f"pk replay {rep}"
+        assert np.array_equal(sk.cpu().numpy(), osk), f"sk replay {rep}"
+        assert np.array_equal(ct.cpu().numpy(), oct_), f"ct replay {rep}"
+        assert np.array_equal(ss.cpu().numpy(), oss), f"ss replay {rep}"
+        assert np.array_equal(ss2.cpu().numpy(), oss), f"dec ss replay {rep}"
+    plan.close()
+
+
+def test_mixed_independent_plan(eng, oracle, torch_cuda):
+    """Independent tasks of both schemes and all three ops in one graph."""
+    import dpcrypto_amd as D
+    torch = torch_cuda
+    # key material produced eagerly, then consumed by the mixed plan
+    n_s, n_f = 130, 4
+    skc = oracle.prng_bytes(31, n_s * 96).reshape(n_s, 96)
+    fkc = oracle.prng_bytes(32, n_f * 64).reshape(n_f, 64)
+    spk, ssk = oracle.keypair(SABER, skc)
+    fpk, fsk = oracle.keypair(FRODO, fkc, threads=4)
+    sm = oracle.prng_bytes(33, n_s * 32).reshape(n_s, 32)
+    fm = oracle.prng_bytes(34, n_f * 24).reshape(n_f, 24)
+    sct, sss = oracle.enc(SABER, spk, sm)
+    fct, fss = oracle.enc(FRODO, fpk, fm, threads=4)
+    d = {k: _dev(torch, v) for k, v in dict(skc=skc, fkc=fkc, spk=spk, fpk=fpk, ssk=ssk, fsk=fsk, sm=sm, fm=fm,
+                                             sct=sct, fct=fct).items()}
+    out = {
+        "spk": torch.empty_like(d["spk"]), "ssk": torch.empty_like(d["ssk"]),
+        "fpk": torch.empty_like(d["fpk"]), "fsk": torch.empty_like(d["fsk"]),
+        "sct": torch.empty_like(d["sct"]), "sss": torch.empty((n_s, 32), dtype=torch.uint8, device="cuda"),
+        "fct": torch.empty_like(d["fct"]), "fss": torch.empty((n_f, 24), dtype=torch.uint8, device="cuda"),
+        "sdss": torch.empty((n_s, 32), dtype=torch.uint8, device="cuda"),
+        "fdss": torch.empty((n_f, 24), dtype=torch.uint8, device="cuda"),
+    }
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    plan = eng.plan(nstreams=4, chunks=2)
+    plan.keypair(SABER, out["spk"], out["ssk"], d["skc"])
+    plan.keypair(FRODO, out["fpk"], out["fsk"], d["fkc"])
+    plan.enc(SABER, out["sct"], out["sss"], d["spk"], d["sm"])
+    plan.enc(FRODO, out["fct"], out["fss"], d["fpk"], d["fm"])
+    plan.dec(SABER, out["sdss"], d["sct"], d["ssk"])
+    plan.dec(FRODO, out["fdss"], d["fct"], d["fsk"])
+    plan.build()
+    for rep in range(2):
+        for t in out.values():
+            t.zero_()
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        exp = dict(spk=spk, ssk=ssk, fpk=fpk, fsk=fsk, sct=sct, sss=sss, fct=fct, fss=fss, sdss=sss, fdss=fss)
+        for k, v in exp.items():
+            assert np.array_equal(out[k].cpu().numpy(), v), f"{k} replay {rep}"
+    plan.close()
