@@ -75,6 +75,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_plan_launch.argtypes = [vp]
         lib.dpkem_plan_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         lib.dpkem_plan_destroy.argtypes = [vp]
+        lib.dpkem_now_us.restype = ctypes.c_double
+        lib.dpkem_server_create.argtypes = [ctypes.POINTER(vp), vp]
+        lib.dpkem_server_submit.argtypes = [vp, vp, sz]
+        lib.dpkem_server_poll.argtypes = [vp, vp, sz, ctypes.POINTER(sz)]
+        lib.dpkem_server_drain.argtypes = [vp]
+        lib.dpkem_server_release.argtypes = [vp, i, i, ctypes.c_uint32]
+        lib.dpkem_server_read.argtypes = [vp, i, i, ctypes.c_uint32, vp]
+        lib.dpkem_server_stats.argtypes = [vp, ctypes.c_char_p, sz]
+        lib.dpkem_server_destroy.argtypes = [vp]
         lib.dpkem_engine_kernel_stats.argtypes = [vp, ctypes.c_char_p, sz, i]
         _lib = lib
     return _lib
@@ -279,3 +288,86 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- serving loop
+REQ_DTYPE = np.dtype([("id", "<u8"), ("scheme", "<i4"), ("op", "<i4"), ("key", "<u4"), ("ct", "<u4"),
+                      ("t_arrival_us", "<f8"), ("coins", "u1", (96,))])
+COMP_DTYPE = np.dtype([("id", "<u8"), ("status", "<i4"), ("scheme", "<i4"), ("op", "<i4"), ("key", "<u4"),
+                       ("ct", "<u4"), ("reserved", "<u4"), ("t_submit_us", "<f8"), ("t_done_us", "<f8"),
+                       ("ss", "u1", (32,))])
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("form", ctypes.c_int), ("lanes", ctypes.c_int),
+                ("hbm_budget_bytes", ctypes.c_size_t), ("key_slots", ctypes.c_size_t * 2),
+                ("ct_slots", ctypes.c_size_t * 2), ("max_batch", ctypes.c_size_t * 2)]
+
+
+def now_us() -> float:
+    """The serving loop's clock (steady clock, microseconds)."""
+    return load_library().dpkem_now_us()
+
+
+class Server:
+    """Continuous-batching request loop over a device-resident key store (dpkem_server_*)."""
+
+    def __init__(self, device=0, lanes=4, hbm_budget=0, key_slots=(1024, 4096), ct_slots=(4096, 16384),
+                 max_batch=(0, 0), form=FORM_BATCHED):
+        self.lib = load_library()
+        cfg = _ServerConfig(device, form, lanes, hbm_budget, (ctypes.c_size_t * 2)(*key_slots),
+                            (ctypes.c_size_t * 2)(*ct_slots), (ctypes.c_size_t * 2)(*max_batch))
+        h = ctypes.c_void_p()
+        rc = self.lib.dpkem_server_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc:
+            raise DpkemError(rc, "server_create")
+        self.h = h
+
+    def submit(self, reqs: np.ndarray) -> int:
+        reqs = np.ascontiguousarray(reqs, dtype=REQ_DTYPE)
+        return self.lib.dpkem_server_submit(self.h, reqs.ctypes.data_as(ctypes.c_void_p), len(reqs))
+
+    def poll(self, cap: int = 65536) -> np.ndarray:
+        out = np.empty(cap, COMP_DTYPE)
+        got = ctypes.c_size_t()
+        rc = self.lib.dpkem_server_poll(self.h, out.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(got))
+        if rc:
+            raise DpkemError(rc, "server_poll")
+        return out[: got.value]
+
+    def drain(self):
+        rc = self.lib.dpkem_server_drain(self.h)
+        if rc:
+            raise DpkemError(rc, "server_drain")
+
+    def release(self, scheme, kind, handle):
+        return self.lib.dpkem_server_release(self.h, scheme, kind, handle)
+
+    def read(self, scheme, kind, handle) -> np.ndarray:
+        size = SIZES[scheme][("pk", "sk", "ct")[kind]]
+        out = np.empty(size, np.uint8)
+        rc = self.lib.dpkem_server_read(self.h, scheme, kind, handle, out.ctypes.data_as(ctypes.c_void_p))
+        if rc:
+            raise DpkemError(rc, "server_read")
+        return out
+
+    def stats(self) -> dict:
+        import json
+        buf = ctypes.create_string_buffer(1024)
+        self.lib.dpkem_server_stats(self.h, buf, len(buf))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dpkem_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_requests(n: int) -> np.ndarray:
+    return np.zeros(n, REQ_DTYPE)

@@ -118,6 +118,58 @@ int dpkem_plan_launch(dpkem_plan *p);
 int dpkem_plan_info(dpkem_plan *p, size_t *subtasks, size_t *scratch_bytes);
 int dpkem_plan_destroy(dpkem_plan *p);
 
+/* ---------------------------------------------------------------- serving loop
+ * Continuous batching over a device-resident key / ciphertext store.
+ * Requests go in one at a time (or in arrays); a dispatcher thread groups
+ * pending requests by (scheme, op) and admits the group with the oldest
+ * request into any free in-flight lane, as a batch capped by that lane's
+ * share of the HBM budget.  Lanes run on their own streams, so batches
+ * admitted while others run overlap on the device.  Keygen returns a key
+ * handle, encaps a ciphertext handle; decaps names both.  Only coins enter
+ * and shared secrets leave host memory.  No counterpart in the reference,
+ * whose drivers take one fixed batch per call
+ * (FrodoKEM976-GPU/src/cuda_kernel.cu:21-335). */
+typedef struct dpkem_server dpkem_server;
+typedef struct {
+  uint64_t id;          /* caller's id, echoed in the completion */
+  int32_t scheme, op;   /* DPKEM_FRODO976 / DPKEM_SABER, DPKEM_OP_* */
+  uint32_t key;         /* ENC, DEC: key handle from a KEYGEN completion */
+  uint32_t ct;          /* DEC: ciphertext handle from an ENC completion */
+  double t_arrival_us;  /* optional arrival time on dpkem_now_us()'s clock (0 = submit time) */
+  uint8_t coins[96];    /* KEYGEN: keygen coins; ENC: encaps coins */
+} dpkem_request;
+typedef struct {
+  uint64_t id;
+  int32_t status;       /* DPKEM_OK or an error (ENOTFOUND: stale handle, EBUDGET: store full) */
+  int32_t scheme, op;
+  uint32_t key;         /* KEYGEN: new key handle */
+  uint32_t ct;          /* ENC: new ciphertext handle */
+  uint32_t reserved;
+  double t_submit_us;   /* arrival (or submit) time */
+  double t_done_us;     /* completion observed by the dispatcher */
+  uint8_t ss[32];       /* ENC, DEC: shared secret (24 bytes for Frodo) */
+} dpkem_completion;
+typedef struct {
+  int device;
+  int form;                 /* DPKEM_FORM_BATCHED, or DPKEM_FORM_PLAIN for the comparator */
+  int lanes;                /* in-flight batches (1..16) */
+  size_t hbm_budget_bytes;  /* store + lanes (0 = 25% of free device memory) */
+  size_t key_slots[2];      /* per scheme (index = DPKEM_FRODO976 / DPKEM_SABER) */
+  size_t ct_slots[2];
+  size_t max_batch[2];      /* per-batch cap (0 = as large as the budget allows); 1 = one request per launch */
+} dpkem_server_config;
+double dpkem_now_us(void);
+int dpkem_server_create(dpkem_server **out, const dpkem_server_config *cfg);
+int dpkem_server_submit(dpkem_server *s, const dpkem_request *reqs, size_t n);
+int dpkem_server_poll(dpkem_server *s, dpkem_completion *out, size_t cap, size_t *got);
+int dpkem_server_drain(dpkem_server *s);
+/* kind: 0 = key handle, 1 = ciphertext handle */
+int dpkem_server_release(dpkem_server *s, int scheme, int kind, uint32_t handle);
+/* kind: 0 = pk, 1 = sk, 2 = ct; copies the stored bytes to host memory */
+int dpkem_server_read(dpkem_server *s, int scheme, int kind, uint32_t handle, uint8_t *out);
+int dpkem_server_stats(dpkem_server *s, char *buf, size_t len);
+int dpkem_server_destroy(dpkem_server *s);
+
 /* ---------------------------------------------------------------- randomness
  * PQClean-shaped wrappers draw coins from a randombytes callback.  The
  * default reads the OS entropy source (getrandom). */

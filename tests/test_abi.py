@@ -32,3 +32,40 @@ def test_sizes_query(scheme):
     s = D.SIZES[scheme]
     assert [v.value for v in vals] == [s["pk"], s["sk"], s["ct"], s["ss"], s["kcoins"], s["ecoins"]]
     assert lib.dpkem_sizes(7, *[None] * 6) == -1
+
+
+def test_server_struct_layouts_match_numpy(tmp_path):
+    """dpkem_request / dpkem_completion / dpkem_server_config as compiled by the
+    host C compiler have the layouts the Python binding uses."""
+    import os
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "dpkem.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(dpkem_request), offsetof(dpkem_request, scheme),
+         offsetof(dpkem_request, key), offsetof(dpkem_request, t_arrival_us), offsetof(dpkem_request, coins),
+         sizeof(dpkem_completion), offsetof(dpkem_completion, ss));
+  printf("%zu %zu %zu\n", offsetof(dpkem_completion, t_submit_us), offsetof(dpkem_completion, key),
+         sizeof(dpkem_server_config));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout"
+    subprocess.run([cc, str(src), "-I", os.path.join(root, "include"), "-o", str(exe)], check=True)
+    a, b = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")[:2]
+    a = list(map(int, a.split())); b = list(map(int, b.split()))
+    R, C = D.REQ_DTYPE, D.COMP_DTYPE
+    assert a == [R.itemsize, R.fields["scheme"][1], R.fields["key"][1], R.fields["t_arrival_us"][1],
+                 R.fields["coins"][1], C.itemsize, C.fields["ss"][1]]
+    assert b[:2] == [C.fields["t_submit_us"][1], C.fields["key"][1]]
+    import ctypes
+    from dpcrypto_amd.kem import _ServerConfig
+    assert b[2] == ctypes.sizeof(_ServerConfig)

@@ -1,0 +1,117 @@
+"""Serving loop (dpkem_server_*): continuous batching over a device-resident store.
+
+Each result must be the same when its request runs alone and when it runs
+inside a mixed stream of other requests (both schemes, all ops, several
+in-flight lanes), and equal to the oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FRODO, SABER = 0, 1
+KEYGEN, ENC, DEC = 0, 1, 2
+
+
+@pytest.fixture(scope="module")
+def server():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import dpcrypto_amd as D
+    s = D.Server(0, lanes=4, hbm_budget=4 << 30, key_slots=(64, 512), ct_slots=(256, 2048))
+    yield s
+    s.close()
+
+
+def _run(server, reqs, timeout_s=60.0):
+    import time
+    import dpcrypto_amd as D
+    assert server.submit(reqs) == 0
+    server.drain()
+    out = server.poll(len(reqs) + 16)
+    assert len(out) == len(reqs)
+    return {int(c["id"]): c for c in out}
+
+
+def _req(rid, scheme, op, coins=None, key=0, ct=0):
+    import dpcrypto_amd as D
+    r = D.make_requests(1)[0]
+    r["id"], r["scheme"], r["op"], r["key"], r["ct"] = rid, scheme, op, key, ct
+    if coins is not None:
+        r["coins"][: len(coins)] = coins
+    return r
+
+
+def test_alone_and_in_mix_are_identical(server, oracle):
+    import dpcrypto_amd as D
+    kc = {SABER: oracle.prng_bytes(5001, 96), FRODO: oracle.prng_bytes(5002, 64)}
+    ec = {SABER: oracle.prng_bytes(5003, 32), FRODO: oracle.prng_bytes(5004, 24)}
+    alone = {}
+    for sc in (SABER, FRODO):
+        # keygen alone
+        c = _run(server, np.array([_req(1, sc, KEYGEN, kc[sc])], D.REQ_DTYPE))[1]
+        assert c["status"] == 0
+        key = int(c["key"])
+        pk, sk = server.read(sc, 0, key), server.read(sc, 1, key)
+        opk, osk = oracle.keypair(sc, kc[sc][None], threads=2)
+        assert np.array_equal(pk, opk[0]) and np.array_equal(sk, osk[0])
+        # encaps alone
+        c = _run(server, np.array([_req(2, sc, ENC, ec[sc], key=key)], D.REQ_DTYPE))[2]
+        assert c["status"] == 0
+        ctid = int(c["ct"])
+        oct_, oss = oracle.enc(sc, opk, ec[sc][None], threads=2)
+        assert np.array_equal(server.read(sc, 2, ctid), oct_[0])
+        nss = D.SIZES[sc]["ss"]
+        assert np.array_equal(c["ss"][:nss], oss[0])
+        # decaps alone
+        c = _run(server, np.array([_req(3, sc, DEC, key=key, ct=ctid)], D.REQ_DTYPE))[3]
+        assert np.array_equal(c["ss"][:nss], oss[0])
+        alone[sc] = (key, ctid, pk, sk, oct_[0], oss[0])
+
+    # the same three requests per scheme inside a mixed stream
+    rng = np.random.default_rng(9)
+    reqs = []
+    rid = 100
+    for _ in range(600):
+        sc = SABER if rng.random() < 0.7 else FRODO
+        op = rng.choice([KEYGEN, ENC, DEC], p=[1 / 9, 4 / 9, 4 / 9])
+        key, ctid = alone[sc][0], alone[sc][1]
+        w = 96 if op == KEYGEN else 32
+        reqs.append(_req(rid, sc, op, rng.integers(0, 256, w, dtype=np.uint8), key=key, ct=ctid))
+        rid += 1
+    probes = {}
+    for sc in (SABER, FRODO):
+        key, ctid = alone[sc][0], alone[sc][1]
+        for op, coins in ((KEYGEN, kc[sc]), (ENC, ec[sc]), (DEC, None)):
+            probes[rid] = (sc, op)
+            reqs.insert(int(rng.integers(0, len(reqs))), _req(rid, sc, op, coins, key=key, ct=ctid))
+            rid += 1
+    done = _run(server, np.array(reqs, D.REQ_DTYPE))
+    assert all(c["status"] == 0 for c in done.values())
+    for pid, (sc, op) in probes.items():
+        c = done[pid]
+        key, ctid, pk, sk, oct0, oss0 = alone[sc]
+        nss = D.SIZES[sc]["ss"]
+        if op == KEYGEN:
+            assert np.array_equal(server.read(sc, 0, int(c["key"])), pk)
+            assert np.array_equal(server.read(sc, 1, int(c["key"])), sk)
+        elif op == ENC:
+            assert np.array_equal(server.read(sc, 2, int(c["ct"])), oct0)
+            assert np.array_equal(c["ss"][:nss], oss0)
+        else:
+            assert np.array_equal(c["ss"][:nss], oss0)
+    st = server.stats()
+    assert st["batches"] >= 6 and st["max_batch_seen"] > 1  # requests were batched together
+    # release everything created in the mix
+    for c in done.values():
+        if c["op"] == KEYGEN:
+            server.release(int(c["scheme"]), 0, int(c["key"]))
+        elif c["op"] == ENC:
+            server.release(int(c["scheme"]), 1, int(c["ct"]))
+
+
+def test_stale_handles_are_rejected(server):
+    import dpcrypto_amd as D
+    c = _run(server, np.array([_req(7, SABER, DEC, key=63, ct=255)], D.REQ_DTYPE))[7]
+    assert c["status"] == -6  # ENOTFOUND: handle not allocated
