@@ -32,11 +32,14 @@ struct Task {
 
 struct Sub {
   int task;
+  int chain = -1;          // >= 0: Saber keygen/enc/dec chain (tasks task, chain, chain2) as one stage DAG
+  int chain2 = -1;
   size_t begin, count;
   int stream;
   size_t scratch_off;
   hipEvent_t done = nullptr;
-  std::vector<int> deps;  // sub indices
+  hipEvent_t ev[8] = {};   // chain DAG events
+  std::vector<int> deps;   // sub indices
 };
 
 }  // namespace
@@ -62,8 +65,11 @@ void release(dpkem_plan* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->exec = nullptr;
   p->graph = nullptr;
-  for (auto& s : p->subs)
+  for (auto& s : p->subs) {
     if (s.done) (void)hipEventDestroy(s.done);
+    for (auto ev : s.ev)
+      if (ev) (void)hipEventDestroy(ev);
+  }
   for (auto ev : p->join) (void)hipEventDestroy(ev);
   p->join.clear();
   if (p->fork) (void)hipEventDestroy(p->fork);
@@ -77,15 +83,63 @@ void release(dpkem_plan* p) {
 
 size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Saber keygen t0 -> enc t1 -> dec t2 over the same arrays (t1 consumes t0's
+// pk, t2 consumes t1's ct and t0's sk): run as one stage DAG per chunk.
+bool find_saber_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
+  const Task& K = p->tasks[t0];
+  if (K.scheme != DPKEM_SABER || K.op != kKeygen || K.after != -1) return false;
+  for (size_t a = t0 + 1; a < p->tasks.size(); ++a) {
+    const Task& E = p->tasks[a];
+    if (E.after != t0 || E.scheme != DPKEM_SABER || E.op != kEnc || E.n != K.n || E.i0 != K.o0) continue;
+    for (size_t b = a + 1; b < p->tasks.size(); ++b) {
+      const Task& Dd = p->tasks[b];
+      if (Dd.after == (int)a && Dd.scheme == DPKEM_SABER && Dd.op == kDec && Dd.n == K.n && Dd.i0 == E.o0 &&
+          Dd.i1 == K.o1) {
+        *t1 = (int)a;
+        *t2 = (int)b;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
 // Split every task into chunks and assign streams and dependencies.
 void schedule(dpkem_plan* p) {
   p->subs.clear();
   std::vector<std::vector<int>> task_subs(p->tasks.size());
   std::vector<int> chain_of(p->tasks.size(), -1);
+  std::vector<int> absorbed(p->tasks.size(), -1);  // task -> chain root it was folded into
   int chains = 0;
   size_t off = 0;
   for (size_t t = 0; t < p->tasks.size(); ++t) {
+    if (absorbed[t] >= 0) {
+      task_subs[t] = task_subs[absorbed[t]];
+      chain_of[t] = chain_of[absorbed[t]];
+      continue;
+    }
     const Task& T = p->tasks[t];
+    int t1 = -1, t2 = -1;
+    if (find_saber_chain(p, (int)t, &t1, &t2)) {
+      absorbed[t1] = absorbed[t2] = (int)t;
+      chain_of[t] = chains++;
+      const size_t k = std::max<size_t>(1, std::min<size_t>((size_t)p->chunks, T.n));
+      const size_t per = (T.n + k - 1) / k;
+      for (size_t c = 0, b = 0; b < T.n; ++c, b += per) {
+        Sub s;
+        s.task = (int)t;
+        s.chain = t1;
+        s.chain2 = t2;
+        s.begin = b;
+        s.count = std::min(per, T.n - b);
+        s.stream = (int)((4 * c + (size_t)chain_of[t] * 4 * k) % (size_t)p->nstreams);
+        s.scratch_off = off;
+        off += round_up(s.count * kSaberChainScratchPerInst + 16 * 256);
+        task_subs[t].push_back((int)p->subs.size());
+        p->subs.push_back(s);
+      }
+      continue;
+    }
     chain_of[t] = T.after >= 0 ? chain_of[T.after] : chains++;
     const size_t k = std::max<size_t>(1, std::min<size_t>((size_t)p->chunks, T.n));
     const size_t per = (T.n + k - 1) / k;
@@ -172,12 +226,19 @@ int dpkem_plan_build(dpkem_plan* p) {
   } while (0)
   PLAN_TRY(hipSetDevice(e->device));
   PLAN_TRY(hipMalloc(&p->scratch, std::max<size_t>(p->scratch_bytes, 256)));
+  bool has_chain = false;
+  for (auto& s : p->subs) has_chain |= s.chain >= 0;
+  if (has_chain && p->nstreams < 4) p->nstreams = 4;  // a chain DAG spans four streams
   p->streams.resize(p->nstreams, nullptr);
   for (auto& st : p->streams) PLAN_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   PLAN_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
   p->join.resize(p->nstreams, nullptr);
   for (auto& ev : p->join) PLAN_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  for (auto& s : p->subs) PLAN_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+  for (auto& s : p->subs) {
+    PLAN_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    if (s.chain >= 0)
+      for (auto& ev : s.ev) PLAN_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
 
   // capture: origin -> fork -> per-stream sub-task chains -> join -> origin
   hipStream_t origin = p->streams[0];
@@ -199,6 +260,18 @@ int dpkem_plan_build(dpkem_plan* p) {
     hipStream_t st = p->streams[s.stream];
     for (int d : s.deps)
       if (p->subs[d].stream != s.stream) ok = ok && hipStreamWaitEvent(st, p->subs[d].done, 0) == hipSuccess;
+    if (s.chain >= 0) {
+      const Task& E = p->tasks[s.chain];
+      const Task& Dd = p->tasks[s.chain2];
+      hipStream_t cs[4];
+      for (int j = 0; j < 4; ++j) cs[j] = p->streams[(s.stream + j) % p->nstreams];
+      const size_t b = s.begin;
+      saber_chain_launch(cs, s.ev, (int)s.count, T.i0 + b * 96, E.i1 + b * 32, T.o0 + b * 992, T.o1 + b * 2304,
+                         E.o0 + b * 1088, E.o1 + b * 32, Dd.o0 + b * 32, p->scratch + s.scratch_off);
+      ok = ok && hipGetLastError() == hipSuccess;
+      ok = ok && hipEventRecord(s.done, st) == hipSuccess;
+      continue;
+    }
     size_t out[2], in[2];
     io_sizes(T.scheme, T.op, out, in);
     uint8_t* o0 = out[0] ? T.o0 + s.begin * out[0] : nullptr;

@@ -48,10 +48,10 @@ using k2::store2_words;
   const int inst = blockIdx.x * 32 + (lane >> 1);        \
   if (inst >= n) return;
 
-__global__ void __launch_bounds__(128) sab_keygen_hash1(int n, const uint8_t* __restrict__ coins,
+__global__ void __launch_bounds__(128) sab_keygen_hash1(int role_sel, int n, const uint8_t* __restrict__ coins,
                                                         uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                         uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
-  const int role = threadIdx.x >> 6;
+  const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
   SAB_PAIR_SETUP(32)
   const uint8_t* c = coins + (size_t)inst * kSaberKeygenCoins;
   uint32_t a[25];
@@ -82,11 +82,11 @@ __global__ void __launch_bounds__(128) sab_keygen_hash1(int n, const uint8_t* __
   }
 }
 
-__global__ void __launch_bounds__(128) sab_enc_hash1(int n, const uint8_t* __restrict__ pk,
+__global__ void __launch_bounds__(128) sab_enc_hash1(int role_sel, int n, const uint8_t* __restrict__ pk,
                                                      const uint8_t* __restrict__ coins, uint8_t* __restrict__ araw,
                                                      uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
                                                      uint8_t* __restrict__ krbuf) {
-  const int role = threadIdx.x >> 6;
+  const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
   SAB_PAIR_SETUP(32)
   const uint8_t* pkp = pk + (size_t)inst * kSaberPk;
   uint32_t a[25];
@@ -120,10 +120,10 @@ __global__ void __launch_bounds__(128) sab_enc_hash1(int n, const uint8_t* __res
   }
 }
 
-__global__ void __launch_bounds__(128) sab_dec_hash1(int n, const uint8_t* __restrict__ sk,
+__global__ void __launch_bounds__(128) sab_dec_hash1(int role_sel, int n, const uint8_t* __restrict__ sk,
                                                      const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
                                                      uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
-  const int role = threadIdx.x >> 6;
+  const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
   SAB_PAIR_SETUP(32)
   const uint8_t* s = sk + (size_t)inst * kSaberSk;
   uint32_t a[25];
@@ -397,7 +397,7 @@ static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
                          uint8_t* sraw) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, dim3(blocks_for(n, 32)), dim3(128), st, n, coins, pk, sk, araw, sraw);
+  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, coins, pk, sk, araw, sraw);
   SabPolyArgs a{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
   DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), dim3(128), st, a);
   DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
@@ -406,7 +406,7 @@ void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* p
 void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, dim3(blocks_for(n, 32)), dim3(128), st, n, pk, coins, araw, sraw, mbuf, krbuf);
+  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, pk, coins, araw, sraw, mbuf, krbuf);
   SabPolyArgs a{n, araw, sraw, pk, (size_t)kSaberPk, mbuf, ct, nullptr, nullptr, nullptr};
   DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), dim3(128), st, a);
   DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, nullptr, nullptr, ss);
@@ -416,10 +416,82 @@ void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
                       uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail) {
   if (n <= 0) return;
   DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), dim3(64), st, n, ct, sk, mbuf);
-  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, dim3(blocks_for(n, 32)), dim3(128), st, n, sk, mbuf, araw, sraw, krbuf);
+  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, sk, mbuf, araw, sraw, krbuf);
   SabPolyArgs a{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
   DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), dim3(128), st, a);
   DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail, ss);
+}
+
+// Chained keygen -> encaps -> decaps of the same n instances as a stage DAG
+// over four streams.  Encaps' and decaps' GenMatrix streams depend only on
+// keygen's seedA, so they leave the critical path, which becomes
+//   seedA+GenMatrix (24) -> poly -> m,H(pk),G,GenSecret (15) -> poly -> inner
+//   -> G,GenSecret (6) -> poly -> H(c),K (10)
+// permutations instead of the 98 of three back-to-back ops.  `ev` holds 8
+// events; `scratch` holds three sets of per-op scratch (kSaberChainScratch
+// bytes per instance).
+void saber_chain_launch(hipStream_t st[4], hipEvent_t ev[8], int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk,
+                        uint8_t* sk, uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+  if (n <= 0) return;
+  size_t off = 0;
+  auto take = [&](size_t per) {
+    uint8_t* p = scratch + off;
+    off += ((size_t)n * per + 255) & ~(size_t)255;
+    return p;
+  };
+  uint8_t *araw[3], *sraw[3], *mbuf[3], *krbuf[3];
+  for (int i = 0; i < 3; ++i) {
+    araw[i] = take(kSaberGenMatrixBytes);
+    sraw[i] = take(kSaberGenSecretBytes);
+    mbuf[i] = take(32);
+    krbuf[i] = take(64);
+  }
+  uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
+  enum { K1a, K1b, K2, K3, E1a, E2, E3, D1a };
+  const dim3 g(blocks_for(n, 32)), b64(64);
+  // S1: keygen GenSecret
+  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, g, b64, st[1], 1, n, kc, pk, sk, araw[0], sraw[0]);
+  (void)hipEventRecord(ev[K1b], st[1]);
+  // S0: keygen seedA + GenMatrix, then the keygen product
+  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, g, b64, st[0], 0, n, kc, pk, sk, araw[0], sraw[0]);
+  (void)hipEventRecord(ev[K1a], st[0]);
+  // S1 / S3: encaps / decaps GenMatrix as soon as seedA exists
+  (void)hipStreamWaitEvent(st[1], ev[K1a], 0);
+  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, g, b64, st[1], 0, n, pk, ec, araw[1], sraw[1], mbuf[1], krbuf[1]);
+  (void)hipEventRecord(ev[E1a], st[1]);
+  (void)hipStreamWaitEvent(st[3], ev[K1a], 0);
+  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, g, b64, st[3], 0, n, sk, mbuf[2], araw[2], sraw[2], krbuf[2]);
+  (void)hipEventRecord(ev[D1a], st[3]);
+  (void)hipStreamWaitEvent(st[0], ev[K1b], 0);
+  SabPolyArgs ak{n, araw[0], sraw[0], nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
+  DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), dim3(128), st[0], ak);
+  (void)hipEventRecord(ev[K2], st[0]);
+  // S2: H(pk) into sk
+  (void)hipStreamWaitEvent(st[2], ev[K2], 0);
+  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, g, b64, st[2], n, pk, sk);
+  (void)hipEventRecord(ev[K3], st[2]);
+  // S0: encaps message chain, product
+  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, g, b64, st[0], 1, n, pk, ec, araw[1], sraw[1], mbuf[1], krbuf[1]);
+  (void)hipStreamWaitEvent(st[0], ev[E1a], 0);
+  SabPolyArgs ae{n, araw[1], sraw[1], pk, (size_t)kSaberPk, mbuf[1], ct, nullptr, nullptr, nullptr};
+  DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), dim3(128), st[0], ae);
+  (void)hipEventRecord(ev[E2], st[0]);
+  // S2: encaps K
+  (void)hipStreamWaitEvent(st[2], ev[E2], 0);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st[2], n, ct, krbuf[1], nullptr, nullptr, ss);
+  (void)hipEventRecord(ev[E3], st[2]);
+  // S0: decaps
+  DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), b64, st[0], n, ct, sk, mbuf[2]);
+  (void)hipStreamWaitEvent(st[0], ev[K3], 0);
+  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, g, b64, st[0], 1, n, sk, mbuf[2], araw[2], sraw[2], krbuf[2]);
+  (void)hipStreamWaitEvent(st[0], ev[D1a], 0);
+  SabPolyArgs ad{n, araw[2], sraw[2], sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf[2], nullptr, nullptr, ct, fail};
+  DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), dim3(128), st[0], ad);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st[0], n, ct, krbuf[2], sk, fail, ss2);
+  // join S1, S2, S3 back into S0
+  (void)hipStreamWaitEvent(st[0], ev[E3], 0);
+  (void)hipStreamWaitEvent(st[0], ev[E1a], 0);
+  (void)hipStreamWaitEvent(st[0], ev[D1a], 0);
 }
 
 }  // namespace dpkem
