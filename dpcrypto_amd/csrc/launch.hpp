@@ -14,11 +14,12 @@ void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* c
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
                       uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail);
 
-// Chained Saber keygen -> enc -> dec of the same instances as a 4-stream
-// stage DAG (saber.hip); ev: 8 events, scratch: kSaberChainScratchPerInst per instance.
-void saber_chain_launch(hipStream_t st[4], hipEvent_t ev[8], int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk,
-                        uint8_t* sk, uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch);
-constexpr size_t kSaberChainScratchPerInst = 3 * (3744 + 768 + 32 + 64) + 4;
+// Chained Saber keygen -> enc -> dec of the same instances on one stream,
+// sharing keygen's expanded matrix (saber.hip); scratch:
+// kSaberChainScratchPerInst bytes per instance.
+void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
+                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch);
+constexpr size_t kSaberChainScratchPerInst = 3744 + 768 + 32 + 64 + 4;
 
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
                          uint32_t* e8);

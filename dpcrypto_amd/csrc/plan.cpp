@@ -32,13 +32,12 @@ struct Task {
 
 struct Sub {
   int task;
-  int chain = -1;          // >= 0: Saber keygen/enc/dec chain (tasks task, chain, chain2) as one stage DAG
+  int chain = -1;          // >= 0: Saber keygen/enc/dec chain (tasks task, chain, chain2) in one launch sequence
   int chain2 = -1;
   size_t begin, count;
   int stream;
   size_t scratch_off;
   hipEvent_t done = nullptr;
-  hipEvent_t ev[8] = {};   // chain DAG events
   std::vector<int> deps;   // sub indices
 };
 
@@ -67,8 +66,6 @@ void release(dpkem_plan* p) {
   p->graph = nullptr;
   for (auto& s : p->subs) {
     if (s.done) (void)hipEventDestroy(s.done);
-    for (auto ev : s.ev)
-      if (ev) (void)hipEventDestroy(ev);
   }
   for (auto ev : p->join) (void)hipEventDestroy(ev);
   p->join.clear();
@@ -84,7 +81,9 @@ void release(dpkem_plan* p) {
 size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Saber keygen t0 -> enc t1 -> dec t2 over the same arrays (t1 consumes t0's
-// pk, t2 consumes t1's ct and t0's sk): run as one stage DAG per chunk.
+// pk, t2 consumes t1's ct and t0's sk): each chunk runs the three ops back to
+// back on one stream and expands the public matrix once.  t1's and t2's
+// sub-tasks are folded into t0's, so their dependants wait for whole chains.
 bool find_saber_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
   const Task& K = p->tasks[t0];
   if (K.scheme != DPKEM_SABER || K.op != kKeygen || K.after != -1) return false;
@@ -132,7 +131,7 @@ void schedule(dpkem_plan* p) {
         s.chain2 = t2;
         s.begin = b;
         s.count = std::min(per, T.n - b);
-        s.stream = (int)((4 * c + (size_t)chain_of[t] * 4 * k) % (size_t)p->nstreams);
+        s.stream = (int)((c + (size_t)chain_of[t] * k) % (size_t)p->nstreams);
         s.scratch_off = off;
         off += round_up(s.count * kSaberChainScratchPerInst + 16 * 256);
         task_subs[t].push_back((int)p->subs.size());
@@ -226,19 +225,12 @@ int dpkem_plan_build(dpkem_plan* p) {
   } while (0)
   PLAN_TRY(hipSetDevice(e->device));
   PLAN_TRY(hipMalloc(&p->scratch, std::max<size_t>(p->scratch_bytes, 256)));
-  bool has_chain = false;
-  for (auto& s : p->subs) has_chain |= s.chain >= 0;
-  if (has_chain && p->nstreams < 4) p->nstreams = 4;  // a chain DAG spans four streams
   p->streams.resize(p->nstreams, nullptr);
   for (auto& st : p->streams) PLAN_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   PLAN_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
   p->join.resize(p->nstreams, nullptr);
   for (auto& ev : p->join) PLAN_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  for (auto& s : p->subs) {
-    PLAN_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-    if (s.chain >= 0)
-      for (auto& ev : s.ev) PLAN_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  }
+  for (auto& s : p->subs) PLAN_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
 
   // capture: origin -> fork -> per-stream sub-task chains -> join -> origin
   hipStream_t origin = p->streams[0];
@@ -263,10 +255,8 @@ int dpkem_plan_build(dpkem_plan* p) {
     if (s.chain >= 0) {
       const Task& E = p->tasks[s.chain];
       const Task& Dd = p->tasks[s.chain2];
-      hipStream_t cs[4];
-      for (int j = 0; j < 4; ++j) cs[j] = p->streams[(s.stream + j) % p->nstreams];
       const size_t b = s.begin;
-      saber_chain_launch(cs, s.ev, (int)s.count, T.i0 + b * 96, E.i1 + b * 32, T.o0 + b * 992, T.o1 + b * 2304,
+      saber_chain_launch(st, (int)s.count, T.i0 + b * 96, E.i1 + b * 32, T.o0 + b * 992, T.o1 + b * 2304,
                          E.o0 + b * 1088, E.o1 + b * 32, Dd.o0 + b * 32, p->scratch + s.scratch_off);
       ok = ok && hipGetLastError() == hipSuccess;
       ok = ok && hipEventRecord(s.done, st) == hipSuccess;

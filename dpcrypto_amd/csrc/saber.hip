@@ -200,9 +200,15 @@ struct SabPolyArgs {
   uint32_t* fail;        // reenc: per-instance mismatch flag
 };
 
+constexpr int kARow = 264;  // u16 row stride of the plain A / b polynomials (spreads LDS banks)
+
+// The secret (s or s', small) is the Toeplitz-staged factor: three staged
+// polynomials serve all nine MatVec products and the three inner-product
+// products; A and b are read as plain rows, each 16-byte read broadcast to
+// the 32 threads of an output row.
 struct SabPolyLds {
-  uint16_t T[12 * kToepStride];  // 9 A polynomials + 3 b polynomials
-  uint16_t S[3 * 256];
+  uint16_t Ts[3 * kToepStride];      // Toeplitz form of s / s'
+  uint16_t Ap[12 * kARow];           // A (9 polynomials) then b (3)
   alignas(8) uint32_t stage[kSaberGenMatrixBytes / 4];
   uint8_t outb[kSaberCt + 64];
   uint32_t diff;
@@ -215,8 +221,8 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
   const int inst = blockIdx.x;
   if (inst >= a.n) return;
 
-  // 1. stage GenMatrix bytes, expand GenSecret with CBD (mu = 8)
-  // the hash kernels leave both streams in the lane-pair sliced layout
+  // 1. rebuild the GenMatrix bytes (lane-pair sliced stream) in LDS; expand
+  //    GenSecret with CBD (mu = 8) straight into the Toeplitz form of s
   const uint32_t* ar = reinterpret_cast<const uint32_t*>(a.araw + (size_t)inst * kSaberGenMatrixBytes);
   for (int w = tid; w < kSabGenMatrixWords; w += 128)
     reinterpret_cast<uint64_t*>(L.stage)[w] = k2::sliced_word(ar, w);
@@ -226,20 +232,24 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const uint32_t byte = (uint32_t)(v >> (8 * b)) & 0xFF;
-      L.S[8 * w + b] = (uint16_t)(popc(byte & 0xF) - popc(byte >> 4));
+      const int c = 8 * w + b;
+      toep_put(L.Ts, c >> 8, c & 255, (uint16_t)(popc(byte & 0xF) - popc(byte >> 4)));
     }
   }
   if (tid == 0) L.diff = 0;
   __syncthreads();
 
-  // 2. unpack A (9 polys, 288 groups of 8) into Toeplitz form; b (3 polys) from pk
+  // 2. unpack A (288 groups of 8) and b (192 groups of 4, from pk) as plain rows
   const uint8_t* sb = reinterpret_cast<const uint8_t*>(L.stage);
   for (int g = tid; g < 288; g += 128) {
     uint16_t v[8];
     unpack13_group(v, sb + 13 * g);
-    const int P = g >> 5, q0 = (g & 31) * 8;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) toep_put(L.T, P, q0 + i, v[i]);
+    uint4 q;
+    q.x = v[0] | ((uint32_t)v[1] << 16);
+    q.y = v[2] | ((uint32_t)v[3] << 16);
+    q.z = v[4] | ((uint32_t)v[5] << 16);
+    q.w = v[6] | ((uint32_t)v[7] << 16);
+    *reinterpret_cast<uint4*>(L.Ap + (g >> 5) * kARow + (g & 31) * 8) = q;
   }
   if (MODE != kSabKeygen) {
     const uint8_t* pkb = a.pk + (size_t)inst * a.pk_stride;
@@ -249,9 +259,10 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
       for (int i = 0; i < 5; ++i) bytes[i] = pkb[5 * g + i];
       uint16_t v[4];
       unpack10_group(v, bytes);
-      const int P = 9 + (g >> 6), q0 = (g & 63) * 4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) toep_put(L.T, P, q0 + i, v[i]);
+      uint2 q;
+      q.x = v[0] | ((uint32_t)v[1] << 16);
+      q.y = v[2] | ((uint32_t)v[3] << 16);
+      *reinterpret_cast<uint2*>(L.Ap + (9 + (g >> 6)) * kARow + (g & 63) * 4) = q;
     }
   }
   __syncthreads();
@@ -263,7 +274,7 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
   if (MODE != kSabKeygen || row < 3) {
     for (int j = 0; j < 3; ++j) {
       const int P = row < 3 ? (MODE == kSabKeygen ? j * 3 + row : row * 3 + j) : 9 + j;
-      toep_dot(acc, L.T, P, L.S + 256 * j, t0);
+      toep_dot(acc, L.Ts, j, L.Ap + P * kARow, t0);
     }
   }
 
@@ -307,12 +318,14 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
       reinterpret_cast<uint32_t*>(pk)[i] = ob[i];
       reinterpret_cast<uint32_t*>(sk + kSaberPolyVecBytes)[i] = ob[i];
     }
-    // sk[0:1248] = POLVECq(s): 96 groups of 8 coefficients -> 13 bytes
+    // sk[0:1248] = POLVECq(s): 96 groups of 8 coefficients -> 13 bytes; s is
+    // read back from the Toeplitz copy (r[-q] = s[q] at offset 256 - q)
     if (tid < 96) {
       uint64_t lo = 0, hi = 0;
 #pragma unroll
       for (int d = 0; d < 8; ++d) {
-        const uint64_t v = L.S[8 * tid + d] & 0x1FFF;
+        const int c = 8 * tid + d;
+        const uint64_t v = L.Ts[(c >> 8) * kToepStride + 256 - (c & 255)] & 0x1FFF;
         const int bit = 13 * d;
         if (bit < 64) lo |= v << bit;
         if (bit + 13 > 64) hi |= bit >= 64 ? v << (bit - 64) : v >> (64 - bit);
@@ -339,11 +352,11 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
 }
 
 // Decaps inner product v = b' . s and message recovery
-// m'_t = ((v + h2 - c_m * 2^6) >> 9) & 1.  Two instances per 64-thread block.
+// m'_t = ((v + h2 - c_m * 2^6) >> 9) & 1.  Two instances per 64-thread block;
+// s (from sk) is the Toeplitz-staged factor, b' (from ct) the plain one.
 struct SabInnerLds {
-  uint16_t T[2][3 * kToepStride];
-  uint16_t S[2][3 * 256];
-  uint8_t m[2][32];
+  uint16_t Ts[2][3 * kToepStride];
+  uint16_t Bp[2][3 * kARow];
 };
 
 __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __restrict__ ct,
@@ -361,8 +374,10 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
       for (int i = 0; i < 5; ++i) bytes[i] = c[5 * g + i];
       uint16_t v[4];
       unpack10_group(v, bytes);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) toep_put(L.T[half], g >> 6, (g & 63) * 4 + i, v[i]);
+      uint2 q;
+      q.x = v[0] | ((uint32_t)v[1] << 16);
+      q.y = v[2] | ((uint32_t)v[3] << 16);
+      *reinterpret_cast<uint2*>(L.Bp[half] + (g >> 6) * kARow + (g & 63) * 4) = q;
     }
     for (int g = t; g < 96; g += 32) {  // s from sk (13-bit)
       uint8_t bytes[13];
@@ -371,14 +386,14 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
       uint16_t v[8];
       unpack13_group(v, bytes);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) L.S[half][8 * g + i] = v[i];
+      for (int i = 0; i < 8; ++i) toep_put(L.Ts[half], g >> 5, (g & 31) * 8 + i, v[i]);
     }
   }
   __syncthreads();
   if (!live) return;
   uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int t0 = 8 * t;
-  for (int j = 0; j < 3; ++j) toep_dot(acc, L.T[half], j, L.S[half] + 256 * j, t0);
+  for (int j = 0; j < 3; ++j) toep_dot(acc, L.Ts[half], j, L.Bp[half] + j * kARow, t0);
   const uint32_t cm = *reinterpret_cast<const uint32_t*>(c + kSaberPolyVecCompBytes + 4 * t);
   uint32_t byte = 0;
 #pragma unroll
@@ -390,7 +405,7 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
 }
 
 // ----------------------------------------------------------------- host launchers
-// (declared in engine.hpp; all run on the given stream)
+// (declared in launch.hpp; all run on the given stream)
 
 static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 
@@ -422,76 +437,44 @@ void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
   DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail, ss);
 }
 
-// Chained keygen -> encaps -> decaps of the same n instances as a stage DAG
-// over four streams.  Encaps' and decaps' GenMatrix streams depend only on
-// keygen's seedA, so they leave the critical path, which becomes
-//   seedA+GenMatrix (24) -> poly -> m,H(pk),G,GenSecret (15) -> poly -> inner
-//   -> G,GenSecret (6) -> poly -> H(c),K (10)
-// permutations instead of the 98 of three back-to-back ops.  `ev` holds 8
-// events; `scratch` holds three sets of per-op scratch (kSaberChainScratch
-// bytes per instance).
-void saber_chain_launch(hipStream_t st[4], hipEvent_t ev[8], int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk,
-                        uint8_t* sk, uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+// Chained keygen -> encaps -> decaps of the same n instances on one stream.
+// Encaps reads seedA from the pk keygen just wrote and decaps from the copy
+// in sk, so both GenMatrix streams equal keygen's: the chain expands A once
+// (23 of 98 serial Keccak permutations per instance saved twice) and the
+// encaps / decaps hash kernels run only their message roles.  scratch holds
+// kSaberChainScratchPerInst bytes per instance; the buffers are reused from
+// op to op, which the stream order makes safe.
+void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
+                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
   if (n <= 0) return;
   size_t off = 0;
   auto take = [&](size_t per) {
-    uint8_t* p = scratch + off;
+    uint8_t* q = scratch + off;
     off += ((size_t)n * per + 255) & ~(size_t)255;
-    return p;
+    return q;
   };
-  uint8_t *araw[3], *sraw[3], *mbuf[3], *krbuf[3];
-  for (int i = 0; i < 3; ++i) {
-    araw[i] = take(kSaberGenMatrixBytes);
-    sraw[i] = take(kSaberGenSecretBytes);
-    mbuf[i] = take(32);
-    krbuf[i] = take(64);
-  }
+  uint8_t* araw = take(kSaberGenMatrixBytes);
+  uint8_t* sraw = take(kSaberGenSecretBytes);
+  uint8_t* mbuf = take(32);
+  uint8_t* krbuf = take(64);
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
-  enum { K1a, K1b, K2, K3, E1a, E2, E3, D1a };
-  const dim3 g(blocks_for(n, 32)), b64(64);
-  // S1: keygen GenSecret
-  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, g, b64, st[1], 1, n, kc, pk, sk, araw[0], sraw[0]);
-  (void)hipEventRecord(ev[K1b], st[1]);
-  // S0: keygen seedA + GenMatrix, then the keygen product
-  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, g, b64, st[0], 0, n, kc, pk, sk, araw[0], sraw[0]);
-  (void)hipEventRecord(ev[K1a], st[0]);
-  // S1 / S3: encaps / decaps GenMatrix as soon as seedA exists
-  (void)hipStreamWaitEvent(st[1], ev[K1a], 0);
-  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, g, b64, st[1], 0, n, pk, ec, araw[1], sraw[1], mbuf[1], krbuf[1]);
-  (void)hipEventRecord(ev[E1a], st[1]);
-  (void)hipStreamWaitEvent(st[3], ev[K1a], 0);
-  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, g, b64, st[3], 0, n, sk, mbuf[2], araw[2], sraw[2], krbuf[2]);
-  (void)hipEventRecord(ev[D1a], st[3]);
-  (void)hipStreamWaitEvent(st[0], ev[K1b], 0);
-  SabPolyArgs ak{n, araw[0], sraw[0], nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), dim3(128), st[0], ak);
-  (void)hipEventRecord(ev[K2], st[0]);
-  // S2: H(pk) into sk
-  (void)hipStreamWaitEvent(st[2], ev[K2], 0);
-  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, g, b64, st[2], n, pk, sk);
-  (void)hipEventRecord(ev[K3], st[2]);
-  // S0: encaps message chain, product
-  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, g, b64, st[0], 1, n, pk, ec, araw[1], sraw[1], mbuf[1], krbuf[1]);
-  (void)hipStreamWaitEvent(st[0], ev[E1a], 0);
-  SabPolyArgs ae{n, araw[1], sraw[1], pk, (size_t)kSaberPk, mbuf[1], ct, nullptr, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), dim3(128), st[0], ae);
-  (void)hipEventRecord(ev[E2], st[0]);
-  // S2: encaps K
-  (void)hipStreamWaitEvent(st[2], ev[E2], 0);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st[2], n, ct, krbuf[1], nullptr, nullptr, ss);
-  (void)hipEventRecord(ev[E3], st[2]);
-  // S0: decaps
-  DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), b64, st[0], n, ct, sk, mbuf[2]);
-  (void)hipStreamWaitEvent(st[0], ev[K3], 0);
-  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, g, b64, st[0], 1, n, sk, mbuf[2], araw[2], sraw[2], krbuf[2]);
-  (void)hipStreamWaitEvent(st[0], ev[D1a], 0);
-  SabPolyArgs ad{n, araw[2], sraw[2], sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf[2], nullptr, nullptr, ct, fail};
-  DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), dim3(128), st[0], ad);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st[0], n, ct, krbuf[2], sk, fail, ss2);
-  // join S1, S2, S3 back into S0
-  (void)hipStreamWaitEvent(st[0], ev[E3], 0);
-  (void)hipStreamWaitEvent(st[0], ev[E1a], 0);
-  (void)hipStreamWaitEvent(st[0], ev[D1a], 0);
+  const dim3 g(blocks_for(n, 32)), b64(64), b128(128);
+  // keygen
+  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, g, b128, st, -1, n, kc, pk, sk, araw, sraw);
+  SabPolyArgs ak{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
+  DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), b128, st, ak);
+  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, g, b64, st, n, pk, sk);
+  // encaps: message role only, A from keygen
+  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, g, b64, st, 1, n, pk, ec, araw, sraw, mbuf, krbuf);
+  SabPolyArgs ae{n, araw, sraw, pk, (size_t)kSaberPk, mbuf, ct, nullptr, nullptr, nullptr};
+  DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), b128, st, ae);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st, n, ct, krbuf, nullptr, nullptr, ss);
+  // decaps: message role only, A from keygen
+  DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf);
+  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, g, b64, st, 1, n, sk, mbuf, araw, sraw, krbuf);
+  SabPolyArgs ad{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
+  DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), b128, st, ad);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st, n, ct, krbuf, sk, fail, ss2);
 }
 
 }  // namespace dpkem

@@ -34,19 +34,21 @@ __device__ __forceinline__ void toep_put(uint16_t* T, int P, int q, uint16_t v) 
   }
 }
 
-// acc[d] += sum_k r_P[k - (t0 + d)] * s[k],  d = 0..7
+// acc[d] += sum_k r_P[k - (t0 + d)] * S[k],  d = 0..7, where r_P is the
+// Toeplitz form of polynomial P in T and S a plain polynomial (16-byte aligned).
+// The negacyclic product is commutative, so either factor may be the staged one.
 __device__ __forceinline__ void toep_dot(uint32_t acc[8], const uint16_t* T, int P, const uint16_t* S, int t0) {
-  const uint32_t* r1 = reinterpret_cast<const uint32_t*>(T + P * kToepStride);
-  const uint32_t* r2 = reinterpret_cast<const uint32_t*>(T + P * kToepStride + 520);
-  const uint32_t* sv = reinterpret_cast<const uint32_t*>(S);
-  const int base = (256 - t0) >> 1;  // dword index of r[-t0] pair in each copy
-  uint4 p1 = *reinterpret_cast<const uint4*>(r1 + base - 4);
-  uint4 p2 = *reinterpret_cast<const uint4*>(r2 + base - 4);
+  const uint4* r1 = reinterpret_cast<const uint4*>(T + P * kToepStride);
+  const uint4* r2 = reinterpret_cast<const uint4*>(T + P * kToepStride + 520);
+  const uint4* sv = reinterpret_cast<const uint4*>(S);
+  const int base = (256 - t0) >> 3;  // 16-byte index of the r[-t0] pair in each copy
+  uint4 p1 = r1[base - 1];
+  uint4 p2 = r2[base - 1];
 #pragma unroll
   for (int kk = 0; kk < 256; kk += 8) {
-    const uint4 c1 = *reinterpret_cast<const uint4*>(r1 + base + (kk >> 1));
-    const uint4 c2 = *reinterpret_cast<const uint4*>(r2 + base + (kk >> 1));
-    const uint4 s4 = *reinterpret_cast<const uint4*>(sv + (kk >> 1));
+    const uint4 c1 = r1[base + (kk >> 3)];
+    const uint4 c2 = r2[base + (kk >> 3)];
+    const uint4 s4 = sv[kk >> 3];
     const uint32_t w1[8] = {p1.x, p1.y, p1.z, p1.w, c1.x, c1.y, c1.z, c1.w};  // dwords -4..3
     const uint32_t w2[8] = {p2.x, p2.y, p2.z, p2.w, c2.x, c2.y, c2.z, c2.w};
     const uint32_t sp[4] = {s4.x, s4.y, s4.z, s4.w};

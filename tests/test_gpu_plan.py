@@ -55,7 +55,7 @@ def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, sc
     t1 = plan.enc(scheme, ct, ss, pk, dec_, after=t0)
     plan.dec(scheme, ss2, ct, sk, after=t1)
     plan.build()
-    # Saber keygen -> enc -> dec on the same arrays runs as one stage DAG per chunk
+    # Saber keygen -> enc -> dec on the same arrays runs as one launch sequence per chunk
     per_chunk = 1 if scheme == SABER else 3
     assert plan.info()["subtasks"] == per_chunk * min(chunks, n)
     for rep in range(3):

@@ -14,12 +14,15 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CXX = shutil.which("g++") or shutil.which("clang++")
+# ext_vector_type (used by the dot-product helpers) needs clang
+CLANGXX = next((c for c in ("/opt/rocm/llvm/bin/clang++", shutil.which("clang++")) if c and os.path.exists(c)), None)
 
 
-def _build_and_run(src, tmp_path, oracle):
+def _build_and_run(src, tmp_path, oracle, cxx=None):
     exe = str(tmp_path / "prog")
     odir = os.path.join(ROOT, "oracle")
-    subprocess.run([CXX, "-O1", "-std=c++17", os.path.join(HERE, "host", src), "-o", exe,
+    subprocess.run([cxx or CXX, "-O1", "-std=c++17", "-I", os.path.join(HERE, "host", "include"),
+                    os.path.join(HERE, "host", src), "-o", exe,
                     f"-L{odir}", "-loracle", f"-Wl,-rpath,{odir}"], check=True)
     out = subprocess.run([exe], capture_output=True, text=True)
     return out.returncode, out.stdout
@@ -28,4 +31,10 @@ def _build_and_run(src, tmp_path, oracle):
 @pytest.mark.skipif(CXX is None, reason="no host C++ compiler")
 def test_lane_pair_keccak_matches_oracle(tmp_path, oracle):
     rc, out = _build_and_run("k2_emul.cpp", tmp_path, oracle)
+    assert rc == 0 and out.strip() == "OK", out
+
+
+@pytest.mark.skipif(CLANGXX is None, reason="no clang++")
+def test_saber_toeplitz_product_matches_oracle(tmp_path, oracle):
+    rc, out = _build_and_run("toep_emul.cpp", tmp_path, oracle, cxx=CLANGXX)
     assert rc == 0 and out.strip() == "OK", out
