@@ -6,12 +6,22 @@ of 4096 instances, one step = keygen + encaps + decaps of every instance,
 inputs (coins) resident in HBM before the timed region, outputs left in HBM.
 value = KEM operations per second = 3 * 4096 * steps / seconds.
 
+Every op recomputes everything from its own inputs (encaps expands the
+public matrix from pk, decaps from sk), as separate parties would.
+
 Reported beside it in the same JSON line:
   frodo976       FrodoKEM-976-SHAKE, batch 8192, keygen + encaps + decaps
   plain          the one-workgroup-per-KEM comparator form on the same inputs
   latency        p50 / p99 of the per-step wall time (ms)
   roofline       dominant kernel of the headline workload, from HIP events on
                  the engine's stream during the timed steps
+  mixed          16,384 instances in random Frodo / Saber batches of U[1,4096],
+                 ops 1:4:4, one autotuned graph vs plain per-batch launches
+  serving        Poisson arrivals into the continuous-batching server
+                 (70% Saber, ops 1:4:4) vs one request per launch
+  saber_shared_matrix_chain
+                 the headline steps with the plan's opt-in matrix sharing
+                 (encaps / decaps reuse keygen's A); not the headline value
   cpu_baseline   the CPU oracle on the host cores, on a bounded sample
 """
 from __future__ import annotations
@@ -114,7 +124,7 @@ def run_workload(torch, eng, scheme, n, steps, warmup, seed, profile=False):
                 ops_per_s=3 * n * steps / wall)
 
 
-def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=60.0):
+def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=60.0, share_matrix=False):
     """Time `steps` keygen+enc+dec steps, each one replay of a graph-captured
     plan (keygen -> enc -> dec chained per instance chunk); the plan's
     (streams, chunks) schedule is picked by the autotuner first."""
@@ -135,14 +145,14 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
     eng.set_stream(stream.cuda_stream)
 
     def build(ns, ch):
-        p = eng.plan(nstreams=ns, chunks=ch)
+        p = eng.plan(nstreams=ns, chunks=ch, share_matrix=share_matrix)
         t0 = p.keypair(scheme, pk, sk, kc)
         t1 = p.enc(scheme, ct, ss, pk, ec, after=t0)
         p.dec(scheme, ss2, ct, sk, after=t1)
         return p.build()
 
-    best, tried = autotune.tune(torch, eng, f"{'frodo976' if scheme == 0 else 'saber'}|kg-enc-dec|n={n}", build,
-                                use_cache=False, budget_s=tune_budget_s)
+    key = f"{'frodo976' if scheme == 0 else 'saber'}|kg-enc-dec|n={n}|share={int(share_matrix)}"
+    best, tried = autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s)
     plan = build(*best)
     for _ in range(warmup):
         plan.launch()
@@ -186,26 +196,46 @@ def roofline_from(stats, n):
     }
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(budget_s: float):
-    """The oracle (a scalar C port) on the host cores, bounded sample."""
+    """The oracle (a scalar C port) on the host cores, bounded sample; also
+    one core for a quarter of the budget."""
     import numpy as np
     from oracle import pyoracle as O
     threads = max(1, min(os.cpu_count() or 1, 16))
-    n = threads * 8
-    rng = np.random.default_rng(1)
-    kc = rng.integers(0, 256, (n, 96), dtype=np.uint8)
-    ec = rng.integers(0, 256, (n, 32), dtype=np.uint8)
-    ops = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        pk, sk = O.keypair(1, kc, threads=threads)
-        ct, ss = O.enc(1, pk, ec, threads=threads)
-        O.dec(1, ct, sk, threads=threads)
-        ops += 3 * n
-    dt = time.perf_counter() - t0
-    return {"value": round(ops / dt, 1), "unit": "KEM ops/s", "cores": threads, "kind": "port",
-            "sample": f"Saber l=3 keygen+encaps+decaps, {n} instances per round, {ops // (3 * n)} rounds "
-                      f"({dt:.1f} s) on {threads} threads of the CPU oracle (oracle/, -O3 C)"}
+
+    def rate(th, seconds):
+        n = th * 8
+        rng = np.random.default_rng(1)
+        kc = rng.integers(0, 256, (n, 96), dtype=np.uint8)
+        ec = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+        ops = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            pk, sk = O.keypair(1, kc, threads=th)
+            ct, ss = O.enc(1, pk, ec, threads=th)
+            O.dec(1, ct, sk, threads=th)
+            ops += 3 * n
+        dt = time.perf_counter() - t0
+        return ops / dt, n, ops // (3 * n), dt
+
+    v, n, rounds, dt = rate(threads, budget_s)
+    v1, _, _, dt1 = rate(1, budget_s / 4)
+    return {"value": round(v, 1), "unit": "KEM ops/s", "cores": threads, "kind": "port",
+            "sample": f"Saber l=3 keygen+encaps+decaps, {n} instances per round, {rounds} rounds "
+                      f"({dt:.1f} s) on {threads} threads of the CPU oracle (oracle/, -O3 C)",
+            "single_core_value": round(v1, 1), "single_core_seconds": round(dt1, 1),
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
 
 
 def main():
@@ -220,6 +250,10 @@ def main():
     ap.add_argument("--skip-frodo", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--skip-mixed", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=0.5, help="arrival window per rate")
+    ap.add_argument("--serving-rates", default="10000,50000,100000,200000,500000")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -292,6 +326,32 @@ def main():
             plain["frodo976_speedup_batched_over_plain"] = round(fr["ops_per_s"] / fp["ops_per_s"], 2)
         pe.close()
         result["plain"] = plain
+    progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
+    shared = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
+                               tune_budget_s=20.0, share_matrix=True)
+    result["saber_shared_matrix_chain"] = {
+        "ops_per_s": round(shared["ops_per_s"], 1), "roundtrip_ok": shared["roundtrip_ok"],
+        "schedule": {"nstreams": shared["schedule"]["nstreams"], "chunks": shared["schedule"]["chunks"]},
+        "note": "encaps/decaps reuse keygen's expanded matrix; valid only when one caller runs all three ops"}
+    if not args.skip_mixed:
+        from dpcrypto_amd import workloads as W
+        pe = D.Engine(0, form=D.FORM_PLAIN)
+        result["mixed"] = W.run_mixed(torch, eng, pe, progress=progress)
+        pe.close()
+        progress(f"mixed: {result['mixed']['batched_ops_per_s']:.0f} ops/s batched, "
+                 f"{result['mixed']['plain_ops_per_s']:.0f} plain")
+    if not args.skip_serving:
+        from dpcrypto_amd import workloads as W
+        rates = tuple(int(r) for r in args.serving_rates.split(","))
+        bat = W.run_serving(rates, args.serving_seconds, progress=progress)
+        one = W.run_serving(rates, args.serving_seconds, one_per_launch=True, progress=progress)
+        result["serving"] = {
+            "workload": f"Poisson arrivals, {args.serving_seconds} s per rate, 70% Saber / 30% Frodo-976, "
+                        f"keygen:enc:dec = 1:4:4, keys and ciphertexts resident in HBM, budget 32 GiB; "
+                        f"latency = completion - scheduled arrival; requests still queued 2 s after the last "
+                        f"arrival are cancelled",
+            "batched": bat["rows"], "batched_server": bat["server"],
+            "one_request_per_launch": one["rows"]}
     if args.cpu_seconds > 0:
         progress(f"cpu baseline ({args.cpu_seconds:.0f} s)")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -46,6 +46,7 @@ struct Sub {
 struct dpkem_plan {
   dpkem_engine* e = nullptr;
   int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED;
+  unsigned flags = 0;
   std::vector<Task> tasks;
   std::vector<Sub> subs;
   std::vector<hipStream_t> streams;
@@ -85,6 +86,7 @@ size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
 // back on one stream and expands the public matrix once.  t1's and t2's
 // sub-tasks are folded into t0's, so their dependants wait for whole chains.
 bool find_saber_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
+  if (!(p->flags & DPKEM_PLAN_SHARE_MATRIX)) return false;
   const Task& K = p->tasks[t0];
   if (K.scheme != DPKEM_SABER || K.op != kKeygen || K.after != -1) return false;
   for (size_t a = t0 + 1; a < p->tasks.size(); ++a) {
@@ -200,6 +202,12 @@ int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uin
   }
   p->tasks.push_back({scheme, op, n, o0, o1, i0, i1, after});
   if (task_id) *task_id = (int)p->tasks.size() - 1;
+  return DPKEM_OK;
+}
+
+int dpkem_plan_set_options(dpkem_plan* p, unsigned flags) {
+  if (!p || p->exec || (flags & ~DPKEM_PLAN_SHARE_MATRIX)) return DPKEM_EINVAL;
+  p->flags = flags;
   return DPKEM_OK;
 }
 

@@ -427,6 +427,36 @@ int dpkem_server_drain(dpkem_server* s) {
   return DPKEM_OK;
 }
 
+int dpkem_server_cancel(dpkem_server* s, size_t* cancelled) {
+  if (!s) return DPKEM_EINVAL;
+  size_t k = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    const double t = now_us();
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 3; ++b) {
+        for (const Pending& P : s->pending[a][b]) {
+          dpkem_completion c;
+          memset(&c, 0, sizeof c);
+          c.id = P.r.id;
+          c.status = DPKEM_ECANCELED;
+          c.scheme = a;
+          c.op = b;
+          c.t_submit_us = P.r.t_arrival_us > 0 ? P.r.t_arrival_us : P.t_submit;
+          c.t_done_us = t;
+          s->done.push_back(c);
+        }
+        k += s->pending[a][b].size();
+        s->pending[a][b].clear();
+      }
+    s->inflight_requests -= k;
+    if (s->inflight_requests == 0) s->drained.notify_all();
+  }
+  s->cv.notify_one();
+  if (cancelled) *cancelled = k;
+  return DPKEM_OK;
+}
+
 int dpkem_server_release(dpkem_server* s, int scheme, int kind, uint32_t handle) {
   if (!s || scheme < 0 || scheme > 1) return DPKEM_EINVAL;
   std::lock_guard<std::mutex> lk(s->mu);
@@ -466,6 +496,7 @@ int dpkem_server_stats(dpkem_server* s, char* buf, size_t len) {
 
 int dpkem_server_destroy(dpkem_server* s) {
   if (!s) return DPKEM_EINVAL;
+  (void)dpkem_server_cancel(s, nullptr);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;

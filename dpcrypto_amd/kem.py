@@ -30,7 +30,7 @@ FORM_BATCHED = 0
 FORM_PLAIN = 1
 
 ERRORS = {
-    -1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EBUDGET", -5: "EBUSY", -6: "ENOTFOUND",
+    -1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EBUDGET", -5: "EBUSY", -6: "ENOTFOUND", -7: "ECANCELED",
 }
 
 SIZES = {
@@ -71,6 +71,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_engine_set_profiling.argtypes = [vp, i]
         lib.dpkem_plan_create.argtypes = [vp, ctypes.POINTER(vp), i, i]
         lib.dpkem_plan_add.argtypes = [vp, i, i, sz, vp, vp, vp, vp, i, ctypes.POINTER(i)]
+        lib.dpkem_plan_set_options.argtypes = [vp, ctypes.c_uint]
         lib.dpkem_plan_build.argtypes = [vp]
         lib.dpkem_plan_launch.argtypes = [vp]
         lib.dpkem_plan_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
@@ -80,6 +81,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_server_submit.argtypes = [vp, vp, sz]
         lib.dpkem_server_poll.argtypes = [vp, vp, sz, ctypes.POINTER(sz)]
         lib.dpkem_server_drain.argtypes = [vp]
+        lib.dpkem_server_cancel.argtypes = [vp, ctypes.POINTER(sz)]
         lib.dpkem_server_release.argtypes = [vp, i, i, ctypes.c_uint32]
         lib.dpkem_server_read.argtypes = [vp, i, i, ctypes.c_uint32, vp]
         lib.dpkem_server_stats.argtypes = [vp, ctypes.c_char_p, sz]
@@ -220,8 +222,8 @@ class Engine:
         self._check(self.lib.dpkem_dec(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), mem), "dec")
         return ss
 
-    def plan(self, nstreams: int = 1, chunks: int = 1) -> "Plan":
-        return Plan(self, nstreams, chunks)
+    def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False) -> "Plan":
+        return Plan(self, nstreams, chunks, share_matrix)
 
     # raw pointer entry points (device arrays preallocated by the caller)
     def keypair_into(self, scheme, n, pk, sk, coins):
@@ -235,18 +237,21 @@ class Engine:
 
 
 OP_KEYGEN, OP_ENC, OP_DEC = 0, 1, 2
+PLAN_SHARE_MATRIX = 1
 
 
 class Plan:
     """A graph-captured set of KEM tasks over device arrays (dpkem_plan_*)."""
 
-    def __init__(self, engine: Engine, nstreams: int = 1, chunks: int = 1):
+    def __init__(self, engine: Engine, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False):
         self.eng = engine
         self.lib = engine.lib
         h = ctypes.c_void_p()
         engine._check(self.lib.dpkem_plan_create(engine.h, ctypes.byref(h), nstreams, chunks), "plan_create")
         self.h = h
         self._keep = []  # arrays referenced by the graph stay alive with the plan
+        if share_matrix:
+            engine._check(self.lib.dpkem_plan_set_options(self.h, PLAN_SHARE_MATRIX), "plan_set_options")
 
     def add(self, scheme, op, n, o0=None, o1=None, i0=None, i1=None, after=-1) -> int:
         tid = ctypes.c_int(-1)
@@ -339,6 +344,14 @@ class Server:
         rc = self.lib.dpkem_server_drain(self.h)
         if rc:
             raise DpkemError(rc, "server_drain")
+
+    def cancel(self) -> int:
+        """Drop requests not yet admitted (their completions carry ECANCELED)."""
+        k = ctypes.c_size_t()
+        rc = self.lib.dpkem_server_cancel(self.h, ctypes.byref(k))
+        if rc:
+            raise DpkemError(rc, "server_cancel")
+        return k.value
 
     def release(self, scheme, kind, handle):
         return self.lib.dpkem_server_release(self.h, scheme, kind, handle)

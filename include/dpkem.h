@@ -40,6 +40,7 @@ extern "C" {
 #define DPKEM_EBUDGET -4   /* request exceeds the engine's HBM budget */
 #define DPKEM_EBUSY -5     /* serving queue full (try again) */
 #define DPKEM_ENOTFOUND -6 /* unknown request / key handle */
+#define DPKEM_ECANCELED -7 /* serving request dropped before it was admitted */
 
 typedef struct dpkem_engine dpkem_engine;
 
@@ -113,6 +114,14 @@ typedef struct dpkem_plan dpkem_plan;
 int dpkem_plan_create(dpkem_engine *e, dpkem_plan **out, int nstreams, int chunks);
 int dpkem_plan_add(dpkem_plan *p, int scheme, int op, size_t n, uint8_t *o0, uint8_t *o1, const uint8_t *i0,
                    const uint8_t *i1, int after, int *task_id);
+/* Options (before dpkem_plan_build).  DPKEM_PLAN_SHARE_MATRIX: a Saber
+ * keygen -> enc -> dec chain over the same arrays (enc reads keygen's pk,
+ * dec reads enc's ct and keygen's sk) expands the public matrix once per
+ * instance instead of three times; results are identical.  Off by default:
+ * it applies only when one caller runs all three ops on the same instances
+ * (self-tests, known-answer runs), never across parties. */
+#define DPKEM_PLAN_SHARE_MATRIX 1u
+int dpkem_plan_set_options(dpkem_plan *p, unsigned flags);
 int dpkem_plan_build(dpkem_plan *p);
 int dpkem_plan_launch(dpkem_plan *p);
 int dpkem_plan_info(dpkem_plan *p, size_t *subtasks, size_t *scratch_bytes);
@@ -163,6 +172,10 @@ int dpkem_server_create(dpkem_server **out, const dpkem_server_config *cfg);
 int dpkem_server_submit(dpkem_server *s, const dpkem_request *reqs, size_t n);
 int dpkem_server_poll(dpkem_server *s, dpkem_completion *out, size_t cap, size_t *got);
 int dpkem_server_drain(dpkem_server *s);
+/* Drop every request not yet admitted to a lane; each gets a completion with
+ * status DPKEM_ECANCELED.  Batches already in flight finish normally.
+ * dpkem_server_destroy does this first. */
+int dpkem_server_cancel(dpkem_server *s, size_t *cancelled);
 /* kind: 0 = key handle, 1 = ciphertext handle */
 int dpkem_server_release(dpkem_server *s, int scheme, int kind, uint32_t handle);
 /* kind: 0 = pk, 1 = sk, 2 = ct; copies the stored bytes to host memory */

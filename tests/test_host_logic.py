@@ -38,3 +38,11 @@ def test_lane_pair_keccak_matches_oracle(tmp_path, oracle):
 def test_saber_toeplitz_product_matches_oracle(tmp_path, oracle):
     rc, out = _build_and_run("toep_emul.cpp", tmp_path, oracle, cxx=CLANGXX)
     assert rc == 0 and out.strip() == "OK", out
+
+
+def test_mixed_batch_sizes_sum_to_the_total():
+    from dpcrypto_amd import workloads as W
+    for seed in (1, 2, 3):
+        b = W.mixed_batches(seed, total=16384, max_batch=4096)
+        assert sum(x[2] for x in b) == 16384
+        assert all(1 <= x[2] <= 4096 and x[0] in (0, 1) and x[1] in (0, 1, 2) for x in b)
