@@ -124,7 +124,8 @@ def run_workload(torch, eng, scheme, n, steps, warmup, seed, profile=False):
                 ops_per_s=3 * n * steps / wall)
 
 
-def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=60.0, share_matrix=False):
+def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=60.0, share_matrix=False,
+                      schedule=None):
     """Time `steps` keygen+enc+dec steps, each one replay of a graph-captured
     plan (keygen -> enc -> dec chained per instance chunk); the plan's
     (streams, chunks) schedule is picked by the autotuner first."""
@@ -152,7 +153,10 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
         return p.build()
 
     key = f"{'frodo976' if scheme == 0 else 'saber'}|kg-enc-dec|n={n}|share={int(share_matrix)}"
-    best, tried = autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s)
+    if schedule:
+        best, tried = schedule, {}
+    else:
+        best, tried = autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s)
     plan = build(*best)
     for _ in range(warmup):
         plan.launch()
@@ -254,6 +258,10 @@ def main():
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=0.5, help="arrival window per rate")
     ap.add_argument("--serving-rates", default="10000,50000,100000,200000,500000")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed plan schedule STREAMSxCHUNKS instead of autotuning (profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one elementwise kernel over 256 MiB in / 256 MiB out (HBM counter scale)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -268,9 +276,18 @@ def main():
     def progress(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    sched = tuple(int(x) for x in args.schedule.split("x")) if args.schedule else None
+    if args.calibrate_copy:
+        x = torch.ones(1 << 26, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        del x, y
     eng = D.Engine(0)
-    progress(f"saber batch {args.saber_batch}: autotune, {args.warmup} warmup + {args.steps} timed graph steps")
-    sab = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001)
+    progress(f"saber batch {args.saber_batch}: {'autotune, ' if not sched else ''}"
+             f"{args.warmup} warmup + {args.steps} timed graph steps")
+    sab = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
+                            schedule=sched)
     progress(f"saber: {sab['ops_per_s']:.0f} ops/s with schedule {sab['schedule']['nstreams']}x{sab['schedule']['chunks']}")
     progress("saber eager pass with per-kernel HIP events")
     sab_eager = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, 1, 0x44504B454D000001, True)
@@ -299,7 +316,8 @@ def main():
     }
     if not args.skip_frodo:
         progress(f"frodo976 batch {args.frodo_batch}: autotune, 1 warmup + {args.frodo_steps} timed graph steps")
-        fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002)
+        fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002,
+                               schedule=sched)
         progress(f"frodo976: {fr['ops_per_s']:.0f} ops/s; eager pass with per-kernel HIP events")
         fr_eager = run_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002, True)
         result["frodo976"] = {
@@ -328,7 +346,7 @@ def main():
         result["plain"] = plain
     progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
     shared = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
-                               tune_budget_s=20.0, share_matrix=True)
+                               tune_budget_s=20.0, share_matrix=True, schedule=sched)
     result["saber_shared_matrix_chain"] = {
         "ops_per_s": round(shared["ops_per_s"], 1), "roundtrip_ok": shared["roundtrip_ok"],
         "schedule": {"nstreams": shared["schedule"]["nstreams"], "chunks": shared["schedule"]["chunks"]},
