@@ -18,7 +18,7 @@ import os
 import statistics
 import time
 
-DEFAULT_CANDIDATES = ((1, 1), (2, 2), (3, 3), (4, 4), (4, 8), (6, 6), (8, 8))
+DEFAULT_CANDIDATES = ((1, 1), (2, 1), (4, 1), (8, 1), (2, 2), (4, 2), (4, 4), (8, 4), (4, 8), (8, 8))
 
 
 def cache_path() -> str:

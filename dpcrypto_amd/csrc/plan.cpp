@@ -133,7 +133,7 @@ void schedule(dpkem_plan* p) {
         s.chain2 = t2;
         s.begin = b;
         s.count = std::min(per, T.n - b);
-        s.stream = (int)((c + (size_t)chain_of[t] * k) % (size_t)p->nstreams);
+        s.stream = (int)((c + (size_t)chain_of[t]) % (size_t)p->nstreams);
         s.scratch_off = off;
         off += round_up(s.count * kSaberChainScratchPerInst + 16 * 256);
         task_subs[t].push_back((int)p->subs.size());
@@ -149,8 +149,9 @@ void schedule(dpkem_plan* p) {
       s.task = (int)t;
       s.begin = b;
       s.count = std::min(per, T.n - b);
-      // chunk c of a dependency chain stays on one stream; chains rotate
-      s.stream = (int)((c + (size_t)chain_of[t] * k) % (size_t)p->nstreams);
+      // chunk c of a dependency chain stays on one stream; independent chains
+      // start one stream further on, so their chunks spread over all streams
+      s.stream = (int)((c + (size_t)chain_of[t]) % (size_t)p->nstreams);
       s.scratch_off = off;
       off += round_up(chunk_scratch_bytes(T.scheme, p->form, s.count));
       if (T.after >= 0) {

@@ -80,6 +80,7 @@ struct Pending {
 };
 
 struct Lane {
+  int only_scheme = -1;         // >= 0: reserved for this scheme
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   uint8_t* scratch = nullptr;   // device: kernel scratch + dense I/O + index arrays
@@ -197,18 +198,25 @@ void launch_batch(dpkem_server* s, Lane& L) {
   (void)hipEventRecord(L.done, st);
 }
 
-// Choose the (scheme, op) group whose oldest request arrived first.
-bool pick_group(dpkem_server* s, int* sc, int* op) {
+// Choose the (scheme, op) group whose oldest request arrived first, among the
+// schemes lane `only` admits (-1: both).
+bool pick_group(dpkem_server* s, int only, int* sc, int* op) {
   double best = 0;
   bool found = false;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 3; ++b)
-      if (!s->pending[a][b].empty() && (!found || s->pending[a][b].front().t_submit < best)) {
-        best = s->pending[a][b].front().t_submit;
+  for (int a = 0; a < 2; ++a) {
+    if (only >= 0 && only != a) continue;
+    for (int b = 0; b < 3; ++b) {
+      if (s->pending[a][b].empty()) continue;
+      const Pending& P = s->pending[a][b].front();
+      const double t = P.r.t_arrival_us > 0 ? P.r.t_arrival_us : P.t_submit;
+      if (!found || t < best) {
+        best = t;
         *sc = a;
         *op = b;
         found = true;
       }
+    }
+  }
   return found;
 }
 
@@ -246,7 +254,7 @@ void worker_loop(dpkem_server* s) {
     for (auto& L : s->lanes) {
       if (L.busy) continue;
       int sc, op;
-      if (!pick_group(s, &sc, &op)) break;
+      if (!pick_group(s, L.only_scheme, &sc, &op)) continue;
       auto& q = s->pending[sc][op];
       const size_t m = std::min(q.size(), s->max_batch[sc]);
       L.scheme = sc;
@@ -326,6 +334,9 @@ double dpkem_now_us(void) { return now_us(); }
 
 int dpkem_server_create(dpkem_server** out, const dpkem_server_config* cfg) {
   if (!out || !cfg || cfg->lanes < 1 || cfg->lanes > 16) return DPKEM_EINVAL;
+  if (cfg->reserve_lanes[0] < 0 || cfg->reserve_lanes[1] < 0 ||
+      cfg->reserve_lanes[0] + cfg->reserve_lanes[1] > cfg->lanes)
+    return DPKEM_EINVAL;
   *out = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || cfg->device < 0 || cfg->device >= count) return DPKEM_EDEVICE;
@@ -379,6 +390,8 @@ int dpkem_server_create(dpkem_server** out, const dpkem_server_config* cfg) {
   const size_t lane_host =
       a256(std::max(s->max_batch[0], s->max_batch[1]) * (96 + 8 + 32)) + 4096 + 3 * 256;
   s->lanes.resize(cfg->lanes);
+  for (int i = 0; i < cfg->lanes; ++i)
+    s->lanes[i].only_scheme = i < cfg->reserve_lanes[0] ? 0 : i < cfg->reserve_lanes[0] + cfg->reserve_lanes[1] ? 1 : -1;
   for (auto& L : s->lanes)
     if (rc == DPKEM_OK) rc = init_lane(s, L, lane_dev, lane_host);
   if (rc != DPKEM_OK) {

@@ -311,8 +311,10 @@ def run_serving(rates=(10_000, 50_000, 100_000, 200_000, 500_000), duration_s: f
     peak = max(rates) * duration_s * 1.3 + 1024
     key_slots = (int(peak * 0.3 / 9) + 1024, int(peak * 0.7 / 9) + 4096)
     ct_slots = (int(peak * 0.3 * 4 / 9) + 1024, int(peak * 0.7 * 4 / 9) + 4096)
+    # batched: one lane reserved per scheme, the rest shared
     server = K.Server(0, lanes=1 if one_per_launch else lanes, hbm_budget=hbm_budget, key_slots=key_slots,
-                      ct_slots=ct_slots, max_batch=(1, 1) if one_per_launch else (0, 0))
+                      ct_slots=ct_slots, max_batch=(1, 1) if one_per_launch else (0, 0),
+                      reserve_lanes=(0, 0) if one_per_launch or lanes < 3 else (1, 1))
     try:
         pools = _populate(server, rng, {K.FRODO976: 64, K.SABER: 256})
         rows = []

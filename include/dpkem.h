@@ -166,6 +166,8 @@ typedef struct {
   size_t key_slots[2];      /* per scheme (index = DPKEM_FRODO976 / DPKEM_SABER) */
   size_t ct_slots[2];
   size_t max_batch[2];      /* per-batch cap (0 = as large as the budget allows); 1 = one request per launch */
+  int reserve_lanes[2];     /* lanes only this scheme may use, so that a burst of long Frodo batches cannot
+                               hold every lane while Saber requests wait (sum <= lanes; 0, 0 = all shared) */
 } dpkem_server_config;
 double dpkem_now_us(void);
 int dpkem_server_create(dpkem_server **out, const dpkem_server_config *cfg);

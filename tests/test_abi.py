@@ -53,8 +53,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(dpkem_request), offsetof(dpkem_request, scheme),
          offsetof(dpkem_request, key), offsetof(dpkem_request, t_arrival_us), offsetof(dpkem_request, coins),
          sizeof(dpkem_completion), offsetof(dpkem_completion, ss));
-  printf("%zu %zu %zu\n", offsetof(dpkem_completion, t_submit_us), offsetof(dpkem_completion, key),
-         sizeof(dpkem_server_config));
+  printf("%zu %zu %zu %zu\n", offsetof(dpkem_completion, t_submit_us), offsetof(dpkem_completion, key),
+         sizeof(dpkem_server_config), offsetof(dpkem_server_config, reserve_lanes));
   return 0;
 }
 ''')
@@ -69,3 +69,4 @@ int main(void) {
     import ctypes
     from dpcrypto_amd.kem import _ServerConfig
     assert b[2] == ctypes.sizeof(_ServerConfig)
+    assert b[3] == _ServerConfig.reserve_lanes.offset
