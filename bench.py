@@ -64,6 +64,34 @@ KERNEL_WORK = {
 }
 
 
+# HBM bytes (read, write) per launch at the profiled batch sizes (Saber 4096,
+# Frodo 8192 instances), from two rocprofv3 --pmc passes over
+#   bench.py --schedule 1x1 --calibrate-copy --skip-plain --skip-mixed --skip-serving --cpu-seconds 0 --frodo-steps 2
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
+# counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
+# FETCH_SIZE x 1.99976, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+MEASURED_HBM_BYTES = {
+    ("frodo_as_fused", 8192): (129320458, 256245771),
+    ("frodo_dec_mu", 8192): (301635258, 262144),
+    ("frodo_encdec_pre<false>", 8192): (129741165, 128713013),
+    ("frodo_encdec_pre<true>", 8192): (1794857, 128712704),
+    ("frodo_keygen_post", 8192): (129114019, 262144),
+    ("frodo_keygen_pre", 8192): (1015004, 257055461),
+    ("frodo_sa_fused<false>", 8192): (260598655, 128974848),
+    ("frodo_sa_fused<true>", 8192): (389558666, 262144),
+    ("frodo_ss", 8192): (130182606, 196608),
+    ("sab_dec_hash1", 4096): (893845, 18872735),
+    ("sab_dec_inner", 4096): (9887613, 131072),
+    ("sab_enc_hash1", 4096): (4571636, 19043161),
+    ("sab_hash2", 4096): (5159319, 131072),
+    ("sab_keygen_hash1", 4096): (628004, 18992698),
+    ("sab_keygen_hash2", 4096): (4201993, 131072),
+    ("sab_poly<kSabKeygen>", 4096): (19083120, 12976128),
+    ("sab_poly<kSabEnc>", 4096): (23936207, 4456448),
+    ("sab_poly<kSabReenc>", 4096): (29050475, 131072),
+}
+
+
 def lane_ops(kernel: str, n: int) -> float:
     perms, macs = KERNEL_WORK.get(kernel, (0, 0))
     return n * (perms * KECCAK_LANE_OPS + macs / 2.0)
@@ -186,6 +214,7 @@ def roofline_from(stats, n):
     avg_s = total_ms / count / 1e3
     work = lane_ops(name, n)
     achieved = work / avg_s
+    hbm = MEASURED_HBM_BYTES.get((name, n))
     return {
         "kernel": name,
         "bound": "valu",
@@ -193,7 +222,9 @@ def roofline_from(stats, n):
         "peak": round(VALU_PEAK / 1e12, 3),
         "unit": "Tlane-op/s",
         "frac": round(achieved / VALU_PEAK, 4),
-        "traffic": None,
+        "traffic": None if hbm is None else hbm[0] + hbm[1],
+        "traffic_read_write": None if hbm is None else list(hbm),
+        "traffic_GBps": None if hbm is None else round((hbm[0] + hbm[1]) / avg_s / 1e9, 1),
         "avg_launch_ms": round(avg_s * 1e3, 4),
         "algorithmic_lane_ops_per_launch": work,
         "share_of_step": round(total_ms / sum(v[1] for v in stats.values()), 3),

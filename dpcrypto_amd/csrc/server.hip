@@ -89,6 +89,7 @@ struct Lane {
   size_t host_bytes = 0;
   bool busy = false;
   int scheme = 0, op = 0;
+  double t_admit = 0;
   std::vector<Pending> batch;
   std::vector<uint32_t> new_handles;
   uint8_t* host_ss = nullptr;
@@ -237,6 +238,7 @@ void worker_loop(dpkem_server* s) {
         c.scheme = L.scheme;
         c.op = L.op;
         c.t_submit_us = L.batch[i].r.t_arrival_us > 0 ? L.batch[i].r.t_arrival_us : L.batch[i].t_submit;
+        c.t_admit_us = L.t_admit;
         c.t_done_us = t;
         if (L.op == kKeygen) c.key = L.new_handles[i];
         if (L.op == kEnc) c.ct = L.new_handles[i];
@@ -275,8 +277,8 @@ void worker_loop(dpkem_server* s) {
           c.status = (op == kDec || (op == kEnc && !s->keys[sc].valid(P.r.key))) ? DPKEM_ENOTFOUND : DPKEM_EBUDGET;
           c.scheme = sc;
           c.op = op;
-          c.t_submit_us = P.t_submit;
-          c.t_done_us = now_us();
+          c.t_submit_us = P.r.t_arrival_us > 0 ? P.r.t_arrival_us : P.t_submit;
+          c.t_done_us = c.t_admit_us = now_us();
           s->done.push_back(c);
           s->inflight_requests--;
           q.pop_front();
@@ -288,6 +290,7 @@ void worker_loop(dpkem_server* s) {
       }
       if (L.batch.empty()) continue;
       L.busy = true;
+      L.t_admit = now_us();
       s->batches++;
       s->requests += L.batch.size();
       s->max_seen_batch = std::max<uint64_t>(s->max_seen_batch, L.batch.size());
@@ -456,7 +459,7 @@ int dpkem_server_cancel(dpkem_server* s, size_t* cancelled) {
           c.scheme = a;
           c.op = b;
           c.t_submit_us = P.r.t_arrival_us > 0 ? P.r.t_arrival_us : P.t_submit;
-          c.t_done_us = t;
+          c.t_done_us = c.t_admit_us = t;
           s->done.push_back(c);
         }
         k += s->pending[a][b].size();

@@ -299,7 +299,8 @@ class Plan:
 REQ_DTYPE = np.dtype([("id", "<u8"), ("scheme", "<i4"), ("op", "<i4"), ("key", "<u4"), ("ct", "<u4"),
                       ("t_arrival_us", "<f8"), ("coins", "u1", (96,))])
 COMP_DTYPE = np.dtype([("id", "<u8"), ("status", "<i4"), ("scheme", "<i4"), ("op", "<i4"), ("key", "<u4"),
-                       ("ct", "<u4"), ("reserved", "<u4"), ("t_submit_us", "<f8"), ("t_done_us", "<f8"),
+                       ("ct", "<u4"), ("reserved", "<u4"), ("t_submit_us", "<f8"), ("t_admit_us", "<f8"),
+                       ("t_done_us", "<f8"),
                        ("ss", "u1", (32,))])
 
 

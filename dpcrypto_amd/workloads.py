@@ -286,10 +286,15 @@ def summarize(comp, t0, n_offered, rate):
         "p50_ms": _pct(lat_ms, 0.5),
         "p99_ms": _pct(lat_ms, 0.99),
     }
+    wait_ms = (ok["t_admit_us"] - ok["t_submit_us"]) / 1e3
+    service_ms = (ok["t_done_us"] - ok["t_admit_us"]) / 1e3
+    out["queue_p50_ms"], out["service_p50_ms"] = _pct(wait_ms, 0.5), _pct(service_ms, 0.5)
     for sc, name in ((K.SABER, "saber"), (K.FRODO976, "frodo976")):
         m = ok["scheme"] == sc
         out[f"{name}_p50_ms"] = _pct(lat_ms[m], 0.5)
         out[f"{name}_p99_ms"] = _pct(lat_ms[m], 0.99)
+        out[f"{name}_queue_p50_ms"] = _pct(wait_ms[m], 0.5)
+        out[f"{name}_service_p50_ms"] = _pct(service_ms[m], 0.5)
     for k, v in list(out.items()):
         if isinstance(v, float) and k.endswith("_ms"):
             out[k] = round(v, 3)
@@ -306,7 +311,7 @@ def release_created(server, comp):
 
 def run_serving(rates=(10_000, 50_000, 100_000, 200_000, 500_000), duration_s: float = 0.5, one_per_launch=False,
                 seed: int = 0x44504B454D000005, hbm_budget: int = 32 << 30, lanes: int = 4, drain_cap_s: float = 2.0,
-                progress=None):
+                progress=None, saber_frac: float = 0.7, reserve_lanes=None):
     rng = np.random.default_rng(seed)
     peak = max(rates) * duration_s * 1.3 + 1024
     key_slots = (int(peak * 0.3 / 9) + 1024, int(peak * 0.7 / 9) + 4096)
@@ -314,12 +319,13 @@ def run_serving(rates=(10_000, 50_000, 100_000, 200_000, 500_000), duration_s: f
     # batched: one lane reserved per scheme, the rest shared
     server = K.Server(0, lanes=1 if one_per_launch else lanes, hbm_budget=hbm_budget, key_slots=key_slots,
                       ct_slots=ct_slots, max_batch=(1, 1) if one_per_launch else (0, 0),
-                      reserve_lanes=(0, 0) if one_per_launch or lanes < 3 else (1, 1))
+                      reserve_lanes=reserve_lanes if reserve_lanes is not None
+                      else (0, 0) if one_per_launch or lanes < 3 else (1, 1))
     try:
         pools = _populate(server, rng, {K.FRODO976: 64, K.SABER: 256})
         rows = []
         for rate in rates:
-            reqs, t = _make_stream(rng, rate, duration_s, pools)
+            reqs, t = _make_stream(rng, rate, duration_s, pools, saber_frac)
             if progress:
                 progress(f"serving{' (one request per launch)' if one_per_launch else ''}: "
                          f"{rate} req/s, {len(reqs)} requests")

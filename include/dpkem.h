@@ -155,6 +155,7 @@ typedef struct {
   uint32_t ct;          /* ENC: new ciphertext handle */
   uint32_t reserved;
   double t_submit_us;   /* arrival (or submit) time */
+  double t_admit_us;    /* batch launched on a lane (queue wait = t_admit_us - t_submit_us) */
   double t_done_us;     /* completion observed by the dispatcher */
   uint8_t ss[32];       /* ENC, DEC: shared secret (24 bytes for Frodo) */
 } dpkem_completion;

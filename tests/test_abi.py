@@ -53,8 +53,9 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(dpkem_request), offsetof(dpkem_request, scheme),
          offsetof(dpkem_request, key), offsetof(dpkem_request, t_arrival_us), offsetof(dpkem_request, coins),
          sizeof(dpkem_completion), offsetof(dpkem_completion, ss));
-  printf("%zu %zu %zu %zu\n", offsetof(dpkem_completion, t_submit_us), offsetof(dpkem_completion, key),
-         sizeof(dpkem_server_config), offsetof(dpkem_server_config, reserve_lanes));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(dpkem_completion, t_submit_us), offsetof(dpkem_completion, key),
+         sizeof(dpkem_server_config), offsetof(dpkem_server_config, reserve_lanes),
+         offsetof(dpkem_completion, t_admit_us), offsetof(dpkem_completion, t_done_us));
   return 0;
 }
 ''')
@@ -70,3 +71,4 @@ int main(void) {
     from dpcrypto_amd.kem import _ServerConfig
     assert b[2] == ctypes.sizeof(_ServerConfig)
     assert b[3] == _ServerConfig.reserve_lanes.offset
+    assert b[4:6] == [C.fields["t_admit_us"][1], C.fields["t_done_us"][1]]
