@@ -390,10 +390,21 @@ def main():
         progress(f"mixed: {result['mixed']['batched_ops_per_s']:.0f} ops/s batched, "
                  f"{result['mixed']['plain_ops_per_s']:.0f} plain")
     if not args.skip_serving:
-        from dpcrypto_amd import workloads as W
-        rates = tuple(int(r) for r in args.serving_rates.split(","))
-        bat = W.run_serving(rates, args.serving_seconds, progress=progress)
-        one = W.run_serving(rates, args.serving_seconds, one_per_launch=True, progress=progress)
+        # each server runs in a fresh child process, as a serving process
+        # would: its lanes' streams then get hardware queues of their own
+        # instead of queues shared with the streams this process created
+        import subprocess
+
+        def serve(extra):
+            cmd = [sys.executable, "-u", "-m", "dpcrypto_amd.workloads", "serving", "--rates", args.serving_rates,
+                   "--seconds", str(args.serving_seconds)] + extra
+            out = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, timeout=300, check=True).stdout
+            return json.loads(out.decode().strip().splitlines()[-1])
+
+        progress("serving: continuous batching (child process)")
+        bat = serve([])
+        progress("serving: one request per launch (child process)")
+        one = serve(["--one-per-launch"])
         result["serving"] = {
             "workload": f"Poisson arrivals, {args.serving_seconds} s per rate, 70% Saber / 30% Frodo-976, "
                         f"keygen:enc:dec = 1:4:4, keys and ciphertexts resident in HBM, budget 32 GiB; "

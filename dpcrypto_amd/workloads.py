@@ -336,3 +336,29 @@ def run_serving(rates=(10_000, 50_000, 100_000, 200_000, 500_000), duration_s: f
     finally:
         server.close()
     return {"rows": rows, "server": st}
+
+
+def _main(argv=None):
+    """`python -m dpcrypto_amd.workloads serving [--rates ...]`: one serving
+    measurement in this (fresh) process, printed as one JSON line."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["serving"])
+    ap.add_argument("--rates", default="10000,50000,100000,200000,500000")
+    ap.add_argument("--seconds", type=float, default=0.5)
+    ap.add_argument("--one-per-launch", action="store_true")
+    ap.add_argument("--lanes", type=int, default=4)
+    a = ap.parse_args(argv)
+
+    def progress(msg):
+        print(f"[serving] {msg}", file=sys.stderr, flush=True)
+
+    r = run_serving(tuple(int(x) for x in a.rates.split(",")), a.seconds, one_per_launch=a.one_per_launch,
+                    lanes=a.lanes, progress=progress)
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    _main()
