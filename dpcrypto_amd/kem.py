@@ -47,6 +47,11 @@ class DpkemError(RuntimeError):
         self.code = code
 
 
+def _check_rc(rc, what):
+    if rc:
+        raise DpkemError(rc, what)
+
+
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     """Load libdpkem.so (raises if it has not been built)."""
     global _lib
@@ -87,6 +92,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_server_stats.argtypes = [vp, ctypes.c_char_p, sz]
         lib.dpkem_server_destroy.argtypes = [vp]
         lib.dpkem_engine_kernel_stats.argtypes = [vp, ctypes.c_char_p, sz, i]
+        lib.dpkem_aes256_ecb.argtypes = [vp, vp, vp, sz]
+        lib.dpkem_drbg_init.argtypes = [vp, vp, vp]
+        lib.dpkem_drbg_bytes.argtypes = [vp, vp, sz]
+        lib.dpkem_kat_seeds.argtypes = [sz, vp]
+        lib.dpkem_kat_coins.argtypes = [i, sz, vp, vp, vp]
         _lib = lib
     return _lib
 

@@ -192,6 +192,29 @@ int dpkem_server_destroy(dpkem_server *s);
 typedef void (*dpkem_randombytes_fn)(uint8_t *out, size_t len, void *ctx);
 int dpkem_set_randombytes(dpkem_randombytes_fn fn, void *ctx);
 
+/* NIST SP 800-90A AES-256 CTR_DRBG without derivation function (the
+ * generator behind the NIST PQC known-answer files) and KAT coin streams.
+ * No counterpart in the reference, which hard-codes its randomness
+ * (FrodoKEM976-GPU/src/cuda_kernel.cu:51, SABER-GPU/src/cuda_kernel.cu:62)
+ * and only declares AES (FrodoKEM976-GPU/include/aes.h:59-67). */
+typedef struct {
+  uint8_t key[32];
+  uint8_t v[16];
+  uint64_t reseed_counter;
+} dpkem_drbg;
+int dpkem_aes256_ecb(const uint8_t key[32], const uint8_t *in, uint8_t *out, size_t nblocks);
+int dpkem_drbg_init(dpkem_drbg *d, const uint8_t entropy[48], const uint8_t *personalization /* 48 B or NULL */);
+int dpkem_drbg_bytes(dpkem_drbg *d, uint8_t *out, size_t len);
+/* a dpkem_randombytes_fn over ctx = dpkem_drbg* (for dpkem_set_randombytes) */
+void dpkem_randombytes_drbg(uint8_t *out, size_t len, void *ctx);
+/* The 48-byte per-count seeds of a KAT file: the DRBG seeded with bytes
+ * 0, 1, ..., 47, then one 48-byte draw per count. */
+int dpkem_kat_seeds(size_t n, uint8_t *seeds);
+/* Keygen and encaps coins of n KAT counts: per count, the DRBG seeded with
+ * that count's seed, drawn with the scheme's randombytes call pattern
+ * (Frodo-976: 64 B, then 24 B; Saber: 3 x 32 B, then 32 B). */
+int dpkem_kat_coins(int scheme, size_t n, const uint8_t *seeds, uint8_t *kcoins, uint8_t *ecoins);
+
 /* crypto_kem_{keypair,enc,dec} over arrays of n instances (host memory),
  * on a process-wide default engine bound to device 0.
  * Replace PQCLEAN_FRODOKEM976*_crypto_kem_* (FrodoKEM976-GPU/include/api.h:14-18). */
