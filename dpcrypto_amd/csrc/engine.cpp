@@ -139,8 +139,37 @@ size_t chunk_scratch_bytes(int scheme, int form, size_t n) { return n * scratch_
 
 namespace {
 
+// Encaps / decaps of a chunk whose key input (ENC: pk, DEC: sk) is one row
+// shared by all instances.  Frodo: groups of kFrodoGroup instances expand A
+// once.  Saber: the key row is broadcast into `bcast` and the ordinary
+// kernels run (its matrix is a small share of the work).
+void launch_chunk_shared(hipStream_t st, int scheme, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+                         const uint8_t* i1, uint8_t* scratch, uint8_t* bcast) {
+  if (scheme == DPKEM_SABER) {
+    if (op == kEnc) {
+      broadcast_rows_launch(st, n, i0, kSizes[scheme].pk, bcast);
+      launch_chunk_on(st, DPKEM_FORM_BATCHED, scheme, op, n, o0, o1, bcast, i1, scratch);
+    } else {
+      broadcast_rows_launch(st, n, i1, kSizes[scheme].sk, bcast);
+      launch_chunk_on(st, DPKEM_FORM_BATCHED, scheme, op, n, o0, o1, i0, bcast, scratch);
+    }
+    return;
+  }
+  Carver c{scratch};
+  uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
+  uint32_t* e8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
+  uint32_t* epp8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 64));
+  uint8_t* kbuf = c.take((size_t)n * 64);
+  uint8_t* mubuf = c.take((size_t)n * 32);
+  uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
+  if (op == kEnc) frodo_enc_grouped_launch(st, n, i0, 0, i1, o0, o1, s8, e8, epp8, kbuf, nullptr, 0);
+  else frodo_dec_grouped_launch(st, n, i0, i1, 0, o0, s8, e8, epp8, kbuf, mubuf, fail, nullptr, 0);
+}
+
+// shared_in: -1, or the index of the input that is one row for all n
+// instances (ENC: 0 = pk, DEC: 1 = sk).
 int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
-           const uint8_t* i1, int mem) {
+           const uint8_t* i1, int mem, int shared_in = -1) {
   if (!e || (scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER)) return DPKEM_EINVAL;
   if (mem != DPKEM_MEM_HOST && mem != DPKEM_MEM_DEVICE) return DPKEM_EINVAL;
   if (n == 0) return DPKEM_OK;
@@ -156,9 +185,18 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
       return DPKEM_EINVAL;
     }
   }
+  if (shared_in >= 0 && e->form != DPKEM_FORM_BATCHED) {
+    snprintf(e->last_err, sizeof e->last_err, "shared-key ops run in the batched form only");
+    return DPKEM_EINVAL;
+  }
   std::lock_guard<std::mutex> lock(e->mu);
   HIP_TRY(e, hipSetDevice(e->device));
-  const size_t io_per = mem == DPKEM_MEM_HOST ? align_up(out[0]) + align_up(out[1]) + align_up(in[0]) + align_up(in[1]) : 0;
+  size_t io_per = 0;
+  for (int k = 0; k < 2; ++k) {
+    if (mem == DPKEM_MEM_HOST) io_per += align_up(out[k]) + (k == shared_in ? 0 : align_up(in[k]));
+  }
+  // Saber broadcasts the shared key into a dense array
+  if (shared_in >= 0 && scheme == DPKEM_SABER) io_per += align_up(in[shared_in]);
   const size_t per = scratch_per_inst(scheme, e->form) + io_per;
   size_t chunk = std::min(n, (size_t)INT32_MAX);
   while (chunk > 1 && chunk * per + 24 * kAlign > e->budget) chunk = (chunk + 1) / 2;
@@ -171,18 +209,26 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
     const uint8_t* din[2];
     Carver io{e->arena + align_up(chunk_scratch_bytes(scheme, e->form, chunk))};
     for (int k = 0; k < 2; ++k) {
+      const bool one = k == shared_in;
       if (mem == DPKEM_MEM_DEVICE) {
         dout[k] = out[k] ? outs[k] + done * out[k] : nullptr;
-        din[k] = in[k] ? ins[k] + done * in[k] : nullptr;
+        din[k] = in[k] ? ins[k] + (one ? 0 : done * in[k]) : nullptr;
       } else {
         dout[k] = out[k] ? io.take((size_t)c * out[k]) : nullptr;
-        uint8_t* d = in[k] ? io.take((size_t)c * in[k]) : nullptr;
-        if (d) HIP_TRY(e, hipMemcpyAsync(d, ins[k] + done * in[k], (size_t)c * in[k], hipMemcpyHostToDevice, e->stream));
+        const size_t rows = one ? 1 : (size_t)c;
+        uint8_t* d = in[k] ? io.take(rows * in[k]) : nullptr;
+        if (d) HIP_TRY(e, hipMemcpyAsync(d, ins[k] + (one ? 0 : done * in[k]), rows * in[k], hipMemcpyHostToDevice,
+                                         e->stream));
         din[k] = d;
       }
     }
     tls_profiler = e->profiling ? &e->prof : nullptr;
-    launch_chunk_on(e->stream, e->form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
+    if (shared_in >= 0) {
+      uint8_t* bcast = scheme == DPKEM_SABER ? io.take((size_t)c * in[shared_in]) : nullptr;
+      launch_chunk_shared(e->stream, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, bcast);
+    } else {
+      launch_chunk_on(e->stream, e->form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
+    }
     tls_profiler = nullptr;
     HIP_TRY(e, hipGetLastError());
     if (mem == DPKEM_MEM_HOST) {
@@ -379,6 +425,16 @@ int dpkem_enc(dpkem_engine* e, int scheme, size_t n, uint8_t* ct, uint8_t* ss, c
 
 int dpkem_dec(dpkem_engine* e, int scheme, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, int mem) {
   return run_op(e, scheme, kDec, n, ss, nullptr, ct, sk, mem);
+}
+
+int dpkem_enc_shared_key(dpkem_engine* e, int scheme, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk,
+                         const uint8_t* coins, int mem) {
+  return run_op(e, scheme, kEnc, n, ct, ss, pk, coins, mem, 0);
+}
+
+int dpkem_dec_shared_key(dpkem_engine* e, int scheme, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,
+                         int mem) {
+  return run_op(e, scheme, kDec, n, ss, nullptr, ct, sk, mem, 1);
 }
 
 int dpkem_set_randombytes(dpkem_randombytes_fn fn, void* ctx) {

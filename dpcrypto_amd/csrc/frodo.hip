@@ -32,6 +32,7 @@
 #include "sponge.hpp"
 #include "keccak2.hpp"
 #include "prof.hpp"
+#include "launch.hpp"
 
 namespace dpkem {
 
@@ -196,8 +197,8 @@ __global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __r
 
 // ss = SHAKE256(ct || kbar, 24); for decaps kbar = fail ? s : k'.
 __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ kbuf,
-                                               const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
-                                               uint8_t* __restrict__ ss) {
+                                               const uint8_t* __restrict__ sk, size_t sk_stride,
+                                               const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
   FR_PAIR_SETUP
   const uint8_t* c = ct + (size_t)inst * kFrodoCt;
   constexpr int RW = kRateShake256;
@@ -218,7 +219,7 @@ __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict_
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const uint64_t k = ld64(kbuf + (size_t)inst * 64 + 8 * i);
-    const uint64_t s = sk ? ld64(sk + (size_t)inst * kFrodoSk + 8 * i) : 0;
+    const uint64_t s = sk ? ld64(sk + (size_t)inst * sk_stride + 8 * i) : 0;
     a[REM + i] ^= k2::deint((k & ~mask) | (s & mask), p);
   }
   a[RW - 1] ^= k2::half_const(kDsShake ^ kPadEnd, p);
@@ -228,14 +229,14 @@ __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict_
 
 // decaps: W = C - B' S, mu' = Decode(W).  256 threads per instance.
 __global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
-                                                    uint8_t* __restrict__ mubuf) {
+                                                    size_t sk_stride, uint8_t* __restrict__ mubuf) {
   __shared__ uint32_t W[64];
   const int inst = blockIdx.x, tid = threadIdx.x;
   if (inst >= n) return;
   if (tid < 64) W[tid] = 0;
   __syncthreads();
   const uint8_t* c = ct + (size_t)inst * kFrodoCt;
-  const uint8_t* sT = sk + (size_t)inst * kFrodoSk + kFSkS;
+  const uint8_t* sT = sk + (size_t)inst * sk_stride + kFSkS;
   const int o = tid & 63, i = o >> 3, j = o & 7, slice = tid >> 6;  // 4 slices of 244
   uint32_t acc = 0;
   for (int k = slice * 244; k < slice * 244 + 244; ++k) {
@@ -518,6 +519,156 @@ __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __r
   }
 }
 
+
+// enc / dec over groups of up to G instances that share one public key (same
+// seedA, same B): the group's A is generated once and multiplied by the S'
+// of every member.  Two members fill the 16 rows of one product MFMA
+// (2 x nbar = 16), so G = 4 costs two product MFMAs per limb beside the four
+// shared transposes.  A 4-member group needs 158.9 KB of LDS (one workgroup
+// per CU, as the 125-VGPR budget allows anyway); groups of 1 to 3 leave
+// slots empty.  Group b holds instances first .. first+cnt-1:
+// groups[b] = (first, cnt), or, with groups == nullptr, (G b, min(G, n - G b)).
+// Members are independent KEM instances; only A, B and the pk bytes are common.
+template <bool DEC, int G>
+__global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint2* __restrict__ groups, int n,
+                                                         const uint8_t* __restrict__ pkb, size_t pk_stride,
+                                                         const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8,
+                                                         const int8_t* __restrict__ epp8, const uint8_t* __restrict__ mu,
+                                                         size_t mu_stride, uint8_t* __restrict__ ct_out,
+                                                         const uint8_t* __restrict__ ct_in, uint32_t* __restrict__ fail) {
+  static_assert(G % 2 == 0, "two members per product MFMA");
+  __shared__ __align__(16) uint32_t Bacc[G * kFMat];
+  __shared__ __align__(16) int8_t Sp[G * 8 * 1024];  // member j, row k at (8 j + k) * 1024; zero past 976
+  __shared__ int rowsum[G * 8];
+  __shared__ uint32_t V[G * 64];
+  __shared__ uint32_t diff[G];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  if (b >= ngroups) return;
+  const int first = groups ? (int)groups[b].x : G * b;
+  const int cnt = groups ? (int)groups[b].y : min(G, n - G * b);
+  uint32_t* Sd = reinterpret_cast<uint32_t*>(Sp);
+  for (int x = tid; x < G * 8 * 256; x += 1024) {
+    const int j = x >> 11, k = (x >> 8) & 7, i4 = x & 255;
+    Sd[x] = (j < cnt && i4 < kFN / 4) ? s8[(size_t)(first + j) * (kFMat / 4) + k * (kFN / 4) + i4] : 0u;
+  }
+  for (int x = tid; x < G * kFMat; x += 1024) Bacc[x] = 0;
+  if (tid < G * 8) rowsum[tid] = 0;
+  if (tid < G * 64) V[tid] = 0;
+  if (tid < G) diff[tid] = 0;
+  __syncthreads();
+  for (int x = tid; x < G * 8 * 256; x += 1024) {
+    const uint32_t v = Sd[x];
+    const int sum = (int8_t)v + (int8_t)(v >> 8) + (int8_t)(v >> 16) + (int8_t)(v >> 24);
+    atomicAdd(&rowsum[x >> 8], sum);
+  }
+
+  const uint8_t* pk = pkb + (size_t)first * pk_stride;
+  const uint64_t s0 = ld64(pk), s1 = ld64(pk + 8);
+  const int g = lane >> 4, nn = lane & 15;
+  // product p, A operand: lane (g, nn) holds S' row (nn & 7) of member 2p + nn / 8
+  // at wave rows 16 q + 4 g + i (dword q, byte i)
+  v4i sop[G / 2];
+#pragma unroll
+  for (int pp = 0; pp < G / 2; ++pp)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+      sop[pp][qq] = (int)Sd[((2 * pp + (nn >> 3)) * 8 + (nn & 7)) * 256 + (wave * 64 + 16 * qq + 4 * g) / 4];
+  const int onehot = 1 << (8 * (nn & 3));
+  v4i sel[4];
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+    for (int d = 0; d < 4; ++d) sel[qq][d] = (g == qq && d == (nn >> 2)) ? onehot : 0;
+  const v4i zero = {0, 0, 0, 0};
+
+  int u = 0;
+  gen_row(wave * 64 + lane, s0, s1, [&](const uint64_t w[4]) {
+    if (u < kFUnits) {
+      v4i lm[2];
+      unit_limbs(w, lm[0], lm[1]);
+      v4i packed[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const v4i t = __builtin_amdgcn_mfma_i32_16x16x64_i8(lm[h], sel[qq], zero, 0, 0, 0);
+          packed[h][qq] = (int)(((uint32_t)t[0] & 0xFF) | (((uint32_t)t[1] & 0xFF) << 8) |
+                                (((uint32_t)t[2] & 0xFF) << 16) | ((uint32_t)t[3] << 24));
+        }
+      const int col = 16 * u + nn;
+#pragma unroll
+      for (int pp = 0; pp < G / 2; ++pp) {
+        const v4i p0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(sop[pp], packed[0], zero, 0, 0, 0);
+        const v4i p1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(sop[pp], packed[1], zero, 0, 0, 0);
+        // D row 4 g + i = member 2 pp + g / 2, S' row 4 (g & 1) + i
+        const int j = 2 * pp + (g >> 1);
+        if (j < cnt && col < kFN) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            atomicAdd(&Bacc[j * kFMat + (4 * (g & 1) + i) * kFN + col], (uint32_t)p0[i] + 256u * (uint32_t)p1[i]);
+        }
+      }
+    }
+    ++u;
+  });
+  __syncthreads();
+
+  // B' = acc + 128 rowsum + E' per member
+  uint32_t dj[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) dj[j] = 0;
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    if (j >= cnt) break;
+    const size_t inst = (size_t)(first + j);
+    const int8_t* ep = e8 + inst * kFMat;
+    for (int x = tid; x < kFMat; x += 1024) {
+      const uint32_t v = (Bacc[j * kFMat + x] + 128u * (uint32_t)rowsum[j * 8 + x / kFN] + (uint32_t)(int32_t)ep[x]) & 0xFFFF;
+      const uint16_t be = (uint16_t)((v << 8) | (v >> 8));
+      if (DEC) dj[j] |= (uint32_t)(*reinterpret_cast<const uint16_t*>(ct_in + inst * kFrodoCt + 2 * x) ^ be);
+      else *reinterpret_cast<uint16_t*>(ct_out + inst * kFrodoCt + 2 * x) = be;
+    }
+  }
+  // V = S' B for every member (B read once per k)
+  {
+    const int o = tid & 63, i = o >> 3, jj = o & 7, slice = tid >> 6;
+    uint32_t acc[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) acc[j] = 0;
+    for (int k = slice * 61; k < slice * 61 + 61; ++k) {
+      const uint8_t* bp = pk + kFrodoSeedA + 2 * (k * kFNB + jj);
+      const uint32_t bv = ((uint32_t)bp[0] << 8) | bp[1];
+#pragma unroll
+      for (int j = 0; j < G; ++j) acc[j] += (uint32_t)(int32_t)Sp[(j * 8 + i) * 1024 + k] * bv;
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < cnt) atomicAdd(&V[j * 64 + o], acc[j]);
+  }
+  __syncthreads();
+  if (tid < 64) {
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (j >= cnt) break;
+      const size_t inst = (size_t)(first + j);
+      const uint8_t* m = mu + inst * mu_stride + 3 * (tid >> 3);
+      const uint32_t t24 = m[0] | ((uint32_t)m[1] << 8) | ((uint32_t)m[2] << 16);
+      const uint32_t enc = ((t24 >> (3 * (tid & 7))) & 7u) << 13;
+      const uint32_t cv = (V[j * 64 + tid] + (uint32_t)(int32_t)epp8[inst * 64 + tid] + enc) & 0xFFFF;
+      const uint16_t be = (uint16_t)((cv << 8) | (cv >> 8));
+      if (DEC) dj[j] |= (uint32_t)(*reinterpret_cast<const uint16_t*>(ct_in + inst * kFrodoCt + kFCtC1 + 2 * tid) ^ be);
+      else *reinterpret_cast<uint16_t*>(ct_out + inst * kFrodoCt + kFCtC1 + 2 * tid) = be;
+    }
+  }
+  if (DEC) {
+#pragma unroll
+    for (int j = 0; j < G; ++j) atomicOr(&diff[j], dj[j] != 0 ? 1u : 0u);
+    __syncthreads();
+    if (tid < cnt) fail[first + tid] = diff[tid];
+  }
+}
+
 // ----------------------------------------------------------------- host launchers
 
 static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
@@ -538,19 +689,49 @@ void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* c
   DPKEM_LAUNCH("frodo_sa_fused<false>", frodo_sa_fused<false>, dim3(n), dim3(1024), st, n, pk, (size_t)kFrodoPk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), coins,
                      (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, nullptr, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, (size_t)0, nullptr, ss);
 }
 
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
                       uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, mubuf);
+  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, (size_t)kFrodoSk, mubuf);
   DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk, (size_t)kFrodoSk, mubuf,
                      (size_t)32, kbuf, s8, e8, epp8);
   DPKEM_LAUNCH("frodo_sa_fused<true>", frodo_sa_fused<true>, dim3(n), dim3(1024), st, n, sk + kFrodoSec, (size_t)kFrodoSk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32,
                      nullptr, ct, fail);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, fail, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, (size_t)kFrodoSk, fail, ss);
+}
+
+
+// Encaps / decaps where instances share public keys in groups (see
+// frodo_sa_grouped).  pk_stride / sk_stride = 0: one key for all n.
+void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t pk_stride, const uint8_t* coins,
+                              uint8_t* ct, uint8_t* ss, uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf,
+                              const uint2* groups, int ngroups) {
+  if (n <= 0) return;
+  if (!groups) ngroups = blocks_for(n, kFrodoGroup);
+  DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk,
+               pk_stride, coins, (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8);
+  DPKEM_LAUNCH("frodo_sa_grouped<false>", (frodo_sa_grouped<false, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
+               ngroups, groups, n, pk, pk_stride, s8, reinterpret_cast<const int8_t*>(e8),
+               reinterpret_cast<const int8_t*>(epp8), coins, (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, (size_t)0, nullptr, ss);
+}
+
+void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, size_t sk_stride, uint8_t* ss,
+                              uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail,
+                              const uint2* groups, int ngroups) {
+  if (n <= 0) return;
+  if (!groups) ngroups = blocks_for(n, kFrodoGroup);
+  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, sk_stride, mubuf);
+  DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk,
+               sk_stride, mubuf, (size_t)32, kbuf, s8, e8, epp8);
+  DPKEM_LAUNCH("frodo_sa_grouped<true>", (frodo_sa_grouped<true, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
+               ngroups, groups, n, sk + kFrodoSec, sk_stride, s8, reinterpret_cast<const int8_t*>(e8),
+               reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, sk_stride, fail, ss);
 }
 
 }  // namespace dpkem

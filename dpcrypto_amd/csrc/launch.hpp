@@ -14,6 +14,8 @@ void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* c
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
                       uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail);
 
+// dst[i] = src for n rows of `bytes` bytes (a multiple of 4).
+void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t bytes, uint8_t* dst);
 // Chained Saber keygen -> enc -> dec of the same instances on one stream,
 // sharing keygen's expanded matrix (saber.hip); scratch:
 // kSaberChainScratchPerInst bytes per instance.
@@ -25,6 +27,16 @@ void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* p
                          uint32_t* e8);
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf);
+// Groups of up to kFrodoGroup instances sharing one public key expand A once;
+// groups[b] = (first, count) or nullptr for consecutive groups of
+// kFrodoGroup; key strides may be 0 (one key for all n).
+constexpr int kFrodoGroup = 4;
+void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t pk_stride, const uint8_t* coins,
+                              uint8_t* ct, uint8_t* ss, uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf,
+                              const uint2* groups, int ngroups);
+void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, size_t sk_stride, uint8_t* ss,
+                              uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail,
+                              const uint2* groups, int ngroups);
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
                       uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail);
 

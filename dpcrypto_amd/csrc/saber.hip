@@ -25,6 +25,8 @@
 #include "prof.hpp"
 #include "saber_poly.hpp"
 
+#include <algorithm>
+
 namespace dpkem {
 
 constexpr int kSabGenMatrixWords = kSaberGenMatrixBytes / 8;  // 468
@@ -435,6 +437,23 @@ void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
   SabPolyArgs a{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
   DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), dim3(128), st, a);
   DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail, ss);
+}
+
+// dst[i] = src for i < n (bytes per row a multiple of 4).
+__global__ void __launch_bounds__(256) broadcast_rows(int n, const uint32_t* __restrict__ src, int words,
+                                                      uint32_t* __restrict__ dst) {
+  const size_t total = (size_t)n * words;
+  for (size_t x = (size_t)blockIdx.x * 256 + threadIdx.x; x < total; x += (size_t)gridDim.x * 256)
+    dst[x] = src[x % words];
+}
+
+void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t bytes, uint8_t* dst) {
+  if (n <= 0) return;
+  const int words = (int)(bytes / 4);
+  const size_t total = (size_t)n * words;
+  const int blocks = (int)std::min<size_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(broadcast_rows, dim3(blocks), dim3(256), 0, st, n, reinterpret_cast<const uint32_t*>(src), words,
+                     reinterpret_cast<uint32_t*>(dst));
 }
 
 // Chained keygen -> encaps -> decaps of the same n instances on one stream.

@@ -92,6 +92,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_server_stats.argtypes = [vp, ctypes.c_char_p, sz]
         lib.dpkem_server_destroy.argtypes = [vp]
         lib.dpkem_engine_kernel_stats.argtypes = [vp, ctypes.c_char_p, sz, i]
+        lib.dpkem_enc_shared_key.argtypes = [vp, i, sz, vp, vp, vp, vp, i]
+        lib.dpkem_dec_shared_key.argtypes = [vp, i, sz, vp, vp, vp, i]
         lib.dpkem_aes256_ecb.argtypes = [vp, vp, vp, sz]
         lib.dpkem_drbg_init.argtypes = [vp, vp, vp]
         lib.dpkem_drbg_bytes.argtypes = [vp, vp, sz]
@@ -230,6 +232,33 @@ class Engine:
         ss = self._empty_like_kind(ct, n, s["ss"])
         mem = self._mem(ct, sk)
         self._check(self.lib.dpkem_dec(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), mem), "dec")
+        return ss
+
+    def enc_shared_key(self, scheme: int, pk, coins):
+        """n encapsulations to one public key (pk: one row)."""
+        s = SIZES[scheme]
+        pk = self._host(pk, s["pk"]); coins = self._host(coins, s["ecoins"])
+        if pk.shape[0] != 1:
+            raise ValueError("pk must be one row")
+        n = coins.shape[0]
+        ct = self._empty_like_kind(coins, n, s["ct"])
+        ss = self._empty_like_kind(coins, n, s["ss"])
+        mem = self._mem(pk, coins)
+        self._check(self.lib.dpkem_enc_shared_key(self.h, scheme, n, _ptr(ct), _ptr(ss), _ptr(pk), _ptr(coins), mem),
+                    "enc_shared_key")
+        return ct, ss
+
+    def dec_shared_key(self, scheme: int, ct, sk):
+        """n decapsulations with one secret key (sk: one row)."""
+        s = SIZES[scheme]
+        ct = self._host(ct, s["ct"]); sk = self._host(sk, s["sk"])
+        if sk.shape[0] != 1:
+            raise ValueError("sk must be one row")
+        n = ct.shape[0]
+        ss = self._empty_like_kind(ct, n, s["ss"])
+        mem = self._mem(ct, sk)
+        self._check(self.lib.dpkem_dec_shared_key(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), mem),
+                    "dec_shared_key")
         return ss
 
     def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False) -> "Plan":

@@ -96,6 +96,19 @@ int dpkem_enc(dpkem_engine *e, int scheme, size_t n, uint8_t *ct, uint8_t *ss, c
 int dpkem_dec(dpkem_engine *e, int scheme, size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk,
               int mem);
 
+/* n encapsulations to ONE public key (pk: one row) / n decapsulations with ONE
+ * secret key (sk: one row); other arrays as in dpkem_enc / dpkem_dec.
+ * Results equal dpkem_enc / dpkem_dec with the key repeated n times.  Frodo
+ * expands the public matrix A once per group of 4 instances instead of once
+ * per instance (A is ~97% of a Frodo op's Keccak work); Saber broadcasts the
+ * key and runs its ordinary kernels.  Batched form only.  No counterpart in
+ * the reference, whose drivers take one key per instance
+ * (FrodoKEM976-GPU/src/cuda_kernel.cu:108-212). */
+int dpkem_enc_shared_key(dpkem_engine *e, int scheme, size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk,
+                         const uint8_t *coins, int mem);
+int dpkem_dec_shared_key(dpkem_engine *e, int scheme, size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk,
+                         int mem);
+
 /* ---------------------------------------------------------------- plans (graph-captured task sets)
  * A plan records KEM tasks over device-resident arrays, optionally chained
  * instance-wise (task `after` produces what this task consumes, e.g.
