@@ -206,6 +206,34 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
                           "tuning_ms": {f"{a}x{b}": round(v, 4) for (a, b), v in tried.items()}})
 
 
+def run_shared_key(torch, eng, scheme, n, steps, seed):
+    """encaps + decaps of n instances against ONE key per step (serving
+    pattern: many requests for a server's long-term key)."""
+    import numpy as np
+    import dpcrypto_amd as D
+    s = D.SIZES[scheme]
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    eng.set_stream(stream.cuda_stream)
+    kc = torch.from_numpy(rng.integers(0, 256, (1, s["kcoins"]), dtype=np.uint8)).to(dev)
+    pk, sk = eng.keypair(scheme, kc)
+    ec = torch.from_numpy(rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)).to(dev)
+    ct, ss = eng.enc_shared_key(scheme, pk, ec)  # warm-up
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        ct, ss = eng.enc_shared_key(scheme, pk, ec)
+        ss2 = eng.dec_shared_key(scheme, ct, sk)
+    b.record(stream)
+    b.synchronize()
+    ms = a.elapsed_time(b) / steps
+    return {"workload": f"{'frodokem-976-shake' if scheme == 0 else 'saber-l3'}: {n} encaps + {n} decaps "
+                        f"against one key per step", "ops_per_s": round(2 * n / (ms / 1e3), 1),
+            "ms_per_step": round(ms, 3), "roundtrip_ok": bool(torch.equal(ss, ss2))}
+
+
 def roofline_from(stats, n):
     """Dominant kernel by total time -> achieved VALU lane-ops/s vs peak."""
     if not stats:
@@ -375,6 +403,10 @@ def main():
             plain["frodo976_speedup_batched_over_plain"] = round(fr["ops_per_s"] / fp["ops_per_s"], 2)
         pe.close()
         result["plain"] = plain
+    if not args.skip_frodo:
+        progress("frodo976 encaps + decaps against one key (A expanded once per 4 instances)")
+        result["frodo976_shared_key"] = run_shared_key(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps,
+                                                       0x44504B454D000006)
     progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
     shared = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                tune_budget_s=20.0, share_matrix=True, schedule=sched)
