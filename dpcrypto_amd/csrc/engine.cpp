@@ -198,10 +198,12 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
   // Saber broadcasts the shared key into a dense array
   if (shared_in >= 0 && scheme == DPKEM_SABER) io_per += align_up(in[shared_in]);
   const size_t per = scratch_per_inst(scheme, e->form) + io_per;
+  // a shared key staged from host memory is one row per chunk
+  const size_t fixed = 24 * kAlign + (shared_in >= 0 && mem == DPKEM_MEM_HOST ? align_up(in[shared_in]) : 0);
   size_t chunk = std::min(n, (size_t)INT32_MAX);
-  while (chunk > 1 && chunk * per + 24 * kAlign > e->budget) chunk = (chunk + 1) / 2;
-  if (chunk * per + 24 * kAlign > e->budget) return DPKEM_EBUDGET;
-  int rc = ensure_arena(e, chunk * per + 24 * kAlign);
+  while (chunk > 1 && chunk * per + fixed > e->budget) chunk = (chunk + 1) / 2;
+  if (chunk * per + fixed > e->budget) return DPKEM_EBUDGET;
+  int rc = ensure_arena(e, chunk * per + fixed);
   if (rc) return rc;
   for (size_t done = 0; done < n; done += chunk) {
     const int c = (int)std::min(chunk, n - done);
