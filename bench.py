@@ -435,14 +435,19 @@ def main():
 
         progress("serving: continuous batching (child process)")
         bat = serve([])
+        progress("serving: continuous batching without key grouping (child process)")
+        nogroup = serve(["--no-key-grouping"])
         progress("serving: one request per launch (child process)")
         one = serve(["--one-per-launch"])
         result["serving"] = {
             "workload": f"Poisson arrivals, {args.serving_seconds} s per rate, 70% Saber / 30% Frodo-976, "
-                        f"keygen:enc:dec = 1:4:4, keys and ciphertexts resident in HBM, budget 32 GiB; "
+                        f"keygen:enc:dec = 1:4:4, requests drawn over a store of 64 Frodo / 256 Saber keys "
+                        f"(Frodo batches grouped by key: A expanded once per 4 same-key members), keys and "
+                        f"ciphertexts resident in HBM, budget 32 GiB; "
                         f"latency = completion - scheduled arrival; requests still queued 2 s after the last "
                         f"arrival are cancelled",
             "batched": bat["rows"], "batched_server": bat["server"],
+            "batched_without_key_grouping": nogroup["rows"],
             "one_request_per_launch": one["rows"]}
     if args.cpu_seconds > 0:
         progress(f"cpu baseline ({args.cpu_seconds:.0f} s)")

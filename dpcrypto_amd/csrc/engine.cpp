@@ -131,6 +131,19 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
   }
 }
 
+void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+                          const uint8_t* i1, uint8_t* scratch, const uint2* groups, int ngroups) {
+  Carver c{scratch};
+  uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
+  uint32_t* e8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
+  uint32_t* epp8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 64));
+  uint8_t* kbuf = c.take((size_t)n * 64);
+  uint8_t* mubuf = c.take((size_t)n * 32);
+  uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
+  if (op == kEnc) frodo_enc_grouped_launch(st, n, i0, kSizes[DPKEM_FRODO976].pk, i1, o0, o1, s8, e8, epp8, kbuf, groups, ngroups);
+  else frodo_dec_grouped_launch(st, n, i0, i1, kSizes[DPKEM_FRODO976].sk, o0, s8, e8, epp8, kbuf, mubuf, fail, groups, ngroups);
+}
+
 // Upper bound on the arena bytes the chunk carver uses for n instances
 // (per-array alignment padding included).
 size_t chunk_scratch_bytes(int scheme, int form, size_t n) { return n * scratch_per_inst(scheme, form) + 16 * kAlign; }

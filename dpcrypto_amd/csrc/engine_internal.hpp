@@ -38,6 +38,11 @@ size_t chunk_scratch_bytes(int scheme, int form, size_t n);
 // I/O byte sizes per instance: outputs out[0..1], inputs in[0..1] (0 = unused).
 void io_sizes(int scheme, int op, size_t out[2], size_t in[2]);
 // Launch one chunk (all pointers on the device) on stream st using `scratch`.
+// Frodo encaps / decaps of a chunk whose instances come in same-key groups:
+// groups[b] = (first, count <= kFrodoGroup), instances of a group carry
+// identical keys (i0 = pk rows for ENC, i1 = sk rows for DEC).
+void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+                          const uint8_t* i1, uint8_t* scratch, const uint2* groups, int ngroups);
 void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
                      const uint8_t* i0, const uint8_t* i1, uint8_t* scratch);
 int record_hip_error(dpkem_engine* e, hipError_t err, const char* what);

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "engine_internal.hpp"
+#include "launch.hpp"
 
 using namespace dpkem;
 
@@ -102,6 +103,7 @@ struct Lane {
 struct dpkem_server {
   int device = 0;
   int form = DPKEM_FORM_BATCHED;
+  bool group_by_key = false;
   size_t budget = 0;
   size_t max_batch[2] = {0, 0};
   // device-resident store
@@ -123,6 +125,7 @@ struct dpkem_server {
   char last_err[256] = {0};
   // statistics
   uint64_t batches = 0, requests = 0, max_seen_batch = 0;
+  uint64_t key_groups = 0, grouped_requests = 0;  // Frodo members launched in same-key groups
 };
 
 namespace {
@@ -144,7 +147,7 @@ size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 size_t lane_bytes(int scheme, int form, size_t m) {
   const SchemeDims& d = kDims[scheme];
   return chunk_scratch_bytes(scheme, form, m) + a256(m * d.pk) + a256(m * d.sk) + a256(m * d.ct) + a256(m * d.ss) +
-         a256(m * 96) + 2 * a256(m * 4) + 4096;
+         a256(m * 96) + 2 * a256(m * 4) + a256(m * 8) + 4096;
 }
 
 // Launch one admitted batch on lane L (called with s->mu NOT held).
@@ -163,12 +166,14 @@ void launch_batch(dpkem_server* s, Lane& L) {
   uint8_t* d_coins = take((size_t)m * 96);
   uint32_t* d_idx0 = reinterpret_cast<uint32_t*>(take((size_t)m * 4));
   uint32_t* d_idx1 = reinterpret_cast<uint32_t*>(take((size_t)m * 4));
-  // pinned staging: coins | idx0 | idx1 | ss
+  uint2* d_groups = reinterpret_cast<uint2*>(take((size_t)m * 8));
+  // pinned staging: coins | idx0 | idx1 | groups | ss (the first four laid out as on the device)
   uint8_t* h = L.host;
   uint8_t* h_coins = h;
   uint32_t* h_idx0 = reinterpret_cast<uint32_t*>(h + a256((size_t)m * 96));
   uint32_t* h_idx1 = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(h_idx0) + a256((size_t)m * 4));
-  L.host_ss = reinterpret_cast<uint8_t*>(h_idx1) + a256((size_t)m * 4);
+  uint2* h_groups = reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(h_idx1) + a256((size_t)m * 4));
+  L.host_ss = reinterpret_cast<uint8_t*>(h_groups) + a256((size_t)m * 8);
   const size_t cw = op == kKeygen ? d.kc : d.ec;
   for (int i = 0; i < m; ++i) {
     const dpkem_request& r = L.batch[i].r;
@@ -176,9 +181,23 @@ void launch_batch(dpkem_server* s, Lane& L) {
     h_idx0[i] = op == kKeygen ? L.new_handles[i] : r.key;
     h_idx1[i] = op == kEnc ? L.new_handles[i] : r.ct;
   }
+  // same-key groups of up to kFrodoGroup consecutive members (the batch was
+  // ordered by key at admission)
+  int ngroups = 0;
+  const bool grouped = s->group_by_key && sc == DPKEM_FRODO976 && op != kKeygen;
+  if (grouped) {
+    for (int i = 0; i < m;) {
+      int j = i + 1;
+      while (j < m && j - i < kFrodoGroup && L.batch[j].r.key == L.batch[i].r.key) ++j;
+      h_groups[ngroups++] = make_uint2((uint32_t)i, (uint32_t)(j - i));
+      i = j;
+    }
+    s->key_groups += (uint64_t)ngroups;
+    s->grouped_requests += (uint64_t)m;
+  }
   hipStream_t st = L.stream;
-  const size_t stage = a256((size_t)m * 96) + 2 * a256((size_t)m * 4);
-  // coins and both index arrays are laid out identically on both sides: one copy
+  const size_t stage = a256((size_t)m * 96) + 2 * a256((size_t)m * 4) + (grouped ? a256((size_t)m * 8) : 0);
+  // coins, index arrays and groups are laid out identically on both sides: one copy
   (void)hipMemcpyAsync(d_coins, h_coins, stage, hipMemcpyHostToDevice, st);
   const dim3 g(m), b(256);
   if (op == kKeygen) {
@@ -187,13 +206,15 @@ void launch_batch(dpkem_server* s, Lane& L) {
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx0, d_sk, s->sk_slab[sc], d.sk, 0);
   } else if (op == kEnc) {
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx0, s->pk_slab[sc], d_pk, d.pk, 1);
-    launch_chunk_on(st, s->form, sc, op, m, d_ct, d_ss, d_pk, d_coins, base);
+    if (grouped) launch_frodo_grouped(st, op, m, d_ct, d_ss, d_pk, d_coins, base, d_groups, ngroups);
+    else launch_chunk_on(st, s->form, sc, op, m, d_ct, d_ss, d_pk, d_coins, base);
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx1, d_ct, s->ct_slab[sc], d.ct, 0);
     (void)hipMemcpyAsync(L.host_ss, d_ss, (size_t)m * d.ss, hipMemcpyDeviceToHost, st);
   } else {
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx0, s->sk_slab[sc], d_sk, d.sk, 1);
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx1, s->ct_slab[sc], d_ct, d.ct, 1);
-    launch_chunk_on(st, s->form, sc, op, m, d_ss, nullptr, d_ct, d_sk, base);
+    if (grouped) launch_frodo_grouped(st, op, m, d_ss, nullptr, d_ct, d_sk, base, d_groups, ngroups);
+    else launch_chunk_on(st, s->form, sc, op, m, d_ss, nullptr, d_ct, d_sk, base);
     (void)hipMemcpyAsync(L.host_ss, d_ss, (size_t)m * d.ss, hipMemcpyDeviceToHost, st);
   }
   (void)hipEventRecord(L.done, st);
@@ -289,6 +310,23 @@ void worker_loop(dpkem_server* s) {
         q.pop_front();
       }
       if (L.batch.empty()) continue;
+      if (s->group_by_key && sc == DPKEM_FRODO976 && op != kKeygen) {
+        // order members by key so that same-key requests sit next to each other
+        std::vector<size_t> order(L.batch.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b2) { return L.batch[a].r.key < L.batch[b2].r.key; });
+        std::vector<Pending> nb;
+        std::vector<uint32_t> nh;
+        nb.reserve(order.size());
+        nh.reserve(order.size());
+        for (size_t i : order) {
+          nb.push_back(L.batch[i]);
+          nh.push_back(L.new_handles[i]);
+        }
+        L.batch.swap(nb);
+        L.new_handles.swap(nh);
+      }
       L.busy = true;
       L.t_admit = now_us();
       s->batches++;
@@ -348,6 +386,7 @@ int dpkem_server_create(dpkem_server** out, const dpkem_server_config* cfg) {
   if (!s) return DPKEM_ENOMEM;
   s->device = cfg->device;
   s->form = cfg->form;
+  s->group_by_key = cfg->group_by_key != 0 && cfg->form == DPKEM_FORM_BATCHED;
   (void)hipSetDevice(s->device);
   size_t budget = cfg->hbm_budget_bytes;
   if (budget == 0) {
@@ -391,7 +430,7 @@ int dpkem_server_create(dpkem_server** out, const dpkem_server_config* cfg) {
   }
   const size_t lane_dev = std::max(lane_bytes(0, s->form, s->max_batch[0]), lane_bytes(1, s->form, s->max_batch[1]));
   const size_t lane_host =
-      a256(std::max(s->max_batch[0], s->max_batch[1]) * (96 + 8 + 32)) + 4096 + 3 * 256;
+      a256(std::max(s->max_batch[0], s->max_batch[1]) * (96 + 8 + 8 + 32)) + 4096 + 4 * 256;
   s->lanes.resize(cfg->lanes);
   for (int i = 0; i < cfg->lanes; ++i)
     s->lanes[i].only_scheme = i < cfg->reserve_lanes[0] ? 0 : i < cfg->reserve_lanes[0] + cfg->reserve_lanes[1] ? 1 : -1;
@@ -503,10 +542,11 @@ int dpkem_server_stats(dpkem_server* s, char* buf, size_t len) {
   std::lock_guard<std::mutex> lk(s->mu);
   const int w = snprintf(buf, len,
                          "{\"batches\": %llu, \"requests\": %llu, \"max_batch_seen\": %llu, \"lanes\": %zu, "
-                         "\"max_batch_frodo976\": %zu, \"max_batch_saber\": %zu, \"hbm_budget\": %zu}",
+                         "\"max_batch_frodo976\": %zu, \"max_batch_saber\": %zu, \"hbm_budget\": %zu, "
+                         "\"frodo_key_groups\": %llu, \"frodo_grouped_requests\": %llu}",
                          (unsigned long long)s->batches, (unsigned long long)s->requests,
                          (unsigned long long)s->max_seen_batch, s->lanes.size(), s->max_batch[0], s->max_batch[1],
-                         s->budget);
+                         s->budget, (unsigned long long)s->key_groups, (unsigned long long)s->grouped_requests);
   return (w > 0 && (size_t)w < len) ? DPKEM_OK : DPKEM_EINVAL;
 }
 

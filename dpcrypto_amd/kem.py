@@ -347,7 +347,7 @@ class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("form", ctypes.c_int), ("lanes", ctypes.c_int),
                 ("hbm_budget_bytes", ctypes.c_size_t), ("key_slots", ctypes.c_size_t * 2),
                 ("ct_slots", ctypes.c_size_t * 2), ("max_batch", ctypes.c_size_t * 2),
-                ("reserve_lanes", ctypes.c_int * 2)]
+                ("reserve_lanes", ctypes.c_int * 2), ("group_by_key", ctypes.c_int)]
 
 
 def now_us() -> float:
@@ -359,11 +359,11 @@ class Server:
     """Continuous-batching request loop over a device-resident key store (dpkem_server_*)."""
 
     def __init__(self, device=0, lanes=4, hbm_budget=0, key_slots=(1024, 4096), ct_slots=(4096, 16384),
-                 max_batch=(0, 0), form=FORM_BATCHED, reserve_lanes=(0, 0)):
+                 max_batch=(0, 0), form=FORM_BATCHED, reserve_lanes=(0, 0), group_by_key=True):
         self.lib = load_library()
         cfg = _ServerConfig(device, form, lanes, hbm_budget, (ctypes.c_size_t * 2)(*key_slots),
                             (ctypes.c_size_t * 2)(*ct_slots), (ctypes.c_size_t * 2)(*max_batch),
-                            (ctypes.c_int * 2)(*reserve_lanes))
+                            (ctypes.c_int * 2)(*reserve_lanes), int(bool(group_by_key)))
         h = ctypes.c_void_p()
         rc = self.lib.dpkem_server_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc:

@@ -311,7 +311,7 @@ def release_created(server, comp):
 
 def run_serving(rates=(10_000, 50_000, 100_000, 200_000, 500_000), duration_s: float = 0.5, one_per_launch=False,
                 seed: int = 0x44504B454D000005, hbm_budget: int = 32 << 30, lanes: int = 4, drain_cap_s: float = 2.0,
-                progress=None, saber_frac: float = 0.7, reserve_lanes=None):
+                progress=None, saber_frac: float = 0.7, reserve_lanes=None, group_by_key: bool = True):
     rng = np.random.default_rng(seed)
     peak = max(rates) * duration_s * 1.3 + 1024
     key_slots = (int(peak * 0.3 / 9) + 1024, int(peak * 0.7 / 9) + 4096)
@@ -320,7 +320,8 @@ def run_serving(rates=(10_000, 50_000, 100_000, 200_000, 500_000), duration_s: f
     server = K.Server(0, lanes=1 if one_per_launch else lanes, hbm_budget=hbm_budget, key_slots=key_slots,
                       ct_slots=ct_slots, max_batch=(1, 1) if one_per_launch else (0, 0),
                       reserve_lanes=reserve_lanes if reserve_lanes is not None
-                      else (0, 0) if one_per_launch or lanes < 3 else (1, 1))
+                      else (0, 0) if one_per_launch or lanes < 3 else (1, 1),
+                      group_by_key=group_by_key and not one_per_launch)
     try:
         pools = _populate(server, rng, {K.FRODO976: 64, K.SABER: 256})
         rows = []
@@ -350,13 +351,14 @@ def _main(argv=None):
     ap.add_argument("--seconds", type=float, default=0.5)
     ap.add_argument("--one-per-launch", action="store_true")
     ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--no-key-grouping", action="store_true")
     a = ap.parse_args(argv)
 
     def progress(msg):
         print(f"[serving] {msg}", file=sys.stderr, flush=True)
 
     r = run_serving(tuple(int(x) for x in a.rates.split(",")), a.seconds, one_per_launch=a.one_per_launch,
-                    lanes=a.lanes, progress=progress)
+                    lanes=a.lanes, progress=progress, group_by_key=not a.no_key_grouping)
     print(json.dumps(r), flush=True)
 
 

@@ -182,6 +182,8 @@ typedef struct {
   size_t max_batch[2];      /* per-batch cap (0 = as large as the budget allows); 1 = one request per launch */
   int reserve_lanes[2];     /* lanes only this scheme may use, so that a burst of long Frodo batches cannot
                                hold every lane while Saber requests wait (sum <= lanes; 0, 0 = all shared) */
+  int group_by_key;         /* nonzero: Frodo encaps / decaps batches are ordered by key handle and members
+                               sharing a key expand the public matrix once per group of 4 */
 } dpkem_server_config;
 double dpkem_now_us(void);
 int dpkem_server_create(dpkem_server **out, const dpkem_server_config *cfg);

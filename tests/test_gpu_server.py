@@ -115,3 +115,41 @@ def test_stale_handles_are_rejected(server):
     import dpcrypto_amd as D
     c = _run(server, np.array([_req(7, SABER, DEC, key=63, ct=255)], D.REQ_DTYPE))[7]
     assert c["status"] == -6  # ENOTFOUND: handle not allocated
+
+
+def test_frodo_requests_grouped_by_key_equal_the_oracle(oracle):
+    """Interleaved encaps / decaps for three Frodo keys: the server orders each
+    batch by key and expands A once per same-key group; every result must
+    still equal the oracle for its own key and coins."""
+    import dpcrypto_amd as D
+    srv = D.Server(0, lanes=2, hbm_budget=4 << 30, key_slots=(8, 8), ct_slots=(64, 8), group_by_key=True)
+    try:
+        keys, pks, sks = [], [], []
+        for k in range(3):
+            kc = oracle.prng_bytes(6000 + k, 64)
+            c = _run(srv, np.array([_req(10 + k, FRODO, KEYGEN, kc)], D.REQ_DTYPE))[10 + k]
+            assert c["status"] == 0
+            keys.append(int(c["key"]))
+            pk, sk = oracle.keypair(FRODO, kc[None], threads=1)
+            pks.append(pk[0]); sks.append(sk[0])
+        rng = np.random.default_rng(5)
+        pick = rng.integers(0, 3, 22)
+        coins = rng.integers(0, 256, (22, 24), dtype=np.uint8)
+        reqs = np.array([_req(100 + i, FRODO, ENC, coins[i], key=keys[pick[i]]) for i in range(22)], D.REQ_DTYPE)
+        done = _run(srv, reqs)
+        oct_, oss = oracle.enc(FRODO, np.stack([pks[p] for p in pick]), coins, threads=8)
+        cts = []
+        for i in range(22):
+            c = done[100 + i]
+            assert c["status"] == 0
+            assert np.array_equal(srv.read(FRODO, 2, int(c["ct"])), oct_[i])
+            assert np.array_equal(c["ss"][:24], oss[i])
+            cts.append(int(c["ct"]))
+        reqs = np.array([_req(200 + i, FRODO, DEC, key=keys[pick[i]], ct=cts[i]) for i in range(22)], D.REQ_DTYPE)
+        done = _run(srv, reqs)
+        for i in range(22):
+            assert np.array_equal(done[200 + i]["ss"][:24], oss[i])
+        st = srv.stats()
+        assert st["frodo_grouped_requests"] >= 44 and st["frodo_key_groups"] < st["frodo_grouped_requests"]
+    finally:
+        srv.close()
