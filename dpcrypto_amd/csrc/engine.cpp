@@ -102,7 +102,7 @@ void io_sizes(int scheme, int op, size_t out[2], size_t in[2]) {
 
 // Launch one chunk with all pointers on the device.
 void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
-                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch) {
+                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh) {
   Carver c{scratch};
   if (form == DPKEM_FORM_PLAIN) {
     if (scheme == DPKEM_SABER) plain_saber_launch(st, op, n, o0, o1, i0, i1, scratch);
@@ -126,13 +126,14 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     uint8_t* mubuf = c.take((size_t)n * 32);
     uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
     if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8);
-    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf);
+    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh);
     else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail);
   }
 }
 
 void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
-                          const uint8_t* i1, uint8_t* scratch, const uint2* groups, int ngroups) {
+                          const uint8_t* i1, uint8_t* scratch, const uint2* groups, int ngroups,
+                          const uint8_t* frodo_pkh) {
   Carver c{scratch};
   uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
   uint32_t* e8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
@@ -140,7 +141,9 @@ void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o
   uint8_t* kbuf = c.take((size_t)n * 64);
   uint8_t* mubuf = c.take((size_t)n * 32);
   uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
-  if (op == kEnc) frodo_enc_grouped_launch(st, n, i0, kSizes[DPKEM_FRODO976].pk, i1, o0, o1, s8, e8, epp8, kbuf, groups, ngroups);
+  if (op == kEnc)
+    frodo_enc_grouped_launch(st, n, i0, kSizes[DPKEM_FRODO976].pk, i1, o0, o1, s8, e8, epp8, kbuf, groups, ngroups,
+                             frodo_pkh);
   else frodo_dec_grouped_launch(st, n, i0, i1, kSizes[DPKEM_FRODO976].sk, o0, s8, e8, epp8, kbuf, mubuf, fail, groups, ngroups);
 }
 

@@ -41,10 +41,12 @@ void io_sizes(int scheme, int op, size_t out[2], size_t in[2]);
 // Frodo encaps / decaps of a chunk whose instances come in same-key groups:
 // groups[b] = (first, count <= kFrodoGroup), instances of a group carry
 // identical keys (i0 = pk rows for ENC, i1 = sk rows for DEC).
+// frodo_pkh (ENC only): nullptr, or one cached 24-byte pkh row per instance.
 void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
-                          const uint8_t* i1, uint8_t* scratch, const uint2* groups, int ngroups);
+                          const uint8_t* i1, uint8_t* scratch, const uint2* groups, int ngroups,
+                          const uint8_t* frodo_pkh = nullptr);
 void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
-                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch);
+                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh = nullptr);
 int record_hip_error(dpkem_engine* e, hipError_t err, const char* what);
 
 }  // namespace dpkem

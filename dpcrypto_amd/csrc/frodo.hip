@@ -158,20 +158,24 @@ __global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __
 
 // ----------------------------------------------------------------- enc / dec chains
 
-// enc: pkh = SHAKE256(pk); mu = coins.   dec: pkh from sk; mu = recovered mu'.
+// enc: pkh = SHAKE256(pk), or read from `pkh` rows when the caller holds it
+// (a key store keeps each key's pkh); mu = coins.   dec: pkh from sk;
+// mu = recovered mu'.
 // G2 = SHAKE256(pkh || mu, 48) = seedSE || k; S' || E' || E'' from
 // SHAKE256(0x96 || seedSE, 31360).  k goes to kbuf[0:24].
 template <bool DEC>
 __global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __restrict__ keys, size_t key_stride,
                                                        const uint8_t* __restrict__ mu, size_t mu_stride,
                                                        uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
-                                                       uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8) {
+                                                       uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
+                                                       const uint8_t* __restrict__ pkh) {
   FR_PAIR_SETUP
   const uint8_t* key = keys + (size_t)inst * key_stride;
   uint32_t a[25], g[6];
-  if (DEC) {
+  if (DEC || pkh) {
+    const uint8_t* h = DEC ? key + kFSkPkh : pkh + (size_t)inst * kFrodoSec;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g[i] = k2::deint(ld64(key + kFSkPkh + 8 * i), p);
+    for (int i = 0; i < 3; ++i) g[i] = k2::deint(ld64(h + 8 * i), p);
   } else {
     absorb2_global<kRateShake256, kFPkWords>(a, p, key, kDsShake);
 #pragma unroll
@@ -682,10 +686,10 @@ void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* p
 }
 
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf) {
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh) {
   if (n <= 0) return;
   DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, (size_t)kFrodoPk, coins,
-                     (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8);
+                     (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8, pkh);
   DPKEM_LAUNCH("frodo_sa_fused<false>", frodo_sa_fused<false>, dim3(n), dim3(1024), st, n, pk, (size_t)kFrodoPk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), coins,
                      (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
@@ -697,7 +701,7 @@ void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
   if (n <= 0) return;
   DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, (size_t)kFrodoSk, mubuf);
   DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk, (size_t)kFrodoSk, mubuf,
-                     (size_t)32, kbuf, s8, e8, epp8);
+                     (size_t)32, kbuf, s8, e8, epp8, nullptr);
   DPKEM_LAUNCH("frodo_sa_fused<true>", frodo_sa_fused<true>, dim3(n), dim3(1024), st, n, sk + kFrodoSec, (size_t)kFrodoSk, s8,
                      reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32,
                      nullptr, ct, fail);
@@ -709,11 +713,11 @@ void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
 // frodo_sa_grouped).  pk_stride / sk_stride = 0: one key for all n.
 void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t pk_stride, const uint8_t* coins,
                               uint8_t* ct, uint8_t* ss, uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf,
-                              const uint2* groups, int ngroups) {
+                              const uint2* groups, int ngroups, const uint8_t* pkh) {
   if (n <= 0) return;
   if (!groups) ngroups = blocks_for(n, kFrodoGroup);
   DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk,
-               pk_stride, coins, (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8);
+               pk_stride, coins, (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8, pkh);
   DPKEM_LAUNCH("frodo_sa_grouped<false>", (frodo_sa_grouped<false, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
                ngroups, groups, n, pk, pk_stride, s8, reinterpret_cast<const int8_t*>(e8),
                reinterpret_cast<const int8_t*>(epp8), coins, (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
@@ -727,7 +731,7 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
   if (!groups) ngroups = blocks_for(n, kFrodoGroup);
   DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, sk_stride, mubuf);
   DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk,
-               sk_stride, mubuf, (size_t)32, kbuf, s8, e8, epp8);
+               sk_stride, mubuf, (size_t)32, kbuf, s8, e8, epp8, nullptr);
   DPKEM_LAUNCH("frodo_sa_grouped<true>", (frodo_sa_grouped<true, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
                ngroups, groups, n, sk + kFrodoSec, sk_stride, s8, reinterpret_cast<const int8_t*>(e8),
                reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);

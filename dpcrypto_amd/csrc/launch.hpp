@@ -25,15 +25,16 @@ constexpr size_t kSaberChainScratchPerInst = 3744 + 768 + 32 + 64 + 4;
 
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
                          uint32_t* e8);
+// pkh: nullptr (hash each pk) or one 24-byte pkh row per instance
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf);
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh = nullptr);
 // Groups of up to kFrodoGroup instances sharing one public key expand A once;
 // groups[b] = (first, count) or nullptr for consecutive groups of
 // kFrodoGroup; key strides may be 0 (one key for all n).
 constexpr int kFrodoGroup = 4;
 void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t pk_stride, const uint8_t* coins,
                               uint8_t* ct, uint8_t* ss, uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf,
-                              const uint2* groups, int ngroups);
+                              const uint2* groups, int ngroups, const uint8_t* pkh = nullptr);
 void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, size_t sk_stride, uint8_t* ss,
                               uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail,
                               const uint2* groups, int ngroups);

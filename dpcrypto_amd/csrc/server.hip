@@ -51,6 +51,15 @@ __global__ void __launch_bounds__(256) slab_copy(int n, const uint32_t* __restri
   for (size_t k = threadIdx.x; k < bytes / 4; k += 256) dp[k] = sp[k];
 }
 
+// dst[idx[i]] (24-byte rows) = the pkh field of dense Frodo sk row i.
+__global__ void __launch_bounds__(64) pkh_scatter(int n, const uint32_t* __restrict__ idx, const uint8_t* __restrict__ sk,
+                                                  uint8_t* __restrict__ pkh_slab) {
+  const int i = blockIdx.x * 8 + threadIdx.x / 8, w = threadIdx.x % 8;  // 8 rows per 64-thread block
+  if (i >= n || w >= 6) return;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(sk + (size_t)i * 31296 + 31272);
+  reinterpret_cast<uint32_t*>(pkh_slab + (size_t)idx[i] * 24)[w] = src[w];
+}
+
 struct SlotPool {
   std::vector<uint32_t> free_list;
   std::vector<uint8_t> used;
@@ -110,6 +119,7 @@ struct dpkem_server {
   uint8_t* pk_slab[2] = {nullptr, nullptr};
   uint8_t* sk_slab[2] = {nullptr, nullptr};
   uint8_t* ct_slab[2] = {nullptr, nullptr};
+  uint8_t* pkh_slab = nullptr;  // Frodo: pkh = SHAKE256(pk) of every key, so encaps need not rehash pk
   SlotPool keys[2], cts[2];
   std::vector<Lane> lanes;
   // queues
@@ -147,7 +157,7 @@ size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 size_t lane_bytes(int scheme, int form, size_t m) {
   const SchemeDims& d = kDims[scheme];
   return chunk_scratch_bytes(scheme, form, m) + a256(m * d.pk) + a256(m * d.sk) + a256(m * d.ct) + a256(m * d.ss) +
-         a256(m * 96) + 2 * a256(m * 4) + a256(m * 8) + 4096;
+         a256(m * 96) + 2 * a256(m * 4) + a256(m * 8) + a256(m * 24) + 4096;
 }
 
 // Launch one admitted batch on lane L (called with s->mu NOT held).
@@ -167,6 +177,7 @@ void launch_batch(dpkem_server* s, Lane& L) {
   uint32_t* d_idx0 = reinterpret_cast<uint32_t*>(take((size_t)m * 4));
   uint32_t* d_idx1 = reinterpret_cast<uint32_t*>(take((size_t)m * 4));
   uint2* d_groups = reinterpret_cast<uint2*>(take((size_t)m * 8));
+  uint8_t* d_pkh = take((size_t)m * 24);
   // pinned staging: coins | idx0 | idx1 | groups | ss (the first four laid out as on the device)
   uint8_t* h = L.host;
   uint8_t* h_coins = h;
@@ -204,10 +215,17 @@ void launch_batch(dpkem_server* s, Lane& L) {
     launch_chunk_on(st, s->form, sc, op, m, d_pk, d_sk, d_coins, nullptr, base);
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx0, d_pk, s->pk_slab[sc], d.pk, 0);
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx0, d_sk, s->sk_slab[sc], d.sk, 0);
+    if (sc == DPKEM_FRODO976)
+      hipLaunchKernelGGL(pkh_scatter, dim3((m + 7) / 8), dim3(64), 0, st, m, d_idx0, d_sk, s->pkh_slab);
   } else if (op == kEnc) {
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx0, s->pk_slab[sc], d_pk, d.pk, 1);
-    if (grouped) launch_frodo_grouped(st, op, m, d_ct, d_ss, d_pk, d_coins, base, d_groups, ngroups);
-    else launch_chunk_on(st, s->form, sc, op, m, d_ct, d_ss, d_pk, d_coins, base);
+    const uint8_t* pkh = nullptr;
+    if (sc == DPKEM_FRODO976) {
+      hipLaunchKernelGGL(slab_copy, g, dim3(64), 0, st, m, d_idx0, s->pkh_slab, d_pkh, (size_t)24, 1);
+      pkh = d_pkh;
+    }
+    if (grouped) launch_frodo_grouped(st, op, m, d_ct, d_ss, d_pk, d_coins, base, d_groups, ngroups, pkh);
+    else launch_chunk_on(st, s->form, sc, op, m, d_ct, d_ss, d_pk, d_coins, base, pkh);
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx1, d_ct, s->ct_slab[sc], d.ct, 0);
     (void)hipMemcpyAsync(L.host_ss, d_ss, (size_t)m * d.ss, hipMemcpyDeviceToHost, st);
   } else {
@@ -365,6 +383,7 @@ void destroy_all(dpkem_server* s) {
     if (s->sk_slab[i]) (void)hipFree(s->sk_slab[i]);
     if (s->ct_slab[i]) (void)hipFree(s->ct_slab[i]);
   }
+  if (s->pkh_slab) (void)hipFree(s->pkh_slab);
 }
 
 }  // namespace
@@ -399,7 +418,7 @@ int dpkem_server_create(dpkem_server** out, const dpkem_server_config* cfg) {
   size_t store = 0;
   for (int sc = 0; sc < 2; ++sc) {
     const SchemeDims& d = kDims[sc];
-    store += cfg->key_slots[sc] * (d.pk + d.sk) + cfg->ct_slots[sc] * d.ct;
+    store += cfg->key_slots[sc] * (d.pk + d.sk + (sc == DPKEM_FRODO976 ? 24 : 0)) + cfg->ct_slots[sc] * d.ct;
   }
   if (store >= budget) {
     delete s;
@@ -426,6 +445,9 @@ int dpkem_server_create(dpkem_server** out, const dpkem_server_config* cfg) {
       if (rc == DPKEM_OK && hipMalloc(&s->sk_slab[sc], cfg->key_slots[sc] * d.sk) != hipSuccess) rc = DPKEM_ENOMEM;
     }
     if (rc == DPKEM_OK && cfg->ct_slots[sc] && hipMalloc(&s->ct_slab[sc], cfg->ct_slots[sc] * d.ct) != hipSuccess)
+      rc = DPKEM_ENOMEM;
+    if (rc == DPKEM_OK && sc == DPKEM_FRODO976 && cfg->key_slots[sc] &&
+        hipMalloc(&s->pkh_slab, cfg->key_slots[sc] * 24) != hipSuccess)
       rc = DPKEM_ENOMEM;
   }
   const size_t lane_dev = std::max(lane_bytes(0, s->form, s->max_batch[0]), lane_bytes(1, s->form, s->max_batch[1]));
