@@ -316,7 +316,7 @@ def main():
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=0.5, help="arrival window per rate")
-    ap.add_argument("--serving-rates", default="10000,50000,100000,200000,500000")
+    ap.add_argument("--serving-rates", default="10000,50000,100000,200000,500000,1000000,2000000")
     ap.add_argument("--schedule", default=None,
                     help="fixed plan schedule STREAMSxCHUNKS instead of autotuning (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
