@@ -16,6 +16,11 @@
  * The reference's batched kernels do not keep instances apart and derive the
  * A-row seed from an uninitialised byte; the published layout
  * A[i] = SHAKE128(LE16(i) || seedA) is used here instead.
+ *
+ * FrodoKEM-976-AES (orc_frodo_aes_*) differs only in A: entries j..j+7 of
+ * row i are AES-128_seedA(LE16(i) || LE16(j) || 0^12) read as little-endian
+ * u16 (the variant named at FrodoKEM976-GPU/include/api.h:12; its AES is
+ * declared but not defined, FrodoKEM976-GPU/include/gen_a_mat_mul.cuh:6).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -39,6 +44,19 @@ void orc_frodo_sample(uint16_t *s, size_t n) {
     for (int j = 0; j < 10; ++j) e = (uint16_t)(e + ((uint16_t)(kCdf[j] - t) >> 15));
     s[i] = (uint16_t)(((uint16_t)(-sign) ^ e) + sign);
   }
+}
+
+void orc_frodo_aes_gen_a_row(uint16_t row[N], const uint8_t seed_a[16], uint16_t i) {
+  uint8_t in[2 * N], buf[2 * N];
+  memset(in, 0, sizeof in);
+  for (int j = 0; j < N; j += 8) {
+    in[2 * j] = (uint8_t)i;
+    in[2 * j + 1] = (uint8_t)(i >> 8);
+    in[2 * j + 2] = (uint8_t)j;
+    in[2 * j + 3] = (uint8_t)(j >> 8);
+  }
+  orc_aes128_ecb(seed_a, in, buf, N / 8);
+  for (int j = 0; j < N; ++j) row[j] = (uint16_t)(buf[2 * j] | (buf[2 * j + 1] << 8));
 }
 
 void orc_frodo_gen_a_row(uint16_t row[N], const uint8_t seed_a[16], uint16_t i) {
@@ -82,10 +100,15 @@ void orc_frodo_decode(uint8_t mu[24], const uint16_t in[64]) {
   }
 }
 
-void orc_frodo_as_plus_e(uint16_t *out, const uint16_t *st, const uint16_t *e, const uint8_t seed_a[16]) {
+static void gen_row(uint16_t row[N], const uint8_t seed_a[16], uint16_t i, int aes) {
+  if (aes) orc_frodo_aes_gen_a_row(row, seed_a, i);
+  else orc_frodo_gen_a_row(row, seed_a, i);
+}
+
+static void as_plus_e(uint16_t *out, const uint16_t *st, const uint16_t *e, const uint8_t seed_a[16], int aes) {
   uint16_t row[N];
   for (int i = 0; i < N; ++i) {
-    orc_frodo_gen_a_row(row, seed_a, (uint16_t)i);
+    gen_row(row, seed_a, (uint16_t)i, aes);
     for (int k = 0; k < NBAR; ++k) {
       uint16_t acc = e[i * NBAR + k];
       for (int j = 0; j < N; ++j) acc = (uint16_t)(acc + row[j] * st[k * N + j]);
@@ -94,16 +117,24 @@ void orc_frodo_as_plus_e(uint16_t *out, const uint16_t *st, const uint16_t *e, c
   }
 }
 
-void orc_frodo_sa_plus_e(uint16_t *out, const uint16_t *sp, const uint16_t *ep, const uint8_t seed_a[16]) {
+void orc_frodo_as_plus_e(uint16_t *out, const uint16_t *st, const uint16_t *e, const uint8_t seed_a[16]) {
+  as_plus_e(out, st, e, seed_a, 0);
+}
+
+static void sa_plus_e(uint16_t *out, const uint16_t *sp, const uint16_t *ep, const uint8_t seed_a[16], int aes) {
   uint16_t row[N];
   memcpy(out, ep, sizeof(uint16_t) * N * NBAR);
   for (int i = 0; i < N; ++i) {
-    orc_frodo_gen_a_row(row, seed_a, (uint16_t)i);
+    gen_row(row, seed_a, (uint16_t)i, aes);
     for (int k = 0; k < NBAR; ++k) {
       uint16_t s = sp[k * N + i];
       for (int j = 0; j < N; ++j) out[k * N + j] = (uint16_t)(out[k * N + j] + s * row[j]);
     }
   }
+}
+
+void orc_frodo_sa_plus_e(uint16_t *out, const uint16_t *sp, const uint16_t *ep, const uint8_t seed_a[16]) {
+  sa_plus_e(out, sp, ep, seed_a, 0);
 }
 
 /* V (NBAR x NBAR) = S' (NBAR x N) * B (N x NBAR) + E'' */
@@ -130,7 +161,7 @@ static void le_bytes_to_u16(uint16_t *out, const uint8_t *in, size_t n) {
   for (size_t i = 0; i < n; ++i) out[i] = (uint16_t)(in[2 * i] | (in[2 * i + 1] << 8));
 }
 
-int orc_frodo_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins) {
+static int frodo_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins, int aes) {
   const uint8_t *s = coins, *seed_se = coins + LEN_SEC, *z = coins + 2 * LEN_SEC;
   uint8_t se_in[1 + LEN_SEC];
   uint8_t *r = malloc(4 * N * NBAR);
@@ -151,7 +182,7 @@ int orc_frodo_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins) {
   le_bytes_to_u16(e, r + 2 * N * NBAR, N * NBAR);
   orc_frodo_sample(st, N * NBAR);
   orc_frodo_sample(e, N * NBAR);
-  orc_frodo_as_plus_e(b, st, e, pk);
+  as_plus_e(b, st, e, pk, aes);
   orc_frodo_pack16(pk + LEN_SEEDA, b, N * NBAR);
   /* sk = s || pk || S^T (LE) || SHAKE256(pk, 24) */
   memcpy(sk, s, LEN_SEC);
@@ -168,7 +199,7 @@ int orc_frodo_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins) {
 /* Shared by encaps and the decaps re-encryption: from (seedSE, mu, pk) build
  * B' (NBAR x N) and C (NBAR x NBAR). */
 static int frodo_encrypt(uint16_t *bp, uint16_t *c, const uint8_t *seed_se, const uint8_t *mu,
-                         const uint8_t *pk) {
+                         const uint8_t *pk, int aes) {
   uint8_t se_in[1 + LEN_SEC];
   const size_t rlen = 2 * (2 * N + NBAR) * NBAR;
   uint8_t *r = malloc(rlen);
@@ -185,7 +216,7 @@ static int frodo_encrypt(uint16_t *bp, uint16_t *c, const uint8_t *seed_se, cons
   le_bytes_to_u16(sp, r, (2 * N + NBAR) * NBAR);
   orc_frodo_sample(sp, (2 * N + NBAR) * NBAR);
   /* sp = S' (NBAR x N) | E' (NBAR x N) | E'' (NBAR x NBAR) */
-  orc_frodo_sa_plus_e(bp, sp, sp + N * NBAR, pk);
+  sa_plus_e(bp, sp, sp + N * NBAR, pk, aes);
   orc_frodo_unpack16(b, pk + LEN_SEEDA, N * NBAR);
   sb_plus_e(v, b, sp, sp + 2 * N * NBAR);
   orc_frodo_encode(enc, mu);
@@ -194,7 +225,7 @@ static int frodo_encrypt(uint16_t *bp, uint16_t *c, const uint8_t *seed_se, cons
   return 0;
 }
 
-int orc_frodo_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu) {
+static int frodo_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu, int aes) {
   uint8_t g2in[LEN_SEC + LEN_MU], g2out[2 * LEN_SEC];
   uint16_t *bp = malloc(sizeof(uint16_t) * N * NBAR);
   uint16_t c[NBAR * NBAR];
@@ -206,7 +237,7 @@ int orc_frodo_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu
   orc_shake256(g2in, LEN_SEC, pk, ORC_FRODO_PK); /* pkh */
   memcpy(g2in + LEN_SEC, mu, LEN_MU);
   orc_shake256(g2out, 2 * LEN_SEC, g2in, sizeof g2in); /* seedSE || k */
-  if (frodo_encrypt(bp, c, g2out, mu, pk)) {
+  if (frodo_encrypt(bp, c, g2out, mu, pk, aes)) {
     free(bp); free(fin);
     return -1;
   }
@@ -219,7 +250,7 @@ int orc_frodo_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu
   return 0;
 }
 
-int orc_frodo_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk) {
+static int frodo_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk, int aes) {
   const uint8_t *sk_s = sk, *pk = sk + LEN_SEC, *sk_st = sk + LEN_SEC + ORC_FRODO_PK;
   const uint8_t *pkh = sk + ORC_FRODO_SK - LEN_SEC;
   uint16_t *st = malloc(sizeof(uint16_t) * N * NBAR);
@@ -240,7 +271,7 @@ int orc_frodo_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk) {
   memcpy(g2in, pkh, LEN_SEC);
   orc_frodo_decode(g2in + LEN_SEC, w);
   orc_shake256(g2out, 2 * LEN_SEC, g2in, sizeof g2in);
-  if (frodo_encrypt(bbp, cc, g2out, g2in + LEN_SEC, pk)) {
+  if (frodo_encrypt(bbp, cc, g2out, g2in + LEN_SEC, pk, aes)) {
     free(st); free(bp); free(bbp); free(fin);
     return -1;
   }
@@ -256,3 +287,12 @@ int orc_frodo_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk) {
   free(st); free(bp); free(bbp); free(fin);
   return 0;
 }
+
+int orc_frodo_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins) { return frodo_keypair(pk, sk, coins, 0); }
+int orc_frodo_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu) { return frodo_enc(ct, ss, pk, mu, 0); }
+int orc_frodo_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk) { return frodo_dec(ss, ct, sk, 0); }
+int orc_frodo_aes_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins) { return frodo_keypair(pk, sk, coins, 1); }
+int orc_frodo_aes_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu) {
+  return frodo_enc(ct, ss, pk, mu, 1);
+}
+int orc_frodo_aes_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk) { return frodo_dec(ss, ct, sk, 1); }

@@ -52,6 +52,12 @@ void orc_prng_fill(uint64_t seed, uint8_t *out, size_t len);
 #define ORC_FRODO_KEYGEN_COINS 64 /* s(24) || seedSE(24) || z(16) */
 #define ORC_FRODO_ENC_COINS 24    /* mu */
 
+/* FrodoKEM-976-AES: same sizes and coins; A rows from AES-128 (orc_frodo_aes_gen_a_row). */
+int orc_frodo_aes_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins);
+int orc_frodo_aes_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu);
+int orc_frodo_aes_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+void orc_frodo_aes_gen_a_row(uint16_t row[ORC_FRODO_N], const uint8_t seed_a[16], uint16_t i);
+void orc_aes128_ecb(const uint8_t key[16], const uint8_t *in, uint8_t *out, size_t nblocks);
 int orc_frodo_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins);
 int orc_frodo_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu);
 int orc_frodo_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
@@ -93,7 +99,7 @@ void orc_saber_unpack10(uint16_t *out, const uint8_t *in, size_t npoly);
 void orc_saber_indcpa_enc(uint8_t *ct, const uint8_t m[32], const uint8_t seed_sp[32], const uint8_t *pk);
 
 /* ---------------- batched / threaded drivers (CPU baseline) ---------------- */
-/* scheme: 0 = Frodo-976, 1 = Saber.  op: 0 keygen, 1 enc, 2 dec.
+/* scheme: 0 = Frodo-976-SHAKE, 1 = Saber, 2 = Frodo-976-AES.  op: 0 keygen, 1 enc, 2 dec.
  * Runs `n` independent instances over `threads` POSIX threads. */
 int orc_batch(int scheme, int op, size_t n, int threads,
               uint8_t *a, uint8_t *b, const uint8_t *c, const uint8_t *d);

@@ -18,10 +18,12 @@ _lib = None
 
 FRODO = 0
 SABER = 1
+FRODO_AES = 2
 
 SIZES = {
     FRODO: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
     SABER: dict(pk=992, sk=2304, ct=1088, ss=32, kcoins=96, ecoins=32),
+    FRODO_AES: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
 }
 
 
@@ -50,6 +52,8 @@ def lib():
         _lib.orc_batch.restype = ctypes.c_int
         _lib.orc_frodo_sample.argtypes = [vp, sz]
         _lib.orc_frodo_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
+        _lib.orc_frodo_aes_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
+        _lib.orc_aes128_ecb.argtypes = [vp, vp, vp, sz]
         _lib.orc_frodo_pack16.argtypes = [vp, vp, sz]
         _lib.orc_frodo_unpack16.argtypes = [vp, vp, sz]
         _lib.orc_frodo_encode.argtypes = [vp, vp]
@@ -160,6 +164,18 @@ def frodo_gen_a_row(seed_a: bytes, i: int) -> np.ndarray:
     row = np.zeros(976, np.uint16); sa = _u8(seed_a)
     lib().orc_frodo_gen_a_row(_p(row), _p(sa), i)
     return row
+
+
+def frodo_aes_gen_a_row(seed_a: bytes, i: int) -> np.ndarray:
+    row = np.zeros(976, np.uint16); sa = _u8(seed_a)
+    lib().orc_frodo_aes_gen_a_row(_p(row), _p(sa), i)
+    return row
+
+
+def aes128_ecb(key: bytes, data: bytes) -> bytes:
+    k = _u8(key); d = _u8(data); out = np.zeros(len(data), np.uint8)
+    lib().orc_aes128_ecb(_p(k), _p(d), _p(out), len(data) // 16)
+    return out.tobytes()
 
 
 def frodo_pack16(v: np.ndarray) -> bytes:

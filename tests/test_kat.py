@@ -9,9 +9,6 @@ after the first 32 bytes).  KAT responses computed by the oracle round-trip
 through the .rsp format.  The official .rsp files for these two schemes are
 not in the container, so their contents stay unpinned beyond this.
 """
-import ctypes
-import ctypes.util
-
 import numpy as np
 import pytest
 
@@ -29,34 +26,9 @@ def test_aes256_fips197_example():
     assert kat.aes256_ecb(key, pt).hex() == "8ea2b7ca516745bfeafc49904b496089"
 
 
-def _openssl_aes256_ecb():
-    path = ctypes.util.find_library("crypto")
-    if not path:
-        return None
-    lib = ctypes.CDLL(path)
-    lib.EVP_CIPHER_CTX_new.restype = ctypes.c_void_p
-    lib.EVP_aes_256_ecb.restype = ctypes.c_void_p
-    lib.EVP_EncryptInit_ex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
-                                       ctypes.c_char_p]
-    lib.EVP_CIPHER_CTX_set_padding.argtypes = [ctypes.c_void_p, ctypes.c_int]
-    lib.EVP_EncryptUpdate.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
-                                      ctypes.c_char_p, ctypes.c_int]
-    lib.EVP_CIPHER_CTX_free.argtypes = [ctypes.c_void_p]
-
-    def enc(key, data):
-        ctx = lib.EVP_CIPHER_CTX_new()
-        assert lib.EVP_EncryptInit_ex(ctx, lib.EVP_aes_256_ecb(), None, key, None) == 1
-        lib.EVP_CIPHER_CTX_set_padding(ctx, 0)
-        out = ctypes.create_string_buffer(len(data) + 32)
-        n = ctypes.c_int(0)
-        assert lib.EVP_EncryptUpdate(ctx, out, ctypes.byref(n), data, len(data)) == 1
-        lib.EVP_CIPHER_CTX_free(ctx)
-        return out.raw[: n.value]
-    return enc
-
-
 def test_aes256_matches_openssl():
-    ref = _openssl_aes256_ecb()
+    from openssl_aes import aes_ecb
+    ref = aes_ecb()
     if ref is None:
         pytest.skip("libcrypto not found")
     rng = np.random.default_rng(11)
