@@ -25,10 +25,15 @@ struct SchemeSizes {
   size_t pk, sk, ct, ss, kcoins, ecoins;
 };
 
-const SchemeSizes kSizes[2] = {
+const SchemeSizes kSizes[3] = {
     {15632, 31296, 15744, 24, 64, 24},  // FrodoKEM-976-SHAKE
     {992, 2304, 1088, 32, 96, 32},      // Saber
+    {15632, 31296, 15744, 24, 64, 24},  // FrodoKEM-976-AES
 };
+
+bool valid_scheme(int scheme) {
+  return scheme == DPKEM_FRODO976 || scheme == DPKEM_SABER || scheme == DPKEM_FRODO976_AES;
+}
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
@@ -85,7 +90,7 @@ int record_hip_error(dpkem_engine* e, hipError_t err, const char* what) { return
 // Bytes of device scratch one instance needs in the given form, excluding staged I/O.
 size_t scratch_per_inst(int scheme, int form) {
   if (form == DPKEM_FORM_PLAIN) return align_up(plain_scratch_per_inst(scheme));
-  return scheme == DPKEM_FRODO976 ? align_up(kFrodoScratchPerInst) : align_up(kSaberScratchPerInst);
+  return scheme == DPKEM_SABER ? align_up(kSaberScratchPerInst) : align_up(kFrodoScratchPerInst);
 }
 
 // I/O byte sizes per instance: two outputs and two inputs (0 if unused).
@@ -125,9 +130,10 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     uint8_t* kbuf = c.take((size_t)n * 64);
     uint8_t* mubuf = c.take((size_t)n * 32);
     uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
-    if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8);
-    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh);
-    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail);
+    const bool aes = scheme == DPKEM_FRODO976_AES;
+    if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8, aes);
+    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh, aes);
+    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail, aes);
   }
 }
 
@@ -156,12 +162,12 @@ size_t chunk_scratch_bytes(int scheme, int form, size_t n) { return n * scratch_
 namespace {
 
 // Encaps / decaps of a chunk whose key input (ENC: pk, DEC: sk) is one row
-// shared by all instances.  Frodo: groups of kFrodoGroup instances expand A
-// once.  Saber: the key row is broadcast into `bcast` and the ordinary
-// kernels run (its matrix is a small share of the work).
+// shared by all instances.  Frodo-SHAKE: groups of kFrodoGroup instances
+// expand A once.  Saber and Frodo-AES: the key row is broadcast into `bcast`
+// and the ordinary kernels run.
 void launch_chunk_shared(hipStream_t st, int scheme, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
                          const uint8_t* i1, uint8_t* scratch, uint8_t* bcast) {
-  if (scheme == DPKEM_SABER) {
+  if (scheme != DPKEM_FRODO976) {
     if (op == kEnc) {
       broadcast_rows_launch(st, n, i0, kSizes[scheme].pk, bcast);
       launch_chunk_on(st, DPKEM_FORM_BATCHED, scheme, op, n, o0, o1, bcast, i1, scratch);
@@ -186,7 +192,11 @@ void launch_chunk_shared(hipStream_t st, int scheme, int op, int n, uint8_t* o0,
 // instances (ENC: 0 = pk, DEC: 1 = sk).
 int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
            const uint8_t* i1, int mem, int shared_in = -1) {
-  if (!e || (scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER)) return DPKEM_EINVAL;
+  if (!e || !valid_scheme(scheme)) return DPKEM_EINVAL;
+  if (scheme == DPKEM_FRODO976_AES && e->form != DPKEM_FORM_BATCHED) {
+    snprintf(e->last_err, sizeof e->last_err, "FrodoKEM-976-AES runs in the batched form only");
+    return DPKEM_EINVAL;
+  }
   if (mem != DPKEM_MEM_HOST && mem != DPKEM_MEM_DEVICE) return DPKEM_EINVAL;
   if (n == 0) return DPKEM_OK;
   size_t out[2], in[2];
@@ -212,7 +222,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
     if (mem == DPKEM_MEM_HOST) io_per += align_up(out[k]) + (k == shared_in ? 0 : align_up(in[k]));
   }
   // Saber broadcasts the shared key into a dense array
-  if (shared_in >= 0 && scheme == DPKEM_SABER) io_per += align_up(in[shared_in]);
+  if (shared_in >= 0 && scheme != DPKEM_FRODO976) io_per += align_up(in[shared_in]);
   const size_t per = scratch_per_inst(scheme, e->form) + io_per;
   // a shared key staged from host memory is one row per chunk
   const size_t fixed = 24 * kAlign + (shared_in >= 0 && mem == DPKEM_MEM_HOST ? align_up(in[shared_in]) : 0);
@@ -242,7 +252,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
     }
     tls_profiler = e->profiling ? &e->prof : nullptr;
     if (shared_in >= 0) {
-      uint8_t* bcast = scheme == DPKEM_SABER ? io.take((size_t)c * in[shared_in]) : nullptr;
+      uint8_t* bcast = scheme != DPKEM_FRODO976 ? io.take((size_t)c * in[shared_in]) : nullptr;
       launch_chunk_shared(e->stream, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, bcast);
     } else {
       launch_chunk_on(e->stream, e->form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
@@ -315,7 +325,7 @@ int wrapped(int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_
 extern "C" {
 
 int dpkem_sizes(int scheme, size_t* pk, size_t* sk, size_t* ct, size_t* ss, size_t* kc, size_t* ec) {
-  if (scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER) return DPKEM_EINVAL;
+  if (!valid_scheme(scheme)) return DPKEM_EINVAL;
   const SchemeSizes& s = kSizes[scheme];
   if (pk) *pk = s.pk;
   if (sk) *sk = s.sk;
@@ -470,6 +480,15 @@ int dpkem_frodo976_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint
 }
 int dpkem_frodo976_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
   return wrapped(DPKEM_FRODO976, kDec, n, ss, nullptr, ct, sk);
+}
+int dpkem_frodo976aes_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
+  return wrapped(DPKEM_FRODO976_AES, kKeygen, n, pk, sk, nullptr, nullptr);
+}
+int dpkem_frodo976aes_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  return wrapped(DPKEM_FRODO976_AES, kEnc, n, ct, ss, pk, nullptr);
+}
+int dpkem_frodo976aes_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  return wrapped(DPKEM_FRODO976_AES, kDec, n, ss, nullptr, ct, sk);
 }
 int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
   return wrapped(DPKEM_SABER, kKeygen, n, pk, sk, nullptr, nullptr);

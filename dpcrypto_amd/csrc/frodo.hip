@@ -326,13 +326,140 @@ __device__ __forceinline__ void gen_row(int row, uint64_t s0, uint64_t s1, UNIT&
   }
 }
 
+
+// ----------------------------------------------------------------- FrodoKEM-976-AES rows
+// A[i][j..j+7] = AES-128_seedA(LE16(i) || LE16(j) || 0^12) as little-endian
+// u16; lane = row as for SHAKE128, two blocks per 16-column MFMA unit.  One
+// T-table Te[x] = 2S(x) | S(x) << 8 | S(x) << 16 | 3S(x) << 24 (the MixColumns
+// column of input row 0; rows 1-3 are its byte rotations) is replicated 32
+// times in LDS, entry x of copy c at dword 32 x + c: lane l reads copy l % 32,
+// so the 32 lanes of a half-wave hit 32 different banks.  The S-box is
+// computed from x^254 in GF(2^8) when a workgroup starts; round keys are
+// wave-uniform and live in scalar registers.
+constexpr int kAesCopies = 32;
+constexpr int kAesLdsWords = 256 * kAesCopies + 44;  // tables + 11 round keys
+
+__device__ __forceinline__ uint32_t gf_mul2(uint32_t a) { return ((a << 1) ^ ((a >> 7) * 0x1Bu)) & 0xFFu; }
+
+__device__ __forceinline__ uint32_t gf_mul(uint32_t a, uint32_t b) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    r ^= (b & 1u) ? a : 0u;
+    a = gf_mul2(a);
+    b >>= 1;
+  }
+  return r;
+}
+
+__device__ uint32_t aes_sbox(uint32_t x) {
+  uint32_t inv = 1, base = x;  // x^254 = x^-1 (0 -> 0)
+#pragma unroll
+  for (int e = 254; e; e >>= 1) {
+    if (e & 1) inv = gf_mul(inv, base);
+    base = gf_mul(base, base);
+  }
+  uint32_t r = inv;
+#pragma unroll
+  for (int k = 1; k < 5; ++k) r ^= ((inv << k) | (inv >> (8 - k))) & 0xFFu;
+  return r ^ 0x63u;
+}
+
+// Build the replicated T-table and expand seedA (16 bytes) into T[256*32 ...].
+__device__ void aes_setup(uint32_t* T, const uint8_t* seed_a, int tid, int nthreads) {
+  for (int x = tid; x < 256; x += nthreads) {
+    const uint32_t sx = aes_sbox((uint32_t)x), m2 = gf_mul2(sx);
+    T[x * kAesCopies] = m2 | (sx << 8) | (sx << 16) | ((m2 ^ sx) << 24);
+  }
+  __syncthreads();
+  for (int t = tid; t < 256 * kAesCopies; t += nthreads)
+    if (t & (kAesCopies - 1)) T[t] = T[t & ~(kAesCopies - 1)];
+  if (tid == 0) {
+    uint32_t* rk = T + 256 * kAesCopies;
+    for (int i = 0; i < 4; ++i)
+      rk[i] = seed_a[4 * i] | ((uint32_t)seed_a[4 * i + 1] << 8) | ((uint32_t)seed_a[4 * i + 2] << 16) |
+              ((uint32_t)seed_a[4 * i + 3] << 24);
+    uint32_t rcon = 1;
+    for (int i = 4; i < 44; ++i) {
+      uint32_t t = rk[i - 1];
+      if (i % 4 == 0) {
+        t = (t >> 8) | (t << 24);  // RotWord (bytes are little-endian)
+        uint32_t sub = 0;
+        for (int b = 0; b < 4; ++b) sub |= ((T[((t >> (8 * b)) & 0xFF) * kAesCopies] >> 8) & 0xFFu) << (8 * b);
+        t = sub ^ rcon;
+        rcon = gf_mul2(rcon);
+      }
+      rk[i] = rk[i - 4] ^ t;
+    }
+  }
+  __syncthreads();
+}
+
+struct AesRows {
+  const uint8_t* tb;  // T-table base in bytes
+  uint32_t c4;        // this lane's copy offset in bytes
+  uint32_t rk[44];
+
+  __device__ __forceinline__ uint32_t te(uint32_t s, int byte) const {
+    const uint32_t x = __builtin_amdgcn_ubfe(s, 8 * byte, 8);
+    return *reinterpret_cast<const uint32_t*>(tb + ((x << 7) | c4));
+  }
+  __device__ __forceinline__ static uint32_t rotl8(uint32_t v, int k) {
+    return k ? __builtin_amdgcn_alignbit(v, v, 32 - 8 * k) : v;
+  }
+  // one block: in = (w0, 0, 0, 0) -> out[4]
+  __device__ __forceinline__ void encrypt(uint32_t w0, uint32_t out[4]) const {
+    uint32_t s0 = w0 ^ rk[0], s1 = rk[1], s2 = rk[2], s3 = rk[3];
+#pragma unroll
+    for (int r = 1; r < 10; ++r) {
+      const uint32_t t0 = te(s0, 0) ^ rotl8(te(s1, 1), 1) ^ rotl8(te(s2, 2), 2) ^ rotl8(te(s3, 3), 3) ^ rk[4 * r];
+      const uint32_t t1 = te(s1, 0) ^ rotl8(te(s2, 1), 1) ^ rotl8(te(s3, 2), 2) ^ rotl8(te(s0, 3), 3) ^ rk[4 * r + 1];
+      const uint32_t t2 = te(s2, 0) ^ rotl8(te(s3, 1), 1) ^ rotl8(te(s0, 2), 2) ^ rotl8(te(s1, 3), 3) ^ rk[4 * r + 2];
+      const uint32_t t3 = te(s3, 0) ^ rotl8(te(s0, 1), 1) ^ rotl8(te(s1, 2), 2) ^ rotl8(te(s2, 3), 3) ^ rk[4 * r + 3];
+      s0 = t0; s1 = t1; s2 = t2; s3 = t3;
+    }
+    // last round: S(x) is byte 1 of Te[x]
+    const uint32_t st[4] = {s0, s1, s2, s3};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t b0 = (te(st[c], 0) >> 8) & 0xFFu;
+      const uint32_t b1 = te(st[(c + 1) & 3], 1) & 0xFF00u;
+      const uint32_t b2 = (te(st[(c + 2) & 3], 2) << 8) & 0xFF0000u;
+      const uint32_t b3 = (te(st[(c + 3) & 3], 3) << 16) & 0xFF000000u;
+      out[c] = (b0 | b1 | b2 | b3) ^ rk[40 + c];
+    }
+  }
+  // row i: 61 units of 16 columns (= 976), each from blocks j and j + 8
+  template <class UNIT>
+  __device__ __forceinline__ void operator()(int row, UNIT&& unit) const {
+    for (int u = 0; u < kFUnits; ++u) {
+      uint32_t a[4], b[4];
+      encrypt((uint32_t)(row & 0xFFFF) | ((uint32_t)(16 * u) << 16), a);
+      encrypt((uint32_t)(row & 0xFFFF) | ((uint32_t)(16 * u + 8) << 16), b);
+      const uint64_t w[4] = {a[0] | ((uint64_t)a[1] << 32), a[2] | ((uint64_t)a[3] << 32),
+                             b[0] | ((uint64_t)b[1] << 32), b[2] | ((uint64_t)b[3] << 32)};
+      unit(w);
+    }
+  }
+};
+
+__device__ __forceinline__ void aes_rows_init(AesRows& g, const uint32_t* T, int lane) {
+  g.tb = reinterpret_cast<const uint8_t*>(T);
+  g.c4 = (uint32_t)(lane & (kAesCopies - 1)) * 4u;
+#pragma unroll
+  for (int i = 0; i < 44; ++i) g.rk[i] = __builtin_amdgcn_readfirstlane(T[256 * kAesCopies + i]);
+}
+
 constexpr int kSRow = 1008;  // LDS row stride of int8 S^T (63 units, zero beyond 976)
 
 // keygen: B = A S + E.  pk[16:] and sk[40:15656] receive B (big-endian u16).
+// AES: FrodoKEM-976-AES rows instead of SHAKE128 rows.
+template <bool AES>
 __global__ void __launch_bounds__(1024) frodo_as_fused(int n, uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                        const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8) {
   __shared__ __align__(16) int8_t Sl[9 * kSRow];  // rows 0..7 = S^T, row 8 = zeros
   __shared__ int colsum[8];
+  __shared__ __align__(16) uint32_t aes_lds[AES ? kAesLdsWords : 1];
   const int inst = blockIdx.x, tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   if (inst >= n) return;
@@ -354,6 +481,11 @@ __global__ void __launch_bounds__(1024) frodo_as_fused(int n, uint8_t* __restric
 
   const uint8_t* p = pk + (size_t)inst * kFrodoPk;
   const uint64_t s0 = ld64(p), s1 = ld64(p + 8);
+  AesRows aesg;
+  if constexpr (AES) {
+    aes_setup(aes_lds, p, tid, 1024);
+    aes_rows_init(aesg, aes_lds, lane);
+  }
   const int q = lane >> 4, nn = lane & 15;
   // B operand: lane (q, nn) carries S^T column 4r + nn%4 when nn/4 == q, else zeros
   const bool mine = (nn >> 2) == q;
@@ -361,7 +493,7 @@ __global__ void __launch_bounds__(1024) frodo_as_fused(int n, uint8_t* __restric
   const int boff1 = mine ? (4 + (nn & 3)) * kSRow : 8 * kSRow;
   v4i acc_lo0 = {0, 0, 0, 0}, acc_lo1 = {0, 0, 0, 0}, acc_hi0 = {0, 0, 0, 0}, acc_hi1 = {0, 0, 0, 0};
   int u = 0;
-  gen_row(wave * 64 + lane, s0, s1, [&](const uint64_t w[4]) {
+  auto unit = [&](const uint64_t w[4]) {
     if (u < kFUnits) {
       v4i lo, hi;
       unit_limbs(w, lo, hi);
@@ -373,7 +505,9 @@ __global__ void __launch_bounds__(1024) frodo_as_fused(int n, uint8_t* __restric
       acc_hi1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(hi, b1, acc_hi1, 0, 0, 0);
     }
     ++u;
-  });
+  };
+  if constexpr (AES) aesg(wave * 64 + lane, unit);
+  else gen_row(wave * 64 + lane, s0, s1, unit);
 
   // D layout: lane (g, nn), element i = C[4g + i][nn].  C row m comes from
   // A-operand lane 16 * (nn / 4) + m; the column is 4r + nn % 4.
@@ -402,7 +536,7 @@ __global__ void __launch_bounds__(1024) frodo_as_fused(int n, uint8_t* __restric
 
 // enc / dec: B' = S' A + E', then V = S' B + E'' and C = V + Encode(mu).
 // enc writes ct; dec compares with the received ct and raises fail[inst].
-template <bool DEC>
+template <bool DEC, bool AES>
 __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __restrict__ pkb, size_t pk_stride,
                                                        const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8,
                                                        const int8_t* __restrict__ epp8, const uint8_t* __restrict__ mu,
@@ -413,6 +547,7 @@ __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __r
   __shared__ int rowsum[8];
   __shared__ uint32_t V[64];
   __shared__ uint32_t diff;
+  __shared__ __align__(16) uint32_t aes_lds[AES ? kAesLdsWords : 1];
   const int inst = blockIdx.x, tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   if (inst >= n) return;
@@ -435,6 +570,11 @@ __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __r
 
   const uint8_t* pk = pkb + (size_t)inst * pk_stride;
   const uint64_t s0 = ld64(pk), s1 = ld64(pk + 8);
+  AesRows aesg;
+  if constexpr (AES) {
+    aes_setup(aes_lds, pk, tid, 1024);
+    aes_rows_init(aesg, aes_lds, lane);
+  }
   const int g = lane >> 4, nn = lane & 15;
   // product A operand: lane (g, m = nn) holds S'[m][wave rows 16 q + 4 g + i] in dword q, byte i
   v4i sop = {0, 0, 0, 0};
@@ -452,7 +592,7 @@ __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __r
   const v4i zero = {0, 0, 0, 0};
 
   int u = 0;
-  gen_row(wave * 64 + lane, s0, s1, [&](const uint64_t w[4]) {
+  auto unit = [&](const uint64_t w[4]) {
     if (u < kFUnits) {
       v4i lm[2];
       unit_limbs(w, lm[0], lm[1]);
@@ -477,7 +617,9 @@ __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __r
       }
     }
     ++u;
-  });
+  };
+  if constexpr (AES) aesg(wave * 64 + lane, unit);
+  else gen_row(wave * 64 + lane, s0, s1, unit);
   __syncthreads();
 
   // B' = acc + 128 rowsum + E'  (k-major, 8 x 976)
@@ -678,33 +820,48 @@ __global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint
 static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
-                         uint32_t* e8) {
+                         uint32_t* e8, bool aes) {
   if (n <= 0) return;
   DPKEM_LAUNCH("frodo_keygen_pre", frodo_keygen_pre, dim3(blocks_for(n, 32)), dim3(64), st, n, coins, pk, sk, s8, e8);
-  DPKEM_LAUNCH("frodo_as_fused", frodo_as_fused, dim3(n), dim3(1024), st, n, pk, sk, s8, reinterpret_cast<const int8_t*>(e8));
+  if (aes)
+    DPKEM_LAUNCH("frodo_as_fused<aes>", frodo_as_fused<true>, dim3(n), dim3(1024), st, n, pk, sk, s8,
+                 reinterpret_cast<const int8_t*>(e8));
+  else
+    DPKEM_LAUNCH("frodo_as_fused", frodo_as_fused<false>, dim3(n), dim3(1024), st, n, pk, sk, s8,
+                 reinterpret_cast<const int8_t*>(e8));
   DPKEM_LAUNCH("frodo_keygen_post", frodo_keygen_post, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
 }
 
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh) {
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh, bool aes) {
   if (n <= 0) return;
   DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, (size_t)kFrodoPk, coins,
                      (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8, pkh);
-  DPKEM_LAUNCH("frodo_sa_fused<false>", frodo_sa_fused<false>, dim3(n), dim3(1024), st, n, pk, (size_t)kFrodoPk, s8,
-                     reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), coins,
-                     (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
+  if (aes)
+    DPKEM_LAUNCH("frodo_sa_fused<false,aes>", (frodo_sa_fused<false, true>), dim3(n), dim3(1024), st, n, pk,
+                 (size_t)kFrodoPk, s8, reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8),
+                 coins, (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
+  else
+    DPKEM_LAUNCH("frodo_sa_fused<false>", (frodo_sa_fused<false, false>), dim3(n), dim3(1024), st, n, pk,
+                 (size_t)kFrodoPk, s8, reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8),
+                 coins, (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
   DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, (size_t)0, nullptr, ss);
 }
 
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
-                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail) {
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes) {
   if (n <= 0) return;
   DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, (size_t)kFrodoSk, mubuf);
   DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk, (size_t)kFrodoSk, mubuf,
                      (size_t)32, kbuf, s8, e8, epp8, nullptr);
-  DPKEM_LAUNCH("frodo_sa_fused<true>", frodo_sa_fused<true>, dim3(n), dim3(1024), st, n, sk + kFrodoSec, (size_t)kFrodoSk, s8,
-                     reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32,
-                     nullptr, ct, fail);
+  if (aes)
+    DPKEM_LAUNCH("frodo_sa_fused<true,aes>", (frodo_sa_fused<true, true>), dim3(n), dim3(1024), st, n,
+                 sk + kFrodoSec, (size_t)kFrodoSk, s8, reinterpret_cast<const int8_t*>(e8),
+                 reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
+  else
+    DPKEM_LAUNCH("frodo_sa_fused<true>", (frodo_sa_fused<true, false>), dim3(n), dim3(1024), st, n,
+                 sk + kFrodoSec, (size_t)kFrodoSk, s8, reinterpret_cast<const int8_t*>(e8),
+                 reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
   DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, (size_t)kFrodoSk, fail, ss);
 }
 

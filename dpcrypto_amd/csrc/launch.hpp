@@ -23,11 +23,13 @@ void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t*
                         uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch);
 constexpr size_t kSaberChainScratchPerInst = 3744 + 768 + 32 + 64 + 4;
 
+// aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
-                         uint32_t* e8);
+                         uint32_t* e8, bool aes = false);
 // pkh: nullptr (hash each pk) or one 24-byte pkh row per instance
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh = nullptr);
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh = nullptr,
+                      bool aes = false);
 // Groups of up to kFrodoGroup instances sharing one public key expand A once;
 // groups[b] = (first, count) or nullptr for consecutive groups of
 // kFrodoGroup; key strides may be 0 (one key for all n).
@@ -39,7 +41,7 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
                               uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail,
                               const uint2* groups, int ngroups);
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
-                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail);
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes = false);
 
 // ---- plain comparator form (plain.hip); op 0 keygen, 1 enc, 2 dec with the
 // same output / input roles as the batched launchers.

@@ -33,9 +33,12 @@ ERRORS = {
     -1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EBUDGET", -5: "EBUSY", -6: "ENOTFOUND", -7: "ECANCELED",
 }
 
+FRODO976_AES = 2  # FrodoKEM-976-AES: Frodo-976 with A rows from AES-128 (batched form only)
+
 SIZES = {
     FRODO976: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
     SABER: dict(pk=992, sk=2304, ct=1088, ss=32, kcoins=96, ecoins=32),
+    FRODO976_AES: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
 }
 
 _lib = None

@@ -24,8 +24,12 @@ extern "C" {
 #endif
 
 /* ---------------------------------------------------------------- constants */
-#define DPKEM_FRODO976 0 /* FrodoKEM-976-SHAKE */
-#define DPKEM_SABER 1    /* Saber, l = 3 */
+#define DPKEM_FRODO976 0     /* FrodoKEM-976-SHAKE */
+#define DPKEM_SABER 1        /* Saber, l = 3 */
+#define DPKEM_FRODO976_AES 2 /* FrodoKEM-976-AES: as DPKEM_FRODO976 (same sizes and coins) with A rows from
+                                AES-128 (named at FrodoKEM976-GPU/include/api.h:12, its AES declared at
+                                FrodoKEM976-GPU/include/gen_a_mat_mul.cuh:6); engine, plans and KAT only,
+                                batched form only */
 
 #define DPKEM_MEM_HOST 0   /* pointers are host memory (pageable or pinned) */
 #define DPKEM_MEM_DEVICE 1 /* pointers are device memory on the engine's GPU */
@@ -227,7 +231,7 @@ void dpkem_randombytes_drbg(uint8_t *out, size_t len, void *ctx);
 int dpkem_kat_seeds(size_t n, uint8_t *seeds);
 /* Keygen and encaps coins of n KAT counts: per count, the DRBG seeded with
  * that count's seed, drawn with the scheme's randombytes call pattern
- * (Frodo-976: 64 B, then 24 B; Saber: 3 x 32 B, then 32 B). */
+ * (Frodo-976 SHAKE / AES: 64 B, then 24 B; Saber: 3 x 32 B, then 32 B). */
 int dpkem_kat_coins(int scheme, size_t n, const uint8_t *seeds, uint8_t *kcoins, uint8_t *ecoins);
 
 /* crypto_kem_{keypair,enc,dec} over arrays of n instances (host memory),
@@ -236,6 +240,9 @@ int dpkem_kat_coins(int scheme, size_t n, const uint8_t *seeds, uint8_t *kcoins,
 int dpkem_frodo976_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
 int dpkem_frodo976_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
 int dpkem_frodo976_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+int dpkem_frodo976aes_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
+int dpkem_frodo976aes_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int dpkem_frodo976aes_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
 int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
 int dpkem_saber_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
 int dpkem_saber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
