@@ -27,7 +27,8 @@ static void sizes(int scheme, size_t *pk, size_t *sk, size_t *ct, size_t *ss, si
     *pk = ORC_FRODO_PK; *sk = ORC_FRODO_SK; *ct = ORC_FRODO_CT; *ss = ORC_FRODO_SS;
     *kc = ORC_FRODO_KEYGEN_COINS; *ec = ORC_FRODO_ENC_COINS;
   } else {
-    *pk = ORC_SABER_PK; *sk = ORC_SABER_SK; *ct = ORC_SABER_CT; *ss = ORC_SABER_SS;
+    const size_t l = scheme == 1 ? 3 : scheme == 3 ? 2 : 4, et = l == 2 ? 3 : l == 3 ? 4 : 6;
+    *pk = 320 * l + 32; *sk = 416 * l + *pk + 64; *ct = 320 * l + 32 * et; *ss = ORC_SABER_SS;
     *kc = ORC_SABER_KEYGEN_COINS; *ec = ORC_SABER_ENC_COINS;
   }
 }
@@ -38,18 +39,19 @@ static void *run(void *arg) {
   sizes(j->scheme, &pk, &sk, &ct, &ss, &kc, &ec);
   for (size_t i = j->begin; i < j->end; ++i) {
     int rc;
+    const int l = j->scheme == 1 ? 3 : j->scheme == 3 ? 2 : 4; /* Saber family rank */
     if (j->op == 0)
       rc = j->scheme == 0   ? orc_frodo_keypair(j->a + i * pk, j->b + i * sk, j->c + i * kc)
            : j->scheme == 2 ? orc_frodo_aes_keypair(j->a + i * pk, j->b + i * sk, j->c + i * kc)
-                            : orc_saber_keypair(j->a + i * pk, j->b + i * sk, j->c + i * kc);
+                            : orc_saber_l_keypair(l, j->a + i * pk, j->b + i * sk, j->c + i * kc);
     else if (j->op == 1)
       rc = j->scheme == 0   ? orc_frodo_enc(j->a + i * ct, j->b + i * ss, j->c + i * pk, j->d + i * ec)
            : j->scheme == 2 ? orc_frodo_aes_enc(j->a + i * ct, j->b + i * ss, j->c + i * pk, j->d + i * ec)
-                            : orc_saber_enc(j->a + i * ct, j->b + i * ss, j->c + i * pk, j->d + i * ec);
+                            : orc_saber_l_enc(l, j->a + i * ct, j->b + i * ss, j->c + i * pk, j->d + i * ec);
     else
       rc = j->scheme == 0   ? orc_frodo_dec(j->a + i * ss, j->c + i * ct, j->d + i * sk)
            : j->scheme == 2 ? orc_frodo_aes_dec(j->a + i * ss, j->c + i * ct, j->d + i * sk)
-                            : orc_saber_dec(j->a + i * ss, j->c + i * ct, j->d + i * sk);
+                            : orc_saber_l_dec(l, j->a + i * ss, j->c + i * ct, j->d + i * sk);
     if (rc) j->rc = rc;
   }
   return NULL;
@@ -57,7 +59,7 @@ static void *run(void *arg) {
 
 int orc_batch(int scheme, int op, size_t n, int threads, uint8_t *a, uint8_t *b, const uint8_t *c,
               const uint8_t *d) {
-  if (scheme < 0 || scheme > 2 || op < 0 || op > 2 || threads < 1) return -1;
+  if (scheme < 0 || scheme > 4 || op < 0 || op > 2 || threads < 1) return -1;
   if ((size_t)threads > n) threads = n ? (int)n : 1;
   pthread_t *tid = calloc((size_t)threads, sizeof *tid);
   job_t *jobs = calloc((size_t)threads, sizeof *jobs);

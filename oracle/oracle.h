@@ -82,6 +82,13 @@ void orc_frodo_sa_plus_e(uint16_t *out, const uint16_t *sp, const uint16_t *ep, 
 #define ORC_SABER_KEYGEN_COINS 96 /* seedA_raw(32) || seed_s(32) || z(32) */
 #define ORC_SABER_ENC_COINS 32    /* m_raw */
 
+/* Saber family by module rank l: 2 = LightSaber, 3 = Saber, 4 = FireSaber.
+ * pk = 320 l + 32, sk = 416 l + pk + 64, ct = 320 l + 32 eT (eT = 3, 4, 6). */
+int orc_saber_l_keypair(int l, uint8_t *pk, uint8_t *sk, const uint8_t *coins);
+int orc_saber_l_enc(int l, uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *m_raw);
+int orc_saber_l_dec(int l, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+void orc_saber_l_gen_matrix(int l, uint16_t *a, const uint8_t seed[32]);
+void orc_saber_l_gen_secret(int l, uint16_t *s, const uint8_t seed[32]);
 int orc_saber_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins);
 int orc_saber_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *m_raw);
 int orc_saber_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
@@ -99,7 +106,8 @@ void orc_saber_unpack10(uint16_t *out, const uint8_t *in, size_t npoly);
 void orc_saber_indcpa_enc(uint8_t *ct, const uint8_t m[32], const uint8_t seed_sp[32], const uint8_t *pk);
 
 /* ---------------- batched / threaded drivers (CPU baseline) ---------------- */
-/* scheme: 0 = Frodo-976-SHAKE, 1 = Saber, 2 = Frodo-976-AES.  op: 0 keygen, 1 enc, 2 dec.
+/* scheme: 0 = Frodo-976-SHAKE, 1 = Saber, 2 = Frodo-976-AES, 3 = LightSaber, 4 = FireSaber.
+ * op: 0 keygen, 1 enc, 2 dec.
  * Runs `n` independent instances over `threads` POSIX threads. */
 int orc_batch(int scheme, int op, size_t n, int threads,
               uint8_t *a, uint8_t *b, const uint8_t *c, const uint8_t *d);

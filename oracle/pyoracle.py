@@ -19,11 +19,15 @@ _lib = None
 FRODO = 0
 SABER = 1
 FRODO_AES = 2
+LIGHTSABER = 3
+FIRESABER = 4
 
 SIZES = {
     FRODO: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
     SABER: dict(pk=992, sk=2304, ct=1088, ss=32, kcoins=96, ecoins=32),
     FRODO_AES: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
+    LIGHTSABER: dict(pk=672, sk=1568, ct=736, ss=32, kcoins=96, ecoins=32),
+    FIRESABER: dict(pk=1312, sk=3040, ct=1472, ss=32, kcoins=96, ecoins=32),
 }
 
 
@@ -54,6 +58,8 @@ def lib():
         _lib.orc_frodo_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
         _lib.orc_frodo_aes_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
         _lib.orc_aes128_ecb.argtypes = [vp, vp, vp, sz]
+        _lib.orc_saber_l_gen_matrix.argtypes = [ctypes.c_int, vp, vp]
+        _lib.orc_saber_l_gen_secret.argtypes = [ctypes.c_int, vp, vp]
         _lib.orc_frodo_pack16.argtypes = [vp, vp, sz]
         _lib.orc_frodo_unpack16.argtypes = [vp, vp, sz]
         _lib.orc_frodo_encode.argtypes = [vp, vp]
@@ -176,6 +182,18 @@ def aes128_ecb(key: bytes, data: bytes) -> bytes:
     k = _u8(key); d = _u8(data); out = np.zeros(len(data), np.uint8)
     lib().orc_aes128_ecb(_p(k), _p(d), _p(out), len(data) // 16)
     return out.tobytes()
+
+
+def saber_l_gen_matrix(l: int, seed: bytes) -> np.ndarray:
+    a = np.zeros(l * l * 256, np.uint16); sd = _u8(seed)
+    lib().orc_saber_l_gen_matrix(l, _p(a), _p(sd))
+    return a
+
+
+def saber_l_gen_secret(l: int, seed: bytes) -> np.ndarray:
+    s = np.zeros(l * 256, np.uint16); sd = _u8(seed)
+    lib().orc_saber_l_gen_secret(l, _p(s), _p(sd))
+    return s
 
 
 def frodo_pack16(v: np.ndarray) -> bytes:
