@@ -180,7 +180,7 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
         p.dec(scheme, ss2, ct, sk, after=t1)
         return p.build()
 
-    key = f"{'frodo976' if scheme == 0 else 'saber'}|kg-enc-dec|n={n}|share={int(share_matrix)}"
+    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}"
     if schedule:
         best, tried = schedule, {}
     else:
@@ -414,6 +414,15 @@ def main():
         progress("frodo976 encaps + decaps against one key (A expanded once per 4 instances)")
         result["frodo976_shared_key"] = run_shared_key(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps,
                                                        0x44504B454D000006)
+    for name, sc in (("lightsaber", D.LIGHTSABER), ("firesaber", D.FIRESABER)):
+        progress(f"{name} batch {args.saber_batch}: autotuned graph steps")
+        fam = run_plan_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup, 0x44504B454D000008 + sc,
+                                tune_budget_s=20.0, schedule=sched)
+        result[name] = {"workload": f"{name} batch {args.saber_batch}: keygen+encaps+decaps per step",
+                        "ops_per_s": round(fam["ops_per_s"], 1),
+                        "ms_per_step": round(fam["wall_s"] * 1e3 / args.steps, 4),
+                        "roundtrip_ok": fam["roundtrip_ok"],
+                        "schedule": {"nstreams": fam["schedule"]["nstreams"], "chunks": fam["schedule"]["chunks"]}}
     progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
     shared = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                tune_budget_s=20.0, share_matrix=True, schedule=sched)

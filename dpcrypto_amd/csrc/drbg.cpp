@@ -173,13 +173,13 @@ int dpkem_kat_seeds(size_t n, uint8_t* seeds) {
 }
 
 int dpkem_kat_coins(int scheme, size_t n, const uint8_t* seeds, uint8_t* kcoins, uint8_t* ecoins) {
-  if ((scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER && scheme != DPKEM_FRODO976_AES) ||
-      (n && (!seeds || !kcoins || !ecoins)))
+  if (scheme < DPKEM_FRODO976 || scheme > DPKEM_FIRESABER || (n && (!seeds || !kcoins || !ecoins)))
     return DPKEM_EINVAL;
+  const bool frodo = scheme == DPKEM_FRODO976 || scheme == DPKEM_FRODO976_AES;
   for (size_t i = 0; i < n; ++i) {
     dpkem_drbg d;
     dpkem_drbg_init(&d, seeds + 48 * i, nullptr);
-    if (scheme != DPKEM_SABER) {
+    if (frodo) {
       // keypair: randombytes(s || seedSE || z, 24 + 24 + 16); encaps: randombytes(mu, 24)
       dpkem_drbg_bytes(&d, kcoins + 64 * i, 64);
       dpkem_drbg_bytes(&d, ecoins + 24 * i, 24);

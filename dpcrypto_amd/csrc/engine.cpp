@@ -25,15 +25,21 @@ struct SchemeSizes {
   size_t pk, sk, ct, ss, kcoins, ecoins;
 };
 
-const SchemeSizes kSizes[3] = {
+const SchemeSizes kSizes[5] = {
     {15632, 31296, 15744, 24, 64, 24},  // FrodoKEM-976-SHAKE
     {992, 2304, 1088, 32, 96, 32},      // Saber
     {15632, 31296, 15744, 24, 64, 24},  // FrodoKEM-976-AES
+    {672, 1568, 736, 32, 96, 32},       // LightSaber
+    {1312, 3040, 1472, 32, 96, 32},     // FireSaber
 };
 
-bool valid_scheme(int scheme) {
-  return scheme == DPKEM_FRODO976 || scheme == DPKEM_SABER || scheme == DPKEM_FRODO976_AES;
+bool valid_scheme(int scheme) { return scheme >= DPKEM_FRODO976 && scheme <= DPKEM_FIRESABER; }
+bool saber_family(int scheme) {
+  return scheme == DPKEM_SABER || scheme == DPKEM_LIGHTSABER || scheme == DPKEM_FIRESABER;
 }
+int saber_rank(int scheme) { return scheme == DPKEM_LIGHTSABER ? 2 : scheme == DPKEM_FIRESABER ? 4 : 3; }
+// schemes with no plain (one-workgroup-per-KEM) comparator form
+bool batched_only(int scheme) { return scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER; }
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
@@ -90,7 +96,7 @@ int record_hip_error(dpkem_engine* e, hipError_t err, const char* what) { return
 // Bytes of device scratch one instance needs in the given form, excluding staged I/O.
 size_t scratch_per_inst(int scheme, int form) {
   if (form == DPKEM_FORM_PLAIN) return align_up(plain_scratch_per_inst(scheme));
-  return scheme == DPKEM_SABER ? align_up(kSaberScratchPerInst) : align_up(kFrodoScratchPerInst);
+  return saber_family(scheme) ? align_up(saber_scratch_per_inst(saber_rank(scheme))) : align_up(kFrodoScratchPerInst);
 }
 
 // I/O byte sizes per instance: two outputs and two inputs (0 if unused).
@@ -114,15 +120,17 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     else plain_frodo_launch(st, op, n, o0, o1, i0, i1, scratch);
     return;
   }
-  if (scheme == DPKEM_SABER) {
-    uint8_t* araw = c.take((size_t)n * 3744);
-    uint8_t* sraw = c.take((size_t)n * 768);
+  if (saber_family(scheme)) {
+    const int l = saber_rank(scheme);
+    const int mu = l == 2 ? 10 : l == 3 ? 8 : 6;
+    uint8_t* araw = c.take((size_t)n * 416 * l * l);  // GenMatrix streams
+    uint8_t* sraw = c.take((size_t)n * 32 * l * mu);  // GenSecret streams
     uint8_t* mbuf = c.take((size_t)n * 32);
     uint8_t* krbuf = c.take((size_t)n * 64);
     uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
-    if (op == kKeygen) saber_keygen_launch(st, n, i0, o0, o1, araw, sraw);
-    else if (op == kEnc) saber_enc_launch(st, n, i0, i1, o0, o1, araw, sraw, mbuf, krbuf);
-    else saber_dec_launch(st, n, i0, i1, o0, araw, sraw, mbuf, krbuf, fail);
+    if (op == kKeygen) saber_keygen_launch(st, n, i0, o0, o1, araw, sraw, l);
+    else if (op == kEnc) saber_enc_launch(st, n, i0, i1, o0, o1, araw, sraw, mbuf, krbuf, l);
+    else saber_dec_launch(st, n, i0, i1, o0, araw, sraw, mbuf, krbuf, fail, l);
   } else {
     uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
     uint32_t* e8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
@@ -193,8 +201,8 @@ void launch_chunk_shared(hipStream_t st, int scheme, int op, int n, uint8_t* o0,
 int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
            const uint8_t* i1, int mem, int shared_in = -1) {
   if (!e || !valid_scheme(scheme)) return DPKEM_EINVAL;
-  if (scheme == DPKEM_FRODO976_AES && e->form != DPKEM_FORM_BATCHED) {
-    snprintf(e->last_err, sizeof e->last_err, "FrodoKEM-976-AES runs in the batched form only");
+  if (batched_only(scheme) && e->form != DPKEM_FORM_BATCHED) {
+    snprintf(e->last_err, sizeof e->last_err, "this parameter set runs in the batched form only");
     return DPKEM_EINVAL;
   }
   if (mem != DPKEM_MEM_HOST && mem != DPKEM_MEM_DEVICE) return DPKEM_EINVAL;
@@ -489,6 +497,24 @@ int dpkem_frodo976aes_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const u
 }
 int dpkem_frodo976aes_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
   return wrapped(DPKEM_FRODO976_AES, kDec, n, ss, nullptr, ct, sk);
+}
+int dpkem_lightsaber_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
+  return wrapped(DPKEM_LIGHTSABER, kKeygen, n, pk, sk, nullptr, nullptr);
+}
+int dpkem_lightsaber_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  return wrapped(DPKEM_LIGHTSABER, kEnc, n, ct, ss, pk, nullptr);
+}
+int dpkem_lightsaber_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  return wrapped(DPKEM_LIGHTSABER, kDec, n, ss, nullptr, ct, sk);
+}
+int dpkem_firesaber_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
+  return wrapped(DPKEM_FIRESABER, kKeygen, n, pk, sk, nullptr, nullptr);
+}
+int dpkem_firesaber_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  return wrapped(DPKEM_FIRESABER, kEnc, n, ct, ss, pk, nullptr);
+}
+int dpkem_firesaber_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  return wrapped(DPKEM_FIRESABER, kDec, n, ss, nullptr, ct, sk);
 }
 int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
   return wrapped(DPKEM_SABER, kKeygen, n, pk, sk, nullptr, nullptr);

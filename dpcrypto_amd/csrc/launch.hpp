@@ -7,12 +7,15 @@
 namespace dpkem {
 
 // ---- batched form (saber.hip, frodo.hip)
+// Saber family: l = 2 (LightSaber), 3 (Saber), 4 (FireSaber); araw / sraw
+// hold saber_scratch_per_inst(l) bytes per instance with the rest.
+size_t saber_scratch_per_inst(int l);
 void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
-                         uint8_t* sraw);
+                         uint8_t* sraw, int l = 3);
 void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf);
+                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l = 3);
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
-                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail);
+                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l = 3);
 
 // dst[i] = src for n rows of `bytes` bytes (a multiple of 4).
 void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t bytes, uint8_t* dst);

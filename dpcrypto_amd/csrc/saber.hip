@@ -1,21 +1,24 @@
-// Saber l=3 batched kernels for gfx950.
+// Saber-family batched kernels for gfx950: LightSaber (l = 2), Saber (l = 3)
+// and FireSaber (l = 4); every kernel is a template over the rank l.
 //
 // Reference path replaced (benlwk/DPCrypto SABER-GPU/):
 //   hashing      sha3_256_gpu / sha3_512_gpu / GenSecret_gpu (<<<1,BATCH>>>)
 //                and the 25-thread shake128_gpu GenMatrix   src/poly.cu:897-1093
+//   CBD          cbd_gpu (mu = 10 / 8 / 6 branches)          src/poly.cu:815-891
 //   products     DoDP4Av7 / DoDP4Av7_inner (+ operand repackers)  src/poly.cu:14-206
 //   rounding     post_process{,2,3}                          src/poly.cu:360-382
 //   packers      POLVECp2BS / BS2POLVECp / BS2POLVECq / POLT2BS / msg  src/pack.cu:8-211
 //   FO glue      copysk / verify_gpu / cmov_gpu              src/pack.cu:215-253
 //   drivers      saber_enc / saber_dec                       src/cuda_kernel.cu:19-299
+//   parameters   LightSaber / Saber / FireSaber              include/params.h:10-19
 // Keygen, absent from the reference, follows the published scheme.
 //
 // Design: each op is three launches.
 //   hash1  lane-pair Keccak (two lanes per state, keccak2.hpp), split in two
-//          waves per 32 instances so GenMatrix (23 permutations) runs beside
-//          the message / secret chain;
-//   poly   one workgroup of 128 threads per instance: unpacking, CBD, the
-//          negacyclic products as LDS-staged Toeplitz dot products
+//          waves per 32 instances so GenMatrix (10 / 23 / 40 permutations
+//          for l = 2 / 3 / 4) runs beside the message / secret chain;
+//   poly   one workgroup of 32 (l + 1) threads per instance: unpacking, CBD,
+//          the negacyclic products as LDS-staged Toeplitz dot products
 //          (v_dot2_u32_u16, 8 outputs per thread), rounding and packing;
 //   hash2  lane-pair closing hashes (H(pk) or H(ct) and K).
 #include "common.hpp"
@@ -24,19 +27,40 @@
 #include "keccak2.hpp"
 #include "prof.hpp"
 #include "saber_poly.hpp"
+#include "launch.hpp"
 
 #include <algorithm>
 
 namespace dpkem {
 
-constexpr int kSabGenMatrixWords = kSaberGenMatrixBytes / 8;  // 468
-constexpr int kSabGenSecretWords = kSaberGenSecretBytes / 8;  // 96
+// Sizes of the Saber parameter set of rank L (mu, eT as published).
+template <int L>
+struct SaberP {
+  static_assert(L >= 2 && L <= 4, "LightSaber, Saber, FireSaber");
+  static constexpr int mu = L == 2 ? 10 : L == 3 ? 8 : 6;
+  static constexpr int et = L == 2 ? 3 : L == 3 ? 4 : 6;
+  static constexpr uint32_t h2 = 256 - (1u << (10 - et - 1)) + 4;
+  static constexpr int vecq = 416 * L;  // POLVECq bytes (13 bits)
+  static constexpr int vecp = 320 * L;  // POLVECp bytes (10 bits)
+  static constexpr int pk = vecp + 32;
+  static constexpr int sk = vecq + pk + 64;
+  static constexpr int ct = vecp + 32 * et;
+  static constexpr int matrix_bytes = L * vecq;   // GenMatrix stream
+  static constexpr int secret_bytes = 32 * L * mu;  // GenSecret stream (256 mu / 8 per polynomial)
+  static constexpr int threads = 32 * (L + 1);    // product workgroup
+};
+static_assert(SaberP<3>::pk == kSaberPk && SaberP<3>::sk == kSaberSk && SaberP<3>::ct == kSaberCt, "Saber sizes");
+static_assert(SaberP<3>::matrix_bytes == kSaberGenMatrixBytes && SaberP<3>::secret_bytes == kSaberGenSecretBytes,
+              "Saber streams");
+static_assert(SaberP<2>::pk == 672 && SaberP<2>::sk == 1568 && SaberP<2>::ct == 736, "LightSaber sizes");
+static_assert(SaberP<4>::pk == 1312 && SaberP<4>::sk == 3040 && SaberP<4>::ct == 1472, "FireSaber sizes");
 
 // ----------------------------------------------------------------- hash1
 // Lane-pair sponges (keccak2.hpp): lanes 2k / 2k+1 hold the even / odd bit
 // halves of instance k's state.  blockDim = 128: wave 0 runs role 0 and wave
-// 1 role 1 for the same 32 instances.  GenMatrix / GenSecret streams leave in
-// the sliced layout (half of word w at u32 index 2w + p).
+// 1 role 1 for the same 32 instances (role_sel >= 0: that role only, 64
+// threads).  GenMatrix / GenSecret streams leave in the sliced layout (half of
+// word w at u32 index 2w + p).
 
 using k2::absorb2_global;
 using k2::absorb2_halves;
@@ -44,17 +68,19 @@ using k2::absorb2_regs;
 using k2::squeeze2_sliced;
 using k2::store2_words;
 
-#define SAB_PAIR_SETUP(PER_BLOCK)                        \
+#define SAB_PAIR_SETUP                                   \
   const int lane = threadIdx.x & 63;                     \
   const uint32_t p = lane & 1;                           \
   const int inst = blockIdx.x * 32 + (lane >> 1);        \
   if (inst >= n) return;
 
+template <int L>
 __global__ void __launch_bounds__(128) sab_keygen_hash1(int role_sel, int n, const uint8_t* __restrict__ coins,
                                                         uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                         uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
+  using P = SaberP<L>;
   const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
-  SAB_PAIR_SETUP(32)
+  SAB_PAIR_SETUP
   const uint8_t* c = coins + (size_t)inst * kSaberKeygenCoins;
   uint32_t a[25];
   if (role == 0) {
@@ -65,40 +91,42 @@ __global__ void __launch_bounds__(128) sab_keygen_hash1(int role_sel, int n, con
     uint32_t h[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) h[i] = a[i];
-    store2_words<4>(h, p, pk + (size_t)inst * kSaberPk + kSaberPolyVecCompBytes);
-    store2_words<4>(h, p, sk + (size_t)inst * kSaberSk + kSaberPolyVecBytes + kSaberPolyVecCompBytes);
-    absorb2_halves<kRateShake128, 4>(a, p, h, kDsShake);  // GenMatrix = SHAKE128(seedA, 3744)
-    squeeze2_sliced<kRateShake128, kSabGenMatrixWords>(
-        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * kSaberGenMatrixBytes));
+    store2_words<4>(h, p, pk + (size_t)inst * P::pk + P::vecp);
+    store2_words<4>(h, p, sk + (size_t)inst * P::sk + P::vecq + P::vecp);
+    absorb2_halves<kRateShake128, 4>(a, p, h, kDsShake);  // GenMatrix = SHAKE128(seedA)
+    squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
+        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * P::matrix_bytes));
   } else {
     uint64_t m[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) m[i] = ld64(c + 32 + 8 * i);
     absorb2_regs<kRateShake128, 4>(a, p, m, kDsShake);  // GenSecret stream
-    squeeze2_sliced<kRateShake128, kSabGenSecretWords>(
-        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * kSaberGenSecretBytes));
-    uint32_t* z = reinterpret_cast<uint32_t*>(sk + (size_t)inst * kSaberSk + kSaberSk - 32);
+    squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
+        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
+    uint32_t* z = reinterpret_cast<uint32_t*>(sk + (size_t)inst * P::sk + P::sk - 32);
     const uint32_t* zc = reinterpret_cast<const uint32_t*>(c + 64);
 #pragma unroll
     for (int i = 0; i < 4; ++i) z[2 * i + p] = zc[2 * i + p];
   }
 }
 
+template <int L>
 __global__ void __launch_bounds__(128) sab_enc_hash1(int role_sel, int n, const uint8_t* __restrict__ pk,
                                                      const uint8_t* __restrict__ coins, uint8_t* __restrict__ araw,
                                                      uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
                                                      uint8_t* __restrict__ krbuf) {
+  using P = SaberP<L>;
   const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
-  SAB_PAIR_SETUP(32)
-  const uint8_t* pkp = pk + (size_t)inst * kSaberPk;
+  SAB_PAIR_SETUP
+  const uint8_t* pkp = pk + (size_t)inst * P::pk;
   uint32_t a[25];
   if (role == 0) {
     uint64_t seed[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) seed[i] = ld64(pkp + kSaberPolyVecCompBytes + 8 * i);
+    for (int i = 0; i < 4; ++i) seed[i] = ld64(pkp + P::vecp + 8 * i);
     absorb2_regs<kRateShake128, 4>(a, p, seed, kDsShake);
-    squeeze2_sliced<kRateShake128, kSabGenMatrixWords>(
-        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * kSaberGenMatrixBytes));
+    squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
+        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * P::matrix_bytes));
   } else {
     uint64_t raw[4];
 #pragma unroll
@@ -107,7 +135,7 @@ __global__ void __launch_bounds__(128) sab_enc_hash1(int role_sel, int n, const 
     absorb2_regs<kRateSha3_256, 4>(a, p, raw, kDsSha3);  // m = H(m_raw)
 #pragma unroll
     for (int i = 0; i < 4; ++i) g[i] = a[i];
-    absorb2_global<kRateSha3_256, kSaberPk / 8>(a, p, pkp, kDsSha3);  // H(pk)
+    absorb2_global<kRateSha3_256, P::pk / 8>(a, p, pkp, kDsSha3);  // H(pk)
 #pragma unroll
     for (int i = 0; i < 4; ++i) g[4 + i] = a[i];
     store2_words<4>(g, p, mbuf + (size_t)inst * 32);
@@ -117,31 +145,33 @@ __global__ void __launch_bounds__(128) sab_enc_hash1(int role_sel, int n, const 
     for (int i = 0; i < 8; ++i) kr[i] = a[i];
     store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
     absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);  // GenSecret(kr[32:64])
-    squeeze2_sliced<kRateShake128, kSabGenSecretWords>(
-        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * kSaberGenSecretBytes));
+    squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
+        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
   }
 }
 
+template <int L>
 __global__ void __launch_bounds__(128) sab_dec_hash1(int role_sel, int n, const uint8_t* __restrict__ sk,
                                                      const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
                                                      uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
+  using P = SaberP<L>;
   const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
-  SAB_PAIR_SETUP(32)
-  const uint8_t* s = sk + (size_t)inst * kSaberSk;
+  SAB_PAIR_SETUP
+  const uint8_t* s = sk + (size_t)inst * P::sk;
   uint32_t a[25];
   if (role == 0) {
     uint64_t seed[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) seed[i] = ld64(s + kSaberPolyVecBytes + kSaberPolyVecCompBytes + 8 * i);
+    for (int i = 0; i < 4; ++i) seed[i] = ld64(s + P::vecq + P::vecp + 8 * i);
     absorb2_regs<kRateShake128, 4>(a, p, seed, kDsShake);
-    squeeze2_sliced<kRateShake128, kSabGenMatrixWords>(
-        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * kSaberGenMatrixBytes));
+    squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
+        a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * P::matrix_bytes));
   } else {
     uint64_t buf[8];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       buf[i] = ld64(mbuf + (size_t)inst * 32 + 8 * i);  // m'
-      buf[4 + i] = ld64(s + kSaberSk - 64 + 8 * i);     // H(pk) from sk
+      buf[4 + i] = ld64(s + P::sk - 64 + 8 * i);        // H(pk) from sk
     }
     absorb2_regs<kRateSha3_512, 8>(a, p, buf, kDsSha3);
     uint32_t kr[8];
@@ -149,28 +179,32 @@ __global__ void __launch_bounds__(128) sab_dec_hash1(int role_sel, int n, const 
     for (int i = 0; i < 8; ++i) kr[i] = a[i];
     store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
     absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
-    squeeze2_sliced<kRateShake128, kSabGenSecretWords>(
-        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * kSaberGenSecretBytes));
+    squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
+        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
   }
 }
 
 // ----------------------------------------------------------------- hash2
 
+template <int L>
 __global__ void __launch_bounds__(64) sab_keygen_hash2(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
-  SAB_PAIR_SETUP(32)
+  using P = SaberP<L>;
+  SAB_PAIR_SETUP
   uint32_t a[25];
-  absorb2_global<kRateSha3_256, kSaberPk / 8>(a, p, pk + (size_t)inst * kSaberPk, kDsSha3);
-  store2_words<4>(a, p, sk + (size_t)inst * kSaberSk + kSaberSk - 64);
+  absorb2_global<kRateSha3_256, P::pk / 8>(a, p, pk + (size_t)inst * P::pk, kDsSha3);
+  store2_words<4>(a, p, sk + (size_t)inst * P::sk + P::sk - 64);
 }
 
 // K = SHA3-256(kr[0:32] || H(ct)); for decaps kr[0:32] is replaced by z when
 // the re-encryption differs (constant-time select on the mismatch flag).
+template <int L>
 __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ krbuf,
                                                 const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
                                                 uint8_t* __restrict__ ss) {
-  SAB_PAIR_SETUP(32)
+  using P = SaberP<L>;
+  SAB_PAIR_SETUP
   uint32_t a[25], m[8];
-  absorb2_global<kRateSha3_256, kSaberCt / 8>(a, p, ct + (size_t)inst * kSaberCt, kDsSha3);
+  absorb2_global<kRateSha3_256, P::ct / 8>(a, p, ct + (size_t)inst * P::ct, kDsSha3);
 #pragma unroll
   for (int i = 0; i < 4; ++i) m[4 + i] = a[i];
   uint64_t mask = 0;
@@ -178,7 +212,7 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint64_t k = ld64(krbuf + (size_t)inst * 64 + 8 * i);
-    const uint64_t z = sk ? ld64(sk + (size_t)inst * kSaberSk + kSaberSk - 32 + 8 * i) : 0;
+    const uint64_t z = sk ? ld64(sk + (size_t)inst * P::sk + P::sk - 32 + 8 * i) : 0;
     m[i] = k2::deint((k & ~mask) | (z & mask), p);
   }
   absorb2_halves<kRateSha3_256, 8>(a, p, m, kDsSha3);
@@ -191,8 +225,8 @@ enum SabMode { kSabKeygen = 0, kSabEnc = 1, kSabReenc = 2 };
 
 struct SabPolyArgs {
   int n;
-  const uint8_t* araw;   // GenMatrix stream, 3744 B / instance
-  const uint8_t* sraw;   // GenSecret stream, 768 B / instance
+  const uint8_t* araw;   // GenMatrix stream per instance
+  const uint8_t* sraw;   // GenSecret stream per instance
   const uint8_t* pk;     // keygen: out (b part); enc: in; reenc: pk inside sk
   size_t pk_stride;      // bytes between instances' pk
   const uint8_t* mbuf;   // enc / reenc: message, 32 B / instance
@@ -204,46 +238,64 @@ struct SabPolyArgs {
 
 constexpr int kARow = 264;  // u16 row stride of the plain A / b polynomials (spreads LDS banks)
 
-// The secret (s or s', small) is the Toeplitz-staged factor: three staged
-// polynomials serve all nine MatVec products and the three inner-product
+// The secret (s or s', small) is the Toeplitz-staged factor: L staged
+// polynomials serve all L^2 MatVec products and the L inner-product
 // products; A and b are read as plain rows, each 16-byte read broadcast to
 // the 32 threads of an output row.
+template <int L>
 struct SabPolyLds {
-  uint16_t Ts[3 * kToepStride];      // Toeplitz form of s / s'
-  uint16_t Ap[12 * kARow];           // A (9 polynomials) then b (3)
-  alignas(8) uint32_t stage[kSaberGenMatrixBytes / 4];
-  uint8_t outb[kSaberCt + 64];
+  uint16_t Ts[L * kToepStride];           // Toeplitz form of s / s'
+  uint16_t Ap[(L * L + L) * kARow];       // A (L^2 polynomials) then b (L)
+  alignas(8) uint32_t stage[SaberP<L>::matrix_bytes / 4];
+  alignas(8) uint32_t sstage[SaberP<L>::secret_bytes / 4];
+  uint8_t outb[SaberP<L>::ct + 64];
   uint32_t diff;
 };
 
-template <int MODE>
-__global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
-  __shared__ __align__(16) SabPolyLds L;
+// Byte-wise little-endian load of up to 8 bytes from LDS.
+__device__ __forceinline__ uint64_t lds_le(const uint8_t* b, int nbytes) {
+  uint64_t v = 0;
+  for (int i = 0; i < nbytes; ++i) v |= (uint64_t)b[i] << (8 * i);
+  return v;
+}
+
+template <int MODE, int L>
+__global__ void __launch_bounds__(SaberP<L>::threads) sab_poly(SabPolyArgs a) {
+  using P = SaberP<L>;
+  constexpr int NT = P::threads;
+  __shared__ __align__(16) SabPolyLds<L> S;
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
   if (inst >= a.n) return;
 
-  // 1. rebuild the GenMatrix bytes (lane-pair sliced stream) in LDS; expand
-  //    GenSecret with CBD (mu = 8) straight into the Toeplitz form of s
-  const uint32_t* ar = reinterpret_cast<const uint32_t*>(a.araw + (size_t)inst * kSaberGenMatrixBytes);
-  for (int w = tid; w < kSabGenMatrixWords; w += 128)
-    reinterpret_cast<uint64_t*>(L.stage)[w] = k2::sliced_word(ar, w);
-  const uint32_t* sr = reinterpret_cast<const uint32_t*>(a.sraw + (size_t)inst * kSaberGenSecretBytes);
-  for (int w = tid; w < kSabGenSecretWords; w += 128) {
-    const uint64_t v = k2::sliced_word(sr, w);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint32_t byte = (uint32_t)(v >> (8 * b)) & 0xFF;
-      const int c = 8 * w + b;
-      toep_put(L.Ts, c >> 8, c & 255, (uint16_t)(popc(byte & 0xF) - popc(byte >> 4)));
-    }
-  }
-  if (tid == 0) L.diff = 0;
+  // 1. rebuild the GenMatrix and GenSecret bytes (lane-pair sliced streams) in LDS
+  const uint32_t* ar = reinterpret_cast<const uint32_t*>(a.araw + (size_t)inst * P::matrix_bytes);
+  for (int w = tid; w < P::matrix_bytes / 8; w += NT)
+    reinterpret_cast<uint64_t*>(S.stage)[w] = k2::sliced_word(ar, w);
+  const uint32_t* sr = reinterpret_cast<const uint32_t*>(a.sraw + (size_t)inst * P::secret_bytes);
+  for (int w = tid; w < P::secret_bytes / 8; w += NT)
+    reinterpret_cast<uint64_t*>(S.sstage)[w] = k2::sliced_word(sr, w);
+  if (tid == 0) S.diff = 0;
   __syncthreads();
 
-  // 2. unpack A (288 groups of 8) and b (192 groups of 4, from pk) as plain rows
-  const uint8_t* sb = reinterpret_cast<const uint8_t*>(L.stage);
-  for (int g = tid; g < 288; g += 128) {
+  // 2. CBD (mu bits per coefficient: HW(low mu/2) - HW(high mu/2)) straight
+  //    into the Toeplitz form of s; A (groups of 8 13-bit values) and b (groups
+  //    of 4 10-bit values, from pk) unpacked as plain rows
+  const uint8_t* ssb = reinterpret_cast<const uint8_t*>(S.sstage);
+  #pragma unroll 1
+  for (int g = tid; g < 64 * L; g += NT) {  // 4 coefficients from mu / 2 bytes
+    const uint64_t v = lds_le(ssb + g * (P::mu / 2), P::mu / 2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t x = (uint32_t)(v >> (P::mu * k)) & ((1u << P::mu) - 1);
+      const int c = 4 * g + k;
+      const int half = P::mu / 2;
+      toep_put(S.Ts, c >> 8, c & 255, (uint16_t)(popc(x & ((1u << half) - 1)) - popc(x >> half)));
+    }
+  }
+  const uint8_t* sb = reinterpret_cast<const uint8_t*>(S.stage);
+  #pragma unroll 1
+  for (int g = tid; g < 32 * L * L; g += NT) {
     uint16_t v[8];
     unpack13_group(v, sb + 13 * g);
     uint4 q;
@@ -251,11 +303,12 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
     q.y = v[2] | ((uint32_t)v[3] << 16);
     q.z = v[4] | ((uint32_t)v[5] << 16);
     q.w = v[6] | ((uint32_t)v[7] << 16);
-    *reinterpret_cast<uint4*>(L.Ap + (g >> 5) * kARow + (g & 31) * 8) = q;
+    *reinterpret_cast<uint4*>(S.Ap + (g >> 5) * kARow + (g & 31) * 8) = q;
   }
   if (MODE != kSabKeygen) {
     const uint8_t* pkb = a.pk + (size_t)inst * a.pk_stride;
-    for (int g = tid; g < 192; g += 128) {
+    #pragma unroll 1
+    for (int g = tid; g < 64 * L; g += NT) {
       uint8_t bytes[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i) bytes[i] = pkb[5 * g + i];
@@ -264,24 +317,25 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
       uint2 q;
       q.x = v[0] | ((uint32_t)v[1] << 16);
       q.y = v[2] | ((uint32_t)v[3] << 16);
-      *reinterpret_cast<uint2*>(L.Ap + (9 + (g >> 6)) * kARow + (g & 63) * 4) = q;
+      *reinterpret_cast<uint2*>(S.Ap + (L * L + (g >> 6)) * kARow + (g & 63) * 4) = q;
     }
   }
   __syncthreads();
 
-  // 3. products: threads 0..95 -> b' row i = tid/32 (A or A^T); 96..127 -> v' = b . s'
+  // 3. products: threads 0..32L-1 -> b' row i = tid/32 (A or A^T); the last 32 -> v' = b . s'
   const int t0 = 8 * (tid & 31);
-  const int row = tid >> 5;  // 0..2 matrix rows, 3 inner product
+  const int row = tid >> 5;  // 0..L-1 matrix rows, L the inner product
   uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (MODE != kSabKeygen || row < 3) {
-    for (int j = 0; j < 3; ++j) {
-      const int P = row < 3 ? (MODE == kSabKeygen ? j * 3 + row : row * 3 + j) : 9 + j;
-      toep_dot(acc, L.Ts, j, L.Ap + P * kARow, t0);
+  if (MODE != kSabKeygen || row < L) {
+#pragma unroll 1
+    for (int j = 0; j < L; ++j) {  // each toep_dot is already 1024 unrolled dot2
+      const int Pi = row < L ? (MODE == kSabKeygen ? j * L + row : row * L + j) : L * L + j;
+      toep_dot(acc, S.Ts, j, S.Ap + Pi * kARow, t0);
     }
   }
 
   // 4. rounding + packing into the LDS output image
-  if (row < 3) {
+  if (row < L) {
     // b' = (x + h1) >> 3 -> 10 bits, 8 coefficients -> 10 bytes
     uint64_t lo = 0;
     uint32_t hi = 0;
@@ -292,42 +346,44 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
       if (bit < 64) lo |= v << bit;
       if (bit + 10 > 64) hi |= (uint32_t)(bit >= 64 ? v << (bit - 64) : v >> (64 - bit));
     }
-    uint8_t* o = L.outb + row * 320 + 10 * (tid & 31);
+    uint8_t* o = S.outb + row * 320 + 10 * (tid & 31);
 #pragma unroll
     for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
     o[8] = (uint8_t)hi;
     o[9] = (uint8_t)(hi >> 8);
   } else if (MODE != kSabKeygen) {
-    // c_m = (v' - m * 2^9 + h1) >> 6 -> 4 bits; byte g of m holds the bits of t0..t0+7
+    // c_m = (v' - m 2^9 + h1) >> (10 - eT) -> eT bits; byte g of m holds the bits of t0..t0+7
     const uint32_t mbyte = a.mbuf[(size_t)inst * 32 + (tid & 31)];
-    uint32_t packed = 0;
+    uint64_t packed = 0;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
       const uint32_t mb = (mbyte >> d) & 1;
-      const uint32_t c = ((acc[d] - (mb << 9) + 4u) >> 6) & 0xF;
-      packed |= c << (4 * d);
+      const uint64_t c = ((acc[d] - (mb << 9) + 4u) >> (10 - P::et)) & ((1u << P::et) - 1);
+      packed |= c << (P::et * d);
     }
-    *reinterpret_cast<uint32_t*>(L.outb + kSaberPolyVecCompBytes + 4 * (tid & 31)) = packed;
+    uint8_t* o = S.outb + P::vecp + P::et * (tid & 31);
+#pragma unroll
+    for (int i = 0; i < P::et; ++i) o[i] = (uint8_t)(packed >> (8 * i));
   }
   __syncthreads();
 
   // 5. emit
   if (MODE == kSabKeygen) {
-    uint8_t* pk = a.out + (size_t)inst * kSaberPk;
-    uint8_t* sk = a.sk + (size_t)inst * kSaberSk;
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(L.outb);
-    for (int i = tid; i < kSaberPolyVecCompBytes / 4; i += 128) {
+    uint8_t* pk = a.out + (size_t)inst * P::pk;
+    uint8_t* sk = a.sk + (size_t)inst * P::sk;
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(S.outb);
+    for (int i = tid; i < P::vecp / 4; i += NT) {
       reinterpret_cast<uint32_t*>(pk)[i] = ob[i];
-      reinterpret_cast<uint32_t*>(sk + kSaberPolyVecBytes)[i] = ob[i];
+      reinterpret_cast<uint32_t*>(sk + P::vecq)[i] = ob[i];
     }
-    // sk[0:1248] = POLVECq(s): 96 groups of 8 coefficients -> 13 bytes; s is
+    // sk[0:vecq] = POLVECq(s): 32 L groups of 8 coefficients -> 13 bytes; s is
     // read back from the Toeplitz copy (r[-q] = s[q] at offset 256 - q)
-    if (tid < 96) {
+    if (tid < 32 * L) {
       uint64_t lo = 0, hi = 0;
 #pragma unroll
       for (int d = 0; d < 8; ++d) {
         const int c = 8 * tid + d;
-        const uint64_t v = L.Ts[(c >> 8) * kToepStride + 256 - (c & 255)] & 0x1FFF;
+        const uint64_t v = S.Ts[(c >> 8) * kToepStride + 256 - (c & 255)] & 0x1FFF;
         const int bit = 13 * d;
         if (bit < 64) lo |= v << bit;
         if (bit + 13 > 64) hi |= bit >= 64 ? v << (bit - 64) : v >> (64 - bit);
@@ -339,38 +395,42 @@ __global__ void __launch_bounds__(128) sab_poly(SabPolyArgs a) {
       for (int i = 0; i < 5; ++i) o[8 + i] = (uint8_t)(hi >> (8 * i));
     }
   } else if (MODE == kSabEnc) {
-    uint32_t* ct = reinterpret_cast<uint32_t*>(a.out + (size_t)inst * kSaberCt);
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(L.outb);
-    for (int i = tid; i < kSaberCt / 4; i += 128) ct[i] = ob[i];
+    uint32_t* ct = reinterpret_cast<uint32_t*>(a.out + (size_t)inst * P::ct);
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(S.outb);
+    for (int i = tid; i < P::ct / 4; i += NT) ct[i] = ob[i];
   } else {
-    const uint32_t* ct = reinterpret_cast<const uint32_t*>(a.ct_in + (size_t)inst * kSaberCt);
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(L.outb);
+    const uint32_t* ct = reinterpret_cast<const uint32_t*>(a.ct_in + (size_t)inst * P::ct);
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(S.outb);
     uint32_t d = 0;
-    for (int i = tid; i < kSaberCt / 4; i += 128) d |= ct[i] ^ ob[i];
-    if (d) atomicOr(&L.diff, 1u);
+    for (int i = tid; i < P::ct / 4; i += NT) d |= ct[i] ^ ob[i];
+    atomicOr(&S.diff, d != 0 ? 1u : 0u);
     __syncthreads();
-    if (tid == 0) a.fail[inst] = L.diff;
+    if (tid == 0) a.fail[inst] = S.diff;
   }
 }
 
 // Decaps inner product v = b' . s and message recovery
-// m'_t = ((v + h2 - c_m * 2^6) >> 9) & 1.  Two instances per 64-thread block;
-// s (from sk) is the Toeplitz-staged factor, b' (from ct) the plain one.
+// m'_t = ((v + h2 - c_m 2^(10 - eT)) >> 9) & 1.  Two instances per 64-thread
+// block; s (from sk) is the Toeplitz-staged factor, b' (from ct) the plain one.
+template <int L>
 struct SabInnerLds {
-  uint16_t Ts[2][3 * kToepStride];
-  uint16_t Bp[2][3 * kARow];
+  uint16_t Ts[2][L * kToepStride];
+  uint16_t Bp[2][L * kARow];
 };
 
+template <int L>
 __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __restrict__ ct,
                                                     const uint8_t* __restrict__ sk, uint8_t* __restrict__ mbuf) {
-  __shared__ __align__(16) SabInnerLds L;
+  using P = SaberP<L>;
+  __shared__ __align__(16) SabInnerLds<L> S;
   const int half = threadIdx.x >> 5, t = threadIdx.x & 31;
   const int inst = blockIdx.x * 2 + half;
   const bool live = inst < n;
-  const uint8_t* c = ct + (size_t)(live ? inst : 0) * kSaberCt;
-  const uint8_t* s = sk + (size_t)(live ? inst : 0) * kSaberSk;
+  const uint8_t* c = ct + (size_t)(live ? inst : 0) * P::ct;
+  const uint8_t* s = sk + (size_t)(live ? inst : 0) * P::sk;
   if (live) {
-    for (int g = t; g < 192; g += 32) {  // b' from ct (10-bit)
+    #pragma unroll 1
+    for (int g = t; g < 64 * L; g += 32) {  // b' from ct (10-bit)
       uint8_t bytes[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i) bytes[i] = c[5 * g + i];
@@ -379,29 +439,33 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
       uint2 q;
       q.x = v[0] | ((uint32_t)v[1] << 16);
       q.y = v[2] | ((uint32_t)v[3] << 16);
-      *reinterpret_cast<uint2*>(L.Bp[half] + (g >> 6) * kARow + (g & 63) * 4) = q;
+      *reinterpret_cast<uint2*>(S.Bp[half] + (g >> 6) * kARow + (g & 63) * 4) = q;
     }
-    for (int g = t; g < 96; g += 32) {  // s from sk (13-bit)
+    #pragma unroll 1
+    for (int g = t; g < 32 * L; g += 32) {  // s from sk (13-bit)
       uint8_t bytes[13];
 #pragma unroll
       for (int i = 0; i < 13; ++i) bytes[i] = s[13 * g + i];
       uint16_t v[8];
       unpack13_group(v, bytes);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) toep_put(L.Ts[half], g >> 5, (g & 31) * 8 + i, v[i]);
+      for (int i = 0; i < 8; ++i) toep_put(S.Ts[half], g >> 5, (g & 31) * 8 + i, v[i]);
     }
   }
   __syncthreads();
   if (!live) return;
   uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int t0 = 8 * t;
-  for (int j = 0; j < 3; ++j) toep_dot(acc, L.Ts[half], j, L.Bp[half] + j * kARow, t0);
-  const uint32_t cm = *reinterpret_cast<const uint32_t*>(c + kSaberPolyVecCompBytes + 4 * t);
+#pragma unroll 1
+  for (int j = 0; j < L; ++j) toep_dot(acc, S.Ts[half], j, S.Bp[half] + j * kARow, t0);
+  uint64_t cm = 0;  // eT bits for each of t0..t0+7
+#pragma unroll
+  for (int i = 0; i < P::et; ++i) cm |= (uint64_t)c[P::vecp + P::et * t + i] << (8 * i);
   uint32_t byte = 0;
 #pragma unroll
   for (int d = 0; d < 8; ++d) {
-    const uint32_t cmd = (cm >> (4 * d)) & 0xF;
-    byte |= (((acc[d] + 228u - (cmd << 6)) >> 9) & 1u) << d;
+    const uint32_t cmd = (uint32_t)(cm >> (P::et * d)) & ((1u << P::et) - 1);
+    byte |= (((acc[d] + P::h2 - (cmd << (10 - P::et))) >> 9) & 1u) << d;
   }
   mbuf[(size_t)inst * 32 + t] = (uint8_t)byte;
 }
@@ -411,32 +475,76 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
 
 static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 
-void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
-                         uint8_t* sraw) {
-  if (n <= 0) return;
-  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, coins, pk, sk, araw, sraw);
+// Profiler names: Saber keeps the plain kernel names, the other ranks add ",l<L>".
+template <int L>
+const char* kname(const char* base, const char* l2, const char* l4) {
+  return L == 3 ? base : L == 2 ? l2 : l4;
+}
+#define SAB_NAME(base) kname<L>(base, base ",l2", base ",l4")
+
+template <int L>
+void keygen_l(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw, uint8_t* sraw) {
+  using P = SaberP<L>;
+  DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash1"), sab_keygen_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n,
+               coins, pk, sk, araw, sraw);
   SabPolyArgs a{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), dim3(128), st, a);
-  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
+  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), dim3(P::threads), st, a);
+  DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash2"), sab_keygen_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
+}
+
+template <int L>
+void enc_l(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss, uint8_t* araw,
+           uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf) {
+  using P = SaberP<L>;
+  DPKEM_LAUNCH(SAB_NAME("sab_enc_hash1"), sab_enc_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, pk, coins,
+               araw, sraw, mbuf, krbuf);
+  SabPolyArgs a{n, araw, sraw, pk, (size_t)P::pk, mbuf, ct, nullptr, nullptr, nullptr};
+  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), dim3(P::threads), st, a);
+  DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, nullptr,
+               nullptr, ss);
+}
+
+template <int L>
+void dec_l(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw, uint8_t* sraw,
+           uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail) {
+  using P = SaberP<L>;
+  DPKEM_LAUNCH(SAB_NAME("sab_dec_inner"), sab_dec_inner<L>, dim3(blocks_for(n, 2)), dim3(64), st, n, ct, sk, mbuf);
+  DPKEM_LAUNCH(SAB_NAME("sab_dec_hash1"), sab_dec_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, sk, mbuf,
+               araw, sraw, krbuf);
+  SabPolyArgs a{n, araw, sraw, sk + P::vecq, (size_t)P::sk, mbuf, nullptr, nullptr, ct, fail};
+  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabReenc>"), (sab_poly<kSabReenc, L>), dim3(n), dim3(P::threads), st, a);
+  DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail,
+               ss);
+}
+
+size_t saber_scratch_per_inst(int l) {
+  const size_t m = l == 2 ? SaberP<2>::matrix_bytes : l == 3 ? SaberP<3>::matrix_bytes : SaberP<4>::matrix_bytes;
+  const size_t s = l == 2 ? SaberP<2>::secret_bytes : l == 3 ? SaberP<3>::secret_bytes : SaberP<4>::secret_bytes;
+  return m + s + 32 + 64 + 16;
+}
+
+void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
+                         uint8_t* sraw, int l) {
+  if (n <= 0) return;
+  if (l == 2) keygen_l<2>(st, n, coins, pk, sk, araw, sraw);
+  else if (l == 4) keygen_l<4>(st, n, coins, pk, sk, araw, sraw);
+  else keygen_l<3>(st, n, coins, pk, sk, araw, sraw);
 }
 
 void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf) {
+                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, pk, coins, araw, sraw, mbuf, krbuf);
-  SabPolyArgs a{n, araw, sraw, pk, (size_t)kSaberPk, mbuf, ct, nullptr, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), dim3(128), st, a);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, nullptr, nullptr, ss);
+  if (l == 2) enc_l<2>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf);
+  else if (l == 4) enc_l<4>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf);
+  else enc_l<3>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf);
 }
 
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
-                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail) {
+                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), dim3(64), st, n, ct, sk, mbuf);
-  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, sk, mbuf, araw, sraw, krbuf);
-  SabPolyArgs a{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
-  DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), dim3(128), st, a);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail, ss);
+  if (l == 2) dec_l<2>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail);
+  else if (l == 4) dec_l<4>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail);
+  else dec_l<3>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail);
 }
 
 // dst[i] = src for i < n (bytes per row a multiple of 4).
@@ -456,15 +564,17 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
                      reinterpret_cast<uint32_t*>(dst));
 }
 
-// Chained keygen -> encaps -> decaps of the same n instances on one stream.
-// Encaps reads seedA from the pk keygen just wrote and decaps from the copy
-// in sk, so both GenMatrix streams equal keygen's: the chain expands A once
-// (23 of 98 serial Keccak permutations per instance saved twice) and the
-// encaps / decaps hash kernels run only their message roles.  scratch holds
-// kSaberChainScratchPerInst bytes per instance; the buffers are reused from
-// op to op, which the stream order makes safe.
+// Chained keygen -> encaps -> decaps of the same n Saber (l = 3) instances on
+// one stream.  Encaps reads seedA from the pk keygen just wrote and decaps
+// from the copy in sk, so both GenMatrix streams equal keygen's: the chain
+// expands A once (23 of 98 serial Keccak permutations per instance saved
+// twice) and the encaps / decaps hash kernels run only their message roles.
+// scratch holds kSaberChainScratchPerInst bytes per instance; the buffers are
+// reused from op to op, which the stream order makes safe.
 void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
                         uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+  constexpr int L = 3;
+  using P = SaberP<L>;
   if (n <= 0) return;
   size_t off = 0;
   auto take = [&](size_t per) {
@@ -472,28 +582,28 @@ void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t*
     off += ((size_t)n * per + 255) & ~(size_t)255;
     return q;
   };
-  uint8_t* araw = take(kSaberGenMatrixBytes);
-  uint8_t* sraw = take(kSaberGenSecretBytes);
+  uint8_t* araw = take(P::matrix_bytes);
+  uint8_t* sraw = take(P::secret_bytes);
   uint8_t* mbuf = take(32);
   uint8_t* krbuf = take(64);
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
-  const dim3 g(blocks_for(n, 32)), b64(64), b128(128);
+  const dim3 g(blocks_for(n, 32)), b64(64), bp(P::threads), b128(128);
   // keygen
-  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1, g, b128, st, -1, n, kc, pk, sk, araw, sraw);
+  DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1<L>, g, b128, st, -1, n, kc, pk, sk, araw, sraw);
   SabPolyArgs ak{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabKeygen>", sab_poly<kSabKeygen>, dim3(n), b128, st, ak);
-  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2, g, b64, st, n, pk, sk);
+  DPKEM_LAUNCH("sab_poly<kSabKeygen>", (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
+  DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2<L>, g, b64, st, n, pk, sk);
   // encaps: message role only, A from keygen
-  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1, g, b64, st, 1, n, pk, ec, araw, sraw, mbuf, krbuf);
-  SabPolyArgs ae{n, araw, sraw, pk, (size_t)kSaberPk, mbuf, ct, nullptr, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabEnc>", sab_poly<kSabEnc>, dim3(n), b128, st, ae);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st, n, ct, krbuf, nullptr, nullptr, ss);
+  DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1<L>, g, b64, st, 1, n, pk, ec, araw, sraw, mbuf, krbuf);
+  SabPolyArgs ae{n, araw, sraw, pk, (size_t)P::pk, mbuf, ct, nullptr, nullptr, nullptr};
+  DPKEM_LAUNCH("sab_poly<kSabEnc>", (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2<L>, g, b64, st, n, ct, krbuf, nullptr, nullptr, ss);
   // decaps: message role only, A from keygen
-  DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf);
-  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1, g, b64, st, 1, n, sk, mbuf, araw, sraw, krbuf);
-  SabPolyArgs ad{n, araw, sraw, sk + kSaberPolyVecBytes, (size_t)kSaberSk, mbuf, nullptr, nullptr, ct, fail};
-  DPKEM_LAUNCH("sab_poly<kSabReenc>", sab_poly<kSabReenc>, dim3(n), b128, st, ad);
-  DPKEM_LAUNCH("sab_hash2", sab_hash2, g, b64, st, n, ct, krbuf, sk, fail, ss2);
+  DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner<L>, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf);
+  DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1<L>, g, b64, st, 1, n, sk, mbuf, araw, sraw, krbuf);
+  SabPolyArgs ad{n, araw, sraw, sk + P::vecq, (size_t)P::sk, mbuf, nullptr, nullptr, ct, fail};
+  DPKEM_LAUNCH("sab_poly<kSabReenc>", (sab_poly<kSabReenc, L>), dim3(n), bp, st, ad);
+  DPKEM_LAUNCH("sab_hash2", sab_hash2<L>, g, b64, st, n, ct, krbuf, sk, fail, ss2);
 }
 
 }  // namespace dpkem

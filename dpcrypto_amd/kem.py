@@ -34,11 +34,15 @@ ERRORS = {
 }
 
 FRODO976_AES = 2  # FrodoKEM-976-AES: Frodo-976 with A rows from AES-128 (batched form only)
+LIGHTSABER = 3    # Saber family, rank 2 (batched form only)
+FIRESABER = 4     # Saber family, rank 4 (batched form only)
 
 SIZES = {
     FRODO976: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
     SABER: dict(pk=992, sk=2304, ct=1088, ss=32, kcoins=96, ecoins=32),
     FRODO976_AES: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
+    LIGHTSABER: dict(pk=672, sk=1568, ct=736, ss=32, kcoins=96, ecoins=32),
+    FIRESABER: dict(pk=1312, sk=3040, ct=1472, ss=32, kcoins=96, ecoins=32),
 }
 
 _lib = None

@@ -30,6 +30,10 @@ extern "C" {
                                 AES-128 (named at FrodoKEM976-GPU/include/api.h:12, its AES declared at
                                 FrodoKEM976-GPU/include/gen_a_mat_mul.cuh:6); engine, plans and KAT only,
                                 batched form only */
+#define DPKEM_LIGHTSABER 3   /* LightSaber (l = 2): pk 672, sk 1568, ct 736 B */
+#define DPKEM_FIRESABER 4    /* FireSaber (l = 4): pk 1312, sk 3040, ct 1472 B
+                                (both parameter sets at SABER-GPU/include/params.h:10-19, coins as Saber;
+                                engine, plans and KAT only, batched form only) */
 
 #define DPKEM_MEM_HOST 0   /* pointers are host memory (pageable or pinned) */
 #define DPKEM_MEM_DEVICE 1 /* pointers are device memory on the engine's GPU */
@@ -231,7 +235,7 @@ void dpkem_randombytes_drbg(uint8_t *out, size_t len, void *ctx);
 int dpkem_kat_seeds(size_t n, uint8_t *seeds);
 /* Keygen and encaps coins of n KAT counts: per count, the DRBG seeded with
  * that count's seed, drawn with the scheme's randombytes call pattern
- * (Frodo-976 SHAKE / AES: 64 B, then 24 B; Saber: 3 x 32 B, then 32 B). */
+ * (Frodo-976 SHAKE / AES: 64 B, then 24 B; Saber family: 3 x 32 B, then 32 B). */
 int dpkem_kat_coins(int scheme, size_t n, const uint8_t *seeds, uint8_t *kcoins, uint8_t *ecoins);
 
 /* crypto_kem_{keypair,enc,dec} over arrays of n instances (host memory),
@@ -243,6 +247,12 @@ int dpkem_frodo976_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, cons
 int dpkem_frodo976aes_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
 int dpkem_frodo976aes_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
 int dpkem_frodo976aes_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+int dpkem_lightsaber_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
+int dpkem_lightsaber_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int dpkem_lightsaber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+int dpkem_firesaber_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
+int dpkem_firesaber_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int dpkem_firesaber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
 int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
 int dpkem_saber_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
 int dpkem_saber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);

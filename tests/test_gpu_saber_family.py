@@ -1,0 +1,75 @@
+"""LightSaber (l = 2) and FireSaber (l = 4) on the GPU against the oracle:
+keygen / encaps / decaps at ragged batch sizes, implicit rejection, a
+device-resident batch with a full round trip, KAT coins and shared keys."""
+import numpy as np
+import pytest
+
+import dpcrypto_amd as D
+
+pytestmark = pytest.mark.gpu
+SCHEMES = [D.LIGHTSABER, D.FIRESABER]
+
+
+@pytest.fixture(scope="module")
+def eng():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    e = D.Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("n", [1, 33, 70])
+def test_ops_equal_oracle(eng, oracle, scheme, n):
+    s = D.SIZES[scheme]
+    kc = oracle.prng_bytes(8000 + 10 * scheme + n, n * 96).reshape(n, 96)
+    pk, sk = eng.keypair(scheme, kc)
+    opk, osk = oracle.keypair(scheme, kc, threads=8)
+    assert pk.shape == (n, s["pk"]) and np.array_equal(pk, opk) and np.array_equal(sk, osk)
+    ec = oracle.prng_bytes(8100 + 10 * scheme + n, n * 32).reshape(n, 32)
+    ct, ss = eng.enc(scheme, pk, ec)
+    oct_, oss = oracle.enc(scheme, opk, ec, threads=8)
+    assert np.array_equal(ct, oct_) and np.array_equal(ss, oss)
+    assert np.array_equal(eng.dec(scheme, ct, sk), oss)
+    bad = ct.copy()
+    bad[:, 3] ^= 0x01          # in b'
+    bad[:, -1] ^= 0x80         # in c_m
+    assert np.array_equal(eng.dec(scheme, bad, sk), oracle.dec(scheme, bad, osk, threads=8))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_device_batch_roundtrip(eng, oracle, scheme):
+    import torch
+    n = 2048
+    kc = oracle.prng_bytes(8200 + scheme, n * 96).reshape(n, 96)
+    ec = oracle.prng_bytes(8300 + scheme, n * 32).reshape(n, 32)
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    pk, sk = eng.keypair(scheme, torch.from_numpy(kc).cuda())
+    ct, ss = eng.enc(scheme, pk, torch.from_numpy(ec).cuda())
+    ss2 = eng.dec(scheme, ct, sk)
+    torch.cuda.synchronize()
+    assert torch.equal(ss, ss2)
+    idx = [0, 777, n - 1]
+    opk, _ = oracle.keypair(scheme, kc[idx], threads=3)
+    oct_, oss = oracle.enc(scheme, opk, ec[idx], threads=3)
+    assert np.array_equal(pk[idx].cpu().numpy(), opk)
+    assert np.array_equal(ct[idx].cpu().numpy(), oct_) and np.array_equal(ss[idx].cpu().numpy(), oss)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_kat_and_shared_key(eng, oracle, scheme):
+    from dpcrypto_amd import kat
+    sd = kat.seeds(4)
+    gpu = kat.format_rsp(scheme, sd, *kat.compute(scheme, sd, eng.keypair, eng.enc, eng.dec))
+    ops = (lambda sc, kc: oracle.keypair(sc, kc, threads=4), lambda sc, pk, ec: oracle.enc(sc, pk, ec, threads=4),
+           lambda sc, ct, sk: oracle.dec(sc, ct, sk, threads=4))
+    assert gpu == kat.format_rsp(scheme, sd, *kat.compute(scheme, sd, *ops))
+    kc = oracle.prng_bytes(8400 + scheme, 96).reshape(1, 96)
+    pk, sk = oracle.keypair(scheme, kc)
+    ec = oracle.prng_bytes(8401 + scheme, 9 * 32).reshape(9, 32)
+    ct, ss = eng.enc_shared_key(scheme, pk, ec)
+    oct_, oss = oracle.enc(scheme, np.repeat(pk, 9, axis=0), ec, threads=8)
+    assert np.array_equal(ct, oct_) and np.array_equal(ss, oss)
+    assert np.array_equal(eng.dec_shared_key(scheme, ct, sk), oss)

@@ -55,13 +55,13 @@ def test_every_draw_ends_with_an_update():
     assert kat.Drbg(seed, pers).bytes(32) == kat.Drbg(bytes(a ^ 0xA5 for a in seed)).bytes(32)
 
 
-@pytest.mark.parametrize("scheme", [D.FRODO976, D.SABER])
+@pytest.mark.parametrize("scheme", [D.FRODO976, D.SABER, D.FRODO976_AES, D.LIGHTSABER, D.FIRESABER])
 def test_kat_coins_follow_the_schemes_call_pattern(scheme):
     sd = kat.seeds(3)
     kc, ec = kat.coins(scheme, sd)
     for i in range(3):
         d = kat.Drbg(bytes(sd[i]))
-        if scheme == D.FRODO976:
+        if scheme in (D.FRODO976, D.FRODO976_AES):
             want_k, want_e = d.bytes(64), d.bytes(24)
         else:
             want_k = d.bytes(32) + d.bytes(32) + d.bytes(32)
