@@ -26,6 +26,10 @@ static void sizes(int scheme, size_t *pk, size_t *sk, size_t *ct, size_t *ss, si
   if (scheme == 0 || scheme == 2) { /* FrodoKEM-976-SHAKE / -AES */
     *pk = ORC_FRODO_PK; *sk = ORC_FRODO_SK; *ct = ORC_FRODO_CT; *ss = ORC_FRODO_SS;
     *kc = ORC_FRODO_KEYGEN_COINS; *ec = ORC_FRODO_ENC_COINS;
+  } else if (scheme == 5 || scheme == 6) { /* Frodo-640 / Frodo-1344 (SHAKE) */
+    const size_t n = scheme == 5 ? 640 : 1344, d = scheme == 5 ? 15 : 16, ls = scheme == 5 ? 16 : 32;
+    *pk = 16 + d * n; *sk = ls + *pk + 16 * n + ls; *ct = d * (n + 8); *ss = ls;
+    *kc = 2 * ls + 16; *ec = ls; /* s || seedSE || z(16) */
   } else {
     const size_t l = scheme == 1 ? 3 : scheme == 3 ? 2 : 4, et = l == 2 ? 3 : l == 3 ? 4 : 6;
     *pk = 320 * l + 32; *sk = 416 * l + *pk + 64; *ct = 320 * l + 32 * et; *ss = ORC_SABER_SS;
@@ -40,7 +44,12 @@ static void *run(void *arg) {
   for (size_t i = j->begin; i < j->end; ++i) {
     int rc;
     const int l = j->scheme == 1 ? 3 : j->scheme == 3 ? 2 : 4; /* Saber family rank */
-    if (j->op == 0)
+    const int fn = j->scheme == 5 ? 640 : 1344;                 /* Frodo-640 / -1344 */
+    if (j->scheme == 5 || j->scheme == 6)
+      rc = j->op == 0   ? orc_frodo_n_keypair(fn, j->a + i * pk, j->b + i * sk, j->c + i * kc)
+           : j->op == 1 ? orc_frodo_n_enc(fn, j->a + i * ct, j->b + i * ss, j->c + i * pk, j->d + i * ec)
+                        : orc_frodo_n_dec(fn, j->a + i * ss, j->c + i * ct, j->d + i * sk);
+    else if (j->op == 0)
       rc = j->scheme == 0   ? orc_frodo_keypair(j->a + i * pk, j->b + i * sk, j->c + i * kc)
            : j->scheme == 2 ? orc_frodo_aes_keypair(j->a + i * pk, j->b + i * sk, j->c + i * kc)
                             : orc_saber_l_keypair(l, j->a + i * pk, j->b + i * sk, j->c + i * kc);
@@ -59,7 +68,7 @@ static void *run(void *arg) {
 
 int orc_batch(int scheme, int op, size_t n, int threads, uint8_t *a, uint8_t *b, const uint8_t *c,
               const uint8_t *d) {
-  if (scheme < 0 || scheme > 4 || op < 0 || op > 2 || threads < 1) return -1;
+  if (scheme < 0 || scheme > 6 || op < 0 || op > 2 || threads < 1) return -1;
   if ((size_t)threads > n) threads = n ? (int)n : 1;
   pthread_t *tid = calloc((size_t)threads, sizeof *tid);
   job_t *jobs = calloc((size_t)threads, sizeof *jobs);

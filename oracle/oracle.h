@@ -58,6 +58,14 @@ int orc_frodo_aes_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t
 int orc_frodo_aes_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
 void orc_frodo_aes_gen_a_row(uint16_t row[ORC_FRODO_N], const uint8_t seed_a[16], uint16_t i);
 void orc_aes128_ecb(const uint8_t key[16], const uint8_t *in, uint8_t *out, size_t nblocks);
+/* Frodo-640 / Frodo-1344 (SHAKE), selected by n = 640 or 1344 (976 also
+ * accepted): pk 16 + D*n, sk len_sec + pk + 16n + len_sec, ct D*(n + 8),
+ * ss len_sec; coins s || seedSE || z (2 len_sec + 16) and mu (len_sec). */
+int orc_frodo_n_keypair(int n, uint8_t *pk, uint8_t *sk, const uint8_t *coins);
+int orc_frodo_n_enc(int n, uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu);
+int orc_frodo_n_dec(int n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+void orc_frodo_n_sample(int n, uint16_t *s, size_t count);
+
 int orc_frodo_keypair(uint8_t *pk, uint8_t *sk, const uint8_t *coins);
 int orc_frodo_enc(uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu);
 int orc_frodo_dec(uint8_t *ss, const uint8_t *ct, const uint8_t *sk);

@@ -21,6 +21,8 @@ SABER = 1
 FRODO_AES = 2
 LIGHTSABER = 3
 FIRESABER = 4
+FRODO640 = 5
+FRODO1344 = 6
 
 SIZES = {
     FRODO: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
@@ -28,6 +30,8 @@ SIZES = {
     FRODO_AES: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
     LIGHTSABER: dict(pk=672, sk=1568, ct=736, ss=32, kcoins=96, ecoins=32),
     FIRESABER: dict(pk=1312, sk=3040, ct=1472, ss=32, kcoins=96, ecoins=32),
+    FRODO640: dict(pk=9616, sk=19888, ct=9720, ss=16, kcoins=48, ecoins=16),
+    FRODO1344: dict(pk=21520, sk=43088, ct=21632, ss=32, kcoins=80, ecoins=32),
 }
 
 
@@ -55,6 +59,7 @@ def lib():
         _lib.orc_batch.argtypes = [ctypes.c_int, ctypes.c_int, sz, ctypes.c_int, vp, vp, vp, vp]
         _lib.orc_batch.restype = ctypes.c_int
         _lib.orc_frodo_sample.argtypes = [vp, sz]
+        _lib.orc_frodo_n_sample.argtypes = [ctypes.c_int, vp, sz]
         _lib.orc_frodo_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
         _lib.orc_frodo_aes_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
         _lib.orc_aes128_ecb.argtypes = [vp, vp, vp, sz]
@@ -163,6 +168,13 @@ def dec(scheme: int, ct: np.ndarray, sk: np.ndarray, threads: int = 1):
 def frodo_sample(r: np.ndarray) -> np.ndarray:
     s = np.ascontiguousarray(r, np.uint16).copy()
     lib().orc_frodo_sample(_p(s), s.size)
+    return s
+
+
+def frodo_n_sample(n: int, r: np.ndarray) -> np.ndarray:
+    """The error sampler of Frodo-<n> (640, 976 or 1344) over raw 16-bit words."""
+    s = np.ascontiguousarray(r, np.uint16).copy()
+    lib().orc_frodo_n_sample(n, _p(s), s.size)
     return s
 
 
