@@ -173,16 +173,19 @@ int dpkem_kat_seeds(size_t n, uint8_t* seeds) {
 }
 
 int dpkem_kat_coins(int scheme, size_t n, const uint8_t* seeds, uint8_t* kcoins, uint8_t* ecoins) {
-  if (scheme < DPKEM_FRODO976 || scheme > DPKEM_FIRESABER || (n && (!seeds || !kcoins || !ecoins)))
+  if (scheme < DPKEM_FRODO976 || scheme > DPKEM_FRODO1344 || (n && (!seeds || !kcoins || !ecoins)))
     return DPKEM_EINVAL;
-  const bool frodo = scheme == DPKEM_FRODO976 || scheme == DPKEM_FRODO976_AES;
+  const bool frodo = scheme == DPKEM_FRODO976 || scheme == DPKEM_FRODO976_AES || scheme == DPKEM_FRODO640 ||
+                     scheme == DPKEM_FRODO1344;
+  // FrodoKEM secret length: 16 / 24 / 32 bytes for Frodo-640 / -976 / -1344
+  const size_t sec = scheme == DPKEM_FRODO640 ? 16 : scheme == DPKEM_FRODO1344 ? 32 : 24;
   for (size_t i = 0; i < n; ++i) {
     dpkem_drbg d;
     dpkem_drbg_init(&d, seeds + 48 * i, nullptr);
     if (frodo) {
-      // keypair: randombytes(s || seedSE || z, 24 + 24 + 16); encaps: randombytes(mu, 24)
-      dpkem_drbg_bytes(&d, kcoins + 64 * i, 64);
-      dpkem_drbg_bytes(&d, ecoins + 24 * i, 24);
+      // keypair: randombytes(s || seedSE || z, 2 sec + 16); encaps: randombytes(mu, sec)
+      dpkem_drbg_bytes(&d, kcoins + (2 * sec + 16) * i, 2 * sec + 16);
+      dpkem_drbg_bytes(&d, ecoins + sec * i, sec);
     } else {
       // keypair: randombytes(seedA, 32); randombytes(seed_s, 32); randombytes(z, 32);
       // encaps: randombytes(m, 32)

@@ -25,19 +25,23 @@ struct SchemeSizes {
   size_t pk, sk, ct, ss, kcoins, ecoins;
 };
 
-const SchemeSizes kSizes[5] = {
+const SchemeSizes kSizes[7] = {
     {15632, 31296, 15744, 24, 64, 24},  // FrodoKEM-976-SHAKE
     {992, 2304, 1088, 32, 96, 32},      // Saber
     {15632, 31296, 15744, 24, 64, 24},  // FrodoKEM-976-AES
     {672, 1568, 736, 32, 96, 32},       // LightSaber
     {1312, 3040, 1472, 32, 96, 32},     // FireSaber
+    {9616, 19888, 9720, 16, 48, 16},    // Frodo-640-SHAKE
+    {21520, 43088, 21632, 32, 80, 32},  // Frodo-1344-SHAKE
 };
 
-bool valid_scheme(int scheme) { return scheme >= DPKEM_FRODO976 && scheme <= DPKEM_FIRESABER; }
+bool valid_scheme(int scheme) { return scheme >= DPKEM_FRODO976 && scheme <= DPKEM_FRODO1344; }
 bool saber_family(int scheme) {
   return scheme == DPKEM_SABER || scheme == DPKEM_LIGHTSABER || scheme == DPKEM_FIRESABER;
 }
 int saber_rank(int scheme) { return scheme == DPKEM_LIGHTSABER ? 2 : scheme == DPKEM_FIRESABER ? 4 : 3; }
+// FrodoKEM parameter set (n) of a Frodo scheme
+int frodo_set(int scheme) { return scheme == DPKEM_FRODO640 ? 640 : scheme == DPKEM_FRODO1344 ? 1344 : 976; }
 // schemes with no plain (one-workgroup-per-KEM) comparator form
 bool batched_only(int scheme) { return scheme != DPKEM_FRODO976 && scheme != DPKEM_SABER; }
 
@@ -96,7 +100,8 @@ int record_hip_error(dpkem_engine* e, hipError_t err, const char* what) { return
 // Bytes of device scratch one instance needs in the given form, excluding staged I/O.
 size_t scratch_per_inst(int scheme, int form) {
   if (form == DPKEM_FORM_PLAIN) return align_up(plain_scratch_per_inst(scheme));
-  return saber_family(scheme) ? align_up(saber_scratch_per_inst(saber_rank(scheme))) : align_up(kFrodoScratchPerInst);
+  return saber_family(scheme) ? align_up(saber_scratch_per_inst(saber_rank(scheme)))
+                              : align_up(frodo_scratch_per_inst(frodo_set(scheme)));
 }
 
 // I/O byte sizes per instance: two outputs and two inputs (0 if unused).
@@ -132,16 +137,17 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     else if (op == kEnc) saber_enc_launch(st, n, i0, i1, o0, o1, araw, sraw, mbuf, krbuf, l);
     else saber_dec_launch(st, n, i0, i1, o0, araw, sraw, mbuf, krbuf, fail, l);
   } else {
-    uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
-    uint32_t* e8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 7808));
+    const int set = frodo_set(scheme);
+    uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 8 * set));
+    uint32_t* e8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 8 * set));
     uint32_t* epp8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 64));
     uint8_t* kbuf = c.take((size_t)n * 64);
     uint8_t* mubuf = c.take((size_t)n * 32);
     uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
     const bool aes = scheme == DPKEM_FRODO976_AES;
-    if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8, aes);
-    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh, aes);
-    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail, aes);
+    if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8, aes, set);
+    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh, aes, set);
+    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail, aes, set);
   }
 }
 
@@ -170,9 +176,9 @@ size_t chunk_scratch_bytes(int scheme, int form, size_t n) { return n * scratch_
 namespace {
 
 // Encaps / decaps of a chunk whose key input (ENC: pk, DEC: sk) is one row
-// shared by all instances.  Frodo-SHAKE: groups of kFrodoGroup instances
-// expand A once.  Saber and Frodo-AES: the key row is broadcast into `bcast`
-// and the ordinary kernels run.
+// shared by all instances.  FrodoKEM-976-SHAKE: groups of kFrodoGroup
+// instances expand A once.  The other schemes: the key row is broadcast into
+// `bcast` and the ordinary kernels run.
 void launch_chunk_shared(hipStream_t st, int scheme, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
                          const uint8_t* i1, uint8_t* scratch, uint8_t* bcast) {
   if (scheme != DPKEM_FRODO976) {
@@ -515,6 +521,24 @@ int dpkem_firesaber_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uin
 }
 int dpkem_firesaber_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
   return wrapped(DPKEM_FIRESABER, kDec, n, ss, nullptr, ct, sk);
+}
+int dpkem_frodo640_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
+  return wrapped(DPKEM_FRODO640, kKeygen, n, pk, sk, nullptr, nullptr);
+}
+int dpkem_frodo640_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  return wrapped(DPKEM_FRODO640, kEnc, n, ct, ss, pk, nullptr);
+}
+int dpkem_frodo640_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  return wrapped(DPKEM_FRODO640, kDec, n, ss, nullptr, ct, sk);
+}
+int dpkem_frodo1344_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
+  return wrapped(DPKEM_FRODO1344, kKeygen, n, pk, sk, nullptr, nullptr);
+}
+int dpkem_frodo1344_crypto_kem_enc(size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk) {
+  return wrapped(DPKEM_FRODO1344, kEnc, n, ct, ss, pk, nullptr);
+}
+int dpkem_frodo1344_crypto_kem_dec(size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk) {
+  return wrapped(DPKEM_FRODO1344, kDec, n, ss, nullptr, ct, sk);
 }
 int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t* pk, uint8_t* sk) {
   return wrapped(DPKEM_SABER, kKeygen, n, pk, sk, nullptr, nullptr);

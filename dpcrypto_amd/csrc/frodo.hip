@@ -38,35 +38,123 @@ namespace dpkem {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int kFN = kFrodoN, kFNB = kFrodoNbar;
-constexpr int kFMat = kFN * kFNB;         // 7808 entries of S / E / S' / E' / B / B'
-constexpr int kFPkWords = kFrodoPk / 8;   // 1954
-constexpr int kFCtWords = kFrodoCt / 8;   // 1968
-constexpr int kFCtC1 = 2 * kFMat;         // 15616: offset of C in ct
-constexpr int kFSkS = kFrodoSec + kFrodoPk;  // 15656: offset of S^T in sk
-constexpr int kFSkPkh = kFrodoSk - kFrodoSec;  // 31272
-constexpr int kFUnits = kFN / 16;         // 61 MFMA K-units of 16 columns per row
+// ----------------------------------------------------------------- parameter sets
+//
+// FrodoKEM-976 is the reference's set (FrodoKEM976-GPU/include/params.h:9-24);
+// Frodo-640 and Frodo-1344 are the other two round-3 SHAKE sets.  Each differs
+// in n, log2 q (D), message bits per entry (B), secret length, the SHAKE of
+// the serial chains and the CDF table; A rows are SHAKE128 in all three.
+// threads x passes covers the n rows of A in the fused kernels (one row per
+// lane and pass); the shapes keep 14-16 waves per CU resident.
+__constant__ static const uint16_t kCdf640[12] = {4643, 13363, 20579, 25843, 29227, 31145,
+                                                  32103, 32525, 32689, 32745, 32762, 32766};
+__constant__ static const uint16_t kCdf976[10] = {5638, 15915, 23689, 28571, 31116,
+                                                  32217, 32613, 32731, 32760, 32766};
+__constant__ static const uint16_t kCdf1344[6] = {9142, 23462, 30338, 32361, 32725, 32765};
 
-__constant__ static const uint16_t kFrodoCdf[10] = {5638, 15915, 23689, 28571, 31116,
-                                                    32217, 32613, 32731, 32760, 32766};
+template <int N>
+struct FrodoSet;
+template <>
+struct FrodoSet<640> {
+  static constexpr int n = 640, logq = 15, b = 2, sec = 16, chain_rate = kRateShake128, threads = 320, passes = 2,
+                       ncdf = 12;
+  static __device__ const uint16_t* cdf() { return kCdf640; }
+};
+template <>
+struct FrodoSet<976> {
+  static constexpr int n = 976, logq = 16, b = 3, sec = 24, chain_rate = kRateShake256, threads = 1024, passes = 1,
+                       ncdf = 10;
+  static __device__ const uint16_t* cdf() { return kCdf976; }
+};
+template <>
+struct FrodoSet<1344> {
+  static constexpr int n = 1344, logq = 16, b = 4, sec = 32, chain_rate = kRateShake256, threads = 448, passes = 3,
+                       ncdf = 6;
+  static __device__ const uint16_t* cdf() { return kCdf1344; }
+};
+
+template <int N>
+struct FrodoP : FrodoSet<N> {
+  using S = FrodoSet<N>;
+  static constexpr int mat = S::n * kFrodoNbar;                  // entries of S / E / S' / E' / B / B'
+  static constexpr int pk = kFrodoSeedA + S::logq * S::n;        // seedA || pack_D(B)
+  static constexpr int sk = S::sec + pk + 2 * mat + S::sec;      // s || pk || S^T (LE u16) || pkh
+  static constexpr int ct = S::logq * (S::n + kFrodoNbar);       // pack_D(B') || pack_D(C)
+  static constexpr int kcoins = 2 * S::sec + 16;                 // s || seedSE || z
+  static constexpr int ecoins = S::sec;                          // mu
+  static constexpr int ls = S::sec / 8;                          // secret length in 64-bit words
+  static constexpr int ct_c = S::logq * S::n;                    // offset of C in ct
+  static constexpr int sk_s = S::sec + pk;                       // offset of S^T in sk
+  static constexpr int sk_pkh = sk - S::sec;                     // offset of pkh in sk
+  static constexpr int units = S::n / 16;                        // MFMA K-units of 16 columns per row
+  static constexpr int row_blocks = (S::n + 83) / 84;            // SHAKE128 blocks per row of A
+  static constexpr int srow = 16 * units + 32;                   // LDS row stride of int8 S^T (keygen)
+  static constexpr int sprow = S::threads * S::passes;           // LDS row stride of int8 S' (enc / dec)
+  static constexpr uint32_t qmask = (1u << S::logq) - 1;
+  static_assert(S::n % 16 == 0 && row_blocks % 4 == 0, "row generation works in groups of four blocks");
+  static_assert(S::threads % 64 == 0 && sprow >= S::n && sprow - S::threads < S::n, "every pass has rows");
+  static_assert(pk % 8 == 0 && sk % 8 == 0 && ct % 8 == 0 && ct_c % 4 == 0 && kcoins % 8 == 0,
+                "rows and fields stay 8-byte aligned");
+  static_assert(S::b * 64 == 8 * S::sec, "mu fills the 8 x 8 message matrix");
+};
+using F976 = FrodoP<976>;
+static_assert(F976::pk == kFrodoPk && F976::sk == kFrodoSk && F976::ct == kFrodoCt &&
+              F976::kcoins == kFrodoKeygenCoins, "FrodoKEM976-GPU/include/api.h:7-10");
+
+// FrodoKEM-976 names kept for the AES rows and the shared-key kernels
+constexpr int kFN = kFrodoN, kFNB = kFrodoNbar;
+constexpr int kFMat = F976::mat;          // 7808
+constexpr int kFCtC1 = F976::ct_c;        // 15616: offset of C in ct
+constexpr int kFUnits = F976::units;      // 61
 
 // CDF sampler on one 16-bit draw (FrodoKEM976-GPU/src/sample_pack.cu:13-32):
-// magnitude = #{j < 10 : CDF[j] < r >> 1}, sign = r & 1.
+// magnitude = #{j : CDF[j] < r >> 1} over all but the table's final entry,
+// sign = r & 1.
+template <class P>
 __device__ __forceinline__ int32_t frodo_sample(uint32_t r) {
   const uint32_t t = (r >> 1) & 0x7FFF;
+  const uint16_t* cdf = P::cdf();
   uint32_t e = 0;
 #pragma unroll
-  for (int j = 0; j < 10; ++j) e += ((uint32_t)kFrodoCdf[j] - t) >> 31;
+  for (int j = 0; j < P::ncdf; ++j) e += ((uint32_t)cdf[j] - t) >> 31;
   const uint32_t s = r & 1;
   return (int32_t)((e ^ (0u - s)) + s);
 }
 
+// Entry idx of a D-bit, most-significant-bit-first packed stream (D = 16:
+// big-endian u16).  Reads only the bytes the entry occupies.
+template <int D>
+__device__ __forceinline__ uint32_t unpack_at(const uint8_t* __restrict__ base, int idx) {
+  if constexpr (D == 16) {
+    return ((uint32_t)base[2 * idx] << 8) | base[2 * idx + 1];
+  } else {
+    static_assert(D == 15, "D = 15 or 16");
+    const int bit = 15 * idx, byte = bit >> 3, sh = bit & 7;
+    uint32_t v = ((uint32_t)base[byte] << 16) | ((uint32_t)base[byte + 1] << 8);
+    if (sh > 1) v |= base[byte + 2];
+    return (v >> (9 - sh)) & 0x7FFFu;
+  }
+}
+
+// Output word w (bytes 4w .. 4w + 3, as stored little-endian) of the 15-bit
+// packed stream of src[0 .. count): the stream's bits 32 w .. 32 w + 31 span
+// at most four entries.
+template <class W>
+__device__ __forceinline__ uint32_t pack15_word(const W* src, int count, int w) {
+  const int bit = 32 * w, e = bit / 15, off = bit % 15;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc = (acc << 15) | (e + t < count ? ((uint64_t)src[e + t] & 0x7FFF) : 0);
+  return __builtin_bswap32((uint32_t)(acc >> (28 - off)));
+}
+
 // ----------------------------------------------------------------- chains (lane pairs)
 //
-// Serial SHAKE256 chains run on lane pairs (keccak2.hpp): lanes 2k / 2k+1
-// hold the even / odd bit halves of instance k's state, 32 instances per
-// 64-thread block.  Squeezed noise words are rebuilt as 32-bit halves, so
-// each lane samples two of the four entries of every word.
+// Serial SHAKE chains (SHAKE256 for Frodo-976 / -1344, SHAKE128 for
+// Frodo-640) run on lane pairs (keccak2.hpp): lanes 2k / 2k+1 hold the even
+// / odd bit halves of instance k's state, 32 instances per 64-thread block.
+// Squeezed noise words are rebuilt as 32-bit halves, so each lane samples two
+// of the four entries of every word.
 
 using k2::absorb2_global;
 using k2::absorb2_halves;
@@ -81,134 +169,149 @@ using k2::store2_words;
 
 // Two samples from the 32-bit half of a squeezed word: int8 pair (low 16
 // bits) and the little-endian u16 pair (full 32 bits).
+template <class P>
 __device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t* le) {
-  const int32_t e0 = frodo_sample(x & 0xFFFF), e1 = frodo_sample(x >> 16);
+  const int32_t e0 = frodo_sample<P>(x & 0xFFFF), e1 = frodo_sample<P>(x >> 16);
   *le = ((uint32_t)e0 & 0xFFFF) | ((uint32_t)e1 << 16);
   return ((uint32_t)e0 & 0xFF) | (((uint32_t)e1 & 0xFF) << 8);
 }
 
-// SHAKE256(dom || seed[24 bytes]) on a lane pair, squeezing OUTW words; f(w,
+// SHAKE(dom || seed[sec bytes]) on a lane pair, squeezing OUTW words; f(w,
 // half) receives this lane's 32-bit half (low half for the even lane, high
 // for the odd lane) of every output word w.
-template <int OUTW, class F>
-__device__ __forceinline__ void shake256_seed_squeeze2(uint32_t p, uint64_t dom, const uint64_t seed[3], F&& f) {
-  const uint64_t m[4] = {dom | (seed[0] << 8), (seed[0] >> 56) | (seed[1] << 8), (seed[1] >> 56) | (seed[2] << 8),
-                         (seed[2] >> 56) | (kDsShake << 8)};
+template <class P, int OUTW, class F>
+__device__ __forceinline__ void shake_seed_squeeze2(uint32_t p, uint64_t dom, const uint64_t seed[P::ls], F&& f) {
+  constexpr int RW = P::chain_rate, LS = P::ls;
+  uint64_t m[LS + 1];
+  m[0] = dom | (seed[0] << 8);
+#pragma unroll
+  for (int i = 1; i < LS; ++i) m[i] = (seed[i - 1] >> 56) | (seed[i] << 8);
+  m[LS] = (seed[LS - 1] >> 56) | (kDsShake << 8);
   uint32_t a[25];
   k2::zero2(a);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) a[i] = k2::deint(m[i], p);
-  a[kRateShake256 - 1] ^= k2::half_const(kPadEnd, p);
+  for (int i = 0; i <= LS; ++i) a[i] = k2::deint(m[i], p);
+  a[RW - 1] ^= k2::half_const(kPadEnd, p);
   k2::keccakf2(a, p);
-  constexpr int NB = OUTW / kRateShake256, REM = OUTW % kRateShake256;
+  constexpr int NB = OUTW / RW, REM = OUTW % RW;
   for (int b = 0; b < NB; ++b) {
 #pragma unroll
-    for (int i = 0; i < kRateShake256; ++i) f(b * kRateShake256 + i, k2::out_half(a[i], p));
+    for (int i = 0; i < RW; ++i) f(b * RW + i, k2::out_half(a[i], p));
     k2::keccakf2(a, p);
   }
 #pragma unroll
-  for (int i = 0; i < REM; ++i) f(NB * kRateShake256 + i, k2::out_half(a[i], p));
+  for (int i = 0; i < REM; ++i) f(NB * RW + i, k2::out_half(a[i], p));
 }
 
 // ----------------------------------------------------------------- keygen chains
 
-// seedA = SHAKE256(z, 16); S^T || E = sample(SHAKE256(0x5F || seedSE, 31232)).
-// Writes pk[0:16], sk[0:24] = s, sk[24:40] = seedA, sk S^T part (LE u16) and
-// the int8 copies of S^T ([k][j]) and E ([i][k]) for the fused kernel.
+// seedA = SHAKE(z, 16); S^T || E = sample(SHAKE(0x5F || seedSE, 32 n)).
+// Writes pk[0:16], sk[0:sec] = s, sk[sec:sec+16] = seedA, the sk S^T part (LE
+// u16) and the int8 copies of S^T ([k][j]) and E ([i][k]) for the fused kernel.
+template <class P>
 __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                        uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
   FR_PAIR_SETUP
-  const uint8_t* c = coins + (size_t)inst * kFrodoKeygenCoins;
-  uint8_t* pp = pk + (size_t)inst * kFrodoPk;
-  uint8_t* q = sk + (size_t)inst * kFrodoSk;
+  constexpr int RW = P::chain_rate, LS = P::ls;
+  const uint8_t* c = coins + (size_t)inst * P::kcoins;
+  uint8_t* pp = pk + (size_t)inst * P::pk;
+  uint8_t* q = sk + (size_t)inst * P::sk;
   {
-    const uint64_t z[2] = {ld64(c + 48), ld64(c + 56)};
+    const uint64_t z[2] = {ld64(c + 2 * P::sec), ld64(c + 2 * P::sec + 8)};
     uint32_t a[25];
-    absorb2_regs<kRateShake256, 2>(a, p, z, kDsShake);
+    absorb2_regs<RW, 2>(a, p, z, kDsShake);
     store2_words<2>(a, p, pp);
-    store2_words<2>(a, p, q + 24);
+    store2_words<2>(a, p, q + P::sec);
     const uint32_t* sc = reinterpret_cast<const uint32_t*>(c);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) reinterpret_cast<uint32_t*>(q)[2 * i + p] = sc[2 * i + p];
+    for (int i = 0; i < LS; ++i) reinterpret_cast<uint32_t*>(q)[2 * i + p] = sc[2 * i + p];
   }
-  const uint64_t seed[3] = {ld64(c + 24), ld64(c + 32), ld64(c + 40)};
-  uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (kFMat / 4)) + 2 * p;
-  uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (kFMat / 4)) + 2 * p;
-  uint8_t* skS = q + kFSkS + 4 * p;
-  shake256_seed_squeeze2<2 * kFMat / 4>(p, 0x5F, seed, [&](int w, uint32_t x) {
+  uint64_t seed[LS];
+#pragma unroll
+  for (int i = 0; i < LS; ++i) seed[i] = ld64(c + P::sec + 8 * i);
+  uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (P::mat / 4)) + 2 * p;
+  uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (P::mat / 4)) + 2 * p;
+  uint8_t* skS = q + P::sk_s + 4 * p;
+  shake_seed_squeeze2<P, 2 * P::mat / 4>(p, 0x5F, seed, [&](int w, uint32_t x) {
     uint32_t le;
-    const uint32_t b = sample2(x, &le);
-    if (w < kFMat / 4) {
+    const uint32_t b = sample2<P>(x, &le);
+    if (w < P::mat / 4) {
       *reinterpret_cast<uint16_t*>(sd + 4 * w) = (uint16_t)b;
       *reinterpret_cast<uint32_t*>(skS + 8 * w) = le;
     } else {
-      *reinterpret_cast<uint16_t*>(ed + 4 * (w - kFMat / 4)) = (uint16_t)b;
+      *reinterpret_cast<uint16_t*>(ed + 4 * (w - P::mat / 4)) = (uint16_t)b;
     }
   });
 }
 
-// pkh = SHAKE256(pk, 24); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
+// pkh = SHAKE(pk, sec); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
+template <class P>
 __global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
   FR_PAIR_SETUP
   uint32_t a[25];
-  absorb2_global<kRateShake256, kFPkWords>(a, p, pk + (size_t)inst * kFrodoPk, kDsShake);
-  store2_words<3>(a, p, sk + (size_t)inst * kFrodoSk + kFSkPkh);
+  absorb2_global<P::chain_rate, P::pk / 8>(a, p, pk + (size_t)inst * P::pk, kDsShake);
+  store2_words<P::ls>(a, p, sk + (size_t)inst * P::sk + P::sk_pkh);
 }
 
 // ----------------------------------------------------------------- enc / dec chains
 
-// enc: pkh = SHAKE256(pk), or read from `pkh` rows when the caller holds it
+// enc: pkh = SHAKE(pk, sec), or read from `pkh` rows when the caller holds it
 // (a key store keeps each key's pkh); mu = coins.   dec: pkh from sk;
 // mu = recovered mu'.
-// G2 = SHAKE256(pkh || mu, 48) = seedSE || k; S' || E' || E'' from
-// SHAKE256(0x96 || seedSE, 31360).  k goes to kbuf[0:24].
-template <bool DEC>
+// G2 = SHAKE(pkh || mu, 2 sec) = seedSE || k; S' || E' || E'' from
+// SHAKE(0x96 || seedSE, 32 n + 128).  k goes to kbuf[0:sec].
+template <class P, bool DEC>
 __global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __restrict__ keys, size_t key_stride,
                                                        const uint8_t* __restrict__ mu, size_t mu_stride,
                                                        uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
                                                        uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
                                                        const uint8_t* __restrict__ pkh) {
   FR_PAIR_SETUP
+  constexpr int RW = P::chain_rate, LS = P::ls;
   const uint8_t* key = keys + (size_t)inst * key_stride;
-  uint32_t a[25], g[6];
+  uint32_t a[25], g[2 * LS];
   if (DEC || pkh) {
-    const uint8_t* h = DEC ? key + kFSkPkh : pkh + (size_t)inst * kFrodoSec;
+    const uint8_t* h = DEC ? key + P::sk_pkh : pkh + (size_t)inst * P::sec;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g[i] = k2::deint(ld64(h + 8 * i), p);
+    for (int i = 0; i < LS; ++i) g[i] = k2::deint(ld64(h + 8 * i), p);
   } else {
-    absorb2_global<kRateShake256, kFPkWords>(a, p, key, kDsShake);
+    absorb2_global<RW, P::pk / 8>(a, p, key, kDsShake);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g[i] = a[i];
+    for (int i = 0; i < LS; ++i) g[i] = a[i];
   }
   const uint8_t* m = mu + (size_t)inst * mu_stride;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) g[3 + i] = k2::deint(ld64(m + 8 * i), p);
-  absorb2_halves<kRateShake256, 6>(a, p, g, kDsShake);
-  store2_words<3>(a + 3, p, kbuf + (size_t)inst * 64);
-  const uint64_t seed[3] = {k2::out_full(a[0], p), k2::out_full(a[1], p), k2::out_full(a[2], p)};
-  uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (kFMat / 4)) + 2 * p;
-  uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (kFMat / 4)) + 2 * p;
+  for (int i = 0; i < LS; ++i) g[LS + i] = k2::deint(ld64(m + 8 * i), p);
+  absorb2_halves<RW, 2 * LS>(a, p, g, kDsShake);
+  store2_words<LS>(a + LS, p, kbuf + (size_t)inst * 64);
+  uint64_t seed[LS];
+#pragma unroll
+  for (int i = 0; i < LS; ++i) seed[i] = k2::out_full(a[i], p);
+  uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (P::mat / 4)) + 2 * p;
+  uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* pd = reinterpret_cast<uint8_t*>(epp8 + (size_t)inst * 16) + 2 * p;
-  shake256_seed_squeeze2<(2 * kFMat + 64) / 4>(p, 0x96, seed, [&](int w, uint32_t x) {
+  shake_seed_squeeze2<P, (2 * P::mat + 64) / 4>(p, 0x96, seed, [&](int w, uint32_t x) {
     uint32_t le;
-    const uint16_t b = (uint16_t)sample2(x, &le);
-    if (w < kFMat / 4) *reinterpret_cast<uint16_t*>(sd + 4 * w) = b;
-    else if (w < 2 * kFMat / 4) *reinterpret_cast<uint16_t*>(ed + 4 * (w - kFMat / 4)) = b;
-    else *reinterpret_cast<uint16_t*>(pd + 4 * (w - 2 * kFMat / 4)) = b;
+    const uint16_t b = (uint16_t)sample2<P>(x, &le);
+    if (w < P::mat / 4) *reinterpret_cast<uint16_t*>(sd + 4 * w) = b;
+    else if (w < 2 * P::mat / 4) *reinterpret_cast<uint16_t*>(ed + 4 * (w - P::mat / 4)) = b;
+    else *reinterpret_cast<uint16_t*>(pd + 4 * (w - 2 * P::mat / 4)) = b;
   });
 }
 
-// ss = SHAKE256(ct || kbar, 24); for decaps kbar = fail ? s : k'.
+// ss = SHAKE(ct || kbar, sec); for decaps kbar = fail ? s : k'.
+template <class P>
 __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ kbuf,
                                                const uint8_t* __restrict__ sk, size_t sk_stride,
                                                const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
   FR_PAIR_SETUP
-  const uint8_t* c = ct + (size_t)inst * kFrodoCt;
-  constexpr int RW = kRateShake256;
-  constexpr int NB = (kFCtWords + 3) / RW;           // 115 full blocks
-  constexpr int REM = kFCtWords - NB * RW;           // 13 ct words in the last block
-  static_assert(REM + 3 == RW - 1, "k and padding share the final block");
+  const uint8_t* c = ct + (size_t)inst * P::ct;
+  constexpr int RW = P::chain_rate, LS = P::ls;
+  constexpr int CTW = P::ct / 8;
+  constexpr int NB = CTW / RW;                       // full blocks of ct words
+  constexpr int REM = CTW - NB * RW;                 // ct words in the last block
+  static_assert(REM + LS <= RW - 1, "kbar and the padding share the final block");
   uint32_t a[25];
   k2::zero2(a);
   for (int b = 0; b < NB; ++b) {
@@ -221,17 +324,20 @@ __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict_
   uint64_t mask = 0;
   if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < LS; ++i) {
     const uint64_t k = ld64(kbuf + (size_t)inst * 64 + 8 * i);
     const uint64_t s = sk ? ld64(sk + (size_t)inst * sk_stride + 8 * i) : 0;
     a[REM + i] ^= k2::deint((k & ~mask) | (s & mask), p);
   }
-  a[RW - 1] ^= k2::half_const(kDsShake ^ kPadEnd, p);
+  a[REM + LS] ^= k2::half_const(kDsShake, p);
+  a[RW - 1] ^= k2::half_const(kPadEnd, p);
   k2::keccakf2(a, p);
-  store2_words<3>(a, p, ss + (size_t)inst * kFrodoSs);
+  store2_words<LS>(a, p, ss + (size_t)inst * P::sec);
 }
 
-// decaps: W = C - B' S, mu' = Decode(W).  256 threads per instance.
+// decaps: W = C - B' S, mu' = Decode(W) (B bits per entry, rounded).  256
+// threads per instance: 64 outputs x 4 slices of n / 4 terms.
+template <class P>
 __global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
                                                     size_t sk_stride, uint8_t* __restrict__ mubuf) {
   __shared__ uint32_t W[64];
@@ -239,30 +345,30 @@ __global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __rest
   if (inst >= n) return;
   if (tid < 64) W[tid] = 0;
   __syncthreads();
-  const uint8_t* c = ct + (size_t)inst * kFrodoCt;
-  const uint8_t* sT = sk + (size_t)inst * sk_stride + kFSkS;
-  const int o = tid & 63, i = o >> 3, j = o & 7, slice = tid >> 6;  // 4 slices of 244
+  const uint8_t* c = ct + (size_t)inst * P::ct;
+  const uint8_t* sT = sk + (size_t)inst * sk_stride + P::sk_s;
+  constexpr int CH = P::n / 4;
+  const int o = tid & 63, i = o >> 3, j = o & 7, slice = tid >> 6;
   uint32_t acc = 0;
-  for (int k = slice * 244; k < slice * 244 + 244; ++k) {
-    const uint32_t bp = ((uint32_t)c[2 * (i * kFN + k)] << 8) | c[2 * (i * kFN + k) + 1];
-    const uint32_t s = sT[2 * (j * kFN + k)] | ((uint32_t)sT[2 * (j * kFN + k) + 1] << 8);
+  for (int k = slice * CH; k < slice * CH + CH; ++k) {
+    const uint32_t bp = unpack_at<P::logq>(c, i * P::n + k);
+    const uint32_t s = sT[2 * (j * P::n + k)] | ((uint32_t)sT[2 * (j * P::n + k) + 1] << 8);
     acc += bp * s;
   }
   atomicAdd(&W[o], acc);
   __syncthreads();
   if (tid < 8) {
+    constexpr int SH = P::logq - P::b;
     uint32_t word = 0;
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
       const int idx = 8 * tid + jj;
-      const uint32_t cv = ((uint32_t)c[kFCtC1 + 2 * idx] << 8) | c[kFCtC1 + 2 * idx + 1];
-      const uint32_t w = (cv - W[idx]) & 0xFFFF;
-      word |= (((w + (1u << 12)) >> 13) & 7u) << (3 * jj);
+      const uint32_t w = (unpack_at<P::logq>(c + P::ct_c, idx) - W[idx]) & P::qmask;
+      word |= (((w + (1u << (SH - 1))) >> SH) & ((1u << P::b) - 1)) << (P::b * jj);
     }
-    uint8_t* m = mubuf + (size_t)inst * 32 + 3 * tid;
-    m[0] = (uint8_t)word;
-    m[1] = (uint8_t)(word >> 8);
-    m[2] = (uint8_t)(word >> 16);
+    uint8_t* m = mubuf + (size_t)inst * 32 + P::b * tid;
+#pragma unroll
+    for (int t = 0; t < P::b; ++t) m[t] = (uint8_t)(word >> (8 * t));
   }
 }
 
@@ -304,7 +410,9 @@ __device__ __forceinline__ void consume_block(const uint64_t A[25], uint64_t pen
   for (int m = 0; m < CO; ++m) pend[m] = A[kRateShake128 - CO + m];
 }
 
-template <class UNIT>
+// Row `row` of A = SHAKE128(LE16(row) || seedA), in units of 16 columns; the
+// last units may run past n (the caller drops them).
+template <int BLOCKS, class UNIT>
 __device__ __forceinline__ void gen_row(int row, uint64_t s0, uint64_t s1, UNIT&& unit) {
   uint64_t A[25];
   keccak_zero(A);
@@ -313,8 +421,8 @@ __device__ __forceinline__ void gen_row(int row, uint64_t s0, uint64_t s1, UNIT&
   A[2] = (s1 >> 48) | (kDsShake << 16);
   A[kRateShake128 - 1] ^= kPadEnd;
   uint64_t pend[3] = {0, 0, 0};
-  // 12 squeeze blocks cover 1008 >= 976 columns
-  for (int t = 0; t < 12; t += 4) {
+  static_assert(BLOCKS % 4 == 0, "carry pattern repeats every four blocks");
+  for (int t = 0; t < BLOCKS; t += 4) {
     keccakf(A);
     consume_block<0>(A, pend, unit);
     keccakf(A);
@@ -450,100 +558,121 @@ __device__ __forceinline__ void aes_rows_init(AesRows& g, const uint32_t* T, int
   for (int i = 0; i < 44; ++i) g.rk[i] = __builtin_amdgcn_readfirstlane(T[256 * kAesCopies + i]);
 }
 
-constexpr int kSRow = 1008;  // LDS row stride of int8 S^T (63 units, zero beyond 976)
-
-// keygen: B = A S + E.  pk[16:] and sk[40:15656] receive B (big-endian u16).
-// AES: FrodoKEM-976-AES rows instead of SHAKE128 rows.
-template <bool AES>
-__global__ void __launch_bounds__(1024) frodo_as_fused(int n, uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
-                                                       const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8) {
-  __shared__ __align__(16) int8_t Sl[9 * kSRow];  // rows 0..7 = S^T, row 8 = zeros
+// keygen: B = A S + E.  pk[16:] and sk[sec+16:] receive pack_D(B).
+// AES: FrodoKEM-976-AES rows instead of SHAKE128 rows.  Lanes take rows
+// wave * 64 + lane + pass * threads; a wave whose rows all lie past n in the
+// last pass skips it.
+template <class P, bool AES>
+__global__ void __launch_bounds__(P::threads) frodo_as_fused(int n, uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                             const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8) {
+  static_assert(!AES || P::n == 976, "AES rows are defined for FrodoKEM-976");
+  constexpr int T = P::threads, SROW = P::srow, N = P::n;
+  __shared__ __align__(16) int8_t Sl[9 * SROW];  // rows 0..7 = S^T, row 8 = zeros
   __shared__ int colsum[8];
   __shared__ __align__(16) uint32_t aes_lds[AES ? kAesLdsWords : 1];
+  __shared__ __align__(16) uint16_t Bst[P::logq < 16 ? P::mat : 1];  // D < 16: B staged for packing
   const int inst = blockIdx.x, tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   if (inst >= n) return;
-  const uint32_t* s = s8 + (size_t)inst * (kFMat / 4);
+  const uint32_t* s = s8 + (size_t)inst * (P::mat / 4);
   uint32_t* Sd = reinterpret_cast<uint32_t*>(Sl);
-  for (int x = tid; x < 9 * kSRow / 4; x += 1024) {
-    const int k = x / (kSRow / 4), j4 = x % (kSRow / 4);
-    Sd[x] = (k < 8 && j4 < kFN / 4) ? s[k * (kFN / 4) + j4] : 0u;
+  for (int x = tid; x < 9 * SROW / 4; x += T) {
+    const int k = x / (SROW / 4), j4 = x % (SROW / 4);
+    Sd[x] = (k < 8 && j4 < N / 4) ? s[k * (N / 4) + j4] : 0u;
   }
   if (tid < 8) colsum[tid] = 0;
   __syncthreads();
-  for (int x = tid; x < kFMat / 4; x += 1024) {
-    const int k = x / (kFN / 4), j4 = x % (kFN / 4);
-    const uint32_t v = Sd[k * (kSRow / 4) + j4];
+  for (int x = tid; x < P::mat / 4; x += T) {
+    const int k = x / (N / 4), j4 = x % (N / 4);
+    const uint32_t v = Sd[k * (SROW / 4) + j4];
     const int sum = (int8_t)v + (int8_t)(v >> 8) + (int8_t)(v >> 16) + (int8_t)(v >> 24);
     atomicAdd(&colsum[k], sum);
   }
   __syncthreads();
 
-  const uint8_t* p = pk + (size_t)inst * kFrodoPk;
+  const uint8_t* p = pk + (size_t)inst * P::pk;
   const uint64_t s0 = ld64(p), s1 = ld64(p + 8);
   AesRows aesg;
   if constexpr (AES) {
-    aes_setup(aes_lds, p, tid, 1024);
+    aes_setup(aes_lds, p, tid, T);
     aes_rows_init(aesg, aes_lds, lane);
   }
   const int q = lane >> 4, nn = lane & 15;
   // B operand: lane (q, nn) carries S^T column 4r + nn%4 when nn/4 == q, else zeros
   const bool mine = (nn >> 2) == q;
-  const int boff0 = mine ? (nn & 3) * kSRow : 8 * kSRow;
-  const int boff1 = mine ? (4 + (nn & 3)) * kSRow : 8 * kSRow;
-  v4i acc_lo0 = {0, 0, 0, 0}, acc_lo1 = {0, 0, 0, 0}, acc_hi0 = {0, 0, 0, 0}, acc_hi1 = {0, 0, 0, 0};
-  int u = 0;
-  auto unit = [&](const uint64_t w[4]) {
-    if (u < kFUnits) {
-      v4i lo, hi;
-      unit_limbs(w, lo, hi);
-      const v4i b0 = *reinterpret_cast<const v4i*>(Sl + boff0 + 16 * u);
-      const v4i b1 = *reinterpret_cast<const v4i*>(Sl + boff1 + 16 * u);
-      acc_lo0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(lo, b0, acc_lo0, 0, 0, 0);
-      acc_lo1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(lo, b1, acc_lo1, 0, 0, 0);
-      acc_hi0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(hi, b0, acc_hi0, 0, 0, 0);
-      acc_hi1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(hi, b1, acc_hi1, 0, 0, 0);
-    }
-    ++u;
-  };
-  if constexpr (AES) aesg(wave * 64 + lane, unit);
-  else gen_row(wave * 64 + lane, s0, s1, unit);
-
-  // D layout: lane (g, nn), element i = C[4g + i][nn].  C row m comes from
-  // A-operand lane 16 * (nn / 4) + m; the column is 4r + nn % 4.
+  const int boff0 = mine ? (nn & 3) * SROW : 8 * SROW;
+  const int boff1 = mine ? (4 + (nn & 3)) * SROW : 8 * SROW;
   const int g = lane >> 4;
-  const int8_t* e = e8 + (size_t)inst * kFMat;
-  uint8_t* pb = pk + (size_t)inst * kFrodoPk + kFrodoSeedA;
-  uint8_t* sb = sk + (size_t)inst * kFrodoSk + kFrodoSec + kFrodoSeedA;
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const v4i lo = r ? acc_lo1 : acc_lo0, hi = r ? acc_hi1 : acc_hi0;
-    const int col = 4 * r + (nn & 3);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int grow = wave * 64 + 16 * (nn >> 2) + 4 * g + i;
-      if (grow < kFN) {
-        const uint32_t v = (uint32_t)lo[i] + 256u * (uint32_t)hi[i] + 128u * (uint32_t)colsum[col] +
-                           (uint32_t)(int32_t)e[grow * kFNB + col];
-        const int idx = grow * kFNB + col;
-        const uint16_t be = (uint16_t)(((v & 0xFF) << 8) | ((v >> 8) & 0xFF));
-        *reinterpret_cast<uint16_t*>(pb + 2 * idx) = be;
-        *reinterpret_cast<uint16_t*>(sb + 2 * idx) = be;
+  const int8_t* e = e8 + (size_t)inst * P::mat;
+  uint8_t* pb = pk + (size_t)inst * P::pk + kFrodoSeedA;
+  uint8_t* sb = sk + (size_t)inst * P::sk + P::sec + kFrodoSeedA;
+  for (int pass = 0; pass < P::passes; ++pass) {
+    const int row0 = pass * T + wave * 64;
+    if (row0 >= N) break;
+    v4i acc_lo0 = {0, 0, 0, 0}, acc_lo1 = {0, 0, 0, 0}, acc_hi0 = {0, 0, 0, 0}, acc_hi1 = {0, 0, 0, 0};
+    int u = 0;
+    auto unit = [&](const uint64_t w[4]) {
+      if (u < P::units) {
+        v4i lo, hi;
+        unit_limbs(w, lo, hi);
+        const v4i b0 = *reinterpret_cast<const v4i*>(Sl + boff0 + 16 * u);
+        const v4i b1 = *reinterpret_cast<const v4i*>(Sl + boff1 + 16 * u);
+        acc_lo0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(lo, b0, acc_lo0, 0, 0, 0);
+        acc_lo1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(lo, b1, acc_lo1, 0, 0, 0);
+        acc_hi0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(hi, b0, acc_hi0, 0, 0, 0);
+        acc_hi1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(hi, b1, acc_hi1, 0, 0, 0);
       }
+      ++u;
+    };
+    if constexpr (AES) aesg(row0 + lane, unit);
+    else gen_row<P::row_blocks>(row0 + lane, s0, s1, unit);
+
+    // D layout: lane (g, nn), element i = C[4g + i][nn].  C row m comes from
+    // A-operand lane 16 * (nn / 4) + m; the column is 4r + nn % 4.
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const v4i lo = r ? acc_lo1 : acc_lo0, hi = r ? acc_hi1 : acc_hi0;
+      const int col = 4 * r + (nn & 3);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int grow = row0 + 16 * (nn >> 2) + 4 * g + i;
+        if (grow < N) {
+          const uint32_t v = (uint32_t)lo[i] + 256u * (uint32_t)hi[i] + 128u * (uint32_t)colsum[col] +
+                             (uint32_t)(int32_t)e[grow * kFrodoNbar + col];
+          const int idx = grow * kFrodoNbar + col;
+          if constexpr (P::logq == 16) {
+            const uint16_t be = (uint16_t)(((v & 0xFF) << 8) | ((v >> 8) & 0xFF));
+            *reinterpret_cast<uint16_t*>(pb + 2 * idx) = be;
+            *reinterpret_cast<uint16_t*>(sb + 2 * idx) = be;
+          } else {
+            Bst[idx] = (uint16_t)(v & P::qmask);
+          }
+        }
+      }
+    }
+  }
+  if constexpr (P::logq < 16) {
+    __syncthreads();
+    for (int w = tid; w < P::logq * P::mat / 32; w += T) {
+      const uint32_t word = pack15_word(Bst, P::mat, w);
+      reinterpret_cast<uint32_t*>(pb)[w] = word;
+      reinterpret_cast<uint32_t*>(sb)[w] = word;
     }
   }
 }
 
 // enc / dec: B' = S' A + E', then V = S' B + E'' and C = V + Encode(mu).
 // enc writes ct; dec compares with the received ct and raises fail[inst].
-template <bool DEC, bool AES>
-__global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __restrict__ pkb, size_t pk_stride,
-                                                       const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8,
-                                                       const int8_t* __restrict__ epp8, const uint8_t* __restrict__ mu,
-                                                       size_t mu_stride, uint8_t* __restrict__ ct_out,
-                                                       const uint8_t* __restrict__ ct_in, uint32_t* __restrict__ fail) {
-  __shared__ __align__(16) uint32_t Bacc[kFMat];
-  __shared__ __align__(16) int8_t Sp[8 * 1024];  // S' rows, zero for i >= 976
+template <class P, bool DEC, bool AES>
+__global__ void __launch_bounds__(P::threads) frodo_sa_fused(int n, const uint8_t* __restrict__ pkb, size_t pk_stride,
+                                                             const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8,
+                                                             const int8_t* __restrict__ epp8, const uint8_t* __restrict__ mu,
+                                                             size_t mu_stride, uint8_t* __restrict__ ct_out,
+                                                             const uint8_t* __restrict__ ct_in, uint32_t* __restrict__ fail) {
+  static_assert(!AES || P::n == 976, "AES rows are defined for FrodoKEM-976");
+  constexpr int T = P::threads, SP = P::sprow, N = P::n;
+  __shared__ __align__(16) uint32_t Bacc[P::mat];
+  __shared__ __align__(16) int8_t Sp[8 * SP];  // S' rows, zero for i >= n
   __shared__ int rowsum[8];
   __shared__ uint32_t V[64];
   __shared__ uint32_t diff;
@@ -551,37 +680,31 @@ __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __r
   const int inst = blockIdx.x, tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   if (inst >= n) return;
-  const uint32_t* s = s8 + (size_t)inst * (kFMat / 4);
+  const uint32_t* s = s8 + (size_t)inst * (P::mat / 4);
   uint32_t* Sd = reinterpret_cast<uint32_t*>(Sp);
-  for (int x = tid; x < 8 * 256; x += 1024) {
-    const int k = x >> 8, i4 = x & 255;
-    Sd[x] = i4 < kFN / 4 ? s[k * (kFN / 4) + i4] : 0u;
+  for (int x = tid; x < 8 * SP / 4; x += T) {
+    const int k = x / (SP / 4), i4 = x % (SP / 4);
+    Sd[x] = i4 < N / 4 ? s[k * (N / 4) + i4] : 0u;
   }
-  for (int x = tid; x < kFMat; x += 1024) Bacc[x] = 0;
+  for (int x = tid; x < P::mat; x += T) Bacc[x] = 0;
   if (tid < 8) rowsum[tid] = 0;
   if (tid < 64) V[tid] = 0;
   if (tid == 0) diff = 0;
   __syncthreads();
-  for (int x = tid; x < 8 * 256; x += 1024) {
+  for (int x = tid; x < 8 * SP / 4; x += T) {
     const uint32_t v = Sd[x];
     const int sum = (int8_t)v + (int8_t)(v >> 8) + (int8_t)(v >> 16) + (int8_t)(v >> 24);
-    atomicAdd(&rowsum[x >> 8], sum);
+    atomicAdd(&rowsum[x / (SP / 4)], sum);
   }
 
   const uint8_t* pk = pkb + (size_t)inst * pk_stride;
   const uint64_t s0 = ld64(pk), s1 = ld64(pk + 8);
   AesRows aesg;
   if constexpr (AES) {
-    aes_setup(aes_lds, pk, tid, 1024);
+    aes_setup(aes_lds, pk, tid, T);
     aes_rows_init(aesg, aes_lds, lane);
   }
   const int g = lane >> 4, nn = lane & 15;
-  // product A operand: lane (g, m = nn) holds S'[m][wave rows 16 q + 4 g + i] in dword q, byte i
-  v4i sop = {0, 0, 0, 0};
-  if (nn < 8) {
-#pragma unroll
-    for (int qq = 0; qq < 4; ++qq) sop[qq] = (int)Sd[nn * 256 + (wave * 64 + 16 * qq + 4 * g) / 4];
-  }
   // transpose selectors: B operand of transpose q is one-hot(byte nn) on lanes of group q
   const int onehot = 1 << (8 * (nn & 3));
   v4i sel[4];
@@ -591,71 +714,97 @@ __global__ void __launch_bounds__(1024) frodo_sa_fused(int n, const uint8_t* __r
     for (int d = 0; d < 4; ++d) sel[qq][d] = (g == qq && d == (nn >> 2)) ? onehot : 0;
   const v4i zero = {0, 0, 0, 0};
 
-  int u = 0;
-  auto unit = [&](const uint64_t w[4]) {
-    if (u < kFUnits) {
-      v4i lm[2];
-      unit_limbs(w, lm[0], lm[1]);
-      v4i prod[2];
+  for (int pass = 0; pass < P::passes; ++pass) {
+    const int row0 = pass * T + wave * 64;
+    if (row0 >= N) break;
+    // product A operand: lane (g, m = nn) holds S'[m][rows row0 + 16 q + 4 g + i] in dword q, byte i
+    v4i sop = {0, 0, 0, 0};
+    if (nn < 8) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        // t_q: lane (g, nn) element i = limb of row 16 q + 4 g + i at column 16 u + nn
-        v4i packed;
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-          const v4i t = __builtin_amdgcn_mfma_i32_16x16x64_i8(lm[h], sel[qq], zero, 0, 0, 0);
-          packed[qq] = (int)(((uint32_t)t[0] & 0xFF) | (((uint32_t)t[1] & 0xFF) << 8) |
-                             (((uint32_t)t[2] & 0xFF) << 16) | ((uint32_t)t[3] << 24));
-        }
-        prod[h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sop, packed, zero, 0, 0, 0);
-      }
-      const int col = 16 * u + nn;
-      if (g < 2 && col < kFN) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          atomicAdd(&Bacc[(4 * g + i) * kFN + col], (uint32_t)prod[0][i] + 256u * (uint32_t)prod[1][i]);
-      }
+      for (int qq = 0; qq < 4; ++qq) sop[qq] = (int)Sd[nn * (SP / 4) + (row0 + 16 * qq + 4 * g) / 4];
     }
-    ++u;
-  };
-  if constexpr (AES) aesg(wave * 64 + lane, unit);
-  else gen_row(wave * 64 + lane, s0, s1, unit);
+    int u = 0;
+    auto unit = [&](const uint64_t w[4]) {
+      if (u < P::units) {
+        v4i lm[2];
+        unit_limbs(w, lm[0], lm[1]);
+        v4i prod[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          // t_q: lane (g, nn) element i = limb of row 16 q + 4 g + i at column 16 u + nn
+          v4i packed;
+#pragma unroll
+          for (int qq = 0; qq < 4; ++qq) {
+            const v4i t = __builtin_amdgcn_mfma_i32_16x16x64_i8(lm[h], sel[qq], zero, 0, 0, 0);
+            packed[qq] = (int)(((uint32_t)t[0] & 0xFF) | (((uint32_t)t[1] & 0xFF) << 8) |
+                               (((uint32_t)t[2] & 0xFF) << 16) | ((uint32_t)t[3] << 24));
+          }
+          prod[h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sop, packed, zero, 0, 0, 0);
+        }
+        const int col = 16 * u + nn;
+        if (g < 2 && col < N) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            atomicAdd(&Bacc[(4 * g + i) * N + col], (uint32_t)prod[0][i] + 256u * (uint32_t)prod[1][i]);
+        }
+      }
+      ++u;
+    };
+    if constexpr (AES) aesg(row0 + lane, unit);
+    else gen_row<P::row_blocks>(row0 + lane, s0, s1, unit);
+  }
   __syncthreads();
 
-  // B' = acc + 128 rowsum + E'  (k-major, 8 x 976)
-  const int8_t* ep = e8 + (size_t)inst * kFMat;
+  // B' = acc + 128 rowsum + E'  (k-major, 8 x n)
+  const int8_t* ep = e8 + (size_t)inst * P::mat;
+  const uint8_t* cin = DEC ? ct_in + (size_t)inst * P::ct : nullptr;
+  uint8_t* cout = DEC ? nullptr : ct_out + (size_t)inst * P::ct;
   uint32_t d = 0;
-  for (int x = tid; x < kFMat; x += 1024) {
-    const uint32_t v = (Bacc[x] + 128u * (uint32_t)rowsum[x / kFN] + (uint32_t)(int32_t)ep[x]) & 0xFFFF;
-    const uint16_t be = (uint16_t)((v << 8) | (v >> 8));
-    if (DEC) {
-      d |= (uint32_t)(*reinterpret_cast<const uint16_t*>(ct_in + (size_t)inst * kFrodoCt + 2 * x) ^ be);
+  for (int x = tid; x < P::mat; x += T) {
+    const uint32_t v = (Bacc[x] + 128u * (uint32_t)rowsum[x / N] + (uint32_t)(int32_t)ep[x]) & P::qmask;
+    if constexpr (P::logq == 16) {
+      const uint16_t be = (uint16_t)((v << 8) | (v >> 8));
+      if (DEC) d |= (uint32_t)(*reinterpret_cast<const uint16_t*>(cin + 2 * x) ^ be);
+      else *reinterpret_cast<uint16_t*>(cout + 2 * x) = be;
     } else {
-      *reinterpret_cast<uint16_t*>(ct_out + (size_t)inst * kFrodoCt + 2 * x) = be;
+      Bacc[x] = v;  // packed below
     }
   }
-  // V = S' B (B = pk.b, 976 x 8 big-endian): 64 outputs, 16 slices of 61 rows
+  // V = S' B (B = pack_D(B) in pk, n x 8): 64 outputs, T / 64 slices of rows
   {
+    constexpr int NSL = T / 64, CH = (N + NSL - 1) / NSL;
     const int o = tid & 63, i = o >> 3, j = o & 7, slice = tid >> 6;
+    const int k1 = min(N, slice * CH + CH);
     uint32_t acc = 0;
-    for (int k = slice * 61; k < slice * 61 + 61; ++k) {
-      const uint8_t* bp = pk + kFrodoSeedA + 2 * (k * kFNB + j);
-      const uint32_t b = ((uint32_t)bp[0] << 8) | bp[1];
-      acc += (uint32_t)(int32_t)Sp[i * 1024 + k] * b;
-    }
+    for (int k = slice * CH; k < k1; ++k)
+      acc += (uint32_t)(int32_t)Sp[i * SP + k] * unpack_at<P::logq>(pk + kFrodoSeedA, k * kFrodoNbar + j);
     atomicAdd(&V[o], acc);
   }
   __syncthreads();
   if (tid < 64) {
-    const uint8_t* m = mu + (size_t)inst * mu_stride + 3 * (tid >> 3);
-    const uint32_t t24 = m[0] | ((uint32_t)m[1] << 8) | ((uint32_t)m[2] << 16);
-    const uint32_t enc = ((t24 >> (3 * (tid & 7))) & 7u) << 13;
-    const uint32_t cv = (V[tid] + (uint32_t)(int32_t)epp8[(size_t)inst * 64 + tid] + enc) & 0xFFFF;
-    const uint16_t be = (uint16_t)((cv << 8) | (cv >> 8));
-    if (DEC) {
-      d |= (uint32_t)(*reinterpret_cast<const uint16_t*>(ct_in + (size_t)inst * kFrodoCt + kFCtC1 + 2 * tid) ^ be);
+    // B message bits of entry tid, least significant first from the bytes of mu
+    const uint8_t* m = mu + (size_t)inst * mu_stride;
+    const int bit = P::b * tid, sh = bit & 7;
+    uint32_t bits = m[bit >> 3];
+    if (sh + P::b > 8) bits |= (uint32_t)m[(bit >> 3) + 1] << 8;
+    const uint32_t enc = ((bits >> sh) & ((1u << P::b) - 1)) << (P::logq - P::b);
+    const uint32_t cv = (V[tid] + (uint32_t)(int32_t)epp8[(size_t)inst * 64 + tid] + enc) & P::qmask;
+    if constexpr (P::logq == 16) {
+      const uint16_t be = (uint16_t)((cv << 8) | (cv >> 8));
+      if (DEC) d |= (uint32_t)(*reinterpret_cast<const uint16_t*>(cin + P::ct_c + 2 * tid) ^ be);
+      else *reinterpret_cast<uint16_t*>(cout + P::ct_c + 2 * tid) = be;
     } else {
-      *reinterpret_cast<uint16_t*>(ct_out + (size_t)inst * kFrodoCt + kFCtC1 + 2 * tid) = be;
+      V[tid] = cv;
+    }
+  }
+  if constexpr (P::logq < 16) {
+    // pack_15(B') || pack_15(C): ct words 0 .. ct_c / 4 from B', then C
+    __syncthreads();
+    constexpr int WB = P::ct_c / 4, WC = (P::ct - P::ct_c) / 4;
+    for (int w = tid; w < WB + WC; w += T) {
+      const uint32_t word = w < WB ? pack15_word(Bacc, P::mat, w) : pack15_word(V, 64, w - WB);
+      if (DEC) d |= reinterpret_cast<const uint32_t*>(cin)[w] ^ word;
+      else reinterpret_cast<uint32_t*>(cout)[w] = word;
     }
   }
   if (DEC) {
@@ -729,7 +878,7 @@ __global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint
   const v4i zero = {0, 0, 0, 0};
 
   int u = 0;
-  gen_row(wave * 64 + lane, s0, s1, [&](const uint64_t w[4]) {
+  gen_row<F976::row_blocks>(wave * 64 + lane, s0, s1, [&](const uint64_t w[4]) {
     if (u < kFUnits) {
       v4i lm[2];
       unit_limbs(w, lm[0], lm[1]);
@@ -819,50 +968,72 @@ __global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint
 
 static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 
+// profiler names: FrodoKEM-976 keeps the plain kernel name, the others carry ",640" / ",1344"
+template <class P>
+static const char* fr_name(const char* n976, const char* n640, const char* n1344) {
+  return P::n == 976 ? n976 : P::n == 640 ? n640 : n1344;
+}
+#define FR_NAME(base) fr_name<P>(base, base ",640", base ",1344")
+
+template <class P, bool AES>
+static void keygen_t(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8, uint32_t* e8) {
+  DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, coins, pk,
+               sk, s8, e8);
+  DPKEM_LAUNCH(AES ? "frodo_as_fused<aes>" : FR_NAME("frodo_as_fused"), (frodo_as_fused<P, AES>), dim3(n),
+               dim3(P::threads), st, n, pk, sk, s8, reinterpret_cast<const int8_t*>(e8));
+  DPKEM_LAUNCH(FR_NAME("frodo_keygen_post"), frodo_keygen_post<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
+}
+
+template <class P, bool AES>
+static void enc_t(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss, uint32_t* s8,
+                  uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh) {
+  DPKEM_LAUNCH(FR_NAME("frodo_encdec_pre<false>"), (frodo_encdec_pre<P, false>), dim3(blocks_for(n, 32)), dim3(64), st,
+               n, pk, (size_t)P::pk, coins, (size_t)P::ecoins, kbuf, s8, e8, epp8, pkh);
+  DPKEM_LAUNCH(AES ? "frodo_sa_fused<false,aes>" : FR_NAME("frodo_sa_fused<false>"), (frodo_sa_fused<P, false, AES>),
+               dim3(n), dim3(P::threads), st, n, pk, (size_t)P::pk, s8, reinterpret_cast<const int8_t*>(e8),
+               reinterpret_cast<const int8_t*>(epp8), coins, (size_t)P::ecoins, ct, nullptr, nullptr);
+  DPKEM_LAUNCH(FR_NAME("frodo_ss"), frodo_ss<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr,
+               (size_t)0, nullptr, ss);
+}
+
+template <class P, bool AES>
+static void dec_t(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8, uint32_t* e8,
+                  uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail) {
+  DPKEM_LAUNCH(FR_NAME("frodo_dec_mu"), frodo_dec_mu<P>, dim3(n), dim3(256), st, n, ct, sk, (size_t)P::sk, mubuf);
+  DPKEM_LAUNCH(FR_NAME("frodo_encdec_pre<true>"), (frodo_encdec_pre<P, true>), dim3(blocks_for(n, 32)), dim3(64), st,
+               n, sk, (size_t)P::sk, mubuf, (size_t)32, kbuf, s8, e8, epp8, nullptr);
+  DPKEM_LAUNCH(AES ? "frodo_sa_fused<true,aes>" : FR_NAME("frodo_sa_fused<true>"), (frodo_sa_fused<P, true, AES>),
+               dim3(n), dim3(P::threads), st, n, sk + P::sec, (size_t)P::sk, s8, reinterpret_cast<const int8_t*>(e8),
+               reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
+  DPKEM_LAUNCH(FR_NAME("frodo_ss"), frodo_ss<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, (size_t)P::sk,
+               fail, ss);
+}
+
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
-                         uint32_t* e8, bool aes) {
+                         uint32_t* e8, bool aes, int set) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("frodo_keygen_pre", frodo_keygen_pre, dim3(blocks_for(n, 32)), dim3(64), st, n, coins, pk, sk, s8, e8);
-  if (aes)
-    DPKEM_LAUNCH("frodo_as_fused<aes>", frodo_as_fused<true>, dim3(n), dim3(1024), st, n, pk, sk, s8,
-                 reinterpret_cast<const int8_t*>(e8));
-  else
-    DPKEM_LAUNCH("frodo_as_fused", frodo_as_fused<false>, dim3(n), dim3(1024), st, n, pk, sk, s8,
-                 reinterpret_cast<const int8_t*>(e8));
-  DPKEM_LAUNCH("frodo_keygen_post", frodo_keygen_post, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
+  if (set == 640) keygen_t<FrodoP<640>, false>(st, n, coins, pk, sk, s8, e8);
+  else if (set == 1344) keygen_t<FrodoP<1344>, false>(st, n, coins, pk, sk, s8, e8);
+  else if (aes) keygen_t<F976, true>(st, n, coins, pk, sk, s8, e8);
+  else keygen_t<F976, false>(st, n, coins, pk, sk, s8, e8);
 }
 
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh, bool aes) {
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh, bool aes, int set) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, (size_t)kFrodoPk, coins,
-                     (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8, pkh);
-  if (aes)
-    DPKEM_LAUNCH("frodo_sa_fused<false,aes>", (frodo_sa_fused<false, true>), dim3(n), dim3(1024), st, n, pk,
-                 (size_t)kFrodoPk, s8, reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8),
-                 coins, (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
-  else
-    DPKEM_LAUNCH("frodo_sa_fused<false>", (frodo_sa_fused<false, false>), dim3(n), dim3(1024), st, n, pk,
-                 (size_t)kFrodoPk, s8, reinterpret_cast<const int8_t*>(e8), reinterpret_cast<const int8_t*>(epp8),
-                 coins, (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, (size_t)0, nullptr, ss);
+  if (set == 640) enc_t<FrodoP<640>, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
+  else if (set == 1344) enc_t<FrodoP<1344>, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
+  else if (aes) enc_t<F976, true>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
+  else enc_t<F976, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
 }
 
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
-                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes) {
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes, int set) {
   if (n <= 0) return;
-  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, (size_t)kFrodoSk, mubuf);
-  DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk, (size_t)kFrodoSk, mubuf,
-                     (size_t)32, kbuf, s8, e8, epp8, nullptr);
-  if (aes)
-    DPKEM_LAUNCH("frodo_sa_fused<true,aes>", (frodo_sa_fused<true, true>), dim3(n), dim3(1024), st, n,
-                 sk + kFrodoSec, (size_t)kFrodoSk, s8, reinterpret_cast<const int8_t*>(e8),
-                 reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
-  else
-    DPKEM_LAUNCH("frodo_sa_fused<true>", (frodo_sa_fused<true, false>), dim3(n), dim3(1024), st, n,
-                 sk + kFrodoSec, (size_t)kFrodoSk, s8, reinterpret_cast<const int8_t*>(e8),
-                 reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, (size_t)kFrodoSk, fail, ss);
+  if (set == 640) dec_t<FrodoP<640>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
+  else if (set == 1344) dec_t<FrodoP<1344>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
+  else if (aes) dec_t<F976, true>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
+  else dec_t<F976, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
 }
 
 
@@ -873,12 +1044,13 @@ void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t p
                               const uint2* groups, int ngroups, const uint8_t* pkh) {
   if (n <= 0) return;
   if (!groups) ngroups = blocks_for(n, kFrodoGroup);
-  DPKEM_LAUNCH("frodo_encdec_pre<false>", frodo_encdec_pre<false>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk,
-               pk_stride, coins, (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8, pkh);
+  DPKEM_LAUNCH("frodo_encdec_pre<false>", (frodo_encdec_pre<F976, false>), dim3(blocks_for(n, 32)), dim3(64), st, n,
+               pk, pk_stride, coins, (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8, pkh);
   DPKEM_LAUNCH("frodo_sa_grouped<false>", (frodo_sa_grouped<false, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
                ngroups, groups, n, pk, pk_stride, s8, reinterpret_cast<const int8_t*>(e8),
                reinterpret_cast<const int8_t*>(epp8), coins, (size_t)kFrodoEncCoins, ct, nullptr, nullptr);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, (size_t)0, nullptr, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss<F976>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr, (size_t)0,
+               nullptr, ss);
 }
 
 void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, size_t sk_stride, uint8_t* ss,
@@ -886,13 +1058,14 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
                               const uint2* groups, int ngroups) {
   if (n <= 0) return;
   if (!groups) ngroups = blocks_for(n, kFrodoGroup);
-  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu, dim3(n), dim3(256), st, n, ct, sk, sk_stride, mubuf);
-  DPKEM_LAUNCH("frodo_encdec_pre<true>", frodo_encdec_pre<true>, dim3(blocks_for(n, 32)), dim3(64), st, n, sk,
+  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu<F976>, dim3(n), dim3(256), st, n, ct, sk, sk_stride, mubuf);
+  DPKEM_LAUNCH("frodo_encdec_pre<true>", (frodo_encdec_pre<F976, true>), dim3(blocks_for(n, 32)), dim3(64), st, n, sk,
                sk_stride, mubuf, (size_t)32, kbuf, s8, e8, epp8, nullptr);
   DPKEM_LAUNCH("frodo_sa_grouped<true>", (frodo_sa_grouped<true, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
                ngroups, groups, n, sk + kFrodoSec, sk_stride, s8, reinterpret_cast<const int8_t*>(e8),
                reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, sk_stride, fail, ss);
+  DPKEM_LAUNCH("frodo_ss", frodo_ss<F976>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, sk_stride, fail,
+               ss);
 }
 
 }  // namespace dpkem

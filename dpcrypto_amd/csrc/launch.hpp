@@ -26,13 +26,15 @@ void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t*
                         uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch);
 constexpr size_t kSaberChainScratchPerInst = 3744 + 768 + 32 + 64 + 4;
 
-// aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE
+// set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
+// aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch
+// per instance: s8 / e8 n * 8 bytes each (frodo_scratch_per_inst).
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
-                         uint32_t* e8, bool aes = false);
-// pkh: nullptr (hash each pk) or one 24-byte pkh row per instance
+                         uint32_t* e8, bool aes = false, int set = 976);
+// pkh: nullptr (hash each pk) or one sec-byte pkh row per instance
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh = nullptr,
-                      bool aes = false);
+                      bool aes = false, int set = 976);
 // Groups of up to kFrodoGroup instances sharing one public key expand A once;
 // groups[b] = (first, count) or nullptr for consecutive groups of
 // kFrodoGroup; key strides may be 0 (one key for all n).
@@ -44,7 +46,8 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
                               uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail,
                               const uint2* groups, int ngroups);
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
-                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes = false);
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes = false,
+                      int set = 976);
 
 // ---- plain comparator form (plain.hip); op 0 keygen, 1 enc, 2 dec with the
 // same output / input roles as the batched launchers.
@@ -57,5 +60,7 @@ void plain_frodo_launch(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1,
 // Scratch bytes per instance for each (scheme, op) of the batched form.
 constexpr size_t kSaberScratchPerInst = 3744 + 768 + 32 + 64 + 16;
 constexpr size_t kFrodoScratchPerInst = 7808 + 7808 + 64 + 64 + 32 + 16;
+// s8 (8 n) | e8 (8 n) | epp8 64 | kbuf 64 | mubuf 32 | fail 4 (+ padding)
+constexpr size_t frodo_scratch_per_inst(int set) { return 16 * (size_t)set + 64 + 64 + 32 + 16; }
 
 }  // namespace dpkem

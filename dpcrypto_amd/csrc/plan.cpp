@@ -189,7 +189,7 @@ int dpkem_plan_create(dpkem_engine* e, dpkem_plan** out, int nstreams, int chunk
 
 int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
                    const uint8_t* i1, int after, int* task_id) {
-  if (!p || p->exec || scheme < DPKEM_FRODO976 || scheme > DPKEM_FIRESABER ||
+  if (!p || p->exec || scheme < DPKEM_FRODO976 || scheme > DPKEM_FRODO1344 ||
       (scheme >= DPKEM_FRODO976_AES && p->form != DPKEM_FORM_BATCHED) || op < 0 || op > 2 || n == 0 ||
       n > (size_t)INT32_MAX)
     return DPKEM_EINVAL;

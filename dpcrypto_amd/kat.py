@@ -22,7 +22,8 @@ import numpy as np
 from . import kem as K
 
 NAMES = {K.FRODO976: "FrodoKEM-976-SHAKE", K.SABER: "Saber", K.FRODO976_AES: "FrodoKEM-976-AES",
-         K.LIGHTSABER: "LightSaber", K.FIRESABER: "FireSaber"}
+         K.LIGHTSABER: "LightSaber", K.FIRESABER: "FireSaber", K.FRODO640: "FrodoKEM-640-SHAKE",
+         K.FRODO1344: "FrodoKEM-1344-SHAKE"}
 
 
 class Drbg:
@@ -121,12 +122,14 @@ def _main(argv=None):
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["generate", "verify"])
-    ap.add_argument("--scheme", choices=["frodo976", "frodo976aes", "lightsaber", "saber", "firesaber"], required=True)
+    ap.add_argument("--scheme", choices=["frodo640", "frodo976", "frodo1344", "frodo976aes", "lightsaber", "saber",
+                                         "firesaber"], required=True)
     ap.add_argument("--count", type=int, default=100)
     ap.add_argument("file", nargs="?")
     a = ap.parse_args(argv)
     scheme = {"frodo976": K.FRODO976, "frodo976aes": K.FRODO976_AES, "saber": K.SABER,
-              "lightsaber": K.LIGHTSABER, "firesaber": K.FIRESABER}[a.scheme]
+              "lightsaber": K.LIGHTSABER, "firesaber": K.FIRESABER, "frodo640": K.FRODO640,
+              "frodo1344": K.FRODO1344}[a.scheme]
     eng, kp, en, de = _engine_ops()
     try:
         if a.cmd == "generate":

@@ -36,6 +36,8 @@ ERRORS = {
 FRODO976_AES = 2  # FrodoKEM-976-AES: Frodo-976 with A rows from AES-128 (batched form only)
 LIGHTSABER = 3    # Saber family, rank 2 (batched form only)
 FIRESABER = 4     # Saber family, rank 4 (batched form only)
+FRODO640 = 5      # Frodo-640-SHAKE (batched form only)
+FRODO1344 = 6     # Frodo-1344-SHAKE (batched form only)
 
 SIZES = {
     FRODO976: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
@@ -43,6 +45,8 @@ SIZES = {
     FRODO976_AES: dict(pk=15632, sk=31296, ct=15744, ss=24, kcoins=64, ecoins=24),
     LIGHTSABER: dict(pk=672, sk=1568, ct=736, ss=32, kcoins=96, ecoins=32),
     FIRESABER: dict(pk=1312, sk=3040, ct=1472, ss=32, kcoins=96, ecoins=32),
+    FRODO640: dict(pk=9616, sk=19888, ct=9720, ss=16, kcoins=48, ecoins=16),
+    FRODO1344: dict(pk=21520, sk=43088, ct=21632, ss=32, kcoins=80, ecoins=32),
 }
 
 _lib = None

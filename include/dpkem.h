@@ -34,6 +34,13 @@ extern "C" {
 #define DPKEM_FIRESABER 4    /* FireSaber (l = 4): pk 1312, sk 3040, ct 1472 B
                                 (both parameter sets at SABER-GPU/include/params.h:10-19, coins as Saber;
                                 engine, plans and KAT only, batched form only) */
+#define DPKEM_FRODO640 5     /* Frodo-640-SHAKE: pk 9616, sk 19888, ct 9720, ss 16 B; keygen coins
+                                s(16) || seedSE(16) || z(16) = 48 B, encaps coins mu = 16 B */
+#define DPKEM_FRODO1344 6    /* Frodo-1344-SHAKE: pk 21520, sk 43088, ct 21632, ss 32 B; keygen coins
+                                s(32) || seedSE(32) || z(16) = 80 B, encaps coins mu = 32 B
+                                (the round-3 FrodoKEM sets beside the reference's FrodoKEM-976,
+                                FrodoKEM976-GPU/include/params.h:9-24; engine, plans and KAT only,
+                                batched form only) */
 
 #define DPKEM_MEM_HOST 0   /* pointers are host memory (pageable or pinned) */
 #define DPKEM_MEM_DEVICE 1 /* pointers are device memory on the engine's GPU */
@@ -235,7 +242,8 @@ void dpkem_randombytes_drbg(uint8_t *out, size_t len, void *ctx);
 int dpkem_kat_seeds(size_t n, uint8_t *seeds);
 /* Keygen and encaps coins of n KAT counts: per count, the DRBG seeded with
  * that count's seed, drawn with the scheme's randombytes call pattern
- * (Frodo-976 SHAKE / AES: 64 B, then 24 B; Saber family: 3 x 32 B, then 32 B). */
+ * (FrodoKEM: 2 sec + 16 B, then sec B, sec = 16 / 24 / 32 for Frodo-640 / -976 / -1344;
+ * Saber family: 3 x 32 B, then 32 B). */
 int dpkem_kat_coins(int scheme, size_t n, const uint8_t *seeds, uint8_t *kcoins, uint8_t *ecoins);
 
 /* crypto_kem_{keypair,enc,dec} over arrays of n instances (host memory),
@@ -253,6 +261,12 @@ int dpkem_lightsaber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, co
 int dpkem_firesaber_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
 int dpkem_firesaber_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
 int dpkem_firesaber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+int dpkem_frodo640_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
+int dpkem_frodo640_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int dpkem_frodo640_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
+int dpkem_frodo1344_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
+int dpkem_frodo1344_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
+int dpkem_frodo1344_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);
 int dpkem_saber_crypto_kem_keypair(size_t n, uint8_t *pk, uint8_t *sk);
 int dpkem_saber_crypto_kem_enc(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk);
 int dpkem_saber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk);

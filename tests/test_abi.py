@@ -24,7 +24,8 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-@pytest.mark.parametrize("scheme", [D.FRODO976, D.SABER, D.FRODO976_AES, D.LIGHTSABER, D.FIRESABER])
+@pytest.mark.parametrize("scheme", [D.FRODO976, D.SABER, D.FRODO976_AES, D.LIGHTSABER, D.FIRESABER, D.FRODO640,
+                                    D.FRODO1344])
 def test_sizes_query(scheme):
     lib = D.load_library()
     vals = [ctypes.c_size_t() for _ in range(6)]
@@ -32,7 +33,7 @@ def test_sizes_query(scheme):
     s = D.SIZES[scheme]
     assert [v.value for v in vals] == [s["pk"], s["sk"], s["ct"], s["ss"], s["kcoins"], s["ecoins"]]
     assert lib.dpkem_sizes(7, *[None] * 6) == -1
-    assert lib.dpkem_sizes(5, *[None] * 6) == -1
+    assert lib.dpkem_sizes(-1, *[None] * 6) == -1
 
 
 def test_server_struct_layouts_match_numpy(tmp_path):

@@ -55,14 +55,18 @@ def test_every_draw_ends_with_an_update():
     assert kat.Drbg(seed, pers).bytes(32) == kat.Drbg(bytes(a ^ 0xA5 for a in seed)).bytes(32)
 
 
-@pytest.mark.parametrize("scheme", [D.FRODO976, D.SABER, D.FRODO976_AES, D.LIGHTSABER, D.FIRESABER])
+@pytest.mark.parametrize("scheme", [D.FRODO976, D.SABER, D.FRODO976_AES, D.LIGHTSABER, D.FIRESABER, D.FRODO640,
+                                    D.FRODO1344])
 def test_kat_coins_follow_the_schemes_call_pattern(scheme):
     sd = kat.seeds(3)
     kc, ec = kat.coins(scheme, sd)
+    # FrodoKEM: one draw of s || seedSE || z (2 sec + 16 bytes), one of mu (sec)
+    frodo_sec = {D.FRODO976: 24, D.FRODO976_AES: 24, D.FRODO640: 16, D.FRODO1344: 32}
     for i in range(3):
         d = kat.Drbg(bytes(sd[i]))
-        if scheme in (D.FRODO976, D.FRODO976_AES):
-            want_k, want_e = d.bytes(64), d.bytes(24)
+        if scheme in frodo_sec:
+            sec = frodo_sec[scheme]
+            want_k, want_e = d.bytes(2 * sec + 16), d.bytes(sec)
         else:
             want_k = d.bytes(32) + d.bytes(32) + d.bytes(32)
             want_e = d.bytes(32)
