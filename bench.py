@@ -7,10 +7,15 @@ inputs (coins) resident in HBM before the timed region, outputs left in HBM.
 value = KEM operations per second = 3 * 4096 * steps / seconds.
 
 Every op recomputes everything from its own inputs (encaps expands the
-public matrix from pk, decaps from sk), as separate parties would.
+public matrix from pk, decaps from sk), as separate parties would.  The step
+is a graph-captured plan; its keygen -> encaps -> decaps chain runs
+pipelined (stages of different ops that read disjoint inputs share launches,
+e.g. decaps' GenMatrix beside encaps' first hash stage).
 
 Reported beside it in the same JSON line:
   frodo976       FrodoKEM-976-SHAKE, batch 8192, keygen + encaps + decaps
+  frodo640 / frodo1344 / frodo976_aes / lightsaber / firesaber
+                 the other parameter sets on the same step
   plain          the one-workgroup-per-KEM comparator form on the same inputs
   latency        p50 / p99 of the per-step wall time (ms)
   roofline       dominant kernel of the headline workload, from HIP events on
@@ -19,6 +24,8 @@ Reported beside it in the same JSON line:
                  ops 1:4:4, one autotuned graph vs plain per-batch launches
   serving        Poisson arrivals into the continuous-batching server
                  (70% Saber, ops 1:4:4) vs one request per launch
+  saber_unpipelined
+                 the headline step as three dependent plan tasks
   saber_shared_matrix_chain
                  the headline steps with the plan's opt-in matrix sharing
                  (encaps / decaps reuse keygen's A); not the headline value
@@ -61,6 +68,26 @@ KERNEL_WORK = {
     "frodo_sa_fused<true>": (976 * 12, 0),
     "frodo_ss": (116, 0),
     "frodo_dec_mu": (0, 64 * 976),
+    # Frodo-640 (SHAKE128 chains, rate 168 B) and Frodo-1344 (SHAKE256, 136 B):
+    # A rows take 8 / 16 SHAKE128 blocks; noise streams 32n + 128 bytes
+    "frodo_keygen_pre,640": (1 + 122, 0),
+    "frodo_as_fused,640": (640 * 8, 0),
+    "frodo_keygen_post,640": (58, 0),
+    "frodo_encdec_pre<false>,640": (58 + 1 + 123, 0),
+    "frodo_encdec_pre<true>,640": (1 + 123, 0),
+    "frodo_sa_fused<false>,640": (640 * 8, 0),
+    "frodo_sa_fused<true>,640": (640 * 8, 0),
+    "frodo_ss,640": (58, 0),
+    "frodo_dec_mu,640": (0, 64 * 640),
+    "frodo_keygen_pre,1344": (1 + 317, 0),
+    "frodo_as_fused,1344": (1344 * 16, 0),
+    "frodo_keygen_post,1344": (159, 0),
+    "frodo_encdec_pre<false>,1344": (159 + 1 + 318, 0),
+    "frodo_encdec_pre<true>,1344": (1 + 318, 0),
+    "frodo_sa_fused<false>,1344": (1344 * 16, 0),
+    "frodo_sa_fused<true>,1344": (1344 * 16, 0),
+    "frodo_ss,1344": (160, 0),
+    "frodo_dec_mu,1344": (0, 64 * 1344),
 }
 
 
@@ -153,7 +180,7 @@ def run_workload(torch, eng, scheme, n, steps, warmup, seed, profile=False):
 
 
 def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=60.0, share_matrix=False,
-                      schedule=None):
+                      schedule=None, pipeline=True):
     """Time `steps` keygen+enc+dec steps, each one replay of a graph-captured
     plan (keygen -> enc -> dec chained per instance chunk); the plan's
     (streams, chunks) schedule is picked by the autotuner first."""
@@ -174,13 +201,13 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
     eng.set_stream(stream.cuda_stream)
 
     def build(ns, ch):
-        p = eng.plan(nstreams=ns, chunks=ch, share_matrix=share_matrix)
+        p = eng.plan(nstreams=ns, chunks=ch, share_matrix=share_matrix, pipeline=pipeline)
         t0 = p.keypair(scheme, pk, sk, kc)
         t1 = p.enc(scheme, ct, ss, pk, ec, after=t0)
         p.dec(scheme, ss2, ct, sk, after=t1)
         return p.build()
 
-    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}"
+    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}|pipe={int(pipeline)}"
     if schedule:
         best, tried = schedule, {}
     else:
@@ -414,6 +441,18 @@ def main():
         progress("frodo976 encaps + decaps against one key (A expanded once per 4 instances)")
         result["frodo976_shared_key"] = run_shared_key(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps,
                                                        0x44504B454D000006)
+        for name, sc in (("frodo640", D.FRODO640), ("frodo1344", D.FRODO1344)):
+            progress(f"{name} batch {args.frodo_batch}: autotuned graph steps, then an eager pass with HIP events")
+            fs = run_plan_workload(torch, eng, sc, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000010 + sc,
+                                   tune_budget_s=15.0, schedule=sched)
+            fs_eager = run_workload(torch, eng, sc, args.frodo_batch, 1, 0, 0x44504B454D000010 + sc, True)
+            result[name] = {
+                "workload": f"{name}-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
+                "ops_per_s": round(fs["ops_per_s"], 1), "ms_per_step": round(fs["wall_s"] * 1e3 / args.frodo_steps, 3),
+                "roundtrip_ok": fs["roundtrip_ok"] and fs_eager["roundtrip_ok"],
+                "schedule": {"nstreams": fs["schedule"]["nstreams"], "chunks": fs["schedule"]["chunks"]},
+                "roofline": roofline_from(fs_eager["kernels"], args.frodo_batch),
+                "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fs_eager["kernels"].items()}}
     for name, sc in (("lightsaber", D.LIGHTSABER), ("firesaber", D.FIRESABER)):
         progress(f"{name} batch {args.saber_batch}: autotuned graph steps")
         fam = run_plan_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup, 0x44504B454D000008 + sc,
@@ -423,6 +462,15 @@ def main():
                         "ms_per_step": round(fam["wall_s"] * 1e3 / args.steps, 4),
                         "roundtrip_ok": fam["roundtrip_ok"],
                         "schedule": {"nstreams": fam["schedule"]["nstreams"], "chunks": fam["schedule"]["chunks"]}}
+    progress("saber with the chain run as three dependent tasks (no stage pipelining)")
+    unpiped = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
+                                tune_budget_s=20.0, schedule=sched, pipeline=False)
+    result["saber_unpipelined"] = {
+        "ops_per_s": round(unpiped["ops_per_s"], 1), "roundtrip_ok": unpiped["roundtrip_ok"],
+        "ms_per_step": round(unpiped["wall_s"] * 1e3 / args.steps, 4),
+        "schedule": {"nstreams": unpiped["schedule"]["nstreams"], "chunks": unpiped["schedule"]["chunks"]},
+        "note": "keygen, encaps and decaps as three dependent plan tasks (ten launches per chunk); the headline "
+                "runs the same work with stages of different ops sharing launches (eight per chunk)"}
     progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
     shared = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                tune_budget_s=20.0, share_matrix=True, schedule=sched)

@@ -25,6 +25,12 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
 void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
                         uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch);
 constexpr size_t kSaberChainScratchPerInst = 3744 + 768 + 32 + 64 + 4;
+// Pipelined keygen -> encaps -> decaps of the same instances on one stream
+// (saber.hip): stages of different ops that read disjoint inputs share
+// launches, every op computes all of its own hashes.  Saber family, rank l.
+size_t saber_pipe_scratch_per_inst(int l);
+void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l);
 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch

@@ -30,10 +30,13 @@ struct Task {
   int after;  // task index this one depends on, or -1
 };
 
+enum ChainKind { kNoChain = 0, kShareChain = 1, kPipeChain = 2 };
+
 struct Sub {
   int task;
   int chain = -1;          // >= 0: Saber keygen/enc/dec chain (tasks task, chain, chain2) in one launch sequence
   int chain2 = -1;
+  int chain_kind = kNoChain;
   size_t begin, count;
   int stream;
   size_t scratch_off;
@@ -81,28 +84,38 @@ void release(dpkem_plan* p) {
 
 size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Saber keygen t0 -> enc t1 -> dec t2 over the same arrays (t1 consumes t0's
-// pk, t2 consumes t1's ct and t0's sk): each chunk runs the three ops back to
-// back on one stream and expands the public matrix once.  t1's and t2's
-// sub-tasks are folded into t0's, so their dependants wait for whole chains.
-bool find_saber_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
-  if (!(p->flags & DPKEM_PLAN_SHARE_MATRIX)) return false;
+int saber_rank_of(int scheme) {
+  return scheme == DPKEM_SABER ? 3 : scheme == DPKEM_LIGHTSABER ? 2 : scheme == DPKEM_FIRESABER ? 4 : 0;
+}
+
+// Saber-family keygen t0 -> enc t1 -> dec t2 over the same arrays (t1
+// consumes t0's pk, t2 consumes t1's ct and t0's sk): each chunk runs the
+// three ops as one launch sequence on one stream, either pipelined (default:
+// stages of different ops that read disjoint inputs share launches) or, with
+// DPKEM_PLAN_SHARE_MATRIX and Saber l = 3, expanding the public matrix once.
+// t1's and t2's sub-tasks are folded into t0's, so their dependants wait for
+// whole chains.  Returns the chain kind.
+int find_saber_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
   const Task& K = p->tasks[t0];
-  if (K.scheme != DPKEM_SABER || K.op != kKeygen || K.after != -1) return false;
+  if (p->form != DPKEM_FORM_BATCHED || !saber_rank_of(K.scheme) || K.op != kKeygen || K.after != -1) return kNoChain;
+  const int kind = (p->flags & DPKEM_PLAN_SHARE_MATRIX) && K.scheme == DPKEM_SABER ? kShareChain
+                   : (p->flags & DPKEM_PLAN_NO_PIPELINE)                           ? kNoChain
+                                                                                   : kPipeChain;
+  if (kind == kNoChain) return kNoChain;
   for (size_t a = t0 + 1; a < p->tasks.size(); ++a) {
     const Task& E = p->tasks[a];
-    if (E.after != t0 || E.scheme != DPKEM_SABER || E.op != kEnc || E.n != K.n || E.i0 != K.o0) continue;
+    if (E.after != t0 || E.scheme != K.scheme || E.op != kEnc || E.n != K.n || E.i0 != K.o0) continue;
     for (size_t b = a + 1; b < p->tasks.size(); ++b) {
       const Task& Dd = p->tasks[b];
-      if (Dd.after == (int)a && Dd.scheme == DPKEM_SABER && Dd.op == kDec && Dd.n == K.n && Dd.i0 == E.o0 &&
+      if (Dd.after == (int)a && Dd.scheme == K.scheme && Dd.op == kDec && Dd.n == K.n && Dd.i0 == E.o0 &&
           Dd.i1 == K.o1) {
         *t1 = (int)a;
         *t2 = (int)b;
-        return true;
+        return kind;
       }
     }
   }
-  return false;
+  return kNoChain;
 }
 
 // Split every task into chunks and assign streams and dependencies.
@@ -121,7 +134,7 @@ void schedule(dpkem_plan* p) {
     }
     const Task& T = p->tasks[t];
     int t1 = -1, t2 = -1;
-    if (find_saber_chain(p, (int)t, &t1, &t2)) {
+    if (const int kind = find_saber_chain(p, (int)t, &t1, &t2)) {
       absorbed[t1] = absorbed[t2] = (int)t;
       chain_of[t] = chains++;
       const size_t k = std::max<size_t>(1, std::min<size_t>((size_t)p->chunks, T.n));
@@ -131,11 +144,14 @@ void schedule(dpkem_plan* p) {
         s.task = (int)t;
         s.chain = t1;
         s.chain2 = t2;
+        s.chain_kind = kind;
         s.begin = b;
         s.count = std::min(per, T.n - b);
         s.stream = (int)((c + (size_t)chain_of[t]) % (size_t)p->nstreams);
         s.scratch_off = off;
-        off += round_up(s.count * kSaberChainScratchPerInst + 16 * 256);
+        const size_t per_inst =
+            kind == kShareChain ? kSaberChainScratchPerInst : saber_pipe_scratch_per_inst(saber_rank_of(T.scheme));
+        off += round_up(s.count * per_inst + 16 * 256);
         task_subs[t].push_back((int)p->subs.size());
         p->subs.push_back(s);
       }
@@ -208,7 +224,7 @@ int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uin
 }
 
 int dpkem_plan_set_options(dpkem_plan* p, unsigned flags) {
-  if (!p || p->exec || (flags & ~DPKEM_PLAN_SHARE_MATRIX)) return DPKEM_EINVAL;
+  if (!p || p->exec || (flags & ~(DPKEM_PLAN_SHARE_MATRIX | DPKEM_PLAN_NO_PIPELINE))) return DPKEM_EINVAL;
   p->flags = flags;
   return DPKEM_OK;
 }
@@ -266,8 +282,18 @@ int dpkem_plan_build(dpkem_plan* p) {
       const Task& E = p->tasks[s.chain];
       const Task& Dd = p->tasks[s.chain2];
       const size_t b = s.begin;
-      saber_chain_launch(st, (int)s.count, T.i0 + b * 96, E.i1 + b * 32, T.o0 + b * 992, T.o1 + b * 2304,
-                         E.o0 + b * 1088, E.o1 + b * 32, Dd.o0 + b * 32, p->scratch + s.scratch_off);
+      size_t ko[2], ki[2], eo[2], ei[2];
+      io_sizes(T.scheme, kKeygen, ko, ki);
+      io_sizes(T.scheme, kEnc, eo, ei);
+      const uint8_t* kc = T.i0 + b * ki[0];
+      const uint8_t* ec = E.i1 + b * ei[1];
+      uint8_t *pk = T.o0 + b * ko[0], *sk = T.o1 + b * ko[1];
+      uint8_t *ct = E.o0 + b * eo[0], *ss = E.o1 + b * eo[1], *ss2 = Dd.o0 + b * eo[1];
+      if (s.chain_kind == kShareChain)
+        saber_chain_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off);
+      else
+        saber_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
+                          saber_rank_of(T.scheme));
       ok = ok && hipGetLastError() == hipSuccess;
       ok = ok && hipEventRecord(s.done, st) == hipSuccess;
       continue;

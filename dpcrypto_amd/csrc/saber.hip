@@ -57,10 +57,9 @@ static_assert(SaberP<4>::pk == 1312 && SaberP<4>::sk == 3040 && SaberP<4>::ct ==
 
 // ----------------------------------------------------------------- hash1
 // Lane-pair sponges (keccak2.hpp): lanes 2k / 2k+1 hold the even / odd bit
-// halves of instance k's state.  blockDim = 128: wave 0 runs role 0 and wave
-// 1 role 1 for the same 32 instances (role_sel >= 0: that role only, 64
-// threads).  GenMatrix / GenSecret streams leave in the sliced layout (half of
-// word w at u32 index 2w + p).
+// halves of instance k's state, 32 instances per wave.  Role 0 is the
+// GenMatrix chain, role 1 the message / secret chain.  GenMatrix / GenSecret
+// streams leave in the sliced layout (half of word w at u32 index 2w + p).
 
 using k2::absorb2_global;
 using k2::absorb2_halves;
@@ -68,18 +67,19 @@ using k2::absorb2_regs;
 using k2::squeeze2_sliced;
 using k2::store2_words;
 
+// The bodies below take the 32-instance group `vb` and the role explicitly
+// so that the pipelined chain kernels can run several of them in one launch.
 #define SAB_PAIR_SETUP                                   \
   const int lane = threadIdx.x & 63;                     \
   const uint32_t p = lane & 1;                           \
-  const int inst = blockIdx.x * 32 + (lane >> 1);        \
+  const int inst = vb * 32 + (lane >> 1);                \
   if (inst >= n) return;
 
 template <int L>
-__global__ void __launch_bounds__(128) sab_keygen_hash1(int role_sel, int n, const uint8_t* __restrict__ coins,
-                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
-                                                        uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
+__device__ __forceinline__ void keygen_hash1_body(int vb, int role, int n, const uint8_t* __restrict__ coins,
+                                                  uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                  uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
   using P = SaberP<L>;
-  const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
   SAB_PAIR_SETUP
   const uint8_t* c = coins + (size_t)inst * kSaberKeygenCoins;
   uint32_t a[25];
@@ -111,12 +111,11 @@ __global__ void __launch_bounds__(128) sab_keygen_hash1(int role_sel, int n, con
 }
 
 template <int L>
-__global__ void __launch_bounds__(128) sab_enc_hash1(int role_sel, int n, const uint8_t* __restrict__ pk,
-                                                     const uint8_t* __restrict__ coins, uint8_t* __restrict__ araw,
-                                                     uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
-                                                     uint8_t* __restrict__ krbuf) {
+__device__ __forceinline__ void enc_hash1_body(int vb, int role, int n, const uint8_t* __restrict__ pk,
+                                               const uint8_t* __restrict__ coins, uint8_t* __restrict__ araw,
+                                               uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
+                                               uint8_t* __restrict__ krbuf) {
   using P = SaberP<L>;
-  const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
   SAB_PAIR_SETUP
   const uint8_t* pkp = pk + (size_t)inst * P::pk;
   uint32_t a[25];
@@ -151,11 +150,10 @@ __global__ void __launch_bounds__(128) sab_enc_hash1(int role_sel, int n, const 
 }
 
 template <int L>
-__global__ void __launch_bounds__(128) sab_dec_hash1(int role_sel, int n, const uint8_t* __restrict__ sk,
-                                                     const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
-                                                     uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
+__device__ __forceinline__ void dec_hash1_body(int vb, int role, int n, const uint8_t* __restrict__ sk,
+                                               const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
+                                               uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
   using P = SaberP<L>;
-  const int role = role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6);
   SAB_PAIR_SETUP
   const uint8_t* s = sk + (size_t)inst * P::sk;
   uint32_t a[25];
@@ -186,8 +184,10 @@ __global__ void __launch_bounds__(128) sab_dec_hash1(int role_sel, int n, const 
 
 // ----------------------------------------------------------------- hash2
 
+// H(pk) into sk.
 template <int L>
-__global__ void __launch_bounds__(64) sab_keygen_hash2(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
+__device__ __forceinline__ void keygen_hash2_body(int vb, int n, const uint8_t* __restrict__ pk,
+                                                  uint8_t* __restrict__ sk) {
   using P = SaberP<L>;
   SAB_PAIR_SETUP
   uint32_t a[25];
@@ -198,9 +198,9 @@ __global__ void __launch_bounds__(64) sab_keygen_hash2(int n, const uint8_t* __r
 // K = SHA3-256(kr[0:32] || H(ct)); for decaps kr[0:32] is replaced by z when
 // the re-encryption differs (constant-time select on the mismatch flag).
 template <int L>
-__global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ krbuf,
-                                                const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
-                                                uint8_t* __restrict__ ss) {
+__device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restrict__ ct,
+                                           const uint8_t* __restrict__ krbuf, const uint8_t* __restrict__ sk,
+                                           const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
   using P = SaberP<L>;
   SAB_PAIR_SETUP
   uint32_t a[25], m[8];
@@ -217,6 +217,74 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
   }
   absorb2_halves<kRateSha3_256, 8>(a, p, m, kDsSha3);
   store2_words<4>(a, p, ss + (size_t)inst * kSaberSs);
+}
+
+// ----------------------------------------------------------------- hash kernels
+// hash1: blockDim = 128, wave 0 runs role 0 and wave 1 role 1 for the same 32
+// instances (role_sel >= 0: that role only, 64 threads).  hash2: 64 threads
+// per 32 instances.
+
+template <int L>
+__global__ void __launch_bounds__(128) sab_keygen_hash1(int role_sel, int n, const uint8_t* __restrict__ coins,
+                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                        uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
+  keygen_hash1_body<L>(blockIdx.x, role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6), n, coins, pk, sk, araw, sraw);
+}
+
+template <int L>
+__global__ void __launch_bounds__(128) sab_enc_hash1(int role_sel, int n, const uint8_t* __restrict__ pk,
+                                                     const uint8_t* __restrict__ coins, uint8_t* __restrict__ araw,
+                                                     uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
+                                                     uint8_t* __restrict__ krbuf) {
+  enc_hash1_body<L>(blockIdx.x, role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6), n, pk, coins, araw, sraw, mbuf,
+                    krbuf);
+}
+
+template <int L>
+__global__ void __launch_bounds__(128) sab_dec_hash1(int role_sel, int n, const uint8_t* __restrict__ sk,
+                                                     const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
+                                                     uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
+  dec_hash1_body<L>(blockIdx.x, role_sel >= 0 ? role_sel : (int)(threadIdx.x >> 6), n, sk, mbuf, araw, sraw, krbuf);
+}
+
+template <int L>
+__global__ void __launch_bounds__(64) sab_keygen_hash2(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
+  keygen_hash2_body<L>(blockIdx.x, n, pk, sk);
+}
+
+template <int L>
+__global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ krbuf,
+                                                const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
+                                                uint8_t* __restrict__ ss) {
+  hash2_body<L>(blockIdx.x, n, ct, krbuf, sk, fail, ss);
+}
+
+// Pipelined keygen -> encaps -> decaps chain (saber_pipe_launch).  Stages of
+// different ops that read disjoint inputs share a launch:
+//   x1  blocks [0, nb): encaps' hash1 (both roles); blocks [nb, 2 nb): wave 0
+//       keygen's H(pk) into sk, wave 1 decaps' GenMatrix from the seedA in sk
+//       (written by keygen's hash1, two launches earlier);
+//   x2  wave 0: decaps' message chain (G, GenSecret); wave 1: encaps' K.
+// Every op still computes all of its own hashes; only their order changes.
+template <int L>
+__global__ void __launch_bounds__(128) sab_pipe_x1(int n, int nb, const uint8_t* __restrict__ pk,
+                                                   const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ sk,
+                                                   uint8_t* __restrict__ araw_e, uint8_t* __restrict__ sraw_e,
+                                                   uint8_t* __restrict__ mbuf_e, uint8_t* __restrict__ krbuf_e,
+                                                   uint8_t* __restrict__ araw_d) {
+  const int wave = threadIdx.x >> 6, b = blockIdx.x;
+  if (b < nb) enc_hash1_body<L>(b, wave, n, pk, ecoins, araw_e, sraw_e, mbuf_e, krbuf_e);
+  else if (wave == 0) keygen_hash2_body<L>(b - nb, n, pk, sk);
+  else dec_hash1_body<L>(b - nb, 0, n, sk, nullptr, araw_d, nullptr, nullptr);
+}
+
+template <int L>
+__global__ void __launch_bounds__(128) sab_pipe_x2(int n, const uint8_t* __restrict__ sk,
+                                                   const uint8_t* __restrict__ mbuf_d, uint8_t* __restrict__ sraw_d,
+                                                   uint8_t* __restrict__ krbuf_d, const uint8_t* __restrict__ ct,
+                                                   const uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ ss) {
+  if ((threadIdx.x >> 6) == 0) dec_hash1_body<L>(blockIdx.x, 1, n, sk, mbuf_d, nullptr, sraw_d, krbuf_d);
+  else hash2_body<L>(blockIdx.x, n, ct, krbuf_e, nullptr, nullptr, ss);
 }
 
 // ----------------------------------------------------------------- products
@@ -562,6 +630,68 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
   const int blocks = (int)std::min<size_t>((total + 255) / 256, 4096);
   hipLaunchKernelGGL(broadcast_rows, dim3(blocks), dim3(256), 0, st, n, reinterpret_cast<const uint32_t*>(src), words,
                      reinterpret_cast<uint32_t*>(dst));
+}
+
+// Pipelined keygen -> encaps -> decaps of the same n instances on one stream
+// (see sab_pipe_x1 / sab_pipe_x2).  Per instance the serial Keccak path drops
+// from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 23 + 10 + 10 permutations
+// (keygen's H(pk), encaps' K and decaps' GenMatrix run beside other stages),
+// and ten launches become eight.  scratch: saber_pipe_scratch_per_inst(l)
+// bytes per instance: two GenMatrix, two GenSecret, two m and two kr buffers
+// (encaps' and decaps'; keygen's streams reuse decaps' buffers, which the
+// stream order makes safe) and the mismatch flags, plus 256-byte alignment of
+// nine arrays per launch.
+template <int L>
+static size_t pipe_scratch() {
+  using P = SaberP<L>;
+  return 2 * ((size_t)P::matrix_bytes + P::secret_bytes + 32 + 64) + 4;
+}
+
+size_t saber_pipe_scratch_per_inst(int l) {
+  return l == 2 ? pipe_scratch<2>() : l == 4 ? pipe_scratch<4>() : pipe_scratch<3>();
+}
+
+template <int L>
+static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk, uint8_t* ct,
+                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+  using P = SaberP<L>;
+  size_t off = 0;
+  auto take = [&](size_t per) {
+    uint8_t* q = scratch + off;
+    off += ((size_t)n * per + 255) & ~(size_t)255;
+    return q;
+  };
+  uint8_t* araw_d = take(P::matrix_bytes);  // keygen's GenMatrix, then decaps'
+  uint8_t* sraw_d = take(P::secret_bytes);  // keygen's GenSecret, then decaps'
+  uint8_t* araw_e = take(P::matrix_bytes);
+  uint8_t* sraw_e = take(P::secret_bytes);
+  uint8_t* mbuf_e = take(32);
+  uint8_t* mbuf_d = take(32);
+  uint8_t* krbuf_e = take(64);
+  uint8_t* krbuf_d = take(64);
+  uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
+  const int nb = blocks_for(n, 32);
+  const dim3 g(nb), b64(64), b128(128), bp(P::threads);
+  DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash1"), sab_keygen_hash1<L>, g, b128, st, -1, n, kc, pk, sk, araw_d, sraw_d);
+  SabPolyArgs ak{n, araw_d, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
+  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, dim3(2 * nb), b128, st, n, nb, pk, ec, sk, araw_e, sraw_e,
+               mbuf_e, krbuf_e, araw_d);
+  SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
+  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
+  DPKEM_LAUNCH(SAB_NAME("sab_dec_inner"), sab_dec_inner<L>, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf_d);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, g, b128, st, n, sk, mbuf_d, sraw_d, krbuf_d, ct, krbuf_e, ss);
+  SabPolyArgs ad{n, araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, fail};
+  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabReenc>"), (sab_poly<kSabReenc, L>), dim3(n), bp, st, ad);
+  DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, g, b64, st, n, ct, krbuf_d, sk, fail, ss2);
+}
+
+void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l) {
+  if (n <= 0) return;
+  if (l == 2) pipe_l<2>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+  else if (l == 4) pipe_l<4>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+  else pipe_l<3>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
 }
 
 // Chained keygen -> encaps -> decaps of the same n Saber (l = 3) instances on

@@ -34,9 +34,15 @@ def _dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("scheme,n,nstreams,chunks,share", [(SABER, 300, 4, 4, False), (SABER, 300, 4, 4, True),
-                                                            (SABER, 97, 2, 3, True), (FRODO, 10, 3, 3, False)])
-def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share):
+LIGHTSABER, FIRESABER = 3, 4
+
+
+@pytest.mark.parametrize("scheme,n,nstreams,chunks,share,pipe", [
+    (SABER, 300, 4, 4, False, True), (SABER, 300, 4, 4, False, False), (SABER, 300, 4, 4, True, True),
+    (SABER, 97, 2, 3, True, True), (SABER, 97, 2, 3, False, True), (LIGHTSABER, 70, 2, 2, False, True),
+    (FIRESABER, 45, 3, 2, False, True), (FRODO, 10, 3, 3, False, True)])
+def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share,
+                                                     pipe):
     import dpcrypto_amd as D
     torch = torch_cuda
     s = D.SIZES[scheme]
@@ -51,13 +57,15 @@ def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, sc
     ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
     ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
     eng.set_stream(torch.cuda.current_stream().cuda_stream)
-    plan = eng.plan(nstreams=nstreams, chunks=chunks, share_matrix=share)
+    print(f"plan scheme={scheme} n={n} {nstreams}x{chunks} share={share} pipe={pipe}", flush=True)
+    plan = eng.plan(nstreams=nstreams, chunks=chunks, share_matrix=share, pipeline=pipe)
     t0 = plan.keypair(scheme, pk, sk, dkc)
     t1 = plan.enc(scheme, ct, ss, pk, dec_, after=t0)
     plan.dec(scheme, ss2, ct, sk, after=t1)
     plan.build()
-    # with matrix sharing, Saber keygen -> enc -> dec runs as one launch sequence per chunk
-    per_chunk = 1 if share and scheme == SABER else 3
+    # a Saber-family keygen -> enc -> dec chain runs as one launch sequence per
+    # chunk (pipelined, or with matrix sharing); otherwise three dependent tasks
+    per_chunk = 1 if scheme != FRODO and ((share and scheme == SABER) or pipe) else 3
     assert plan.info()["subtasks"] == per_chunk * min(chunks, n)
     for rep in range(3):
         for t in (pk, sk, ct, ss, ss2):
