@@ -198,13 +198,19 @@ struct DevOps {
 };
 
 // Full permutation on this lane's half of the state (p = lane parity).
+// Both halves of the round constant are read at the uniform index r (scalar
+// loads, waited on with lgkmcnt) and the lane's half is selected with a mask:
+// a per-lane table index would become a vector load whose vmcnt wait also
+// waits for every store the sponge still has in flight.
 __device__ __forceinline__ void keccakf2(uint32_t a[25], uint32_t p) {
   uint32_t sh[32];
 #pragma unroll
   for (int k = 0; k < 32; ++k) sh[k] = DevOps::shift(p, k);  // only the used entries survive
+  const uint32_t odd = 0u - p;
 #pragma unroll 2
   for (int r = 0; r < 24; ++r) {
-    const uint32_t rc = p ? kRCo[r] : kRCe[r];
+    const uint32_t re = kRCe[r], ro = kRCo[r];
+    const uint32_t rc = re ^ ((re ^ ro) & odd);
     round2<DevOps>(a, sh, rc);
   }
 }

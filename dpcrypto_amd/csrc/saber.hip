@@ -47,7 +47,7 @@ struct SaberP {
   static constexpr int ct = vecp + 32 * et;
   static constexpr int matrix_bytes = L * vecq;   // GenMatrix stream
   static constexpr int secret_bytes = 32 * L * mu;  // GenSecret stream (256 mu / 8 per polynomial)
-  static constexpr int threads = 32 * (L + 1);    // product workgroup
+  static constexpr int threads = 64;              // product workgroup: one wave (int8 MFMA)
 };
 static_assert(SaberP<3>::pk == kSaberPk && SaberP<3>::sk == kSaberSk && SaberP<3>::ct == kSaberCt, "Saber sizes");
 static_assert(SaberP<3>::matrix_bytes == kSaberGenMatrixBytes && SaberP<3>::secret_bytes == kSaberGenSecretBytes,
@@ -304,19 +304,28 @@ struct SabPolyArgs {
   uint32_t* fail;        // reenc: per-instance mismatch flag
 };
 
-constexpr int kARow = 264;  // u16 row stride of the plain A / b polynomials (spreads LDS banks)
+constexpr int kARow = 264;  // u16 row stride of plain polynomials in the dot2 products (spreads LDS banks)
 
-// The secret (s or s', small) is the Toeplitz-staged factor: L staged
-// polynomials serve all L^2 MatVec products and the L inner-product
-// products; A and b are read as plain rows, each 16-byte read broadcast to
-// the 32 threads of an output row.
+// Products on int8 MFMA (saber_poly.hpp, toep8_*): one 64-thread wave per
+// instance.  The secret (s or s', small) is the staged Toeplitz factor; the
+// B operand holds, per K-block j, the signed limbs of the matrix polynomials
+// (A[i][j], or A[j][i] for keygen's A^T s) in columns i / 8 + i and of b[j]
+// (inner product) in columns L / 8 + L.  Per instance: L x 64 matrix
+// instructions, 28 L Toeplitz operand reads.
 template <int L>
 struct SabPolyLds {
-  uint16_t Ts[L * kToepStride];           // Toeplitz form of s / s'
-  uint16_t Ap[(L * L + L) * kARow];       // A (L^2 polynomials) then b (L)
-  alignas(8) uint32_t stage[SaberP<L>::matrix_bytes / 4];
-  alignas(8) uint32_t sstage[SaberP<L>::secret_bytes / 4];
-  uint8_t outb[SaberP<L>::ct + 64];
+  alignas(16) int8_t V[L * kVPoly];                // staged Toeplitz copies of s / s'
+  alignas(16) int8_t Lb[L * 2 * (L + 1) * 256];    // B-operand limbs: [K-block j][slot][k]
+  union {
+    struct {
+      alignas(8) uint32_t stage[SaberP<L>::matrix_bytes / 4];
+      alignas(8) uint32_t sstage[SaberP<L>::secret_bytes / 4];
+    } in;
+    struct {
+      alignas(16) uint16_t part[2 * (L + 1) * 256];  // limb sums per slot (mod 2^16)
+      alignas(8) uint8_t outb[SaberP<L>::ct + 64];
+    } out;
+  } u;
   uint32_t diff;
 };
 
@@ -328,9 +337,10 @@ __device__ __forceinline__ uint64_t lds_le(const uint8_t* b, int nbytes) {
 }
 
 template <int MODE, int L>
-__global__ void __launch_bounds__(SaberP<L>::threads) sab_poly(SabPolyArgs a) {
+__global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
   using P = SaberP<L>;
-  constexpr int NT = P::threads;
+  constexpr int NT = 64;
+  constexpr int SLOTS = 2 * (L + 1);
   __shared__ __align__(16) SabPolyLds<L> S;
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
@@ -339,17 +349,17 @@ __global__ void __launch_bounds__(SaberP<L>::threads) sab_poly(SabPolyArgs a) {
   // 1. rebuild the GenMatrix and GenSecret bytes (lane-pair sliced streams) in LDS
   const uint32_t* ar = reinterpret_cast<const uint32_t*>(a.araw + (size_t)inst * P::matrix_bytes);
   for (int w = tid; w < P::matrix_bytes / 8; w += NT)
-    reinterpret_cast<uint64_t*>(S.stage)[w] = k2::sliced_word(ar, w);
+    reinterpret_cast<uint64_t*>(S.u.in.stage)[w] = k2::sliced_word(ar, w);
   const uint32_t* sr = reinterpret_cast<const uint32_t*>(a.sraw + (size_t)inst * P::secret_bytes);
   for (int w = tid; w < P::secret_bytes / 8; w += NT)
-    reinterpret_cast<uint64_t*>(S.sstage)[w] = k2::sliced_word(sr, w);
+    reinterpret_cast<uint64_t*>(S.u.in.sstage)[w] = k2::sliced_word(sr, w);
   if (tid == 0) S.diff = 0;
   __syncthreads();
 
-  // 2. CBD (mu bits per coefficient: HW(low mu/2) - HW(high mu/2)) straight
-  //    into the Toeplitz form of s; A (groups of 8 13-bit values) and b (groups
-  //    of 4 10-bit values, from pk) unpacked as plain rows
-  const uint8_t* ssb = reinterpret_cast<const uint8_t*>(S.sstage);
+  // 2. CBD (mu bits per coefficient: HW(low mu/2) - HW(high mu/2)) into the
+  //    Toeplitz copies; A (groups of 8 13-bit values) and b (groups of 4
+  //    10-bit values, from pk) into the limb columns
+  const uint8_t* ssb = reinterpret_cast<const uint8_t*>(S.u.in.sstage);
   #pragma unroll 1
   for (int g = tid; g < 64 * L; g += NT) {  // 4 coefficients from mu / 2 bytes
     const uint64_t v = lds_le(ssb + g * (P::mu / 2), P::mu / 2);
@@ -358,20 +368,27 @@ __global__ void __launch_bounds__(SaberP<L>::threads) sab_poly(SabPolyArgs a) {
       const uint32_t x = (uint32_t)(v >> (P::mu * k)) & ((1u << P::mu) - 1);
       const int c = 4 * g + k;
       const int half = P::mu / 2;
-      toep_put(S.Ts, c >> 8, c & 255, (uint16_t)(popc(x & ((1u << half) - 1)) - popc(x >> half)));
+      toep8_put(S.V + (c >> 8) * kVPoly, c & 255, (int)popc(x & ((1u << half) - 1)) - (int)popc(x >> half));
     }
   }
-  const uint8_t* sb = reinterpret_cast<const uint8_t*>(S.stage);
+  const uint8_t* sb = reinterpret_cast<const uint8_t*>(S.u.in.stage);
   #pragma unroll 1
   for (int g = tid; g < 32 * L * L; g += NT) {
     uint16_t v[8];
     unpack13_group(v, sb + 13 * g);
-    uint4 q;
-    q.x = v[0] | ((uint32_t)v[1] << 16);
-    q.y = v[2] | ((uint32_t)v[3] << 16);
-    q.z = v[4] | ((uint32_t)v[5] << 16);
-    q.w = v[6] | ((uint32_t)v[7] << 16);
-    *reinterpret_cast<uint4*>(S.Ap + (g >> 5) * kARow + (g & 31) * 8) = q;
+    const int poly = g >> 5, r = poly / L, c = poly % L;
+    const int col = MODE == kSabKeygen ? c : r, j = MODE == kSabKeygen ? r : c;
+    uint64_t lo = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      int8_t l8, h8;
+      limbs8(v[i], &l8, &h8);
+      lo |= (uint64_t)(uint8_t)l8 << (8 * i);
+      hi |= (uint64_t)(uint8_t)h8 << (8 * i);
+    }
+    int8_t* blk = S.Lb + j * SLOTS * 256 + 8 * (g & 31);
+    *reinterpret_cast<uint64_t*>(blk + col * 256) = lo;
+    *reinterpret_cast<uint64_t*>(blk + (L + 1 + col) * 256) = hi;
   }
   if (MODE != kSabKeygen) {
     const uint8_t* pkb = a.pk + (size_t)inst * a.pk_stride;
@@ -382,56 +399,91 @@ __global__ void __launch_bounds__(SaberP<L>::threads) sab_poly(SabPolyArgs a) {
       for (int i = 0; i < 5; ++i) bytes[i] = pkb[5 * g + i];
       uint16_t v[4];
       unpack10_group(v, bytes);
-      uint2 q;
-      q.x = v[0] | ((uint32_t)v[1] << 16);
-      q.y = v[2] | ((uint32_t)v[3] << 16);
-      *reinterpret_cast<uint2*>(S.Ap + (L * L + (g >> 6)) * kARow + (g & 63) * 4) = q;
+      uint32_t lo = 0, hi = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int8_t l8, h8;
+        limbs8(v[i], &l8, &h8);
+        lo |= (uint32_t)(uint8_t)l8 << (8 * i);
+        hi |= (uint32_t)(uint8_t)h8 << (8 * i);
+      }
+      int8_t* blk = S.Lb + (g >> 6) * SLOTS * 256 + 4 * (g & 63);
+      *reinterpret_cast<uint32_t*>(blk + L * 256) = lo;
+      *reinterpret_cast<uint32_t*>(blk + (2 * L + 1) * 256) = hi;
     }
   }
   __syncthreads();
 
-  // 3. products: threads 0..32L-1 -> b' row i = tid/32 (A or A^T); the last 32 -> v' = b . s'
-  const int t0 = 8 * (tid & 31);
-  const int row = tid >> 5;  // 0..L-1 matrix rows, L the inner product
-  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (MODE != kSabKeygen || row < L) {
-#pragma unroll 1
-    for (int j = 0; j < L; ++j) {  // each toep_dot is already 1024 unrolled dot2
-      const int Pi = row < L ? (MODE == kSabKeygen ? j * L + row : row * L + j) : L * L + j;
-      toep_dot(acc, S.Ts, j, S.Ap + Pi * kARow, t0);
+  // 3. products: result tile a, lane (g, n) = rows 16 a + 4 g + 0..3 of column n
+  const int slot = toep8_slot(tid & 15, L, MODE != kSabKeygen);
+  v4i8x4 acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = v4i8x4{0, 0, 0, 0};
+  #pragma unroll 1
+  for (int j = 0; j < L; ++j) {
+    v4i8x4 bop[4];
+    const int8_t* colp = S.Lb + (j * SLOTS + (slot < 0 ? 0 : slot)) * 256 + 16 * (tid >> 4);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      bop[b] = slot >= 0 ? *reinterpret_cast<const v4i8x4*>(colp + 64 * b) : v4i8x4{0, 0, 0, 0};
+    toep8_block(acc, S.V + j * kVPoly, bop, tid, [](v4i8x4 x, v4i8x4 y, v4i8x4 z) {
+      return __builtin_amdgcn_mfma_i32_16x16x64_i8(x, y, z, 0, 0, 0);
+    });
+  }
+  __syncthreads();  // the limb sums overwrite the stage area
+  if (slot >= 0) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      uint2 w;
+      w.x = ((uint32_t)acc[t][0] & 0xFFFF) | ((uint32_t)acc[t][1] << 16);
+      w.y = ((uint32_t)acc[t][2] & 0xFFFF) | ((uint32_t)acc[t][3] << 16);
+      *reinterpret_cast<uint2*>(S.u.out.part + slot * 256 + 16 * t + 4 * (tid >> 4)) = w;
     }
   }
+  __syncthreads();
 
-  // 4. rounding + packing into the LDS output image
-  if (row < L) {
-    // b' = (x + h1) >> 3 -> 10 bits, 8 coefficients -> 10 bytes
-    uint64_t lo = 0;
-    uint32_t hi = 0;
+  // 4. rounding + packing into the LDS output image: rows 0..L-1 are b' (or
+  //    keygen's b), row L the inner product v'; 8 coefficients per item
+  constexpr int ROWS = MODE == kSabKeygen ? L : L + 1;
+  uint8_t* outb = S.u.out.outb;
+  #pragma unroll 1
+  for (int x = tid; x < 32 * ROWS; x += NT) {
+    const int row = x >> 5, grp = x & 31;
+    uint32_t acc8[8];
+    const uint16_t* plo = S.u.out.part + row * 256 + 8 * grp;
+    const uint16_t* phi = S.u.out.part + (L + 1 + row) * 256 + 8 * grp;
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      const uint64_t v = ((acc[d] + 4u) >> 3) & 0x3FF;
-      const int bit = 10 * d;
-      if (bit < 64) lo |= v << bit;
-      if (bit + 10 > 64) hi |= (uint32_t)(bit >= 64 ? v << (bit - 64) : v >> (64 - bit));
+    for (int d = 0; d < 8; ++d) acc8[d] = (uint32_t)plo[d] + 256u * (uint32_t)phi[d];
+    if (row < L) {
+      // b' = (x + h1) >> 3 -> 10 bits, 8 coefficients -> 10 bytes
+      uint64_t lo = 0;
+      uint32_t hi = 0;
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const uint64_t v = ((acc8[d] + 4u) >> 3) & 0x3FF;
+        const int bit = 10 * d;
+        if (bit < 64) lo |= v << bit;
+        if (bit + 10 > 64) hi |= (uint32_t)(bit >= 64 ? v << (bit - 64) : v >> (64 - bit));
+      }
+      uint8_t* o = outb + row * 320 + 10 * grp;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
+      o[8] = (uint8_t)hi;
+      o[9] = (uint8_t)(hi >> 8);
+    } else {
+      // c_m = (v' - m 2^9 + h1) >> (10 - eT) -> eT bits; byte grp of m holds the bits of 8 grp .. 8 grp + 7
+      const uint32_t mbyte = a.mbuf[(size_t)inst * 32 + grp];
+      uint64_t packed = 0;
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const uint32_t mb = (mbyte >> d) & 1;
+        const uint64_t c = ((acc8[d] - (mb << 9) + 4u) >> (10 - P::et)) & ((1u << P::et) - 1);
+        packed |= c << (P::et * d);
+      }
+      uint8_t* o = outb + P::vecp + P::et * grp;
+#pragma unroll
+      for (int i = 0; i < P::et; ++i) o[i] = (uint8_t)(packed >> (8 * i));
     }
-    uint8_t* o = S.outb + row * 320 + 10 * (tid & 31);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
-    o[8] = (uint8_t)hi;
-    o[9] = (uint8_t)(hi >> 8);
-  } else if (MODE != kSabKeygen) {
-    // c_m = (v' - m 2^9 + h1) >> (10 - eT) -> eT bits; byte g of m holds the bits of t0..t0+7
-    const uint32_t mbyte = a.mbuf[(size_t)inst * 32 + (tid & 31)];
-    uint64_t packed = 0;
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      const uint32_t mb = (mbyte >> d) & 1;
-      const uint64_t c = ((acc[d] - (mb << 9) + 4u) >> (10 - P::et)) & ((1u << P::et) - 1);
-      packed |= c << (P::et * d);
-    }
-    uint8_t* o = S.outb + P::vecp + P::et * (tid & 31);
-#pragma unroll
-    for (int i = 0; i < P::et; ++i) o[i] = (uint8_t)(packed >> (8 * i));
   }
   __syncthreads();
 
@@ -439,24 +491,25 @@ __global__ void __launch_bounds__(SaberP<L>::threads) sab_poly(SabPolyArgs a) {
   if (MODE == kSabKeygen) {
     uint8_t* pk = a.out + (size_t)inst * P::pk;
     uint8_t* sk = a.sk + (size_t)inst * P::sk;
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(S.outb);
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(outb);
     for (int i = tid; i < P::vecp / 4; i += NT) {
       reinterpret_cast<uint32_t*>(pk)[i] = ob[i];
       reinterpret_cast<uint32_t*>(sk + P::vecq)[i] = ob[i];
     }
     // sk[0:vecq] = POLVECq(s): 32 L groups of 8 coefficients -> 13 bytes; s is
-    // read back from the Toeplitz copy (r[-q] = s[q] at offset 256 - q)
-    if (tid < 32 * L) {
+    // read back from Toeplitz copy 0 (s[q] at byte 256 - q)
+    #pragma unroll 1
+    for (int x = tid; x < 32 * L; x += NT) {
       uint64_t lo = 0, hi = 0;
 #pragma unroll
       for (int d = 0; d < 8; ++d) {
-        const int c = 8 * tid + d;
-        const uint64_t v = S.Ts[(c >> 8) * kToepStride + 256 - (c & 255)] & 0x1FFF;
+        const int c = 8 * x + d;
+        const uint64_t v = (uint64_t)(int64_t)S.V[(c >> 8) * kVPoly + 256 - (c & 255)] & 0x1FFF;
         const int bit = 13 * d;
         if (bit < 64) lo |= v << bit;
         if (bit + 13 > 64) hi |= bit >= 64 ? v << (bit - 64) : v >> (64 - bit);
       }
-      uint8_t* o = sk + 13 * tid;
+      uint8_t* o = sk + 13 * x;
 #pragma unroll
       for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
 #pragma unroll
@@ -464,11 +517,11 @@ __global__ void __launch_bounds__(SaberP<L>::threads) sab_poly(SabPolyArgs a) {
     }
   } else if (MODE == kSabEnc) {
     uint32_t* ct = reinterpret_cast<uint32_t*>(a.out + (size_t)inst * P::ct);
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(S.outb);
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(outb);
     for (int i = tid; i < P::ct / 4; i += NT) ct[i] = ob[i];
   } else {
     const uint32_t* ct = reinterpret_cast<const uint32_t*>(a.ct_in + (size_t)inst * P::ct);
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(S.outb);
+    const uint32_t* ob = reinterpret_cast<const uint32_t*>(outb);
     uint32_t d = 0;
     for (int i = tid; i < P::ct / 4; i += NT) d |= ct[i] ^ ob[i];
     atomicOr(&S.diff, d != 0 ? 1u : 0u);

@@ -66,6 +66,96 @@ __device__ __forceinline__ void toep_dot(uint32_t acc[8], const uint16_t* T, int
   }
 }
 
+// ---------------------------------------------------------------- int8 MFMA form
+//
+// The same negacyclic products on v_mfma_i32_16x16x64_i8.  Per K-block j
+// (one staged polynomial s_j) the product is C += T(s_j) B_j, where T(s) is
+// the 256 x 256 negacyclic Toeplitz matrix, T[t][k] = s[t - k] for t >= k and
+// -s[t - k + 256] otherwise, and B_j holds up to 16 columns of 256 int8 limbs.
+// A 13-bit (or 10-bit) coefficient a splits into signed limbs
+//   hi = (a + 128) >> 8,  lo = a - 256 hi  (lo in [-128, 127], hi in [0, 32])
+// so a = 256 hi + lo exactly and the i32 sums need no bias correction.
+//
+// T(s) is held as the reversed sequence V[e] = T[t][t + e] (e = k - t in
+// [-255, 255]): V[e] = s[-e] for e <= 0 and -s[256 - e] for e > 0.  Four
+// copies, copy c holding V[e] at byte e + 256 + c, make every 16-byte run
+// V[e0 .. e0 + 15] a 4-byte aligned read from copy (-e0) & 3.
+//
+// Tiles: M-tile a covers outputs t = 16a .. 16a + 15, K-chunk b covers
+// k = 64b .. 64b + 63.  The A operand of tile (a, b) depends only on the
+// diagonal delta = 16a - 64b (28 values), so it is read once per diagonal and
+// used for up to four tiles.  Operand lanes: lane (q, m) = lane 16 q + m holds
+// row m (A) or column m (B) at k = 16 q + 0..15 of the chunk; the result tile
+// lane (g, n) holds rows 4 g .. 4 g + 3 of column n.
+
+typedef int v4i8x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kVCopy = 520;                 // bytes per shifted copy
+constexpr int kVPoly = 4 * kVCopy;          // bytes per staged polynomial
+constexpr int kToepDiagonals = 28;          // delta = 16 a - 64 b, a in [0, 16), b in [0, 4)
+
+__host__ __device__ constexpr int toep8_delta(int di) { return 16 * di - 192; }
+// M-tile served by diagonal di at K-chunk b, or -1
+__host__ __device__ constexpr int toep8_tile(int di, int b) {
+  return (toep8_delta(di) + 64 * b >= 0 && toep8_delta(di) + 64 * b < 256) ? (toep8_delta(di) + 64 * b) / 16 : -1;
+}
+
+// Store coefficient q (value v) of the staged polynomial into its four copies.
+__host__ __device__ __forceinline__ void toep8_put(int8_t* V, int q, int v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    V[c * kVCopy + 256 - q + c] = (int8_t)v;
+    if (q) V[c * kVCopy + 512 - q + c] = (int8_t)(-v);
+  }
+}
+
+// Byte offset (within the staged polynomial) of the aligned 16-byte run
+// V[e0 .. e0 + 15].
+__host__ __device__ __forceinline__ int toep8_run(int e0) {
+  const int c = (-e0) & 3;
+  return c * kVCopy + e0 + 256 + c;
+}
+
+// e0 of lane (q, m) for diagonal di
+__host__ __device__ __forceinline__ int toep8_e0(int di, int q, int m) { return 16 * q - m - toep8_delta(di); }
+
+// Signed limbs of a coefficient below 2^13.
+__host__ __device__ __forceinline__ void limbs8(uint32_t a, int8_t* lo, int8_t* hi) {
+  const int h = (int)((a + 128) >> 8);
+  *hi = (int8_t)h;
+  *lo = (int8_t)((int)a - 256 * h);
+}
+
+// B-operand columns: output column n (0 .. L - 1 the matrix rows, L the
+// inner product) has its low limbs in column n and its high limbs in column
+// 8 + n.  Limb arrays are stored per K-block in slot order lo 0..L, hi 0..L;
+// returns the slot of result / operand column nl, or -1 (unused column).
+__host__ __device__ __forceinline__ int toep8_slot(int nl, int L, bool inner) {
+  const int last = inner ? L : L - 1;
+  if (nl <= last) return nl;
+  if (nl >= 8 && nl - 8 <= last) return L + 1 + (nl - 8);
+  return -1;
+}
+
+// acc[a] += T(s) B for one K-block: bop[b] is this lane's B operand of K-chunk
+// b, V the staged polynomial, mfma(a, b, c) the matrix instruction.
+template <class MFMA>
+__device__ __forceinline__ void toep8_block(v4i8x4 acc[16], const int8_t* V, const v4i8x4 bop[4], int lane,
+                                           MFMA&& mfma) {
+  const int q = lane >> 4, m = lane & 15;
+#pragma unroll
+  for (int di = 0; di < kToepDiagonals; ++di) {
+    const uint32_t* r = reinterpret_cast<const uint32_t*>(V + toep8_run(toep8_e0(di, q, m)));
+    const v4i8x4 aop = {(int)r[0], (int)r[1], (int)r[2], (int)r[3]};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      constexpr int kUnused = -1;
+      const int a = toep8_tile(di, b);
+      if (a != kUnused) acc[a] = mfma(aop, bop[b], acc[a]);
+    }
+  }
+}
+
 // 13-bit little-endian group: 8 coefficients from 13 bytes.
 __device__ __forceinline__ void unpack13_group(uint16_t v[8], const uint8_t* b) {
   uint64_t lo = 0;

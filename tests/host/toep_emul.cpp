@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#define __host__
 #define __device__
 #define __forceinline__ inline
 struct uint4 { uint32_t x, y, z, w; };

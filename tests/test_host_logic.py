@@ -3,7 +3,9 @@
 The lane-pair Keccak round template in dpcrypto_amd/csrc/keccak2.hpp is run
 in a two-lane host emulation (partner exchange = swapping the two lanes) and
 compared with the oracle's permutation; the Saber Toeplitz helpers in
-saber_poly.hpp are compared with the oracle's negacyclic product.
+saber_poly.hpp (the dot2 form and the int8 MFMA tile walk, whose matrix
+instructions are replayed as 16 x 16 x 64 products) are compared with the
+oracle's negacyclic product.
 """
 import os
 import shutil
@@ -37,6 +39,12 @@ def test_lane_pair_keccak_matches_oracle(tmp_path, oracle):
 @pytest.mark.skipif(CLANGXX is None, reason="no clang++")
 def test_saber_toeplitz_product_matches_oracle(tmp_path, oracle):
     rc, out = _build_and_run("toep_emul.cpp", tmp_path, oracle, cxx=CLANGXX)
+    assert rc == 0 and out.strip() == "OK", out
+
+
+@pytest.mark.skipif(CLANGXX is None, reason="no clang++")
+def test_saber_mfma_toeplitz_tiling_matches_oracle(tmp_path, oracle):
+    rc, out = _build_and_run("toep8_emul.cpp", tmp_path, oracle, cxx=CLANGXX)
     assert rc == 0 and out.strip() == "OK", out
 
 
