@@ -320,6 +320,7 @@ struct SabPolyLds {
     struct {
       alignas(8) uint32_t stage[SaberP<L>::matrix_bytes / 4];
       alignas(8) uint32_t sstage[SaberP<L>::secret_bytes / 4];
+      alignas(8) uint32_t bstage[SaberP<L>::vecp / 4];  // b bytes of pk (encaps / re-encryption)
     } in;
     struct {
       alignas(16) uint16_t part[2 * (L + 1) * 256];  // limb sums per slot (mod 2^16)
@@ -346,13 +347,36 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
   const int inst = blockIdx.x;
   if (inst >= a.n) return;
 
-  // 1. rebuild the GenMatrix and GenSecret bytes (lane-pair sliced streams) in LDS
-  const uint32_t* ar = reinterpret_cast<const uint32_t*>(a.araw + (size_t)inst * P::matrix_bytes);
-  for (int w = tid; w < P::matrix_bytes / 8; w += NT)
-    reinterpret_cast<uint64_t*>(S.u.in.stage)[w] = k2::sliced_word(ar, w);
-  const uint32_t* sr = reinterpret_cast<const uint32_t*>(a.sraw + (size_t)inst * P::secret_bytes);
-  for (int w = tid; w < P::secret_bytes / 8; w += NT)
-    reinterpret_cast<uint64_t*>(S.u.in.sstage)[w] = k2::sliced_word(sr, w);
+  // 1. rebuild the GenMatrix and GenSecret bytes (lane-pair sliced streams)
+  //    and copy pk's b bytes into LDS; every global load of a thread is issued
+  //    before the first is used (one wave per instance hides little latency)
+  {
+    constexpr int MW = P::matrix_bytes / 8, SW = P::secret_bytes / 8, BW = MODE == kSabKeygen ? 0 : P::vecp / 4;
+    constexpr int MPT = (MW + NT - 1) / NT, SPT = (SW + NT - 1) / NT, BPT = (BW + NT - 1) / NT;
+    const uint2* ar = reinterpret_cast<const uint2*>(a.araw + (size_t)inst * P::matrix_bytes);
+    const uint2* sr = reinterpret_cast<const uint2*>(a.sraw + (size_t)inst * P::secret_bytes);
+    const uint32_t* bp = MODE == kSabKeygen ? nullptr : reinterpret_cast<const uint32_t*>(a.pk + (size_t)inst * a.pk_stride);
+    uint2 mv[MPT], sv[SPT];
+    uint32_t bv[BPT > 0 ? BPT : 1];
+#pragma unroll
+    for (int i = 0; i < MPT; ++i)
+      if (tid + i * NT < MW) mv[i] = ar[tid + i * NT];
+#pragma unroll
+    for (int i = 0; i < SPT; ++i)
+      if (tid + i * NT < SW) sv[i] = sr[tid + i * NT];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i)
+      if (tid + i * NT < BW) bv[i] = bp[tid + i * NT];
+#pragma unroll
+    for (int i = 0; i < MPT; ++i)
+      if (tid + i * NT < MW) reinterpret_cast<uint64_t*>(S.u.in.stage)[tid + i * NT] = k2::reint_full(mv[i].x, mv[i].y);
+#pragma unroll
+    for (int i = 0; i < SPT; ++i)
+      if (tid + i * NT < SW) reinterpret_cast<uint64_t*>(S.u.in.sstage)[tid + i * NT] = k2::reint_full(sv[i].x, sv[i].y);
+#pragma unroll
+    for (int i = 0; i < BPT; ++i)
+      if (tid + i * NT < BW) S.u.in.bstage[tid + i * NT] = bv[i];
+  }
   if (tid == 0) S.diff = 0;
   __syncthreads();
 
@@ -391,14 +415,11 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
     *reinterpret_cast<uint64_t*>(blk + (L + 1 + col) * 256) = hi;
   }
   if (MODE != kSabKeygen) {
-    const uint8_t* pkb = a.pk + (size_t)inst * a.pk_stride;
+    const uint8_t* pkb = reinterpret_cast<const uint8_t*>(S.u.in.bstage);
     #pragma unroll 1
     for (int g = tid; g < 64 * L; g += NT) {
-      uint8_t bytes[5];
-#pragma unroll
-      for (int i = 0; i < 5; ++i) bytes[i] = pkb[5 * g + i];
       uint16_t v[4];
-      unpack10_group(v, bytes);
+      unpack10_group(v, pkb + 5 * g);
       uint32_t lo = 0, hi = 0;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
