@@ -91,31 +91,70 @@ KERNEL_WORK = {
 }
 
 
-# HBM bytes (read, write) per launch at the profiled batch sizes (Saber 4096,
-# Frodo 8192 instances), from two rocprofv3 --pmc passes over
-#   bench.py --schedule 1x1 --calibrate-copy --skip-plain --skip-mixed --skip-serving --cpu-seconds 0 --frodo-steps 2
+# HBM bytes (read, write) per launch at the profiled batch sizes (Saber family
+# 4096, Frodo 8192 instances), from two rocprofv3 --pmc passes over
+#   bench.py --schedule 1x1 --calibrate-copy --skip-mixed --skip-serving --cpu-seconds 0 --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99976, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ("frodo_as_fused", 8192): (129320458, 256245771),
-    ("frodo_dec_mu", 8192): (301635258, 262144),
-    ("frodo_encdec_pre<false>", 8192): (129741165, 128713013),
-    ("frodo_encdec_pre<true>", 8192): (1794857, 128712704),
-    ("frodo_keygen_post", 8192): (129114019, 262144),
-    ("frodo_keygen_pre", 8192): (1015004, 257055461),
-    ("frodo_sa_fused<false>", 8192): (260598655, 128974848),
-    ("frodo_sa_fused<true>", 8192): (389558666, 262144),
-    ("frodo_ss", 8192): (130182606, 196608),
-    ("sab_dec_hash1", 4096): (893845, 18872735),
-    ("sab_dec_inner", 4096): (9887613, 131072),
-    ("sab_enc_hash1", 4096): (4571636, 19043161),
-    ("sab_hash2", 4096): (5159319, 131072),
-    ("sab_keygen_hash1", 4096): (628004, 18992698),
-    ("sab_keygen_hash2", 4096): (4201993, 131072),
-    ("sab_poly<kSabKeygen>", 4096): (19083120, 12976128),
-    ("sab_poly<kSabEnc>", 4096): (23936207, 4456448),
-    ("sab_poly<kSabReenc>", 4096): (29050475, 131072),
+    ('frodo_as_fused', 8192): (129322559, 256245760),
+    ('frodo_as_fused,1344', 8192): (177553276, 353391904),
+    ('frodo_as_fused,640', 8192): (85272040, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129306583, 256247434),
+    ('frodo_dec_mu', 8192): (262124678, 262144),
+    ('frodo_dec_mu,1344', 8192): (385332908, 262144),
+    ('frodo_dec_mu,640', 8192): (223157593, 262144),
+    ('frodo_encdec_pre<false>', 8192): (97555128, 128712784),
+    ('frodo_encdec_pre<false>,1344', 8192): (177928108, 176947200),
+    ('frodo_encdec_pre<false>,640', 8192): (80523712, 84672512),
+    ('frodo_encdec_pre<true>', 8192): (1607151, 128712704),
+    ('frodo_encdec_pre<true>,1344', 8192): (1818906, 176947200),
+    ('frodo_encdec_pre<true>,640', 8192): (1911566, 84672512),
+    ('frodo_keygen_post', 8192): (129115204, 262144),
+    ('frodo_keygen_post,1344', 8192): (177318009, 393216),
+    ('frodo_keygen_post,640', 8192): (79816858, 262144),
+    ('frodo_keygen_pre', 8192): (1021353, 257076081),
+    ('frodo_keygen_pre,1344', 8192): (1017087, 353243496),
+    ('frodo_keygen_pre,640', 8192): (1081175, 168558592),
+    ('frodo_sa_fused<false,aes>', 8192): (259483755, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260583349, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356853499, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (167093477, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388313172, 262144),
+    ('frodo_sa_fused<true>', 8192): (389542840, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (534057315, 266192),
+    ('frodo_sa_fused<true>,640', 8192): (247706075, 262168),
+    ('frodo_ss', 8192): (130060727, 196608),
+    ('frodo_ss,1344', 8192): (178453594, 262144),
+    ('frodo_ss,640', 8192): (81833243, 131072),
+    ('sab_dec_hash1', 4096): (900075, 18874279),
+    ('sab_dec_inner', 4096): (9888560, 131072),
+    ('sab_dec_inner,l2', 4096): (7136378, 131072),
+    ('sab_dec_inner,l4', 4096): (13417824, 131072),
+    ('sab_enc_hash1', 4096): (4580858, 19046557),
+    ('sab_hash2', 4096): (5202977, 131072),
+    ('sab_hash2,l2', 4096): (4004358, 131072),
+    ('sab_hash2,l4', 4096): (7015561, 131072),
+    ('sab_keygen_hash1', 4096): (633207, 18999686),
+    ('sab_keygen_hash1,l2', 4096): (635172, 9830401),
+    ('sab_keygen_hash1,l4', 4096): (634276, 30801938),
+    ('sab_keygen_hash2', 4096): (4205036, 131072),
+    ('sab_pipe_x1', 4096): (5303722, 34420947),
+    ('sab_pipe_x1,l2', 4096): (4031039, 16777216),
+    ('sab_pipe_x1,l4', 4096): (6644002, 58196840),
+    ('sab_pipe_x2', 4096): (5694768, 3538944),
+    ('sab_pipe_x2,l2', 4096): (4266468, 3014656),
+    ('sab_pipe_x2,l4', 4096): (7290982, 3538944),
+    ('sab_poly<kSabEnc>', 4096): (23889973, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13120902, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (36761050, 6029312),
+    ('sab_poly<kSabKeygen>', 4096): (19024540, 12976128),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9559879, 8650752),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30591206, 17301509),
+    ('sab_poly<kSabReenc>', 4096): (29006337, 131373),
+    ('sab_poly<kSabReenc>,l2', 4096): (16532438, 131134),
+    ('sab_poly<kSabReenc>,l4', 4096): (43056831, 131212),
 }
 
 
