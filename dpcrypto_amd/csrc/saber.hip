@@ -219,6 +219,39 @@ __device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restr
   store2_words<4>(a, p, ss + (size_t)inst * kSaberSs);
 }
 
+// The two halves of K for the pipelined chain: H(ct) of the received
+// ciphertext (it does not depend on the re-encryption), kept as this lane's
+// interleaved halves (hc[8 inst + 2 w + p]), then K from it.
+template <int L>
+__device__ __forceinline__ void hct_body(int vb, int n, const uint8_t* __restrict__ ct, uint32_t* __restrict__ hc) {
+  using P = SaberP<L>;
+  SAB_PAIR_SETUP
+  uint32_t a[25];
+  absorb2_global<kRateSha3_256, P::ct / 8>(a, p, ct + (size_t)inst * P::ct, kDsSha3);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) hc[(size_t)inst * 8 + 2 * i + p] = a[i];
+}
+
+template <int L>
+__device__ __forceinline__ void k_body(int vb, int n, const uint32_t* __restrict__ hc,
+                                       const uint8_t* __restrict__ krbuf, const uint8_t* __restrict__ sk,
+                                       const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+  using P = SaberP<L>;
+  SAB_PAIR_SETUP
+  uint32_t a[25], m[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) m[4 + i] = hc[(size_t)inst * 8 + 2 * i + p];
+  const uint64_t mask = 0 - (uint64_t)(fail[inst] != 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint64_t k = ld64(krbuf + (size_t)inst * 64 + 8 * i);
+    const uint64_t z = ld64(sk + (size_t)inst * P::sk + P::sk - 32 + 8 * i);
+    m[i] = k2::deint((k & ~mask) | (z & mask), p);
+  }
+  absorb2_halves<kRateSha3_256, 8>(a, p, m, kDsSha3);
+  store2_words<4>(a, p, ss + (size_t)inst * kSaberSs);
+}
+
 // ----------------------------------------------------------------- hash kernels
 // hash1: blockDim = 128, wave 0 runs role 0 and wave 1 role 1 for the same 32
 // instances (role_sel >= 0: that role only, 64 threads).  hash2: 64 threads
@@ -264,7 +297,9 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 //   x1  blocks [0, nb): encaps' hash1 (both roles); blocks [nb, 2 nb): wave 0
 //       keygen's H(pk) into sk, wave 1 decaps' GenMatrix from the seedA in sk
 //       (written by keygen's hash1, two launches earlier);
-//   x2  wave 0: decaps' message chain (G, GenSecret); wave 1: encaps' K.
+//   x2  wave 0: decaps' message chain (G, GenSecret); wave 1: encaps' K;
+//       wave 2: decaps' H(ct), leaving one permutation (sab_k) after the
+//       re-encryption.
 // Every op still computes all of its own hashes; only their order changes.
 template <int L>
 __global__ void __launch_bounds__(128) sab_pipe_x1(int n, int nb, const uint8_t* __restrict__ pk,
@@ -279,12 +314,23 @@ __global__ void __launch_bounds__(128) sab_pipe_x1(int n, int nb, const uint8_t*
 }
 
 template <int L>
-__global__ void __launch_bounds__(128) sab_pipe_x2(int n, const uint8_t* __restrict__ sk,
+__global__ void __launch_bounds__(192) sab_pipe_x2(int n, const uint8_t* __restrict__ sk,
                                                    const uint8_t* __restrict__ mbuf_d, uint8_t* __restrict__ sraw_d,
                                                    uint8_t* __restrict__ krbuf_d, const uint8_t* __restrict__ ct,
-                                                   const uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ ss) {
-  if ((threadIdx.x >> 6) == 0) dec_hash1_body<L>(blockIdx.x, 1, n, sk, mbuf_d, nullptr, sraw_d, krbuf_d);
-  else hash2_body<L>(blockIdx.x, n, ct, krbuf_e, nullptr, nullptr, ss);
+                                                   const uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ ss,
+                                                   uint32_t* __restrict__ hc_d) {
+  const int wave = threadIdx.x >> 6;
+  if (wave == 0) dec_hash1_body<L>(blockIdx.x, 1, n, sk, mbuf_d, nullptr, sraw_d, krbuf_d);
+  else if (wave == 1) hash2_body<L>(blockIdx.x, n, ct, krbuf_e, nullptr, nullptr, ss);
+  else hct_body<L>(blockIdx.x, n, ct, hc_d);
+}
+
+// decaps' K from the stored H(ct): K = SHA3-256((fail ? z : kr[0:32]) || H(ct))
+template <int L>
+__global__ void __launch_bounds__(64) sab_k(int n, const uint32_t* __restrict__ hc, const uint8_t* __restrict__ krbuf,
+                                            const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
+                                            uint8_t* __restrict__ ss) {
+  k_body<L>(blockIdx.x, n, hc, krbuf, sk, fail, ss);
 }
 
 // ----------------------------------------------------------------- products
@@ -708,17 +754,17 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
 
 // Pipelined keygen -> encaps -> decaps of the same n instances on one stream
 // (see sab_pipe_x1 / sab_pipe_x2).  Per instance the serial Keccak path drops
-// from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 23 + 10 + 10 permutations
-// (keygen's H(pk), encaps' K and decaps' GenMatrix run beside other stages),
-// and ten launches become eight.  scratch: saber_pipe_scratch_per_inst(l)
+// from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 23 + 10 + 1 permutations
+// (keygen's H(pk), encaps' K, decaps' GenMatrix and decaps' H(ct) run beside
+// other stages), and ten launches become eight.  scratch: saber_pipe_scratch_per_inst(l)
 // bytes per instance: two GenMatrix, two GenSecret, two m and two kr buffers
 // (encaps' and decaps'; keygen's streams reuse decaps' buffers, which the
-// stream order makes safe) and the mismatch flags, plus 256-byte alignment of
-// nine arrays per launch.
+// stream order makes safe), decaps' H(ct) and the mismatch flags, plus
+// 256-byte alignment of ten arrays per launch.
 template <int L>
 static size_t pipe_scratch() {
   using P = SaberP<L>;
-  return 2 * ((size_t)P::matrix_bytes + P::secret_bytes + 32 + 64) + 4;
+  return 2 * ((size_t)P::matrix_bytes + P::secret_bytes + 32 + 64) + 32 + 4;
 }
 
 size_t saber_pipe_scratch_per_inst(int l) {
@@ -743,9 +789,10 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint8_t* mbuf_d = take(32);
   uint8_t* krbuf_e = take(64);
   uint8_t* krbuf_d = take(64);
+  uint32_t* hc_d = reinterpret_cast<uint32_t*>(take(32));
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   const int nb = blocks_for(n, 32);
-  const dim3 g(nb), b64(64), b128(128), bp(P::threads);
+  const dim3 g(nb), b64(64), b128(128), b192(192), bp(P::threads);
   DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash1"), sab_keygen_hash1<L>, g, b128, st, -1, n, kc, pk, sk, araw_d, sraw_d);
   SabPolyArgs ak{n, araw_d, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
@@ -754,10 +801,11 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
   DPKEM_LAUNCH(SAB_NAME("sab_dec_inner"), sab_dec_inner<L>, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf_d);
-  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, g, b128, st, n, sk, mbuf_d, sraw_d, krbuf_d, ct, krbuf_e, ss);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, g, b192, st, n, sk, mbuf_d, sraw_d, krbuf_d, ct, krbuf_e, ss,
+               hc_d);
   SabPolyArgs ad{n, araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, fail};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabReenc>"), (sab_poly<kSabReenc, L>), dim3(n), bp, st, ad);
-  DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, g, b64, st, n, ct, krbuf_d, sk, fail, ss2);
+  DPKEM_LAUNCH(SAB_NAME("sab_k"), sab_k<L>, g, b64, st, n, hc_d, krbuf_d, sk, fail, ss2);
 }
 
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
