@@ -294,23 +294,61 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 
 // Pipelined keygen -> encaps -> decaps chain (saber_pipe_launch).  Stages of
 // different ops that read disjoint inputs share a launch:
-//   x1  blocks [0, nb): encaps' hash1 (both roles); blocks [nb, 2 nb): wave 0
-//       keygen's H(pk) into sk, wave 1 decaps' GenMatrix from the seedA in sk
-//       (written by keygen's hash1, two launches earlier);
+//   x0  four waves per 32 instances: wave 0 computes keygen's seedA and
+//       stores it into pk and sk; after one workgroup barrier (which makes
+//       those bytes visible to the workgroup) wave 0 runs keygen's
+//       GenMatrix, wave 1 keygen's GenSecret, wave 2 encaps' GenMatrix from
+//       the seedA in pk and wave 3 decaps' GenMatrix from the seedA in sk;
+//   x1  wave 0: encaps' message chain (m, H(pk), G, GenSecret); wave 1:
+//       keygen's H(pk) into sk;
 //   x2  wave 0: decaps' message chain (G, GenSecret); wave 1: encaps' K;
 //       wave 2: decaps' H(ct), leaving one permutation (sab_k) after the
 //       re-encryption.
 // Every op still computes all of its own hashes; only their order changes.
 template <int L>
-__global__ void __launch_bounds__(128) sab_pipe_x1(int n, int nb, const uint8_t* __restrict__ pk,
-                                                   const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ sk,
-                                                   uint8_t* __restrict__ araw_e, uint8_t* __restrict__ sraw_e,
-                                                   uint8_t* __restrict__ mbuf_e, uint8_t* __restrict__ krbuf_e,
+__global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restrict__ kc, uint8_t* __restrict__ pk,
+                                                   uint8_t* __restrict__ sk, uint8_t* __restrict__ araw_k,
+                                                   uint8_t* __restrict__ sraw_k, uint8_t* __restrict__ araw_e,
                                                    uint8_t* __restrict__ araw_d) {
-  const int wave = threadIdx.x >> 6, b = blockIdx.x;
-  if (b < nb) enc_hash1_body<L>(b, wave, n, pk, ecoins, araw_e, sraw_e, mbuf_e, krbuf_e);
-  else if (wave == 0) keygen_hash2_body<L>(b - nb, n, pk, sk);
-  else dec_hash1_body<L>(b - nb, 0, n, sk, nullptr, araw_d, nullptr, nullptr);
+  using P = SaberP<L>;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, vb = blockIdx.x;
+  const uint32_t p = lane & 1;
+  const int inst = vb * 32 + (lane >> 1);
+  const bool live = inst < n;  // every block holds at least one live instance
+  uint32_t a[25], h[4];
+  if (wave == 0 && live) {
+    const uint8_t* c = kc + (size_t)inst * kSaberKeygenCoins;
+    uint64_t m[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m[i] = ld64(c + 8 * i);
+    absorb2_regs<kRateShake128, 4>(a, p, m, kDsShake);  // seedA = SHAKE128(seedA_raw, 32)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[i] = a[i];
+    store2_words<4>(h, p, pk + (size_t)inst * P::pk + P::vecp);
+    store2_words<4>(h, p, sk + (size_t)inst * P::sk + P::vecq + P::vecp);
+  }
+  __syncthreads();
+  if (!live) return;
+  if (wave == 0) {
+    absorb2_halves<kRateShake128, 4>(a, p, h, kDsShake);  // keygen's GenMatrix = SHAKE128(seedA)
+    squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
+        a, p, reinterpret_cast<uint32_t*>(araw_k + (size_t)inst * P::matrix_bytes));
+  } else if (wave == 1) {
+    keygen_hash1_body<L>(vb, 1, n, kc, pk, sk, nullptr, sraw_k);
+  } else if (wave == 2) {
+    enc_hash1_body<L>(vb, 0, n, pk, nullptr, araw_e, nullptr, nullptr, nullptr);
+  } else {
+    dec_hash1_body<L>(vb, 0, n, sk, nullptr, araw_d, nullptr, nullptr);
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(128) sab_pipe_x1(int n, const uint8_t* __restrict__ pk,
+                                                   const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ sk,
+                                                   uint8_t* __restrict__ sraw_e, uint8_t* __restrict__ mbuf_e,
+                                                   uint8_t* __restrict__ krbuf_e) {
+  if ((threadIdx.x >> 6) == 0) enc_hash1_body<L>(blockIdx.x, 1, n, pk, ecoins, nullptr, sraw_e, mbuf_e, krbuf_e);
+  else keygen_hash2_body<L>(blockIdx.x, n, pk, sk);
 }
 
 template <int L>
@@ -753,18 +791,19 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
 }
 
 // Pipelined keygen -> encaps -> decaps of the same n instances on one stream
-// (see sab_pipe_x1 / sab_pipe_x2).  Per instance the serial Keccak path drops
-// from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 23 + 10 + 1 permutations
-// (keygen's H(pk), encaps' K, decaps' GenMatrix and decaps' H(ct) run beside
-// other stages), and ten launches become eight.  scratch: saber_pipe_scratch_per_inst(l)
-// bytes per instance: two GenMatrix, two GenSecret, two m and two kr buffers
-// (encaps' and decaps'; keygen's streams reuse decaps' buffers, which the
-// stream order makes safe), decaps' H(ct) and the mismatch flags, plus
-// 256-byte alignment of ten arrays per launch.
+// (see sab_pipe_x0 .. sab_pipe_x2).  Per instance the serial Keccak path drops
+// from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 15 + 10 + 1 permutations (all
+// three GenMatrix streams start from the seedA bytes of pk / sk as soon as
+// keygen has written them; keygen's H(pk), encaps' K and decaps' H(ct) run
+// beside other stages), and ten launches become eight.  scratch:
+// saber_pipe_scratch_per_inst(l) bytes per instance: three GenMatrix streams,
+// two GenSecret, two m and two kr buffers (keygen's GenSecret reuses decaps'
+// buffer, which the stream order makes safe), decaps' H(ct) and the mismatch
+// flags, plus 256-byte alignment of eleven arrays per launch.
 template <int L>
 static size_t pipe_scratch() {
   using P = SaberP<L>;
-  return 2 * ((size_t)P::matrix_bytes + P::secret_bytes + 32 + 64) + 32 + 4;
+  return 3 * (size_t)P::matrix_bytes + 2 * ((size_t)P::secret_bytes + 32 + 64) + 32 + 4;
 }
 
 size_t saber_pipe_scratch_per_inst(int l) {
@@ -781,7 +820,8 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
     off += ((size_t)n * per + 255) & ~(size_t)255;
     return q;
   };
-  uint8_t* araw_d = take(P::matrix_bytes);  // keygen's GenMatrix, then decaps'
+  uint8_t* araw_k = take(P::matrix_bytes);
+  uint8_t* araw_d = take(P::matrix_bytes);
   uint8_t* sraw_d = take(P::secret_bytes);  // keygen's GenSecret, then decaps'
   uint8_t* araw_e = take(P::matrix_bytes);
   uint8_t* sraw_e = take(P::secret_bytes);
@@ -793,11 +833,11 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   const int nb = blocks_for(n, 32);
   const dim3 g(nb), b64(64), b128(128), b192(192), bp(P::threads);
-  DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash1"), sab_keygen_hash1<L>, g, b128, st, -1, n, kc, pk, sk, araw_d, sraw_d);
-  SabPolyArgs ak{n, araw_d, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x0"), sab_pipe_x0<L>, g, dim3(256), st, n, kc, pk, sk, araw_k, sraw_d, araw_e,
+               araw_d);
+  SabPolyArgs ak{n, araw_k, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
-  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, dim3(2 * nb), b128, st, n, nb, pk, ec, sk, araw_e, sraw_e,
-               mbuf_e, krbuf_e, araw_d);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, b128, st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e);
   SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
   DPKEM_LAUNCH(SAB_NAME("sab_dec_inner"), sab_dec_inner<L>, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf_d);
