@@ -161,10 +161,12 @@ using k2::absorb2_halves;
 using k2::absorb2_regs;
 using k2::store2_words;
 
+// The chain bodies take the 32-instance group vb explicitly, so that the
+// pipelined chain kernels can run several of them in one launch.
 #define FR_PAIR_SETUP                                   \
   const int lane = threadIdx.x & 63;                    \
   const uint32_t p = lane & 1;                          \
-  const int inst = blockIdx.x * 32 + (lane >> 1);       \
+  const int inst = vb * 32 + (lane >> 1);               \
   if (inst >= n) return;
 
 // Two samples from the 32-bit half of a squeezed word: int8 pair (low 16
@@ -212,6 +214,7 @@ template <class P>
 __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                        uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
+  const int vb = blockIdx.x;
   FR_PAIR_SETUP
   constexpr int RW = P::chain_rate, LS = P::ls;
   const uint8_t* c = coins + (size_t)inst * P::kcoins;
@@ -247,11 +250,16 @@ __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __r
 
 // pkh = SHAKE(pk, sec); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
 template <class P>
-__global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
+__device__ __forceinline__ void keygen_post_body(int vb, int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
   FR_PAIR_SETUP
   uint32_t a[25];
   absorb2_global<P::chain_rate, P::pk / 8>(a, p, pk + (size_t)inst * P::pk, kDsShake);
   store2_words<P::ls>(a, p, sk + (size_t)inst * P::sk + P::sk_pkh);
+}
+
+template <class P>
+__global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
+  keygen_post_body<P>(blockIdx.x, n, pk, sk);
 }
 
 // ----------------------------------------------------------------- enc / dec chains
@@ -262,11 +270,11 @@ __global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __
 // G2 = SHAKE(pkh || mu, 2 sec) = seedSE || k; S' || E' || E'' from
 // SHAKE(0x96 || seedSE, 32 n + 128).  k goes to kbuf[0:sec].
 template <class P, bool DEC>
-__global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __restrict__ keys, size_t key_stride,
-                                                       const uint8_t* __restrict__ mu, size_t mu_stride,
-                                                       uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
-                                                       uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
-                                                       const uint8_t* __restrict__ pkh) {
+__device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __restrict__ keys, size_t key_stride,
+                                                const uint8_t* __restrict__ mu, size_t mu_stride,
+                                                uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
+                                                uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
+                                                const uint8_t* __restrict__ pkh) {
   FR_PAIR_SETUP
   constexpr int RW = P::chain_rate, LS = P::ls;
   const uint8_t* key = keys + (size_t)inst * key_stride;
@@ -300,39 +308,77 @@ __global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __r
   });
 }
 
-// ss = SHAKE(ct || kbar, sec); for decaps kbar = fail ? s : k'.
+template <class P, bool DEC>
+__global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __restrict__ keys, size_t key_stride,
+                                                       const uint8_t* __restrict__ mu, size_t mu_stride,
+                                                       uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
+                                                       uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
+                                                       const uint8_t* __restrict__ pkh) {
+  encdec_pre_body<P, DEC>(blockIdx.x, n, keys, key_stride, mu, mu_stride, kbuf, s8, e8, epp8, pkh);
+}
+
+// ss = SHAKE(ct || kbar, sec); for decaps kbar = fail ? s : k'.  The full
+// blocks hold only ct words (ss_absorb); the last block holds the remaining
+// ct words, kbar and the padding (ss_finish).  The pipelined chain absorbs
+// decaps' ct while the re-encryption runs and keeps the state per lane in
+// `state` (25 words per lane, word-major: state[i * 2n + 2 inst + p]).
+template <class P>
+struct FrodoSsShape {
+  static constexpr int RW = P::chain_rate, LS = P::ls;
+  static constexpr int CTW = P::ct / 8;
+  static constexpr int NB = CTW / RW;        // full blocks of ct words
+  static constexpr int REM = CTW - NB * RW;  // ct words in the last block
+  static_assert(REM + LS <= RW - 1, "kbar and the padding share the final block");
+};
+
+template <class P>
+__device__ __forceinline__ void ss_absorb(uint32_t a[25], uint32_t p, const uint8_t* __restrict__ c) {
+  using Q = FrodoSsShape<P>;
+  k2::zero2(a);
+  for (int b = 0; b < Q::NB; ++b) {
+#pragma unroll
+    for (int i = 0; i < Q::RW; ++i) a[i] ^= k2::deint(ld64(c + 8 * (b * Q::RW + i)), p);
+    k2::keccakf2(a, p);
+  }
+}
+
+template <class P>
+__device__ __forceinline__ void ss_finish(uint32_t a[25], uint32_t p, int inst, const uint8_t* __restrict__ c,
+                                          const uint8_t* __restrict__ kbuf, const uint8_t* __restrict__ sk,
+                                          size_t sk_stride, const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+  using Q = FrodoSsShape<P>;
+#pragma unroll
+  for (int i = 0; i < Q::REM; ++i) a[i] ^= k2::deint(ld64(c + 8 * (Q::NB * Q::RW + i)), p);
+  uint64_t mask = 0;
+  if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
+#pragma unroll
+  for (int i = 0; i < Q::LS; ++i) {
+    const uint64_t k = ld64(kbuf + (size_t)inst * 64 + 8 * i);
+    const uint64_t s = sk ? ld64(sk + (size_t)inst * sk_stride + 8 * i) : 0;
+    a[Q::REM + i] ^= k2::deint((k & ~mask) | (s & mask), p);
+  }
+  a[Q::REM + Q::LS] ^= k2::half_const(kDsShake, p);
+  a[Q::RW - 1] ^= k2::half_const(kPadEnd, p);
+  k2::keccakf2(a, p);
+  store2_words<Q::LS>(a, p, ss + (size_t)inst * P::sec);
+}
+
+template <class P>
+__device__ __forceinline__ void ss_body(int vb, int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ kbuf,
+                                        const uint8_t* __restrict__ sk, size_t sk_stride,
+                                        const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+  FR_PAIR_SETUP
+  const uint8_t* c = ct + (size_t)inst * P::ct;
+  uint32_t a[25];
+  ss_absorb<P>(a, p, c);
+  ss_finish<P>(a, p, inst, c, kbuf, sk, sk_stride, fail, ss);
+}
+
 template <class P>
 __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ kbuf,
                                                const uint8_t* __restrict__ sk, size_t sk_stride,
                                                const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
-  FR_PAIR_SETUP
-  const uint8_t* c = ct + (size_t)inst * P::ct;
-  constexpr int RW = P::chain_rate, LS = P::ls;
-  constexpr int CTW = P::ct / 8;
-  constexpr int NB = CTW / RW;                       // full blocks of ct words
-  constexpr int REM = CTW - NB * RW;                 // ct words in the last block
-  static_assert(REM + LS <= RW - 1, "kbar and the padding share the final block");
-  uint32_t a[25];
-  k2::zero2(a);
-  for (int b = 0; b < NB; ++b) {
-#pragma unroll
-    for (int i = 0; i < RW; ++i) a[i] ^= k2::deint(ld64(c + 8 * (b * RW + i)), p);
-    k2::keccakf2(a, p);
-  }
-#pragma unroll
-  for (int i = 0; i < REM; ++i) a[i] ^= k2::deint(ld64(c + 8 * (NB * RW + i)), p);
-  uint64_t mask = 0;
-  if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
-#pragma unroll
-  for (int i = 0; i < LS; ++i) {
-    const uint64_t k = ld64(kbuf + (size_t)inst * 64 + 8 * i);
-    const uint64_t s = sk ? ld64(sk + (size_t)inst * sk_stride + 8 * i) : 0;
-    a[REM + i] ^= k2::deint((k & ~mask) | (s & mask), p);
-  }
-  a[REM + LS] ^= k2::half_const(kDsShake, p);
-  a[RW - 1] ^= k2::half_const(kPadEnd, p);
-  k2::keccakf2(a, p);
-  store2_words<LS>(a, p, ss + (size_t)inst * P::sec);
+  ss_body<P>(blockIdx.x, n, ct, kbuf, sk, sk_stride, fail, ss);
 }
 
 // decaps: W = C - B' S, mu' = Decode(W) (B bits per entry, rounded).  256
@@ -964,6 +1010,59 @@ __global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint
   }
 }
 
+// ----------------------------------------------------------------- pipelined keygen -> encaps -> decaps
+//
+// For keygen -> encaps -> decaps of the same instances (a plan chain), the
+// serial chains of different ops that read disjoint inputs share launches:
+//   x1   wave 0: keygen's pkh into sk; wave 1: encaps' pkh, G2 and noise
+//        (both only need pk, complete after the keygen product);
+//   x2   wave 0: encaps' ss; wave 1: decaps' G2 and noise (after dec_mu);
+//        wave 2: decaps' SHAKE over the full ct blocks, kept in `state`;
+//   ssd  decaps' last ss block (ct tail, kbar chosen by the FO compare).
+// Every op still computes all of its own hashes; only their order changes.
+template <class P>
+__global__ void __launch_bounds__(128) frodo_pipe_x1(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                     const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ kbuf_e,
+                                                     uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
+                                                     uint32_t* __restrict__ epp8) {
+  if ((threadIdx.x >> 6) == 0) keygen_post_body<P>(blockIdx.x, n, pk, sk);
+  else encdec_pre_body<P, false>(blockIdx.x, n, pk, (size_t)P::pk, ecoins, (size_t)P::ecoins, kbuf_e, s8, e8, epp8,
+                                 nullptr);
+}
+
+template <class P>
+__global__ void __launch_bounds__(192) frodo_pipe_x2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
+                                                     const uint8_t* __restrict__ kbuf_e, uint8_t* __restrict__ ss_e,
+                                                     const uint8_t* __restrict__ mubuf, uint8_t* __restrict__ kbuf_d,
+                                                     uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
+                                                     uint32_t* __restrict__ epp8, uint32_t* __restrict__ state) {
+  const int wave = threadIdx.x >> 6, vb = blockIdx.x;
+  if (wave == 0) {
+    ss_body<P>(vb, n, ct, kbuf_e, nullptr, 0, nullptr, ss_e);
+  } else if (wave == 1) {
+    encdec_pre_body<P, true>(vb, n, sk, (size_t)P::sk, mubuf, 32, kbuf_d, s8, e8, epp8, nullptr);
+  } else {
+    FR_PAIR_SETUP
+    uint32_t a[25];
+    ss_absorb<P>(a, p, ct + (size_t)inst * P::ct);
+#pragma unroll
+    for (int i = 0; i < 25; ++i) state[(size_t)i * 2 * n + 2 * inst + p] = a[i];
+  }
+}
+
+template <class P>
+__global__ void __launch_bounds__(64) frodo_pipe_ssd(int n, const uint8_t* __restrict__ ct,
+                                                     const uint32_t* __restrict__ state, const uint8_t* __restrict__ kbuf_d,
+                                                     const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
+                                                     uint8_t* __restrict__ ss) {
+  const int vb = blockIdx.x;
+  FR_PAIR_SETUP
+  uint32_t a[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) a[i] = state[(size_t)i * 2 * n + 2 * inst + p];
+  ss_finish<P>(a, p, inst, ct + (size_t)inst * P::ct, kbuf_d, sk, (size_t)P::sk, fail, ss);
+}
+
 // ----------------------------------------------------------------- host launchers
 
 static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
@@ -1036,6 +1135,58 @@ void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
   else dec_t<F976, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
 }
 
+
+// Pipelined keygen -> encaps -> decaps of the same n instances on one stream
+// (frodo_pipe_x1 / x2 / ssd).  Per instance the serial SHAKE path loses
+// keygen's pkh (115 permutations for FrodoKEM-976), encaps' ss (116) and all
+// but the last block of decaps' ss; ten launches become eight.  scratch:
+// frodo_pipe_scratch_per_inst(set) bytes per instance.
+size_t frodo_pipe_scratch_per_inst(int set) { return 16 * (size_t)set + 3 * 64 + 32 + 4 + 200; }
+
+template <class P, bool AES>
+static void pipe_t(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk, uint8_t* ct,
+                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+  size_t off = 0;
+  auto take = [&](size_t per) {
+    uint8_t* q = scratch + off;
+    off += ((size_t)n * per + 255) & ~(size_t)255;
+    return q;
+  };
+  uint32_t* s8 = reinterpret_cast<uint32_t*>(take(P::mat));  // keygen's, then encaps', then decaps'
+  uint32_t* e8 = reinterpret_cast<uint32_t*>(take(P::mat));
+  uint32_t* epp8 = reinterpret_cast<uint32_t*>(take(64));
+  uint8_t* kbuf_e = take(64);
+  uint8_t* kbuf_d = take(64);
+  uint8_t* mubuf = take(32);
+  uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
+  uint32_t* state = reinterpret_cast<uint32_t*>(take(200));
+  const dim3 g(blocks_for(n, 32));
+  const int8_t* e8b = reinterpret_cast<const int8_t*>(e8);
+  const int8_t* eppb = reinterpret_cast<const int8_t*>(epp8);
+  DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, g, dim3(64), st, n, kc, pk, sk, s8, e8);
+  DPKEM_LAUNCH(AES ? "frodo_as_fused<aes>" : FR_NAME("frodo_as_fused"), (frodo_as_fused<P, AES>), dim3(n),
+               dim3(P::threads), st, n, pk, sk, s8, e8b);
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_x1"), frodo_pipe_x1<P>, g, dim3(128), st, n, pk, sk, ec, kbuf_e, s8, e8, epp8);
+  DPKEM_LAUNCH(AES ? "frodo_sa_fused<false,aes>" : FR_NAME("frodo_sa_fused<false>"), (frodo_sa_fused<P, false, AES>),
+               dim3(n), dim3(P::threads), st, n, pk, (size_t)P::pk, s8, e8b, eppb, ec, (size_t)P::ecoins, ct, nullptr,
+               nullptr);
+  DPKEM_LAUNCH(FR_NAME("frodo_dec_mu"), frodo_dec_mu<P>, dim3(n), dim3(256), st, n, ct, sk, (size_t)P::sk, mubuf);
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_x2"), frodo_pipe_x2<P>, g, dim3(192), st, n, ct, sk, kbuf_e, ss, mubuf, kbuf_d, s8,
+               e8, epp8, state);
+  DPKEM_LAUNCH(AES ? "frodo_sa_fused<true,aes>" : FR_NAME("frodo_sa_fused<true>"), (frodo_sa_fused<P, true, AES>),
+               dim3(n), dim3(P::threads), st, n, sk + P::sec, (size_t)P::sk, s8, e8b, eppb, mubuf, (size_t)32, nullptr,
+               ct, fail);
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, g, dim3(64), st, n, ct, state, kbuf_d, sk, fail, ss2);
+}
+
+void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes) {
+  if (n <= 0) return;
+  if (set == 640) pipe_t<FrodoP<640>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+  else if (set == 1344) pipe_t<FrodoP<1344>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+  else if (aes) pipe_t<F976, true>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+  else pipe_t<F976, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+}
 
 // Encaps / decaps where instances share public keys in groups (see
 // frodo_sa_grouped).  pk_stride / sk_stride = 0: one key for all n.

@@ -41,6 +41,11 @@ void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* p
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh = nullptr,
                       bool aes = false, int set = 976);
+// Pipelined keygen -> encaps -> decaps of the same instances on one stream
+// (frodo.hip); set / aes as above.
+size_t frodo_pipe_scratch_per_inst(int set);
+void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes);
 // Groups of up to kFrodoGroup instances sharing one public key expand A once;
 // groups[b] = (first, count) or nullptr for consecutive groups of
 // kFrodoGroup; key strides may be 0 (one key for all n).

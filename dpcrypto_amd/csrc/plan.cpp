@@ -87,17 +87,26 @@ size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
 int saber_rank_of(int scheme) {
   return scheme == DPKEM_SABER ? 3 : scheme == DPKEM_LIGHTSABER ? 2 : scheme == DPKEM_FIRESABER ? 4 : 0;
 }
+// FrodoKEM parameter set (n) of a Frodo scheme, 0 otherwise
+int frodo_set_of(int scheme) {
+  return scheme == DPKEM_FRODO640 ? 640
+         : scheme == DPKEM_FRODO1344 ? 1344
+         : (scheme == DPKEM_FRODO976 || scheme == DPKEM_FRODO976_AES) ? 976
+                                                                       : 0;
+}
 
-// Saber-family keygen t0 -> enc t1 -> dec t2 over the same arrays (t1
+// Keygen t0 -> enc t1 -> dec t2 of one scheme over the same arrays (t1
 // consumes t0's pk, t2 consumes t1's ct and t0's sk): each chunk runs the
 // three ops as one launch sequence on one stream, either pipelined (default:
-// stages of different ops that read disjoint inputs share launches) or, with
-// DPKEM_PLAN_SHARE_MATRIX and Saber l = 3, expanding the public matrix once.
-// t1's and t2's sub-tasks are folded into t0's, so their dependants wait for
-// whole chains.  Returns the chain kind.
-int find_saber_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
+// serial stages of different ops that read disjoint inputs share launches)
+// or, with DPKEM_PLAN_SHARE_MATRIX and Saber l = 3, expanding the public
+// matrix once.  t1's and t2's sub-tasks are folded into t0's, so their
+// dependants wait for whole chains.  Returns the chain kind.
+int find_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
   const Task& K = p->tasks[t0];
-  if (p->form != DPKEM_FORM_BATCHED || !saber_rank_of(K.scheme) || K.op != kKeygen || K.after != -1) return kNoChain;
+  if (p->form != DPKEM_FORM_BATCHED || (!saber_rank_of(K.scheme) && !frodo_set_of(K.scheme)) || K.op != kKeygen ||
+      K.after != -1)
+    return kNoChain;
   const int kind = (p->flags & DPKEM_PLAN_SHARE_MATRIX) && K.scheme == DPKEM_SABER ? kShareChain
                    : (p->flags & DPKEM_PLAN_NO_PIPELINE)                           ? kNoChain
                                                                                    : kPipeChain;
@@ -134,7 +143,7 @@ void schedule(dpkem_plan* p) {
     }
     const Task& T = p->tasks[t];
     int t1 = -1, t2 = -1;
-    if (const int kind = find_saber_chain(p, (int)t, &t1, &t2)) {
+    if (const int kind = find_chain(p, (int)t, &t1, &t2)) {
       absorbed[t1] = absorbed[t2] = (int)t;
       chain_of[t] = chains++;
       const size_t k = std::max<size_t>(1, std::min<size_t>((size_t)p->chunks, T.n));
@@ -149,8 +158,9 @@ void schedule(dpkem_plan* p) {
         s.count = std::min(per, T.n - b);
         s.stream = (int)((c + (size_t)chain_of[t]) % (size_t)p->nstreams);
         s.scratch_off = off;
-        const size_t per_inst =
-            kind == kShareChain ? kSaberChainScratchPerInst : saber_pipe_scratch_per_inst(saber_rank_of(T.scheme));
+        const size_t per_inst = kind == kShareChain          ? kSaberChainScratchPerInst
+                                : saber_rank_of(T.scheme) ? saber_pipe_scratch_per_inst(saber_rank_of(T.scheme))
+                                                          : frodo_pipe_scratch_per_inst(frodo_set_of(T.scheme));
         off += round_up(s.count * per_inst + 16 * 256);
         task_subs[t].push_back((int)p->subs.size());
         p->subs.push_back(s);
@@ -291,9 +301,12 @@ int dpkem_plan_build(dpkem_plan* p) {
       uint8_t *ct = E.o0 + b * eo[0], *ss = E.o1 + b * eo[1], *ss2 = Dd.o0 + b * eo[1];
       if (s.chain_kind == kShareChain)
         saber_chain_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off);
-      else
+      else if (saber_rank_of(T.scheme))
         saber_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
                           saber_rank_of(T.scheme));
+      else
+        frodo_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
+                          frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES);
       ok = ok && hipGetLastError() == hipSuccess;
       ok = ok && hipEventRecord(s.done, st) == hipSuccess;
       continue;

@@ -149,12 +149,14 @@ int dpkem_plan_add(dpkem_plan *p, int scheme, int op, size_t n, uint8_t *o0, uin
  * it applies only when one caller runs all three ops on the same instances
  * (self-tests, known-answer runs), never across parties. */
 #define DPKEM_PLAN_SHARE_MATRIX 1u
-/* By default a Saber-family keygen -> enc -> dec chain over the same arrays
- * runs pipelined: per chunk one launch sequence in which stages of different
- * ops that read disjoint inputs share launches (keygen's H(pk), encaps' K and
- * decaps' GenMatrix beside other stages).  Every op still computes all of its
- * own hashes and results are identical.  DPKEM_PLAN_NO_PIPELINE runs such
- * chains as three dependent tasks instead. */
+/* By default a keygen -> enc -> dec chain of one scheme over the same arrays
+ * runs pipelined: per chunk one launch sequence in which serial stages of
+ * different ops that read disjoint inputs share launches (Saber family: the
+ * three GenMatrix streams together, keygen's H(pk), encaps' K, decaps' H(ct);
+ * Frodo: keygen's pkh beside encaps' chain, encaps' ss and decaps' ct absorb
+ * beside decaps' chain).  Every op still computes all of its own hashes and
+ * results are identical.  DPKEM_PLAN_NO_PIPELINE runs such chains as three
+ * dependent tasks instead. */
 #define DPKEM_PLAN_NO_PIPELINE 2u
 int dpkem_plan_set_options(dpkem_plan *p, unsigned flags);
 int dpkem_plan_build(dpkem_plan *p);

@@ -34,13 +34,14 @@ def _dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-LIGHTSABER, FIRESABER = 3, 4
+FRODO_AES, LIGHTSABER, FIRESABER = 2, 3, 4
 
 
 @pytest.mark.parametrize("scheme,n,nstreams,chunks,share,pipe", [
     (SABER, 300, 4, 4, False, True), (SABER, 300, 4, 4, False, False), (SABER, 300, 4, 4, True, True),
     (SABER, 97, 2, 3, True, True), (SABER, 97, 2, 3, False, True), (LIGHTSABER, 70, 2, 2, False, True),
-    (FIRESABER, 45, 3, 2, False, True), (FRODO, 10, 3, 3, False, True)])
+    (FIRESABER, 45, 3, 2, False, True), (FRODO, 10, 3, 3, False, True), (FRODO, 10, 3, 3, False, False),
+    (FRODO, 37, 2, 1, False, True), (FRODO_AES, 6, 2, 2, False, True)])
 def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share,
                                                      pipe):
     import dpcrypto_amd as D
@@ -63,9 +64,9 @@ def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, sc
     t1 = plan.enc(scheme, ct, ss, pk, dec_, after=t0)
     plan.dec(scheme, ss2, ct, sk, after=t1)
     plan.build()
-    # a Saber-family keygen -> enc -> dec chain runs as one launch sequence per
-    # chunk (pipelined, or with matrix sharing); otherwise three dependent tasks
-    per_chunk = 1 if scheme != FRODO and ((share and scheme == SABER) or pipe) else 3
+    # a keygen -> enc -> dec chain runs as one launch sequence per chunk
+    # (pipelined, or Saber with matrix sharing); otherwise three dependent tasks
+    per_chunk = 1 if (share and scheme == SABER) or pipe else 3
     assert plan.info()["subtasks"] == per_chunk * min(chunks, n)
     for rep in range(3):
         for t in (pk, sk, ct, ss, ss2):
