@@ -67,15 +67,16 @@ using k2::absorb2_regs;
 using k2::squeeze2_sliced;
 using k2::store2_words;
 
-// The bodies below take the 32-instance group `vb` and the role explicitly
-// so that the pipelined chain kernels can run several of them in one launch.
+// The bodies below take the instance group `vb` (G instances, lane pair k
+// of the wave serving instance vb * G + k) and the role explicitly so that
+// the pipelined chain kernels can run several of them in one launch.
 #define SAB_PAIR_SETUP                                   \
   const int lane = threadIdx.x & 63;                     \
   const uint32_t p = lane & 1;                           \
-  const int inst = vb * 32 + (lane >> 1);                \
-  if (inst >= n) return;
+  const int inst = vb * G + (lane >> 1);                 \
+  if ((lane >> 1) >= G || inst >= n) return;
 
-template <int L>
+template <int L, int G = 32>
 __device__ __forceinline__ void keygen_hash1_body(int vb, int role, int n, const uint8_t* __restrict__ coins,
                                                   uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                   uint8_t* __restrict__ araw, uint8_t* __restrict__ sraw) {
@@ -110,7 +111,38 @@ __device__ __forceinline__ void keygen_hash1_body(int vb, int role, int n, const
   }
 }
 
+// Encaps' m = SHA3-256(m_raw) for instance inst: this lane's halves into
+// h[0..3], the bytes into mbuf.
+__device__ __forceinline__ void enc_m(int inst, uint32_t p, const uint8_t* __restrict__ coins, uint8_t* __restrict__ mbuf,
+                                      uint32_t h[4]) {
+  uint32_t a[25];
+  uint64_t raw[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) raw[i] = ld64(coins + (size_t)inst * kSaberEncCoins + 8 * i);
+  absorb2_regs<kRateSha3_256, 4>(a, p, raw, kDsSha3);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) h[i] = a[i];
+  store2_words<4>(h, p, mbuf + (size_t)inst * 32);
+}
+
+// Encaps' kr = G(m || H(pk)) from the halves g[0..7] (m, H(pk)), then the
+// GenSecret stream from kr[32:64].
 template <int L>
+__device__ __forceinline__ void enc_g_chain(int inst, uint32_t p, const uint32_t g[8], uint8_t* __restrict__ sraw,
+                                            uint8_t* __restrict__ krbuf) {
+  using P = SaberP<L>;
+  uint32_t a[25];
+  absorb2_halves<kRateSha3_512, 8>(a, p, g, kDsSha3);
+  uint32_t kr[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) kr[i] = a[i];
+  store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
+  absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
+  squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
+      a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
+}
+
+template <int L, int G = 32>
 __device__ __forceinline__ void enc_hash1_body(int vb, int role, int n, const uint8_t* __restrict__ pk,
                                                const uint8_t* __restrict__ coins, uint8_t* __restrict__ araw,
                                                uint8_t* __restrict__ sraw, uint8_t* __restrict__ mbuf,
@@ -127,29 +159,41 @@ __device__ __forceinline__ void enc_hash1_body(int vb, int role, int n, const ui
     squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
         a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * P::matrix_bytes));
   } else {
-    uint64_t raw[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) raw[i] = ld64(coins + (size_t)inst * kSaberEncCoins + 8 * i);
     uint32_t g[8];
-    absorb2_regs<kRateSha3_256, 4>(a, p, raw, kDsSha3);  // m = H(m_raw)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) g[i] = a[i];
+    enc_m(inst, p, coins, mbuf, g);
     absorb2_global<kRateSha3_256, P::pk / 8>(a, p, pkp, kDsSha3);  // H(pk)
 #pragma unroll
     for (int i = 0; i < 4; ++i) g[4 + i] = a[i];
-    store2_words<4>(g, p, mbuf + (size_t)inst * 32);
-    absorb2_halves<kRateSha3_512, 8>(a, p, g, kDsSha3);  // kr = G(m || H(pk))
-    uint32_t kr[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) kr[i] = a[i];
-    store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
-    absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);  // GenSecret(kr[32:64])
-    squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
-        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
+    enc_g_chain<L>(inst, p, g, sraw, krbuf);
   }
 }
 
+// Decaps' message chain for instance inst: kr = G(m' || H(pk)) (H(pk) from
+// sk), then the GenSecret stream from kr[32:64].  m points at the instance's
+// 32 bytes of m' (global memory or LDS).
 template <int L>
+__device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint8_t* __restrict__ sk, const uint8_t* m,
+                                              uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
+  using P = SaberP<L>;
+  const uint8_t* s = sk + (size_t)inst * P::sk;
+  uint32_t a[25];
+  uint64_t buf[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    buf[i] = ld64(m + 8 * i);                   // m'
+    buf[4 + i] = ld64(s + P::sk - 64 + 8 * i);  // H(pk) from sk
+  }
+  absorb2_regs<kRateSha3_512, 8>(a, p, buf, kDsSha3);
+  uint32_t kr[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) kr[i] = a[i];
+  store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
+  absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
+  squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
+      a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
+}
+
+template <int L, int G = 32>
 __device__ __forceinline__ void dec_hash1_body(int vb, int role, int n, const uint8_t* __restrict__ sk,
                                                const uint8_t* __restrict__ mbuf, uint8_t* __restrict__ araw,
                                                uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
@@ -165,27 +209,14 @@ __device__ __forceinline__ void dec_hash1_body(int vb, int role, int n, const ui
     squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
         a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * P::matrix_bytes));
   } else {
-    uint64_t buf[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      buf[i] = ld64(mbuf + (size_t)inst * 32 + 8 * i);  // m'
-      buf[4 + i] = ld64(s + P::sk - 64 + 8 * i);        // H(pk) from sk
-    }
-    absorb2_regs<kRateSha3_512, 8>(a, p, buf, kDsSha3);
-    uint32_t kr[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) kr[i] = a[i];
-    store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
-    absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
-    squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
-        a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
+    dec_msg_chain<L>(inst, p, sk, mbuf + (size_t)inst * 32, sraw, krbuf);
   }
 }
 
 // ----------------------------------------------------------------- hash2
 
 // H(pk) into sk.
-template <int L>
+template <int L, int G = 32>
 __device__ __forceinline__ void keygen_hash2_body(int vb, int n, const uint8_t* __restrict__ pk,
                                                   uint8_t* __restrict__ sk) {
   using P = SaberP<L>;
@@ -197,7 +228,7 @@ __device__ __forceinline__ void keygen_hash2_body(int vb, int n, const uint8_t* 
 
 // K = SHA3-256(kr[0:32] || H(ct)); for decaps kr[0:32] is replaced by z when
 // the re-encryption differs (constant-time select on the mismatch flag).
-template <int L>
+template <int L, int G = 32>
 __device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restrict__ ct,
                                            const uint8_t* __restrict__ krbuf, const uint8_t* __restrict__ sk,
                                            const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
@@ -222,7 +253,7 @@ __device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restr
 // The two halves of K for the pipelined chain: H(ct) of the received
 // ciphertext (it does not depend on the re-encryption), kept as this lane's
 // interleaved halves (hc[8 inst + 2 w + p]), then K from it.
-template <int L>
+template <int L, int G = 32>
 __device__ __forceinline__ void hct_body(int vb, int n, const uint8_t* __restrict__ ct, uint32_t* __restrict__ hc) {
   using P = SaberP<L>;
   SAB_PAIR_SETUP
@@ -232,7 +263,7 @@ __device__ __forceinline__ void hct_body(int vb, int n, const uint8_t* __restric
   for (int i = 0; i < 4; ++i) hc[(size_t)inst * 8 + 2 * i + p] = a[i];
 }
 
-template <int L>
+template <int L, int G = 32>
 __device__ __forceinline__ void k_body(int vb, int n, const uint32_t* __restrict__ hc,
                                        const uint8_t* __restrict__ krbuf, const uint8_t* __restrict__ sk,
                                        const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
@@ -299,11 +330,12 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 //       those bytes visible to the workgroup) wave 0 runs keygen's
 //       GenMatrix, wave 1 keygen's GenSecret, wave 2 encaps' GenMatrix from
 //       the seedA in pk and wave 3 decaps' GenMatrix from the seedA in sk;
-//   x1  wave 0: encaps' message chain (m, H(pk), G, GenSecret); wave 1:
-//       keygen's H(pk) into sk;
-//   x2  wave 0: decaps' message chain (G, GenSecret); wave 1: encaps' K;
-//       wave 2: decaps' H(ct), leaving one permutation (sab_k) after the
-//       re-encryption.
+//   x1  wave 0: encaps' H(pk), then (after one workgroup barrier) G and
+//       GenSecret; wave 1: keygen's H(pk) into sk; wave 2: encaps' m, handed
+//       to wave 0 through LDS;
+//   x2  (after the encaps products, below sab_dec_inner) decaps' inner
+//       product and message chain beside encaps' K and decaps' H(ct),
+//       leaving one permutation (sab_k) after the re-encryption.
 // Every op still computes all of its own hashes; only their order changes.
 template <int L>
 __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restrict__ kc, uint8_t* __restrict__ pk,
@@ -343,24 +375,37 @@ __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restr
 }
 
 template <int L>
-__global__ void __launch_bounds__(128) sab_pipe_x1(int n, const uint8_t* __restrict__ pk,
+__global__ void __launch_bounds__(192) sab_pipe_x1(int n, const uint8_t* __restrict__ pk,
                                                    const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ sk,
                                                    uint8_t* __restrict__ sraw_e, uint8_t* __restrict__ mbuf_e,
                                                    uint8_t* __restrict__ krbuf_e) {
-  if ((threadIdx.x >> 6) == 0) enc_hash1_body<L>(blockIdx.x, 1, n, pk, ecoins, nullptr, sraw_e, mbuf_e, krbuf_e);
-  else keygen_hash2_body<L>(blockIdx.x, n, pk, sk);
-}
-
-template <int L>
-__global__ void __launch_bounds__(192) sab_pipe_x2(int n, const uint8_t* __restrict__ sk,
-                                                   const uint8_t* __restrict__ mbuf_d, uint8_t* __restrict__ sraw_d,
-                                                   uint8_t* __restrict__ krbuf_d, const uint8_t* __restrict__ ct,
-                                                   const uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ ss,
-                                                   uint32_t* __restrict__ hc_d) {
-  const int wave = threadIdx.x >> 6;
-  if (wave == 0) dec_hash1_body<L>(blockIdx.x, 1, n, sk, mbuf_d, nullptr, sraw_d, krbuf_d);
-  else if (wave == 1) hash2_body<L>(blockIdx.x, n, ct, krbuf_e, nullptr, nullptr, ss);
-  else hct_body<L>(blockIdx.x, n, ct, hc_d);
+  using P = SaberP<L>;
+  __shared__ uint32_t mh[4][64];  // wave 2's halves of m, by lane
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t p = lane & 1;
+  const int inst = blockIdx.x * 32 + (lane >> 1);
+  const bool live = inst < n;
+  uint32_t g[8];
+  if (live) {
+    if (wave == 0) {
+      uint32_t a[25];
+      absorb2_global<kRateSha3_256, P::pk / 8>(a, p, pk + (size_t)inst * P::pk, kDsSha3);  // encaps' H(pk)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) g[4 + i] = a[i];
+    } else if (wave == 1) {
+      keygen_hash2_body<L>(blockIdx.x, n, pk, sk);
+    } else {
+      uint32_t h[4];
+      enc_m(inst, p, ecoins, mbuf_e, h);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) mh[i][lane] = h[i];
+    }
+  }
+  __syncthreads();
+  if (!live || wave != 0) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) g[i] = mh[i][lane];
+  enc_g_chain<L>(inst, p, g, sraw_e, krbuf_e);
 }
 
 // decaps' K from the stored H(ct): K = SHA3-256((fail ? z : kr[0:32]) || H(ct))
@@ -696,6 +741,134 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
   mbuf[(size_t)inst * 32 + t] = (uint8_t)byte;
 }
 
+// Decaps' inner product v = b'^T s of one instance on 32 lanes (t = lane in
+// the half-wave) and its message byte t: m'_t = ((v + h2 - c_m 2^(10 - eT))
+// >> 9) & 1 bit by bit, as in sab_dec_inner, but one K-block at a time
+// through a single Toeplitz buffer Ts (kToepStride u16) and b' row Bp (kARow
+// u16) so that sixteen instances fit one workgroup's LDS.  The 32 lanes of an
+// instance belong to one wave; LDS ordering between them needs no workgroup
+// barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t dec_inner_half(int t, const uint8_t* __restrict__ c, const uint8_t* __restrict__ s,
+                                                   uint16_t* Ts, uint16_t* Bp) {
+  using P = SaberP<L>;
+  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int t0 = 8 * t;
+#pragma unroll 1
+  for (int j = 0; j < L; ++j) {
+    uint8_t bb[2][5], sb[13];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) bb[h][i] = c[320 * j + 5 * (t + 32 * h) + i];  // b'_j, 10-bit groups of 4
+#pragma unroll
+    for (int i = 0; i < 13; ++i) sb[i] = s[416 * j + 13 * t + i];  // s_j, 13-bit groups of 8
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint16_t v[4];
+      unpack10_group(v, bb[h]);
+      uint2 q;
+      q.x = v[0] | ((uint32_t)v[1] << 16);
+      q.y = v[2] | ((uint32_t)v[3] << 16);
+      *reinterpret_cast<uint2*>(Bp + 4 * (t + 32 * h)) = q;
+    }
+    uint16_t v[8];
+    unpack13_group(v, sb);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) toep_put(Ts, 0, 8 * t + i, v[i]);
+    wave_lds_sync();
+    toep_dot(acc, Ts, 0, Bp, t0);
+    wave_lds_sync();  // the next K-block overwrites Ts / Bp
+  }
+  uint64_t cm = 0;  // eT bits for each of t0..t0+7
+#pragma unroll
+  for (int i = 0; i < P::et; ++i) cm |= (uint64_t)c[P::vecp + P::et * t + i] << (8 * i);
+  uint32_t byte = 0;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    const uint32_t cmd = (uint32_t)(cm >> (P::et * d)) & ((1u << P::et) - 1);
+    byte |= (((acc[d] + P::h2 - (cmd << (10 - P::et))) >> 9) & 1u) << d;
+  }
+  return byte;
+}
+
+// x2 of the pipelined chain: one workgroup of sixteen waves per 16
+// instances.  A workgroup's waves go to the CU's four SIMDs in a repeating
+// order of period four (measured on MI355X: waves w and w + 4 always share a
+// SIMD), so the roles are laid out by w mod 4 and the two long hash chains
+// each get a SIMD of their own from the start:
+//   w = 0          encaps' K = SHA3-256(kr[0:32] || H(ct));
+//   w = 1          decaps' H(ct) into hc_d (sab_k finishes K after the
+//                  re-encryption);
+//   w mod 4 = 2, 3 (eight waves) decaps' inner products, two instances per
+//                  wave; m' goes to LDS and to mbuf_d (the re-encryption
+//                  reads it).  Wave kX2ChainWave then waits until all eight
+//                  have posted (an LDS counter, not a workgroup barrier, so
+//                  the hash waves never wait) and runs decaps' message chain
+//                  (G, GenSecret) for the 16 instances on lane pairs;
+//   w = 4, 5, 8, 9, 12, 13 leave at once (they only keep the layout).
+constexpr int kX2Group = 16;
+constexpr int kX2InnerWaves = kX2Group / 2;
+constexpr int kX2ChainWave = 2;
+constexpr int kX2Threads = 1024;
+
+struct X2Lds {
+  alignas(16) uint16_t Ts[kX2Group][kToepStride];
+  alignas(16) uint16_t Bp[kX2Group][kARow];
+  alignas(8) uint8_t mq[kX2Group][32];
+  uint32_t posted;
+};
+
+template <int L>
+__global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* __restrict__ sk,
+                                                          const uint8_t* __restrict__ ct, uint8_t* __restrict__ mbuf_d,
+                                                          uint8_t* __restrict__ sraw_d, uint8_t* __restrict__ krbuf_d,
+                                                          const uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ ss,
+                                                          uint32_t* __restrict__ hc_d) {
+  using P = SaberP<L>;
+  __shared__ X2Lds S;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) S.posted = 0;
+  __syncthreads();  // the only workgroup barrier: every wave passes it before any work
+  if ((wave & 3) < 2) {
+    if (wave == 0) hash2_body<L, kX2Group>(blockIdx.x, n, ct, krbuf_e, nullptr, nullptr, ss);
+    else if (wave == 1) hct_body<L, kX2Group>(blockIdx.x, n, ct, hc_d);
+    return;
+  }
+  const int iw = (wave >> 2) * 2 + (wave & 1);  // inner wave 0..7
+  const int base = blockIdx.x * kX2Group;
+  {
+    const int k = 2 * iw + (lane >> 5), t = lane & 31, inst = base + k;
+    const bool live = inst < n;
+    const int ci = live ? inst : 0;
+    const uint32_t byte = dec_inner_half<L>(t, ct + (size_t)ci * P::ct, sk + (size_t)ci * P::sk, S.Ts[k], S.Bp[k]);
+    if (live) {
+      S.mq[k][t] = (uint8_t)byte;
+      mbuf_d[(size_t)inst * 32 + t] = (uint8_t)byte;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_fetch_add(&S.posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  if (wave != kX2ChainWave) return;
+  // Every wave of a workgroup is resident from its start and each inner wave
+  // posts once after bounded work, so the count reaches kX2InnerWaves; the
+  // iteration bound only makes the loop's end unconditional.
+  for (int spin = 0; spin < (1 << 22); ++spin) {
+    if (__hip_atomic_load(&S.posted, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= (uint32_t)kX2InnerWaves) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  const uint32_t p = lane & 1;
+  const int k = lane >> 1;
+  if (k >= kX2Group || base + k >= n) return;
+  dec_msg_chain<L>(base + k, p, sk, S.mq[k], sraw_d, krbuf_d);
+}
+
 // ----------------------------------------------------------------- host launchers
 // (declared in launch.hpp; all run on the given stream)
 
@@ -792,10 +965,11 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
 
 // Pipelined keygen -> encaps -> decaps of the same n instances on one stream
 // (see sab_pipe_x0 .. sab_pipe_x2).  Per instance the serial Keccak path drops
-// from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 15 + 10 + 1 permutations (all
+// from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 14 + 10 + 1 permutations (all
 // three GenMatrix streams start from the seedA bytes of pk / sk as soon as
-// keygen has written them; keygen's H(pk), encaps' K and decaps' H(ct) run
-// beside other stages), and ten launches become eight.  scratch:
+// keygen has written them; keygen's H(pk), encaps' m and K and decaps' H(ct)
+// run beside other stages, decaps' inner product beside encaps' K), and ten
+// launches become seven.  scratch:
 // saber_pipe_scratch_per_inst(l) bytes per instance: three GenMatrix streams,
 // two GenSecret, two m and two kr buffers (keygen's GenSecret reuses decaps'
 // buffer, which the stream order makes safe), decaps' H(ct) and the mismatch
@@ -832,17 +1006,16 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint32_t* hc_d = reinterpret_cast<uint32_t*>(take(32));
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   const int nb = blocks_for(n, 32);
-  const dim3 g(nb), b64(64), b128(128), b192(192), bp(P::threads);
+  const dim3 g(nb), b64(64), b192(192), bp(P::threads);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x0"), sab_pipe_x0<L>, g, dim3(256), st, n, kc, pk, sk, araw_k, sraw_d, araw_e,
                araw_d);
   SabPolyArgs ak{n, araw_k, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
-  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, b128, st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, b192, st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e);
   SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
-  DPKEM_LAUNCH(SAB_NAME("sab_dec_inner"), sab_dec_inner<L>, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf_d);
-  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, g, b192, st, n, sk, mbuf_d, sraw_d, krbuf_d, ct, krbuf_e, ss,
-               hc_d);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, dim3(blocks_for(n, kX2Group)), dim3(kX2Threads), st, n, sk, ct,
+               mbuf_d, sraw_d, krbuf_d, krbuf_e, ss, hc_d);
   SabPolyArgs ad{n, araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, fail};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabReenc>"), (sab_poly<kSabReenc, L>), dim3(n), bp, st, ad);
   DPKEM_LAUNCH(SAB_NAME("sab_k"), sab_k<L>, g, b64, st, n, hc_d, krbuf_d, sk, fail, ss2);
