@@ -441,10 +441,16 @@ constexpr int kARow = 264;  // u16 row stride of plain polynomials in the dot2 p
 // (A[i][j], or A[j][i] for keygen's A^T s) in columns i / 8 + i and of b[j]
 // (inner product) in columns L / 8 + L.  Per instance: L x 64 matrix
 // instructions, 28 L Toeplitz operand reads.
+// Slot strides of the B-operand limbs and the limb sums are padded off a
+// multiple of the LDS bank count: the sixteen lanes of one access read or
+// write sixteen different slots at the same k.
+constexpr int kLbSlot = 272;    // bytes per B-operand slot (256 used)
+constexpr int kPartSlot = 264;  // u16 per limb-sum slot (256 used)
+
 template <int L>
 struct SabPolyLds {
   alignas(16) int8_t V[L * kVPoly];                // staged Toeplitz copies of s / s'
-  alignas(16) int8_t Lb[L * 2 * (L + 1) * 256];    // B-operand limbs: [K-block j][slot][k]
+  alignas(16) int8_t Lb[L * 2 * (L + 1) * kLbSlot];  // B-operand limbs: [K-block j][slot][k]
   union {
     struct {
       alignas(8) uint32_t stage[SaberP<L>::matrix_bytes / 4];
@@ -452,7 +458,7 @@ struct SabPolyLds {
       alignas(8) uint32_t bstage[SaberP<L>::vecp / 4];  // b bytes of pk (encaps / re-encryption)
     } in;
     struct {
-      alignas(16) uint16_t part[2 * (L + 1) * 256];  // limb sums per slot (mod 2^16)
+      alignas(16) uint16_t part[2 * (L + 1) * kPartSlot];  // limb sums per slot (mod 2^16)
       alignas(8) uint8_t outb[SaberP<L>::ct + 64];
     } out;
   } u;
@@ -539,9 +545,9 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
       lo |= (uint64_t)(uint8_t)l8 << (8 * i);
       hi |= (uint64_t)(uint8_t)h8 << (8 * i);
     }
-    int8_t* blk = S.Lb + j * SLOTS * 256 + 8 * (g & 31);
-    *reinterpret_cast<uint64_t*>(blk + col * 256) = lo;
-    *reinterpret_cast<uint64_t*>(blk + (L + 1 + col) * 256) = hi;
+    int8_t* blk = S.Lb + j * SLOTS * kLbSlot + 8 * (g & 31);
+    *reinterpret_cast<uint64_t*>(blk + col * kLbSlot) = lo;
+    *reinterpret_cast<uint64_t*>(blk + (L + 1 + col) * kLbSlot) = hi;
   }
   if (MODE != kSabKeygen) {
     const uint8_t* pkb = reinterpret_cast<const uint8_t*>(S.u.in.bstage);
@@ -557,9 +563,9 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
         lo |= (uint32_t)(uint8_t)l8 << (8 * i);
         hi |= (uint32_t)(uint8_t)h8 << (8 * i);
       }
-      int8_t* blk = S.Lb + (g >> 6) * SLOTS * 256 + 4 * (g & 63);
-      *reinterpret_cast<uint32_t*>(blk + L * 256) = lo;
-      *reinterpret_cast<uint32_t*>(blk + (2 * L + 1) * 256) = hi;
+      int8_t* blk = S.Lb + (g >> 6) * SLOTS * kLbSlot + 4 * (g & 63);
+      *reinterpret_cast<uint32_t*>(blk + L * kLbSlot) = lo;
+      *reinterpret_cast<uint32_t*>(blk + (2 * L + 1) * kLbSlot) = hi;
     }
   }
   __syncthreads();
@@ -572,7 +578,7 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
   #pragma unroll 1
   for (int j = 0; j < L; ++j) {
     v4i8x4 bop[4];
-    const int8_t* colp = S.Lb + (j * SLOTS + (slot < 0 ? 0 : slot)) * 256 + 16 * (tid >> 4);
+    const int8_t* colp = S.Lb + (j * SLOTS + (slot < 0 ? 0 : slot)) * kLbSlot + 16 * (tid >> 4);
 #pragma unroll
     for (int b = 0; b < 4; ++b)
       bop[b] = slot >= 0 ? *reinterpret_cast<const v4i8x4*>(colp + 64 * b) : v4i8x4{0, 0, 0, 0};
@@ -587,7 +593,7 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
       uint2 w;
       w.x = ((uint32_t)acc[t][0] & 0xFFFF) | ((uint32_t)acc[t][1] << 16);
       w.y = ((uint32_t)acc[t][2] & 0xFFFF) | ((uint32_t)acc[t][3] << 16);
-      *reinterpret_cast<uint2*>(S.u.out.part + slot * 256 + 16 * t + 4 * (tid >> 4)) = w;
+      *reinterpret_cast<uint2*>(S.u.out.part + slot * kPartSlot + 16 * t + 4 * (tid >> 4)) = w;
     }
   }
   __syncthreads();
@@ -600,8 +606,8 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
   for (int x = tid; x < 32 * ROWS; x += NT) {
     const int row = x >> 5, grp = x & 31;
     uint32_t acc8[8];
-    const uint16_t* plo = S.u.out.part + row * 256 + 8 * grp;
-    const uint16_t* phi = S.u.out.part + (L + 1 + row) * 256 + 8 * grp;
+    const uint16_t* plo = S.u.out.part + row * kPartSlot + 8 * grp;
+    const uint16_t* phi = S.u.out.part + (L + 1 + row) * kPartSlot + 8 * grp;
 #pragma unroll
     for (int d = 0; d < 8; ++d) acc8[d] = (uint32_t)plo[d] + 256u * (uint32_t)phi[d];
     if (row < L) {

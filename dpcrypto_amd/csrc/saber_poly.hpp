@@ -79,7 +79,10 @@ __device__ __forceinline__ void toep_dot(uint32_t acc[8], const uint16_t* T, int
 // T(s) is held as the reversed sequence V[e] = T[t][t + e] (e = k - t in
 // [-255, 255]): V[e] = s[-e] for e <= 0 and -s[256 - e] for e > 0.  Four
 // copies, copy c holding V[e] at byte e + 256 + c, make every 16-byte run
-// V[e0 .. e0 + 15] a 4-byte aligned read from copy (-e0) & 3.
+// V[e0 .. e0 + 15] a 4-byte aligned read from copy (-e0) & 3.  The copies
+// are 136 dwords apart (8 mod 32 banks): the lanes of one operand read use
+// copy m & 3 and runs a dword apart within a copy, so each copy's runs fall
+// in a bank window of their own.
 //
 // Tiles: M-tile a covers outputs t = 16a .. 16a + 15, K-chunk b covers
 // k = 64b .. 64b + 63.  The A operand of tile (a, b) depends only on the
@@ -90,7 +93,7 @@ __device__ __forceinline__ void toep_dot(uint32_t acc[8], const uint16_t* T, int
 
 typedef int v4i8x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kVCopy = 520;                 // bytes per shifted copy
+constexpr int kVCopy = 544;                 // bytes per shifted copy (515 used)
 constexpr int kVPoly = 4 * kVCopy;          // bytes per staged polynomial
 constexpr int kToepDiagonals = 28;          // delta = 16 a - 64 b, a in [0, 16), b in [0, 4)
 
