@@ -238,18 +238,36 @@ __device__ __forceinline__ uint64_t out_full(uint32_t mine, uint32_t p) {
 
 // Absorb NW message words from global memory with rate RW (64-bit words),
 // pad with domain byte ds and run the final permutation.
+// The words of block b + 1 (or of the final partial block) are loaded while
+// block b is permuted, so a block's loads never wait out a full memory
+// latency on the chain's critical path.
 template <int RW, int NW>
 __device__ __forceinline__ void absorb2_global(uint32_t a[25], uint32_t p, const uint8_t* __restrict__ src,
                                                uint64_t ds) {
   constexpr int NB = NW / RW, REM = NW % RW;
+  const uint64_t* w = reinterpret_cast<const uint64_t*>(src);
   zero2(a);
-  for (int b = 0; b < NB; ++b) {
+  uint64_t nxt[RW];
 #pragma unroll
-    for (int i = 0; i < RW; ++i) a[i] ^= deint(*reinterpret_cast<const uint64_t*>(src + 8 * (b * RW + i)), p);
+  for (int i = 0; i < RW; ++i)
+    if (NB > 0 || i < REM) nxt[i] = w[i];  // block 0, or the whole message when shorter than a block
+  for (int b = 0; b < NB; ++b) {
+    uint64_t cur[RW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i) cur[i] = nxt[i];
+    if (b + 1 < NB) {
+#pragma unroll
+      for (int i = 0; i < RW; ++i) nxt[i] = w[(b + 1) * RW + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < REM; ++i) nxt[i] = w[NB * RW + i];
+    }
+#pragma unroll
+    for (int i = 0; i < RW; ++i) a[i] ^= deint(cur[i], p);
     keccakf2(a, p);
   }
 #pragma unroll
-  for (int i = 0; i < REM; ++i) a[i] ^= deint(*reinterpret_cast<const uint64_t*>(src + 8 * (NB * RW + i)), p);
+  for (int i = 0; i < REM; ++i) a[i] ^= deint(nxt[i], p);
   a[REM] ^= half_const(ds, p);
   a[RW - 1] ^= half_const(0x8000000000000000ULL, p);
   keccakf2(a, p);
