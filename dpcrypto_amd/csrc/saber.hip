@@ -441,152 +441,216 @@ constexpr int kARow = 264;  // u16 row stride of plain polynomials in the dot2 p
 // (A[i][j], or A[j][i] for keygen's A^T s) in columns i / 8 + i and of b[j]
 // (inner product) in columns L / 8 + L.  Per instance: L x 64 matrix
 // instructions, 28 L Toeplitz operand reads.
+//
+// The K-blocks stream through LDS one at a time (the Toeplitz copies of s_j,
+// the limbs of block j and its raw bytes, ~6-7 KB): block j + 1's global
+// words are loaded into registers while block j is unpacked and multiplied,
+// and the small footprint lets all of a 4096-instance batch be resident at
+// once instead of in two rounds.
 // Slot strides of the B-operand limbs and the limb sums are padded off a
 // multiple of the LDS bank count: the sixteen lanes of one access read or
 // write sixteen different slots at the same k.
 constexpr int kLbSlot = 272;    // bytes per B-operand slot (256 used)
 constexpr int kPartSlot = 264;  // u16 per limb-sum slot (256 used)
+constexpr int kPolyWords = 52;  // 64-bit words of one 13-bit packed polynomial (416 B)
 
 template <int L>
 struct SabPolyLds {
-  alignas(16) int8_t V[L * kVPoly];                // staged Toeplitz copies of s / s'
-  alignas(16) int8_t Lb[L * 2 * (L + 1) * kLbSlot];  // B-operand limbs: [K-block j][slot][k]
   union {
     struct {
-      alignas(8) uint32_t stage[SaberP<L>::matrix_bytes / 4];
-      alignas(8) uint32_t sstage[SaberP<L>::secret_bytes / 4];
-      alignas(8) uint32_t bstage[SaberP<L>::vecp / 4];  // b bytes of pk (encaps / re-encryption)
-    } in;
+      alignas(16) int8_t V[kVPoly];                    // Toeplitz copies of s_j
+      alignas(16) int8_t Lb[2 * (L + 1) * kLbSlot];    // B-operand limbs of K-block j: [slot][k]
+      alignas(16) uint64_t mstage[L * kPolyWords];     // matrix polynomials of K-block j (bytes)
+      alignas(16) uint64_t sstage[SaberP<L>::mu * 4];  // s_j's CBD bytes (32 mu)
+      alignas(16) uint32_t bstage[80 + 4];             // b_j (320 B, 10-bit), + slack for the last group's read
+    } k;
     struct {
       alignas(16) uint16_t part[2 * (L + 1) * kPartSlot];  // limb sums per slot (mod 2^16)
       alignas(8) uint8_t outb[SaberP<L>::ct + 64];
     } out;
   } u;
-  uint32_t diff;
 };
 
-// Byte-wise little-endian load of up to 8 bytes from LDS.
-__device__ __forceinline__ uint64_t lds_le(const uint8_t* b, int nbytes) {
-  uint64_t v = 0;
-  for (int i = 0; i < nbytes; ++i) v |= (uint64_t)b[i] << (8 * i);
-  return v;
+// Ordering of LDS accesses between the lanes of one wave: a wave's LDS
+// operations complete in issue order, so wavefront-scope fences (no wait
+// instructions) and a code-motion barrier suffice; in particular no wait for
+// the global loads still in flight.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Little-endian bytes [off, off + 16 - (off & 3)) of a 4-byte aligned LDS
+// buffer as a 128-bit value shifted down to byte off: four aligned dword
+// reads instead of byte reads.
+__device__ __forceinline__ void lds_bytes128(const uint8_t* base, int off, uint64_t* lo, uint64_t* hi) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(base + (off & ~3));
+  const uint32_t d0 = w[0], d1 = w[1], d2 = w[2], d3 = w[3];
+  const int sh = 8 * (off & 3);
+  uint64_t l = ((uint64_t)d1 << 32) | d0, h = ((uint64_t)d3 << 32) | d2;
+  if (sh) {
+    l = (l >> sh) | (h << (64 - sh));
+    h >>= sh;
+  }
+  *lo = l;
+  *hi = h;
+}
+// Up to 5 bytes at off (two aligned dword reads).
+__device__ __forceinline__ uint64_t lds_bytes64(const uint8_t* base, int off) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(base + (off & ~3));
+  return ((((uint64_t)w[1] << 32) | w[0]) >> (8 * (off & 3)));
 }
 
 template <int MODE, int L>
-__global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) sab_poly(SabPolyArgs a) {
   using P = SaberP<L>;
   constexpr int NT = 64;
-  constexpr int SLOTS = 2 * (L + 1);
+  constexpr bool HAS_B = MODE != kSabKeygen;
+  constexpr int MW = L * kPolyWords, MPT = (MW + NT - 1) / NT;  // sliced matrix words per K-block
+  constexpr int SW = P::mu * 4;                                // sliced secret words per polynomial
+  constexpr int CW = MODE == kSabReenc ? P::ct / 4 : 0, CPT = (CW + NT - 1) / NT;
   __shared__ __align__(16) SabPolyLds<L> S;
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
   if (inst >= a.n) return;
 
-  // 1. rebuild the GenMatrix and GenSecret bytes (lane-pair sliced streams)
-  //    and copy pk's b bytes into LDS; every global load of a thread is issued
-  //    before the first is used (one wave per instance hides little latency)
-  {
-    constexpr int MW = P::matrix_bytes / 8, SW = P::secret_bytes / 8, BW = MODE == kSabKeygen ? 0 : P::vecp / 4;
-    constexpr int MPT = (MW + NT - 1) / NT, SPT = (SW + NT - 1) / NT, BPT = (BW + NT - 1) / NT;
-    const uint2* ar = reinterpret_cast<const uint2*>(a.araw + (size_t)inst * P::matrix_bytes);
-    const uint2* sr = reinterpret_cast<const uint2*>(a.sraw + (size_t)inst * P::secret_bytes);
-    const uint32_t* bp = MODE == kSabKeygen ? nullptr : reinterpret_cast<const uint32_t*>(a.pk + (size_t)inst * a.pk_stride);
-    uint2 mv[MPT], sv[SPT];
-    uint32_t bv[BPT > 0 ? BPT : 1];
+  const uint2* ar = reinterpret_cast<const uint2*>(a.araw + (size_t)inst * P::matrix_bytes);
+  const uint2* sr = reinterpret_cast<const uint2*>(a.sraw + (size_t)inst * P::secret_bytes);
+  const uint32_t* bp = HAS_B ? reinterpret_cast<const uint32_t*>(a.pk + (size_t)inst * a.pk_stride) : nullptr;
+  // Word t of K-block j's matrix polynomials: polynomial t / 52 of the block
+  // is output column t / 52, i.e. A[col][j] (encaps) or A[j][col] (keygen).
+  uint2 mv[MPT], sv = {0, 0};
+  uint32_t bv[2] = {0, 0};
+  auto fetch = [&](int j) {
 #pragma unroll
-    for (int i = 0; i < MPT; ++i)
-      if (tid + i * NT < MW) mv[i] = ar[tid + i * NT];
+    for (int r = 0; r < MPT; ++r) {
+      const int t = tid + r * NT;
+      if (t < MW) {
+        const int col = t / kPolyWords, w = t - kPolyWords * col;
+        const int poly = MODE == kSabKeygen ? j * L + col : col * L + j;
+        mv[r] = ar[kPolyWords * poly + w];
+      }
+    }
+    if (tid < SW) sv = sr[SW * j + tid];
+    if (HAS_B) {
 #pragma unroll
-    for (int i = 0; i < SPT; ++i)
-      if (tid + i * NT < SW) sv[i] = sr[tid + i * NT];
+      for (int r = 0; r < 2; ++r)
+        if (tid + r * NT < 80) bv[r] = bp[80 * j + tid + r * NT];
+    }
+  };
+  uint32_t cv[CPT > 0 ? CPT : 1];  // re-encryption: the received ciphertext, compared at the end
+  if (MODE == kSabReenc) {
+    const uint32_t* ct = reinterpret_cast<const uint32_t*>(a.ct_in + (size_t)inst * P::ct);
 #pragma unroll
-    for (int i = 0; i < BPT; ++i)
-      if (tid + i * NT < BW) bv[i] = bp[tid + i * NT];
-#pragma unroll
-    for (int i = 0; i < MPT; ++i)
-      if (tid + i * NT < MW) reinterpret_cast<uint64_t*>(S.u.in.stage)[tid + i * NT] = k2::reint_full(mv[i].x, mv[i].y);
-#pragma unroll
-    for (int i = 0; i < SPT; ++i)
-      if (tid + i * NT < SW) reinterpret_cast<uint64_t*>(S.u.in.sstage)[tid + i * NT] = k2::reint_full(sv[i].x, sv[i].y);
-#pragma unroll
-    for (int i = 0; i < BPT; ++i)
-      if (tid + i * NT < BW) S.u.in.bstage[tid + i * NT] = bv[i];
+    for (int r = 0; r < CPT; ++r)
+      if (tid + r * NT < CW) cv[r] = ct[tid + r * NT];
   }
-  if (tid == 0) S.diff = 0;
-  __syncthreads();
+  fetch(0);
 
-  // 2. CBD (mu bits per coefficient: HW(low mu/2) - HW(high mu/2)) into the
-  //    Toeplitz copies; A (groups of 8 13-bit values) and b (groups of 4
-  //    10-bit values, from pk) into the limb columns
-  const uint8_t* ssb = reinterpret_cast<const uint8_t*>(S.u.in.sstage);
-  #pragma unroll 1
-  for (int g = tid; g < 64 * L; g += NT) {  // 4 coefficients from mu / 2 bytes
-    const uint64_t v = lds_le(ssb + g * (P::mu / 2), P::mu / 2);
+  const int slot = toep8_slot(tid & 15, L, HAS_B);
+  v4i8x4 acc[16];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t x = (uint32_t)(v >> (P::mu * k)) & ((1u << P::mu) - 1);
-      const int c = 4 * g + k;
-      const int half = P::mu / 2;
-      toep8_put(S.V + (c >> 8) * kVPoly, c & 255, (int)popc(x & ((1u << half) - 1)) - (int)popc(x >> half));
-    }
-  }
-  const uint8_t* sb = reinterpret_cast<const uint8_t*>(S.u.in.stage);
-  #pragma unroll 1
-  for (int g = tid; g < 32 * L * L; g += NT) {
-    uint16_t v[8];
-    unpack13_group(v, sb + 13 * g);
-    const int poly = g >> 5, r = poly / L, c = poly % L;
-    const int col = MODE == kSabKeygen ? c : r, j = MODE == kSabKeygen ? r : c;
-    uint64_t lo = 0, hi = 0;
+  for (int t = 0; t < 16; ++t) acc[t] = v4i8x4{0, 0, 0, 0};
+
+#pragma unroll 1
+  for (int j = 0; j < L; ++j) {
+    // 1. this block's bytes into LDS (sliced words rebuilt), then block j + 1's loads
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      int8_t l8, h8;
-      limbs8(v[i], &l8, &h8);
-      lo |= (uint64_t)(uint8_t)l8 << (8 * i);
-      hi |= (uint64_t)(uint8_t)h8 << (8 * i);
+    for (int r = 0; r < MPT; ++r)
+      if (tid + r * NT < MW) S.u.k.mstage[tid + r * NT] = k2::reint_full(mv[r].x, mv[r].y);
+    if (tid < SW) S.u.k.sstage[tid] = k2::reint_full(sv.x, sv.y);
+    if (HAS_B) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+        if (tid + r * NT < 80) S.u.k.bstage[tid + r * NT] = bv[r];
     }
-    int8_t* blk = S.Lb + j * SLOTS * kLbSlot + 8 * (g & 31);
-    *reinterpret_cast<uint64_t*>(blk + col * kLbSlot) = lo;
-    *reinterpret_cast<uint64_t*>(blk + (L + 1 + col) * kLbSlot) = hi;
-  }
-  if (MODE != kSabKeygen) {
-    const uint8_t* pkb = reinterpret_cast<const uint8_t*>(S.u.in.bstage);
-    #pragma unroll 1
-    for (int g = tid; g < 64 * L; g += NT) {
-      uint16_t v[4];
-      unpack10_group(v, pkb + 5 * g);
+    if (j + 1 < L) fetch(j + 1);
+    wave_sync();
+
+    // 2. CBD of s_j (mu bits per coefficient: HW(low mu/2) - HW(high mu/2))
+    //    into the Toeplitz copies; the matrix polynomials (groups of 8 13-bit
+    //    values) and b_j (groups of 4 10-bit values) into the limb slots
+    {
+      const int g = tid;  // coefficients 4g .. 4g + 3 from mu / 2 bytes
+      const uint64_t v = lds_bytes64(reinterpret_cast<const uint8_t*>(S.u.k.sstage), g * (P::mu / 2));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t x = (uint32_t)(v >> (P::mu * k)) & ((1u << P::mu) - 1);
+        const int half = P::mu / 2;
+        toep8_put(S.u.k.V, 4 * g + k, (int)popc(x & ((1u << half) - 1)) - (int)popc(x >> half));
+      }
+    }
+    const uint8_t* mb = reinterpret_cast<const uint8_t*>(S.u.k.mstage);
+#pragma unroll
+    for (int r = 0; r < (32 * L + NT - 1) / NT; ++r) {
+      const int g = tid + r * NT;
+      if (g < 32 * L) {
+        uint64_t lo13, hi13;
+        lds_bytes128(mb, 13 * g, &lo13, &hi13);
+        uint64_t lo = 0, hi = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int bit = 13 * i;
+          const uint64_t x = bit < 64 ? (lo13 >> bit) | (bit > 51 ? (hi13 << (64 - bit)) : 0) : (hi13 >> (bit - 64));
+          int8_t l8, h8;
+          limbs8((uint32_t)(x & 0x1FFF), &l8, &h8);
+          lo |= (uint64_t)(uint8_t)l8 << (8 * i);
+          hi |= (uint64_t)(uint8_t)h8 << (8 * i);
+        }
+        const int col = g >> 5;
+        int8_t* blk = S.u.k.Lb + 8 * (g & 31);
+        *reinterpret_cast<uint64_t*>(blk + col * kLbSlot) = lo;
+        *reinterpret_cast<uint64_t*>(blk + (L + 1 + col) * kLbSlot) = hi;
+      }
+    }
+    if (HAS_B) {
+      const int g = tid;  // b_j coefficients 4g .. 4g + 3
+      const uint64_t x = lds_bytes64(reinterpret_cast<const uint8_t*>(S.u.k.bstage), 5 * g);
       uint32_t lo = 0, hi = 0;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         int8_t l8, h8;
-        limbs8(v[i], &l8, &h8);
+        limbs8((uint32_t)(x >> (10 * i)) & 0x3FF, &l8, &h8);
         lo |= (uint32_t)(uint8_t)l8 << (8 * i);
         hi |= (uint32_t)(uint8_t)h8 << (8 * i);
       }
-      int8_t* blk = S.Lb + (g >> 6) * SLOTS * kLbSlot + 4 * (g & 63);
+      int8_t* blk = S.u.k.Lb + 4 * g;
       *reinterpret_cast<uint32_t*>(blk + L * kLbSlot) = lo;
       *reinterpret_cast<uint32_t*>(blk + (2 * L + 1) * kLbSlot) = hi;
     }
-  }
-  __syncthreads();
+    wave_sync();
 
-  // 3. products: result tile a, lane (g, n) = rows 16 a + 4 g + 0..3 of column n
-  const int slot = toep8_slot(tid & 15, L, MODE != kSabKeygen);
-  v4i8x4 acc[16];
+    // keygen: sk[416 j .. 416 j + 416) = s_j in 13 bits (two's complement),
+    // read back from Toeplitz copy 0 (s[q] at byte 256 - q)
+    if (MODE == kSabKeygen && tid < 32) {
+      uint64_t lo = 0, hi = 0;
 #pragma unroll
-  for (int t = 0; t < 16; ++t) acc[t] = v4i8x4{0, 0, 0, 0};
-  #pragma unroll 1
-  for (int j = 0; j < L; ++j) {
+      for (int d = 0; d < 8; ++d) {
+        const uint64_t v = (uint64_t)(int64_t)S.u.k.V[256 - (8 * tid + d)] & 0x1FFF;
+        const int bit = 13 * d;
+        if (bit < 64) lo |= v << bit;
+        if (bit + 13 > 64) hi |= bit >= 64 ? v << (bit - 64) : v >> (64 - bit);
+      }
+      uint8_t* o = a.sk + (size_t)inst * P::sk + 416 * j + 13 * tid;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
+#pragma unroll
+      for (int i = 0; i < 5; ++i) o[8 + i] = (uint8_t)(hi >> (8 * i));
+    }
+
+    // 3. products: result tile t, lane (g, n) = rows 16 t + 4 g + 0..3 of column n
     v4i8x4 bop[4];
-    const int8_t* colp = S.Lb + (j * SLOTS + (slot < 0 ? 0 : slot)) * kLbSlot + 16 * (tid >> 4);
+    const int8_t* colp = S.u.k.Lb + (slot < 0 ? 0 : slot) * kLbSlot + 16 * (tid >> 4);
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-      bop[b] = slot >= 0 ? *reinterpret_cast<const v4i8x4*>(colp + 64 * b) : v4i8x4{0, 0, 0, 0};
-    toep8_block(acc, S.V + j * kVPoly, bop, tid, [](v4i8x4 x, v4i8x4 y, v4i8x4 z) {
+    for (int bb = 0; bb < 4; ++bb)
+      bop[bb] = slot >= 0 ? *reinterpret_cast<const v4i8x4*>(colp + 64 * bb) : v4i8x4{0, 0, 0, 0};
+    toep8_block(acc, S.u.k.V, bop, tid, [](v4i8x4 x, v4i8x4 y, v4i8x4 z) {
       return __builtin_amdgcn_mfma_i32_16x16x64_i8(x, y, z, 0, 0, 0);
     });
+    wave_sync();  // the next block (or the limb sums) overwrite V / Lb / the stages
   }
-  __syncthreads();  // the limb sums overwrite the stage area
+
   if (slot >= 0) {
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
@@ -596,13 +660,13 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
       *reinterpret_cast<uint2*>(S.u.out.part + slot * kPartSlot + 16 * t + 4 * (tid >> 4)) = w;
     }
   }
-  __syncthreads();
+  wave_sync();
 
   // 4. rounding + packing into the LDS output image: rows 0..L-1 are b' (or
   //    keygen's b), row L the inner product v'; 8 coefficients per item
-  constexpr int ROWS = MODE == kSabKeygen ? L : L + 1;
+  constexpr int ROWS = HAS_B ? L + 1 : L;
   uint8_t* outb = S.u.out.outb;
-  #pragma unroll 1
+#pragma unroll 1
   for (int x = tid; x < 32 * ROWS; x += NT) {
     const int row = x >> 5, grp = x & 31;
     uint32_t acc8[8];
@@ -641,48 +705,27 @@ __global__ void __launch_bounds__(64) sab_poly(SabPolyArgs a) {
       for (int i = 0; i < P::et; ++i) o[i] = (uint8_t)(packed >> (8 * i));
     }
   }
-  __syncthreads();
+  wave_sync();
 
   // 5. emit
+  const uint32_t* ob = reinterpret_cast<const uint32_t*>(outb);
   if (MODE == kSabKeygen) {
     uint8_t* pk = a.out + (size_t)inst * P::pk;
     uint8_t* sk = a.sk + (size_t)inst * P::sk;
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(outb);
     for (int i = tid; i < P::vecp / 4; i += NT) {
       reinterpret_cast<uint32_t*>(pk)[i] = ob[i];
       reinterpret_cast<uint32_t*>(sk + P::vecq)[i] = ob[i];
     }
-    // sk[0:vecq] = POLVECq(s): 32 L groups of 8 coefficients -> 13 bytes; s is
-    // read back from Toeplitz copy 0 (s[q] at byte 256 - q)
-    #pragma unroll 1
-    for (int x = tid; x < 32 * L; x += NT) {
-      uint64_t lo = 0, hi = 0;
-#pragma unroll
-      for (int d = 0; d < 8; ++d) {
-        const int c = 8 * x + d;
-        const uint64_t v = (uint64_t)(int64_t)S.V[(c >> 8) * kVPoly + 256 - (c & 255)] & 0x1FFF;
-        const int bit = 13 * d;
-        if (bit < 64) lo |= v << bit;
-        if (bit + 13 > 64) hi |= bit >= 64 ? v << (bit - 64) : v >> (64 - bit);
-      }
-      uint8_t* o = sk + 13 * x;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
-#pragma unroll
-      for (int i = 0; i < 5; ++i) o[8 + i] = (uint8_t)(hi >> (8 * i));
-    }
   } else if (MODE == kSabEnc) {
     uint32_t* ct = reinterpret_cast<uint32_t*>(a.out + (size_t)inst * P::ct);
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(outb);
     for (int i = tid; i < P::ct / 4; i += NT) ct[i] = ob[i];
   } else {
-    const uint32_t* ct = reinterpret_cast<const uint32_t*>(a.ct_in + (size_t)inst * P::ct);
-    const uint32_t* ob = reinterpret_cast<const uint32_t*>(outb);
     uint32_t d = 0;
-    for (int i = tid; i < P::ct / 4; i += NT) d |= ct[i] ^ ob[i];
-    atomicOr(&S.diff, d != 0 ? 1u : 0u);
-    __syncthreads();
-    if (tid == 0) a.fail[inst] = S.diff;
+#pragma unroll
+    for (int r = 0; r < CPT; ++r)
+      if (tid + r * NT < CW) d |= cv[r] ^ ob[tid + r * NT];
+    const bool differ = __builtin_amdgcn_ballot_w64(d != 0) != 0;
+    if (tid == 0) a.fail[inst] = differ ? 1u : 0u;
   }
 }
 
