@@ -18,8 +18,11 @@ Reported beside it in the same JSON line:
                  the other parameter sets on the same step
   plain          the one-workgroup-per-KEM comparator form on the same inputs
   latency        p50 / p99 of the per-step wall time (ms)
-  roofline       dominant kernel of the headline workload, from HIP events on
-                 the engine's stream during the timed steps
+  roofline       dominant kernel of the headline step: the plan's own kernels
+                 launched without a graph (same launches, streams and
+                 kernels), each bracketed by HIP events on its stream;
+                 kernels_ms lists them all, eager_kernels_ms the op-by-op
+                 launches
   mixed          16,384 instances in random Frodo / Saber batches of U[1,4096],
                  ops 1:4:4, one autotuned graph vs plain per-batch launches
   serving        Poisson arrivals into the continuous-batching server
@@ -70,6 +73,35 @@ KERNEL_WORK = {
     "frodo_dec_mu": (0, 64 * 976),
     # Frodo-640 (SHAKE128 chains, rate 168 B) and Frodo-1344 (SHAKE256, 136 B):
     # A rows take 8 / 16 SHAKE128 blocks; noise streams 32n + 128 bytes
+    # pipelined keygen -> encaps -> decaps chains (the plan kernels the
+    # headline replays): stages of different ops sharing a launch
+    "sab_pipe_x0": (1 + 23 + 5 + 23 + 23, 0),          # keygen seedA, GenMatrix, GenSecret; encaps' / decaps' GenMatrix
+    "sab_pipe_x1": (8 + 1 + 1 + 5 + 8, 0),             # encaps' H(pk), m, G, GenSecret; keygen's H(pk)
+    "sab_pipe_x2": (10 + 9 + 1 + 5, 3 * 65536),        # encaps' K; decaps' H(ct), G, GenSecret; decaps' inner product
+    "sab_k": (1, 0),
+    "sab_pipe_x0,l2": (1 + 10 + 4 + 10 + 10, 0),
+    "sab_pipe_x1,l2": (5 + 1 + 1 + 4 + 5, 0),
+    "sab_pipe_x2,l2": (7 + 6 + 1 + 4, 2 * 65536),
+    "sab_k,l2": (1, 0),
+    "sab_pipe_x0,l4": (1 + 40 + 5 + 40 + 40, 0),
+    "sab_pipe_x1,l4": (10 + 1 + 1 + 5 + 10, 0),
+    "sab_pipe_x2,l4": (12 + 11 + 1 + 5, 4 * 65536),
+    "sab_k,l4": (1, 0),
+    "sab_poly<kSabKeygen>,l2": (0, 4 * 65536),
+    "sab_poly<kSabEnc>,l2": (0, 6 * 65536),
+    "sab_poly<kSabReenc>,l2": (0, 6 * 65536),
+    "sab_poly<kSabKeygen>,l4": (0, 16 * 65536),
+    "sab_poly<kSabEnc>,l4": (0, 20 * 65536),
+    "sab_poly<kSabReenc>,l4": (0, 20 * 65536),
+    "frodo_pipe_x1": (115 + 347, 0),                    # keygen's pkh beside encaps' chain
+    "frodo_pipe_x2": (116 + 232 + 115, 0),              # encaps' ss, decaps' chain, decaps' ct absorb
+    "frodo_pipe_ssd": (1, 0),
+    "frodo_pipe_x1,640": (58 + 182, 0),
+    "frodo_pipe_x2,640": (58 + 124 + 57, 0),
+    "frodo_pipe_ssd,640": (1, 0),
+    "frodo_pipe_x1,1344": (159 + 478, 0),
+    "frodo_pipe_x2,1344": (160 + 319 + 159, 0),
+    "frodo_pipe_ssd,1344": (1, 0),
     "frodo_keygen_pre,640": (1 + 122, 0),
     "frodo_as_fused,640": (640 * 8, 0),
     "frodo_keygen_post,640": (58, 0),
@@ -300,6 +332,43 @@ def run_shared_key(torch, eng, scheme, n, steps, seed):
             "ms_per_step": round(ms, 3), "roundtrip_ok": bool(torch.equal(ss, ss2))}
 
 
+def profile_plan(torch, eng, scheme, n, steps, seed):
+    """Per-kernel HIP-event times of the plan the headline replays (one
+    stream, one chunk, pipelined keygen -> encaps -> decaps chain), issued
+    without a graph (DPKEM_PLAN_NO_GRAPH) so that the engine's profiler
+    brackets every kernel with events on the stream it runs on."""
+    import numpy as np
+    import dpcrypto_amd as D
+    s = D.SIZES[scheme]
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    kc = torch.from_numpy(rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)).to(dev)
+    ec = torch.from_numpy(rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)).to(dev)
+    pk = torch.empty((n, s["pk"]), dtype=torch.uint8, device=dev)
+    sk = torch.empty((n, s["sk"]), dtype=torch.uint8, device=dev)
+    ct = torch.empty((n, s["ct"]), dtype=torch.uint8, device=dev)
+    ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
+    ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
+    eng.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    p = eng.plan(1, 1, graph=False)
+    t0 = p.keypair(scheme, pk, sk, kc)
+    t1 = p.enc(scheme, ct, ss, pk, ec, after=t0)
+    p.dec(scheme, ss2, ct, sk, after=t1)
+    p.build()
+    p.launch()
+    torch.cuda.synchronize()
+    eng.kernel_stats(reset=True)
+    eng.set_profiling(True)
+    for _ in range(steps):
+        p.launch()
+    torch.cuda.synchronize()
+    eng.set_profiling(False)
+    stats = eng.kernel_stats(reset=True)
+    ok = bool(torch.equal(ss, ss2))
+    p.close()
+    return dict(kernels=stats, roundtrip_ok=ok)
+
+
 def roofline_from(stats, n):
     """Dominant kernel by total time -> achieved VALU lane-ops/s vs peak."""
     if not stats:
@@ -414,7 +483,9 @@ def main():
     sab = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                             schedule=sched)
     progress(f"saber: {sab['ops_per_s']:.0f} ops/s with schedule {sab['schedule']['nstreams']}x{sab['schedule']['chunks']}")
-    progress("saber eager pass with per-kernel HIP events")
+    progress("saber: the headline plan's kernels launched without a graph, with per-kernel HIP events")
+    sab_prof = profile_plan(torch, eng, D.SABER, args.saber_batch, args.steps, 0x44504B454D000001)
+    progress("saber eager pass (op by op) with per-kernel HIP events")
     sab_eager = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, 1, 0x44504B454D000001, True)
     result = {
         "metric": "KEM ops/s (keygen+enc+dec) Frodo-976 & Saber at 1 GPU; p99 latency",
@@ -433,17 +504,19 @@ def main():
                    "global_batch": args.saber_batch, "parallelism": "instances (1 GPU)"},
         "latency_ms": {"p50": round(pct(sab["per_step_ms"], 0.5), 4), "p99": round(pct(sab["per_step_ms"], 0.99), 4),
                        "what": "per-step (batch) latency"},
-        "roundtrip_ok": sab["roundtrip_ok"] and sab_eager["roundtrip_ok"],
+        "roundtrip_ok": sab["roundtrip_ok"] and sab_prof["roundtrip_ok"] and sab_eager["roundtrip_ok"],
         "schedule": sab["schedule"],
         "eager_ops_per_s": round(sab_eager["ops_per_s"], 1),
-        "roofline": roofline_from(sab_eager["kernels"], args.saber_batch),
-        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
+        "roofline": roofline_from(sab_prof["kernels"], args.saber_batch),
+        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_prof["kernels"].items()},
+        "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
     }
     if not args.skip_frodo:
         progress(f"frodo976 batch {args.frodo_batch}: autotune, 1 warmup + {args.frodo_steps} timed graph steps")
         fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002,
                                schedule=sched)
-        progress(f"frodo976: {fr['ops_per_s']:.0f} ops/s; eager pass with per-kernel HIP events")
+        progress(f"frodo976: {fr['ops_per_s']:.0f} ops/s; plan kernels and an eager pass with per-kernel HIP events")
+        fr_prof = profile_plan(torch, eng, D.FRODO976, args.frodo_batch, 1, 0x44504B454D000002)
         fr_eager = run_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002, True)
         result["frodo976"] = {
             "workload": f"frodokem-976-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
@@ -451,11 +524,12 @@ def main():
             "ms_per_step": round(fr["wall_s"] * 1e3 / args.frodo_steps, 3),
             "latency_ms": {"p50": round(pct(fr["per_step_ms"], 0.5), 3), "p99": round(pct(fr["per_step_ms"], 0.99), 3)},
             "steps": args.frodo_steps,
-            "roundtrip_ok": fr["roundtrip_ok"] and fr_eager["roundtrip_ok"],
+            "roundtrip_ok": fr["roundtrip_ok"] and fr_prof["roundtrip_ok"] and fr_eager["roundtrip_ok"],
             "schedule": fr["schedule"],
             "eager_ops_per_s": round(fr_eager["ops_per_s"], 1),
-            "roofline": roofline_from(fr_eager["kernels"], args.frodo_batch),
-            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_eager["kernels"].items()},
+            "roofline": roofline_from(fr_prof["kernels"], args.frodo_batch),
+            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_prof["kernels"].items()},
+            "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_eager["kernels"].items()},
         }
     if not args.skip_plain:
         progress("plain comparator forms")
@@ -487,23 +561,26 @@ def main():
             progress(f"{name} batch {args.frodo_batch}: autotuned graph steps, then an eager pass with HIP events")
             fs = run_plan_workload(torch, eng, sc, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000010 + sc,
                                    tune_budget_s=15.0, schedule=sched)
-            fs_eager = run_workload(torch, eng, sc, args.frodo_batch, 1, 0, 0x44504B454D000010 + sc, True)
+            fs_prof = profile_plan(torch, eng, sc, args.frodo_batch, 1, 0x44504B454D000010 + sc)
             result[name] = {
                 "workload": f"{name}-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
                 "ops_per_s": round(fs["ops_per_s"], 1), "ms_per_step": round(fs["wall_s"] * 1e3 / args.frodo_steps, 3),
-                "roundtrip_ok": fs["roundtrip_ok"] and fs_eager["roundtrip_ok"],
+                "roundtrip_ok": fs["roundtrip_ok"] and fs_prof["roundtrip_ok"],
                 "schedule": {"nstreams": fs["schedule"]["nstreams"], "chunks": fs["schedule"]["chunks"]},
-                "roofline": roofline_from(fs_eager["kernels"], args.frodo_batch),
-                "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fs_eager["kernels"].items()}}
+                "roofline": roofline_from(fs_prof["kernels"], args.frodo_batch),
+                "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fs_prof["kernels"].items()}}
     for name, sc in (("lightsaber", D.LIGHTSABER), ("firesaber", D.FIRESABER)):
         progress(f"{name} batch {args.saber_batch}: autotuned graph steps")
         fam = run_plan_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup, 0x44504B454D000008 + sc,
                                 tune_budget_s=20.0, schedule=sched)
+        fam_prof = profile_plan(torch, eng, sc, args.saber_batch, 3, 0x44504B454D000008 + sc)
         result[name] = {"workload": f"{name} batch {args.saber_batch}: keygen+encaps+decaps per step",
                         "ops_per_s": round(fam["ops_per_s"], 1),
                         "ms_per_step": round(fam["wall_s"] * 1e3 / args.steps, 4),
-                        "roundtrip_ok": fam["roundtrip_ok"],
-                        "schedule": {"nstreams": fam["schedule"]["nstreams"], "chunks": fam["schedule"]["chunks"]}}
+                        "roundtrip_ok": fam["roundtrip_ok"] and fam_prof["roundtrip_ok"],
+                        "schedule": {"nstreams": fam["schedule"]["nstreams"], "chunks": fam["schedule"]["chunks"]},
+                        "roofline": roofline_from(fam_prof["kernels"], args.saber_batch),
+                        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fam_prof["kernels"].items()}}
     progress("saber with the chain run as three dependent tasks (no stage pipelining)")
     unpiped = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                 tune_budget_s=20.0, schedule=sched, pipeline=False)
@@ -512,7 +589,7 @@ def main():
         "ms_per_step": round(unpiped["wall_s"] * 1e3 / args.steps, 4),
         "schedule": {"nstreams": unpiped["schedule"]["nstreams"], "chunks": unpiped["schedule"]["chunks"]},
         "note": "keygen, encaps and decaps as three dependent plan tasks (ten launches per chunk); the headline "
-                "runs the same work with stages of different ops sharing launches (eight per chunk)"}
+                "runs the same work with stages of different ops sharing launches (seven per chunk)"}
     progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
     shared = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                tune_budget_s=20.0, share_matrix=True, schedule=sched)

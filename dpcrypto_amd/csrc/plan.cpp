@@ -59,11 +59,13 @@ struct dpkem_plan {
   size_t scratch_bytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool built = false;  // dpkem_plan_build succeeded (graph instantiated, or DPKEM_PLAN_NO_GRAPH)
 };
 
 namespace {
 
 void release(dpkem_plan* p) {
+  p->built = false;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->exec = nullptr;
@@ -197,92 +199,12 @@ void schedule(dpkem_plan* p) {
   p->scratch_bytes = off;
 }
 
-}  // namespace
-
-extern "C" {
-
-int dpkem_plan_create(dpkem_engine* e, dpkem_plan** out, int nstreams, int chunks) {
-  if (!e || !out || nstreams < 1 || nstreams > 16 || chunks < 1 || chunks > 64) return DPKEM_EINVAL;
-  dpkem_plan* p = new (std::nothrow) dpkem_plan();
-  if (!p) return DPKEM_ENOMEM;
-  p->e = e;
-  p->nstreams = nstreams;
-  p->chunks = chunks;
-  p->form = e->form;
-  *out = p;
-  return DPKEM_OK;
-}
-
-int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
-                   const uint8_t* i1, int after, int* task_id) {
-  if (!p || p->exec || scheme < DPKEM_FRODO976 || scheme > DPKEM_FRODO1344 ||
-      (scheme >= DPKEM_FRODO976_AES && p->form != DPKEM_FORM_BATCHED) || op < 0 || op > 2 || n == 0 ||
-      n > (size_t)INT32_MAX)
-    return DPKEM_EINVAL;
-  if (after >= (int)p->tasks.size() || after < -1) return DPKEM_EINVAL;
-  size_t out[2], in[2];
-  io_sizes(scheme, op, out, in);
-  uint8_t* outs[2] = {o0, o1};
-  const uint8_t* ins[2] = {i0, i1};
-  for (int k = 0; k < 2; ++k) {
-    if ((out[k] && (!outs[k] || ((uintptr_t)outs[k] & 7))) || (in[k] && (!ins[k] || ((uintptr_t)ins[k] & 7))))
-      return DPKEM_EINVAL;
-  }
-  p->tasks.push_back({scheme, op, n, o0, o1, i0, i1, after});
-  if (task_id) *task_id = (int)p->tasks.size() - 1;
-  return DPKEM_OK;
-}
-
-int dpkem_plan_set_options(dpkem_plan* p, unsigned flags) {
-  if (!p || p->exec || (flags & ~(DPKEM_PLAN_SHARE_MATRIX | DPKEM_PLAN_NO_PIPELINE))) return DPKEM_EINVAL;
-  p->flags = flags;
-  return DPKEM_OK;
-}
-
-int dpkem_plan_build(dpkem_plan* p) {
-  if (!p || p->tasks.empty()) return DPKEM_EINVAL;
-  dpkem_engine* e = p->e;
-  std::lock_guard<std::mutex> lock(e->mu);
-  release(p);
-  schedule(p);
-  if (p->scratch_bytes > e->budget) {
-    snprintf(e->last_err, sizeof e->last_err, "plan scratch %zu B exceeds HBM budget %zu B", p->scratch_bytes, e->budget);
-    return DPKEM_EBUDGET;
-  }
-  hipError_t err;
-#define PLAN_TRY(call)                                              \
-  do {                                                              \
-    err = (call);                                                   \
-    if (err != hipSuccess) {                                        \
-      int rc = record_hip_error(e, err, #call);                     \
-      release(p);                                                   \
-      return rc;                                                    \
-    }                                                               \
-  } while (0)
-  PLAN_TRY(hipSetDevice(e->device));
-  PLAN_TRY(hipMalloc(&p->scratch, std::max<size_t>(p->scratch_bytes, 256)));
-  p->streams.resize(p->nstreams, nullptr);
-  for (auto& st : p->streams) PLAN_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  PLAN_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
-  p->join.resize(p->nstreams, nullptr);
-  for (auto& ev : p->join) PLAN_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  for (auto& s : p->subs) PLAN_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-
-  // capture: origin -> fork -> per-stream sub-task chains -> join -> origin
-  hipStream_t origin = p->streams[0];
-  hipStream_t cap;
-  PLAN_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-  err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-  if (err != hipSuccess) {
-    (void)hipStreamDestroy(cap);
-    int rc = record_hip_error(e, err, "hipStreamBeginCapture");
-    release(p);
-    return rc;
-  }
-  bool ok = hipEventRecord(p->fork, cap) == hipSuccess;
+// Issue every sub-task: origin -> fork -> per-stream sub-task chains ->
+// join -> origin.  Recorded once into the graph, or run on each launch of a
+// DPKEM_PLAN_NO_GRAPH plan.
+bool issue(dpkem_plan* p, hipStream_t origin) {
+  bool ok = hipEventRecord(p->fork, origin) == hipSuccess;
   for (auto st : p->streams) ok = ok && hipStreamWaitEvent(st, p->fork, 0) == hipSuccess;
-  Profiler* saved = tls_profiler;
-  tls_profiler = nullptr;  // no timing events inside the graph
   for (auto& s : p->subs) {
     const Task& T = p->tasks[s.task];
     hipStream_t st = p->streams[s.stream];
@@ -321,14 +243,105 @@ int dpkem_plan_build(dpkem_plan* p) {
     ok = ok && hipGetLastError() == hipSuccess;
     ok = ok && hipEventRecord(s.done, st) == hipSuccess;
   }
-  tls_profiler = saved;
   for (int k = 0; k < p->nstreams; ++k) {
     ok = ok && hipEventRecord(p->join[k], p->streams[k]) == hipSuccess;
-    ok = ok && hipStreamWaitEvent(cap, p->join[k], 0) == hipSuccess;
+    ok = ok && hipStreamWaitEvent(origin, p->join[k], 0) == hipSuccess;
   }
+  return ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpkem_plan_create(dpkem_engine* e, dpkem_plan** out, int nstreams, int chunks) {
+  if (!e || !out || nstreams < 1 || nstreams > 16 || chunks < 1 || chunks > 64) return DPKEM_EINVAL;
+  dpkem_plan* p = new (std::nothrow) dpkem_plan();
+  if (!p) return DPKEM_ENOMEM;
+  p->e = e;
+  p->nstreams = nstreams;
+  p->chunks = chunks;
+  p->form = e->form;
+  *out = p;
+  return DPKEM_OK;
+}
+
+int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+                   const uint8_t* i1, int after, int* task_id) {
+  if (!p || p->built || scheme < DPKEM_FRODO976 || scheme > DPKEM_FRODO1344 ||
+      (scheme >= DPKEM_FRODO976_AES && p->form != DPKEM_FORM_BATCHED) || op < 0 || op > 2 || n == 0 ||
+      n > (size_t)INT32_MAX)
+    return DPKEM_EINVAL;
+  if (after >= (int)p->tasks.size() || after < -1) return DPKEM_EINVAL;
+  size_t out[2], in[2];
+  io_sizes(scheme, op, out, in);
+  uint8_t* outs[2] = {o0, o1};
+  const uint8_t* ins[2] = {i0, i1};
+  for (int k = 0; k < 2; ++k) {
+    if ((out[k] && (!outs[k] || ((uintptr_t)outs[k] & 7))) || (in[k] && (!ins[k] || ((uintptr_t)ins[k] & 7))))
+      return DPKEM_EINVAL;
+  }
+  p->tasks.push_back({scheme, op, n, o0, o1, i0, i1, after});
+  if (task_id) *task_id = (int)p->tasks.size() - 1;
+  return DPKEM_OK;
+}
+
+int dpkem_plan_set_options(dpkem_plan* p, unsigned flags) {
+  if (!p || p->built || (flags & ~(DPKEM_PLAN_SHARE_MATRIX | DPKEM_PLAN_NO_PIPELINE | DPKEM_PLAN_NO_GRAPH)))
+    return DPKEM_EINVAL;
+  p->flags = flags;
+  return DPKEM_OK;
+}
+
+int dpkem_plan_build(dpkem_plan* p) {
+  if (!p || p->tasks.empty()) return DPKEM_EINVAL;
+  dpkem_engine* e = p->e;
+  std::lock_guard<std::mutex> lock(e->mu);
+  release(p);
+  schedule(p);
+  if (p->scratch_bytes > e->budget) {
+    snprintf(e->last_err, sizeof e->last_err, "plan scratch %zu B exceeds HBM budget %zu B", p->scratch_bytes, e->budget);
+    return DPKEM_EBUDGET;
+  }
+  hipError_t err;
+#define PLAN_TRY(call)                                              \
+  do {                                                              \
+    err = (call);                                                   \
+    if (err != hipSuccess) {                                        \
+      int rc = record_hip_error(e, err, #call);                     \
+      release(p);                                                   \
+      return rc;                                                    \
+    }                                                               \
+  } while (0)
+  PLAN_TRY(hipSetDevice(e->device));
+  PLAN_TRY(hipMalloc(&p->scratch, std::max<size_t>(p->scratch_bytes, 256)));
+  p->streams.resize(p->nstreams, nullptr);
+  for (auto& st : p->streams) PLAN_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  PLAN_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+  p->join.resize(p->nstreams, nullptr);
+  for (auto& ev : p->join) PLAN_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  for (auto& s : p->subs) PLAN_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+
+  if (p->flags & DPKEM_PLAN_NO_GRAPH) {  // launched directly by dpkem_plan_launch
+    p->built = true;
+    return DPKEM_OK;
+  }
+  // capture: origin -> fork -> per-stream sub-task chains -> join -> origin
+  hipStream_t cap;
+  PLAN_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) {
+    (void)hipStreamDestroy(cap);
+    int rc = record_hip_error(e, err, "hipStreamBeginCapture");
+    release(p);
+    return rc;
+  }
+  Profiler* saved = tls_profiler;
+  tls_profiler = nullptr;  // no timing events inside the graph
+  bool ok = issue(p, cap);
+  tls_profiler = saved;
   err = hipStreamEndCapture(cap, &p->graph);
   (void)hipStreamDestroy(cap);
-  (void)origin;
   if (err != hipSuccess || !ok) {
     int rc = record_hip_error(e, err != hipSuccess ? err : hipErrorUnknown, "graph capture");
     release(p);
@@ -336,15 +349,25 @@ int dpkem_plan_build(dpkem_plan* p) {
   }
   PLAN_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
 #undef PLAN_TRY
+  p->built = true;
   return DPKEM_OK;
 }
 
 int dpkem_plan_launch(dpkem_plan* p) {
-  if (!p || !p->exec) return DPKEM_EINVAL;
+  if (!p || !p->built) return DPKEM_EINVAL;
   dpkem_engine* e = p->e;
   std::lock_guard<std::mutex> lock(e->mu);
   hipError_t err = hipSetDevice(e->device);
-  if (err == hipSuccess) err = hipGraphLaunch(p->exec, e->stream);
+  if (err != hipSuccess) return record_hip_error(e, err, "hipSetDevice");
+  if (p->flags & DPKEM_PLAN_NO_GRAPH) {
+    // the same launches, events and streams as the graph, issued now (the
+    // engine's profiler, when on, times each kernel)
+    tls_profiler = e->profiling ? &e->prof : nullptr;
+    const bool ok = issue(p, e->stream);
+    tls_profiler = nullptr;
+    return ok ? DPKEM_OK : record_hip_error(e, hipGetLastError(), "plan launch");
+  }
+  err = hipGraphLaunch(p->exec, e->stream);
   return err == hipSuccess ? DPKEM_OK : record_hip_error(e, err, "hipGraphLaunch");
 }
 

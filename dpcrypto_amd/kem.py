@@ -272,8 +272,9 @@ class Engine:
                     "dec_shared_key")
         return ss
 
-    def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False, pipeline: bool = True) -> "Plan":
-        return Plan(self, nstreams, chunks, share_matrix, pipeline)
+    def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False, pipeline: bool = True,
+             graph: bool = True) -> "Plan":
+        return Plan(self, nstreams, chunks, share_matrix, pipeline, graph)
 
     # raw pointer entry points (device arrays preallocated by the caller)
     def keypair_into(self, scheme, n, pk, sk, coins):
@@ -289,20 +290,22 @@ class Engine:
 OP_KEYGEN, OP_ENC, OP_DEC = 0, 1, 2
 PLAN_SHARE_MATRIX = 1
 PLAN_NO_PIPELINE = 2
+PLAN_NO_GRAPH = 4
 
 
 class Plan:
     """A graph-captured set of KEM tasks over device arrays (dpkem_plan_*)."""
 
     def __init__(self, engine: Engine, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False,
-                 pipeline: bool = True):
+                 pipeline: bool = True, graph: bool = True):
         self.eng = engine
         self.lib = engine.lib
         h = ctypes.c_void_p()
         engine._check(self.lib.dpkem_plan_create(engine.h, ctypes.byref(h), nstreams, chunks), "plan_create")
         self.h = h
         self._keep = []  # arrays referenced by the graph stay alive with the plan
-        flags = (PLAN_SHARE_MATRIX if share_matrix else 0) | (0 if pipeline else PLAN_NO_PIPELINE)
+        flags = ((PLAN_SHARE_MATRIX if share_matrix else 0) | (0 if pipeline else PLAN_NO_PIPELINE)
+                 | (0 if graph else PLAN_NO_GRAPH))
         if flags:
             engine._check(self.lib.dpkem_plan_set_options(self.h, flags), "plan_set_options")
 

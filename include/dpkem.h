@@ -158,6 +158,11 @@ int dpkem_plan_add(dpkem_plan *p, int scheme, int op, size_t n, uint8_t *o0, uin
  * results are identical.  DPKEM_PLAN_NO_PIPELINE runs such chains as three
  * dependent tasks instead. */
 #define DPKEM_PLAN_NO_PIPELINE 2u
+/* DPKEM_PLAN_NO_GRAPH: dpkem_plan_launch issues the plan's launches directly
+ * (same streams, events and kernels as the graph) instead of replaying a
+ * captured graph, so that the engine's profiler (dpkem_engine_set_profiling)
+ * can time each kernel of a plan.  Slower to launch; for measurement. */
+#define DPKEM_PLAN_NO_GRAPH 4u
 int dpkem_plan_set_options(dpkem_plan *p, unsigned flags);
 int dpkem_plan_build(dpkem_plan *p);
 int dpkem_plan_launch(dpkem_plan *p);

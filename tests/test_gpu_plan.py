@@ -44,6 +44,18 @@ FRODO_AES, LIGHTSABER, FIRESABER = 2, 3, 4
     (FRODO, 37, 2, 1, False, True), (FRODO_AES, 6, 2, 2, False, True)])
 def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share,
                                                      pipe):
+    _chain_replays(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share, pipe, True)
+
+
+@pytest.mark.parametrize("scheme,n,nstreams,chunks", [(SABER, 97, 2, 3), (LIGHTSABER, 33, 1, 1), (FRODO, 10, 3, 3)])
+def test_plan_without_graph_matches_oracle(eng, oracle, torch_cuda, scheme, n, nstreams, chunks):
+    """DPKEM_PLAN_NO_GRAPH issues the same launches directly on every
+    dpkem_plan_launch (used to time a plan's kernels); the engine's profiler
+    then sees every kernel of the pipelined chain."""
+    _chain_replays(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, False, True, False)
+
+
+def _chain_replays(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share, pipe, graph):
     import dpcrypto_amd as D
     torch = torch_cuda
     s = D.SIZES[scheme]
@@ -58,8 +70,8 @@ def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, sc
     ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
     ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
     eng.set_stream(torch.cuda.current_stream().cuda_stream)
-    print(f"plan scheme={scheme} n={n} {nstreams}x{chunks} share={share} pipe={pipe}", flush=True)
-    plan = eng.plan(nstreams=nstreams, chunks=chunks, share_matrix=share, pipeline=pipe)
+    print(f"plan scheme={scheme} n={n} {nstreams}x{chunks} share={share} pipe={pipe} graph={graph}", flush=True)
+    plan = eng.plan(nstreams=nstreams, chunks=chunks, share_matrix=share, pipeline=pipe, graph=graph)
     t0 = plan.keypair(scheme, pk, sk, dkc)
     t1 = plan.enc(scheme, ct, ss, pk, dec_, after=t0)
     plan.dec(scheme, ss2, ct, sk, after=t1)
@@ -73,8 +85,15 @@ def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, sc
             t.zero_()
         scratch = torch.randn(1 << 20, device="cuda").sum()  # unrelated eager work
         torch.cuda.synchronize()
+        if not graph:
+            eng.set_profiling(rep == 1)
         plan.launch()
         torch.cuda.synchronize()
+        if not graph and rep == 1:
+            eng.set_profiling(False)
+            names = set(eng.kernel_stats(reset=True))
+            expect = "sab_pipe_x0" if scheme != FRODO else "frodo_pipe_x1"
+            assert any(k.startswith(expect) for k in names), names
         assert float(scratch) == float(scratch)
         assert np.array_equal(pk.cpu().numpy(), opk), f"pk replay {rep}"
         assert np.array_equal(sk.cpu().numpy(), osk), f"sk replay {rep}"
