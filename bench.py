@@ -130,63 +130,64 @@ KERNEL_WORK = {
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
 # FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129322559, 256245760),
-    ('frodo_as_fused,1344', 8192): (177553276, 353391904),
-    ('frodo_as_fused,640', 8192): (85272040, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129306583, 256247434),
-    ('frodo_dec_mu', 8192): (262124678, 262144),
-    ('frodo_dec_mu,1344', 8192): (385332908, 262144),
-    ('frodo_dec_mu,640', 8192): (223157593, 262144),
-    ('frodo_encdec_pre<false>', 8192): (97555128, 128712784),
-    ('frodo_encdec_pre<false>,1344', 8192): (177928108, 176947200),
-    ('frodo_encdec_pre<false>,640', 8192): (80523712, 84672512),
-    ('frodo_encdec_pre<true>', 8192): (1607151, 128712704),
-    ('frodo_encdec_pre<true>,1344', 8192): (1818906, 176947200),
-    ('frodo_encdec_pre<true>,640', 8192): (1911566, 84672512),
-    ('frodo_keygen_post', 8192): (129115204, 262144),
-    ('frodo_keygen_post,1344', 8192): (177318009, 393216),
-    ('frodo_keygen_post,640', 8192): (79816858, 262144),
-    ('frodo_keygen_pre', 8192): (1021353, 257076081),
-    ('frodo_keygen_pre,1344', 8192): (1017087, 353243496),
-    ('frodo_keygen_pre,640', 8192): (1081175, 168558592),
-    ('frodo_sa_fused<false,aes>', 8192): (259483755, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260583349, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356853499, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (167093477, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388313172, 262144),
-    ('frodo_sa_fused<true>', 8192): (389542840, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (534057315, 266192),
-    ('frodo_sa_fused<true>,640', 8192): (247706075, 262168),
-    ('frodo_ss', 8192): (130060727, 196608),
-    ('frodo_ss,1344', 8192): (178453594, 262144),
-    ('frodo_ss,640', 8192): (81833243, 131072),
-    ('sab_dec_hash1', 4096): (900075, 18874279),
-    ('sab_dec_inner', 4096): (9888560, 131072),
-    ('sab_dec_inner,l2', 4096): (7136378, 131072),
-    ('sab_dec_inner,l4', 4096): (13417824, 131072),
-    ('sab_enc_hash1', 4096): (4580858, 19046557),
-    ('sab_hash2', 4096): (5202977, 131072),
-    ('sab_hash2,l2', 4096): (4004358, 131072),
-    ('sab_hash2,l4', 4096): (7015561, 131072),
-    ('sab_keygen_hash1', 4096): (633207, 18999686),
-    ('sab_keygen_hash1,l2', 4096): (635172, 9830401),
-    ('sab_keygen_hash1,l4', 4096): (634276, 30801938),
-    ('sab_keygen_hash2', 4096): (4205036, 131072),
-    ('sab_pipe_x1', 4096): (5303722, 34420947),
-    ('sab_pipe_x1,l2', 4096): (4031039, 16777216),
-    ('sab_pipe_x1,l4', 4096): (6644002, 58196840),
-    ('sab_pipe_x2', 4096): (5694768, 3538944),
-    ('sab_pipe_x2,l2', 4096): (4266468, 3014656),
-    ('sab_pipe_x2,l4', 4096): (7290982, 3538944),
-    ('sab_poly<kSabEnc>', 4096): (23889973, 4456448),
-    ('sab_poly<kSabEnc>,l2', 4096): (13120902, 3014656),
-    ('sab_poly<kSabEnc>,l4', 4096): (36761050, 6029312),
-    ('sab_poly<kSabKeygen>', 4096): (19024540, 12976128),
-    ('sab_poly<kSabKeygen>,l2', 4096): (9559879, 8650752),
-    ('sab_poly<kSabKeygen>,l4', 4096): (30591206, 17301509),
-    ('sab_poly<kSabReenc>', 4096): (29006337, 131373),
-    ('sab_poly<kSabReenc>,l2', 4096): (16532438, 131134),
-    ('sab_poly<kSabReenc>,l4', 4096): (43056831, 131212),
+    ('frodo_as_fused', 8192): (129323084, 256245760),
+    ('frodo_as_fused,1344', 8192): (177550961, 353392262),
+    ('frodo_as_fused,640', 8192): (85287668, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129297695, 256247280),
+    ('frodo_dec_mu', 8192): (268446691, 262144),
+    ('frodo_dec_mu,1344', 8192): (385432641, 262144),
+    ('frodo_dec_mu,640', 8192): (223005418, 262144),
+    ('frodo_encdec_pre<false>', 8192): (74558215, 128712704),
+    ('frodo_encdec_pre<true>', 8192): (1445709, 128712704),
+    ('frodo_keygen_post', 8192): (129114550, 262144),
+    ('frodo_keygen_pre', 8192): (1024524, 257072725),
+    ('frodo_keygen_pre,1344', 8192): (1024615, 353252531),
+    ('frodo_keygen_pre,640', 8192): (1079905, 168558592),
+    ('frodo_pipe_ssd', 8192): (4398560, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4516306, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5513559, 131072),
+    ('frodo_pipe_x1', 8192): (130155093, 128975260),
+    ('frodo_pipe_x1,1344', 8192): (178031133, 177340454),
+    ('frodo_pipe_x1,640', 8192): (80640971, 84934656),
+    ('frodo_pipe_x2', 8192): (131478674, 130557420),
+    ('frodo_pipe_x2,1344', 8192): (179721215, 178848755),
+    ('frodo_pipe_x2,640', 8192): (83246039, 86442131),
+    ('frodo_sa_fused<false,aes>', 8192): (259484401, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260583817, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356872738, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (167105839, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388305068, 262144),
+    ('frodo_sa_fused<true>', 8192): (389544994, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (534057614, 265766),
+    ('frodo_sa_fused<true>,640', 8192): (247706803, 262227),
+    ('frodo_ss', 8192): (129981771, 196608),
+    ('sab_dec_hash1', 4096): (899984, 18876066),
+    ('sab_dec_inner', 4096): (9888591, 131072),
+    ('sab_enc_hash1', 4096): (4585958, 19046982),
+    ('sab_hash2', 4096): (5166465, 131072),
+    ('sab_k', 4096): (1016706, 131072),
+    ('sab_k,l2', 4096): (1007491, 131072),
+    ('sab_k,l4', 4096): (1016706, 131072),
+    ('sab_keygen_hash1', 4096): (623051, 18999648),
+    ('sab_keygen_hash2', 4096): (4209144, 131072),
+    ('sab_pipe_x0', 4096): (1838620, 49609125),
+    ('sab_pipe_x0,l2', 4096): (1837317, 23486498),
+    ('sab_pipe_x0,l4', 4096): (1836084, 85362705),
+    ('sab_pipe_x1', 4096): (4645407, 3670016),
+    ('sab_pipe_x1,l2', 4096): (3383902, 3145728),
+    ('sab_pipe_x1,l4', 4096): (5981469, 3670016),
+    ('sab_pipe_x2', 4096): (11084455, 3801088),
+    ('sab_pipe_x2,l2', 4096): (7958569, 3276800),
+    ('sab_pipe_x2,l4', 4096): (15341820, 4718592),
+    ('sab_poly<kSabEnc>', 4096): (23847957, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13088175, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (38802404, 8126464),
+    ('sab_poly<kSabKeygen>', 4096): (18993915, 12978659),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9543525, 8650817),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30533084, 17327415),
+    ('sab_poly<kSabReenc>', 4096): (28979933, 131072),
+    ('sab_poly<kSabReenc>,l2', 4096): (16501770, 131072),
+    ('sab_poly<kSabReenc>,l4', 4096): (48247059, 5111808),
 }
 
 
