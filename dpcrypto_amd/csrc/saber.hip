@@ -171,17 +171,25 @@ __device__ __forceinline__ void enc_hash1_body(int vb, int role, int n, const ui
 // Decaps' message chain for instance inst: kr = G(m' || H(pk)) (H(pk) from
 // sk), then the GenSecret stream from kr[32:64].  m points at the instance's
 // 32 bytes of m' (global memory or LDS).
+// hpk: the instance's H(pk) words from sk (sk[sk - 64 .. sk - 32)), loaded by
+// the caller.
 template <int L>
-__device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint8_t* __restrict__ sk, const uint8_t* m,
+__device__ __forceinline__ void load_hpk(int inst, const uint8_t* __restrict__ sk, uint64_t hpk[4]) {
+  using P = SaberP<L>;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) hpk[i] = ld64(sk + (size_t)inst * P::sk + P::sk - 64 + 8 * i);
+}
+
+template <int L>
+__device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint64_t hpk[4], const uint8_t* m,
                                               uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
   using P = SaberP<L>;
-  const uint8_t* s = sk + (size_t)inst * P::sk;
   uint32_t a[25];
   uint64_t buf[8];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    buf[i] = ld64(m + 8 * i);                   // m'
-    buf[4 + i] = ld64(s + P::sk - 64 + 8 * i);  // H(pk) from sk
+    buf[i] = ld64(m + 8 * i);  // m'
+    buf[4 + i] = hpk[i];       // H(pk) from sk
   }
   absorb2_regs<kRateSha3_512, 8>(a, p, buf, kDsSha3);
   uint32_t kr[8];
@@ -209,7 +217,9 @@ __device__ __forceinline__ void dec_hash1_body(int vb, int role, int n, const ui
     squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
         a, p, reinterpret_cast<uint32_t*>(araw + (size_t)inst * P::matrix_bytes));
   } else {
-    dec_msg_chain<L>(inst, p, sk, mbuf + (size_t)inst * 32, sraw, krbuf);
+    uint64_t hpk[4];
+    load_hpk<L>(inst, sk, hpk);
+    dec_msg_chain<L>(inst, p, hpk, mbuf + (size_t)inst * 32, sraw, krbuf);
   }
 }
 
@@ -235,17 +245,18 @@ __device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restr
   using P = SaberP<L>;
   SAB_PAIR_SETUP
   uint32_t a[25], m[8];
+  uint64_t k[4], z[4];  // loaded before H(ct) so that their latency is off the chain
+  const uint64_t mask = fail ? 0 - (uint64_t)(fail[inst] != 0) : 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    k[i] = ld64(krbuf + (size_t)inst * 64 + 8 * i);
+    z[i] = sk ? ld64(sk + (size_t)inst * P::sk + P::sk - 32 + 8 * i) : 0;
+  }
   absorb2_global<kRateSha3_256, P::ct / 8>(a, p, ct + (size_t)inst * P::ct, kDsSha3);
 #pragma unroll
   for (int i = 0; i < 4; ++i) m[4 + i] = a[i];
-  uint64_t mask = 0;
-  if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint64_t k = ld64(krbuf + (size_t)inst * 64 + 8 * i);
-    const uint64_t z = sk ? ld64(sk + (size_t)inst * P::sk + P::sk - 32 + 8 * i) : 0;
-    m[i] = k2::deint((k & ~mask) | (z & mask), p);
-  }
+  for (int i = 0; i < 4; ++i) m[i] = k2::deint((k[i] & ~mask) | (z[i] & mask), p);
   absorb2_halves<kRateSha3_256, 8>(a, p, m, kDsSha3);
   store2_words<4>(a, p, ss + (size_t)inst * kSaberSs);
 }
@@ -326,10 +337,10 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 // Pipelined keygen -> encaps -> decaps chain (saber_pipe_launch).  Stages of
 // different ops that read disjoint inputs share a launch:
 //   x0  four waves per 32 instances: wave 0 computes keygen's seedA and
-//       stores it into pk and sk; after one workgroup barrier (which makes
-//       those bytes visible to the workgroup) wave 0 runs keygen's
-//       GenMatrix, wave 1 keygen's GenSecret, wave 2 encaps' GenMatrix from
-//       the seedA in pk and wave 3 decaps' GenMatrix from the seedA in sk;
+//       stores it into pk and sk; after one workgroup barrier wave 0 runs
+//       keygen's GenMatrix, wave 1 keygen's GenSecret, wave 2 encaps'
+//       GenMatrix from the seedA bytes of pk and wave 3 decaps' GenMatrix
+//       from those of sk (the same bytes, handed over through LDS);
 //   x1  wave 0: encaps' H(pk), then (after one workgroup barrier) G and
 //       GenSecret; wave 1: keygen's H(pk) into sk; wave 2: encaps' m, handed
 //       to wave 0 through LDS;
@@ -343,6 +354,7 @@ __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restr
                                                    uint8_t* __restrict__ sraw_k, uint8_t* __restrict__ araw_e,
                                                    uint8_t* __restrict__ araw_d) {
   using P = SaberP<L>;
+  __shared__ uint32_t seed_h[4][64];  // wave 0's halves of seedA, by lane
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, vb = blockIdx.x;
   const uint32_t p = lane & 1;
   const int inst = vb * 32 + (lane >> 1);
@@ -355,23 +367,31 @@ __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restr
     for (int i = 0; i < 4; ++i) m[i] = ld64(c + 8 * i);
     absorb2_regs<kRateShake128, 4>(a, p, m, kDsShake);  // seedA = SHAKE128(seedA_raw, 32)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) h[i] = a[i];
+    for (int i = 0; i < 4; ++i) {
+      h[i] = a[i];
+      seed_h[i][lane] = a[i];
+    }
     store2_words<4>(h, p, pk + (size_t)inst * P::pk + P::vecp);
     store2_words<4>(h, p, sk + (size_t)inst * P::sk + P::vecq + P::vecp);
   }
   __syncthreads();
   if (!live) return;
-  if (wave == 0) {
-    absorb2_halves<kRateShake128, 4>(a, p, h, kDsShake);  // keygen's GenMatrix = SHAKE128(seedA)
-    squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
-        a, p, reinterpret_cast<uint32_t*>(araw_k + (size_t)inst * P::matrix_bytes));
-  } else if (wave == 1) {
+  if (wave == 1) {
     keygen_hash1_body<L>(vb, 1, n, kc, pk, sk, nullptr, sraw_k);
-  } else if (wave == 2) {
-    enc_hash1_body<L>(vb, 0, n, pk, nullptr, araw_e, nullptr, nullptr, nullptr);
-  } else {
-    dec_hash1_body<L>(vb, 0, n, sk, nullptr, araw_d, nullptr, nullptr);
+    return;
   }
+  // GenMatrix = SHAKE128(seedA): keygen's (wave 0), encaps' from the seedA
+  // bytes of pk (wave 2) and decaps' from those of sk (wave 3); the bytes
+  // wave 0 just stored are handed over through LDS, so the two waves need no
+  // global round trip before their first permutation
+  if (wave != 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[i] = seed_h[i][lane];
+  }
+  absorb2_halves<kRateShake128, 4>(a, p, h, kDsShake);
+  uint8_t* dst = wave == 0 ? araw_k : wave == 2 ? araw_e : araw_d;
+  squeeze2_sliced<kRateShake128, P::matrix_bytes / 8>(
+      a, p, reinterpret_cast<uint32_t*>(dst + (size_t)inst * P::matrix_bytes));
 }
 
 template <int L>
@@ -892,6 +912,8 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
   }
   const int iw = (wave >> 2) * 2 + (wave & 1);  // inner wave 0..7
   const int base = blockIdx.x * kX2Group;
+  uint64_t hpk[4];  // the chain wave's H(pk) words, loaded before the inner products
+  if (wave == kX2ChainWave) load_hpk<L>(min(base + (lane >> 1), n - 1), sk, hpk);
   {
     const int k = 2 * iw + (lane >> 5), t = lane & 31, inst = base + k;
     const bool live = inst < n;
@@ -915,7 +937,7 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
   const uint32_t p = lane & 1;
   const int k = lane >> 1;
   if (k >= kX2Group || base + k >= n) return;
-  dec_msg_chain<L>(base + k, p, sk, S.mq[k], sraw_d, krbuf_d);
+  dec_msg_chain<L>(base + k, p, hpk, S.mq[k], sraw_d, krbuf_d);
 }
 
 // ----------------------------------------------------------------- host launchers
