@@ -501,21 +501,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Little-endian bytes [off, off + 16 - (off & 3)) of a 4-byte aligned LDS
-// buffer as a 128-bit value shifted down to byte off: four aligned dword
-// reads instead of byte reads.
-__device__ __forceinline__ void lds_bytes128(const uint8_t* base, int off, uint64_t* lo, uint64_t* hi) {
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(base + (off & ~3));
-  const uint32_t d0 = w[0], d1 = w[1], d2 = w[2], d3 = w[3];
-  const int sh = 8 * (off & 3);
-  uint64_t l = ((uint64_t)d1 << 32) | d0, h = ((uint64_t)d3 << 32) | d2;
-  if (sh) {
-    l = (l >> sh) | (h << (64 - sh));
-    h >>= sh;
-  }
-  *lo = l;
-  *hi = h;
-}
 // Up to 5 bytes at off (two aligned dword reads).
 __device__ __forceinline__ uint64_t lds_bytes64(const uint8_t* base, int off) {
   const uint32_t* w = reinterpret_cast<const uint32_t*>(base + (off & ~3));
@@ -594,47 +579,49 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) sa
     {
       const int g = tid;  // coefficients 4g .. 4g + 3 from mu / 2 bytes
       const uint64_t v = lds_bytes64(reinterpret_cast<const uint8_t*>(S.u.k.sstage), g * (P::mu / 2));
+      uint32_t pv = 0, nv = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t x = (uint32_t)(v >> (P::mu * k)) & ((1u << P::mu) - 1);
         const int half = P::mu / 2;
-        toep8_put(S.u.k.V, 4 * g + k, (int)popc(x & ((1u << half) - 1)) - (int)popc(x >> half));
+        const int c = (int)popc(x & ((1u << half) - 1)) - (int)popc(x >> half);
+        pv |= ((uint32_t)c & 0xFF) << (8 * (3 - k));
+        nv |= ((uint32_t)(-c) & 0xFF) << (8 * (3 - k));
       }
+      toep8_put_wave(S.u.k.V, g, pv, nv);
     }
     const uint8_t* mb = reinterpret_cast<const uint8_t*>(S.u.k.mstage);
 #pragma unroll
     for (int r = 0; r < (32 * L + NT - 1) / NT; ++r) {
       const int g = tid + r * NT;
       if (g < 32 * L) {
-        uint64_t lo13, hi13;
-        lds_bytes128(mb, 13 * g, &lo13, &hi13);
-        uint64_t lo = 0, hi = 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int bit = 13 * i;
-          const uint64_t x = bit < 64 ? (lo13 >> bit) | (bit > 51 ? (hi13 << (64 - bit)) : 0) : (hi13 >> (bit - 64));
-          int8_t l8, h8;
-          limbs8((uint32_t)(x & 0x1FFF), &l8, &h8);
-          lo |= (uint64_t)(uint8_t)l8 << (8 * i);
-          hi |= (uint64_t)(uint8_t)h8 << (8 * i);
-        }
+        // bytes 13 g .. 13 g + 12 as four dwords w0..w3 (little-endian), then
+        // the eight 13-bit fields at bits 13 i
+        const int off = 13 * g;
+        const uint32_t* d = reinterpret_cast<const uint32_t*>(mb + (off & ~3));
+        const uint32_t d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], sh = off & 3;
+        const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, sh), w1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
+        const uint32_t w2 = __builtin_amdgcn_alignbyte(d3, d2, sh), w3 = __builtin_amdgcn_alignbyte(0u, d3, sh);
+        const uint32_t x0 = w0 & 0x1FFF, x1 = (w0 >> 13) & 0x1FFF;
+        const uint32_t x2 = __builtin_amdgcn_alignbit(w1, w0, 26) & 0x1FFF, x3 = (w1 >> 7) & 0x1FFF;
+        const uint32_t x4 = __builtin_amdgcn_alignbit(w2, w1, 20) & 0x1FFF, x5 = (w2 >> 1) & 0x1FFF;
+        const uint32_t x6 = (w2 >> 14) & 0x1FFF, x7 = __builtin_amdgcn_alignbit(w3, w2, 27) & 0x1FFF;
+        uint32_t lo0, hi0, lo1, hi1;
+        limbs8x4(x0 | (x1 << 16), x2 | (x3 << 16), &lo0, &hi0);
+        limbs8x4(x4 | (x5 << 16), x6 | (x7 << 16), &lo1, &hi1);
         const int col = g >> 5;
         int8_t* blk = S.u.k.Lb + 8 * (g & 31);
-        *reinterpret_cast<uint64_t*>(blk + col * kLbSlot) = lo;
-        *reinterpret_cast<uint64_t*>(blk + (L + 1 + col) * kLbSlot) = hi;
+        *reinterpret_cast<uint2*>(blk + col * kLbSlot) = make_uint2(lo0, lo1);
+        *reinterpret_cast<uint2*>(blk + (L + 1 + col) * kLbSlot) = make_uint2(hi0, hi1);
       }
     }
     if (HAS_B) {
-      const int g = tid;  // b_j coefficients 4g .. 4g + 3
+      const int g = tid;  // b_j coefficients 4g .. 4g + 3 (10-bit fields of 5 bytes)
       const uint64_t x = lds_bytes64(reinterpret_cast<const uint8_t*>(S.u.k.bstage), 5 * g);
-      uint32_t lo = 0, hi = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int8_t l8, h8;
-        limbs8((uint32_t)(x >> (10 * i)) & 0x3FF, &l8, &h8);
-        lo |= (uint32_t)(uint8_t)l8 << (8 * i);
-        hi |= (uint32_t)(uint8_t)h8 << (8 * i);
-      }
+      const uint32_t w0 = (uint32_t)x, w1 = (uint32_t)(x >> 32);
+      uint32_t lo, hi;
+      limbs8x4((w0 & 0x3FF) | (((w0 >> 10) & 0x3FF) << 16),
+               ((w0 >> 20) & 0x3FF) | ((__builtin_amdgcn_alignbit(w1, w0, 30) & 0x3FF) << 16), &lo, &hi);
       int8_t* blk = S.u.k.Lb + 4 * g;
       *reinterpret_cast<uint32_t*>(blk + L * kLbSlot) = lo;
       *reinterpret_cast<uint32_t*>(blk + (2 * L + 1) * kLbSlot) = hi;

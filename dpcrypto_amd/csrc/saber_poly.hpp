@@ -112,6 +112,45 @@ __host__ __device__ __forceinline__ void toep8_put(int8_t* V, int q, int v) {
   }
 }
 
+#if defined(__HIP__) || defined(__HIPCC__)
+// The same four copies written by a whole wave at once: lane g holds
+// coefficients 4g .. 4g + 3 of the polynomial, packed as v (byte k =
+// s[4g + 3 - k], the ascending-address order of the copies) and nv (the
+// negated bytes).  In copy c the aligned dwords at bytes 256 - 4g and
+// 512 - 4g hold s[4g + c - i] (negated in the second) in byte i, i.e. bytes
+// of this lane and of lane g - 1: one v_alignbyte each, with lane g - 1's
+// values from a one-lane wave rotation (lane 0 takes lane 63's negated
+// values: the wrap of the negacyclic sequence).  Lane 63 also fills the
+// dword at byte 0 of each copy.  Bytes at 512 + c .. 515 + c of copy c
+// (never read) receive junk.
+__device__ __forceinline__ void toep8_put_wave(int8_t* V, int g, uint32_t v, uint32_t nv) {
+  const uint32_t z = g == 63 ? nv : v;
+  const uint32_t zm = (uint32_t)__builtin_amdgcn_mov_dpp((int)z, 0x13C, 0xF, 0xF, false);   // wave_ror:1
+  const uint32_t nm = (uint32_t)__builtin_amdgcn_mov_dpp((int)nv, 0x13C, 0xF, 0xF, false);  // wave_ror:1
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(V + c * kVCopy);
+    w[64 - g] = __builtin_amdgcn_alignbyte(zm, v, 3 - c);
+    w[128 - g] = __builtin_amdgcn_alignbyte(nm, nv, 3 - c);
+    if (g == 63) w[0] = __builtin_amdgcn_alignbyte(v, v, 3 - c);
+  }
+}
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Signed limbs of four coefficients packed in pairs (u0 = x0 | x1 << 16,
+// u1 = x2 | x3 << 16, each x < 2^13): the low limb of x is its low byte read
+// as int8 (x - 256 hi), the high limb hi = (x + 128) >> 8; lo / hi return the
+// four bytes in coefficient order.
+__device__ __forceinline__ void limbs8x4(uint32_t u0, uint32_t u1, uint32_t* lo, uint32_t* hi) {
+  const u16x2 r = {128, 128}, e = {8, 8};
+  const uint32_t h0 = __builtin_bit_cast(uint32_t, (__builtin_bit_cast(u16x2, u0) + r) >> e);
+  const uint32_t h1 = __builtin_bit_cast(uint32_t, (__builtin_bit_cast(u16x2, u1) + r) >> e);
+  *lo = __builtin_amdgcn_perm(u1, u0, 0x06040200u);  // bytes 0, 2 of u0, then of u1
+  *hi = __builtin_amdgcn_perm(h1, h0, 0x06040200u);
+}
+#endif
+
 // Byte offset (within the staged polynomial) of the aligned 16-byte run
 // V[e0 .. e0 + 15].
 __host__ __device__ __forceinline__ int toep8_run(int e0) {
