@@ -797,64 +797,106 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
   mbuf[(size_t)inst * 32 + t] = (uint8_t)byte;
 }
 
-// Decaps' inner product v = b'^T s of one instance on 32 lanes (t = lane in
-// the half-wave) and its message byte t: m'_t = ((v + h2 - c_m 2^(10 - eT))
-// >> 9) & 1 bit by bit, as in sab_dec_inner, but one K-block at a time
-// through a single Toeplitz buffer Ts (kToepStride u16) and b' row Bp (kARow
-// u16) so that sixteen instances fit one workgroup's LDS.  The 32 lanes of an
-// instance belong to one wave; LDS ordering between them needs no workgroup
-// barrier.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
+// Decaps' inner product v = b'^T s of one instance on a whole wave, on the
+// int8 MFMA Toeplitz tiles of the products (saber_poly.hpp): per K-block j,
+// T(s_j) (s from sk, 13-bit two's complement, small) is the staged factor and
+// b'_j (from ct, 10-bit) the B operand, its low limbs in column 0 and high
+// limbs in column 8; then m'_t = ((v_t + h2 - c_m,t 2^(10 - eT)) >> 9) & 1,
+// as in sab_dec_inner.  One K-block at a time through 3.5 KB of LDS per wave.
+// All accesses are by the one wave, so LDS ordering needs no workgroup
+// barrier (wave_sync).
+struct InnerLds {
+  union {
+    struct {
+      alignas(16) int8_t V[kVPoly];        // Toeplitz copies of s_j
+      alignas(16) int8_t Lb[2 * kLbSlot];  // b'_j limbs: slot 0 low, slot 1 high
+      alignas(16) uint32_t sb[104 + 4];    // s_j bytes (416, 13-bit) + slack for the last window
+      alignas(16) uint32_t bb[80 + 4];     // b'_j bytes (320, 10-bit) + slack
+    } k;
+    alignas(16) int32_t part[2][256];      // column sums: low, high limbs
+  } u;
+};
 
 template <int L>
-__device__ __forceinline__ uint32_t dec_inner_half(int t, const uint8_t* __restrict__ c, const uint8_t* __restrict__ s,
-                                                   uint16_t* Ts, uint16_t* Bp) {
+__device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const uint8_t* __restrict__ c,
+                                                   const uint8_t* __restrict__ s, InnerLds& W) {
   using P = SaberP<L>;
-  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int t0 = 8 * t;
+  v4i8x4 acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = v4i8x4{0, 0, 0, 0};
+  const int slot = toep8_slot(lane & 15, 0, true);
 #pragma unroll 1
   for (int j = 0; j < L; ++j) {
-    uint8_t bb[2][5], sb[13];
+    const uint32_t* sg = reinterpret_cast<const uint32_t*>(s + 416 * j);
+    const uint32_t* cg = reinterpret_cast<const uint32_t*>(c + 320 * j);
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int i = 0; i < 5; ++i) bb[h][i] = c[320 * j + 5 * (t + 32 * h) + i];  // b'_j, 10-bit groups of 4
-#pragma unroll
-    for (int i = 0; i < 13; ++i) sb[i] = s[416 * j + 13 * t + i];  // s_j, 13-bit groups of 8
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      uint16_t v[4];
-      unpack10_group(v, bb[h]);
-      uint2 q;
-      q.x = v[0] | ((uint32_t)v[1] << 16);
-      q.y = v[2] | ((uint32_t)v[3] << 16);
-      *reinterpret_cast<uint2*>(Bp + 4 * (t + 32 * h)) = q;
+    for (int r = 0; r < 2; ++r) {
+      const int i = lane + 64 * r;
+      if (i < 104) W.u.k.sb[i] = sg[i];
+      if (i < 80) W.u.k.bb[i] = cg[i];
     }
-    uint16_t v[8];
-    unpack13_group(v, sb);
+    wave_sync();
+    {
+      // s_j coefficients 4g .. 4g + 3 (g = lane): the 13-bit fields at bit 52 g
+      const int bit = 52 * lane, off = bit >> 3;
+      const uint32_t* d = reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(W.u.k.sb) + (off & ~3));
+      const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
+      const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, off & 3), w1 = __builtin_amdgcn_alignbyte(d2, d1, off & 3);
+      const uint32_t sh = bit & 7;  // 0 or 4
+      const uint32_t x[4] = {(w0 >> sh) & 0x1FFF, (w0 >> (sh + 13)) & 0x1FFF,
+                             __builtin_amdgcn_alignbit(w1, w0, sh + 26) & 0x1FFF, (w1 >> (sh + 7)) & 0x1FFF};
+      uint32_t pv = 0, nv = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) toep_put(Ts, 0, 8 * t + i, v[i]);
-    wave_lds_sync();
-    toep_dot(acc, Ts, 0, Bp, t0);
-    wave_lds_sync();  // the next K-block overwrites Ts / Bp
+      for (int k = 0; k < 4; ++k) {
+        const int v = (int)(x[k] << 19) >> 19;  // 13-bit two's complement
+        pv |= ((uint32_t)v & 0xFF) << (8 * (3 - k));
+        nv |= ((uint32_t)(-v) & 0xFF) << (8 * (3 - k));
+      }
+      toep8_put_wave(W.u.k.V, lane, pv, nv);
+      // b'_j coefficients 4g .. 4g + 3: 10-bit fields of 5 bytes
+      const uint64_t y = lds_bytes64(reinterpret_cast<const uint8_t*>(W.u.k.bb), 5 * lane);
+      const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
+      uint32_t lo, hi;
+      limbs8x4((y0 & 0x3FF) | (((y0 >> 10) & 0x3FF) << 16),
+               ((y0 >> 20) & 0x3FF) | ((__builtin_amdgcn_alignbit(y1, y0, 30) & 0x3FF) << 16), &lo, &hi);
+      *reinterpret_cast<uint32_t*>(W.u.k.Lb + 4 * lane) = lo;
+      *reinterpret_cast<uint32_t*>(W.u.k.Lb + kLbSlot + 4 * lane) = hi;
+    }
+    wave_sync();
+    v4i8x4 bop[4];
+    const int8_t* colp = W.u.k.Lb + (slot < 0 ? 0 : slot) * kLbSlot + 16 * (lane >> 4);
+#pragma unroll
+    for (int bq = 0; bq < 4; ++bq)
+      bop[bq] = slot >= 0 ? *reinterpret_cast<const v4i8x4*>(colp + 64 * bq) : v4i8x4{0, 0, 0, 0};
+    toep8_block(acc, W.u.k.V, bop, lane, [](v4i8x4 x, v4i8x4 y, v4i8x4 z) {
+      return __builtin_amdgcn_mfma_i32_16x16x64_i8(x, y, z, 0, 0, 0);
+    });
+    wave_sync();
   }
-  uint64_t cm = 0;  // eT bits for each of t0..t0+7
+  if (slot >= 0) {
 #pragma unroll
-  for (int i = 0; i < P::et; ++i) cm |= (uint64_t)c[P::vecp + P::et * t + i] << (8 * i);
+    for (int t = 0; t < 16; ++t)
+      *reinterpret_cast<int4*>(&W.u.part[slot][16 * t + 4 * (lane >> 4)]) =
+          make_int4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
+  }
+  wave_sync();
   uint32_t byte = 0;
+  if (lane < 32) {
+    uint64_t cm = 0;  // eT bits for each of 8 lane .. 8 lane + 7
 #pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    const uint32_t cmd = (uint32_t)(cm >> (P::et * d)) & ((1u << P::et) - 1);
-    byte |= (((acc[d] + P::h2 - (cmd << (10 - P::et))) >> 9) & 1u) << d;
+    for (int i = 0; i < P::et; ++i) cm |= (uint64_t)c[P::vecp + P::et * lane + i] << (8 * i);
+#pragma unroll
+    for (int dd = 0; dd < 8; ++dd) {
+      const uint32_t v = (uint32_t)W.u.part[0][8 * lane + dd] + 256u * (uint32_t)W.u.part[1][8 * lane + dd];
+      const uint32_t cmd = (uint32_t)(cm >> (P::et * dd)) & ((1u << P::et) - 1);
+      byte |= (((v + P::h2 - (cmd << (10 - P::et))) >> 9) & 1u) << dd;
+    }
   }
+  wave_sync();  // the next instance overwrites the sums
   return byte;
 }
 
-// x2 of the pipelined chain: one workgroup of sixteen waves per 16
+// x2 of the pipelined chain: one workgroup of eight waves per 16
 // instances.  A workgroup's waves go to the CU's four SIMDs in a repeating
 // order of period four (measured on MI355X: waves w and w + 4 always share a
 // SIMD), so the roles are laid out by w mod 4 and the two long hash chains
@@ -862,21 +904,25 @@ __device__ __forceinline__ uint32_t dec_inner_half(int t, const uint8_t* __restr
 //   w = 0          encaps' K = SHA3-256(kr[0:32] || H(ct));
 //   w = 1          decaps' H(ct) into hc_d (sab_k finishes K after the
 //                  re-encryption);
-//   w mod 4 = 2, 3 (eight waves) decaps' inner products, two instances per
-//                  wave; m' goes to LDS and to mbuf_d (the re-encryption
-//                  reads it).  Wave kX2ChainWave then waits until all eight
-//                  have posted (an LDS counter, not a workgroup barrier, so
-//                  the hash waves never wait) and runs decaps' message chain
-//                  (G, GenSecret) for the 16 instances on lane pairs;
-//   w = 4, 5, 8, 9, 12, 13 leave at once (they only keep the layout).
+//   w = 2, 3, 6, 7 decaps' inner products on int8 MFMA, four instances per
+//                  wave one after the other; m' goes to LDS and to mbuf_d
+//                  (the re-encryption reads it).  Wave kX2ChainWave then
+//                  waits until all four have posted (an LDS counter, not a
+//                  workgroup barrier, so the hash waves never wait) and runs
+//                  decaps' message chain (G, GenSecret) for the 16
+//                  instances on lane pairs;
+//   w = 4, 5       leave at once (they only keep the layout).
+// Eight waves (two per SIMD) leave each wave up to 256 registers: the MFMA
+// accumulators of the inner products and the hash chains' prefetched blocks
+// fit without spills.
 constexpr int kX2Group = 16;
-constexpr int kX2InnerWaves = kX2Group / 2;
+constexpr int kX2InnerWaves = 4;
+constexpr int kX2PerInnerWave = kX2Group / kX2InnerWaves;
 constexpr int kX2ChainWave = 2;
-constexpr int kX2Threads = 1024;
+constexpr int kX2Threads = 512;
 
 struct X2Lds {
-  alignas(16) uint16_t Ts[kX2Group][kToepStride];
-  alignas(16) uint16_t Bp[kX2Group][kARow];
+  InnerLds w[kX2InnerWaves];
   alignas(8) uint8_t mq[kX2Group][32];
   uint32_t posted;
 };
@@ -897,22 +943,23 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
     else if (wave == 1) hct_body<L, kX2Group>(blockIdx.x, n, ct, hc_d);
     return;
   }
-  const int iw = (wave >> 2) * 2 + (wave & 1);  // inner wave 0..7
+  const int iw = (wave >> 2) * 2 + (wave & 1);  // inner wave 0..3
   const int base = blockIdx.x * kX2Group;
   uint64_t hpk[4];  // the chain wave's H(pk) words, loaded before the inner products
   if (wave == kX2ChainWave) load_hpk<L>(min(base + (lane >> 1), n - 1), sk, hpk);
-  {
-    const int k = 2 * iw + (lane >> 5), t = lane & 31, inst = base + k;
+#pragma unroll 1
+  for (int h = 0; h < kX2PerInnerWave; ++h) {
+    const int k = kX2PerInnerWave * iw + h, inst = base + k;
     const bool live = inst < n;
     const int ci = live ? inst : 0;
-    const uint32_t byte = dec_inner_half<L>(t, ct + (size_t)ci * P::ct, sk + (size_t)ci * P::sk, S.Ts[k], S.Bp[k]);
-    if (live) {
-      S.mq[k][t] = (uint8_t)byte;
-      mbuf_d[(size_t)inst * 32 + t] = (uint8_t)byte;
+    const uint32_t byte = dec_inner_mfma<L>(lane, ct + (size_t)ci * P::ct, sk + (size_t)ci * P::sk, S.w[iw]);
+    if (live && lane < 32) {
+      S.mq[k][lane] = (uint8_t)byte;
+      mbuf_d[(size_t)inst * 32 + lane] = (uint8_t)byte;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (lane == 0) __hip_atomic_fetch_add(&S.posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if (lane == 0) __hip_atomic_fetch_add(&S.posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   if (wave != kX2ChainWave) return;
   // Every wave of a workgroup is resident from its start and each inner wave
   // posts once after bounded work, so the count reaches kX2InnerWaves; the
