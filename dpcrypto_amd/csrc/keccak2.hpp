@@ -273,6 +273,77 @@ __device__ __forceinline__ void absorb2_global(uint32_t a[25], uint32_t p, const
   keccakf2(a, p);
 }
 
+// Message staging by a helper wave.  A lane-pair sponge spends ~22 VALU ops
+// per absorbed 64-bit word on deint, about one permutation's worth over an
+// eight-block message; a wave on another SIMD can do that conversion ahead
+// of it.  The helper writes word i of block b, half p of the sponge's
+// instance slot s (s = 2 k + p), to msg[(b RW + i) SLOTS + s] and after each
+// block raises the LDS count `ready`; the sponge waits for a block's count
+// before reading it.  Every wave of a workgroup is resident from its start
+// and the helper publishes each block once after bounded work, so the waits
+// end; their iteration bound only makes the loops' ends unconditional.
+__device__ __forceinline__ void lds_wait_count(const uint32_t* cnt, uint32_t target) {
+  for (int spin = 0; spin < (1 << 22); ++spin) {
+    if (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Helper side: the NW message words at w for slot s with half p; lane 0 of
+// the wave publishes.  Block b + 1's words are loaded while block b is
+// converted (uniform branches only, so the loads stay in flight).
+template <int RW, int NW, int SLOTS>
+__device__ __forceinline__ void stage2_message(uint32_t* msg, uint32_t* ready, uint32_t p, int s,
+                                               const uint64_t* __restrict__ w, bool lane0) {
+  constexpr int NB = NW / RW, REM = NW % RW;
+  uint64_t cur[RW];
+#pragma unroll
+  for (int i = 0; i < RW; ++i)
+    if (NB > 0 || i < REM) cur[i] = w[i];
+  for (int b = 0; b < NB; ++b) {
+    uint64_t nxt[RW];
+    if (b + 1 < NB) {
+#pragma unroll
+      for (int i = 0; i < RW; ++i) nxt[i] = w[(b + 1) * RW + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < REM; ++i) nxt[i] = w[NB * RW + i];
+    }
+#pragma unroll
+    for (int i = 0; i < RW; ++i) msg[(b * RW + i) * SLOTS + s] = deint(cur[i], p);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane0) __hip_atomic_fetch_add(ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+    for (int i = 0; i < RW; ++i) cur[i] = nxt[i];
+  }
+  if (REM > 0) {
+#pragma unroll
+    for (int i = 0; i < REM; ++i) msg[(NB * RW + i) * SLOTS + s] = deint(cur[i], p);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane0) __hip_atomic_fetch_add(ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
+// Sponge side: absorb2_global over the staged halves of slot s.
+template <int RW, int NW, int SLOTS>
+__device__ __forceinline__ void absorb2_staged(uint32_t a[25], uint32_t p, const uint32_t* msg, const uint32_t* ready,
+                                               int s, uint64_t ds) {
+  constexpr int NB = NW / RW, REM = NW % RW;
+  zero2(a);
+  for (int b = 0; b < NB; ++b) {
+    lds_wait_count(ready, (uint32_t)(b + 1));
+#pragma unroll
+    for (int i = 0; i < RW; ++i) a[i] ^= msg[(b * RW + i) * SLOTS + s];
+    keccakf2(a, p);
+  }
+  if (REM > 0) lds_wait_count(ready, (uint32_t)(NB + 1));
+#pragma unroll
+  for (int i = 0; i < REM; ++i) a[i] ^= msg[(NB * RW + i) * SLOTS + s];
+  a[REM] ^= half_const(ds, p);
+  a[RW - 1] ^= half_const(0x8000000000000000ULL, p);
+  keccakf2(a, p);
+}
+
 // Absorb a register-resident message of NW (< RW) words.
 template <int RW, int NW>
 __device__ __forceinline__ void absorb2_regs(uint32_t a[25], uint32_t p, const uint64_t* m, uint64_t ds) {

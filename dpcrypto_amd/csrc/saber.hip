@@ -238,10 +238,12 @@ __device__ __forceinline__ void keygen_hash2_body(int vb, int n, const uint8_t* 
 
 // K = SHA3-256(kr[0:32] || H(ct)); for decaps kr[0:32] is replaced by z when
 // the re-encryption differs (constant-time select on the mismatch flag).
-template <int L, int G = 32>
-__device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restrict__ ct,
-                                           const uint8_t* __restrict__ krbuf, const uint8_t* __restrict__ sk,
-                                           const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+// ABS absorbs H(ct) of instance inst into a (absorb2_global over ct, or the
+// LDS-staged halves in the pipelined chain).
+template <int L, int G, class ABS>
+__device__ __forceinline__ void hash2_core(int vb, int n, ABS&& absorb_ct, const uint8_t* __restrict__ krbuf,
+                                           const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
+                                           uint8_t* __restrict__ ss) {
   using P = SaberP<L>;
   SAB_PAIR_SETUP
   uint32_t a[25], m[8];
@@ -252,7 +254,7 @@ __device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restr
     k[i] = ld64(krbuf + (size_t)inst * 64 + 8 * i);
     z[i] = sk ? ld64(sk + (size_t)inst * P::sk + P::sk - 32 + 8 * i) : 0;
   }
-  absorb2_global<kRateSha3_256, P::ct / 8>(a, p, ct + (size_t)inst * P::ct, kDsSha3);
+  absorb_ct(a, p, inst, lane);
 #pragma unroll
   for (int i = 0; i < 4; ++i) m[4 + i] = a[i];
 #pragma unroll
@@ -261,15 +263,27 @@ __device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restr
   store2_words<4>(a, p, ss + (size_t)inst * kSaberSs);
 }
 
+template <int L, int G = 32>
+__device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restrict__ ct,
+                                           const uint8_t* __restrict__ krbuf, const uint8_t* __restrict__ sk,
+                                           const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+  using P = SaberP<L>;
+  hash2_core<L, G>(
+      vb, n,
+      [&](uint32_t* a, uint32_t p, int inst, int) {
+        absorb2_global<kRateSha3_256, P::ct / 8>(a, p, ct + (size_t)inst * P::ct, kDsSha3);
+      },
+      krbuf, sk, fail, ss);
+}
+
 // The two halves of K for the pipelined chain: H(ct) of the received
 // ciphertext (it does not depend on the re-encryption), kept as this lane's
 // interleaved halves (hc[8 inst + 2 w + p]), then K from it.
-template <int L, int G = 32>
-__device__ __forceinline__ void hct_body(int vb, int n, const uint8_t* __restrict__ ct, uint32_t* __restrict__ hc) {
-  using P = SaberP<L>;
+template <int L, int G, class ABS>
+__device__ __forceinline__ void hct_core(int vb, int n, ABS&& absorb_ct, uint32_t* __restrict__ hc) {
   SAB_PAIR_SETUP
   uint32_t a[25];
-  absorb2_global<kRateSha3_256, P::ct / 8>(a, p, ct + (size_t)inst * P::ct, kDsSha3);
+  absorb_ct(a, p, inst, lane);
 #pragma unroll
   for (int i = 0; i < 4; ++i) hc[(size_t)inst * 8 + 2 * i + p] = a[i];
 }
@@ -802,43 +816,66 @@ __global__ void __launch_bounds__(64) sab_dec_inner(int n, const uint8_t* __rest
 // T(s_j) (s from sk, 13-bit two's complement, small) is the staged factor and
 // b'_j (from ct, 10-bit) the B operand, its low limbs in column 0 and high
 // limbs in column 8; then m'_t = ((v_t + h2 - c_m,t 2^(10 - eT)) >> 9) & 1,
-// as in sab_dec_inner.  One K-block at a time through 3.5 KB of LDS per wave.
-// All accesses are by the one wave, so LDS ordering needs no workgroup
-// barrier (wave_sync).
+// as in sab_dec_inner.  The instance's raw bytes (s, b', c_m) arrive in
+// registers (InnerRaw, loaded one instance ahead by the caller) and pass
+// through LDS once; all accesses are by the one wave, so LDS ordering needs
+// no workgroup barrier (wave_sync).
+template <int L>
+struct InnerRaw {
+  static constexpr int SW = 104 * L, BW = 80 * L, CW = 8 * SaberP<L>::et;  // dwords of s, b', c_m
+  uint32_t s[(SW + 63) / 64], b[(BW + 63) / 64], c;
+};
+
+template <int L>
+__device__ __forceinline__ void load_inner_raw(int lane, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
+                                               InnerRaw<L>& r) {
+  using R = InnerRaw<L>;
+  const uint32_t* sg = reinterpret_cast<const uint32_t*>(sk);
+  const uint32_t* cg = reinterpret_cast<const uint32_t*>(ct);
+#pragma unroll
+  for (int i = 0; i < (R::SW + 63) / 64; ++i)
+    if (lane + 64 * i < R::SW) r.s[i] = sg[lane + 64 * i];
+#pragma unroll
+  for (int i = 0; i < (R::BW + 63) / 64; ++i)
+    if (lane + 64 * i < R::BW) r.b[i] = cg[lane + 64 * i];
+  if (lane < R::CW) r.c = cg[R::BW + lane];
+}
+
+template <int L>
 struct InnerLds {
   union {
     struct {
-      alignas(16) int8_t V[kVPoly];        // Toeplitz copies of s_j
-      alignas(16) int8_t Lb[2 * kLbSlot];  // b'_j limbs: slot 0 low, slot 1 high
-      alignas(16) uint32_t sb[104 + 4];    // s_j bytes (416, 13-bit) + slack for the last window
-      alignas(16) uint32_t bb[80 + 4];     // b'_j bytes (320, 10-bit) + slack
+      alignas(16) int8_t V[kVPoly];                       // Toeplitz copies of s_j
+      alignas(16) int8_t Lb[2 * kLbSlot];                 // b'_j limbs: slot 0 low, slot 1 high
+      alignas(16) uint32_t sb[InnerRaw<L>::SW + 4];       // s (13-bit), + slack for the last window
+      alignas(16) uint32_t bb[InnerRaw<L>::BW + 4];       // b' (10-bit) + slack
+      alignas(16) uint32_t cb[InnerRaw<L>::CW];           // c_m (eT-bit)
     } k;
-    alignas(16) int32_t part[2][256];      // column sums: low, high limbs
+    alignas(16) int32_t part[2][256];                     // column sums: low, high limbs
   } u;
 };
 
 template <int L>
-__device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const uint8_t* __restrict__ c,
-                                                   const uint8_t* __restrict__ s, InnerLds& W) {
+__device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const InnerRaw<L>& r, InnerLds<L>& W) {
   using P = SaberP<L>;
+  using R = InnerRaw<L>;
+#pragma unroll
+  for (int i = 0; i < (R::SW + 63) / 64; ++i)
+    if (lane + 64 * i < R::SW) W.u.k.sb[lane + 64 * i] = r.s[i];
+#pragma unroll
+  for (int i = 0; i < (R::BW + 63) / 64; ++i)
+    if (lane + 64 * i < R::BW) W.u.k.bb[lane + 64 * i] = r.b[i];
+  if (lane < R::CW) W.u.k.cb[lane] = r.c;
   v4i8x4 acc[16];
 #pragma unroll
   for (int t = 0; t < 16; ++t) acc[t] = v4i8x4{0, 0, 0, 0};
   const int slot = toep8_slot(lane & 15, 0, true);
+  wave_sync();
 #pragma unroll 1
   for (int j = 0; j < L; ++j) {
-    const uint32_t* sg = reinterpret_cast<const uint32_t*>(s + 416 * j);
-    const uint32_t* cg = reinterpret_cast<const uint32_t*>(c + 320 * j);
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int i = lane + 64 * r;
-      if (i < 104) W.u.k.sb[i] = sg[i];
-      if (i < 80) W.u.k.bb[i] = cg[i];
-    }
-    wave_sync();
     {
       // s_j coefficients 4g .. 4g + 3 (g = lane): the 13-bit fields at bit 52 g
-      const int bit = 52 * lane, off = bit >> 3;
+      const int bit = 52 * lane, off = 416 * j + (bit >> 3);
       const uint32_t* d = reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(W.u.k.sb) + (off & ~3));
       const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
       const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, off & 3), w1 = __builtin_amdgcn_alignbyte(d2, d1, off & 3);
@@ -854,7 +891,7 @@ __device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const uint8_t* __re
       }
       toep8_put_wave(W.u.k.V, lane, pv, nv);
       // b'_j coefficients 4g .. 4g + 3: 10-bit fields of 5 bytes
-      const uint64_t y = lds_bytes64(reinterpret_cast<const uint8_t*>(W.u.k.bb), 5 * lane);
+      const uint64_t y = lds_bytes64(reinterpret_cast<const uint8_t*>(W.u.k.bb), 320 * j + 5 * lane);
       const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
       uint32_t lo, hi;
       limbs8x4((y0 & 0x3FF) | (((y0 >> 10) & 0x3FF) << 16),
@@ -873,6 +910,13 @@ __device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const uint8_t* __re
     });
     wave_sync();
   }
+  uint64_t cm = 0;  // eT bits for each of 8 lane .. 8 lane + 7 (read before the sums overwrite cb)
+  if (lane < 32) {
+    const uint8_t* cbb = reinterpret_cast<const uint8_t*>(W.u.k.cb);
+#pragma unroll
+    for (int i = 0; i < P::et; ++i) cm |= (uint64_t)cbb[P::et * lane + i] << (8 * i);
+  }
+  wave_sync();
   if (slot >= 0) {
 #pragma unroll
     for (int t = 0; t < 16; ++t)
@@ -882,9 +926,6 @@ __device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const uint8_t* __re
   wave_sync();
   uint32_t byte = 0;
   if (lane < 32) {
-    uint64_t cm = 0;  // eT bits for each of 8 lane .. 8 lane + 7
-#pragma unroll
-    for (int i = 0; i < P::et; ++i) cm |= (uint64_t)c[P::vecp + P::et * lane + i] << (8 * i);
 #pragma unroll
     for (int dd = 0; dd < 8; ++dd) {
       const uint32_t v = (uint32_t)W.u.part[0][8 * lane + dd] + 256u * (uint32_t)W.u.part[1][8 * lane + dd];
@@ -896,7 +937,7 @@ __device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const uint8_t* __re
   return byte;
 }
 
-// x2 of the pipelined chain: one workgroup of eight waves per 16
+// x2 of the pipelined chain: one workgroup of twelve waves per 16
 // instances.  A workgroup's waves go to the CU's four SIMDs in a repeating
 // order of period four (measured on MI355X: waves w and w + 4 always share a
 // SIMD), so the roles are laid out by w mod 4 and the two long hash chains
@@ -904,27 +945,35 @@ __device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const uint8_t* __re
 //   w = 0          encaps' K = SHA3-256(kr[0:32] || H(ct));
 //   w = 1          decaps' H(ct) into hc_d (sab_k finishes K after the
 //                  re-encryption);
-//   w = 2, 3, 6, 7 decaps' inner products on int8 MFMA, four instances per
-//                  wave one after the other; m' goes to LDS and to mbuf_d
-//                  (the re-encryption reads it).  Wave kX2ChainWave then
-//                  waits until all four have posted (an LDS counter, not a
-//                  workgroup barrier, so the hash waves never wait) and runs
-//                  decaps' message chain (G, GenSecret) for the 16
-//                  instances on lane pairs;
-//   w = 4, 5       leave at once (they only keep the layout).
-// Eight waves (two per SIMD) leave each wave up to 256 registers: the MFMA
-// accumulators of the inner products and the hash chains' prefetched blocks
-// fit without spills.
+//   w = 3          deinterleaves both chains' copy of ct into LDS block by
+//                  block (stage2_message: lanes 0..31 for encaps' K, 32..63
+//                  for decaps' H(ct)), so the two chains absorb without
+//                  conversion work;
+//   w = 2, 6, 10, 7, 11  decaps' inner products on int8 MFMA, instances w',
+//                  w' + 5, ... (w' = 0 .. 4) one after the other (three waves
+//                  per SIMD hide the Toeplitz operand reads' latency); m'
+//                  goes to LDS and to mbuf_d (the re-encryption reads it).
+//                  Wave kX2ChainWave then waits until all five have posted
+//                  (an LDS counter, not a workgroup barrier, so the hash
+//                  waves never wait) and runs decaps' message chain (G,
+//                  GenSecret) for the 16 instances on lane pairs;
+//   w = 4, 5, 8, 9 leave at once (they only keep the layout).
+// Twelve waves (three per SIMD) leave each wave up to 168 registers: the
+// MFMA accumulators of the inner products and the hash chains' prefetched
+// blocks fit without spills.
 constexpr int kX2Group = 16;
-constexpr int kX2InnerWaves = 4;
-constexpr int kX2PerInnerWave = kX2Group / kX2InnerWaves;
+constexpr int kX2InnerWaves = 5;
 constexpr int kX2ChainWave = 2;
-constexpr int kX2Threads = 512;
+constexpr int kX2HelperWave = 3;
+constexpr int kX2Threads = 768;
 
+template <int L>
 struct X2Lds {
-  InnerLds w[kX2InnerWaves];
+  static constexpr int kCtBlocks = (SaberP<L>::ct / 8 + kRateSha3_256 - 1) / kRateSha3_256;
+  InnerLds<L> w[kX2InnerWaves];
+  uint32_t msg[2][kCtBlocks * kRateSha3_256 * 2 * kX2Group];  // ct halves for encaps' K / decaps' H(ct)
   alignas(8) uint8_t mq[kX2Group][32];
-  uint32_t posted;
+  uint32_t posted, ready;
 };
 
 template <int L>
@@ -934,25 +983,47 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
                                                           const uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ ss,
                                                           uint32_t* __restrict__ hc_d) {
   using P = SaberP<L>;
-  __shared__ X2Lds S;
+  constexpr int CW = P::ct / 8, SL = 2 * kX2Group;
+  __shared__ X2Lds<L> S;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (threadIdx.x == 0) S.posted = 0;
+  const int base = blockIdx.x * kX2Group;
+  if (threadIdx.x == 0) {
+    S.posted = 0;
+    S.ready = 0;
+  }
   __syncthreads();  // the only workgroup barrier: every wave passes it before any work
   if ((wave & 3) < 2) {
-    if (wave == 0) hash2_body<L, kX2Group>(blockIdx.x, n, ct, krbuf_e, nullptr, nullptr, ss);
-    else if (wave == 1) hct_body<L, kX2Group>(blockIdx.x, n, ct, hc_d);
+    auto staged = [&](uint32_t* a, uint32_t p, int, int ln) {
+      k2::absorb2_staged<kRateSha3_256, CW, SL>(a, p, S.msg[wave], &S.ready, ln, kDsSha3);
+    };
+    if (wave == 0) hash2_core<L, kX2Group>(blockIdx.x, n, staged, krbuf_e, nullptr, nullptr, ss);
+    else if (wave == 1) hct_core<L, kX2Group>(blockIdx.x, n, staged, hc_d);
     return;
   }
-  const int iw = (wave >> 2) * 2 + (wave & 1);  // inner wave 0..3
-  const int base = blockIdx.x * kX2Group;
+  if (wave == kX2HelperWave) {
+    const int k = (lane & 31) >> 1, inst = min(base + k, n - 1);  // absent instances stage a copy (unused)
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(ct + (size_t)inst * P::ct);
+    k2::stage2_message<kRateSha3_256, CW, SL>(S.msg[lane >> 5], &S.ready, lane & 1, lane & 31, w, lane == 0);
+    return;
+  }
+  const int iw = wave == 2 ? 0 : wave == 6 ? 1 : wave == 10 ? 2 : wave == 7 ? 3 : 4;  // inner waves 2, 6, 10, 7, 11
   uint64_t hpk[4];  // the chain wave's H(pk) words, loaded before the inner products
   if (wave == kX2ChainWave) load_hpk<L>(min(base + (lane >> 1), n - 1), sk, hpk);
+  InnerRaw<L> raw;  // the next instance's bytes, loaded while the current one is multiplied
+  {
+    const int ci = min(base + iw, n - 1);
+    load_inner_raw<L>(lane, ct + (size_t)ci * P::ct, sk + (size_t)ci * P::sk, raw);
+  }
 #pragma unroll 1
-  for (int h = 0; h < kX2PerInnerWave; ++h) {
-    const int k = kX2PerInnerWave * iw + h, inst = base + k;
+  for (int k = iw; k < kX2Group; k += kX2InnerWaves) {
+    const int inst = base + k;
     const bool live = inst < n;
-    const int ci = live ? inst : 0;
-    const uint32_t byte = dec_inner_mfma<L>(lane, ct + (size_t)ci * P::ct, sk + (size_t)ci * P::sk, S.w[iw]);
+    const InnerRaw<L> cur = raw;
+    if (k + kX2InnerWaves < kX2Group) {
+      const int ci = min(inst + kX2InnerWaves, n - 1);
+      load_inner_raw<L>(lane, ct + (size_t)ci * P::ct, sk + (size_t)ci * P::sk, raw);
+    }
+    const uint32_t byte = dec_inner_mfma<L>(lane, cur, S.w[iw]);
     if (live && lane < 32) {
       S.mq[k][lane] = (uint8_t)byte;
       mbuf_d[(size_t)inst * 32 + lane] = (uint8_t)byte;
@@ -961,13 +1032,8 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   if (lane == 0) __hip_atomic_fetch_add(&S.posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   if (wave != kX2ChainWave) return;
-  // Every wave of a workgroup is resident from its start and each inner wave
-  // posts once after bounded work, so the count reaches kX2InnerWaves; the
-  // iteration bound only makes the loop's end unconditional.
-  for (int spin = 0; spin < (1 << 22); ++spin) {
-    if (__hip_atomic_load(&S.posted, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= (uint32_t)kX2InnerWaves) break;
-    __builtin_amdgcn_s_sleep(1);
-  }
+  // each inner wave posts once after bounded work (lds_wait_count)
+  k2::lds_wait_count(&S.posted, (uint32_t)kX2InnerWaves);
   const uint32_t p = lane & 1;
   const int k = lane >> 1;
   if (k >= kX2Group || base + k >= n) return;
