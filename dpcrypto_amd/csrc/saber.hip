@@ -355,9 +355,9 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 //       keygen's GenMatrix, wave 1 keygen's GenSecret, wave 2 encaps'
 //       GenMatrix from the seedA bytes of pk and wave 3 decaps' GenMatrix
 //       from those of sk (the same bytes, handed over through LDS);
-//   x1  wave 0: encaps' H(pk), then (after one workgroup barrier) G and
-//       GenSecret; wave 1: keygen's H(pk) into sk; wave 2: encaps' m, handed
-//       to wave 0 through LDS;
+//   x1  wave 0: encaps' H(pk), then G and GenSecret; wave 1: keygen's H(pk) into sk; wave 2: encaps' m, handed
+//       to wave 0 through LDS; wave 3 deinterleaves pk into LDS ahead of
+//       wave 0 (stage2_message);
 //   x2  (after the encaps products, below sab_dec_inner) decaps' inner
 //       product and message chain beside encaps' K and decaps' H(ct),
 //       leaving one permutation (sab_k) after the re-encryption.
@@ -409,21 +409,39 @@ __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restr
 }
 
 template <int L>
-__global__ void __launch_bounds__(192) sab_pipe_x1(int n, const uint8_t* __restrict__ pk,
+struct X1Lds {
+  static constexpr int kPkBlocks = (SaberP<L>::pk / 8 + kRateSha3_256 - 1) / kRateSha3_256;
+  uint32_t msg[kPkBlocks * kRateSha3_256 * 64];  // pk halves for encaps' H(pk)
+  uint32_t mh[4][64];                            // wave 2's halves of m, by lane
+  uint32_t ready, m_ready;
+};
+
+template <int L>
+__global__ void __launch_bounds__(256) sab_pipe_x1(int n, const uint8_t* __restrict__ pk,
                                                    const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ sk,
                                                    uint8_t* __restrict__ sraw_e, uint8_t* __restrict__ mbuf_e,
                                                    uint8_t* __restrict__ krbuf_e) {
   using P = SaberP<L>;
-  __shared__ uint32_t mh[4][64];  // wave 2's halves of m, by lane
+  constexpr int PW = P::pk / 8;
+  __shared__ X1Lds<L> S;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t p = lane & 1;
   const int inst = blockIdx.x * 32 + (lane >> 1);
   const bool live = inst < n;
+  if (threadIdx.x == 0) {
+    S.ready = 0;
+    S.m_ready = 0;
+  }
+  __syncthreads();  // the only workgroup barrier
   uint32_t g[8];
-  if (live) {
+  if (wave == 3) {
+    // helper: deinterleaves pk into LDS ahead of encaps' H(pk) (wave 0)
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(pk + (size_t)min(inst, n - 1) * P::pk);
+    k2::stage2_message<kRateSha3_256, PW, 64>(S.msg, &S.ready, p, lane, w, lane == 0);
+  } else if (live) {
     if (wave == 0) {
       uint32_t a[25];
-      absorb2_global<kRateSha3_256, P::pk / 8>(a, p, pk + (size_t)inst * P::pk, kDsSha3);  // encaps' H(pk)
+      k2::absorb2_staged<kRateSha3_256, PW, 64>(a, p, S.msg, &S.ready, lane, kDsSha3);  // encaps' H(pk)
 #pragma unroll
       for (int i = 0; i < 4; ++i) g[4 + i] = a[i];
     } else if (wave == 1) {
@@ -432,13 +450,17 @@ __global__ void __launch_bounds__(192) sab_pipe_x1(int n, const uint8_t* __restr
       uint32_t h[4];
       enc_m(inst, p, ecoins, mbuf_e, h);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) mh[i][lane] = h[i];
+      for (int i = 0; i < 4; ++i) S.mh[i][lane] = h[i];
     }
   }
-  __syncthreads();
+  if (wave == 2) {  // m handed to wave 0 through LDS (an LDS count, so that wave 0 never waits for wave 1)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_fetch_add(&S.m_ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   if (!live || wave != 0) return;
+  k2::lds_wait_count(&S.m_ready, 1u);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) g[i] = mh[i][lane];
+  for (int i = 0; i < 4; ++i) g[i] = S.mh[i][lane];
   enc_g_chain<L>(inst, p, g, sraw_e, krbuf_e);
 }
 
@@ -1177,12 +1199,12 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint32_t* hc_d = reinterpret_cast<uint32_t*>(take(32));
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   const int nb = blocks_for(n, 32);
-  const dim3 g(nb), b64(64), b192(192), bp(P::threads);
+  const dim3 g(nb), b64(64), bp(P::threads);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x0"), sab_pipe_x0<L>, g, dim3(256), st, n, kc, pk, sk, araw_k, sraw_d, araw_e,
                araw_d);
   SabPolyArgs ak{n, araw_k, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
-  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, b192, st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, dim3(256), st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e);
   SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
   DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, dim3(blocks_for(n, kX2Group)), dim3(kX2Threads), st, n, sk, ct,
