@@ -444,7 +444,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--saber-batch", type=int, default=4096)
     ap.add_argument("--frodo-batch", type=int, default=8192)
-    ap.add_argument("--frodo-steps", type=int, default=3)
+    ap.add_argument("--frodo-steps", type=int, default=6)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--skip-frodo", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
@@ -514,7 +514,7 @@ def main():
     }
     if not args.skip_frodo:
         progress(f"frodo976 batch {args.frodo_batch}: autotune, 1 warmup + {args.frodo_steps} timed graph steps")
-        fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002,
+        fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 2, 0x44504B454D000002,
                                schedule=sched)
         progress(f"frodo976: {fr['ops_per_s']:.0f} ops/s; plan kernels and an eager pass with per-kernel HIP events")
         fr_prof = profile_plan(torch, eng, D.FRODO976, args.frodo_batch, 1, 0x44504B454D000002)

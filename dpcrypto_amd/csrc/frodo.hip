@@ -170,12 +170,27 @@ using k2::store2_words;
   if (inst >= n) return;
 
 // Two samples from the 32-bit half of a squeezed word: int8 pair (low 16
-// bits) and the little-endian u16 pair (full 32 bits).
+// bits) and the little-endian u16 pair (full 32 bits).  frodo_sample on both
+// 16-bit draws at once with packed 16-bit ops: for 15-bit values, cdf[j] - t
+// is negative exactly when cdf[j] < t, so an arithmetic shift by 15 gives -1
+// per counted entry; the sign is then applied per half.
+typedef short fr_s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short fr_u16x2 __attribute__((ext_vector_type(2)));
 template <class P>
 __device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t* le) {
-  const int32_t e0 = frodo_sample<P>(x & 0xFFFF), e1 = frodo_sample<P>(x >> 16);
-  *le = ((uint32_t)e0 & 0xFFFF) | ((uint32_t)e1 << 16);
-  return ((uint32_t)e0 & 0xFF) | (((uint32_t)e1 & 0xFF) << 8);
+  const fr_s16x2 t = __builtin_bit_cast(fr_s16x2, (x >> 1) & 0x7FFF7FFFu);
+  const uint16_t* cdf = P::cdf();
+  fr_s16x2 neg = {0, 0};  // minus the count of CDF entries below t, per half
+#pragma unroll
+  for (int j = 0; j < P::ncdf; ++j) {
+    const fr_s16x2 d = __builtin_bit_cast(fr_s16x2, (uint32_t)cdf[j] * 0x10001u) - t;
+    neg += d >> (fr_s16x2){15, 15};
+  }
+  const fr_u16x2 sm = __builtin_bit_cast(fr_u16x2, x & 0x00010001u);
+  const fr_u16x2 e = (fr_u16x2){0, 0} - __builtin_bit_cast(fr_u16x2, neg);
+  const uint32_t v = __builtin_bit_cast(uint32_t, (e ^ ((fr_u16x2){0, 0} - sm)) + sm);
+  *le = v;
+  return __builtin_amdgcn_perm(v, v, 0x0C0C0200u);  // bytes 0 and 2: the int8 pair
 }
 
 // SHAKE(dom || seed[sec bytes]) on a lane pair, squeezing OUTW words; f(w,
