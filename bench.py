@@ -513,7 +513,7 @@ def main():
         "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
     }
     if not args.skip_frodo:
-        progress(f"frodo976 batch {args.frodo_batch}: autotune, 1 warmup + {args.frodo_steps} timed graph steps")
+        progress(f"frodo976 batch {args.frodo_batch}: autotune, 2 warmup + {args.frodo_steps} timed graph steps")
         fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 2, 0x44504B454D000002,
                                schedule=sched)
         progress(f"frodo976: {fr['ops_per_s']:.0f} ops/s; plan kernels and an eager pass with per-kernel HIP events")
