@@ -150,6 +150,12 @@ __device__ __forceinline__ uint32_t pack15_word(const W* src, int count, int w) 
 
 // ----------------------------------------------------------------- chains (lane pairs)
 //
+// The serial chains are latency-bound single waves; in a multi-stream plan
+// they run beside other chunks' fused kernels, whose VALU-saturated waves
+// may share their SIMD.  Chain waves raise their issue priority so that the
+// SIMD issues for them first and the fused waves fill the gaps.
+__device__ __forceinline__ void chain_priority() { __builtin_amdgcn_s_setprio(3); }
+//
 // Serial SHAKE chains (SHAKE256 for Frodo-976 / -1344, SHAKE128 for
 // Frodo-640) run on lane pairs (keccak2.hpp): lanes 2k / 2k+1 hold the even
 // / odd bit halves of instance k's state, 32 instances per 64-thread block.
@@ -229,6 +235,7 @@ template <class P>
 __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                        uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
+  chain_priority();
   const int vb = blockIdx.x;
   FR_PAIR_SETUP
   constexpr int RW = P::chain_rate, LS = P::ls;
@@ -274,6 +281,7 @@ __device__ __forceinline__ void keygen_post_body(int vb, int n, const uint8_t* _
 
 template <class P>
 __global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk) {
+  chain_priority();
   keygen_post_body<P>(blockIdx.x, n, pk, sk);
 }
 
@@ -329,6 +337,7 @@ __global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __r
                                                        uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
                                                        uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
                                                        const uint8_t* __restrict__ pkh) {
+  chain_priority();
   encdec_pre_body<P, DEC>(blockIdx.x, n, keys, key_stride, mu, mu_stride, kbuf, s8, e8, epp8, pkh);
 }
 
@@ -393,6 +402,7 @@ template <class P>
 __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ kbuf,
                                                const uint8_t* __restrict__ sk, size_t sk_stride,
                                                const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+  chain_priority();
   ss_body<P>(blockIdx.x, n, ct, kbuf, sk, sk_stride, fail, ss);
 }
 
@@ -1040,6 +1050,7 @@ __global__ void __launch_bounds__(128) frodo_pipe_x1(int n, const uint8_t* __res
                                                      const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ kbuf_e,
                                                      uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
                                                      uint32_t* __restrict__ epp8) {
+  chain_priority();
   if ((threadIdx.x >> 6) == 0) keygen_post_body<P>(blockIdx.x, n, pk, sk);
   else encdec_pre_body<P, false>(blockIdx.x, n, pk, (size_t)P::pk, ecoins, (size_t)P::ecoins, kbuf_e, s8, e8, epp8,
                                  nullptr);
@@ -1051,6 +1062,7 @@ __global__ void __launch_bounds__(192) frodo_pipe_x2(int n, const uint8_t* __res
                                                      const uint8_t* __restrict__ mubuf, uint8_t* __restrict__ kbuf_d,
                                                      uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
                                                      uint32_t* __restrict__ epp8, uint32_t* __restrict__ state) {
+  chain_priority();
   const int wave = threadIdx.x >> 6, vb = blockIdx.x;
   if (wave == 0) {
     ss_body<P>(vb, n, ct, kbuf_e, nullptr, 0, nullptr, ss_e);
@@ -1070,6 +1082,7 @@ __global__ void __launch_bounds__(64) frodo_pipe_ssd(int n, const uint8_t* __res
                                                      const uint32_t* __restrict__ state, const uint8_t* __restrict__ kbuf_d,
                                                      const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
                                                      uint8_t* __restrict__ ss) {
+  chain_priority();
   const int vb = blockIdx.x;
   FR_PAIR_SETUP
   uint32_t a[25];
