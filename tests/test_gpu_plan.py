@@ -41,7 +41,12 @@ FRODO_AES, LIGHTSABER, FIRESABER = 2, 3, 4
     (SABER, 300, 4, 4, False, True), (SABER, 300, 4, 4, False, False), (SABER, 300, 4, 4, True, True),
     (SABER, 97, 2, 3, True, True), (SABER, 97, 2, 3, False, True), (LIGHTSABER, 70, 2, 2, False, True),
     (FIRESABER, 45, 3, 2, False, True), (FRODO, 10, 3, 3, False, True), (FRODO, 10, 3, 3, False, False),
-    (FRODO, 37, 2, 1, False, True), (FRODO_AES, 6, 2, 2, False, True)])
+    (FRODO, 37, 2, 1, False, True), (FRODO_AES, 6, 2, 2, False, True),
+    # tiny and ragged batches: 32-instance hash groups and 16-instance x2
+    # groups with one or a few live instances, clamped loads past n
+    (SABER, 1, 1, 1, False, True), (SABER, 17, 1, 1, False, True), (SABER, 50, 1, 3, False, True),
+    (LIGHTSABER, 2, 1, 1, False, True), (FIRESABER, 19, 1, 1, False, True), (FRODO, 1, 1, 1, False, True),
+    (FRODO, 3, 1, 2, False, True)])
 def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share,
                                                      pipe):
     _chain_replays(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share, pipe, True)
