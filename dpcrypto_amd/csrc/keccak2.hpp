@@ -161,20 +161,16 @@ inline __attribute__((always_inline)) DPKEM_HD uint64_t reint_full(uint32_t e, u
 namespace dpkem {
 namespace k2 {
 
-__constant__ static const uint32_t kRCe[24] = {
-    compress_even(kRCs[0]), compress_even(kRCs[1]), compress_even(kRCs[2]), compress_even(kRCs[3]),
-    compress_even(kRCs[4]), compress_even(kRCs[5]), compress_even(kRCs[6]), compress_even(kRCs[7]),
-    compress_even(kRCs[8]), compress_even(kRCs[9]), compress_even(kRCs[10]), compress_even(kRCs[11]),
-    compress_even(kRCs[12]), compress_even(kRCs[13]), compress_even(kRCs[14]), compress_even(kRCs[15]),
-    compress_even(kRCs[16]), compress_even(kRCs[17]), compress_even(kRCs[18]), compress_even(kRCs[19]),
-    compress_even(kRCs[20]), compress_even(kRCs[21]), compress_even(kRCs[22]), compress_even(kRCs[23])};
-__constant__ static const uint32_t kRCo[24] = {
-    compress_even(kRCs[0] >> 1), compress_even(kRCs[1] >> 1), compress_even(kRCs[2] >> 1), compress_even(kRCs[3] >> 1),
-    compress_even(kRCs[4] >> 1), compress_even(kRCs[5] >> 1), compress_even(kRCs[6] >> 1), compress_even(kRCs[7] >> 1),
-    compress_even(kRCs[8] >> 1), compress_even(kRCs[9] >> 1), compress_even(kRCs[10] >> 1), compress_even(kRCs[11] >> 1),
-    compress_even(kRCs[12] >> 1), compress_even(kRCs[13] >> 1), compress_even(kRCs[14] >> 1), compress_even(kRCs[15] >> 1),
-    compress_even(kRCs[16] >> 1), compress_even(kRCs[17] >> 1), compress_even(kRCs[18] >> 1), compress_even(kRCs[19] >> 1),
-    compress_even(kRCs[20] >> 1), compress_even(kRCs[21] >> 1), compress_even(kRCs[22] >> 1), compress_even(kRCs[23] >> 1)};
+// Round constant r as {even half, odd half}: one scalar load per round pair.
+struct RcPair {
+  uint32_t e, o;
+};
+#define K2_RC(r) {compress_even(kRCs[r]), compress_even(kRCs[r] >> 1)}
+__constant__ static const RcPair kRC2[24] = {
+    K2_RC(0),  K2_RC(1),  K2_RC(2),  K2_RC(3),  K2_RC(4),  K2_RC(5),  K2_RC(6),  K2_RC(7),
+    K2_RC(8),  K2_RC(9),  K2_RC(10), K2_RC(11), K2_RC(12), K2_RC(13), K2_RC(14), K2_RC(15),
+    K2_RC(16), K2_RC(17), K2_RC(18), K2_RC(19), K2_RC(20), K2_RC(21), K2_RC(22), K2_RC(23)};
+#undef K2_RC
 
 struct DevOps {
   static __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
@@ -201,15 +197,21 @@ struct DevOps {
 // Both halves of the round constant are read at the uniform index r (scalar
 // loads, waited on with lgkmcnt) and the lane's half is selected with a mask:
 // a per-lane table index would become a vector load whose vmcnt wait also
-// waits for every store the sponge still has in flight.
+// waits for every store the sponge still has in flight.  The table address
+// is pinned in SGPRs before the loop; otherwise the compiler rebuilds it
+// (s_getpc and 64-bit adds) in every iteration, scalar instructions that
+// take issue slots of the lone chain wave.
 __device__ __forceinline__ void keccakf2(uint32_t a[25], uint32_t p) {
+  typedef const __attribute__((address_space(4))) RcPair ConstRc;
   uint32_t sh[32];
 #pragma unroll
   for (int k = 0; k < 32; ++k) sh[k] = DevOps::shift(p, k);  // only the used entries survive
   const uint32_t odd = 0u - p;
+  ConstRc* rct = (ConstRc*)kRC2;
+  asm volatile("" : "+s"(rct));
 #pragma unroll 2
   for (int r = 0; r < 24; ++r) {
-    const uint32_t re = kRCe[r], ro = kRCo[r];
+    const uint32_t re = rct[r].e, ro = rct[r].o;
     const uint32_t rc = re ^ ((re ^ ro) & odd);
     round2<DevOps>(a, sh, rc);
   }
