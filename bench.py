@@ -272,16 +272,24 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
     stream = torch.cuda.current_stream(dev)
     eng.set_stream(stream.cuda_stream)
 
-    def build(ns, ch):
+    saber_family = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
+
+    def build(ns, ch, ipw=1):
+        # ipw: the Saber product's instances per workgroup (a launch-shape knob)
+        eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
         p = eng.plan(nstreams=ns, chunks=ch, share_matrix=share_matrix, pipeline=pipeline)
         t0 = p.keypair(scheme, pk, sk, kc)
         t1 = p.enc(scheme, ct, ss, pk, ec, after=t0)
         p.dec(scheme, ss2, ct, sk, after=t1)
         return p.build()
 
-    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}|pipe={int(pipeline)}"
+    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}|pipe={int(pipeline)}|ipw"
     if schedule:
-        best, tried = schedule, {}
+        best, tried = tuple(schedule), {}
+    elif saber_family:
+        best, tried = autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s,
+                                    candidates=[c + (1,) for c in autotune.DEFAULT_CANDIDATES],
+                                    refine=lambda b: [(b[0], b[1], 2), (b[0], b[1], 4)])
     else:
         best, tried = autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s)
     plan = build(*best)
@@ -302,7 +310,8 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
     plan.close()
     return dict(wall_s=wall, per_step_ms=per_step, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
                 schedule={"nstreams": best[0], "chunks": best[1],
-                          "tuning_ms": {f"{a}x{b}": round(v, 4) for (a, b), v in tried.items()}})
+                          **({"saber_poly_ipw": best[2]} if len(best) > 2 else {}),
+                          "tuning_ms": {"x".join(map(str, c)): round(v, 4) for c, v in tried.items()}})
 
 
 def run_shared_key(torch, eng, scheme, n, steps, seed):
@@ -454,7 +463,8 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=0.5, help="arrival window per rate")
     ap.add_argument("--serving-rates", default="10000,50000,100000,200000,500000,1000000,2000000")
     ap.add_argument("--schedule", default=None,
-                    help="fixed plan schedule STREAMSxCHUNKS instead of autotuning (profiling runs)")
+                    help="fixed plan schedule STREAMSxCHUNKS[xIPW] instead of autotuning (profiling runs; "
+                         "IPW = Saber product instances per workgroup, default 1)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one elementwise kernel over 256 MiB in / 256 MiB out (HBM counter scale)")
     args = ap.parse_args()
@@ -553,7 +563,7 @@ def main():
             "workload": f"frodokem-976-aes batch {args.frodo_batch}: keygen+encaps+decaps per step",
             "ops_per_s": round(fa["ops_per_s"], 1), "ms_per_step": round(fa["wall_s"] * 1e3 / args.frodo_steps, 3),
             "roundtrip_ok": fa["roundtrip_ok"] and fa_eager["roundtrip_ok"],
-            "schedule": {"nstreams": fa["schedule"]["nstreams"], "chunks": fa["schedule"]["chunks"]},
+            "schedule": {k: v for k, v in fa["schedule"].items() if k != "tuning_ms"},
             "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fa_eager["kernels"].items()}}
         progress("frodo976 encaps + decaps against one key (A expanded once per 4 instances)")
         result["frodo976_shared_key"] = run_shared_key(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps,
@@ -567,7 +577,7 @@ def main():
                 "workload": f"{name}-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
                 "ops_per_s": round(fs["ops_per_s"], 1), "ms_per_step": round(fs["wall_s"] * 1e3 / args.frodo_steps, 3),
                 "roundtrip_ok": fs["roundtrip_ok"] and fs_prof["roundtrip_ok"],
-                "schedule": {"nstreams": fs["schedule"]["nstreams"], "chunks": fs["schedule"]["chunks"]},
+                "schedule": {k: v for k, v in fs["schedule"].items() if k != "tuning_ms"},
                 "roofline": roofline_from(fs_prof["kernels"], args.frodo_batch),
                 "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fs_prof["kernels"].items()}}
     for name, sc in (("lightsaber", D.LIGHTSABER), ("firesaber", D.FIRESABER)):
@@ -579,7 +589,7 @@ def main():
                         "ops_per_s": round(fam["ops_per_s"], 1),
                         "ms_per_step": round(fam["wall_s"] * 1e3 / args.steps, 4),
                         "roundtrip_ok": fam["roundtrip_ok"] and fam_prof["roundtrip_ok"],
-                        "schedule": {"nstreams": fam["schedule"]["nstreams"], "chunks": fam["schedule"]["chunks"]},
+                        "schedule": {k: v for k, v in fam["schedule"].items() if k != "tuning_ms"},
                         "roofline": roofline_from(fam_prof["kernels"], args.saber_batch),
                         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fam_prof["kernels"].items()}}
     progress("saber with the chain run as three dependent tasks (no stage pipelining)")
@@ -588,7 +598,7 @@ def main():
     result["saber_unpipelined"] = {
         "ops_per_s": round(unpiped["ops_per_s"], 1), "roundtrip_ok": unpiped["roundtrip_ok"],
         "ms_per_step": round(unpiped["wall_s"] * 1e3 / args.steps, 4),
-        "schedule": {"nstreams": unpiped["schedule"]["nstreams"], "chunks": unpiped["schedule"]["chunks"]},
+        "schedule": {k: v for k, v in unpiped["schedule"].items() if k != "tuning_ms"},
         "note": "keygen, encaps and decaps as three dependent plan tasks (ten launches per chunk); the headline "
                 "runs the same work with stages of different ops sharing launches (seven per chunk)"}
     progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
@@ -596,7 +606,7 @@ def main():
                                tune_budget_s=20.0, share_matrix=True, schedule=sched)
     result["saber_shared_matrix_chain"] = {
         "ops_per_s": round(shared["ops_per_s"], 1), "roundtrip_ok": shared["roundtrip_ok"],
-        "schedule": {"nstreams": shared["schedule"]["nstreams"], "chunks": shared["schedule"]["chunks"]},
+        "schedule": {k: v for k, v in shared["schedule"].items() if k != "tuning_ms"},
         "note": "encaps/decaps reuse keygen's expanded matrix; valid only when one caller runs all three ops"}
     if not args.skip_mixed:
         from dpcrypto_amd import workloads as W

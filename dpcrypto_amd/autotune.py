@@ -4,10 +4,13 @@ A plan's schedule has two knobs: how many internal streams it spreads work
 over and how many instance chunks each task is cut into.  Chunks let the
 latency-bound hash chains of one chunk (few waves) run beside the
 throughput-bound products of another; too many chunks add launches and
-shrink the product kernels.  The best point depends on scheme, batch size
-and op mix, so it is measured: every candidate plan is built, replayed and
-timed with HIP events on the engine's stream; the winner is cached in a JSON
-file keyed by device name and workload signature.
+shrink the product kernels.  Kernel launch-shape knobs of the engine
+(``Engine.set_tuning``, e.g. the Saber product's instances per workgroup)
+are tuned after them, one knob at a time at the best schedule (coordinate
+descent: ``refine`` proposes the extra candidates).  The best point depends
+on scheme, batch size and op mix, so it is measured: every candidate plan is
+built, replayed and timed with HIP events on the engine's stream; the winner
+is cached in a JSON file keyed by device name and workload signature.
 
 The cache lives at $DPKEM_TUNE_CACHE (default ~/.cache/dpkem/tune.json).
 """
@@ -63,11 +66,14 @@ def time_plan(torch, eng, plan, reps: int) -> float:
 
 
 def tune(torch, eng, key: str, build_plan, candidates=DEFAULT_CANDIDATES, reps: int = 3,
-         use_cache: bool = True, budget_s: float = 60.0):
-    """Pick (nstreams, chunks) for the workload that `build_plan(ns, ch)` builds.
+         use_cache: bool = True, budget_s: float = 60.0, refine=None):
+    """Pick the candidate (a tuple of knob values, e.g. (nstreams, chunks)) for
+    the workload that `build_plan(*candidate)` builds.
 
-    Returns (best (ns, ch), {candidate: ms}).  `key` names the workload
-    (scheme, op mix, batch size); the device name is added to it.
+    `refine(best)`, if given, returns further candidates that are timed after
+    the first sweep (within the same budget).  Returns (best, {candidate: ms}).
+    `key` names the workload (scheme, op mix, batch size, knob set); the
+    device name is added to it.
     """
     dev = torch.cuda.get_device_name(0)
     full_key = f"{dev}|{key}"
@@ -77,17 +83,25 @@ def tune(torch, eng, key: str, build_plan, candidates=DEFAULT_CANDIDATES, reps: 
         return tuple(e["best"]), {tuple(map(int, k.split("x"))): v for k, v in e["ms"].items()}
     results = {}
     t_end = time.perf_counter() + budget_s
-    for ns, ch in candidates:
-        if time.perf_counter() > t_end:
-            break
-        plan = build_plan(ns, ch)
-        try:
-            results[(ns, ch)] = time_plan(torch, eng, plan, reps)
-        finally:
-            plan.close()
+
+    def sweep(cands):
+        for cand in cands:
+            if cand in results:
+                continue
+            if results and time.perf_counter() > t_end:
+                break
+            plan = build_plan(*cand)
+            try:
+                results[cand] = time_plan(torch, eng, plan, reps)
+            finally:
+                plan.close()
+
+    sweep([tuple(c) for c in candidates])
+    if refine is not None:
+        sweep([tuple(c) for c in refine(min(results, key=results.get))])
     best = min(results, key=results.get)
     db = _load()
-    db[full_key] = {"best": list(best), "ms": {f"{a}x{b}": round(v, 4) for (a, b), v in results.items()}}
+    db[full_key] = {"best": list(best), "ms": {"x".join(map(str, c)): round(v, 4) for c, v in results.items()}}
     if use_cache:
         _store(db)
     return best, results

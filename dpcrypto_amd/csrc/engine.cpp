@@ -118,7 +118,7 @@ void io_sizes(int scheme, int op, size_t out[2], size_t in[2]) {
 
 // Launch one chunk with all pointers on the device.
 void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
-                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh) {
+                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh, int saber_ipw) {
   Carver c{scratch};
   if (form == DPKEM_FORM_PLAIN) {
     if (scheme == DPKEM_SABER) plain_saber_launch(st, op, n, o0, o1, i0, i1, scratch);
@@ -133,9 +133,9 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     uint8_t* mbuf = c.take((size_t)n * 32);
     uint8_t* krbuf = c.take((size_t)n * 64);
     uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
-    if (op == kKeygen) saber_keygen_launch(st, n, i0, o0, o1, araw, sraw, l);
-    else if (op == kEnc) saber_enc_launch(st, n, i0, i1, o0, o1, araw, sraw, mbuf, krbuf, l);
-    else saber_dec_launch(st, n, i0, i1, o0, araw, sraw, mbuf, krbuf, fail, l);
+    if (op == kKeygen) saber_keygen_launch(st, n, i0, o0, o1, araw, sraw, l, saber_ipw);
+    else if (op == kEnc) saber_enc_launch(st, n, i0, i1, o0, o1, araw, sraw, mbuf, krbuf, l, saber_ipw);
+    else saber_dec_launch(st, n, i0, i1, o0, araw, sraw, mbuf, krbuf, fail, l, saber_ipw);
   } else {
     const int set = frodo_set(scheme);
     uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 8 * set));
@@ -269,7 +269,8 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
       uint8_t* bcast = scheme != DPKEM_FRODO976 ? io.take((size_t)c * in[shared_in]) : nullptr;
       launch_chunk_shared(e->stream, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, bcast);
     } else {
-      launch_chunk_on(e->stream, e->form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena);
+      launch_chunk_on(e->stream, e->form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, nullptr,
+                      e->saber_ipw);
     }
     tls_profiler = nullptr;
     HIP_TRY(e, hipGetLastError());
@@ -406,6 +407,13 @@ int dpkem_engine_set_form(dpkem_engine* e, int form) {
   if (!e || (form != DPKEM_FORM_BATCHED && form != DPKEM_FORM_PLAIN)) return DPKEM_EINVAL;
   std::lock_guard<std::mutex> lock(e->mu);
   e->form = form;
+  return DPKEM_OK;
+}
+
+int dpkem_engine_set_tuning(dpkem_engine* e, int knob, int value) {
+  if (!e || knob != DPKEM_TUNE_SABER_POLY_IPW || (value != 1 && value != 2 && value != 4)) return DPKEM_EINVAL;
+  std::lock_guard<std::mutex> lock(e->mu);
+  e->saber_ipw = value;
   return DPKEM_OK;
 }
 

@@ -17,6 +17,7 @@ struct dpkem_engine {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   int form = DPKEM_FORM_BATCHED;
+  int saber_ipw = 1;  // DPKEM_TUNE_SABER_POLY_IPW
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
   char last_err[256] = {0};
@@ -45,8 +46,10 @@ void io_sizes(int scheme, int op, size_t out[2], size_t in[2]);
 void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
                           const uint8_t* i1, uint8_t* scratch, const uint2* groups, int ngroups,
                           const uint8_t* frodo_pkh = nullptr);
+// saber_ipw: Saber-family product instances per workgroup (DPKEM_TUNE_SABER_POLY_IPW).
 void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
-                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh = nullptr);
+                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh = nullptr,
+                     int saber_ipw = 1);
 int record_hip_error(dpkem_engine* e, hipError_t err, const char* what);
 
 }  // namespace dpkem

@@ -8,14 +8,16 @@ namespace dpkem {
 
 // ---- batched form (saber.hip, frodo.hip)
 // Saber family: l = 2 (LightSaber), 3 (Saber), 4 (FireSaber); araw / sraw
-// hold saber_scratch_per_inst(l) bytes per instance with the rest.
+// hold saber_scratch_per_inst(l) bytes per instance with the rest.  ipw:
+// product instances per workgroup (1, 2 or 4; a tuning knob, results are
+// identical).
 size_t saber_scratch_per_inst(int l);
 void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
-                         uint8_t* sraw, int l = 3);
+                         uint8_t* sraw, int l = 3, int ipw = 1);
 void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l = 3);
+                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l = 3, int ipw = 1);
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
-                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l = 3);
+                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l = 3, int ipw = 1);
 
 // dst[i] = src for n rows of `bytes` bytes (a multiple of 4).
 void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t bytes, uint8_t* dst);
@@ -23,14 +25,14 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
 // sharing keygen's expanded matrix (saber.hip); scratch:
 // kSaberChainScratchPerInst bytes per instance.
 void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch);
+                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int ipw = 1);
 constexpr size_t kSaberChainScratchPerInst = 3744 + 768 + 32 + 64 + 4;
 // Pipelined keygen -> encaps -> decaps of the same instances on one stream
 // (saber.hip): stages of different ops that read disjoint inputs share
 // launches, every op computes all of its own hashes.  Saber family, rank l.
 size_t saber_pipe_scratch_per_inst(int l);
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l);
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int ipw = 1);
 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch

@@ -48,7 +48,7 @@ struct Sub {
 
 struct dpkem_plan {
   dpkem_engine* e = nullptr;
-  int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED;
+  int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED, saber_ipw = 1;
   unsigned flags = 0;
   std::vector<Task> tasks;
   std::vector<Sub> subs;
@@ -222,10 +222,10 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
       uint8_t *pk = T.o0 + b * ko[0], *sk = T.o1 + b * ko[1];
       uint8_t *ct = E.o0 + b * eo[0], *ss = E.o1 + b * eo[1], *ss2 = Dd.o0 + b * eo[1];
       if (s.chain_kind == kShareChain)
-        saber_chain_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off);
+        saber_chain_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off, p->saber_ipw);
       else if (saber_rank_of(T.scheme))
         saber_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
-                          saber_rank_of(T.scheme));
+                          saber_rank_of(T.scheme), p->saber_ipw);
       else
         frodo_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
                           frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES);
@@ -239,7 +239,8 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
     uint8_t* o1 = out[1] ? T.o1 + s.begin * out[1] : nullptr;
     const uint8_t* i0 = in[0] ? T.i0 + s.begin * in[0] : nullptr;
     const uint8_t* i1 = in[1] ? T.i1 + s.begin * in[1] : nullptr;
-    launch_chunk_on(st, p->form, T.scheme, T.op, (int)s.count, o0, o1, i0, i1, p->scratch + s.scratch_off);
+    launch_chunk_on(st, p->form, T.scheme, T.op, (int)s.count, o0, o1, i0, i1, p->scratch + s.scratch_off, nullptr,
+                    p->saber_ipw);
     ok = ok && hipGetLastError() == hipSuccess;
     ok = ok && hipEventRecord(s.done, st) == hipSuccess;
   }
@@ -262,6 +263,7 @@ int dpkem_plan_create(dpkem_engine* e, dpkem_plan** out, int nstreams, int chunk
   p->nstreams = nstreams;
   p->chunks = chunks;
   p->form = e->form;
+  p->saber_ipw = e->saber_ipw;
   *out = p;
   return DPKEM_OK;
 }

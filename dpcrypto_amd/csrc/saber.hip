@@ -543,18 +543,15 @@ __device__ __forceinline__ uint64_t lds_bytes64(const uint8_t* base, int off) {
   return ((((uint64_t)w[1] << 32) | w[0]) >> (8 * (off & 3)));
 }
 
+// One instance's product on one wave (tid = its lane) with its own LDS S.
 template <int MODE, int L>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) sab_poly(SabPolyArgs a) {
+__device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L>& S, const int tid, const int inst) {
   using P = SaberP<L>;
   constexpr int NT = 64;
   constexpr bool HAS_B = MODE != kSabKeygen;
   constexpr int MW = L * kPolyWords, MPT = (MW + NT - 1) / NT;  // sliced matrix words per K-block
   constexpr int SW = P::mu * 4;                                // sliced secret words per polynomial
   constexpr int CW = MODE == kSabReenc ? P::ct / 4 : 0, CPT = (CW + NT - 1) / NT;
-  __shared__ __align__(16) SabPolyLds<L> S;
-  const int tid = threadIdx.x;
-  const int inst = blockIdx.x;
-  if (inst >= a.n) return;
 
   const uint2* ar = reinterpret_cast<const uint2*>(a.araw + (size_t)inst * P::matrix_bytes);
   const uint2* sr = reinterpret_cast<const uint2*>(a.sraw + (size_t)inst * P::secret_bytes);
@@ -770,6 +767,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) sa
     const bool differ = __builtin_amdgcn_ballot_w64(d != 0) != 0;
     if (tid == 0) a.fail[inst] = differ ? 1u : 0u;
   }
+}
+
+// IPW instances per workgroup, one wave each (instances-per-workgroup is a
+// tuning knob: it changes how many workgroups the dispatcher places and how
+// the waves spread over CUs, not the arithmetic).
+template <int MODE, int L, int IPW = 1>
+__global__ void __launch_bounds__(64 * IPW) __attribute__((amdgpu_waves_per_eu(4))) sab_poly(SabPolyArgs a) {
+  __shared__ __align__(16) SabPolyLds<L> S[IPW];
+  const int w = IPW == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int inst = (int)blockIdx.x * IPW + w;
+  if (inst >= a.n) return;
+  sab_poly_body<MODE, L>(a, S[w], IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63), inst);
 }
 
 // Decaps inner product v = b' . s and message recovery
@@ -1074,37 +1083,45 @@ const char* kname(const char* base, const char* l2, const char* l4) {
 }
 #define SAB_NAME(base) kname<L>(base, base ",l2", base ",l4")
 
+// The product launch with `ipw` instances per workgroup (1, 2 or 4).
+template <int MODE, int L>
+static void poly_launch(const char* name, hipStream_t st, int n, const SabPolyArgs& a, int ipw) {
+  if (ipw == 4) DPKEM_LAUNCH(name, (sab_poly<MODE, L, 4>), dim3(blocks_for(n, 4)), dim3(256), st, a);
+  else if (ipw == 2) DPKEM_LAUNCH(name, (sab_poly<MODE, L, 2>), dim3(blocks_for(n, 2)), dim3(128), st, a);
+  else DPKEM_LAUNCH(name, (sab_poly<MODE, L, 1>), dim3(n), dim3(64), st, a);
+}
+
 template <int L>
-void keygen_l(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw, uint8_t* sraw) {
-  using P = SaberP<L>;
+void keygen_l(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw, uint8_t* sraw,
+              int ipw) {
   DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash1"), sab_keygen_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n,
                coins, pk, sk, araw, sraw);
   SabPolyArgs a{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), dim3(P::threads), st, a);
+  poly_launch<kSabKeygen, L>(SAB_NAME("sab_poly<kSabKeygen>"), st, n, a, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash2"), sab_keygen_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
 }
 
 template <int L>
 void enc_l(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss, uint8_t* araw,
-           uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf) {
+           uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int ipw) {
   using P = SaberP<L>;
   DPKEM_LAUNCH(SAB_NAME("sab_enc_hash1"), sab_enc_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, pk, coins,
                araw, sraw, mbuf, krbuf);
   SabPolyArgs a{n, araw, sraw, pk, (size_t)P::pk, mbuf, ct, nullptr, nullptr, nullptr};
-  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), dim3(P::threads), st, a);
+  poly_launch<kSabEnc, L>(SAB_NAME("sab_poly<kSabEnc>"), st, n, a, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, nullptr,
                nullptr, ss);
 }
 
 template <int L>
 void dec_l(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw, uint8_t* sraw,
-           uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail) {
+           uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int ipw) {
   using P = SaberP<L>;
   DPKEM_LAUNCH(SAB_NAME("sab_dec_inner"), sab_dec_inner<L>, dim3(blocks_for(n, 2)), dim3(64), st, n, ct, sk, mbuf);
   DPKEM_LAUNCH(SAB_NAME("sab_dec_hash1"), sab_dec_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, sk, mbuf,
                araw, sraw, krbuf);
   SabPolyArgs a{n, araw, sraw, sk + P::vecq, (size_t)P::sk, mbuf, nullptr, nullptr, ct, fail};
-  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabReenc>"), (sab_poly<kSabReenc, L>), dim3(n), dim3(P::threads), st, a);
+  poly_launch<kSabReenc, L>(SAB_NAME("sab_poly<kSabReenc>"), st, n, a, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail,
                ss);
 }
@@ -1116,27 +1133,27 @@ size_t saber_scratch_per_inst(int l) {
 }
 
 void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
-                         uint8_t* sraw, int l) {
+                         uint8_t* sraw, int l, int ipw) {
   if (n <= 0) return;
-  if (l == 2) keygen_l<2>(st, n, coins, pk, sk, araw, sraw);
-  else if (l == 4) keygen_l<4>(st, n, coins, pk, sk, araw, sraw);
-  else keygen_l<3>(st, n, coins, pk, sk, araw, sraw);
+  if (l == 2) keygen_l<2>(st, n, coins, pk, sk, araw, sraw, ipw);
+  else if (l == 4) keygen_l<4>(st, n, coins, pk, sk, araw, sraw, ipw);
+  else keygen_l<3>(st, n, coins, pk, sk, araw, sraw, ipw);
 }
 
 void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l) {
+                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l, int ipw) {
   if (n <= 0) return;
-  if (l == 2) enc_l<2>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf);
-  else if (l == 4) enc_l<4>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf);
-  else enc_l<3>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf);
+  if (l == 2) enc_l<2>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf, ipw);
+  else if (l == 4) enc_l<4>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf, ipw);
+  else enc_l<3>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf, ipw);
 }
 
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
-                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l) {
+                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l, int ipw) {
   if (n <= 0) return;
-  if (l == 2) dec_l<2>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail);
-  else if (l == 4) dec_l<4>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail);
-  else dec_l<3>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail);
+  if (l == 2) dec_l<2>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail, ipw);
+  else if (l == 4) dec_l<4>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail, ipw);
+  else dec_l<3>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail, ipw);
 }
 
 // dst[i] = src for i < n (bytes per row a multiple of 4).
@@ -1179,7 +1196,7 @@ size_t saber_pipe_scratch_per_inst(int l) {
 
 template <int L>
 static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk, uint8_t* ct,
-                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int ipw) {
   using P = SaberP<L>;
   size_t off = 0;
   auto take = [&](size_t per) {
@@ -1199,27 +1216,27 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint32_t* hc_d = reinterpret_cast<uint32_t*>(take(32));
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   const int nb = blocks_for(n, 32);
-  const dim3 g(nb), b64(64), bp(P::threads);
+  const dim3 g(nb), b64(64);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x0"), sab_pipe_x0<L>, g, dim3(256), st, n, kc, pk, sk, araw_k, sraw_d, araw_e,
                araw_d);
   SabPolyArgs ak{n, araw_k, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabKeygen>"), (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
+  poly_launch<kSabKeygen, L>(SAB_NAME("sab_poly<kSabKeygen>"), st, n, ak, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, dim3(256), st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e);
   SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
-  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabEnc>"), (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
+  poly_launch<kSabEnc, L>(SAB_NAME("sab_poly<kSabEnc>"), st, n, ae, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, dim3(blocks_for(n, kX2Group)), dim3(kX2Threads), st, n, sk, ct,
                mbuf_d, sraw_d, krbuf_d, krbuf_e, ss, hc_d);
   SabPolyArgs ad{n, araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, fail};
-  DPKEM_LAUNCH(SAB_NAME("sab_poly<kSabReenc>"), (sab_poly<kSabReenc, L>), dim3(n), bp, st, ad);
+  poly_launch<kSabReenc, L>(SAB_NAME("sab_poly<kSabReenc>"), st, n, ad, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_k"), sab_k<L>, g, b64, st, n, hc_d, krbuf_d, sk, fail, ss2);
 }
 
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l) {
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int ipw) {
   if (n <= 0) return;
-  if (l == 2) pipe_l<2>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
-  else if (l == 4) pipe_l<4>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
-  else pipe_l<3>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+  if (l == 2) pipe_l<2>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw);
+  else if (l == 4) pipe_l<4>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw);
+  else pipe_l<3>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw);
 }
 
 // Chained keygen -> encaps -> decaps of the same n Saber (l = 3) instances on
@@ -1230,7 +1247,7 @@ void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* 
 // scratch holds kSaberChainScratchPerInst bytes per instance; the buffers are
 // reused from op to op, which the stream order makes safe.
 void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int ipw) {
   constexpr int L = 3;
   using P = SaberP<L>;
   if (n <= 0) return;
@@ -1245,22 +1262,22 @@ void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t*
   uint8_t* mbuf = take(32);
   uint8_t* krbuf = take(64);
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
-  const dim3 g(blocks_for(n, 32)), b64(64), bp(P::threads), b128(128);
+  const dim3 g(blocks_for(n, 32)), b64(64), b128(128);
   // keygen
   DPKEM_LAUNCH("sab_keygen_hash1", sab_keygen_hash1<L>, g, b128, st, -1, n, kc, pk, sk, araw, sraw);
   SabPolyArgs ak{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabKeygen>", (sab_poly<kSabKeygen, L>), dim3(n), bp, st, ak);
+  poly_launch<kSabKeygen, L>("sab_poly<kSabKeygen>", st, n, ak, ipw);
   DPKEM_LAUNCH("sab_keygen_hash2", sab_keygen_hash2<L>, g, b64, st, n, pk, sk);
   // encaps: message role only, A from keygen
   DPKEM_LAUNCH("sab_enc_hash1", sab_enc_hash1<L>, g, b64, st, 1, n, pk, ec, araw, sraw, mbuf, krbuf);
   SabPolyArgs ae{n, araw, sraw, pk, (size_t)P::pk, mbuf, ct, nullptr, nullptr, nullptr};
-  DPKEM_LAUNCH("sab_poly<kSabEnc>", (sab_poly<kSabEnc, L>), dim3(n), bp, st, ae);
+  poly_launch<kSabEnc, L>("sab_poly<kSabEnc>", st, n, ae, ipw);
   DPKEM_LAUNCH("sab_hash2", sab_hash2<L>, g, b64, st, n, ct, krbuf, nullptr, nullptr, ss);
   // decaps: message role only, A from keygen
   DPKEM_LAUNCH("sab_dec_inner", sab_dec_inner<L>, dim3(blocks_for(n, 2)), b64, st, n, ct, sk, mbuf);
   DPKEM_LAUNCH("sab_dec_hash1", sab_dec_hash1<L>, g, b64, st, 1, n, sk, mbuf, araw, sraw, krbuf);
   SabPolyArgs ad{n, araw, sraw, sk + P::vecq, (size_t)P::sk, mbuf, nullptr, nullptr, ct, fail};
-  DPKEM_LAUNCH("sab_poly<kSabReenc>", (sab_poly<kSabReenc, L>), dim3(n), bp, st, ad);
+  poly_launch<kSabReenc, L>("sab_poly<kSabReenc>", st, n, ad, ipw);
   DPKEM_LAUNCH("sab_hash2", sab_hash2<L>, g, b64, st, n, ct, krbuf, sk, fail, ss2);
 }
 

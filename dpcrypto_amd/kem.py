@@ -28,6 +28,8 @@ MEM_HOST = 0
 MEM_DEVICE = 1
 FORM_BATCHED = 0
 FORM_PLAIN = 1
+# kernel tuning knobs (dpkem_engine_set_tuning)
+TUNE_SABER_POLY_IPW = 1
 
 ERRORS = {
     -1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EBUDGET", -5: "EBUSY", -6: "ENOTFOUND", -7: "ECANCELED",
@@ -78,6 +80,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_engine_destroy.argtypes = [vp]
         lib.dpkem_engine_set_stream.argtypes = [vp, vp, i]
         lib.dpkem_engine_set_form.argtypes = [vp, i]
+        lib.dpkem_engine_set_tuning.argtypes = [vp, i, i]
         lib.dpkem_engine_sync.argtypes = [vp]
         lib.dpkem_engine_last_error.argtypes = [vp]
         lib.dpkem_engine_last_error.restype = ctypes.c_char_p
@@ -164,6 +167,10 @@ class Engine:
 
     def set_form(self, form: int):
         self._check(self.lib.dpkem_engine_set_form(self.h, form), "set_form")
+
+    def set_tuning(self, knob: int, value: int):
+        """Set a launch-shape knob (TUNE_*) for later calls and plans created afterwards."""
+        self._check(self.lib.dpkem_engine_set_tuning(self.h, knob, value), "set_tuning")
 
     def set_stream(self, stream_handle: int | None, own: bool = False):
         """Bind to a HIP stream handle (0 = null stream), or back to the engine's own."""

@@ -79,6 +79,16 @@ int dpkem_engine_destroy(dpkem_engine *e);
 int dpkem_engine_set_stream(dpkem_engine *e, void *hip_stream, int own);
 /* Select the kernel form used by subsequent calls (DPKEM_FORM_*). */
 int dpkem_engine_set_form(dpkem_engine *e, int form);
+/* Kernel tuning knobs: launch shapes only, never results (the autotuner in
+ * dpcrypto_amd/autotune.py sweeps them with a plan's streams and chunks).
+ * Plans take the engine's setting when they are created.  No counterpart in
+ * the reference, whose launch shapes are compile-time constants
+ * (SABER-GPU/src/cuda_kernel.cu:82-87, DoDP4Av7 / MatVecMul_gpu_shared
+ * at one workgroup per instance).
+ * DPKEM_TUNE_SABER_POLY_IPW: instances per workgroup of the Saber-family
+ * product kernel, one wave each: 1 (default), 2 or 4. */
+#define DPKEM_TUNE_SABER_POLY_IPW 1
+int dpkem_engine_set_tuning(dpkem_engine *e, int knob, int value);
 /* Block until all work submitted on the engine's stream has finished. */
 int dpkem_engine_sync(dpkem_engine *e);
 /* Last HIP error string seen by this engine (static storage). */
@@ -88,7 +98,8 @@ const char *dpkem_engine_last_error(dpkem_engine *e);
  * JSON object {"kernel": [launches, total_ms], ...} into buf (reset != 0
  * clears the counters).  Replaces the reference's cudaEvent pairs around
  * whole drivers (FrodoKEM976-GPU/src/cuda_kernel.cu:55,89-92) and its
- * MEAS_MV / MEAS_IP phase events (SABER-GPU/src/cuda_kernel.cu:77-125). */
+ * MEAS_MV / MEAS_IP phase events (SABER-GPU/src/cuda_kernel.cu:82-87, DoDP4Av7 / MatVecMul_gpu_shared
+ * at one workgroup per instance). */
 int dpkem_engine_set_profiling(dpkem_engine *e, int on);
 int dpkem_engine_kernel_stats(dpkem_engine *e, char *buf, size_t len, int reset);
 

@@ -36,6 +36,12 @@ def test_sizes_query(scheme):
     assert lib.dpkem_sizes(-1, *[None] * 6) == -1
 
 
+def test_set_tuning_rejects_a_null_engine():
+    lib = D.load_library()
+    lib.dpkem_engine_set_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    assert lib.dpkem_engine_set_tuning(None, D.TUNE_SABER_POLY_IPW, 2) == -1
+
+
 def test_server_struct_layouts_match_numpy(tmp_path):
     """dpkem_request / dpkem_completion / dpkem_server_config as compiled by the
     host C compiler have the layouts the Python binding uses."""

@@ -52,6 +52,22 @@ def test_chained_plan_matches_oracle_on_every_replay(eng, oracle, torch_cuda, sc
     _chain_replays(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share, pipe, True)
 
 
+@pytest.mark.parametrize("scheme,n,nstreams,chunks,share,ipw", [
+    (SABER, 97, 2, 3, False, 4), (SABER, 50, 1, 1, False, 2), (SABER, 45, 1, 2, True, 4),
+    (LIGHTSABER, 33, 1, 1, False, 4), (FIRESABER, 19, 1, 1, False, 2), (FIRESABER, 70, 2, 2, False, 4)])
+def test_chained_plan_with_product_ipw_matches_oracle(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share,
+                                                      ipw):
+    """The Saber product's instances-per-workgroup knob (2 or 4 waves per
+    workgroup, ragged tails) leaves every byte unchanged; plans take the
+    engine's setting when they are created."""
+    import dpcrypto_amd as D
+    eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    try:
+        _chain_replays(eng, oracle, torch_cuda, scheme, n, nstreams, chunks, share, True, True)
+    finally:
+        eng.set_tuning(D.TUNE_SABER_POLY_IPW, 1)
+
+
 @pytest.mark.parametrize("scheme,n,nstreams,chunks", [(SABER, 97, 2, 3), (LIGHTSABER, 33, 1, 1), (FRODO, 10, 3, 3)])
 def test_plan_without_graph_matches_oracle(eng, oracle, torch_cuda, scheme, n, nstreams, chunks):
     """DPKEM_PLAN_NO_GRAPH issues the same launches directly on every

@@ -39,6 +39,35 @@ def test_ops_equal_oracle(eng, oracle, scheme, n):
     assert np.array_equal(eng.dec(scheme, bad, sk), oracle.dec(scheme, bad, osk, threads=8))
 
 
+@pytest.mark.parametrize("scheme", [D.SABER] + SCHEMES)
+@pytest.mark.parametrize("ipw", [2, 4])
+@pytest.mark.parametrize("n", [1, 5, 70])
+def test_product_ipw_knob_keeps_every_byte(eng, oracle, scheme, ipw, n):
+    """Op-by-op keygen / encaps / decaps with the product kernel at 2 or 4
+    instances per workgroup (ragged last workgroup) equal the oracle."""
+    eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    try:
+        kc = oracle.prng_bytes(8500 + 10 * scheme + n + ipw, n * 96).reshape(n, 96)
+        pk, sk = eng.keypair(scheme, kc)
+        opk, osk = oracle.keypair(scheme, kc, threads=8)
+        assert np.array_equal(pk, opk) and np.array_equal(sk, osk)
+        ec = oracle.prng_bytes(8600 + 10 * scheme + n + ipw, n * 32).reshape(n, 32)
+        ct, ss = eng.enc(scheme, pk, ec)
+        oct_, oss = oracle.enc(scheme, opk, ec, threads=8)
+        assert np.array_equal(ct, oct_) and np.array_equal(ss, oss)
+        bad = ct.copy()
+        bad[::2, 5] ^= 0x04  # every other instance tampered: mixed flags within a workgroup
+        assert np.array_equal(eng.dec(scheme, bad, sk), oracle.dec(scheme, bad, osk, threads=8))
+    finally:
+        eng.set_tuning(D.TUNE_SABER_POLY_IPW, 1)
+
+
+def test_tuning_knob_rejects_bad_values(eng):
+    for knob, value in ((D.TUNE_SABER_POLY_IPW, 3), (D.TUNE_SABER_POLY_IPW, 0), (99, 1)):
+        with pytest.raises(D.DpkemError):
+            eng.set_tuning(knob, value)
+
+
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_device_batch_roundtrip(eng, oracle, scheme):
     import torch

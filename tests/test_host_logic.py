@@ -54,3 +54,73 @@ def test_mixed_batch_sizes_sum_to_the_total():
         b = W.mixed_batches(seed, total=16384, max_batch=4096)
         assert sum(x[2] for x in b) == 16384
         assert all(1 <= x[2] <= 4096 and x[0] in (0, 1) and x[1] in (0, 1, 2) for x in b)
+
+
+class _FakeEvent:
+    clock = [0.0]
+
+    def __init__(self, enable_timing=True):
+        self.t = None
+
+    def record(self, stream=None):
+        self.t = _FakeEvent.clock[0]
+
+    def synchronize(self):
+        pass
+
+    def elapsed_time(self, other):
+        return other.t - self.t
+
+
+class _FakeTorch:
+    class cuda:
+        Event = _FakeEvent
+
+        @staticmethod
+        def get_device_name(i):
+            return "fake"
+
+        @staticmethod
+        def current_stream():
+            return None
+
+        @staticmethod
+        def synchronize():
+            pass
+
+
+class _FakePlan:
+    def __init__(self, cost):
+        self.cost = cost
+
+    def launch(self):
+        _FakeEvent.clock[0] += self.cost
+
+    def close(self):
+        pass
+
+
+def test_autotune_sweeps_schedules_then_refines_the_knobs():
+    """Coordinate descent: the (streams, chunks) sweep picks (2, 1) at the
+    default knob value, then refine() proposes the knob's other values at
+    that schedule, and the best of all timed candidates wins."""
+    from dpcrypto_amd import autotune
+    cost = {(1, 1, 1): 5.0, (2, 1, 1): 3.0, (4, 1, 1): 4.0, (2, 1, 2): 2.5, (2, 1, 4): 2.0}
+    built = []
+
+    def build(*cand):
+        built.append(cand)
+        return _FakePlan(cost[cand])
+
+    best, tried = autotune.tune(_FakeTorch, None, "test", build, candidates=[(1, 1, 1), (2, 1, 1), (4, 1, 1)],
+                                use_cache=False, refine=lambda b: [(b[0], b[1], 2), (b[0], b[1], 4)])
+    assert best == (2, 1, 4)
+    assert tried == pytest.approx(cost)
+    assert built == [(1, 1, 1), (2, 1, 1), (4, 1, 1), (2, 1, 2), (2, 1, 4)]
+
+
+def test_autotune_times_at_least_one_candidate_within_a_spent_budget():
+    from dpcrypto_amd import autotune
+    best, tried = autotune.tune(_FakeTorch, None, "test", lambda *c: _FakePlan(1.0), candidates=[(1, 1), (2, 1)],
+                                use_cache=False, budget_s=-1.0)
+    assert best == (1, 1) and list(tried) == [(1, 1)]
