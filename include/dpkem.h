@@ -81,10 +81,11 @@ int dpkem_engine_set_stream(dpkem_engine *e, void *hip_stream, int own);
 int dpkem_engine_set_form(dpkem_engine *e, int form);
 /* Kernel tuning knobs: launch shapes only, never results (the autotuner in
  * dpcrypto_amd/autotune.py sweeps them with a plan's streams and chunks).
- * Plans take the engine's setting when they are created.  No counterpart in
- * the reference, whose launch shapes are compile-time constants
- * (SABER-GPU/src/cuda_kernel.cu:82-87, DoDP4Av7 / MatVecMul_gpu_shared
- * at one workgroup per instance).
+ * Plans take the engine's setting when they are created; a server
+ * (dpkem_server_create, which has no engine) keeps the defaults.  No
+ * counterpart in the reference, whose launch shapes are compile-time
+ * constants (SABER-GPU/src/cuda_kernel.cu:82-87, DoDP4Av7 /
+ * MatVecMul_gpu_shared at one workgroup per instance).
  * DPKEM_TUNE_SABER_POLY_IPW: instances per workgroup of the Saber-family
  * product kernel, one wave each: 1 (default), 2 or 4. */
 #define DPKEM_TUNE_SABER_POLY_IPW 1
