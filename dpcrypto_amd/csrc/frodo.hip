@@ -445,11 +445,19 @@ __global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __rest
 
 // ----------------------------------------------------------------- fused A generation + MFMA
 
-// Byte limbs of four u16 entries held in one 64-bit word.
+// Byte limbs of four u16 entries held in one 64-bit word: lo = the low
+// bytes (entry k's at byte k) biased by 0x80, hi = the high bytes.  v_perm
+// selects bytes of {d1, d0} (selector 0-3: d0's bytes, 4-7: d1's).
 __device__ __forceinline__ void limb_split(uint64_t w, uint32_t& lo, uint32_t& hi) {
   const uint32_t d0 = (uint32_t)w, d1 = (uint32_t)(w >> 32);
-  lo = ((d0 & 0xFFu) | ((d0 >> 8) & 0xFF00u) | ((d1 & 0xFFu) << 16) | ((d1 << 8) & 0xFF000000u)) ^ 0x80808080u;
-  hi = ((d0 >> 8) & 0xFFu) | ((d0 >> 16) & 0xFF00u) | ((d1 << 8) & 0xFF0000u) | (d1 & 0xFF000000u);
+  lo = __builtin_amdgcn_perm(d1, d0, 0x06040200u) ^ 0x80808080u;
+  hi = __builtin_amdgcn_perm(d1, d0, 0x07050301u);
+}
+
+// Low bytes of the four lanes of an MFMA result, packed (element i at byte i).
+__device__ __forceinline__ int pack_low_bytes(const v4i t) {
+  return (int)(__builtin_amdgcn_perm((uint32_t)t[1], (uint32_t)t[0], 0x0C0C0400u) |
+               __builtin_amdgcn_perm((uint32_t)t[3], (uint32_t)t[2], 0x04000C0Cu));
 }
 
 __device__ __forceinline__ void unit_limbs(const uint64_t w[4], v4i& lo, v4i& hi) {
@@ -807,8 +815,7 @@ __global__ void __launch_bounds__(P::threads) frodo_sa_fused(int n, const uint8_
 #pragma unroll
           for (int qq = 0; qq < 4; ++qq) {
             const v4i t = __builtin_amdgcn_mfma_i32_16x16x64_i8(lm[h], sel[qq], zero, 0, 0, 0);
-            packed[qq] = (int)(((uint32_t)t[0] & 0xFF) | (((uint32_t)t[1] & 0xFF) << 8) |
-                               (((uint32_t)t[2] & 0xFF) << 16) | ((uint32_t)t[3] << 24));
+            packed[qq] = pack_low_bytes(t);
           }
           prod[h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sop, packed, zero, 0, 0, 0);
         }
@@ -959,8 +966,7 @@ __global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
           const v4i t = __builtin_amdgcn_mfma_i32_16x16x64_i8(lm[h], sel[qq], zero, 0, 0, 0);
-          packed[h][qq] = (int)(((uint32_t)t[0] & 0xFF) | (((uint32_t)t[1] & 0xFF) << 8) |
-                                (((uint32_t)t[2] & 0xFF) << 16) | ((uint32_t)t[3] << 24));
+          packed[h][qq] = pack_low_bytes(t);
         }
       const int col = 16 * u + nn;
 #pragma unroll
