@@ -46,32 +46,56 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // the serial chains and the CDF table; A rows are SHAKE128 in all three.
 // threads x passes covers the n rows of A in the fused kernels (one row per
 // lane and pass); the shapes keep 14-16 waves per CU resident.
-__constant__ static const uint16_t kCdf640[12] = {4643, 13363, 20579, 25843, 29227, 31145,
-                                                  32103, 32525, 32689, 32745, 32762, 32766};
-__constant__ static const uint16_t kCdf976[10] = {5638, 15915, 23689, 28571, 31116,
-                                                  32217, 32613, 32731, 32760, 32766};
-__constant__ static const uint16_t kCdf1344[6] = {9142, 23462, 30338, 32361, 32725, 32765};
-
+// cdf: the sampler's CDF table without its final entry (FrodoKEM976-GPU/include/params.h:24
+// for 976; the round-3 specification's tables for 640 / 1344).
 template <int N>
 struct FrodoSet;
 template <>
 struct FrodoSet<640> {
   static constexpr int n = 640, logq = 15, b = 2, sec = 16, chain_rate = kRateShake128, threads = 320, passes = 2,
                        ncdf = 12;
-  static __device__ const uint16_t* cdf() { return kCdf640; }
+  static constexpr uint16_t cdf[ncdf] = {4643, 13363, 20579, 25843, 29227, 31145,
+                                         32103, 32525, 32689, 32745, 32762, 32766};
 };
 template <>
 struct FrodoSet<976> {
   static constexpr int n = 976, logq = 16, b = 3, sec = 24, chain_rate = kRateShake256, threads = 1024, passes = 1,
                        ncdf = 10;
-  static __device__ const uint16_t* cdf() { return kCdf976; }
+  static constexpr uint16_t cdf[ncdf] = {5638, 15915, 23689, 28571, 31116, 32217, 32613, 32731, 32760, 32766};
 };
 template <>
 struct FrodoSet<1344> {
   static constexpr int n = 1344, logq = 16, b = 4, sec = 32, chain_rate = kRateShake256, threads = 448, passes = 3,
                        ncdf = 6;
-  static __device__ const uint16_t* cdf() { return kCdf1344; }
+  static constexpr uint16_t cdf[ncdf] = {9142, 23462, 30338, 32361, 32725, 32765};
 };
+
+// The CDF sampler (FrodoKEM976-GPU/src/sample_pack.cu:13-32) on a 16-bit
+// draw r: magnitude e = #{j : cdf[j] < r >> 1}, sign = r & 1.  e depends on
+// the 15 bits r >> 1 only, so the serial chains read it from a 32 KB table
+// (one byte per value, built here at compile time as the step function
+// e = k on cdf[k-1] < t <= cdf[k]) staged in LDS.
+template <class S>
+struct alignas(16) SampleLut {
+  uint8_t e[32768];
+};
+template <class S>
+constexpr SampleLut<S> make_sample_lut() {
+  SampleLut<S> l{};
+  int t = 0;
+  for (int k = 0; k <= S::ncdf; ++k) {
+    const int hi = k < S::ncdf ? S::cdf[k] : 32767;
+    for (; t <= hi; ++t) l.e[t] = (uint8_t)k;
+  }
+  return l;
+}
+__device__ const SampleLut<FrodoSet<640>> kSampleLut640 = make_sample_lut<FrodoSet<640>>();
+__device__ const SampleLut<FrodoSet<976>> kSampleLut976 = make_sample_lut<FrodoSet<976>>();
+__device__ const SampleLut<FrodoSet<1344>> kSampleLut1344 = make_sample_lut<FrodoSet<1344>>();
+template <int N>
+__device__ __forceinline__ const uint8_t* sample_lut_global() {
+  return N == 640 ? kSampleLut640.e : N == 976 ? kSampleLut976.e : kSampleLut1344.e;
+}
 
 template <int N>
 struct FrodoP : FrodoSet<N> {
@@ -106,20 +130,6 @@ constexpr int kFN = kFrodoN, kFNB = kFrodoNbar;
 constexpr int kFMat = F976::mat;          // 7808
 constexpr int kFCtC1 = F976::ct_c;        // 15616: offset of C in ct
 constexpr int kFUnits = F976::units;      // 61
-
-// CDF sampler on one 16-bit draw (FrodoKEM976-GPU/src/sample_pack.cu:13-32):
-// magnitude = #{j : CDF[j] < r >> 1} over all but the table's final entry,
-// sign = r & 1.
-template <class P>
-__device__ __forceinline__ int32_t frodo_sample(uint32_t r) {
-  const uint32_t t = (r >> 1) & 0x7FFF;
-  const uint16_t* cdf = P::cdf();
-  uint32_t e = 0;
-#pragma unroll
-  for (int j = 0; j < P::ncdf; ++j) e += ((uint32_t)cdf[j] - t) >> 31;
-  const uint32_t s = r & 1;
-  return (int32_t)((e ^ (0u - s)) + s);
-}
 
 // Entry idx of a D-bit, most-significant-bit-first packed stream (D = 16:
 // big-endian u16).  Reads only the bytes the entry occupies.
@@ -175,26 +185,36 @@ using k2::store2_words;
   const int inst = vb * 32 + (lane >> 1);               \
   if (inst >= n) return;
 
-// Two samples from the 32-bit half of a squeezed word: int8 pair (low 16
-// bits) and the little-endian u16 pair (full 32 bits).  frodo_sample on both
-// 16-bit draws at once with packed 16-bit ops: for 15-bit values, cdf[j] - t
-// is negative exactly when cdf[j] < t, so an arithmetic shift by 15 gives -1
-// per counted entry; the sign is then applied per half.
-typedef short fr_s16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short fr_u16x2 __attribute__((ext_vector_type(2)));
+// Stage the sampler table in `lds` (32 KB) with the 64 lanes of one wave and
+// order the copy before the wave's table reads (a wave's LDS operations
+// complete in issue order, so a wavefront-scope fence suffices).
 template <class P>
-__device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t* le) {
-  const fr_s16x2 t = __builtin_bit_cast(fr_s16x2, (x >> 1) & 0x7FFF7FFFu);
-  const uint16_t* cdf = P::cdf();
-  fr_s16x2 neg = {0, 0};  // minus the count of CDF entries below t, per half
+__device__ __forceinline__ const uint8_t* stage_sample_lut(uint8_t* lds, int lane) {
+  const uint4* src = reinterpret_cast<const uint4*>(sample_lut_global<P::n>());
+  uint4* dst = reinterpret_cast<uint4*>(lds);
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {  // 2048 16-byte rows: 32 per lane in four rounds
+    uint4 v[8];
 #pragma unroll
-  for (int j = 0; j < P::ncdf; ++j) {
-    const fr_s16x2 d = __builtin_bit_cast(fr_s16x2, (uint32_t)cdf[j] * 0x10001u) - t;
-    neg += d >> (fr_s16x2){15, 15};
+    for (int i = 0; i < 8; ++i) v[i] = src[(c * 8 + i) * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dst[(c * 8 + i) * 64 + lane] = v[i];
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return lds;
+}
+
+// Two samples from the 32-bit half of a squeezed word (its two 16-bit draws):
+// the int8 pair (low 16 bits) and the little-endian u16 pair (full 32 bits).
+// Magnitudes from the staged table; the sign is applied per 16-bit half.
+typedef unsigned short fr_u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t* le, const uint8_t* lut) {
+  const uint32_t e = (uint32_t)lut[(x >> 1) & 0x7FFF] | ((uint32_t)lut[x >> 17] << 16);
   const fr_u16x2 sm = __builtin_bit_cast(fr_u16x2, x & 0x00010001u);
-  const fr_u16x2 e = (fr_u16x2){0, 0} - __builtin_bit_cast(fr_u16x2, neg);
-  const uint32_t v = __builtin_bit_cast(uint32_t, (e ^ ((fr_u16x2){0, 0} - sm)) + sm);
+  const fr_u16x2 ev = __builtin_bit_cast(fr_u16x2, e);
+  const uint32_t v = __builtin_bit_cast(uint32_t, (ev ^ ((fr_u16x2){0, 0} - sm)) + sm);
   *le = v;
   return __builtin_amdgcn_perm(v, v, 0x0C0C0200u);  // bytes 0 and 2: the int8 pair
 }
@@ -236,6 +256,8 @@ __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __r
                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                        uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
   chain_priority();
+  __shared__ __align__(16) uint8_t lut_lds[32768];
+  const uint8_t* lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
   const int vb = blockIdx.x;
   FR_PAIR_SETUP
   constexpr int RW = P::chain_rate, LS = P::ls;
@@ -260,7 +282,7 @@ __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __r
   uint8_t* skS = q + P::sk_s + 4 * p;
   shake_seed_squeeze2<P, 2 * P::mat / 4>(p, 0x5F, seed, [&](int w, uint32_t x) {
     uint32_t le;
-    const uint32_t b = sample2<P>(x, &le);
+    const uint32_t b = sample2(x, &le, lut);
     if (w < P::mat / 4) {
       *reinterpret_cast<uint16_t*>(sd + 4 * w) = (uint16_t)b;
       *reinterpret_cast<uint32_t*>(skS + 8 * w) = le;
@@ -298,6 +320,8 @@ __device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __
                                                 uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
                                                 uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
                                                 const uint8_t* __restrict__ pkh) {
+  __shared__ __align__(16) uint8_t lut_lds[32768];  // staged by the one wave of a workgroup that runs this
+  const uint8_t* lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
   FR_PAIR_SETUP
   constexpr int RW = P::chain_rate, LS = P::ls;
   const uint8_t* key = keys + (size_t)inst * key_stride;
@@ -324,7 +348,7 @@ __device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __
   uint8_t* pd = reinterpret_cast<uint8_t*>(epp8 + (size_t)inst * 16) + 2 * p;
   shake_seed_squeeze2<P, (2 * P::mat + 64) / 4>(p, 0x96, seed, [&](int w, uint32_t x) {
     uint32_t le;
-    const uint16_t b = (uint16_t)sample2<P>(x, &le);
+    const uint16_t b = (uint16_t)sample2(x, &le, lut);
     if (w < P::mat / 4) *reinterpret_cast<uint16_t*>(sd + 4 * w) = b;
     else if (w < 2 * P::mat / 4) *reinterpret_cast<uint16_t*>(ed + 4 * (w - P::mat / 4)) = b;
     else *reinterpret_cast<uint16_t*>(pd + 4 * (w - 2 * P::mat / 4)) = b;
