@@ -206,12 +206,12 @@ __device__ __forceinline__ const uint8_t* stage_sample_lut(uint8_t* lds, int lan
   return lds;
 }
 
-// Two samples from the 32-bit half of a squeezed word (its two 16-bit draws):
-// the int8 pair (low 16 bits) and the little-endian u16 pair (full 32 bits).
-// Magnitudes from the staged table; the sign is applied per 16-bit half.
+// Two samples from the 32-bit half x of a squeezed word (its two 16-bit
+// draws) given their table magnitudes e (low / high 16 bits): the int8 pair
+// (returned, low 16 bits) and the little-endian u16 pair (le).  The sign is
+// applied per 16-bit half.
 typedef unsigned short fr_u16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t* le, const uint8_t* lut) {
-  const uint32_t e = (uint32_t)lut[(x >> 1) & 0x7FFF] | ((uint32_t)lut[x >> 17] << 16);
+__device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t e, uint32_t* le) {
   const fr_u16x2 sm = __builtin_bit_cast(fr_u16x2, x & 0x00010001u);
   const fr_u16x2 ev = __builtin_bit_cast(fr_u16x2, e);
   const uint32_t v = __builtin_bit_cast(uint32_t, (ev ^ ((fr_u16x2){0, 0} - sm)) + sm);
@@ -219,11 +219,27 @@ __device__ __forceinline__ uint32_t sample2(uint32_t x, uint32_t* le, const uint
   return __builtin_amdgcn_perm(v, v, 0x0C0C0200u);  // bytes 0 and 2: the int8 pair
 }
 
+// The CNT words of one squeezed block: this lane's halves, then all table
+// reads of the block (so their LDS latency is paid once per block), then
+// f(w, half, magnitudes) per word.
+template <int CNT, class F>
+__device__ __forceinline__ void sample_block(const uint32_t a[25], uint32_t p, const uint8_t* lut, int w0, F&& f) {
+  uint32_t x[CNT], e[CNT];
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) x[i] = k2::out_half(a[i], p);
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) e[i] = (uint32_t)lut[(x[i] >> 1) & 0x7FFF] | ((uint32_t)lut[x[i] >> 17] << 16);
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) f(w0 + i, x[i], e[i]);
+}
+
 // SHAKE(dom || seed[sec bytes]) on a lane pair, squeezing OUTW words; f(w,
-// half) receives this lane's 32-bit half (low half for the even lane, high
-// for the odd lane) of every output word w.
+// half, e) receives this lane's 32-bit half (low half for the even lane, high
+// for the odd lane) of every output word w and the sampler table's
+// magnitudes of its two 16-bit draws (sample_block).
 template <class P, int OUTW, class F>
-__device__ __forceinline__ void shake_seed_squeeze2(uint32_t p, uint64_t dom, const uint64_t seed[P::ls], F&& f) {
+__device__ __forceinline__ void shake_seed_squeeze2(uint32_t p, uint64_t dom, const uint64_t seed[P::ls],
+                                                    const uint8_t* lut, F&& f) {
   constexpr int RW = P::chain_rate, LS = P::ls;
   uint64_t m[LS + 1];
   m[0] = dom | (seed[0] << 8);
@@ -238,12 +254,10 @@ __device__ __forceinline__ void shake_seed_squeeze2(uint32_t p, uint64_t dom, co
   k2::keccakf2(a, p);
   constexpr int NB = OUTW / RW, REM = OUTW % RW;
   for (int b = 0; b < NB; ++b) {
-#pragma unroll
-    for (int i = 0; i < RW; ++i) f(b * RW + i, k2::out_half(a[i], p));
+    sample_block<RW>(a, p, lut, b * RW, f);
     k2::keccakf2(a, p);
   }
-#pragma unroll
-  for (int i = 0; i < REM; ++i) f(NB * RW + i, k2::out_half(a[i], p));
+  if constexpr (REM > 0) sample_block<REM>(a, p, lut, NB * RW, f);
 }
 
 // ----------------------------------------------------------------- keygen chains
@@ -280,9 +294,9 @@ __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __r
   uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* skS = q + P::sk_s + 4 * p;
-  shake_seed_squeeze2<P, 2 * P::mat / 4>(p, 0x5F, seed, [&](int w, uint32_t x) {
+  shake_seed_squeeze2<P, 2 * P::mat / 4>(p, 0x5F, seed, lut, [&](int w, uint32_t x, uint32_t e) {
     uint32_t le;
-    const uint32_t b = sample2(x, &le, lut);
+    const uint32_t b = sample2(x, e, &le);
     if (w < P::mat / 4) {
       *reinterpret_cast<uint16_t*>(sd + 4 * w) = (uint16_t)b;
       *reinterpret_cast<uint32_t*>(skS + 8 * w) = le;
@@ -346,9 +360,9 @@ __device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __
   uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* pd = reinterpret_cast<uint8_t*>(epp8 + (size_t)inst * 16) + 2 * p;
-  shake_seed_squeeze2<P, (2 * P::mat + 64) / 4>(p, 0x96, seed, [&](int w, uint32_t x) {
+  shake_seed_squeeze2<P, (2 * P::mat + 64) / 4>(p, 0x96, seed, lut, [&](int w, uint32_t x, uint32_t e) {
     uint32_t le;
-    const uint16_t b = (uint16_t)sample2(x, &le, lut);
+    const uint16_t b = (uint16_t)sample2(x, e, &le);
     if (w < P::mat / 4) *reinterpret_cast<uint16_t*>(sd + 4 * w) = b;
     else if (w < 2 * P::mat / 4) *reinterpret_cast<uint16_t*>(ed + 4 * (w - P::mat / 4)) = b;
     else *reinterpret_cast<uint16_t*>(pd + 4 * (w - 2 * P::mat / 4)) = b;
