@@ -29,6 +29,7 @@
 // terms every i32 partial sum stays below 2^21 in magnitude.
 #include "common.hpp"
 #include "keccak.hpp"
+#include "frodo_sampler.hpp"
 #include "sponge.hpp"
 #include "keccak2.hpp"
 #include "prof.hpp"
@@ -46,52 +47,25 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // the serial chains and the CDF table; A rows are SHAKE128 in all three.
 // threads x passes covers the n rows of A in the fused kernels (one row per
 // lane and pass); the shapes keep 14-16 waves per CU resident.
-// cdf: the sampler's CDF table without its final entry (FrodoKEM976-GPU/include/params.h:24
-// for 976; the round-3 specification's tables for 640 / 1344).
 template <int N>
 struct FrodoSet;
 template <>
 struct FrodoSet<640> {
-  static constexpr int n = 640, logq = 15, b = 2, sec = 16, chain_rate = kRateShake128, threads = 320, passes = 2,
-                       ncdf = 12;
-  static constexpr uint16_t cdf[ncdf] = {4643, 13363, 20579, 25843, 29227, 31145,
-                                         32103, 32525, 32689, 32745, 32762, 32766};
+  static constexpr int n = 640, logq = 15, b = 2, sec = 16, chain_rate = kRateShake128, threads = 320, passes = 2;
 };
 template <>
 struct FrodoSet<976> {
-  static constexpr int n = 976, logq = 16, b = 3, sec = 24, chain_rate = kRateShake256, threads = 1024, passes = 1,
-                       ncdf = 10;
-  static constexpr uint16_t cdf[ncdf] = {5638, 15915, 23689, 28571, 31116, 32217, 32613, 32731, 32760, 32766};
+  static constexpr int n = 976, logq = 16, b = 3, sec = 24, chain_rate = kRateShake256, threads = 1024, passes = 1;
 };
 template <>
 struct FrodoSet<1344> {
-  static constexpr int n = 1344, logq = 16, b = 4, sec = 32, chain_rate = kRateShake256, threads = 448, passes = 3,
-                       ncdf = 6;
-  static constexpr uint16_t cdf[ncdf] = {9142, 23462, 30338, 32361, 32725, 32765};
+  static constexpr int n = 1344, logq = 16, b = 4, sec = 32, chain_rate = kRateShake256, threads = 448, passes = 3;
 };
 
-// The CDF sampler (FrodoKEM976-GPU/src/sample_pack.cu:13-32) on a 16-bit
-// draw r: magnitude e = #{j : cdf[j] < r >> 1}, sign = r & 1.  e depends on
-// the 15 bits r >> 1 only, so the serial chains read it from a 32 KB table
-// (one byte per value, built here at compile time as the step function
-// e = k on cdf[k-1] < t <= cdf[k]) staged in LDS.
-template <class S>
-struct alignas(16) SampleLut {
-  uint8_t e[32768];
-};
-template <class S>
-constexpr SampleLut<S> make_sample_lut() {
-  SampleLut<S> l{};
-  int t = 0;
-  for (int k = 0; k <= S::ncdf; ++k) {
-    const int hi = k < S::ncdf ? S::cdf[k] : 32767;
-    for (; t <= hi; ++t) l.e[t] = (uint8_t)k;
-  }
-  return l;
-}
-__device__ const SampleLut<FrodoSet<640>> kSampleLut640 = make_sample_lut<FrodoSet<640>>();
-__device__ const SampleLut<FrodoSet<976>> kSampleLut976 = make_sample_lut<FrodoSet<976>>();
-__device__ const SampleLut<FrodoSet<1344>> kSampleLut1344 = make_sample_lut<FrodoSet<1344>>();
+// Sampler magnitude tables (frodo_sampler.hpp), staged into LDS by the chains.
+__device__ const SampleLut<640> kSampleLut640 = make_sample_lut<640>();
+__device__ const SampleLut<976> kSampleLut976 = make_sample_lut<976>();
+__device__ const SampleLut<1344> kSampleLut1344 = make_sample_lut<1344>();
 template <int N>
 __device__ __forceinline__ const uint8_t* sample_lut_global() {
   return N == 640 ? kSampleLut640.e : N == 976 ? kSampleLut976.e : kSampleLut1344.e;

@@ -5,7 +5,8 @@ in a two-lane host emulation (partner exchange = swapping the two lanes) and
 compared with the oracle's permutation; the Saber Toeplitz helpers in
 saber_poly.hpp (the dot2 form and the int8 MFMA tile walk, whose matrix
 instructions are replayed as 16 x 16 x 64 products) are compared with the
-oracle's negacyclic product.
+oracle's negacyclic product; the Frodo sampler's magnitude tables
+(frodo_sampler.hpp) are compared with the oracle's sampler on every draw.
 """
 import os
 import shutil
@@ -45,6 +46,12 @@ def test_saber_toeplitz_product_matches_oracle(tmp_path, oracle):
 @pytest.mark.skipif(CLANGXX is None, reason="no clang++")
 def test_saber_mfma_toeplitz_tiling_matches_oracle(tmp_path, oracle):
     rc, out = _build_and_run("toep8_emul.cpp", tmp_path, oracle, cxx=CLANGXX)
+    assert rc == 0 and out.strip() == "OK", out
+
+
+@pytest.mark.skipif(CXX is None, reason="no host C++ compiler")
+def test_frodo_sampler_tables_match_the_oracle_on_every_draw(tmp_path, oracle):
+    rc, out = _build_and_run("sampler_emul.cpp", tmp_path, oracle)
     assert rc == 0 and out.strip() == "OK", out
 
 
