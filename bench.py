@@ -128,66 +128,57 @@ KERNEL_WORK = {
 #   bench.py --schedule 1x1 --calibrate-copy --skip-mixed --skip-serving --cpu-seconds 0 --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99969, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129318824, 256245767),
-    ('frodo_as_fused,1344', 8192): (177543890, 353387001),
-    ('frodo_as_fused,640', 8192): (85277792, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129303129, 256247536),
-    ('frodo_dec_mu', 8192): (269979306, 262144),
-    ('frodo_dec_mu,1344', 8192): (385522590, 262144),
-    ('frodo_dec_mu,640', 8192): (222787490, 262144),
-    ('frodo_encdec_pre<false>', 8192): (74509285, 128712704),
-    ('frodo_encdec_pre<true>', 8192): (1400294, 128712704),
-    ('frodo_keygen_post', 8192): (129120725, 262144),
-    ('frodo_keygen_pre', 8192): (960835, 256995953),
-    ('frodo_keygen_pre,1344', 8192): (1000856, 353246956),
-    ('frodo_keygen_pre,640', 8192): (984449, 168558592),
-    ('frodo_pipe_ssd', 8192): (4397398, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4516983, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5513385, 131072),
-    ('frodo_pipe_x1', 8192): (129856256, 128974862),
-    ('frodo_pipe_x1,1344', 8192): (178009968, 177340620),
-    ('frodo_pipe_x1,640', 8192): (80564103, 84934656),
-    ('frodo_pipe_x2', 8192): (131431834, 130557557),
-    ('frodo_pipe_x2,1344', 8192): (179681082, 178849024),
-    ('frodo_pipe_x2,640', 8192): (83173132, 86442060),
-    ('frodo_sa_fused<false,aes>', 8192): (259476876, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260565728, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356859845, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (167096741, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388303151, 262144),
-    ('frodo_sa_fused<true>', 8192): (389528327, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (534041910, 265785),
-    ('frodo_sa_fused<true>,640', 8192): (247688719, 262214),
-    ('frodo_ss', 8192): (129973349, 196608),
-    ('sab_dec_hash1', 4096): (880596, 18871978),
-    ('sab_dec_inner', 4096): (9888286, 131072),
-    ('sab_enc_hash1', 4096): (4574552, 19044805),
-    ('sab_hash2', 4096): (5151969, 131072),
-    ('sab_k', 4096): (1015650, 131072),
-    ('sab_k,l2', 4096): (1015650, 131072),
-    ('sab_k,l4', 4096): (1007459, 131072),
-    ('sab_keygen_hash1', 4096): (630132, 18995267),
-    ('sab_keygen_hash2', 4096): (4208081, 131072),
-    ('sab_pipe_x0', 4096): (648196, 49613184),
-    ('sab_pipe_x0,l2', 4096): (637222, 23499282),
-    ('sab_pipe_x0,l4', 4096): (638478, 85369493),
-    ('sab_pipe_x1', 4096): (4686168, 3670016),
-    ('sab_pipe_x1,l2', 4096): (3522052, 3145728),
-    ('sab_pipe_x1,l4', 4096): (6020187, 3670016),
-    ('sab_pipe_x2', 4096): (11483404, 3801088),
-    ('sab_pipe_x2,l2', 4096): (8366831, 3276800),
-    ('sab_pipe_x2,l4', 4096): (14799626, 3801088),
-    ('sab_poly<kSabEnc>', 4096): (23853275, 4456448),
-    ('sab_poly<kSabEnc>,l2', 4096): (13084692, 3014656),
-    ('sab_poly<kSabEnc>,l4', 4096): (36697331, 6029312),
-    ('sab_poly<kSabKeygen>', 4096): (18988198, 12978950),
-    ('sab_poly<kSabKeygen>,l2', 4096): (9541177, 8650863),
-    ('sab_poly<kSabKeygen>,l4', 4096): (30527004, 17326191),
-    ('sab_poly<kSabReenc>', 4096): (28972740, 131072),
-    ('sab_poly<kSabReenc>,l2', 4096): (16494083, 131072),
-    ('sab_poly<kSabReenc>,l4', 4096): (46136650, 3276800),
+    ('frodo_as_fused', 8192): (129247160, 256245760),
+    ('frodo_as_fused,1344', 8192): (177470930, 353383206),
+    ('frodo_as_fused,640', 8192): (85198535, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129277222, 256247672),
+    ('frodo_dec_mu', 8192): (269964549, 262144),
+    ('frodo_dec_mu,1344', 8192): (385413614, 262144),
+    ('frodo_dec_mu,640', 8192): (223176320, 262144),
+    ('frodo_encdec_pre<false>', 8192): (74681119, 128713760),
+    ('frodo_encdec_pre<true>', 8192): (1572326, 128713658),
+    ('frodo_keygen_post', 8192): (129124665, 262144),
+    ('frodo_keygen_pre', 8192): (1135907, 256960622),
+    ('frodo_keygen_pre,1344', 8192): (1230976, 353245606),
+    ('frodo_keygen_pre,640', 8192): (1061525, 168559155),
+    ('frodo_pipe_ssd', 8192): (4397532, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4516301, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5504338, 131072),
+    ('frodo_pipe_x1', 8192): (130032172, 128975936),
+    ('frodo_pipe_x1,1344', 8192): (178579016, 177340454),
+    ('frodo_pipe_x1,640', 8192): (80677472, 84934656),
+    ('frodo_pipe_x2', 8192): (131601413, 130585436),
+    ('frodo_pipe_x2,1344', 8192): (179934521, 178853555),
+    ('frodo_pipe_x2,640', 8192): (83290114, 86443059),
+    ('frodo_sa_fused<false,aes>', 8192): (259478106, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260450373, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356739832, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166973600, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388308377, 262144),
+    ('frodo_sa_fused<true>', 8192): (389408716, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533927124, 265587),
+    ('frodo_sa_fused<true>,640', 8192): (247575511, 262259),
+    ('frodo_ss', 8192): (129980623, 196608),
+    ('sab_dec_hash1', 4096): (888744, 18872977),
+    ('sab_dec_inner', 4096): (9888586, 131072),
+    ('sab_enc_hash1', 4096): (4581855, 19046651),
+    ('sab_hash2', 4096): (5160317, 131072),
+    ('sab_k', 4096): (1015681, 131072),
+    ('sab_k,l2', 4096): (1011586, 131072),
+    ('sab_k,l4', 4096): (1015681, 131072),
+    ('sab_keygen_hash1', 4096): (630199, 18996050),
+    ('sab_keygen_hash2', 4096): (4201973, 131072),
+    ('sab_pipe_x0', 4096): (635970, 49612310),
+    ('sab_pipe_x0,l2', 4096): (627311, 23495632),
+    ('sab_pipe_x0,l4', 4096): (638962, 85360004),
+    ('sab_pipe_x1', 4096): (4685240, 3670016),
+    ('sab_pipe_x1,l2', 4096): (3523145, 3145728),
+    ('sab_pipe_x1,l4', 4096): (6016360, 3670016),
+    ('sab_pipe_x2', 4096): (11483754, 3801088),
+    ('sab_pipe_x2,l2', 4096): (8361967, 3276800),
+    ('sab_pipe_x2,l4', 4096): (14809292, 3801088),
 }
 
 
