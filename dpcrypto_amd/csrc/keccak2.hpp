@@ -145,10 +145,22 @@ inline __attribute__((always_inline)) DPKEM_HD uint32_t deint(uint64_t w, uint32
   return squeeze_bits16(lo) | (squeeze_bits16(hi) << 16);
 }
 // 32-bit half (low if p == 0, high if p == 1) of the word whose even half is
-// e and odd half is o.
+// e and odd half is o.  Device form: one v_perm interleaves the bytes of the
+// two 16-bit pieces ([e0, o0, e1, o1]), then three delta swaps interleave
+// nibbles, bit pairs and bits inside each byte (an outer perfect shuffle).
 inline __attribute__((always_inline)) DPKEM_HD uint32_t reint_half(uint32_t e, uint32_t o, uint32_t p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t x = __builtin_amdgcn_perm(o, e, p ? 0x07030602u : 0x05010400u);
+  uint32_t t = ((x >> 4) ^ x) & 0x00F000F0u;
+  x ^= t ^ (t << 4);
+  t = ((x >> 2) ^ x) & 0x0C0C0C0Cu;
+  x ^= t ^ (t << 2);
+  t = ((x >> 1) ^ x) & 0x22222222u;
+  return x ^ t ^ (t << 1);
+#else
   const uint32_t sh = 16 * p;
   return spread_bits16(e >> sh) | (spread_bits16(o >> sh) << 1);
+#endif
 }
 inline __attribute__((always_inline)) DPKEM_HD uint64_t reint_full(uint32_t e, uint32_t o) {
   return (uint64_t)reint_half(e, o, 0) | ((uint64_t)reint_half(e, o, 1) << 32);
