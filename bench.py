@@ -130,55 +130,55 @@ KERNEL_WORK = {
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
 # FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129247160, 256245760),
-    ('frodo_as_fused,1344', 8192): (177470930, 353383206),
-    ('frodo_as_fused,640', 8192): (85198535, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129277222, 256247672),
-    ('frodo_dec_mu', 8192): (269964549, 262144),
-    ('frodo_dec_mu,1344', 8192): (385413614, 262144),
-    ('frodo_dec_mu,640', 8192): (223176320, 262144),
-    ('frodo_encdec_pre<false>', 8192): (74681119, 128713760),
-    ('frodo_encdec_pre<true>', 8192): (1572326, 128713658),
-    ('frodo_keygen_post', 8192): (129124665, 262144),
-    ('frodo_keygen_pre', 8192): (1135907, 256960622),
-    ('frodo_keygen_pre,1344', 8192): (1230976, 353245606),
-    ('frodo_keygen_pre,640', 8192): (1061525, 168559155),
-    ('frodo_pipe_ssd', 8192): (4397532, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4516301, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5504338, 131072),
-    ('frodo_pipe_x1', 8192): (130032172, 128975936),
-    ('frodo_pipe_x1,1344', 8192): (178579016, 177340454),
-    ('frodo_pipe_x1,640', 8192): (80677472, 84934656),
-    ('frodo_pipe_x2', 8192): (131601413, 130585436),
-    ('frodo_pipe_x2,1344', 8192): (179934521, 178853555),
-    ('frodo_pipe_x2,640', 8192): (83290114, 86443059),
-    ('frodo_sa_fused<false,aes>', 8192): (259478106, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260450373, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356739832, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166973600, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388308377, 262144),
-    ('frodo_sa_fused<true>', 8192): (389408716, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533927124, 265587),
-    ('frodo_sa_fused<true>,640', 8192): (247575511, 262259),
-    ('frodo_ss', 8192): (129980623, 196608),
-    ('sab_dec_hash1', 4096): (888744, 18872977),
-    ('sab_dec_inner', 4096): (9888586, 131072),
-    ('sab_enc_hash1', 4096): (4581855, 19046651),
-    ('sab_hash2', 4096): (5160317, 131072),
-    ('sab_k', 4096): (1015681, 131072),
-    ('sab_k,l2', 4096): (1011586, 131072),
-    ('sab_k,l4', 4096): (1015681, 131072),
-    ('sab_keygen_hash1', 4096): (630199, 18996050),
-    ('sab_keygen_hash2', 4096): (4201973, 131072),
-    ('sab_pipe_x0', 4096): (635970, 49612310),
-    ('sab_pipe_x0,l2', 4096): (627311, 23495632),
-    ('sab_pipe_x0,l4', 4096): (638962, 85360004),
-    ('sab_pipe_x1', 4096): (4685240, 3670016),
-    ('sab_pipe_x1,l2', 4096): (3523145, 3145728),
-    ('sab_pipe_x1,l4', 4096): (6016360, 3670016),
-    ('sab_pipe_x2', 4096): (11483754, 3801088),
-    ('sab_pipe_x2,l2', 4096): (8361967, 3276800),
-    ('sab_pipe_x2,l4', 4096): (14809292, 3801088),
+    ('frodo_as_fused', 8192): (129241918, 256245760),
+    ('frodo_as_fused,1344', 8192): (177474575, 353370291),
+    ('frodo_as_fused,640', 8192): (85211764, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129305096, 256247264),
+    ('frodo_dec_mu', 8192): (269988421, 262144),
+    ('frodo_dec_mu,1344', 8192): (385447008, 262144),
+    ('frodo_dec_mu,640', 8192): (223062202, 262144),
+    ('frodo_encdec_pre<false>', 8192): (74598114, 128712818),
+    ('frodo_encdec_pre<true>', 8192): (1488709, 128712768),
+    ('frodo_keygen_post', 8192): (129115327, 262144),
+    ('frodo_keygen_pre', 8192): (1050718, 256981065),
+    ('frodo_keygen_pre,1344', 8192): (1168211, 353243264),
+    ('frodo_keygen_pre,640', 8192): (949487, 168558636),
+    ('frodo_pipe_ssd', 8192): (4389337, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4492748, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5504333, 131072),
+    ('frodo_pipe_x1', 8192): (129973502, 128975832),
+    ('frodo_pipe_x1,1344', 8192): (178177002, 177340416),
+    ('frodo_pipe_x1,640', 8192): (80573984, 84934656),
+    ('frodo_pipe_x2', 8192): (131515837, 130581571),
+    ('frodo_pipe_x2,1344', 8192): (179837926, 178851776),
+    ('frodo_pipe_x2,640', 8192): (83184524, 86442534),
+    ('frodo_sa_fused<false,aes>', 8192): (259477731, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260452186, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356739389, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166973389, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388307463, 262144),
+    ('frodo_sa_fused<true>', 8192): (389413180, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533926743, 265273),
+    ('frodo_sa_fused<true>,640', 8192): (247573176, 262355),
+    ('frodo_ss', 8192): (129971599, 196608),
+    ('sab_dec_hash1', 4096): (871431, 18874028),
+    ('sab_dec_inner', 4096): (9888582, 131072),
+    ('sab_enc_hash1', 4096): (4543965, 19046611),
+    ('sab_hash2', 4096): (5152121, 131072),
+    ('sab_k', 4096): (1003394, 131072),
+    ('sab_k,l2', 4096): (1003394, 131072),
+    ('sab_k,l4', 4096): (991107, 131072),
+    ('sab_keygen_hash1', 4096): (599469, 18995988),
+    ('sab_keygen_hash2', 4096): (4195825, 131072),
+    ('sab_pipe_x0', 4096): (606091, 49611095),
+    ('sab_pipe_x0,l2', 4096): (594494, 23495472),
+    ('sab_pipe_x0,l4', 4096): (607032, 85385794),
+    ('sab_pipe_x1', 4096): (4637114, 3670016),
+    ('sab_pipe_x1,l2', 4096): (3473996, 3145728),
+    ('sab_pipe_x1,l4', 4096): (5954913, 3670016),
+    ('sab_pipe_x2', 4096): (11445860, 3801088),
+    ('sab_pipe_x2,l2', 4096): (8329195, 3276800),
+    ('sab_pipe_x2,l4', 4096): (14772419, 3801088),
 }
 
 
