@@ -356,8 +356,9 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 //       GenMatrix from the seedA bytes of pk and wave 3 decaps' GenMatrix
 //       from those of sk (the same bytes, handed over through LDS);
 //   x1  wave 0: encaps' H(pk), then G and GenSecret; wave 1: keygen's H(pk) into sk; wave 2: encaps' m, handed
-//       to wave 0 through LDS; wave 3 deinterleaves pk into LDS ahead of
-//       wave 0 (stage2_message);
+//       to wave 0 through LDS, then encaps' and decaps' GenMatrix streams
+//       rewritten as plain words (plain_matrix_streams); wave 3
+//       deinterleaves pk into LDS ahead of wave 0 (stage2_message);
 //   x2  (after the encaps products, below sab_dec_inner) decaps' inner
 //       product and message chain beside encaps' K and decaps' H(ct),
 //       leaving one permutation (sab_k) after the re-encryption.
@@ -408,6 +409,28 @@ __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restr
       a, p, reinterpret_cast<uint32_t*>(dst + (size_t)inst * P::matrix_bytes));
 }
 
+// Encaps' and decaps' GenMatrix streams of this workgroup's 32 instances,
+// written in the sliced layout by sab_pipe_x0, rewritten in place as plain
+// 64-bit words (their products run after this launch and then skip the
+// re-interleave).  One wave, one instance at a time, 8-byte accesses.
+template <int L>
+__device__ __forceinline__ void plain_matrix_streams(int n, int lane, uint8_t* __restrict__ araw_e,
+                                                     uint8_t* __restrict__ araw_d) {
+  constexpr int MW = SaberP<L>::matrix_bytes / 8, J = (MW + 63) / 64;
+  const int first = (int)blockIdx.x * 32, cnt = min(32, n - first);
+#pragma unroll 1
+  for (int q = 0; q < 2 * cnt; ++q) {
+    uint2* w = reinterpret_cast<uint2*>((q < cnt ? araw_e : araw_d) + (size_t)(first + q % cnt) * SaberP<L>::matrix_bytes);
+    uint2 v[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (lane + 64 * j < MW) v[j] = w[lane + 64 * j];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (lane + 64 * j < MW) w[lane + 64 * j] = make_uint2(k2::reint_half(v[j].x, v[j].y, 0), k2::reint_half(v[j].x, v[j].y, 1));
+  }
+}
+
 template <int L>
 struct X1Lds {
   static constexpr int kPkBlocks = (SaberP<L>::pk / 8 + kRateSha3_256 - 1) / kRateSha3_256;
@@ -420,7 +443,8 @@ template <int L>
 __global__ void __launch_bounds__(256) sab_pipe_x1(int n, const uint8_t* __restrict__ pk,
                                                    const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ sk,
                                                    uint8_t* __restrict__ sraw_e, uint8_t* __restrict__ mbuf_e,
-                                                   uint8_t* __restrict__ krbuf_e) {
+                                                   uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ araw_e,
+                                                   uint8_t* __restrict__ araw_d) {
   using P = SaberP<L>;
   constexpr int PW = P::pk / 8;
   __shared__ X1Lds<L> S;
@@ -456,6 +480,7 @@ __global__ void __launch_bounds__(256) sab_pipe_x1(int n, const uint8_t* __restr
   if (wave == 2) {  // m handed to wave 0 through LDS (an LDS count, so that wave 0 never waits for wave 1)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if (lane == 0) __hip_atomic_fetch_add(&S.m_ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    plain_matrix_streams<L>(n, lane, araw_e, araw_d);
   }
   if (!live || wave != 0) return;
   k2::lds_wait_count(&S.m_ready, 1u);
@@ -544,7 +569,9 @@ __device__ __forceinline__ uint64_t lds_bytes64(const uint8_t* base, int off) {
 }
 
 // One instance's product on one wave (tid = its lane) with its own LDS S.
-template <int MODE, int L>
+// A_PLAIN: the matrix stream holds plain 64-bit words (sab_pipe_x1 rewrote
+// it), not the sliced halves of the squeeze.
+template <int MODE, int L, bool A_PLAIN = false>
 __device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L>& S, const int tid, const int inst) {
   using P = SaberP<L>;
   constexpr int NT = 64;
@@ -596,7 +623,9 @@ __device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L
     // 1. this block's bytes into LDS (sliced words rebuilt), then block j + 1's loads
 #pragma unroll
     for (int r = 0; r < MPT; ++r)
-      if (tid + r * NT < MW) S.u.k.mstage[tid + r * NT] = k2::reint_full(mv[r].x, mv[r].y);
+      if (tid + r * NT < MW)
+        S.u.k.mstage[tid + r * NT] = A_PLAIN ? (uint64_t)mv[r].x | ((uint64_t)mv[r].y << 32)
+                                             : k2::reint_full(mv[r].x, mv[r].y);
     if (tid < SW) S.u.k.sstage[tid] = k2::reint_full(sv.x, sv.y);
     if (HAS_B) {
 #pragma unroll
@@ -772,13 +801,13 @@ __device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L
 // IPW instances per workgroup, one wave each (instances-per-workgroup is a
 // tuning knob: it changes how many workgroups the dispatcher places and how
 // the waves spread over CUs, not the arithmetic).
-template <int MODE, int L, int IPW = 1>
+template <int MODE, int L, int IPW = 1, bool A_PLAIN = false>
 __global__ void __launch_bounds__(64 * IPW) __attribute__((amdgpu_waves_per_eu(4))) sab_poly(SabPolyArgs a) {
   __shared__ __align__(16) SabPolyLds<L> S[IPW];
   const int w = IPW == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int inst = (int)blockIdx.x * IPW + w;
   if (inst >= a.n) return;
-  sab_poly_body<MODE, L>(a, S[w], IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63), inst);
+  sab_poly_body<MODE, L, A_PLAIN>(a, S[w], IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63), inst);
 }
 
 // Decaps inner product v = b' . s and message recovery
@@ -1084,11 +1113,11 @@ const char* kname(const char* base, const char* l2, const char* l4) {
 #define SAB_NAME(base) kname<L>(base, base ",l2", base ",l4")
 
 // The product launch with `ipw` instances per workgroup (1, 2 or 4).
-template <int MODE, int L>
+template <int MODE, int L, bool A_PLAIN = false>
 static void poly_launch(const char* name, hipStream_t st, int n, const SabPolyArgs& a, int ipw) {
-  if (ipw == 4) DPKEM_LAUNCH(name, (sab_poly<MODE, L, 4>), dim3(blocks_for(n, 4)), dim3(256), st, a);
-  else if (ipw == 2) DPKEM_LAUNCH(name, (sab_poly<MODE, L, 2>), dim3(blocks_for(n, 2)), dim3(128), st, a);
-  else DPKEM_LAUNCH(name, (sab_poly<MODE, L, 1>), dim3(n), dim3(64), st, a);
+  if (ipw == 4) DPKEM_LAUNCH(name, (sab_poly<MODE, L, 4, A_PLAIN>), dim3(blocks_for(n, 4)), dim3(256), st, a);
+  else if (ipw == 2) DPKEM_LAUNCH(name, (sab_poly<MODE, L, 2, A_PLAIN>), dim3(blocks_for(n, 2)), dim3(128), st, a);
+  else DPKEM_LAUNCH(name, (sab_poly<MODE, L, 1, A_PLAIN>), dim3(n), dim3(64), st, a);
 }
 
 template <int L>
@@ -1221,13 +1250,14 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
                araw_d);
   SabPolyArgs ak{n, araw_k, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
   poly_launch<kSabKeygen, L>(SAB_NAME("sab_poly<kSabKeygen>"), st, n, ak, ipw);
-  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, dim3(256), st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e);
+  DPKEM_LAUNCH(SAB_NAME("sab_pipe_x1"), sab_pipe_x1<L>, g, dim3(256), st, n, pk, ec, sk, sraw_e, mbuf_e, krbuf_e,
+               araw_e, araw_d);
   SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
-  poly_launch<kSabEnc, L>(SAB_NAME("sab_poly<kSabEnc>"), st, n, ae, ipw);
+  poly_launch<kSabEnc, L, true>(SAB_NAME("sab_poly<kSabEnc>"), st, n, ae, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, dim3(blocks_for(n, kX2Group)), dim3(kX2Threads), st, n, sk, ct,
                mbuf_d, sraw_d, krbuf_d, krbuf_e, ss, hc_d);
   SabPolyArgs ad{n, araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, fail};
-  poly_launch<kSabReenc, L>(SAB_NAME("sab_poly<kSabReenc>"), st, n, ad, ipw);
+  poly_launch<kSabReenc, L, true>(SAB_NAME("sab_poly<kSabReenc>"), st, n, ad, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_k"), sab_k<L>, g, b64, st, n, hc_d, krbuf_d, sk, fail, ss2);
 }
 
