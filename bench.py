@@ -370,8 +370,20 @@ def profile_plan(torch, eng, scheme, n, steps, seed):
     return dict(kernels=stats, roundtrip_ok=ok)
 
 
+# Serial lane-pair permutations on the critical path of the pipelined chain
+# launches (the longest chain inside each launch), and the issue floor of
+# one such permutation on a lone wave: 24 rounds x 108 VALU ops, one VALU op
+# per 4 clocks per wave at 2.4 GHz.  A latency-bound launch cannot beat
+# CRITICAL_PERMS x this floor, whatever its VALU throughput.
+CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10, "sab_k": 1,
+                  "sab_pipe_x0,l2": 11, "sab_pipe_x1,l2": 10, "sab_pipe_x2,l2": 7, "sab_k,l2": 1,
+                  "sab_pipe_x0,l4": 41, "sab_pipe_x1,l4": 16, "sab_pipe_x2,l4": 12, "sab_k,l4": 1}
+LANE_PAIR_PERM_FLOOR_S = 24 * 108 * 4 / 2.4e9
+
+
 def roofline_from(stats, n):
-    """Dominant kernel by total time -> achieved VALU lane-ops/s vs peak."""
+    """Dominant kernel by total time -> achieved VALU lane-ops/s vs peak (and,
+    for a serial-chain launch, its time against the chain's issue floor)."""
     if not stats:
         return None
     name, (count, total_ms) = max(stats.items(), key=lambda kv: kv[1][1])
@@ -379,6 +391,12 @@ def roofline_from(stats, n):
     work = lane_ops(name, n)
     achieved = work / avg_s
     hbm = MEASURED_HBM_BYTES.get((name, n))
+    chain = CRITICAL_PERMS.get(name)
+    floor = None if chain is None else {
+        "critical_perms": chain,
+        "floor_ms": round(chain * LANE_PAIR_PERM_FLOOR_S * 1e3, 4),
+        "frac": round(chain * LANE_PAIR_PERM_FLOOR_S / avg_s, 4),
+        "what": "serial lane-pair permutations x 24 rounds x 108 VALU ops x 4 clocks at 2.4 GHz"}
     return {
         "kernel": name,
         "bound": "valu",
@@ -392,6 +410,7 @@ def roofline_from(stats, n):
         "avg_launch_ms": round(avg_s * 1e3, 4),
         "algorithmic_lane_ops_per_launch": work,
         "share_of_step": round(total_ms / sum(v[1] for v in stats.values()), 3),
+        "chain_issue_floor": floor,
     }
 
 
