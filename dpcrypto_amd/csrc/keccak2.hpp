@@ -221,7 +221,7 @@ __device__ __forceinline__ void keccakf2(uint32_t a[25], uint32_t p) {
   const uint32_t odd = 0u - p;
   ConstRc* rct = (ConstRc*)kRC2;
   asm volatile("" : "+s"(rct));
-#pragma unroll 2
+#pragma unroll 4
   for (int r = 0; r < 24; ++r) {
     const uint32_t re = rct[r].e, ro = rct[r].o;
     const uint32_t rc = re ^ ((re ^ ro) & odd);
