@@ -364,7 +364,7 @@ __device__ __forceinline__ void absorb2_staged(uint32_t a[25], uint32_t p, const
     lds_wait_count(ready, (uint32_t)(b + 1));
 #pragma unroll
     for (int i = 0; i < RW; ++i) a[i] ^= msg[(b * RW + i) * SLOTS + s];
-    keccakf2(a, p);
+    keccakf2_unrolled(a, p);
   }
   if (REM > 0) lds_wait_count(ready, (uint32_t)(NB + 1));
 #pragma unroll
