@@ -229,7 +229,7 @@ __device__ __forceinline__ void shake_seed_squeeze2(uint32_t p, uint64_t dom, co
   constexpr int NB = OUTW / RW, REM = OUTW % RW;
   for (int b = 0; b < NB; ++b) {
     sample_block<RW>(a, p, lut, b * RW, f);
-    k2::keccakf2(a, p);
+    k2::keccakf2_unrolled(a, p);
   }
   if constexpr (REM > 0) sample_block<REM>(a, p, lut, NB * RW, f);
 }
