@@ -130,55 +130,55 @@ KERNEL_WORK = {
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
 # FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129241975, 256245760),
-    ('frodo_as_fused,1344', 8192): (177474854, 353359788),
-    ('frodo_as_fused,640', 8192): (85211835, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129306368, 256247272),
-    ('frodo_dec_mu', 8192): (269959491, 262144),
-    ('frodo_dec_mu,1344', 8192): (385438524, 262144),
-    ('frodo_dec_mu,640', 8192): (223071992, 262144),
-    ('frodo_encdec_pre<false>', 8192): (74598605, 128712795),
-    ('frodo_encdec_pre<true>', 8192): (1488711, 128712917),
-    ('frodo_keygen_post', 8192): (129115512, 262144),
-    ('frodo_keygen_pre', 8192): (1050168, 256983726),
-    ('frodo_keygen_pre,1344', 8192): (1167983, 353243788),
-    ('frodo_keygen_pre,640', 8192): (948388, 168558592),
-    ('frodo_pipe_ssd', 8192): (4389343, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4492754, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5504341, 131072),
-    ('frodo_pipe_x1', 8192): (129973475, 128975758),
-    ('frodo_pipe_x1,1344', 8192): (178177257, 177340416),
-    ('frodo_pipe_x1,640', 8192): (80574842, 84934656),
-    ('frodo_pipe_x2', 8192): (131515911, 130581045),
-    ('frodo_pipe_x2,1344', 8192): (179838184, 178851244),
-    ('frodo_pipe_x2,640', 8192): (83184592, 86442572),
-    ('frodo_sa_fused<false,aes>', 8192): (259478838, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260452986, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356739490, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166973884, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388308146, 262144),
-    ('frodo_sa_fused<true>', 8192): (389413737, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533927353, 265452),
-    ('frodo_sa_fused<true>,640', 8192): (247573530, 262169),
-    ('frodo_ss', 8192): (129972258, 196608),
-    ('sab_dec_hash1', 4096): (871315, 18873797),
-    ('sab_dec_inner', 4096): (9888592, 131072),
-    ('sab_enc_hash1', 4096): (4543971, 19047006),
-    ('sab_hash2', 4096): (5152129, 131072),
-    ('sab_k', 4096): (1003395, 131072),
-    ('sab_k,l2', 4096): (1003395, 131072),
-    ('sab_k,l4', 4096): (1003395, 131072),
-    ('sab_keygen_hash1', 4096): (599433, 18996163),
-    ('sab_keygen_hash2', 4096): (4195831, 131072),
-    ('sab_pipe_x0', 4096): (617953, 49611078),
-    ('sab_pipe_x0,l2', 4096): (606768, 23495274),
-    ('sab_pipe_x0,l4', 4096): (606758, 85360978),
-    ('sab_pipe_x1', 4096): (35343461, 34340864),
-    ('sab_pipe_x1,l2', 4096): (17120219, 16777216),
-    ('sab_pipe_x1,l4', 4096): (61295312, 58195968),
-    ('sab_pipe_x2', 4096): (11446900, 3801088),
-    ('sab_pipe_x2,l2', 4096): (8312825, 3276800),
-    ('sab_pipe_x2,l4', 4096): (14767472, 3801088),
+    ('frodo_as_fused', 8192): (129242980, 256245760),
+    ('frodo_as_fused,1344', 8192): (177466062, 353369376),
+    ('frodo_as_fused,640', 8192): (85209113, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129300542, 256247600),
+    ('frodo_dec_mu', 8192): (269999039, 262144),
+    ('frodo_dec_mu,1344', 8192): (385411546, 262144),
+    ('frodo_dec_mu,640', 8192): (223113959, 262144),
+    ('frodo_encdec_pre<false>', 8192): (75005374, 128712758),
+    ('frodo_encdec_pre<true>', 8192): (1841888, 128712709),
+    ('frodo_keygen_post', 8192): (129166675, 262144),
+    ('frodo_keygen_pre', 8192): (1399560, 256981435),
+    ('frodo_keygen_pre,1344', 8192): (1517792, 353244281),
+    ('frodo_keygen_pre,640', 8192): (1297000, 168558681),
+    ('frodo_pipe_ssd', 8192): (4416847, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4531517, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5534880, 131072),
+    ('frodo_pipe_x1', 8192): (130399294, 128975043),
+    ('frodo_pipe_x1,1344', 8192): (178633522, 177340416),
+    ('frodo_pipe_x1,640', 8192): (81022630, 84934656),
+    ('frodo_pipe_x2', 8192): (131945662, 130671466),
+    ('frodo_pipe_x2,1344', 8192): (180284004, 178864934),
+    ('frodo_pipe_x2,640', 8192): (83619831, 86442035),
+    ('frodo_sa_fused<false,aes>', 8192): (259462231, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260444595, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356728479, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166968627, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388295710, 262144),
+    ('frodo_sa_fused<true>', 8192): (389401298, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533910528, 265043),
+    ('frodo_sa_fused<true>,640', 8192): (247561526, 262278),
+    ('frodo_ss', 8192): (130023393, 196608),
+    ('sab_dec_hash1', 4096): (1561356, 18878610),
+    ('sab_dec_inner', 4096): (9888280, 131072),
+    ('sab_enc_hash1', 4096): (5316928, 19052151),
+    ('sab_hash2', 4096): (5234895, 131072),
+    ('sab_k', 4096): (1031007, 131072),
+    ('sab_k,l2', 4096): (1019744, 131072),
+    ('sab_k,l4', 4096): (1031007, 131072),
+    ('sab_keygen_hash1', 4096): (1289484, 19007597),
+    ('sab_keygen_hash2', 4096): (4250985, 131072),
+    ('sab_pipe_x0', 4096): (1304631, 49617159),
+    ('sab_pipe_x0,l2', 4096): (1287147, 23513677),
+    ('sab_pipe_x0,l4', 4096): (1305325, 85363687),
+    ('sab_pipe_x1', 4096): (36109313, 34340864),
+    ('sab_pipe_x1,l2', 4096): (17790287, 16777216),
+    ('sab_pipe_x1,l4', 4096): (62075284, 58195968),
+    ('sab_pipe_x2', 4096): (12493923, 3801088),
+    ('sab_pipe_x2,l2', 4096): (9377353, 3276800),
+    ('sab_pipe_x2,l4', 4096): (15819546, 3801088),
 }
 
 
