@@ -231,8 +231,9 @@ __device__ __forceinline__ void keccakf2(uint32_t a[25], uint32_t p) {
 
 // The same permutation fully unrolled: round constants become immediates (no
 // scalar loads or loop branches on the chain), at ~20 KB of code per call
-// site, so it is used only where one call site carries a whole serial chain
-// (the GenMatrix squeeze: 3-4% off those launches).
+// site, so it is used only in the block loops that carry a whole serial
+// chain (the sliced squeezes, the staged absorbs, Frodo's noise squeeze):
+// 2-6% off those launches.
 __device__ __forceinline__ void keccakf2_unrolled(uint32_t a[25], uint32_t p) {
   uint32_t sh[32];
 #pragma unroll
