@@ -131,3 +131,27 @@ def test_autotune_times_at_least_one_candidate_within_a_spent_budget():
     best, tried = autotune.tune(_FakeTorch, None, "test", lambda *c: _FakePlan(1.0), candidates=[(1, 1), (2, 1)],
                                 use_cache=False, budget_s=-1.0)
     assert best == (1, 1) and list(tried) == [(1, 1)]
+
+
+def test_bench_roofline_arithmetic():
+    """bench.roofline_from: achieved = algorithmic lane-ops per launch / the
+    average launch time, against the VALU peak; a serial-chain launch also
+    against its issue floor (critical permutations x 24 rounds x 108 ops x 4
+    clocks at 2.4 GHz)."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    n = 4096
+    r = bench.roofline_from({"sab_pipe_x0": (10, 10 * 0.2), "sab_poly<kSabEnc>": (10, 10 * 0.05)}, n)
+    assert r["kernel"] == "sab_pipe_x0" and r["avg_launch_ms"] == pytest.approx(0.2)
+    perms, macs = bench.KERNEL_WORK["sab_pipe_x0"]
+    work = n * (perms * bench.KECCAK_LANE_OPS + macs / 2.0)
+    assert r["algorithmic_lane_ops_per_launch"] == pytest.approx(work)
+    assert r["frac"] == pytest.approx(work / 0.2e-3 / bench.VALU_PEAK, abs=1e-4)
+    assert r["share_of_step"] == pytest.approx(0.8)
+    floor_s = 24 * 24 * 108 * 4 / 2.4e9
+    assert r["chain_issue_floor"]["floor_ms"] == pytest.approx(floor_s * 1e3, abs=1e-4)
+    assert r["chain_issue_floor"]["frac"] == pytest.approx(floor_s / 0.2e-3, abs=1e-4)
+    # a throughput kernel has no chain floor
+    r2 = bench.roofline_from({"frodo_sa_fused<false>": (1, 12.0)}, 8192)
+    assert r2["chain_issue_floor"] is None and 0 < r2["frac"] < 1
