@@ -78,15 +78,15 @@ KERNEL_WORK = {
     "sab_pipe_x0": (1 + 23 + 5 + 23 + 23, 0),          # keygen seedA, GenMatrix, GenSecret; encaps' / decaps' GenMatrix
     "sab_pipe_x1": (8 + 1 + 1 + 5 + 8, 0),             # encaps' H(pk), m, G, GenSecret; keygen's H(pk)
     "sab_pipe_x2": (10 + 9 + 1 + 5, 3 * 65536),        # encaps' K; decaps' H(ct), G, GenSecret; decaps' inner product
-    "sab_k": (1, 0),
+    "sab_reenc_k": (1, 12 * 65536),                      # re-encryption product, then decaps' K
     "sab_pipe_x0,l2": (1 + 10 + 4 + 10 + 10, 0),
     "sab_pipe_x1,l2": (5 + 1 + 1 + 4 + 5, 0),
     "sab_pipe_x2,l2": (7 + 6 + 1 + 4, 2 * 65536),
-    "sab_k,l2": (1, 0),
+    "sab_reenc_k,l2": (1, 6 * 65536),
     "sab_pipe_x0,l4": (1 + 40 + 5 + 40 + 40, 0),
     "sab_pipe_x1,l4": (10 + 1 + 1 + 5 + 10, 0),
     "sab_pipe_x2,l4": (12 + 11 + 1 + 5, 4 * 65536),
-    "sab_k,l4": (1, 0),
+    "sab_reenc_k,l4": (1, 20 * 65536),
     "sab_poly<kSabKeygen>,l2": (0, 4 * 65536),
     "sab_poly<kSabEnc>,l2": (0, 6 * 65536),
     "sab_poly<kSabReenc>,l2": (0, 6 * 65536),
@@ -375,9 +375,9 @@ def profile_plan(torch, eng, scheme, n, steps, seed):
 # one such permutation on a lone wave: 24 rounds x 108 VALU ops, one VALU op
 # per 4 clocks per wave at 2.4 GHz.  A latency-bound launch cannot beat
 # CRITICAL_PERMS x this floor, whatever its VALU throughput.
-CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10, "sab_k": 1,
-                  "sab_pipe_x0,l2": 11, "sab_pipe_x1,l2": 10, "sab_pipe_x2,l2": 7, "sab_k,l2": 1,
-                  "sab_pipe_x0,l4": 41, "sab_pipe_x1,l4": 16, "sab_pipe_x2,l4": 12, "sab_k,l4": 1}
+CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10,
+                  "sab_pipe_x0,l2": 11, "sab_pipe_x1,l2": 10, "sab_pipe_x2,l2": 7,
+                  "sab_pipe_x0,l4": 41, "sab_pipe_x1,l4": 16, "sab_pipe_x2,l4": 12}
 LANE_PAIR_PERM_FLOOR_S = 24 * 108 * 4 / 2.4e9
 
 

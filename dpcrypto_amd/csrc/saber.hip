@@ -361,7 +361,8 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 //       deinterleaves pk into LDS ahead of wave 0 (stage2_message);
 //   x2  (after the encaps products, below sab_dec_inner) decaps' inner
 //       product and message chain beside encaps' K and decaps' H(ct),
-//       leaving one permutation (sab_k) after the re-encryption.
+//       leaving one permutation (decaps' K) to the re-encryption launch
+//       (sab_reenc_k).
 // Every op still computes all of its own hashes; only their order changes.
 template <int L>
 __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restrict__ kc, uint8_t* __restrict__ pk,
@@ -487,14 +488,6 @@ __global__ void __launch_bounds__(256) sab_pipe_x1(int n, const uint8_t* __restr
 #pragma unroll
   for (int i = 0; i < 4; ++i) g[i] = S.mh[i][lane];
   enc_g_chain<L>(inst, p, g, sraw_e, krbuf_e);
-}
-
-// decaps' K from the stored H(ct): K = SHA3-256((fail ? z : kr[0:32]) || H(ct))
-template <int L>
-__global__ void __launch_bounds__(64) sab_k(int n, const uint32_t* __restrict__ hc, const uint8_t* __restrict__ krbuf,
-                                            const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
-                                            uint8_t* __restrict__ ss) {
-  k_body<L>(blockIdx.x, n, hc, krbuf, sk, fail, ss);
 }
 
 // ----------------------------------------------------------------- products
@@ -810,6 +803,24 @@ __global__ void __launch_bounds__(64 * IPW) __attribute__((amdgpu_waves_per_eu(4
   sab_poly_body<MODE, L, A_PLAIN>(a, S[w], IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63), inst);
 }
 
+// The pipelined chain's re-encryption, 16 instances per workgroup (one wave
+// each, as sab_poly), then decaps' K for those 16 on wave 0 after a workgroup
+// barrier: the mismatch flags only need workgroup visibility, and the chain
+// needs no separate launch for K.  16 x 6-7 KB of LDS keeps one workgroup
+// (16 waves, 4 per SIMD) per CU.
+constexpr int kReencGroup = 16;
+template <int L>
+__global__ void __launch_bounds__(64 * kReencGroup) __attribute__((amdgpu_waves_per_eu(4)))
+sab_reenc_k(SabPolyArgs a, const uint32_t* __restrict__ hc, const uint8_t* __restrict__ krbuf,
+            const uint8_t* __restrict__ sk, uint8_t* __restrict__ ss) {
+  __shared__ __align__(16) SabPolyLds<L> S[kReencGroup];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int inst = (int)blockIdx.x * kReencGroup + w;
+  if (inst < a.n) sab_poly_body<kSabReenc, L, true>(a, S[w], threadIdx.x & 63, inst);
+  __syncthreads();
+  if (w == 0) k_body<L, kReencGroup>(blockIdx.x, a.n, hc, krbuf, sk, a.fail, ss);
+}
+
 // Decaps inner product v = b' . s and message recovery
 // m'_t = ((v + h2 - c_m 2^(10 - eT)) >> 9) & 1.  Two instances per 64-thread
 // block; s (from sk) is the Toeplitz-staged factor, b' (from ct) the plain one.
@@ -1003,8 +1014,8 @@ __device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const InnerRaw<L>& 
 // SIMD), so the roles are laid out by w mod 4 and the two long hash chains
 // each get a SIMD of their own from the start:
 //   w = 0          encaps' K = SHA3-256(kr[0:32] || H(ct));
-//   w = 1          decaps' H(ct) into hc_d (sab_k finishes K after the
-//                  re-encryption);
+//   w = 1          decaps' H(ct) into hc_d (sab_reenc_k finishes K after
+//                  the re-encryption);
 //   w = 3          deinterleaves both chains' copy of ct into LDS block by
 //                  block (stage2_message: lanes 0..31 for encaps' K, 32..63
 //                  for decaps' H(ct)), so the two chains absorb without
@@ -1207,8 +1218,8 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
 // from 24 + 8 + 23 + 10 + 23 + 10 to 24 + 14 + 10 + 1 permutations (all
 // three GenMatrix streams start from the seedA bytes of pk / sk as soon as
 // keygen has written them; keygen's H(pk), encaps' m and K and decaps' H(ct)
-// run beside other stages, decaps' inner product beside encaps' K), and ten
-// launches become seven.  scratch:
+// run beside other stages, decaps' inner product beside encaps' K, decaps'
+// K in the re-encryption launch), and ten launches become six.  scratch:
 // saber_pipe_scratch_per_inst(l) bytes per instance: three GenMatrix streams,
 // two GenSecret, two m and two kr buffers (keygen's GenSecret reuses decaps'
 // buffer, which the stream order makes safe), decaps' H(ct) and the mismatch
@@ -1245,7 +1256,7 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint32_t* hc_d = reinterpret_cast<uint32_t*>(take(32));
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   const int nb = blocks_for(n, 32);
-  const dim3 g(nb), b64(64);
+  const dim3 g(nb);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x0"), sab_pipe_x0<L>, g, dim3(256), st, n, kc, pk, sk, araw_k, sraw_d, araw_e,
                araw_d);
   SabPolyArgs ak{n, araw_k, sraw_d, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
@@ -1257,8 +1268,8 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, dim3(blocks_for(n, kX2Group)), dim3(kX2Threads), st, n, sk, ct,
                mbuf_d, sraw_d, krbuf_d, krbuf_e, ss, hc_d);
   SabPolyArgs ad{n, araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, fail};
-  poly_launch<kSabReenc, L, true>(SAB_NAME("sab_poly<kSabReenc>"), st, n, ad, ipw);
-  DPKEM_LAUNCH(SAB_NAME("sab_k"), sab_k<L>, g, b64, st, n, hc_d, krbuf_d, sk, fail, ss2);
+  DPKEM_LAUNCH(SAB_NAME("sab_reenc_k"), sab_reenc_k<L>, dim3(blocks_for(n, kReencGroup)), dim3(64 * kReencGroup), st,
+               ad, hc_d, krbuf_d, sk, ss2);
 }
 
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
