@@ -128,57 +128,64 @@ KERNEL_WORK = {
 #   bench.py --schedule 1x1 --calibrate-copy --skip-mixed --skip-serving --cpu-seconds 0 --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99969, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129242980, 256245760),
-    ('frodo_as_fused,1344', 8192): (177466062, 353369376),
+    ('frodo_as_fused', 8192): (129242937, 256245760),
+    ('frodo_as_fused,1344', 8192): (177465986, 353352755),
     ('frodo_as_fused,640', 8192): (85209113, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129300542, 256247600),
-    ('frodo_dec_mu', 8192): (269999039, 262144),
-    ('frodo_dec_mu,1344', 8192): (385411546, 262144),
-    ('frodo_dec_mu,640', 8192): (223113959, 262144),
-    ('frodo_encdec_pre<false>', 8192): (75005374, 128712758),
+    ('frodo_as_fused<aes>', 8192): (129300542, 256247352),
+    ('frodo_dec_mu', 8192): (269967010, 262144),
+    ('frodo_dec_mu,1344', 8192): (385394243, 262144),
+    ('frodo_dec_mu,640', 8192): (223028084, 262144),
+    ('frodo_encdec_pre<false>', 8192): (75005374, 128713037),
     ('frodo_encdec_pre<true>', 8192): (1841888, 128712709),
     ('frodo_keygen_post', 8192): (129166675, 262144),
-    ('frodo_keygen_pre', 8192): (1399560, 256981435),
-    ('frodo_keygen_pre,1344', 8192): (1517792, 353244281),
-    ('frodo_keygen_pre,640', 8192): (1297000, 168558681),
+    ('frodo_keygen_pre', 8192): (1399835, 256982525),
+    ('frodo_keygen_pre,1344', 8192): (1518124, 353243846),
+    ('frodo_keygen_pre,640', 8192): (1296540, 168558720),
     ('frodo_pipe_ssd', 8192): (4416847, 196608),
     ('frodo_pipe_ssd,1344', 8192): (4531517, 262144),
     ('frodo_pipe_ssd,640', 8192): (5534880, 131072),
-    ('frodo_pipe_x1', 8192): (130399294, 128975043),
+    ('frodo_pipe_x1', 8192): (130399265, 128975064),
     ('frodo_pipe_x1,1344', 8192): (178633522, 177340416),
     ('frodo_pipe_x1,640', 8192): (81022630, 84934656),
-    ('frodo_pipe_x2', 8192): (131945662, 130671466),
-    ('frodo_pipe_x2,1344', 8192): (180284004, 178864934),
-    ('frodo_pipe_x2,640', 8192): (83619831, 86442035),
+    ('frodo_pipe_x2', 8192): (131945548, 130677148),
+    ('frodo_pipe_x2,1344', 8192): (180283697, 178863340),
+    ('frodo_pipe_x2,640', 8192): (83619601, 86442028),
     ('frodo_sa_fused<false,aes>', 8192): (259462231, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260444595, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356728479, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166968627, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388295710, 262144),
-    ('frodo_sa_fused<true>', 8192): (389401298, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533910528, 265043),
-    ('frodo_sa_fused<true>,640', 8192): (247561526, 262278),
+    ('frodo_sa_fused<false>', 8192): (260444637, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356728351, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166968525, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388295614, 262144),
+    ('frodo_sa_fused<true>', 8192): (389401326, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533910324, 265344),
+    ('frodo_sa_fused<true>,640', 8192): (247561526, 262169),
     ('frodo_ss', 8192): (130023393, 196608),
-    ('sab_dec_hash1', 4096): (1561356, 18878610),
-    ('sab_dec_inner', 4096): (9888280, 131072),
-    ('sab_enc_hash1', 4096): (5316928, 19052151),
-    ('sab_hash2', 4096): (5234895, 131072),
-    ('sab_k', 4096): (1031007, 131072),
-    ('sab_k,l2', 4096): (1019744, 131072),
-    ('sab_k,l4', 4096): (1031007, 131072),
-    ('sab_keygen_hash1', 4096): (1289484, 19007597),
+    ('sab_dec_hash1', 4096): (1561467, 18877309),
+    ('sab_dec_inner', 4096): (9888439, 131072),
+    ('sab_enc_hash1', 4096): (5316826, 19051865),
+    ('sab_hash2', 4096): (5233871, 131072),
+    ('sab_keygen_hash1', 4096): (1289618, 19004751),
     ('sab_keygen_hash2', 4096): (4250985, 131072),
-    ('sab_pipe_x0', 4096): (1304631, 49617159),
-    ('sab_pipe_x0,l2', 4096): (1287147, 23513677),
-    ('sab_pipe_x0,l4', 4096): (1305325, 85363687),
-    ('sab_pipe_x1', 4096): (36109313, 34340864),
+    ('sab_pipe_x0', 4096): (1307327, 49615762),
+    ('sab_pipe_x0,l2', 4096): (1297603, 23513186),
+    ('sab_pipe_x0,l4', 4096): (1305258, 85363040),
+    ('sab_pipe_x1', 4096): (36095611, 34340864),
     ('sab_pipe_x1,l2', 4096): (17790287, 16777216),
-    ('sab_pipe_x1,l4', 4096): (62075284, 58195968),
+    ('sab_pipe_x1,l4', 4096): (62039990, 58195968),
     ('sab_pipe_x2', 4096): (12493923, 3801088),
     ('sab_pipe_x2,l2', 4096): (9377353, 3276800),
-    ('sab_pipe_x2,l4', 4096): (15819546, 3801088),
+    ('sab_pipe_x2,l4', 4096): (15819518, 3801088),
+    ('sab_poly<kSabEnc>', 4096): (23842469, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13076489, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (36670961, 6029312),
+    ('sab_poly<kSabKeygen>', 4096): (18985163, 12978875),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9522814, 8650852),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30515713, 17330702),
+    ('sab_poly<kSabReenc>', 4096): (28972000, 131072),
+    ('sab_reenc_k', 4096): (28414740, 147456),
+    ('sab_reenc_k,l2', 4096): (16597476, 147456),
+    ('sab_reenc_k,l4', 4096): (48457491, 5128192),
 }
 
 
@@ -610,7 +617,7 @@ def main():
         "ms_per_step": round(unpiped["wall_s"] * 1e3 / args.steps, 4),
         "schedule": {k: v for k, v in unpiped["schedule"].items() if k != "tuning_ms"},
         "note": "keygen, encaps and decaps as three dependent plan tasks (ten launches per chunk); the headline "
-                "runs the same work with stages of different ops sharing launches (seven per chunk)"}
+                "runs the same work with stages of different ops sharing launches (six per chunk)"}
     progress("saber with the plan's opt-in matrix sharing across keygen -> enc -> dec")
     shared = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                tune_budget_s=20.0, share_matrix=True, schedule=sched)
