@@ -419,7 +419,11 @@ __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict_
 }
 
 // decaps: W = C - B' S, mu' = Decode(W) (B bits per entry, rounded).  256
-// threads per instance: 64 outputs x 4 slices of n / 4 terms.
+// threads per instance, each over columns k = tid, tid + 256, ... of all
+// eight rows of B' and S^T (neighbouring lanes read neighbouring entries),
+// holding the 64 sums (i, j) in registers; a butterfly over the wave then
+// leaves lane L with sum L (each step sends the half a lane does not keep),
+// and the four waves add theirs in LDS.
 template <class P>
 __global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
                                                     size_t sk_stride, uint8_t* __restrict__ mubuf) {
@@ -430,15 +434,34 @@ __global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __rest
   __syncthreads();
   const uint8_t* c = ct + (size_t)inst * P::ct;
   const uint8_t* sT = sk + (size_t)inst * sk_stride + P::sk_s;
-  constexpr int CH = P::n / 4;
-  const int o = tid & 63, i = o >> 3, j = o & 7, slice = tid >> 6;
-  uint32_t acc = 0;
-  for (int k = slice * CH; k < slice * CH + CH; ++k) {
-    const uint32_t bp = unpack_at<P::logq>(c, i * P::n + k);
-    const uint32_t s = sT[2 * (j * P::n + k)] | ((uint32_t)sT[2 * (j * P::n + k) + 1] << 8);
-    acc += bp * s;
+  uint32_t acc[64];
+#pragma unroll
+  for (int t = 0; t < 64; ++t) acc[t] = 0;
+  for (int k = tid; k < P::n; k += 256) {
+    uint32_t bp[8], s[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      bp[r] = unpack_at<P::logq>(c, r * P::n + k);
+      s[r] = sT[2 * (r * P::n + k)] | ((uint32_t)sT[2 * (r * P::n + k) + 1] << 8);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[8 * i + j] += bp[i] * s[j];
   }
-  atomicAdd(&W[o], acc);
+  const int lane = tid & 63;
+#pragma unroll
+  for (int m = 64; m > 1; m >>= 1) {
+    const int d = m >> 1;  // partner distance = half the values still held
+    const bool up = (lane & d) != 0;
+#pragma unroll
+    for (int t = 0; t < d; ++t) {
+      const uint32_t keep = up ? acc[t + d] : acc[t];
+      const uint32_t send = up ? acc[t] : acc[t + d];
+      acc[t] = keep + (uint32_t)__shfl_xor((int)send, d, 64);
+    }
+  }
+  atomicAdd(&W[lane], acc[0]);
   __syncthreads();
   if (tid < 8) {
     constexpr int SH = P::logq - P::b;
