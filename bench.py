@@ -128,64 +128,64 @@ KERNEL_WORK = {
 #   bench.py --schedule 1x1 --calibrate-copy --skip-mixed --skip-serving --cpu-seconds 0 --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99969, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129242937, 256245760),
-    ('frodo_as_fused,1344', 8192): (177465986, 353352755),
-    ('frodo_as_fused,640', 8192): (85209113, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129300542, 256247352),
-    ('frodo_dec_mu', 8192): (269967010, 262144),
-    ('frodo_dec_mu,1344', 8192): (385394243, 262144),
-    ('frodo_dec_mu,640', 8192): (223028084, 262144),
-    ('frodo_encdec_pre<false>', 8192): (75005374, 128713037),
-    ('frodo_encdec_pre<true>', 8192): (1841888, 128712709),
-    ('frodo_keygen_post', 8192): (129166675, 262144),
-    ('frodo_keygen_pre', 8192): (1399835, 256982525),
-    ('frodo_keygen_pre,1344', 8192): (1518124, 353243846),
-    ('frodo_keygen_pre,640', 8192): (1296540, 168558720),
-    ('frodo_pipe_ssd', 8192): (4416847, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4531517, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5534880, 131072),
-    ('frodo_pipe_x1', 8192): (130399265, 128975064),
-    ('frodo_pipe_x1,1344', 8192): (178633522, 177340416),
-    ('frodo_pipe_x1,640', 8192): (81022630, 84934656),
-    ('frodo_pipe_x2', 8192): (131945548, 130677148),
-    ('frodo_pipe_x2,1344', 8192): (180283697, 178863340),
-    ('frodo_pipe_x2,640', 8192): (83619601, 86442028),
-    ('frodo_sa_fused<false,aes>', 8192): (259462231, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260444637, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356728351, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166968525, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388295614, 262144),
-    ('frodo_sa_fused<true>', 8192): (389401326, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533910324, 265344),
-    ('frodo_sa_fused<true>,640', 8192): (247561526, 262169),
-    ('frodo_ss', 8192): (130023393, 196608),
-    ('sab_dec_hash1', 4096): (1561467, 18877309),
-    ('sab_dec_inner', 4096): (9888439, 131072),
-    ('sab_enc_hash1', 4096): (5316826, 19051865),
-    ('sab_hash2', 4096): (5233871, 131072),
-    ('sab_keygen_hash1', 4096): (1289618, 19004751),
-    ('sab_keygen_hash2', 4096): (4250985, 131072),
-    ('sab_pipe_x0', 4096): (1307327, 49615762),
-    ('sab_pipe_x0,l2', 4096): (1297603, 23513186),
-    ('sab_pipe_x0,l4', 4096): (1305258, 85363040),
-    ('sab_pipe_x1', 4096): (36095611, 34340864),
-    ('sab_pipe_x1,l2', 4096): (17790287, 16777216),
-    ('sab_pipe_x1,l4', 4096): (62039990, 58195968),
-    ('sab_pipe_x2', 4096): (12493923, 3801088),
-    ('sab_pipe_x2,l2', 4096): (9377353, 3276800),
-    ('sab_pipe_x2,l4', 4096): (15819518, 3801088),
-    ('sab_poly<kSabEnc>', 4096): (23842469, 4456448),
-    ('sab_poly<kSabEnc>,l2', 4096): (13076489, 3014656),
-    ('sab_poly<kSabEnc>,l4', 4096): (36670961, 6029312),
-    ('sab_poly<kSabKeygen>', 4096): (18985163, 12978875),
-    ('sab_poly<kSabKeygen>,l2', 4096): (9522814, 8650852),
-    ('sab_poly<kSabKeygen>,l4', 4096): (30515713, 17330702),
-    ('sab_poly<kSabReenc>', 4096): (28972000, 131072),
-    ('sab_reenc_k', 4096): (28414740, 147456),
-    ('sab_reenc_k,l2', 4096): (16597476, 147456),
-    ('sab_reenc_k,l4', 4096): (48457491, 5128192),
+    ('frodo_as_fused', 8192): (129246971, 256245774),
+    ('frodo_as_fused,1344', 8192): (177471539, 353364147),
+    ('frodo_as_fused,640', 8192): (85211658, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129305196, 256247376),
+    ('frodo_dec_mu', 8192): (241957658, 262144),
+    ('frodo_dec_mu,1344', 8192): (358589132, 262144),
+    ('frodo_dec_mu,640', 8192): (165536535, 262144),
+    ('frodo_encdec_pre<false>', 8192): (74999400, 128712781),
+    ('frodo_encdec_pre<true>', 8192): (1835501, 128712709),
+    ('frodo_keygen_post', 8192): (129170493, 262144),
+    ('frodo_keygen_pre', 8192): (1399965, 256983020),
+    ('frodo_keygen_pre,1344', 8192): (1517862, 353244614),
+    ('frodo_keygen_pre,640', 8192): (1296988, 168558649),
+    ('frodo_pipe_ssd', 8192): (4406739, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4531651, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5524805, 131072),
+    ('frodo_pipe_x1', 8192): (130392454, 128975015),
+    ('frodo_pipe_x1,1344', 8192): (178636755, 177340416),
+    ('frodo_pipe_x1,640', 8192): (81037311, 84934656),
+    ('frodo_pipe_x2', 8192): (131951752, 130667530),
+    ('frodo_pipe_x2,1344', 8192): (180284162, 178866707),
+    ('frodo_pipe_x2,640', 8192): (83622150, 86442009),
+    ('frodo_sa_fused<false,aes>', 8192): (259469324, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260451938, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356739152, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166973332, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388307028, 262144),
+    ('frodo_sa_fused<true>', 8192): (389415510, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533926208, 265523),
+    ('frodo_sa_fused<true>,640', 8192): (247575064, 262201),
+    ('frodo_ss', 8192): (130026764, 196608),
+    ('sab_dec_hash1', 4096): (1561519, 18877325),
+    ('sab_dec_inner', 4096): (9888743, 131072),
+    ('sab_enc_hash1', 4096): (5316983, 19051834),
+    ('sab_hash2', 4096): (5234026, 131072),
+    ('sab_keygen_hash1', 4096): (1289712, 19004992),
+    ('sab_keygen_hash2', 4096): (4251110, 131072),
+    ('sab_pipe_x0', 4096): (1307389, 49616243),
+    ('sab_pipe_x0,l2', 4096): (1297722, 23513640),
+    ('sab_pipe_x0,l4', 4096): (1305150, 85363505),
+    ('sab_pipe_x1', 4096): (36096942, 34340864),
+    ('sab_pipe_x1,l2', 4096): (17790775, 16777216),
+    ('sab_pipe_x1,l4', 4096): (62042554, 58195968),
+    ('sab_pipe_x2', 4096): (12494293, 3801088),
+    ('sab_pipe_x2,l2', 4096): (9377631, 3276800),
+    ('sab_pipe_x2,l4', 4096): (15820033, 3801088),
+    ('sab_poly<kSabEnc>', 4096): (23843366, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13076875, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (36672064, 6029312),
+    ('sab_poly<kSabKeygen>', 4096): (18985725, 12978908),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9523096, 8650816),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30516615, 17332160),
+    ('sab_poly<kSabReenc>', 4096): (28973813, 131072),
+    ('sab_reenc_k', 4096): (28415734, 147456),
+    ('sab_reenc_k,l2', 4096): (16597966, 147456),
+    ('sab_reenc_k,l4', 4096): (48458933, 5128192),
 }
 
 
