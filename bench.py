@@ -249,48 +249,65 @@ def run_workload(torch, eng, scheme, n, steps, warmup, seed, profile=False):
                 ops_per_s=3 * n * steps / wall)
 
 
+def chain_arrays(torch, scheme, n, seed):
+    """Device-resident arrays of one keygen -> encaps -> decaps step over n
+    instances: coins drawn from numpy's generator seeded with `seed` (as in
+    every timed run), outputs left uninitialised."""
+    import numpy as np
+    import dpcrypto_amd as D
+    s = D.SIZES[scheme]
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    a = {"kc": torch.from_numpy(rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)).to(dev),
+         "ec": torch.from_numpy(rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)).to(dev)}
+    for name, w in (("pk", s["pk"]), ("sk", s["sk"]), ("ct", s["ct"]), ("ss", s["ss"]), ("ss2", s["ss"])):
+        a[name] = torch.empty((n, w), dtype=torch.uint8, device=dev)
+    return a
+
+
+def build_chain_plan(eng, scheme, a, nstreams, chunks, ipw=1, share_matrix=False, pipeline=True, graph=True):
+    """The plan a timed step replays: keygen -> encaps -> decaps chained per
+    instance over the arrays of chain_arrays; ipw is the Saber product's
+    instances per workgroup (a launch-shape knob)."""
+    import dpcrypto_amd as D
+    eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    p = eng.plan(nstreams=nstreams, chunks=chunks, share_matrix=share_matrix, pipeline=pipeline, graph=graph)
+    t0 = p.keypair(scheme, a["pk"], a["sk"], a["kc"])
+    t1 = p.enc(scheme, a["ct"], a["ss"], a["pk"], a["ec"], after=t0)
+    p.dec(scheme, a["ss2"], a["ct"], a["sk"], after=t1)
+    return p.build()
+
+
+def tune_chain_schedule(torch, eng, scheme, n, a, tune_budget_s=60.0, share_matrix=False, pipeline=True):
+    """Autotuned (streams, chunks[, ipw]) schedule of the chain plan."""
+    import dpcrypto_amd as D
+    from dpcrypto_amd import autotune
+    saber_family = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
+
+    def build(ns, ch, ipw=1):
+        return build_chain_plan(eng, scheme, a, ns, ch, ipw, share_matrix, pipeline)
+
+    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}|pipe={int(pipeline)}|ipw"
+    if saber_family:
+        return autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s,
+                             candidates=[c + (1,) for c in autotune.DEFAULT_CANDIDATES],
+                             refine=lambda b: [(b[0], b[1], 2), (b[0], b[1], 4)])
+    return autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s)
+
+
 def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=60.0, share_matrix=False,
                       schedule=None, pipeline=True):
     """Time `steps` keygen+enc+dec steps, each one replay of a graph-captured
     plan (keygen -> enc -> dec chained per instance chunk); the plan's
     (streams, chunks) schedule is picked by the autotuner first."""
-    import numpy as np
-    import dpcrypto_amd as D
-    from dpcrypto_amd import autotune
-    s = D.SIZES[scheme]
-    rng = np.random.default_rng(seed)
-    dev = torch.device("cuda:0")
-    kc = torch.from_numpy(rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)).to(dev)
-    ec = torch.from_numpy(rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)).to(dev)
-    pk = torch.empty((n, s["pk"]), dtype=torch.uint8, device=dev)
-    sk = torch.empty((n, s["sk"]), dtype=torch.uint8, device=dev)
-    ct = torch.empty((n, s["ct"]), dtype=torch.uint8, device=dev)
-    ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
-    ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    a = chain_arrays(torch, scheme, n, seed)
+    stream = torch.cuda.current_stream(torch.device("cuda:0"))
     eng.set_stream(stream.cuda_stream)
-
-    saber_family = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
-
-    def build(ns, ch, ipw=1):
-        # ipw: the Saber product's instances per workgroup (a launch-shape knob)
-        eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
-        p = eng.plan(nstreams=ns, chunks=ch, share_matrix=share_matrix, pipeline=pipeline)
-        t0 = p.keypair(scheme, pk, sk, kc)
-        t1 = p.enc(scheme, ct, ss, pk, ec, after=t0)
-        p.dec(scheme, ss2, ct, sk, after=t1)
-        return p.build()
-
-    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}|pipe={int(pipeline)}|ipw"
     if schedule:
         best, tried = tuple(schedule), {}
-    elif saber_family:
-        best, tried = autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s,
-                                    candidates=[c + (1,) for c in autotune.DEFAULT_CANDIDATES],
-                                    refine=lambda b: [(b[0], b[1], 2), (b[0], b[1], 4)])
     else:
-        best, tried = autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s)
-    plan = build(*best)
+        best, tried = tune_chain_schedule(torch, eng, scheme, n, a, tune_budget_s, share_matrix, pipeline)
+    plan = build_chain_plan(eng, scheme, a, *best, share_matrix=share_matrix, pipeline=pipeline)
     for _ in range(warmup):
         plan.launch()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
@@ -303,13 +320,108 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
         stops[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    per_step = [a.elapsed_time(b) for a, b in zip(starts, stops)]
-    ok = bool(torch.equal(ss, ss2))
+    per_step = [x.elapsed_time(y) for x, y in zip(starts, stops)]
+    ok = bool(torch.equal(a["ss"], a["ss2"]))
     plan.close()
     return dict(wall_s=wall, per_step_ms=per_step, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
                 schedule={"nstreams": best[0], "chunks": best[1],
                           **({"saber_poly_ipw": best[2]} if len(best) > 2 else {}),
                           "tuning_ms": {"x".join(map(str, c)): round(v, 4) for c, v in tried.items()}})
+
+
+# Op-level ceilings (ops/s) from the algorithmic work of one op (Keccak
+# permutations at 4,176 VALU lane-ops each plus MACs at 2 per lane-op) at the
+# VALU peak; keygen+enc+dec at 1:1:1 is their harmonic combination.
+OP_CEILING = {
+    ("saber", "keygen"): 87.5e6, ("saber", "enc"): 66.2e6, ("saber", "dec"): 60.1e6,
+    ("frodo976", "keygen"): 780900.0, ("frodo976", "enc"): 773396.0, ("frodo976", "dec"): 780770.0,
+}
+
+
+def chain_ceiling(name):
+    return 3.0 / sum(1.0 / OP_CEILING[(name, op)] for op in ("keygen", "enc", "dec"))
+
+
+def run_per_op(torch, eng, plain_eng, scheme, name, n, steps, seed, tune_budget_s=10.0):
+    """keygen, encaps and decaps timed separately, each as its own
+    graph-captured single-op plan over n device-resident instances (schedule
+    autotuned per op), beside the plain one-workgroup-per-KEM form launched
+    op by op on the same inputs.  Outputs of every form must agree."""
+    import statistics
+    import dpcrypto_amd as D
+    from dpcrypto_amd import autotune
+    a = chain_arrays(torch, scheme, n, seed)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    eng.set_stream(stream.cuda_stream)
+    s = D.SIZES[scheme]
+    # keys and ciphertexts the enc / dec plans read (untimed, eager)
+    eng.keypair_into(scheme, n, a["pk"], a["sk"], a["kc"])
+    eng.enc_into(scheme, n, a["ct"], a["ss"], a["pk"], a["ec"])
+    eng.dec_into(scheme, n, a["ss2"], a["ct"], a["sk"])
+    out = {op: [torch.empty((n, w), dtype=torch.uint8, device=dev) for w in ws]
+           for op, ws in (("keygen", (s["pk"], s["sk"])), ("enc", (s["ct"], s["ss"])), ("dec", (s["ss"],)))}
+    saber_family = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
+
+    def build(op, ns, ch, ipw=1):
+        eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+        p = eng.plan(nstreams=ns, chunks=ch)
+        o = out[op]
+        if op == "keygen":
+            p.keypair(scheme, o[0], o[1], a["kc"])
+        elif op == "enc":
+            p.enc(scheme, o[0], o[1], a["pk"], a["ec"])
+        else:
+            p.dec(scheme, o[0], a["ct"], a["sk"])
+        return p.build()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            x, y = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            x.record(stream)
+            fn()
+            y.record(stream)
+            y.synchronize()
+            ts.append(x.elapsed_time(y))
+        return statistics.median(ts)
+
+    cands = [(1, 1), (2, 2), (4, 4), (8, 4)] if not saber_family else [(1, 1, 1), (1, 1, 4), (2, 2, 1)]
+    res = {}
+    for op in ("keygen", "enc", "dec"):
+        best, tried = autotune.tune(torch, eng, f"scheme{scheme}|{op}|n={n}", lambda *c: build(op, *c),
+                                    candidates=cands, use_cache=False, budget_s=tune_budget_s)
+        plan = build(op, *best)
+        ms = timed(plan.launch, steps)
+        plan.close()
+        ceil = OP_CEILING[(name, op)]
+        res[op] = {"ops_per_s": round(n / (ms / 1e3), 1), "ms_per_batch": round(ms, 4),
+                   "ceiling_ops_per_s": ceil, "frac_of_ceiling": round(n / (ms / 1e3) / ceil, 4),
+                   "schedule": "x".join(map(str, best))}
+    ok = (torch.equal(out["keygen"][0], a["pk"]) and torch.equal(out["keygen"][1], a["sk"])
+          and torch.equal(out["enc"][0], a["ct"]) and torch.equal(out["enc"][1], a["ss"])
+          and torch.equal(out["dec"][0], a["ss"]) and torch.equal(a["ss2"], a["ss"]))
+    if plain_eng is not None:
+        plain_eng.set_stream(stream.cuda_stream)
+        pout = {op: [torch.empty_like(t) for t in ts] for op, ts in out.items()}
+        reps = max(1, min(steps, 3))
+        pms = {
+            "keygen": timed(lambda: plain_eng.keypair_into(scheme, n, pout["keygen"][0], pout["keygen"][1], a["kc"]), reps),
+            "enc": timed(lambda: plain_eng.enc_into(scheme, n, pout["enc"][0], pout["enc"][1], a["pk"], a["ec"]), reps),
+            "dec": timed(lambda: plain_eng.dec_into(scheme, n, pout["dec"][0], a["ct"], a["sk"]), reps)}
+        for op in res:
+            res[op]["plain_ops_per_s"] = round(n / (pms[op] / 1e3), 1)
+            res[op]["batched_over_plain"] = round(pms[op] / res[op]["ms_per_batch"], 2)
+        encdec = res["enc"]["ms_per_batch"] + res["dec"]["ms_per_batch"]
+        res["enc_plus_dec"] = {"ops_per_s": round(2 * n / (encdec / 1e3), 1),
+                               "plain_ops_per_s": round(2 * n / ((pms["enc"] + pms["dec"]) / 1e3), 1),
+                               "batched_over_plain": round((pms["enc"] + pms["dec"]) / encdec, 2)}
+        ok = ok and all(torch.equal(x, y) for op in out for x, y in zip(out[op], pout[op]))
+    res["outputs_agree"] = bool(ok)
+    res["workload"] = f"{name} batch {n}: each op as its own graph-captured plan over device-resident inputs"
+    return res
 
 
 def run_shared_key(torch, eng, scheme, n, steps, seed):
@@ -345,24 +457,9 @@ def profile_plan(torch, eng, scheme, n, steps, seed):
     stream, one chunk, pipelined keygen -> encaps -> decaps chain), issued
     without a graph (DPKEM_PLAN_NO_GRAPH) so that the engine's profiler
     brackets every kernel with events on the stream it runs on."""
-    import numpy as np
-    import dpcrypto_amd as D
-    s = D.SIZES[scheme]
-    rng = np.random.default_rng(seed)
-    dev = torch.device("cuda:0")
-    kc = torch.from_numpy(rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)).to(dev)
-    ec = torch.from_numpy(rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)).to(dev)
-    pk = torch.empty((n, s["pk"]), dtype=torch.uint8, device=dev)
-    sk = torch.empty((n, s["sk"]), dtype=torch.uint8, device=dev)
-    ct = torch.empty((n, s["ct"]), dtype=torch.uint8, device=dev)
-    ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
-    ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device=dev)
-    eng.set_stream(torch.cuda.current_stream(dev).cuda_stream)
-    p = eng.plan(1, 1, graph=False)
-    t0 = p.keypair(scheme, pk, sk, kc)
-    t1 = p.enc(scheme, ct, ss, pk, ec, after=t0)
-    p.dec(scheme, ss2, ct, sk, after=t1)
-    p.build()
+    a = chain_arrays(torch, scheme, n, seed)
+    eng.set_stream(torch.cuda.current_stream(torch.device("cuda:0")).cuda_stream)
+    p = build_chain_plan(eng, scheme, a, 1, 1, graph=False)
     p.launch()
     torch.cuda.synchronize()
     eng.kernel_stats(reset=True)
@@ -372,7 +469,7 @@ def profile_plan(torch, eng, scheme, n, steps, seed):
     torch.cuda.synchronize()
     eng.set_profiling(False)
     stats = eng.kernel_stats(reset=True)
-    ok = bool(torch.equal(ss, ss2))
+    ok = bool(torch.equal(a["ss"], a["ss2"]))
     p.close()
     return dict(kernels=stats, roundtrip_ok=ok)
 
@@ -433,34 +530,58 @@ def _cpu_model():
 
 
 def cpu_baseline(budget_s: float):
-    """The oracle (a scalar C port) on the host cores, bounded sample; also
-    one core for a quarter of the budget."""
+    """The oracle (a scalar C port of the path) on this host's cores, on a
+    bounded sample: Saber l=3 and FrodoKEM-976 keygen+encaps+decaps, one
+    instance per thread at a time, on all usable cores and on one core.  The
+    oracle is compiled here with -O3 -march=native for this host first (the
+    checked-in build targets a portable x86-64 level)."""
+    import tempfile
     import numpy as np
     from oracle import pyoracle as O
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = max(1, min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1,
+                         16))
+    flags = "-O3 -march=native"
+    tmp = tempfile.mkdtemp(prefix="dpkem_orc_")
+    try:
+        O.use_library(O.build_native(os.path.join(tmp, "liboracle_native.so")))
+    except Exception as exc:  # no compiler on this host: time the portable build
+        O.use_library(None)
+        flags = f"-O3 -march=x86-64-v2 (native build failed: {exc.__class__.__name__})"
 
-    def rate(th, seconds):
-        n = th * 8
+    def rate(scheme, th, per_thread, seconds):
+        s = O.SIZES[scheme]
+        n = th * per_thread
         rng = np.random.default_rng(1)
-        kc = rng.integers(0, 256, (n, 96), dtype=np.uint8)
-        ec = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+        kc = rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)
+        ec = rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)
         ops = 0
         t0 = time.perf_counter()
-        while time.perf_counter() - t0 < seconds:
-            pk, sk = O.keypair(1, kc, threads=th)
-            ct, ss = O.enc(1, pk, ec, threads=th)
-            O.dec(1, ct, sk, threads=th)
+        while True:
+            pk, sk = O.keypair(scheme, kc, threads=th)
+            ct, ss = O.enc(scheme, pk, ec, threads=th)
+            O.dec(scheme, ct, sk, threads=th)
             ops += 3 * n
+            if time.perf_counter() - t0 >= seconds:
+                break
         dt = time.perf_counter() - t0
         return ops / dt, n, ops // (3 * n), dt
 
-    v, n, rounds, dt = rate(threads, budget_s)
-    v1, _, _, dt1 = rate(1, budget_s / 4)
+    try:
+        v, n, rounds, dt = rate(O.SABER, threads, 32, budget_s * 0.45)
+        v1, _, _, dt1 = rate(O.SABER, 1, 32, budget_s * 0.1)
+        fv, fn, frounds, fdt = rate(O.FRODO, threads, 2, budget_s * 0.35)
+        fv1, _, _, fdt1 = rate(O.FRODO, 1, 2, budget_s * 0.1)
+    finally:
+        O.use_library(None)
     return {"value": round(v, 1), "unit": "KEM ops/s", "cores": threads, "kind": "port",
             "sample": f"Saber l=3 keygen+encaps+decaps, {n} instances per round, {rounds} rounds "
-                      f"({dt:.1f} s) on {threads} threads of the CPU oracle (oracle/, -O3 C)",
+                      f"({dt:.1f} s) on {threads} threads of the CPU oracle (oracle/, {flags})",
             "single_core_value": round(v1, 1), "single_core_seconds": round(dt1, 1),
-            "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+            "frodo976": {"value": round(fv, 1), "unit": "KEM ops/s", "cores": threads,
+                         "sample": f"FrodoKEM-976-SHAKE keygen+encaps+decaps, {fn} instances per round, {frounds} "
+                                   f"rounds ({fdt:.1f} s) on {threads} threads",
+                         "single_core_value": round(fv1, 1), "single_core_seconds": round(fdt1, 1)},
+            "compiler_flags": flags, "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
 
 
 def main():
@@ -474,6 +595,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--skip-frodo", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-per-op", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
@@ -539,6 +661,14 @@ def main():
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_prof["kernels"].items()},
         "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
     }
+    pe = None if args.skip_plain else D.Engine(0, form=D.FORM_PLAIN)
+    if not args.skip_per_op:
+        progress(f"saber batch {args.saber_batch}: keygen, encaps, decaps timed as separate plans, and in the plain form")
+        result["per_op"] = {"saber": run_per_op(torch, eng, pe, D.SABER, "saber", args.saber_batch, args.steps,
+                                                0x44504B454D000001)}
+    result["ceiling_ops_per_s"] = {"saber_keygen_enc_dec": round(chain_ceiling("saber"), 1),
+                                   "frodo976_keygen_enc_dec": round(chain_ceiling("frodo976"), 1)}
+    result["frac_of_ceiling"] = round(sab["ops_per_s"] / chain_ceiling("saber"), 4)
     if not args.skip_frodo:
         progress(f"frodo976 batch {args.frodo_batch}: autotune, 2 warmup + {args.frodo_steps} timed graph steps")
         fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 2, 0x44504B454D000002,
@@ -549,6 +679,7 @@ def main():
         result["frodo976"] = {
             "workload": f"frodokem-976-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
             "ops_per_s": round(fr["ops_per_s"], 1),
+            "frac_of_ceiling": round(fr["ops_per_s"] / chain_ceiling("frodo976"), 4),
             "ms_per_step": round(fr["wall_s"] * 1e3 / args.frodo_steps, 3),
             "latency_ms": {"p50": round(pct(fr["per_step_ms"], 0.5), 3), "p99": round(pct(fr["per_step_ms"], 0.99), 3)},
             "steps": args.frodo_steps,
@@ -559,9 +690,13 @@ def main():
             "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_prof["kernels"].items()},
             "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_eager["kernels"].items()},
         }
+        if not args.skip_per_op:
+            progress(f"frodo976 batch {args.frodo_batch}: keygen, encaps, decaps timed as separate plans, "
+                     "and in the plain form")
+            result["per_op"]["frodo976"] = run_per_op(torch, eng, pe, D.FRODO976, "frodo976", args.frodo_batch,
+                                                      max(2, args.frodo_steps // 2), 0x44504B454D000002)
     if not args.skip_plain:
         progress("plain comparator forms")
-        pe = D.Engine(0, form=D.FORM_PLAIN)
         sp = run_workload(torch, pe, D.SABER, args.saber_batch, args.plain_steps, 1, 0x44504B454D000001)
         plain = {"saber_ops_per_s": round(sp["ops_per_s"], 1),
                  "saber_speedup_batched_over_plain": round(sab["ops_per_s"] / sp["ops_per_s"], 2)}
@@ -569,7 +704,6 @@ def main():
             fp = run_workload(torch, pe, D.FRODO976, args.frodo_batch, 1, 0, 0x44504B454D000002)
             plain["frodo976_ops_per_s"] = round(fp["ops_per_s"], 1)
             plain["frodo976_speedup_batched_over_plain"] = round(fr["ops_per_s"] / fp["ops_per_s"], 2)
-        pe.close()
         result["plain"] = plain
     if not args.skip_frodo:
         progress(f"frodo976-aes batch {args.frodo_batch}: autotuned graph steps, then an eager pass with HIP events")
@@ -627,9 +761,10 @@ def main():
         "note": "encaps/decaps reuse keygen's expanded matrix; valid only when one caller runs all three ops"}
     if not args.skip_mixed:
         from dpcrypto_amd import workloads as W
-        pe = D.Engine(0, form=D.FORM_PLAIN)
-        result["mixed"] = W.run_mixed(torch, eng, pe, progress=progress)
-        pe.close()
+        mixed_plain = pe if pe is not None else D.Engine(0, form=D.FORM_PLAIN)
+        result["mixed"] = W.run_mixed(torch, eng, mixed_plain, progress=progress)
+        if pe is None:
+            mixed_plain.close()
         progress(f"mixed: {result['mixed']['batched_ops_per_s']:.0f} ops/s batched, "
                  f"{result['mixed']['plain_ops_per_s']:.0f} plain")
     if not args.skip_serving:
@@ -663,6 +798,8 @@ def main():
     if args.cpu_seconds > 0:
         progress(f"cpu baseline ({args.cpu_seconds:.0f} s)")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    if pe is not None:
+        pe.close()
     eng.close()
     print(json.dumps(result), flush=True)
 

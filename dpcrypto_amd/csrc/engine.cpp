@@ -203,14 +203,13 @@ void launch_chunk_shared(hipStream_t st, int scheme, int op, int n, uint8_t* o0,
 }
 
 // shared_in: -1, or the index of the input that is one row for all n
-// instances (ENC: 0 = pk, DEC: 1 = sk).
+// instances (ENC: 0 = pk, DEC: 1 = sk).  form_override: -1 (the engine's
+// form) or the DPKEM_FORM_* this call runs in, without changing the engine's.
 int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
-           const uint8_t* i1, int mem, int shared_in = -1) {
+           const uint8_t* i1, int mem, int shared_in = -1, int form_override = -1) {
   if (!e || !valid_scheme(scheme)) return DPKEM_EINVAL;
-  if (batched_only(scheme) && e->form != DPKEM_FORM_BATCHED) {
-    snprintf(e->last_err, sizeof e->last_err, "this parameter set runs in the batched form only");
+  if (form_override != -1 && form_override != DPKEM_FORM_BATCHED && form_override != DPKEM_FORM_PLAIN)
     return DPKEM_EINVAL;
-  }
   if (mem != DPKEM_MEM_HOST && mem != DPKEM_MEM_DEVICE) return DPKEM_EINVAL;
   if (n == 0) return DPKEM_OK;
   size_t out[2], in[2];
@@ -225,11 +224,16 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
       return DPKEM_EINVAL;
     }
   }
-  if (shared_in >= 0 && e->form != DPKEM_FORM_BATCHED) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  const int form = form_override >= 0 ? form_override : e->form;
+  if (batched_only(scheme) && form != DPKEM_FORM_BATCHED) {
+    snprintf(e->last_err, sizeof e->last_err, "this parameter set runs in the batched form only");
+    return DPKEM_EINVAL;
+  }
+  if (shared_in >= 0 && form != DPKEM_FORM_BATCHED) {
     snprintf(e->last_err, sizeof e->last_err, "shared-key ops run in the batched form only");
     return DPKEM_EINVAL;
   }
-  std::lock_guard<std::mutex> lock(e->mu);
   HIP_TRY(e, hipSetDevice(e->device));
   size_t io_per = 0;
   for (int k = 0; k < 2; ++k) {
@@ -237,7 +241,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
   }
   // Saber broadcasts the shared key into a dense array
   if (shared_in >= 0 && scheme != DPKEM_FRODO976) io_per += align_up(in[shared_in]);
-  const size_t per = scratch_per_inst(scheme, e->form) + io_per;
+  const size_t per = scratch_per_inst(scheme, form) + io_per;
   // a shared key staged from host memory is one row per chunk
   const size_t fixed = 24 * kAlign + (shared_in >= 0 && mem == DPKEM_MEM_HOST ? align_up(in[shared_in]) : 0);
   size_t chunk = std::min(n, (size_t)INT32_MAX);
@@ -249,7 +253,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
     const int c = (int)std::min(chunk, n - done);
     uint8_t* dout[2];
     const uint8_t* din[2];
-    Carver io{e->arena + align_up(chunk_scratch_bytes(scheme, e->form, chunk))};
+    Carver io{e->arena + align_up(chunk_scratch_bytes(scheme, form, chunk))};
     for (int k = 0; k < 2; ++k) {
       const bool one = k == shared_in;
       if (mem == DPKEM_MEM_DEVICE) {
@@ -269,7 +273,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
       uint8_t* bcast = scheme != DPKEM_FRODO976 ? io.take((size_t)c * in[shared_in]) : nullptr;
       launch_chunk_shared(e->stream, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, bcast);
     } else {
-      launch_chunk_on(e->stream, e->form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, nullptr,
+      launch_chunk_on(e->stream, form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, nullptr,
                       e->saber_ipw);
     }
     tls_profiler = nullptr;
@@ -327,11 +331,7 @@ int wrapped(int scheme, int op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_
     randombytes(coins.data(), coins.size());
     i1 = coins.data();
   }
-  const int prev = e->form;
-  e->form = form;
-  const int rc = run_op(e, scheme, op, n, o0, o1, i0, i1, DPKEM_MEM_HOST);
-  e->form = prev;
-  return rc;
+  return run_op(e, scheme, op, n, o0, o1, i0, i1, DPKEM_MEM_HOST, -1, form);
 }
 
 }  // namespace

@@ -12,6 +12,8 @@
 // and ciphertexts never leave HBM: keygen returns a key handle, encaps a
 // ciphertext handle; only coins go in and shared secrets come out.
 #include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -136,6 +138,12 @@ struct dpkem_server {
   // statistics
   uint64_t batches = 0, requests = 0, max_seen_batch = 0;
   uint64_t key_groups = 0, grouped_requests = 0;  // Frodo members launched in same-key groups
+  uint64_t failed_batches = 0;
+  // host-side fault injection for tests of the failure path (environment
+  // variable DPKEM_SERVER_INJECT_FAULTS="L,E": the first L batch launches
+  // report a launch error before any HIP call, and the completion events of
+  // the next E batches read as failed); zero in normal use
+  int inject_launch_faults = 0, inject_event_faults = 0;
 };
 
 namespace {
@@ -160,8 +168,12 @@ size_t lane_bytes(int scheme, int form, size_t m) {
          a256(m * 96) + 2 * a256(m * 4) + a256(m * 8) + a256(m * 24) + 4096;
 }
 
-// Launch one admitted batch on lane L (called with s->mu NOT held).
-void launch_batch(dpkem_server* s, Lane& L) {
+// Launch one admitted batch on lane L (called with s->mu NOT held).  Returns
+// the first HIP error of the launch sequence; *ngroups_out receives the
+// number of same-key groups launched (0 when the batch is not grouped).
+hipError_t launch_batch(dpkem_server* s, Lane& L, bool inject_fault, int* ngroups_out) {
+  *ngroups_out = 0;
+  if (inject_fault) return hipErrorLaunchFailure;
   const int sc = L.scheme, op = L.op;
   const SchemeDims& d = kDims[sc];
   const int m = (int)L.batch.size();
@@ -203,13 +215,16 @@ void launch_batch(dpkem_server* s, Lane& L) {
       h_groups[ngroups++] = make_uint2((uint32_t)i, (uint32_t)(j - i));
       i = j;
     }
-    s->key_groups += (uint64_t)ngroups;
-    s->grouped_requests += (uint64_t)m;
+    *ngroups_out = ngroups;
   }
   hipStream_t st = L.stream;
+  hipError_t first = hipSuccess;
+  auto note = [&](hipError_t e) {
+    if (first == hipSuccess && e != hipSuccess) first = e;
+  };
   const size_t stage = a256((size_t)m * 96) + 2 * a256((size_t)m * 4) + (grouped ? a256((size_t)m * 8) : 0);
   // coins, index arrays and groups are laid out identically on both sides: one copy
-  (void)hipMemcpyAsync(d_coins, h_coins, stage, hipMemcpyHostToDevice, st);
+  note(hipMemcpyAsync(d_coins, h_coins, stage, hipMemcpyHostToDevice, st));
   const dim3 g(m), b(256);
   if (op == kKeygen) {
     launch_chunk_on(st, s->form, sc, op, m, d_pk, d_sk, d_coins, nullptr, base);
@@ -227,15 +242,57 @@ void launch_batch(dpkem_server* s, Lane& L) {
     if (grouped) launch_frodo_grouped(st, op, m, d_ct, d_ss, d_pk, d_coins, base, d_groups, ngroups, pkh);
     else launch_chunk_on(st, s->form, sc, op, m, d_ct, d_ss, d_pk, d_coins, base, pkh);
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx1, d_ct, s->ct_slab[sc], d.ct, 0);
-    (void)hipMemcpyAsync(L.host_ss, d_ss, (size_t)m * d.ss, hipMemcpyDeviceToHost, st);
+    note(hipGetLastError());
+    note(hipMemcpyAsync(L.host_ss, d_ss, (size_t)m * d.ss, hipMemcpyDeviceToHost, st));
   } else {
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx0, s->sk_slab[sc], d_sk, d.sk, 1);
     hipLaunchKernelGGL(slab_copy, g, b, 0, st, m, d_idx1, s->ct_slab[sc], d_ct, d.ct, 1);
     if (grouped) launch_frodo_grouped(st, op, m, d_ss, nullptr, d_ct, d_sk, base, d_groups, ngroups);
     else launch_chunk_on(st, s->form, sc, op, m, d_ss, nullptr, d_ct, d_sk, base);
-    (void)hipMemcpyAsync(L.host_ss, d_ss, (size_t)m * d.ss, hipMemcpyDeviceToHost, st);
+    note(hipGetLastError());
+    note(hipMemcpyAsync(L.host_ss, d_ss, (size_t)m * d.ss, hipMemcpyDeviceToHost, st));
   }
-  (void)hipEventRecord(L.done, st);
+  note(hipGetLastError());
+  note(hipEventRecord(L.done, st));
+  return first;
+}
+
+// Retire lane L's batch (s->mu held).  status DPKEM_OK: every request
+// completes with its result.  Otherwise every request completes with that
+// status and no result, and the key / ciphertext slots its admission took are
+// returned to their pools (after the lane's stream has stopped using them).
+void retire_lane(dpkem_server* s, Lane& L, int status) {
+  const double t = now_us();
+  const SchemeDims& d = kDims[L.scheme];
+  if (status != DPKEM_OK) {
+    (void)hipStreamSynchronize(L.stream);
+    for (uint32_t h : L.new_handles) {
+      if (L.op == kKeygen) s->keys[L.scheme].give(h);
+      else if (L.op == kEnc) s->cts[L.scheme].give(h);
+    }
+    s->failed_batches++;
+  }
+  for (size_t i = 0; i < L.batch.size(); ++i) {
+    dpkem_completion c;
+    memset(&c, 0, sizeof c);
+    c.id = L.batch[i].r.id;
+    c.status = status;
+    c.scheme = L.scheme;
+    c.op = L.op;
+    c.t_submit_us = L.batch[i].r.t_arrival_us > 0 ? L.batch[i].r.t_arrival_us : L.batch[i].t_submit;
+    c.t_admit_us = L.t_admit;
+    c.t_done_us = t;
+    if (status == DPKEM_OK) {
+      if (L.op == kKeygen) c.key = L.new_handles[i];
+      if (L.op == kEnc) c.ct = L.new_handles[i];
+      if (L.op != kKeygen) memcpy(c.ss, L.host_ss + i * d.ss, d.ss);
+    }
+    s->done.push_back(c);
+  }
+  s->inflight_requests -= L.batch.size();
+  L.batch.clear();
+  L.new_handles.clear();
+  L.busy = false;
 }
 
 // Choose the (scheme, op) group whose oldest request arrived first, among the
@@ -264,30 +321,17 @@ void worker_loop(dpkem_server* s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
-    // 1. retire finished lanes
+    // 1. retire finished lanes: the completion event has fired (results
+    // valid) or reports an error (the batch failed)
     for (auto& L : s->lanes) {
-      if (!L.busy || hipEventQuery(L.done) != hipSuccess) continue;
-      const double t = now_us();
-      const SchemeDims& d = kDims[L.scheme];
-      for (size_t i = 0; i < L.batch.size(); ++i) {
-        dpkem_completion c;
-        memset(&c, 0, sizeof c);
-        c.id = L.batch[i].r.id;
-        c.status = DPKEM_OK;
-        c.scheme = L.scheme;
-        c.op = L.op;
-        c.t_submit_us = L.batch[i].r.t_arrival_us > 0 ? L.batch[i].r.t_arrival_us : L.batch[i].t_submit;
-        c.t_admit_us = L.t_admit;
-        c.t_done_us = t;
-        if (L.op == kKeygen) c.key = L.new_handles[i];
-        if (L.op == kEnc) c.ct = L.new_handles[i];
-        if (L.op != kKeygen) memcpy(c.ss, L.host_ss + i * d.ss, d.ss);
-        s->done.push_back(c);
+      if (!L.busy) continue;
+      hipError_t q = hipEventQuery(L.done);
+      if (q == hipErrorNotReady) continue;
+      if (q == hipSuccess && s->inject_event_faults > 0) {
+        s->inject_event_faults--;
+        q = hipErrorLaunchFailure;
       }
-      s->inflight_requests -= L.batch.size();
-      L.batch.clear();
-      L.new_handles.clear();
-      L.busy = false;
+      retire_lane(s, L, q == hipSuccess ? DPKEM_OK : DPKEM_EDEVICE);
     }
     if (s->inflight_requests == 0) s->drained.notify_all();
     // 2. admit new batches into free lanes
@@ -350,9 +394,20 @@ void worker_loop(dpkem_server* s) {
       s->batches++;
       s->requests += L.batch.size();
       s->max_seen_batch = std::max<uint64_t>(s->max_seen_batch, L.batch.size());
+      const bool inject = s->inject_launch_faults > 0;
+      if (inject) s->inject_launch_faults--;
       lk.unlock();
-      launch_batch(s, L);
+      int ngroups = 0;
+      const hipError_t err = launch_batch(s, L, inject, &ngroups);
       lk.lock();
+      if (ngroups) {
+        s->key_groups += (uint64_t)ngroups;
+        s->grouped_requests += (uint64_t)L.batch.size();
+      }
+      if (err != hipSuccess) {
+        snprintf(s->last_err, sizeof s->last_err, "batch launch: %s", hipGetErrorString(err));
+        retire_lane(s, L, DPKEM_EDEVICE);
+      }
       launched = true;
     }
     if (s->stop && s->inflight_requests == 0) break;
@@ -405,6 +460,9 @@ int dpkem_server_create(dpkem_server** out, const dpkem_server_config* cfg) {
   if (!s) return DPKEM_ENOMEM;
   s->device = cfg->device;
   s->form = cfg->form;
+  if (const char* inj = getenv("DPKEM_SERVER_INJECT_FAULTS")) {
+    if (sscanf(inj, "%d,%d", &s->inject_launch_faults, &s->inject_event_faults) < 1) s->inject_launch_faults = 0;
+  }
   s->group_by_key = cfg->group_by_key != 0 && cfg->form == DPKEM_FORM_BATCHED;
   (void)hipSetDevice(s->device);
   size_t budget = cfg->hbm_budget_bytes;
@@ -565,10 +623,11 @@ int dpkem_server_stats(dpkem_server* s, char* buf, size_t len) {
   const int w = snprintf(buf, len,
                          "{\"batches\": %llu, \"requests\": %llu, \"max_batch_seen\": %llu, \"lanes\": %zu, "
                          "\"max_batch_frodo976\": %zu, \"max_batch_saber\": %zu, \"hbm_budget\": %zu, "
-                         "\"frodo_key_groups\": %llu, \"frodo_grouped_requests\": %llu}",
+                         "\"frodo_key_groups\": %llu, \"frodo_grouped_requests\": %llu, \"failed_batches\": %llu}",
                          (unsigned long long)s->batches, (unsigned long long)s->requests,
                          (unsigned long long)s->max_seen_batch, s->lanes.size(), s->max_batch[0], s->max_batch[1],
-                         s->budget, (unsigned long long)s->key_groups, (unsigned long long)s->grouped_requests);
+                         s->budget, (unsigned long long)s->key_groups, (unsigned long long)s->grouped_requests,
+                         (unsigned long long)s->failed_batches);
   return (w > 0 && (size_t)w < len) ? DPKEM_OK : DPKEM_EINVAL;
 }
 

@@ -128,6 +128,52 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
+def check_device_array(x, width, rows=None, device=None, what="array", min_rows=None, placement=True):
+    """Raise unless `x` is a contiguous 2-D uint8 CUDA tensor of `width`
+    bytes per row (exactly `rows` / at least `min_rows` rows, and on GPU
+    `device`, when given).  The library receives only a base pointer, so a
+    tensor of the wrong shape would make its kernels read or write past the
+    tensor's end.  placement=False checks everything but where it lives."""
+    if not _is_torch(x):
+        raise TypeError(f"{what}: device-resident arrays must be torch CUDA tensors, got {type(x).__name__}")
+    import torch
+    if x.dtype != torch.uint8:
+        raise TypeError(f"{what}: dtype must be torch.uint8, got {x.dtype}")
+    if x.dim() != 2 or x.shape[1] != width:
+        raise ValueError(f"{what}: expected shape (n, {width}), got {tuple(x.shape)}")
+    if rows is not None and x.shape[0] != rows:
+        raise ValueError(f"{what}: expected {rows} rows, got {x.shape[0]}")
+    if min_rows is not None and x.shape[0] < min_rows:
+        raise ValueError(f"{what}: {x.shape[0]} rows for {min_rows} instances")
+    if not x.is_contiguous():
+        raise ValueError(f"{what}: tensor must be contiguous")
+    if not placement:
+        return
+    if not x.is_cuda:
+        raise TypeError(f"{what}: tensor must live on a GPU, got device {x.device}")
+    if device is not None and x.device.index != device:
+        raise ValueError(f"{what}: tensor is on cuda:{x.device.index}, the engine on cuda:{device}")
+
+
+def check_role_arrays(scheme, op, n, arrays, device):
+    """Check the (o0, o1, i0, i1) arrays of one op against the scheme's
+    sizes: every shape first, then where each lives."""
+    if scheme not in SIZES or op not in ROLES:
+        raise ValueError(f"unknown scheme {scheme} or op {op}")
+    todo = []
+    for x, role in zip(arrays, ROLES[op]):
+        if role is None:
+            if x is not None:
+                raise ValueError(f"op {op} takes no array in this position")
+            continue
+        if x is None:
+            raise ValueError(f"{role}: missing")
+        todo.append((x, role))
+    for placement in (False, True):
+        for x, role in todo:
+            check_device_array(x, SIZES[scheme][role], device=device, what=role, min_rows=n, placement=placement)
+
+
 def _ptr(x):
     if x is None:
         return None
@@ -198,8 +244,8 @@ class Engine:
         if kinds.pop():
             import torch
             for a in arrays:
-                if a is not None and (not a.is_cuda or not a.is_contiguous() or a.dtype != torch.uint8):
-                    raise TypeError("device arrays must be contiguous uint8 CUDA tensors")
+                if a is not None:
+                    check_device_array(a, a.shape[-1] if a.dim() else 0, device=self.device)
             # order the engine's work after / before torch's on the same stream
             self.set_stream(torch.cuda.current_stream(arrays[0].device).cuda_stream)
             return MEM_DEVICE
@@ -212,16 +258,20 @@ class Engine:
             return torch.empty((n, width), dtype=torch.uint8, device=ref.device)
         return np.empty((n, width), np.uint8)
 
-    @staticmethod
-    def _host(x, width):
+    def _host(self, x, width, what="array"):
+        """Validate one input array: a torch tensor must already be a
+        contiguous (rows, width) uint8 tensor on the engine's GPU (the kernels
+        index it with that row width); host data is copied into a contiguous
+        (rows, width) uint8 array."""
         if _is_torch(x):
+            check_device_array(x, width, what=what, placement=False)
             return x
         return np.ascontiguousarray(x, dtype=np.uint8).reshape(-1, width)
 
     # -------- KEM
     def keypair(self, scheme: int, coins):
         s = SIZES[scheme]
-        coins = self._host(coins, s["kcoins"])
+        coins = self._host(coins, s["kcoins"], "coins")
         n = coins.shape[0]
         pk = self._empty_like_kind(coins, n, s["pk"])
         sk = self._empty_like_kind(coins, n, s["sk"])
@@ -231,7 +281,7 @@ class Engine:
 
     def enc(self, scheme: int, pk, coins):
         s = SIZES[scheme]
-        pk = self._host(pk, s["pk"]); coins = self._host(coins, s["ecoins"])
+        pk = self._host(pk, s["pk"], "pk"); coins = self._host(coins, s["ecoins"], "coins")
         n = pk.shape[0]
         if coins.shape[0] != n:
             raise ValueError("pk and coins batch sizes differ")
@@ -243,7 +293,7 @@ class Engine:
 
     def dec(self, scheme: int, ct, sk):
         s = SIZES[scheme]
-        ct = self._host(ct, s["ct"]); sk = self._host(sk, s["sk"])
+        ct = self._host(ct, s["ct"], "ct"); sk = self._host(sk, s["sk"], "sk")
         n = ct.shape[0]
         if sk.shape[0] != n:
             raise ValueError("ct and sk batch sizes differ")
@@ -255,7 +305,7 @@ class Engine:
     def enc_shared_key(self, scheme: int, pk, coins):
         """n encapsulations to one public key (pk: one row)."""
         s = SIZES[scheme]
-        pk = self._host(pk, s["pk"]); coins = self._host(coins, s["ecoins"])
+        pk = self._host(pk, s["pk"], "pk"); coins = self._host(coins, s["ecoins"], "coins")
         if pk.shape[0] != 1:
             raise ValueError("pk must be one row")
         n = coins.shape[0]
@@ -269,7 +319,7 @@ class Engine:
     def dec_shared_key(self, scheme: int, ct, sk):
         """n decapsulations with one secret key (sk: one row)."""
         s = SIZES[scheme]
-        ct = self._host(ct, s["ct"]); sk = self._host(sk, s["sk"])
+        ct = self._host(ct, s["ct"], "ct"); sk = self._host(sk, s["sk"], "sk")
         if sk.shape[0] != 1:
             raise ValueError("sk must be one row")
         n = ct.shape[0]
@@ -283,18 +333,28 @@ class Engine:
              graph: bool = True) -> "Plan":
         return Plan(self, nstreams, chunks, share_matrix, pipeline, graph)
 
-    # raw pointer entry points (device arrays preallocated by the caller)
+    # raw pointer entry points (device arrays preallocated by the caller; each
+    # holds at least n rows of its role's width)
+    def _check_roles(self, scheme, op, n, arrays):
+        check_role_arrays(scheme, op, n, arrays, self.device)
+
     def keypair_into(self, scheme, n, pk, sk, coins):
+        self._check_roles(scheme, OP_KEYGEN, n, (pk, sk, coins, None))
         self._check(self.lib.dpkem_keypair(self.h, scheme, n, _ptr(pk), _ptr(sk), _ptr(coins), MEM_DEVICE), "keypair")
 
     def enc_into(self, scheme, n, ct, ss, pk, coins):
+        self._check_roles(scheme, OP_ENC, n, (ct, ss, pk, coins))
         self._check(self.lib.dpkem_enc(self.h, scheme, n, _ptr(ct), _ptr(ss), _ptr(pk), _ptr(coins), MEM_DEVICE), "enc")
 
     def dec_into(self, scheme, n, ss, ct, sk):
+        self._check_roles(scheme, OP_DEC, n, (ss, None, ct, sk))
         self._check(self.lib.dpkem_dec(self.h, scheme, n, _ptr(ss), _ptr(ct), _ptr(sk), MEM_DEVICE), "dec")
 
 
 OP_KEYGEN, OP_ENC, OP_DEC = 0, 1, 2
+# per op, the SIZES key of each array role (o0, o1, i0, i1) of dpkem_plan_add
+ROLES = {OP_KEYGEN: ("pk", "sk", "kcoins", None), OP_ENC: ("ct", "ss", "pk", "ecoins"),
+         OP_DEC: ("ss", None, "ct", "sk")}
 PLAN_SHARE_MATRIX = 1
 PLAN_NO_PIPELINE = 2
 PLAN_NO_GRAPH = 4
@@ -317,6 +377,11 @@ class Plan:
             engine._check(self.lib.dpkem_plan_set_options(self.h, flags), "plan_set_options")
 
     def add(self, scheme, op, n, o0=None, o1=None, i0=None, i1=None, after=-1) -> int:
+        """Record a task over device arrays.  A graph replays the recorded
+        pointers later, so every array must be a torch CUDA tensor on the
+        engine's GPU with at least n rows of its role's width (host arrays
+        would be dereferenced by the GPU)."""
+        check_role_arrays(scheme, op, n, (o0, o1, i0, i1), self.eng.device)
         tid = ctypes.c_int(-1)
         self._keep += [a for a in (o0, o1, i0, i1) if a is not None]
         rc = self.lib.dpkem_plan_add(self.h, scheme, op, n, _ptr(o0), _ptr(o1), _ptr(i0), _ptr(i1), after,

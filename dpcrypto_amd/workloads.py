@@ -50,7 +50,9 @@ def mixed_batches(seed: int, total: int = 16384, max_batch: int = 4096):
 
 
 def run_mixed(torch, eng, plain_eng, seed: int = 0x44504B454D000004, total: int = 16384, max_batch: int = 4096,
-              reps: int = 3, tune_budget_s: float = 30.0, progress=None):
+              reps: int = 3, tune_budget_s: float = 30.0, progress=None, keep_io: bool = False):
+    """keep_io: also return every batch's (scheme, op, n, inputs, (plan
+    outputs, plain outputs)) under "io", for checks of individual instances."""
     from . import autotune
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(seed + 1)
@@ -167,6 +169,7 @@ def run_mixed(torch, eng, plain_eng, seed: int = 0x44504B454D000004, total: int 
         "plain_ms": round(plain_ms, 3),
         "speedup": round(plain_ms / batched_ms, 2),
         "outputs_identical": bool(identical),
+        **({"io": jobs} if keep_io else {}),
     }
 
 

@@ -304,6 +304,13 @@ int dpkem_saber_crypto_kem_dec(size_t n, uint8_t *ss, const uint8_t *ct, const u
  * reference, every instance uses its own inputs, and saber_dec returns the
  * shared secrets in h_k.  `mode` 0 selects the batched form, 1 the plain form
  * (the reference's DP / INT32 switch, SABER-GPU/main.cu:25-33).
+ * Keys are INPUTS here: saber_enc reads h_pk and saber_dec reads h_sk (h_pk
+ * is unused: the public key is part of the secret key), so the caller fills
+ * them first with dpkem_saber_crypto_kem_keypair.  The reference instead
+ * overwrites them inside the drivers with its built-in test vector
+ * (SABER-GPU/src/cuda_kernel.cu:59-60,198-199) and its main.cu passes them
+ * uninitialised; a caller relinked against this library must add the
+ * keypair call, or it encapsulates to whatever bytes the buffers hold.
  *   frodoKEMKeypair/Encap/Decap  FrodoKEM976-GPU/include/cuda_kernel.cuh:7-9 (BLOCK = 768, :2)
  *   saber_enc / saber_dec        SABER-GPU/include/cuda_kernel.cuh:6-7 (BATCH = 768, :1)
  * Return 0 on success (the reference returns void). */

@@ -42,45 +42,65 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def build_native(out_path: str) -> str:
+    """Compile the oracle with -O3 -march=native for this host into out_path
+    (used by the benchmark's CPU baseline on the machine it measures)."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", f"NATIVE_OUT={out_path}"], check=True)
+    return out_path
+
+
+def use_library(path: str | None) -> None:
+    """Route the functions of this module to another build of the oracle
+    (e.g. build_native's), or back to liboracle.so with None."""
+    global _lib
+    _lib = _bind(path) if path else None
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        _lib = ctypes.CDLL(_LIB_PATH)
-        vp = ctypes.c_void_p
-        sz = ctypes.c_size_t
-        _lib.orc_shake128.argtypes = [vp, sz, vp, sz]
-        _lib.orc_shake256.argtypes = [vp, sz, vp, sz]
-        _lib.orc_sha3_256.argtypes = [vp, vp, sz]
-        _lib.orc_sha3_512.argtypes = [vp, vp, sz]
-        _lib.orc_keccak_f1600.argtypes = [vp]
-        _lib.orc_prng_fill.argtypes = [ctypes.c_uint64, vp, sz]
-        _lib.orc_batch.argtypes = [ctypes.c_int, ctypes.c_int, sz, ctypes.c_int, vp, vp, vp, vp]
-        _lib.orc_batch.restype = ctypes.c_int
-        _lib.orc_frodo_sample.argtypes = [vp, sz]
-        _lib.orc_frodo_n_sample.argtypes = [ctypes.c_int, vp, sz]
-        _lib.orc_frodo_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
-        _lib.orc_frodo_aes_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
-        _lib.orc_aes128_ecb.argtypes = [vp, vp, vp, sz]
-        _lib.orc_saber_l_gen_matrix.argtypes = [ctypes.c_int, vp, vp]
-        _lib.orc_saber_l_gen_secret.argtypes = [ctypes.c_int, vp, vp]
-        _lib.orc_frodo_pack16.argtypes = [vp, vp, sz]
-        _lib.orc_frodo_unpack16.argtypes = [vp, vp, sz]
-        _lib.orc_frodo_encode.argtypes = [vp, vp]
-        _lib.orc_frodo_decode.argtypes = [vp, vp]
-        _lib.orc_frodo_as_plus_e.argtypes = [vp, vp, vp, vp]
-        _lib.orc_frodo_sa_plus_e.argtypes = [vp, vp, vp, vp]
-        _lib.orc_saber_gen_matrix.argtypes = [vp, vp]
-        _lib.orc_saber_gen_secret.argtypes = [vp, vp]
-        _lib.orc_saber_polymul_acc.argtypes = [vp, vp, vp]
-        _lib.orc_saber_matvec.argtypes = [vp, vp, vp, ctypes.c_int]
-        _lib.orc_saber_pack13.argtypes = [vp, vp, sz]
-        _lib.orc_saber_unpack13.argtypes = [vp, vp, sz]
-        _lib.orc_saber_pack10.argtypes = [vp, vp, sz]
-        _lib.orc_saber_unpack10.argtypes = [vp, vp, sz]
-        _lib.orc_saber_indcpa_enc.argtypes = [vp, vp, vp, vp]
+        _lib = _bind(_LIB_PATH)
     return _lib
+
+
+def _bind(path: str):
+    """Load one build of the oracle and declare its argument types."""
+    lib_ = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    sz = ctypes.c_size_t
+    lib_.orc_shake128.argtypes = [vp, sz, vp, sz]
+    lib_.orc_shake256.argtypes = [vp, sz, vp, sz]
+    lib_.orc_sha3_256.argtypes = [vp, vp, sz]
+    lib_.orc_sha3_512.argtypes = [vp, vp, sz]
+    lib_.orc_keccak_f1600.argtypes = [vp]
+    lib_.orc_prng_fill.argtypes = [ctypes.c_uint64, vp, sz]
+    lib_.orc_batch.argtypes = [ctypes.c_int, ctypes.c_int, sz, ctypes.c_int, vp, vp, vp, vp]
+    lib_.orc_batch.restype = ctypes.c_int
+    lib_.orc_frodo_sample.argtypes = [vp, sz]
+    lib_.orc_frodo_n_sample.argtypes = [ctypes.c_int, vp, sz]
+    lib_.orc_frodo_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
+    lib_.orc_frodo_aes_gen_a_row.argtypes = [vp, vp, ctypes.c_uint16]
+    lib_.orc_aes128_ecb.argtypes = [vp, vp, vp, sz]
+    lib_.orc_saber_l_gen_matrix.argtypes = [ctypes.c_int, vp, vp]
+    lib_.orc_saber_l_gen_secret.argtypes = [ctypes.c_int, vp, vp]
+    lib_.orc_frodo_pack16.argtypes = [vp, vp, sz]
+    lib_.orc_frodo_unpack16.argtypes = [vp, vp, sz]
+    lib_.orc_frodo_encode.argtypes = [vp, vp]
+    lib_.orc_frodo_decode.argtypes = [vp, vp]
+    lib_.orc_frodo_as_plus_e.argtypes = [vp, vp, vp, vp]
+    lib_.orc_frodo_sa_plus_e.argtypes = [vp, vp, vp, vp]
+    lib_.orc_saber_gen_matrix.argtypes = [vp, vp]
+    lib_.orc_saber_gen_secret.argtypes = [vp, vp]
+    lib_.orc_saber_polymul_acc.argtypes = [vp, vp, vp]
+    lib_.orc_saber_matvec.argtypes = [vp, vp, vp, ctypes.c_int]
+    lib_.orc_saber_pack13.argtypes = [vp, vp, sz]
+    lib_.orc_saber_unpack13.argtypes = [vp, vp, sz]
+    lib_.orc_saber_pack10.argtypes = [vp, vp, sz]
+    lib_.orc_saber_unpack10.argtypes = [vp, vp, sz]
+    lib_.orc_saber_indcpa_enc.argtypes = [vp, vp, vp, vp]
+    return lib_
 
 
 def _p(a: np.ndarray):

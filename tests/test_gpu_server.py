@@ -153,3 +153,39 @@ def test_frodo_requests_grouped_by_key_equal_the_oracle(oracle):
         assert st["frodo_grouped_requests"] >= 44 and st["frodo_key_groups"] < st["frodo_grouped_requests"]
     finally:
         srv.close()
+
+
+def test_failed_batches_complete_with_edevice_and_release_their_slots(oracle, monkeypatch):
+    """A batch whose launch fails, or whose completion event reports an error,
+    completes every request with DPKEM_EDEVICE and hands back the key slots
+    its admission took; the server keeps serving.  The faults are injected on
+    the host (DPKEM_SERVER_INJECT_FAULTS: the first two launches fail before
+    any HIP call, the next batch's event reads as failed), so every kernel
+    that does run exits normally."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import dpcrypto_amd as D
+    monkeypatch.setenv("DPKEM_SERVER_INJECT_FAULTS", "2,1")
+    s = D.Server(0, lanes=1, hbm_budget=2 << 30, key_slots=(8, 8), ct_slots=(8, 8), max_batch=(4, 4),
+                 group_by_key=False)
+    monkeypatch.delenv("DPKEM_SERVER_INJECT_FAULTS")
+    try:
+        kc = oracle.prng_bytes(6001, 16 * 96).reshape(16, 96)
+        reqs = np.array([_req(i, SABER, KEYGEN, kc[i]) for i in range(16)], D.REQ_DTYPE)
+        out = _run(s, reqs)
+        st = np.array([out[i]["status"] for i in range(16)])
+        # three failed batches of four, then one that succeeds
+        assert (st == -3).sum() == 12 and (st == 0).sum() == 4, st.tolist()
+        assert s.stats()["failed_batches"] == 3
+        ok = [i for i in range(16) if st[i] == 0]
+        opk, osk = oracle.keypair(SABER, kc[ok], threads=2)
+        for j, i in enumerate(ok):
+            assert np.array_equal(s.read(SABER, 0, int(out[i]["key"])), opk[j])
+        # the failed batches returned their slots: 4 more keys fit into the 8 slots, the 9th does not
+        more = np.array([_req(100 + i, SABER, KEYGEN, kc[i]) for i in range(5)], D.REQ_DTYPE)
+        out2 = _run(s, more)
+        st2 = sorted(int(out2[100 + i]["status"]) for i in range(5))
+        assert st2 == [-4, 0, 0, 0, 0], st2
+    finally:
+        s.close()
