@@ -242,6 +242,31 @@ int dpkem_server_read(dpkem_server *s, int scheme, int kind, uint32_t handle, ui
 int dpkem_server_stats(dpkem_server *s, char *buf, size_t len);
 int dpkem_server_destroy(dpkem_server *s);
 
+/* ---------------------------------------------------------------- allocation-free entry point
+ * No counterpart in the reference (whose drivers cudaMalloc and cudaFree on
+ * every call, FrodoKEM976-GPU/src/cuda_kernel.cu:30-46,96-104).  For callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  One keygen / encaps /
+ * decaps (op = DPKEM_OP_*) of n instances of `scheme` in the batched form,
+ * the same kernels and results as dpkem_keypair / dpkem_enc / dpkem_dec with
+ * DPKEM_MEM_DEVICE, launched asynchronously on `hip_stream` (a hipStream_t;
+ * NULL = the null stream) of GPU `device`.  Array roles as in dpkem_plan_add
+ * (KEYGEN: o0 = pk, o1 = sk, i0 = coins; ENC: o0 = ct, o1 = ss, i0 = pk,
+ * i1 = coins; DEC: o0 = ss, i0 = ct, i1 = sk), all device memory on `device`,
+ * 8-byte aligned.  Kernel scratch comes from one static device array of
+ * DPKEM_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends); the call
+ * makes no hipMalloc / hipFree.  n above what that array holds
+ * ((DPKEM_NOALLOC_SCRATCH_BYTES - 4096) / per-instance scratch: Saber 13,796,
+ * LightSaber 26,212, FireSaber 8,737, Frodo-976 4,227, Frodo-640 6,393,
+ * Frodo-1344 3,083 instances) returns DPKEM_EBUDGET before anything runs;
+ * split larger batches.  All calls on one GPU share its scratch array: calls
+ * that overlap there (other streams or threads) race and must be serialised
+ * by the caller.  The first call on a device also looks up the array's
+ * device address there. */
+#define DPKEM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int dpkem_run_noalloc(int device, void *hip_stream, int scheme, int op, size_t n, uint8_t *o0, uint8_t *o1,
+                      const uint8_t *i0, const uint8_t *i1);
+
 /* ---------------------------------------------------------------- randomness
  * PQClean-shaped wrappers draw coins from a randombytes callback.  The
  * default reads the OS entropy source (getrandom). */
