@@ -189,6 +189,38 @@ MEASURED_HBM_BYTES = {
 }
 
 
+# The table above holds only while the kernels are the ones it was read
+# from: each family's device sources are hashed here and compared with the
+# hash recorded beside the table; a mismatch reports traffic as unmeasured.
+KERNEL_SOURCES = {
+    "saber": ("saber.hip", "saber_poly.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
+    "frodo": ("frodo.hip", "frodo_sampler.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
+}
+MEASURED_HBM_SOURCE_SHA = {"saber": None, "frodo": None}
+
+
+def kernel_family(kernel: str) -> str:
+    return "frodo" if kernel.startswith("frodo") else "saber"
+
+
+def source_sha(family: str) -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES[family]:
+        with open(os.path.join(ROOT, "dpcrypto_amd", "csrc", f), "rb") as fh:
+            h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured_hbm(kernel: str, n: int):
+    """(read, write) HBM bytes per launch from the table, or None when the
+    table has no entry or was read from other kernel sources."""
+    fam = kernel_family(kernel)
+    if MEASURED_HBM_SOURCE_SHA.get(fam) != source_sha(fam):
+        return None
+    return MEASURED_HBM_BYTES.get((kernel, n))
+
+
 def lane_ops(kernel: str, n: int) -> float:
     perms, macs = KERNEL_WORK.get(kernel, (0, 0))
     return n * (perms * KECCAK_LANE_OPS + macs / 2.0)
@@ -494,7 +526,7 @@ def roofline_from(stats, n):
     avg_s = total_ms / count / 1e3
     work = lane_ops(name, n)
     achieved = work / avg_s
-    hbm = MEASURED_HBM_BYTES.get((name, n))
+    hbm = measured_hbm(name, n)
     chain = CRITICAL_PERMS.get(name)
     floor = None if chain is None else {
         "critical_perms": chain,
@@ -511,6 +543,9 @@ def roofline_from(stats, n):
         "traffic": None if hbm is None else hbm[0] + hbm[1],
         "traffic_read_write": None if hbm is None else list(hbm),
         "traffic_GBps": None if hbm is None else round((hbm[0] + hbm[1]) / avg_s / 1e9, 1),
+        "traffic_source": ("rocprofv3 FETCH_SIZE / WRITE_SIZE of these kernel sources (sha "
+                           f"{source_sha(kernel_family(name))}), scaled by a copy kernel"
+                           if hbm is not None else "unmeasured for the current kernel sources"),
         "avg_launch_ms": round(avg_s * 1e3, 4),
         "algorithmic_lane_ops_per_launch": work,
         "share_of_step": round(total_ms / sum(v[1] for v in stats.values()), 3),

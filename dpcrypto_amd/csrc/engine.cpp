@@ -8,6 +8,7 @@
 // cut into chunks; host pointers are staged through device buffers; every HIP
 // error becomes a return code.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 #include <sys/random.h>
 
@@ -73,6 +74,20 @@ int fail_hip(dpkem_engine* e, hipError_t err, const char* what) {
     hipError_t _err = (call);                             \
     if (_err != hipSuccess) return fail_hip(e, _err, #call); \
   } while (0)
+
+// After the engine's stream has drained: did a bounded wait inside a kernel
+// give up (its launch then wrote invalid results)?  Clears the record.
+int check_waits(dpkem_engine* e) {
+  uint32_t expired = 0;
+  HIP_TRY(e, dpkem::saber_wait_expired(&expired, true));
+  if (expired) {
+    snprintf(e->last_err, sizeof e->last_err,
+             "a bounded LDS wait gave up inside a Saber-family kernel (%u lanes); results of launches since the "
+             "last check are invalid", expired);
+    return DPKEM_EDEVICE;
+  }
+  return DPKEM_OK;
+}
 
 int ensure_arena(dpkem_engine* e, size_t bytes) {
   if (bytes <= e->arena_bytes) return DPKEM_OK;
@@ -283,6 +298,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
         if (out[k])
           HIP_TRY(e, hipMemcpyAsync(outs[k] + done * out[k], dout[k], (size_t)c * out[k], hipMemcpyDeviceToHost, e->stream));
       HIP_TRY(e, hipStreamSynchronize(e->stream));
+      if ((rc = check_waits(e)) != DPKEM_OK) return rc;
     }
   }
   return DPKEM_OK;
@@ -370,6 +386,15 @@ int dpkem_engine_create(dpkem_engine** out, int device, size_t budget) {
     budget = free_b / 4;
   }
   e->budget = budget;
+  // bound of the kernels' LDS waits (DPKEM_WAIT_SPIN_LIMIT overrides it, e.g.
+  // 0 to exercise the give-up path); the device value is shared by the process
+  uint32_t limit = 1u << 22;
+  if (const char* v = getenv("DPKEM_WAIT_SPIN_LIMIT")) limit = (uint32_t)strtoul(v, nullptr, 0);
+  if (dpkem::saber_set_wait_spin_limit(limit) != hipSuccess) {
+    (void)hipStreamDestroy(e->own_stream);
+    delete e;
+    return DPKEM_EDEVICE;
+  }
   *out = e;
   return DPKEM_OK;
 }
@@ -422,7 +447,7 @@ int dpkem_engine_sync(dpkem_engine* e) {
   std::lock_guard<std::mutex> lock(e->mu);
   HIP_TRY(e, hipSetDevice(e->device));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
-  return DPKEM_OK;
+  return check_waits(e);
 }
 
 const char* dpkem_engine_last_error(dpkem_engine* e) { return e ? e->last_err : "no engine"; }

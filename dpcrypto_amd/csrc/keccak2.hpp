@@ -313,11 +313,24 @@ __device__ __forceinline__ void absorb2_global(uint32_t a[25], uint32_t p, const
 // before reading it.  Every wave of a workgroup is resident from its start
 // and the helper publishes each block once after bounded work, so the waits
 // end; their iteration bound only makes the loops' ends unconditional.
+//
+// A wait that reaches its bound gives up: the wave goes on with whatever the
+// LDS holds (in-bounds reads, garbage results) and adds to wait_expired, a
+// counter of this translation unit that the host reads and clears
+// (saber_wait_expired) and reports as a failed launch.  wait_spin_limit is
+// the bound (host-settable; 0 makes every wait give up at once, which is how
+// the failure path is exercised without a faulty kernel).
+static __device__ uint32_t wait_spin_limit = 1u << 22;
+static __device__ uint32_t wait_expired = 0;
+
 __device__ __forceinline__ void lds_wait_count(const uint32_t* cnt, uint32_t target) {
-  for (int spin = 0; spin < (1 << 22); ++spin) {
+  const uint32_t limit = wait_spin_limit;
+  for (uint32_t spin = 0; limit != 0; ++spin) {
     if (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return;
+    if (spin >= limit) break;
     __builtin_amdgcn_s_sleep(1);
   }
+  __hip_atomic_fetch_add(&wait_expired, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Helper side: the NW message words at w for slot s with half p; lane 0 of

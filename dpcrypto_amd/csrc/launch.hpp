@@ -19,6 +19,12 @@ void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* c
 void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
                       uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l = 3, int ipw = 1);
 
+// Bounded LDS waits of the Saber-family kernels (keccak2.hpp lds_wait_count)
+// on the current device: how many gave up since the last clear, and their
+// spin bound.
+hipError_t saber_wait_expired(uint32_t* count, bool clear);
+hipError_t saber_set_wait_spin_limit(uint32_t limit);
+
 // dst[i] = src for n rows of `bytes` bytes (a multiple of 4).
 void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t bytes, uint8_t* dst);
 // Chained Saber keygen -> enc -> dec of the same instances on one stream,

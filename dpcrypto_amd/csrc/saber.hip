@@ -1166,6 +1166,19 @@ void dec_l(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t*
                ss);
 }
 
+hipError_t saber_wait_expired(uint32_t* count, bool clear) {
+  hipError_t err = hipMemcpyFromSymbol(count, HIP_SYMBOL(k2::wait_expired), sizeof(uint32_t), 0, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && clear && *count) {
+    const uint32_t zero = 0;
+    err = hipMemcpyToSymbol(HIP_SYMBOL(k2::wait_expired), &zero, sizeof zero, 0, hipMemcpyHostToDevice);
+  }
+  return err;
+}
+
+hipError_t saber_set_wait_spin_limit(uint32_t limit) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(k2::wait_spin_limit), &limit, sizeof limit, 0, hipMemcpyHostToDevice);
+}
+
 size_t saber_scratch_per_inst(int l) {
   const size_t m = l == 2 ? SaberP<2>::matrix_bytes : l == 3 ? SaberP<3>::matrix_bytes : SaberP<4>::matrix_bytes;
   const size_t s = l == 2 ? SaberP<2>::secret_bytes : l == 3 ? SaberP<3>::secret_bytes : SaberP<4>::secret_bytes;

@@ -559,6 +559,13 @@ int dpkem_server_drain(dpkem_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   s->cv.notify_one();
   s->drained.wait(lk, [&] { return s->inflight_requests == 0; });
+  // a bounded wait inside a kernel that gave up invalidates what ran since the last check
+  uint32_t expired = 0;
+  (void)hipSetDevice(s->device);
+  if (saber_wait_expired(&expired, true) != hipSuccess || expired) {
+    snprintf(s->last_err, sizeof s->last_err, "a bounded LDS wait gave up inside a Saber-family kernel");
+    return DPKEM_EDEVICE;
+  }
   return DPKEM_OK;
 }
 

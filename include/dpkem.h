@@ -90,7 +90,11 @@ int dpkem_engine_set_form(dpkem_engine *e, int form);
  * product kernel, one wave each: 1 (default), 2 or 4. */
 #define DPKEM_TUNE_SABER_POLY_IPW 1
 int dpkem_engine_set_tuning(dpkem_engine *e, int knob, int value);
-/* Block until all work submitted on the engine's stream has finished. */
+/* Block until all work submitted on the engine's stream has finished.
+ * Returns DPKEM_EDEVICE (with a message in dpkem_engine_last_error) when a
+ * bounded wait inside a kernel gave up since the last check: the results of
+ * the launches since then are invalid.  Host-memory calls check the same
+ * after their final copy, and dpkem_server_drain for the server. */
 int dpkem_engine_sync(dpkem_engine *e);
 /* Last HIP error string seen by this engine (static storage). */
 const char *dpkem_engine_last_error(dpkem_engine *e);
