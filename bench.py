@@ -120,6 +120,17 @@ KERNEL_WORK = {
     "frodo_sa_fused<true>,1344": (1344 * 16, 0),
     "frodo_ss,1344": (160, 0),
     "frodo_dec_mu,1344": (0, 64 * 1344),
+    # decaps' chain beside ss's ct absorb, then ss's last block
+    "frodo_dec_pre": (1 + 231 + 115, 0),
+    "frodo_dec_pre,640": (1 + 123 + 57, 0),
+    "frodo_dec_pre,1344": (1 + 318 + 159, 0),
+    # multi-task launch sequences (independent tasks of one op in one launch each)
+    "frodo_mt_pre<keygen>": (1 + 230, 0),
+    "frodo_mt_post<keygen>": (115, 0),
+    "frodo_mt_pre<enc>": (115 + 1 + 231, 0),
+    "frodo_mt_post<enc>": (116, 0),
+    "frodo_mt_dec_pre": (1 + 231 + 115, 0),
+    "frodo_mt_post<dec>": (1, 0),
 }
 
 

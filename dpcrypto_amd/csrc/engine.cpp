@@ -159,10 +159,11 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     uint8_t* kbuf = c.take((size_t)n * 64);
     uint8_t* mubuf = c.take((size_t)n * 32);
     uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
+    uint32_t* state = reinterpret_cast<uint32_t*>(c.take((size_t)n * 200));
     const bool aes = scheme == DPKEM_FRODO976_AES;
     if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8, aes, set);
     else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh, aes, set);
-    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail, aes, set);
+    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail, state, aes, set);
   }
 }
 
@@ -176,10 +177,13 @@ void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o
   uint8_t* kbuf = c.take((size_t)n * 64);
   uint8_t* mubuf = c.take((size_t)n * 32);
   uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
+  uint32_t* state = reinterpret_cast<uint32_t*>(c.take((size_t)n * 200));
   if (op == kEnc)
     frodo_enc_grouped_launch(st, n, i0, kSizes[DPKEM_FRODO976].pk, i1, o0, o1, s8, e8, epp8, kbuf, groups, ngroups,
                              frodo_pkh);
-  else frodo_dec_grouped_launch(st, n, i0, i1, kSizes[DPKEM_FRODO976].sk, o0, s8, e8, epp8, kbuf, mubuf, fail, groups, ngroups);
+  else
+    frodo_dec_grouped_launch(st, n, i0, i1, kSizes[DPKEM_FRODO976].sk, o0, s8, e8, epp8, kbuf, mubuf, fail, state,
+                             groups, ngroups);
 }
 
 // Upper bound on the arena bytes the chunk carver uses for n instances
@@ -213,8 +217,9 @@ void launch_chunk_shared(hipStream_t st, int scheme, int op, int n, uint8_t* o0,
   uint8_t* kbuf = c.take((size_t)n * 64);
   uint8_t* mubuf = c.take((size_t)n * 32);
   uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
+  uint32_t* state = reinterpret_cast<uint32_t*>(c.take((size_t)n * 200));
   if (op == kEnc) frodo_enc_grouped_launch(st, n, i0, 0, i1, o0, o1, s8, e8, epp8, kbuf, nullptr, 0);
-  else frodo_dec_grouped_launch(st, n, i0, i1, 0, o0, s8, e8, epp8, kbuf, mubuf, fail, nullptr, 0);
+  else frodo_dec_grouped_launch(st, n, i0, i1, 0, o0, s8, e8, epp8, kbuf, mubuf, fail, state, nullptr, 0);
 }
 
 // shared_in: -1, or the index of the input that is one row for all n
@@ -436,10 +441,18 @@ int dpkem_engine_set_form(dpkem_engine* e, int form) {
 }
 
 int dpkem_engine_set_tuning(dpkem_engine* e, int knob, int value) {
-  if (!e || knob != DPKEM_TUNE_SABER_POLY_IPW || (value != 1 && value != 2 && value != 4)) return DPKEM_EINVAL;
-  std::lock_guard<std::mutex> lock(e->mu);
-  e->saber_ipw = value;
-  return DPKEM_OK;
+  if (!e) return DPKEM_EINVAL;
+  if (knob == DPKEM_TUNE_SABER_POLY_IPW && (value == 1 || value == 2 || value == 4)) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    e->saber_ipw = value;
+    return DPKEM_OK;
+  }
+  if (knob == DPKEM_TUNE_FRODO_PERSIST && (value == 0 || value == 1)) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    e->frodo_persist = value;
+    return DPKEM_OK;
+  }
+  return DPKEM_EINVAL;
 }
 
 int dpkem_engine_sync(dpkem_engine* e) {

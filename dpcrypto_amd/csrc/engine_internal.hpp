@@ -18,6 +18,7 @@ struct dpkem_engine {
   hipStream_t stream = nullptr;
   int form = DPKEM_FORM_BATCHED;
   int saber_ipw = 1;  // DPKEM_TUNE_SABER_POLY_IPW
+  int frodo_persist = 1;  // DPKEM_TUNE_FRODO_PERSIST
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
   char last_err[256] = {0};

@@ -27,6 +27,10 @@
 // lo = (int8)((a & 0xFF) ^ 0x80), so a == 256 hi + lo + 128 (mod 2^16);
 // the 128 * sum(S) term is added once per output.  With |S| <= 10 and 976
 // terms every i32 partial sum stays below 2^21 in magnitude.
+#include <string.h>
+
+#include <algorithm>
+
 #include "common.hpp"
 #include "keccak.hpp"
 #include "frodo_sampler.hpp"
@@ -98,6 +102,51 @@ struct FrodoP : FrodoSet<N> {
 using F976 = FrodoP<976>;
 static_assert(F976::pk == kFrodoPk && F976::sk == kFrodoSk && F976::ct == kFrodoCt &&
               F976::kcoins == kFrodoKeygenCoins, "FrodoKEM976-GPU/include/api.h:7-10");
+
+// Task tables.  The fused kernels and the decaps B'S kernel take their
+// work as a table of tasks (FrMt, a kernel argument): per task its instance
+// count, arrays and scratch; a launch over one array is a table of one task.
+// Several independent tasks of one op and parameter set -- uneven batch
+// sizes, each with its own arrays -- can so share one launch sequence
+// (frodo_mt_launch).  The fused kernels are persistent: a grid of at most
+// the resident workgroups walks the concatenated work list (work item = task
+// instance) with a fixed stride.
+struct FrMtTask {
+  int n, first, first_group;
+  size_t key_stride;  // bytes between the key rows of i1 (decaps' sk; 0 = one key for all)
+  const uint8_t* i0;
+  const uint8_t* i1;
+  uint8_t* o0;
+  uint8_t* o1;
+  uint32_t *s8, *e8, *epp8, *fail, *state;
+  uint8_t *kbuf, *mubuf;
+};
+struct FrMt {
+  int ntasks, total, groups;
+  FrMtTask t[kFrodoMtTasks];
+};
+
+// threadIdx.x behind an empty asm statement, read at the top of each work
+// item: values derived from it (selectors, operand offsets) are then
+// computed per item instead of being hoisted out of the item loop and held
+// in registers across the whole row generation.
+__device__ __forceinline__ int fresh_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+__device__ __forceinline__ int mt_task_of_group(const FrMt& M, int g) {
+  int t = 0;
+  while (t + 1 < M.ntasks && g >= M.t[t + 1].first_group) ++t;
+  return t;
+}
+__device__ __forceinline__ int mt_task_of_item(const FrMt& M, int i) {
+  int t = 0;
+  while (t + 1 < M.ntasks && i >= M.t[t + 1].first) ++t;
+  return t;
+}
+
 
 // FrodoKEM-976 names kept for the AES rows and the shared-key kernels
 constexpr int kFN = kFrodoN, kFNB = kFrodoNbar;
@@ -240,13 +289,11 @@ __device__ __forceinline__ void shake_seed_squeeze2(uint32_t p, uint64_t dom, co
 // Writes pk[0:16], sk[0:sec] = s, sk[sec:sec+16] = seedA, the sk S^T part (LE
 // u16) and the int8 copies of S^T ([k][j]) and E ([i][k]) for the fused kernel.
 template <class P>
-__global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
-                                                       uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
-                                                       uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
-  chain_priority();
+__device__ __forceinline__ void keygen_pre_body(int vb, int n, const uint8_t* __restrict__ coins,
+                                                uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
   __shared__ __align__(16) uint8_t lut_lds[32768];
   const uint8_t* lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
-  const int vb = blockIdx.x;
   FR_PAIR_SETUP
   constexpr int RW = P::chain_rate, LS = P::ls;
   const uint8_t* c = coins + (size_t)inst * P::kcoins;
@@ -278,6 +325,14 @@ __global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __r
       *reinterpret_cast<uint16_t*>(ed + 4 * (w - P::mat / 4)) = (uint16_t)b;
     }
   });
+}
+
+template <class P>
+__global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
+                                                       uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                       uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
+  chain_priority();
+  keygen_pre_body<P>(blockIdx.x, n, coins, pk, sk, s8, e8);
 }
 
 // pkh = SHAKE(pk, sec); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
@@ -424,12 +479,18 @@ __global__ void __launch_bounds__(64) frodo_ss(int n, const uint8_t* __restrict_
 // holding the 64 sums (i, j) in registers; a butterfly over the wave then
 // leaves lane L with sum L (each step sends the half a lane does not keep),
 // and the four waves add theirs in LDS.
+// Task arrays: i0 = ct, i1 = sk rows key_stride bytes apart, mubuf.  One
+// block per work item.
 template <class P>
-__global__ void __launch_bounds__(256) frodo_dec_mu(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
-                                                    size_t sk_stride, uint8_t* __restrict__ mubuf) {
+__global__ void __launch_bounds__(256) frodo_dec_mu(const FrMt M) {
   __shared__ uint32_t W[64];
-  const int inst = blockIdx.x, tid = threadIdx.x;
-  if (inst >= n) return;
+  if ((int)blockIdx.x >= M.total) return;
+  const FrMtTask& task = M.t[mt_task_of_item(M, blockIdx.x)];
+  const int inst = blockIdx.x - task.first, tid = threadIdx.x;
+  const uint8_t* __restrict__ ct = task.i0;
+  const uint8_t* __restrict__ sk = task.i1;
+  const size_t sk_stride = task.key_stride;
+  uint8_t* __restrict__ mubuf = task.mubuf;
   if (tid < 64) W[tid] = 0;
   __syncthreads();
   const uint8_t* c = ct + (size_t)inst * P::ct;
@@ -676,18 +737,26 @@ __device__ __forceinline__ void aes_rows_init(AesRows& g, const uint32_t* T, int
 // AES: FrodoKEM-976-AES rows instead of SHAKE128 rows.  Lanes take rows
 // wave * 64 + lane + pass * threads; a wave whose rows all lie past n in the
 // last pass skips it.
-template <class P, bool AES>
-__global__ void __launch_bounds__(P::threads) frodo_as_fused(int n, uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
-                                                             const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8) {
+// Task arrays: o0 = pk, o1 = sk, s8 / e8 = the chains' int8 S^T and E.
+template <class P, bool AES, bool PERSIST>
+__global__ void __launch_bounds__(P::threads) frodo_as_fused(const FrMt M) {
   static_assert(!AES || P::n == 976, "AES rows are defined for FrodoKEM-976");
   constexpr int T = P::threads, SROW = P::srow, N = P::n;
   __shared__ __align__(16) int8_t Sl[9 * SROW];  // rows 0..7 = S^T, row 8 = zeros
   __shared__ int colsum[8];
   __shared__ __align__(16) uint32_t aes_lds[AES ? kAesLdsWords : 1];
   __shared__ __align__(16) uint16_t Bst[P::logq < 16 ? P::mat : 1];  // D < 16: B staged for packing
-  const int inst = blockIdx.x, tid = threadIdx.x;
+  for (int item = blockIdx.x;; item += (int)gridDim.x) {
+  if (item >= M.total) break;
+  const int tid = PERSIST ? fresh_tid() : (int)threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  if (inst >= n) return;
+  const FrMtTask& task = M.t[mt_task_of_item(M, item)];
+  const int inst = item - task.first;
+  uint8_t* __restrict__ pk = task.o0;
+  uint8_t* __restrict__ sk = task.o1;
+  const uint32_t* __restrict__ s8 = task.s8;
+  const int8_t* __restrict__ e8 = reinterpret_cast<const int8_t*>(task.e8);
+  __syncthreads();  // every wave is done with the previous item's LDS
   const uint32_t* s = s8 + (size_t)inst * (P::mat / 4);
   uint32_t* Sd = reinterpret_cast<uint32_t*>(Sl);
   for (int x = tid; x < 9 * SROW / 4; x += T) {
@@ -773,16 +842,17 @@ __global__ void __launch_bounds__(P::threads) frodo_as_fused(int n, uint8_t* __r
       reinterpret_cast<uint32_t*>(sb)[w] = word;
     }
   }
+  if (!PERSIST) break;  // one work item per workgroup
+  }  // item
 }
 
 // enc / dec: B' = S' A + E', then V = S' B + E'' and C = V + Encode(mu).
 // enc writes ct; dec compares with the received ct and raises fail[inst].
-template <class P, bool DEC, bool AES>
-__global__ void __launch_bounds__(P::threads) frodo_sa_fused(int n, const uint8_t* __restrict__ pkb, size_t pk_stride,
-                                                             const uint32_t* __restrict__ s8, const int8_t* __restrict__ e8,
-                                                             const int8_t* __restrict__ epp8, const uint8_t* __restrict__ mu,
-                                                             size_t mu_stride, uint8_t* __restrict__ ct_out,
-                                                             const uint8_t* __restrict__ ct_in, uint32_t* __restrict__ fail) {
+// Task arrays.  enc: i0 = pk, i1 = coins (mu), o0 = ct.  dec: i0 = ct,
+// i1 = sk (its pk part is used), mubuf = mu', fail.  Both: s8 / e8 / epp8 =
+// the chains' int8 S', E', E''.
+template <class P, bool DEC, bool AES, bool PERSIST>
+__global__ void __launch_bounds__(P::threads) frodo_sa_fused(const FrMt M) {
   static_assert(!AES || P::n == 976, "AES rows are defined for FrodoKEM-976");
   constexpr int T = P::threads, SP = P::sprow, N = P::n;
   __shared__ __align__(16) uint32_t Bacc[P::mat];
@@ -791,9 +861,23 @@ __global__ void __launch_bounds__(P::threads) frodo_sa_fused(int n, const uint8_
   __shared__ uint32_t V[64];
   __shared__ uint32_t diff;
   __shared__ __align__(16) uint32_t aes_lds[AES ? kAesLdsWords : 1];
-  const int inst = blockIdx.x, tid = threadIdx.x;
+  for (int item = blockIdx.x;; item += (int)gridDim.x) {
+  if (item >= M.total) break;
+  const int tid = PERSIST ? fresh_tid() : (int)threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  if (inst >= n) return;
+  const FrMtTask& task = M.t[mt_task_of_item(M, item)];
+  const int inst = item - task.first;
+  const uint8_t* __restrict__ pkb = DEC ? task.i1 + P::sec : task.i0;
+  const size_t pk_stride = DEC ? (size_t)P::sk : (size_t)P::pk;
+  const uint32_t* __restrict__ s8 = task.s8;
+  const int8_t* __restrict__ e8 = reinterpret_cast<const int8_t*>(task.e8);
+  const int8_t* __restrict__ epp8 = reinterpret_cast<const int8_t*>(task.epp8);
+  const uint8_t* __restrict__ mu = DEC ? task.mubuf : task.i1;
+  const size_t mu_stride = DEC ? (size_t)32 : (size_t)P::ecoins;
+  uint8_t* __restrict__ ct_out = task.o0;
+  const uint8_t* __restrict__ ct_in = task.i0;
+  uint32_t* __restrict__ fail = task.fail;
+  __syncthreads();  // every wave is done with the previous item's LDS
   const uint32_t* s = s8 + (size_t)inst * (P::mat / 4);
   uint32_t* Sd = reinterpret_cast<uint32_t*>(Sp);
   for (int x = tid; x < 8 * SP / 4; x += T) {
@@ -925,6 +1009,8 @@ __global__ void __launch_bounds__(P::threads) frodo_sa_fused(int n, const uint8_
     __syncthreads();
     if (tid == 0) fail[inst] = diff;
   }
+  if (!PERSIST) break;  // one work item per workgroup
+  }  // item
 }
 
 
@@ -1118,18 +1204,59 @@ __global__ void __launch_bounds__(192) frodo_pipe_x2(int n, const uint8_t* __res
   }
 }
 
+// decaps' ss from the absorbed ct state: the last block (remaining ct
+// words, kbar, padding) and the squeeze.
 template <class P>
-__global__ void __launch_bounds__(64) frodo_pipe_ssd(int n, const uint8_t* __restrict__ ct,
-                                                     const uint32_t* __restrict__ state, const uint8_t* __restrict__ kbuf_d,
-                                                     const uint8_t* __restrict__ sk, const uint32_t* __restrict__ fail,
-                                                     uint8_t* __restrict__ ss) {
-  chain_priority();
-  const int vb = blockIdx.x;
+__device__ __forceinline__ void ssd_body(int vb, int n, const uint8_t* __restrict__ ct,
+                                         const uint32_t* __restrict__ state, const uint8_t* __restrict__ kbuf_d,
+                                         const uint8_t* __restrict__ sk, size_t sk_stride,
+                                         const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
   FR_PAIR_SETUP
   uint32_t a[25];
 #pragma unroll
   for (int i = 0; i < 25; ++i) a[i] = state[(size_t)i * 2 * n + 2 * inst + p];
-  ss_finish<P>(a, p, inst, ct + (size_t)inst * P::ct, kbuf_d, sk, (size_t)P::sk, fail, ss);
+  ss_finish<P>(a, p, inst, ct + (size_t)inst * P::ct, kbuf_d, sk, sk_stride, fail, ss);
+}
+
+template <class P>
+__global__ void __launch_bounds__(64) frodo_pipe_ssd(int n, const uint8_t* __restrict__ ct,
+                                                     const uint32_t* __restrict__ state, const uint8_t* __restrict__ kbuf_d,
+                                                     const uint8_t* __restrict__ sk, size_t sk_stride,
+                                                     const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+  chain_priority();
+  ssd_body<P>(blockIdx.x, n, ct, state, kbuf_d, sk, sk_stride, fail, ss);
+}
+
+// decaps' chains before the re-encryption, one 128-thread block per 32
+// instances: wave 0 runs G2 and the S' / E' / E'' squeeze (encdec_pre_body),
+// wave 1 absorbs the full blocks of ct for ss (ss_absorb) and leaves the
+// sponge state in `state`, so that after the re-encryption only ss's last
+// block remains (ssd_body) instead of the whole ct absorb.
+template <class P>
+__device__ __forceinline__ void dec_pre_body(int vb, int n, const uint8_t* __restrict__ ct,
+                                             const uint8_t* __restrict__ sk, size_t sk_stride,
+                                             const uint8_t* __restrict__ mubuf, uint8_t* __restrict__ kbuf,
+                                             uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
+                                             uint32_t* __restrict__ epp8, uint32_t* __restrict__ state) {
+  if ((threadIdx.x >> 6) == 0) {
+    encdec_pre_body<P, true>(vb, n, sk, sk_stride, mubuf, 32, kbuf, s8, e8, epp8, nullptr);
+  } else {
+    FR_PAIR_SETUP
+    uint32_t a[25];
+    ss_absorb<P>(a, p, ct + (size_t)inst * P::ct);
+#pragma unroll
+    for (int i = 0; i < 25; ++i) state[(size_t)i * 2 * n + 2 * inst + p] = a[i];
+  }
+}
+
+template <class P>
+__global__ void __launch_bounds__(128) frodo_dec_pre(int n, const uint8_t* __restrict__ ct,
+                                                     const uint8_t* __restrict__ sk, size_t sk_stride,
+                                                     const uint8_t* __restrict__ mubuf, uint8_t* __restrict__ kbuf,
+                                                     uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
+                                                     uint32_t* __restrict__ epp8, uint32_t* __restrict__ state) {
+  chain_priority();
+  dec_pre_body<P>(blockIdx.x, n, ct, sk, sk_stride, mubuf, kbuf, s8, e8, epp8, state);
 }
 
 // ----------------------------------------------------------------- host launchers
@@ -1143,12 +1270,96 @@ static const char* fr_name(const char* n976, const char* n640, const char* n1344
 }
 #define FR_NAME(base) fr_name<P>(base, base ",640", base ",1344")
 
+// Resident workgroups of a kernel on the current device (0 on error).
+template <class K>
+static int resident_blocks(K kernel, int threads) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess)
+    return 0;
+  return cus * per_cu;
+}
+
+// Grid of a fused launch over `total` work items: one workgroup per item,
+// or for the persistent form at most the resident workgroups (looked up
+// once per kernel).
+template <class P, int OP, bool AES, bool PERSIST>
+static dim3 fused_grid(int total) {
+  if (!PERSIST) return dim3(total);
+  static const int res = OP == 0   ? resident_blocks(frodo_as_fused<P, AES, true>, P::threads)
+                         : OP == 1 ? resident_blocks(frodo_sa_fused<P, false, AES, true>, P::threads)
+                                   : resident_blocks(frodo_sa_fused<P, true, AES, true>, P::threads);
+  return dim3(res > 0 ? std::min(res, total) : total);
+}
+
+// A task table entry over one set of arrays, with the chains' scratch.
+static FrMtTask task_of(int n, const uint8_t* i0, const uint8_t* i1, uint8_t* o0, uint8_t* o1, uint32_t* s8,
+                        uint32_t* e8, uint32_t* epp8 = nullptr, uint8_t* kbuf = nullptr, uint8_t* mubuf = nullptr,
+                        uint32_t* fail = nullptr, uint32_t* state = nullptr, size_t key_stride = 0) {
+  FrMtTask t;
+  memset(&t, 0, sizeof t);
+  t.n = n;
+  t.key_stride = key_stride;
+  t.i0 = i0;
+  t.i1 = i1;
+  t.o0 = o0;
+  t.o1 = o1;
+  t.s8 = s8;
+  t.e8 = e8;
+  t.epp8 = epp8;
+  t.kbuf = kbuf;
+  t.mubuf = mubuf;
+  t.fail = fail;
+  t.state = state;
+  return t;
+}
+
+static FrMt table_of(const FrMtTask* tasks, int ntasks) {
+  FrMt M;
+  memset(&M, 0, sizeof M);
+  M.ntasks = ntasks;
+  for (int k = 0; k < ntasks; ++k) {
+    M.t[k] = tasks[k];
+    M.t[k].first = M.total;
+    M.t[k].first_group = M.groups;
+    M.total += tasks[k].n;
+    M.groups += blocks_for(tasks[k].n, 32);
+  }
+  return M;
+}
+
+template <class P, bool AES>
+static void launch_as_fused(hipStream_t st, const FrMt& M, bool persist = false) {
+  const char* name = AES ? "frodo_as_fused<aes>" : FR_NAME("frodo_as_fused");
+  if (persist)
+    DPKEM_LAUNCH(name, (frodo_as_fused<P, AES, true>), (fused_grid<P, 0, AES, true>(M.total)), dim3(P::threads), st, M);
+  else
+    DPKEM_LAUNCH(name, (frodo_as_fused<P, AES, false>), (fused_grid<P, 0, AES, false>(M.total)), dim3(P::threads), st,
+                 M);
+}
+template <class P, bool DEC, bool AES>
+static void launch_sa_fused(hipStream_t st, const FrMt& M, bool persist = false) {
+  const char* name = DEC ? (AES ? "frodo_sa_fused<true,aes>" : FR_NAME("frodo_sa_fused<true>"))
+                         : (AES ? "frodo_sa_fused<false,aes>" : FR_NAME("frodo_sa_fused<false>"));
+  if (persist)
+    DPKEM_LAUNCH(name, (frodo_sa_fused<P, DEC, AES, true>), (fused_grid<P, DEC ? 2 : 1, AES, true>(M.total)),
+                 dim3(P::threads), st, M);
+  else
+    DPKEM_LAUNCH(name, (frodo_sa_fused<P, DEC, AES, false>), (fused_grid<P, DEC ? 2 : 1, AES, false>(M.total)),
+                 dim3(P::threads), st, M);
+}
+template <class P>
+static void launch_dec_mu(hipStream_t st, const FrMt& M) {
+  DPKEM_LAUNCH(FR_NAME("frodo_dec_mu"), frodo_dec_mu<P>, dim3(M.total), dim3(256), st, M);
+}
+
 template <class P, bool AES>
 static void keygen_t(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8, uint32_t* e8) {
   DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, coins, pk,
                sk, s8, e8);
-  DPKEM_LAUNCH(AES ? "frodo_as_fused<aes>" : FR_NAME("frodo_as_fused"), (frodo_as_fused<P, AES>), dim3(n),
-               dim3(P::threads), st, n, pk, sk, s8, reinterpret_cast<const int8_t*>(e8));
+  const FrMtTask t = task_of(n, coins, nullptr, pk, sk, s8, e8);
+  launch_as_fused<P, AES>(st, table_of(&t, 1));
   DPKEM_LAUNCH(FR_NAME("frodo_keygen_post"), frodo_keygen_post<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
 }
 
@@ -1157,24 +1368,23 @@ static void enc_t(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins
                   uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh) {
   DPKEM_LAUNCH(FR_NAME("frodo_encdec_pre<false>"), (frodo_encdec_pre<P, false>), dim3(blocks_for(n, 32)), dim3(64), st,
                n, pk, (size_t)P::pk, coins, (size_t)P::ecoins, kbuf, s8, e8, epp8, pkh);
-  DPKEM_LAUNCH(AES ? "frodo_sa_fused<false,aes>" : FR_NAME("frodo_sa_fused<false>"), (frodo_sa_fused<P, false, AES>),
-               dim3(n), dim3(P::threads), st, n, pk, (size_t)P::pk, s8, reinterpret_cast<const int8_t*>(e8),
-               reinterpret_cast<const int8_t*>(epp8), coins, (size_t)P::ecoins, ct, nullptr, nullptr);
+  const FrMtTask t = task_of(n, pk, coins, ct, ss, s8, e8, epp8, kbuf);
+  launch_sa_fused<P, false, AES>(st, table_of(&t, 1));
   DPKEM_LAUNCH(FR_NAME("frodo_ss"), frodo_ss<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr,
                (size_t)0, nullptr, ss);
 }
 
 template <class P, bool AES>
 static void dec_t(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8, uint32_t* e8,
-                  uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail) {
-  DPKEM_LAUNCH(FR_NAME("frodo_dec_mu"), frodo_dec_mu<P>, dim3(n), dim3(256), st, n, ct, sk, (size_t)P::sk, mubuf);
-  DPKEM_LAUNCH(FR_NAME("frodo_encdec_pre<true>"), (frodo_encdec_pre<P, true>), dim3(blocks_for(n, 32)), dim3(64), st,
-               n, sk, (size_t)P::sk, mubuf, (size_t)32, kbuf, s8, e8, epp8, nullptr);
-  DPKEM_LAUNCH(AES ? "frodo_sa_fused<true,aes>" : FR_NAME("frodo_sa_fused<true>"), (frodo_sa_fused<P, true, AES>),
-               dim3(n), dim3(P::threads), st, n, sk + P::sec, (size_t)P::sk, s8, reinterpret_cast<const int8_t*>(e8),
-               reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
-  DPKEM_LAUNCH(FR_NAME("frodo_ss"), frodo_ss<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, (size_t)P::sk,
-               fail, ss);
+                  uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, uint32_t* state) {
+  const FrMtTask t = task_of(n, ct, sk, ss, nullptr, s8, e8, epp8, kbuf, mubuf, fail, state, (size_t)P::sk);
+  const FrMt M = table_of(&t, 1);
+  launch_dec_mu<P>(st, M);
+  DPKEM_LAUNCH(FR_NAME("frodo_dec_pre"), frodo_dec_pre<P>, dim3(blocks_for(n, 32)), dim3(128), st, n, ct, sk,
+               (size_t)P::sk, mubuf, kbuf, s8, e8, epp8, state);
+  launch_sa_fused<P, true, AES>(st, M);
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, state, kbuf,
+               sk, (size_t)P::sk, fail, ss);
 }
 
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
@@ -1196,12 +1406,13 @@ void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* c
 }
 
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
-                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes, int set) {
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, uint32_t* state,
+                      bool aes, int set) {
   if (n <= 0) return;
-  if (set == 640) dec_t<FrodoP<640>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
-  else if (set == 1344) dec_t<FrodoP<1344>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
-  else if (aes) dec_t<F976, true>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
-  else dec_t<F976, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail);
+  if (set == 640) dec_t<FrodoP<640>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
+  else if (set == 1344) dec_t<FrodoP<1344>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
+  else if (aes) dec_t<F976, true>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
+  else dec_t<F976, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
 }
 
 
@@ -1230,22 +1441,20 @@ static void pipe_t(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   uint32_t* state = reinterpret_cast<uint32_t*>(take(200));
   const dim3 g(blocks_for(n, 32));
-  const int8_t* e8b = reinterpret_cast<const int8_t*>(e8);
-  const int8_t* eppb = reinterpret_cast<const int8_t*>(epp8);
   DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, g, dim3(64), st, n, kc, pk, sk, s8, e8);
-  DPKEM_LAUNCH(AES ? "frodo_as_fused<aes>" : FR_NAME("frodo_as_fused"), (frodo_as_fused<P, AES>), dim3(n),
-               dim3(P::threads), st, n, pk, sk, s8, e8b);
+  const FrMtTask tk = task_of(n, kc, nullptr, pk, sk, s8, e8);
+  launch_as_fused<P, AES>(st, table_of(&tk, 1));
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_x1"), frodo_pipe_x1<P>, g, dim3(128), st, n, pk, sk, ec, kbuf_e, s8, e8, epp8);
-  DPKEM_LAUNCH(AES ? "frodo_sa_fused<false,aes>" : FR_NAME("frodo_sa_fused<false>"), (frodo_sa_fused<P, false, AES>),
-               dim3(n), dim3(P::threads), st, n, pk, (size_t)P::pk, s8, e8b, eppb, ec, (size_t)P::ecoins, ct, nullptr,
-               nullptr);
-  DPKEM_LAUNCH(FR_NAME("frodo_dec_mu"), frodo_dec_mu<P>, dim3(n), dim3(256), st, n, ct, sk, (size_t)P::sk, mubuf);
+  const FrMtTask te = task_of(n, pk, ec, ct, ss, s8, e8, epp8, kbuf_e);
+  launch_sa_fused<P, false, AES>(st, table_of(&te, 1));
+  const FrMtTask td = task_of(n, ct, sk, ss2, nullptr, s8, e8, epp8, kbuf_d, mubuf, fail, state, (size_t)P::sk);
+  const FrMt Md = table_of(&td, 1);
+  launch_dec_mu<P>(st, Md);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_x2"), frodo_pipe_x2<P>, g, dim3(192), st, n, ct, sk, kbuf_e, ss, mubuf, kbuf_d, s8,
                e8, epp8, state);
-  DPKEM_LAUNCH(AES ? "frodo_sa_fused<true,aes>" : FR_NAME("frodo_sa_fused<true>"), (frodo_sa_fused<P, true, AES>),
-               dim3(n), dim3(P::threads), st, n, sk + P::sec, (size_t)P::sk, s8, e8b, eppb, mubuf, (size_t)32, nullptr,
-               ct, fail);
-  DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, g, dim3(64), st, n, ct, state, kbuf_d, sk, fail, ss2);
+  launch_sa_fused<P, true, AES>(st, Md);
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, g, dim3(64), st, n, ct, state, kbuf_d, sk, (size_t)P::sk,
+               fail, ss2);
 }
 
 void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
@@ -1275,17 +1484,107 @@ void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t p
 
 void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, size_t sk_stride, uint8_t* ss,
                               uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail,
-                              const uint2* groups, int ngroups) {
+                              uint32_t* state, const uint2* groups, int ngroups) {
   if (n <= 0) return;
   if (!groups) ngroups = blocks_for(n, kFrodoGroup);
-  DPKEM_LAUNCH("frodo_dec_mu", frodo_dec_mu<F976>, dim3(n), dim3(256), st, n, ct, sk, sk_stride, mubuf);
-  DPKEM_LAUNCH("frodo_encdec_pre<true>", (frodo_encdec_pre<F976, true>), dim3(blocks_for(n, 32)), dim3(64), st, n, sk,
-               sk_stride, mubuf, (size_t)32, kbuf, s8, e8, epp8, nullptr);
+  {
+    using P = F976;
+    const FrMtTask t = task_of(n, ct, sk, ss, nullptr, s8, e8, epp8, kbuf, mubuf, fail, state, sk_stride);
+    launch_dec_mu<P>(st, table_of(&t, 1));
+  }
+  DPKEM_LAUNCH("frodo_dec_pre", frodo_dec_pre<F976>, dim3(blocks_for(n, 32)), dim3(128), st, n, ct, sk, sk_stride,
+               mubuf, kbuf, s8, e8, epp8, state);
   DPKEM_LAUNCH("frodo_sa_grouped<true>", (frodo_sa_grouped<true, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
                ngroups, groups, n, sk + kFrodoSec, sk_stride, s8, reinterpret_cast<const int8_t*>(e8),
                reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
-  DPKEM_LAUNCH("frodo_ss", frodo_ss<F976>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, sk, sk_stride, fail,
-               ss);
+  DPKEM_LAUNCH("frodo_pipe_ssd", frodo_pipe_ssd<F976>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, state, kbuf, sk,
+               sk_stride, fail, ss);
+}
+
+// ----------------------------------------------------------------- multi-task launches
+//
+// Independent tasks of one op and parameter set in one launch sequence: the
+// chain kernels map their 32-instance groups onto the tasks' groups (each
+// task's instances padded to whole groups) and the fused / B'S kernels take
+// the whole task table.
+
+// keygen: seedA, S / E chains | enc: pkh, G2, S' / E' / E'' chains
+template <class P, int OP>
+__global__ void __launch_bounds__(64) frodo_mt_pre(const FrMt M) {
+  chain_priority();
+  const FrMtTask& T = M.t[mt_task_of_group(M, blockIdx.x)];
+  const int vb = blockIdx.x - T.first_group;
+  if constexpr (OP == 0) keygen_pre_body<P>(vb, T.n, T.i0, T.o0, T.o1, T.s8, T.e8);
+  else encdec_pre_body<P, false>(vb, T.n, T.i0, (size_t)P::pk, T.i1, (size_t)P::ecoins, T.kbuf, T.s8, T.e8, T.epp8,
+                                 nullptr);
+}
+
+// decaps: G2 and the S' / E' / E'' chains beside ss's ct absorb
+template <class P>
+__global__ void __launch_bounds__(128) frodo_mt_dec_pre(const FrMt M) {
+  chain_priority();
+  const FrMtTask& T = M.t[mt_task_of_group(M, blockIdx.x)];
+  dec_pre_body<P>(blockIdx.x - T.first_group, T.n, T.i0, T.i1, (size_t)P::sk, T.mubuf, T.kbuf, T.s8, T.e8, T.epp8,
+                  T.state);
+}
+
+// keygen: pkh into sk | enc: ss = SHAKE(ct || k) | dec: ss's last block
+template <class P, int OP>
+__global__ void __launch_bounds__(64) frodo_mt_post(const FrMt M) {
+  chain_priority();
+  const FrMtTask& T = M.t[mt_task_of_group(M, blockIdx.x)];
+  const int vb = blockIdx.x - T.first_group;
+  if constexpr (OP == 0) keygen_post_body<P>(vb, T.n, T.o0, T.o1);
+  else if constexpr (OP == 1) ss_body<P>(vb, T.n, T.o0, T.kbuf, nullptr, 0, nullptr, T.o1);
+  else ssd_body<P>(vb, T.n, T.i0, T.state, T.kbuf, T.i1, (size_t)P::sk, T.fail, T.o0);
+}
+
+template <class P, bool AES>
+static void mt_t(hipStream_t st, int op, const FrodoMtDesc* d, int ntasks, bool persist) {
+  FrMtTask tasks[kFrodoMtTasks];
+  for (int k = 0; k < ntasks; ++k) {
+    const int n = d[k].n;
+    // per-task scratch, carved as the single-task launchers carve it
+    size_t off = 0;
+    auto take = [&](size_t per) {
+      uint8_t* q = d[k].scratch + off;
+      off += ((size_t)n * per + 255) & ~(size_t)255;
+      return q;
+    };
+    uint32_t* s8 = reinterpret_cast<uint32_t*>(take(P::mat));
+    uint32_t* e8 = reinterpret_cast<uint32_t*>(take(P::mat));
+    uint32_t* epp8 = reinterpret_cast<uint32_t*>(take(64));
+    uint8_t* kbuf = take(64);
+    uint8_t* mubuf = take(32);
+    uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
+    uint32_t* state = reinterpret_cast<uint32_t*>(take(200));
+    tasks[k] = task_of(n, d[k].i0, d[k].i1, d[k].o0, d[k].o1, s8, e8, epp8, kbuf, mubuf, fail, state, (size_t)P::sk);
+  }
+  const FrMt M = table_of(tasks, ntasks);
+  if (M.total == 0) return;
+  const dim3 groups(M.groups);
+  if (op == 0) {
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<keygen>"), (frodo_mt_pre<P, 0>), groups, dim3(64), st, M);
+    launch_as_fused<P, AES>(st, M, persist);
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_post<keygen>"), (frodo_mt_post<P, 0>), groups, dim3(64), st, M);
+  } else if (op == 1) {
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<enc>"), (frodo_mt_pre<P, 1>), groups, dim3(64), st, M);
+    launch_sa_fused<P, false, AES>(st, M, persist);
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_post<enc>"), (frodo_mt_post<P, 1>), groups, dim3(64), st, M);
+  } else {
+    launch_dec_mu<P>(st, M);
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_dec_pre"), frodo_mt_dec_pre<P>, groups, dim3(128), st, M);
+    launch_sa_fused<P, true, AES>(st, M, persist);
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_post<dec>"), (frodo_mt_post<P, 2>), groups, dim3(64), st, M);
+  }
+}
+
+void frodo_mt_launch(hipStream_t st, int op, int set, bool aes, const FrodoMtDesc* tasks, int ntasks, bool persist) {
+  if (ntasks <= 0 || ntasks > kFrodoMtTasks) return;
+  if (set == 640) mt_t<FrodoP<640>, false>(st, op, tasks, ntasks, persist);
+  else if (set == 1344) mt_t<FrodoP<1344>, false>(st, op, tasks, ntasks, persist);
+  else if (aes) mt_t<F976, true>(st, op, tasks, ntasks, persist);
+  else mt_t<F976, false>(st, op, tasks, ntasks, persist);
 }
 
 }  // namespace dpkem

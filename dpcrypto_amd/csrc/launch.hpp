@@ -61,12 +61,32 @@ constexpr int kFrodoGroup = 4;
 void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t pk_stride, const uint8_t* coins,
                               uint8_t* ct, uint8_t* ss, uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf,
                               const uint2* groups, int ngroups, const uint8_t* pkh = nullptr);
+// state: 200 bytes per instance (decaps' ss sponge state after the ct absorb)
 void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, size_t sk_stride, uint8_t* ss,
                               uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail,
-                              const uint2* groups, int ngroups);
+                              uint32_t* state, const uint2* groups, int ngroups);
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
-                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, bool aes = false,
-                      int set = 976);
+                      uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, uint32_t* state,
+                      bool aes = false, int set = 976);
+
+// Independent tasks of one Frodo op (0 keygen, 1 enc, 2 dec) and parameter
+// set as ONE launch sequence (frodo.hip, multi-task launches): per task its
+// instance count, arrays in the roles of the single-task launchers (keygen:
+// i0 coins, o0 pk, o1 sk; enc: i0 pk, i1 coins, o0 ct, o1 ss; dec: i0 ct,
+// i1 sk, o0 ss) and frodo_mt_scratch_bytes(set, n) bytes of scratch.
+constexpr int kFrodoMtTasks = 16;
+struct FrodoMtDesc {
+  int n;
+  const uint8_t* i0;
+  const uint8_t* i1;
+  uint8_t* o0;
+  uint8_t* o1;
+  uint8_t* scratch;
+};
+// persist: the fused kernel runs as a persistent grid of resident workgroups
+// walking all tasks' instances, instead of one workgroup per instance.
+void frodo_mt_launch(hipStream_t st, int op, int set, bool aes, const FrodoMtDesc* tasks, int ntasks,
+                     bool persist = true);
 
 // ---- plain comparator form (plain.hip); op 0 keygen, 1 enc, 2 dec with the
 // same output / input roles as the batched launchers.
@@ -78,8 +98,9 @@ void plain_frodo_launch(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1,
 
 // Scratch bytes per instance for each (scheme, op) of the batched form.
 constexpr size_t kSaberScratchPerInst = 3744 + 768 + 32 + 64 + 16;
-constexpr size_t kFrodoScratchPerInst = 7808 + 7808 + 64 + 64 + 32 + 16;
-// s8 (8 n) | e8 (8 n) | epp8 64 | kbuf 64 | mubuf 32 | fail 4 (+ padding)
-constexpr size_t frodo_scratch_per_inst(int set) { return 16 * (size_t)set + 64 + 64 + 32 + 16; }
+// s8 (8 n) | e8 (8 n) | epp8 64 | kbuf 64 | mubuf 32 | fail 4 | state 200 (+ padding)
+constexpr size_t frodo_scratch_per_inst(int set) { return 16 * (size_t)set + 64 + 64 + 32 + 16 + 200; }
+// scratch of one multi-task entry of n instances (seven arrays, each 256-aligned)
+constexpr size_t frodo_mt_scratch_bytes(int set, size_t n) { return n * frodo_scratch_per_inst(set) + 7 * 256; }
 
 }  // namespace dpkem

@@ -42,13 +42,15 @@ struct Sub {
   size_t scratch_off;
   hipEvent_t done = nullptr;
   std::vector<int> deps;   // sub indices
+  std::vector<int> mt;     // non-empty: independent Frodo tasks of one op run as one multi-task launch sequence
+  std::vector<size_t> mt_off;  // their scratch offsets
 };
 
 }  // namespace
 
 struct dpkem_plan {
   dpkem_engine* e = nullptr;
-  int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED, saber_ipw = 1;
+  int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED, saber_ipw = 1, frodo_persist = 1;
   unsigned flags = 0;
   std::vector<Task> tasks;
   std::vector<Sub> subs;
@@ -129,12 +131,43 @@ int find_chain(const dpkem_plan* p, int t0, int* t1, int* t2) {
   return kNoChain;
 }
 
+// Independent Frodo tasks of one (scheme, op) -- no `after`, not part of a
+// keygen -> enc -> dec chain -- share one multi-task launch sequence when
+// there are at least two of them (groups of up to kFrodoMtTasks, in task
+// order).  Returns, per task, the index of its group in `groups`, or -1.
+std::vector<int> merge_groups(const dpkem_plan* p, const std::vector<char>& in_chain,
+                              std::vector<std::vector<int>>* groups) {
+  std::vector<int> group_of(p->tasks.size(), -1);
+  if (p->form != DPKEM_FORM_BATCHED || (p->flags & DPKEM_PLAN_NO_MERGE)) return group_of;
+  for (size_t t = 0; t < p->tasks.size(); ++t) {
+    const Task& T = p->tasks[t];
+    if (group_of[t] >= 0 || in_chain[t] || T.after != -1 || !frodo_set_of(T.scheme)) continue;
+    std::vector<int> g{(int)t};
+    for (size_t u = t + 1; u < p->tasks.size() && g.size() < (size_t)kFrodoMtTasks; ++u) {
+      const Task& U = p->tasks[u];
+      if (group_of[u] < 0 && !in_chain[u] && U.after == -1 && U.scheme == T.scheme && U.op == T.op) g.push_back((int)u);
+    }
+    if (g.size() < 2) continue;
+    for (int u : g) group_of[u] = (int)groups->size();
+    groups->push_back(g);
+  }
+  return group_of;
+}
+
 // Split every task into chunks and assign streams and dependencies.
 void schedule(dpkem_plan* p) {
   p->subs.clear();
   std::vector<std::vector<int>> task_subs(p->tasks.size());
   std::vector<int> chain_of(p->tasks.size(), -1);
   std::vector<int> absorbed(p->tasks.size(), -1);  // task -> chain root it was folded into
+  // tasks that take part in a keygen -> enc -> dec chain run in the chain's launches
+  std::vector<char> in_chain(p->tasks.size(), 0);
+  for (size_t t = 0; t < p->tasks.size(); ++t) {
+    int t1 = -1, t2 = -1;
+    if (!in_chain[t] && find_chain(p, (int)t, &t1, &t2)) in_chain[t] = in_chain[t1] = in_chain[t2] = 1;
+  }
+  std::vector<std::vector<int>> groups;
+  const std::vector<int> group_of = merge_groups(p, in_chain, &groups);
   int chains = 0;
   size_t off = 0;
   for (size_t t = 0; t < p->tasks.size(); ++t) {
@@ -144,6 +177,29 @@ void schedule(dpkem_plan* p) {
       continue;
     }
     const Task& T = p->tasks[t];
+    if (group_of[t] >= 0) {
+      const std::vector<int>& g = groups[group_of[t]];
+      if (g[0] != (int)t) {  // a later member: runs in the group's sub-task
+        task_subs[t] = task_subs[g[0]];
+        chain_of[t] = chain_of[g[0]];
+        continue;
+      }
+      chain_of[t] = chains++;
+      Sub s;
+      s.task = (int)t;
+      s.begin = 0;
+      s.count = T.n;
+      s.stream = chain_of[t] % p->nstreams;
+      s.scratch_off = off;
+      s.mt = g;
+      for (int u : g) {
+        s.mt_off.push_back(off);
+        off += round_up(frodo_mt_scratch_bytes(frodo_set_of(T.scheme), p->tasks[u].n));
+      }
+      task_subs[t].push_back((int)p->subs.size());
+      p->subs.push_back(s);
+      continue;
+    }
     int t1 = -1, t2 = -1;
     if (const int kind = find_chain(p, (int)t, &t1, &t2)) {
       absorbed[t1] = absorbed[t2] = (int)t;
@@ -210,6 +266,18 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
     hipStream_t st = p->streams[s.stream];
     for (int d : s.deps)
       if (p->subs[d].stream != s.stream) ok = ok && hipStreamWaitEvent(st, p->subs[d].done, 0) == hipSuccess;
+    if (!s.mt.empty()) {
+      FrodoMtDesc d[kFrodoMtTasks];
+      for (size_t k = 0; k < s.mt.size(); ++k) {
+        const Task& U = p->tasks[s.mt[k]];
+        d[k] = {(int)U.n, U.i0, U.i1, U.o0, U.o1, p->scratch + s.mt_off[k]};
+      }
+      frodo_mt_launch(st, T.op, frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES, d, (int)s.mt.size(),
+                      p->frodo_persist != 0);
+      ok = ok && hipGetLastError() == hipSuccess;
+      ok = ok && hipEventRecord(s.done, st) == hipSuccess;
+      continue;
+    }
     if (s.chain >= 0) {
       const Task& E = p->tasks[s.chain];
       const Task& Dd = p->tasks[s.chain2];
@@ -264,6 +332,7 @@ int dpkem_plan_create(dpkem_engine* e, dpkem_plan** out, int nstreams, int chunk
   p->chunks = chunks;
   p->form = e->form;
   p->saber_ipw = e->saber_ipw;
+  p->frodo_persist = e->frodo_persist;
   *out = p;
   return DPKEM_OK;
 }
@@ -289,7 +358,8 @@ int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uin
 }
 
 int dpkem_plan_set_options(dpkem_plan* p, unsigned flags) {
-  if (!p || p->built || (flags & ~(DPKEM_PLAN_SHARE_MATRIX | DPKEM_PLAN_NO_PIPELINE | DPKEM_PLAN_NO_GRAPH)))
+  if (!p || p->built ||
+      (flags & ~(DPKEM_PLAN_SHARE_MATRIX | DPKEM_PLAN_NO_PIPELINE | DPKEM_PLAN_NO_GRAPH | DPKEM_PLAN_NO_MERGE)))
     return DPKEM_EINVAL;
   p->flags = flags;
   return DPKEM_OK;

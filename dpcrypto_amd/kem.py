@@ -30,6 +30,7 @@ FORM_BATCHED = 0
 FORM_PLAIN = 1
 # kernel tuning knobs (dpkem_engine_set_tuning)
 TUNE_SABER_POLY_IPW = 1
+TUNE_FRODO_PERSIST = 2
 
 ERRORS = {
     -1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EBUDGET", -5: "EBUSY", -6: "ENOTFOUND", -7: "ECANCELED",
@@ -330,8 +331,8 @@ class Engine:
         return ss
 
     def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False, pipeline: bool = True,
-             graph: bool = True) -> "Plan":
-        return Plan(self, nstreams, chunks, share_matrix, pipeline, graph)
+             graph: bool = True, merge: bool = True) -> "Plan":
+        return Plan(self, nstreams, chunks, share_matrix, pipeline, graph, merge)
 
     # raw pointer entry points (device arrays preallocated by the caller; each
     # holds at least n rows of its role's width)
@@ -358,13 +359,14 @@ ROLES = {OP_KEYGEN: ("pk", "sk", "kcoins", None), OP_ENC: ("ct", "ss", "pk", "ec
 PLAN_SHARE_MATRIX = 1
 PLAN_NO_PIPELINE = 2
 PLAN_NO_GRAPH = 4
+PLAN_NO_MERGE = 8
 
 
 class Plan:
     """A graph-captured set of KEM tasks over device arrays (dpkem_plan_*)."""
 
     def __init__(self, engine: Engine, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False,
-                 pipeline: bool = True, graph: bool = True):
+                 pipeline: bool = True, graph: bool = True, merge: bool = True):
         self.eng = engine
         self.lib = engine.lib
         h = ctypes.c_void_p()
@@ -372,7 +374,7 @@ class Plan:
         self.h = h
         self._keep = []  # arrays referenced by the graph stay alive with the plan
         flags = ((PLAN_SHARE_MATRIX if share_matrix else 0) | (0 if pipeline else PLAN_NO_PIPELINE)
-                 | (0 if graph else PLAN_NO_GRAPH))
+                 | (0 if graph else PLAN_NO_GRAPH) | (0 if merge else PLAN_NO_MERGE))
         if flags:
             engine._check(self.lib.dpkem_plan_set_options(self.h, flags), "plan_set_options")
 

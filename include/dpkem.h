@@ -87,8 +87,13 @@ int dpkem_engine_set_form(dpkem_engine *e, int form);
  * constants (SABER-GPU/src/cuda_kernel.cu:82-87, DoDP4Av7 /
  * MatVecMul_gpu_shared at one workgroup per instance).
  * DPKEM_TUNE_SABER_POLY_IPW: instances per workgroup of the Saber-family
- * product kernel, one wave each: 1 (default), 2 or 4. */
+ * product kernel, one wave each: 1 (default), 2 or 4.
+ * DPKEM_TUNE_FRODO_PERSIST: in a plan's multi-task Frodo launches
+ * (DPKEM_PLAN_NO_MERGE), the fused A-generation + MFMA kernel runs as a
+ * persistent grid of resident workgroups walking every task's instances (1,
+ * default) or as one workgroup per instance (0). */
 #define DPKEM_TUNE_SABER_POLY_IPW 1
+#define DPKEM_TUNE_FRODO_PERSIST 2
 int dpkem_engine_set_tuning(dpkem_engine *e, int knob, int value);
 /* Block until all work submitted on the engine's stream has finished.
  * Returns DPKEM_EDEVICE (with a message in dpkem_engine_last_error) when a
@@ -179,6 +184,14 @@ int dpkem_plan_add(dpkem_plan *p, int scheme, int op, size_t n, uint8_t *o0, uin
  * captured graph, so that the engine's profiler (dpkem_engine_set_profiling)
  * can time each kernel of a plan.  Slower to launch; for measurement. */
 #define DPKEM_PLAN_NO_GRAPH 4u
+/* Independent Frodo tasks of one scheme and op (no `after`, not in a
+ * keygen -> enc -> dec chain), e.g. uneven batches of a mixed workload, run
+ * by default as ONE multi-task launch sequence -- all their chains in one
+ * launch each and one persistent fused kernel walking every task's
+ * instances -- instead of one sequence per task (groups of up to 16 tasks;
+ * a task alone keeps the per-task path).  Results are identical.
+ * DPKEM_PLAN_NO_MERGE keeps one launch sequence per task. */
+#define DPKEM_PLAN_NO_MERGE 8u
 int dpkem_plan_set_options(dpkem_plan *p, unsigned flags);
 int dpkem_plan_build(dpkem_plan *p);
 int dpkem_plan_launch(dpkem_plan *p);
@@ -261,8 +274,8 @@ int dpkem_server_destroy(dpkem_server *s);
  * DPKEM_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends); the call
  * makes no hipMalloc / hipFree.  n above what that array holds
  * ((DPKEM_NOALLOC_SCRATCH_BYTES - 4096) / per-instance scratch: Saber 13,796,
- * LightSaber 26,212, FireSaber 8,737, Frodo-976 4,227, Frodo-640 6,393,
- * Frodo-1344 3,083 instances) returns DPKEM_EBUDGET before anything runs;
+ * LightSaber 26,212, FireSaber 8,737, Frodo-976 4,160, Frodo-640 6,241,
+ * Frodo-1344 3,048 instances) returns DPKEM_EBUDGET before anything runs;
  * split larger batches.  All calls on one GPU share its scratch array: calls
  * that overlap there (other streams or threads) race and must be serialised
  * by the caller.  The first call on a device also looks up the array's

@@ -167,3 +167,73 @@ def test_mixed_independent_plan(eng, oracle, torch_cuda):
         for k, v in exp.items():
             assert np.array_equal(out[k].cpu().numpy(), v), f"{k} replay {rep}"
     plan.close()
+
+
+FRODO640, FRODO1344 = 5, 6
+
+
+@pytest.mark.parametrize("scheme,sizes,nstreams", [
+    (FRODO, (1, 31, 33, 100, 2), 2), (FRODO, (40, 7), 1), (FRODO_AES, (3, 9), 2), (FRODO640, (5, 33), 1),
+    (FRODO1344, (2, 7), 1)])
+def test_independent_frodo_tasks_merge_into_one_launch_sequence(eng, oracle, torch_cuda, scheme, sizes, nstreams):
+    """Independent Frodo tasks of one op (uneven sizes, separate arrays) run
+    as one multi-task launch sequence with a persistent fused kernel; every
+    instance of every task equals the oracle on each of two replays, and the
+    same plan built with DPKEM_PLAN_NO_MERGE (one sequence per task) agrees."""
+    import dpcrypto_amd as D
+    torch = torch_cuda
+    s = D.SIZES[scheme]
+    print(f"merged Frodo tasks: scheme {scheme}, sizes {sizes}", flush=True)
+    kcs = [oracle.prng_bytes(7100 + 10 * scheme + k, n * s["kcoins"]).reshape(n, -1) for k, n in enumerate(sizes)]
+    ecs = [oracle.prng_bytes(7200 + 10 * scheme + k, n * s["ecoins"]).reshape(n, -1) for k, n in enumerate(sizes)]
+    ref = [oracle.keypair(scheme, kc, threads=8) for kc in kcs]
+    refe = [oracle.enc(scheme, pk, ec, threads=8) for (pk, _), ec in zip(ref, ecs)]
+    # decaps inputs: the oracle's ciphertexts, one tampered instance per task
+    cts = []
+    for (ct, _), n in zip(refe, sizes):
+        c = ct.copy()
+        c[n - 1, 11] ^= 4
+        cts.append(c)
+    refd = [oracle.dec(scheme, c, sk, threads=8) for c, (_, sk) in zip(cts, ref)]
+
+    def run(merge):
+        p = eng.plan(nstreams=nstreams, chunks=1, merge=merge)
+        outs = []
+        for k, n in enumerate(sizes):
+            pk = torch.empty((n, s["pk"]), dtype=torch.uint8, device="cuda")
+            sk = torch.empty((n, s["sk"]), dtype=torch.uint8, device="cuda")
+            ct = torch.empty((n, s["ct"]), dtype=torch.uint8, device="cuda")
+            ss = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
+            ss2 = torch.empty((n, s["ss"]), dtype=torch.uint8, device="cuda")
+            kc, ec, rpk, rct, rsk = _dev(torch, kcs[k]), _dev(torch, ecs[k]), _dev(torch, ref[k][0]), \
+                _dev(torch, cts[k]), _dev(torch, ref[k][1])
+            p.keypair(scheme, pk, sk, kc)
+            p.enc(scheme, ct, ss, rpk, ec)
+            p.dec(scheme, ss2, rct, rsk)
+            outs.append((pk, sk, ct, ss, ss2, kc, ec, rpk, rct, rsk))
+        p.build()
+        return p, outs
+
+    for merge in (True, False):
+        p, outs = run(merge)
+        try:
+            info = p.info()
+            # merged: one sub-task per op; unmerged: one per task and op
+            assert info["subtasks"] == (3 if merge else 3 * len(sizes)), info
+            for rep in range(2):
+                for o in outs:
+                    for t in o[:5]:
+                        t.zero_()
+                p.launch()
+                eng.sync()
+                for k, o in enumerate(outs):
+                    what = f"merge={merge} replay {rep} task {k} (n={sizes[k]})"
+                    assert np.array_equal(o[0].cpu().numpy(), ref[k][0]), f"{what}: pk"
+                    assert np.array_equal(o[1].cpu().numpy(), ref[k][1]), f"{what}: sk"
+                    assert np.array_equal(o[2].cpu().numpy(), refe[k][0]), f"{what}: ct"
+                    assert np.array_equal(o[3].cpu().numpy(), refe[k][1]), f"{what}: ss"
+                    assert np.array_equal(o[4].cpu().numpy(), refd[k]), f"{what}: decaps ss"
+                eng.keypair(SABER, _dev(torch, oracle.prng_bytes(5, 96 * 3).reshape(3, 96)))  # unrelated work
+                eng.sync()
+        finally:
+            p.close()

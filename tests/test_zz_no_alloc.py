@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 EBUDGET, EINVAL = -4, -1
 # (DPKEM_NOALLOC_SCRATCH_BYTES - 4096) / per-instance scratch, per scheme
-MAX_N = {D.FRODO976: 4227, D.SABER: 13796, D.FRODO976_AES: 4227, D.LIGHTSABER: 26212, D.FIRESABER: 8737,
-         D.FRODO640: 6393, D.FRODO1344: 3083}
+MAX_N = {D.FRODO976: 4160, D.SABER: 13796, D.FRODO976_AES: 4160, D.LIGHTSABER: 26212, D.FIRESABER: 8737,
+         D.FRODO640: 6241, D.FRODO1344: 3048}
 
 
 @pytest.fixture(scope="module")
