@@ -308,12 +308,15 @@ def chain_arrays(torch, scheme, n, seed):
     return a
 
 
-def build_chain_plan(eng, scheme, a, nstreams, chunks, ipw=1, share_matrix=False, pipeline=True, graph=True):
+def build_chain_plan(eng, scheme, a, nstreams, chunks, knob=None, share_matrix=False, pipeline=True, graph=True):
     """The plan a timed step replays: keygen -> encaps -> decaps chained per
-    instance over the arrays of chain_arrays; ipw is the Saber product's
-    instances per workgroup (a launch-shape knob)."""
+    instance over the arrays of chain_arrays.  knob is the scheme's
+    launch-shape knob: the Saber product's instances per workgroup (1 / 2 /
+    4), or the Frodo fused kernels' workgroup size (1024 / 512)."""
     import dpcrypto_amd as D
-    eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    saber_family = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
+    eng.set_tuning(D.TUNE_SABER_POLY_IPW, knob if saber_family and knob else 1)
+    eng.set_tuning(D.TUNE_FRODO_THREADS, knob if not saber_family and knob else 0)
     p = eng.plan(nstreams=nstreams, chunks=chunks, share_matrix=share_matrix, pipeline=pipeline, graph=graph)
     t0 = p.keypair(scheme, a["pk"], a["sk"], a["kc"])
     t1 = p.enc(scheme, a["ct"], a["ss"], a["pk"], a["ec"], after=t0)
@@ -327,14 +330,18 @@ def tune_chain_schedule(torch, eng, scheme, n, a, tune_budget_s=60.0, share_matr
     from dpcrypto_amd import autotune
     saber_family = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
 
-    def build(ns, ch, ipw=1):
-        return build_chain_plan(eng, scheme, a, ns, ch, ipw, share_matrix, pipeline)
+    def build(ns, ch, knob=None):
+        return build_chain_plan(eng, scheme, a, ns, ch, knob, share_matrix, pipeline)
 
-    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}|pipe={int(pipeline)}|ipw"
+    key = f"scheme{scheme}|kg-enc-dec|n={n}|share={int(share_matrix)}|pipe={int(pipeline)}|knob"
     if saber_family:
         return autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s,
                              candidates=[c + (1,) for c in autotune.DEFAULT_CANDIDATES],
                              refine=lambda b: [(b[0], b[1], 2), (b[0], b[1], 4)])
+    if scheme in (D.FRODO976, D.FRODO976_AES):
+        return autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s,
+                             candidates=[c + (512,) for c in autotune.DEFAULT_CANDIDATES],
+                             refine=lambda b: [(b[0], b[1], 1024)])
     return autotune.tune(torch, eng, key, build, use_cache=False, budget_s=tune_budget_s)
 
 
@@ -351,6 +358,7 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
     else:
         best, tried = tune_chain_schedule(torch, eng, scheme, n, a, tune_budget_s, share_matrix, pipeline)
     plan = build_chain_plan(eng, scheme, a, *best, share_matrix=share_matrix, pipeline=pipeline)
+    knob_name = "saber_poly_ipw" if scheme in (1, 3, 4) else "frodo_threads"
     for _ in range(warmup):
         plan.launch()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
@@ -368,7 +376,7 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
     plan.close()
     return dict(wall_s=wall, per_step_ms=per_step, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
                 schedule={"nstreams": best[0], "chunks": best[1],
-                          **({"saber_poly_ipw": best[2]} if len(best) > 2 else {}),
+                          **({knob_name: best[2]} if len(best) > 2 else {}),
                           "tuning_ms": {"x".join(map(str, c)): round(v, 4) for c, v in tried.items()}})
 
 
@@ -406,8 +414,9 @@ def run_per_op(torch, eng, plain_eng, scheme, name, n, steps, seed, tune_budget_
            for op, ws in (("keygen", (s["pk"], s["sk"])), ("enc", (s["ct"], s["ss"])), ("dec", (s["ss"],)))}
     saber_family = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
 
-    def build(op, ns, ch, ipw=1):
-        eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    def build(op, ns, ch, knob=None):
+        eng.set_tuning(D.TUNE_SABER_POLY_IPW, knob if saber_family and knob else 1)
+        eng.set_tuning(D.TUNE_FRODO_THREADS, knob if not saber_family and knob else 0)
         p = eng.plan(nstreams=ns, chunks=ch)
         o = out[op]
         if op == "keygen":
@@ -431,7 +440,8 @@ def run_per_op(torch, eng, plain_eng, scheme, name, n, steps, seed, tune_budget_
             ts.append(x.elapsed_time(y))
         return statistics.median(ts)
 
-    cands = [(1, 1), (2, 2), (4, 4), (8, 4)] if not saber_family else [(1, 1, 1), (1, 1, 4), (2, 2, 1)]
+    cands = ([(1, 1, 512), (2, 2, 512), (4, 4, 512), (8, 4, 512), (1, 1, 1024), (4, 4, 1024)] if not saber_family
+             else [(1, 1, 1), (1, 1, 4), (2, 2, 1)])
     res = {}
     for op in ("keygen", "enc", "dec"):
         best, tried = autotune.tune(torch, eng, f"scheme{scheme}|{op}|n={n}", lambda *c: build(op, *c),

@@ -133,7 +133,8 @@ void io_sizes(int scheme, int op, size_t out[2], size_t in[2]) {
 
 // Launch one chunk with all pointers on the device.
 void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
-                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh, int saber_ipw) {
+                     const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh, int saber_ipw,
+                     int frodo_threads) {
   Carver c{scratch};
   if (form == DPKEM_FORM_PLAIN) {
     if (scheme == DPKEM_SABER) plain_saber_launch(st, op, n, o0, o1, i0, i1, scratch);
@@ -161,9 +162,9 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
     uint32_t* state = reinterpret_cast<uint32_t*>(c.take((size_t)n * 200));
     const bool aes = scheme == DPKEM_FRODO976_AES;
-    if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8, aes, set);
-    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh, aes, set);
-    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail, state, aes, set);
+    if (op == kKeygen) frodo_keygen_launch(st, n, i0, o0, o1, s8, e8, aes, set, frodo_threads);
+    else if (op == kEnc) frodo_enc_launch(st, n, i0, i1, o0, o1, s8, e8, epp8, kbuf, frodo_pkh, aes, set, frodo_threads);
+    else frodo_dec_launch(st, n, i0, i1, o0, s8, e8, epp8, kbuf, mubuf, fail, state, aes, set, frodo_threads);
   }
 }
 
@@ -294,7 +295,7 @@ int run_op(dpkem_engine* e, int scheme, int op, size_t n, uint8_t* o0, uint8_t* 
       launch_chunk_shared(e->stream, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, bcast);
     } else {
       launch_chunk_on(e->stream, form, scheme, op, c, dout[0], dout[1], din[0], din[1], e->arena, nullptr,
-                      e->saber_ipw);
+                      e->saber_ipw, e->frodo_threads);
     }
     tls_profiler = nullptr;
     HIP_TRY(e, hipGetLastError());
@@ -450,6 +451,11 @@ int dpkem_engine_set_tuning(dpkem_engine* e, int knob, int value) {
   if (knob == DPKEM_TUNE_FRODO_PERSIST && (value == 0 || value == 1)) {
     std::lock_guard<std::mutex> lock(e->mu);
     e->frodo_persist = value;
+    return DPKEM_OK;
+  }
+  if (knob == DPKEM_TUNE_FRODO_THREADS && (value == 0 || value == 1024 || value == 512)) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    e->frodo_threads = value == 512 ? 0 : value;
     return DPKEM_OK;
   }
   return DPKEM_EINVAL;

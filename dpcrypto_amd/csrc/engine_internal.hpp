@@ -19,6 +19,7 @@ struct dpkem_engine {
   int form = DPKEM_FORM_BATCHED;
   int saber_ipw = 1;  // DPKEM_TUNE_SABER_POLY_IPW
   int frodo_persist = 1;  // DPKEM_TUNE_FRODO_PERSIST
+  int frodo_threads = 0;  // DPKEM_TUNE_FRODO_THREADS (0 = the set's default shape, 1024 = FrodoKEM-976's other)
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
   char last_err[256] = {0};
@@ -50,7 +51,7 @@ void launch_frodo_grouped(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o
 // saber_ipw: Saber-family product instances per workgroup (DPKEM_TUNE_SABER_POLY_IPW).
 void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_t* o0, uint8_t* o1,
                      const uint8_t* i0, const uint8_t* i1, uint8_t* scratch, const uint8_t* frodo_pkh = nullptr,
-                     int saber_ipw = 1);
+                     int saber_ipw = 1, int frodo_threads = 0);
 int record_hip_error(dpkem_engine* e, hipError_t err, const char* what);
 
 }  // namespace dpkem

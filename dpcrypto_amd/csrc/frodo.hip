@@ -59,7 +59,7 @@ struct FrodoSet<640> {
 };
 template <>
 struct FrodoSet<976> {
-  static constexpr int n = 976, logq = 16, b = 3, sec = 24, chain_rate = kRateShake256, threads = 1024, passes = 1;
+  static constexpr int n = 976, logq = 16, b = 3, sec = 24, chain_rate = kRateShake256, threads = 512, passes = 2;
 };
 template <>
 struct FrodoSet<1344> {
@@ -99,6 +99,20 @@ struct FrodoP : FrodoSet<N> {
                 "rows and fields stay 8-byte aligned");
   static_assert(S::b * 64 == 8 * S::sec, "mu fills the 8 x 8 message matrix");
 };
+// Launch shape of the fused kernels: TH lanes per workgroup cover the n rows
+// of A in `passes` passes of one row per lane.  FrodoKEM-976's default is
+// 512 x 2: two workgroups per CU, so that one's prologue and epilogue (S'
+// staging, B' / V / C) overlap the other's row generation (measured 8-9%
+// faster than 1024 x 1 at 8192 instances); 1024 x 1 (one row per lane, one
+// workgroup per CU) is the alternative shape the tuning knob selects.
+template <class P, int TH>
+struct FrodoShape {
+  static constexpr int passes = (P::n + TH - 1) / TH;
+  static constexpr int sprow = TH * passes;  // LDS row stride of int8 S' (enc / dec)
+  static_assert(TH % 64 == 0 && sprow >= P::n && sprow - TH < P::n, "every pass has rows");
+};
+constexpr int kFrodoAltThreads = 1024;  // the FrodoKEM-976 alternative shape
+
 using F976 = FrodoP<976>;
 static_assert(F976::pk == kFrodoPk && F976::sk == kFrodoSk && F976::ct == kFrodoCt &&
               F976::kcoins == kFrodoKeygenCoins, "FrodoKEM976-GPU/include/api.h:7-10");
@@ -738,10 +752,11 @@ __device__ __forceinline__ void aes_rows_init(AesRows& g, const uint32_t* T, int
 // wave * 64 + lane + pass * threads; a wave whose rows all lie past n in the
 // last pass skips it.
 // Task arrays: o0 = pk, o1 = sk, s8 / e8 = the chains' int8 S^T and E.
-template <class P, bool AES, bool PERSIST>
-__global__ void __launch_bounds__(P::threads) frodo_as_fused(const FrMt M) {
+template <class P, bool AES, bool PERSIST, int TH>
+__global__ void __launch_bounds__(TH) frodo_as_fused(const FrMt M) {
   static_assert(!AES || P::n == 976, "AES rows are defined for FrodoKEM-976");
-  constexpr int T = P::threads, SROW = P::srow, N = P::n;
+  using SH = FrodoShape<P, TH>;
+  constexpr int T = TH, SROW = P::srow, N = P::n;
   __shared__ __align__(16) int8_t Sl[9 * SROW];  // rows 0..7 = S^T, row 8 = zeros
   __shared__ int colsum[8];
   __shared__ __align__(16) uint32_t aes_lds[AES ? kAesLdsWords : 1];
@@ -789,7 +804,7 @@ __global__ void __launch_bounds__(P::threads) frodo_as_fused(const FrMt M) {
   const int8_t* e = e8 + (size_t)inst * P::mat;
   uint8_t* pb = pk + (size_t)inst * P::pk + kFrodoSeedA;
   uint8_t* sb = sk + (size_t)inst * P::sk + P::sec + kFrodoSeedA;
-  for (int pass = 0; pass < P::passes; ++pass) {
+  for (int pass = 0; pass < SH::passes; ++pass) {
     const int row0 = pass * T + wave * 64;
     if (row0 >= N) break;
     v4i acc_lo0 = {0, 0, 0, 0}, acc_lo1 = {0, 0, 0, 0}, acc_hi0 = {0, 0, 0, 0}, acc_hi1 = {0, 0, 0, 0};
@@ -851,10 +866,11 @@ __global__ void __launch_bounds__(P::threads) frodo_as_fused(const FrMt M) {
 // Task arrays.  enc: i0 = pk, i1 = coins (mu), o0 = ct.  dec: i0 = ct,
 // i1 = sk (its pk part is used), mubuf = mu', fail.  Both: s8 / e8 / epp8 =
 // the chains' int8 S', E', E''.
-template <class P, bool DEC, bool AES, bool PERSIST>
-__global__ void __launch_bounds__(P::threads) frodo_sa_fused(const FrMt M) {
+template <class P, bool DEC, bool AES, bool PERSIST, int TH>
+__global__ void __launch_bounds__(TH) frodo_sa_fused(const FrMt M) {
   static_assert(!AES || P::n == 976, "AES rows are defined for FrodoKEM-976");
-  constexpr int T = P::threads, SP = P::sprow, N = P::n;
+  using SH = FrodoShape<P, TH>;
+  constexpr int T = TH, SP = SH::sprow, N = P::n;
   __shared__ __align__(16) uint32_t Bacc[P::mat];
   __shared__ __align__(16) int8_t Sp[8 * SP];  // S' rows, zero for i >= n
   __shared__ int rowsum[8];
@@ -912,7 +928,7 @@ __global__ void __launch_bounds__(P::threads) frodo_sa_fused(const FrMt M) {
     for (int d = 0; d < 4; ++d) sel[qq][d] = (g == qq && d == (nn >> 2)) ? onehot : 0;
   const v4i zero = {0, 0, 0, 0};
 
-  for (int pass = 0; pass < P::passes; ++pass) {
+  for (int pass = 0; pass < SH::passes; ++pass) {
     const int row0 = pass * T + wave * 64;
     if (row0 >= N) break;
     // product A operand: lane (g, m = nn) holds S'[m][rows row0 + 16 q + 4 g + i] in dword q, byte i
@@ -1284,12 +1300,12 @@ static int resident_blocks(K kernel, int threads) {
 // Grid of a fused launch over `total` work items: one workgroup per item,
 // or for the persistent form at most the resident workgroups (looked up
 // once per kernel).
-template <class P, int OP, bool AES, bool PERSIST>
+template <class P, int OP, bool AES, bool PERSIST, int TH>
 static dim3 fused_grid(int total) {
   if (!PERSIST) return dim3(total);
-  static const int res = OP == 0   ? resident_blocks(frodo_as_fused<P, AES, true>, P::threads)
-                         : OP == 1 ? resident_blocks(frodo_sa_fused<P, false, AES, true>, P::threads)
-                                   : resident_blocks(frodo_sa_fused<P, true, AES, true>, P::threads);
+  static const int res = OP == 0   ? resident_blocks(frodo_as_fused<P, AES, true, TH>, TH)
+                         : OP == 1 ? resident_blocks(frodo_sa_fused<P, false, AES, true, TH>, TH)
+                                   : resident_blocks(frodo_sa_fused<P, true, AES, true, TH>, TH);
   return dim3(res > 0 ? std::min(res, total) : total);
 }
 
@@ -1329,25 +1345,41 @@ static FrMt table_of(const FrMtTask* tasks, int ntasks) {
   return M;
 }
 
-template <class P, bool AES>
-static void launch_as_fused(hipStream_t st, const FrMt& M, bool persist = false) {
+template <class P, bool AES, int TH>
+static void launch_as_fused_t(hipStream_t st, const FrMt& M, bool persist) {
   const char* name = AES ? "frodo_as_fused<aes>" : FR_NAME("frodo_as_fused");
   if (persist)
-    DPKEM_LAUNCH(name, (frodo_as_fused<P, AES, true>), (fused_grid<P, 0, AES, true>(M.total)), dim3(P::threads), st, M);
+    DPKEM_LAUNCH(name, (frodo_as_fused<P, AES, true, TH>), (fused_grid<P, 0, AES, true, TH>(M.total)), dim3(TH), st, M);
   else
-    DPKEM_LAUNCH(name, (frodo_as_fused<P, AES, false>), (fused_grid<P, 0, AES, false>(M.total)), dim3(P::threads), st,
+    DPKEM_LAUNCH(name, (frodo_as_fused<P, AES, false, TH>), (fused_grid<P, 0, AES, false, TH>(M.total)), dim3(TH), st,
                  M);
 }
-template <class P, bool DEC, bool AES>
-static void launch_sa_fused(hipStream_t st, const FrMt& M, bool persist = false) {
+template <class P, bool DEC, bool AES, int TH>
+static void launch_sa_fused_t(hipStream_t st, const FrMt& M, bool persist) {
   const char* name = DEC ? (AES ? "frodo_sa_fused<true,aes>" : FR_NAME("frodo_sa_fused<true>"))
                          : (AES ? "frodo_sa_fused<false,aes>" : FR_NAME("frodo_sa_fused<false>"));
   if (persist)
-    DPKEM_LAUNCH(name, (frodo_sa_fused<P, DEC, AES, true>), (fused_grid<P, DEC ? 2 : 1, AES, true>(M.total)),
-                 dim3(P::threads), st, M);
+    DPKEM_LAUNCH(name, (frodo_sa_fused<P, DEC, AES, true, TH>), (fused_grid<P, DEC ? 2 : 1, AES, true, TH>(M.total)),
+                 dim3(TH), st, M);
   else
-    DPKEM_LAUNCH(name, (frodo_sa_fused<P, DEC, AES, false>), (fused_grid<P, DEC ? 2 : 1, AES, false>(M.total)),
-                 dim3(P::threads), st, M);
+    DPKEM_LAUNCH(name, (frodo_sa_fused<P, DEC, AES, false, TH>), (fused_grid<P, DEC ? 2 : 1, AES, false, TH>(M.total)),
+                 dim3(TH), st, M);
+}
+// threads: 0 (or the set's own) = the set's default shape; kFrodoAltThreads
+// selects FrodoKEM-976's alternative
+template <class P, bool AES>
+static void launch_as_fused(hipStream_t st, const FrMt& M, bool persist = false, int threads = 0) {
+  if constexpr (P::n == 976) {
+    if (threads == kFrodoAltThreads) return launch_as_fused_t<P, AES, kFrodoAltThreads>(st, M, persist);
+  }
+  launch_as_fused_t<P, AES, P::threads>(st, M, persist);
+}
+template <class P, bool DEC, bool AES>
+static void launch_sa_fused(hipStream_t st, const FrMt& M, bool persist = false, int threads = 0) {
+  if constexpr (P::n == 976) {
+    if (threads == kFrodoAltThreads) return launch_sa_fused_t<P, DEC, AES, kFrodoAltThreads>(st, M, persist);
+  }
+  launch_sa_fused_t<P, DEC, AES, P::threads>(st, M, persist);
 }
 template <class P>
 static void launch_dec_mu(hipStream_t st, const FrMt& M) {
@@ -1355,64 +1387,66 @@ static void launch_dec_mu(hipStream_t st, const FrMt& M) {
 }
 
 template <class P, bool AES>
-static void keygen_t(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8, uint32_t* e8) {
+static void keygen_t(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8, uint32_t* e8,
+                     int threads) {
   DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, coins, pk,
                sk, s8, e8);
   const FrMtTask t = task_of(n, coins, nullptr, pk, sk, s8, e8);
-  launch_as_fused<P, AES>(st, table_of(&t, 1));
+  launch_as_fused<P, AES>(st, table_of(&t, 1), false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_keygen_post"), frodo_keygen_post<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
 }
 
 template <class P, bool AES>
 static void enc_t(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss, uint32_t* s8,
-                  uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh) {
+                  uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh, int threads) {
   DPKEM_LAUNCH(FR_NAME("frodo_encdec_pre<false>"), (frodo_encdec_pre<P, false>), dim3(blocks_for(n, 32)), dim3(64), st,
                n, pk, (size_t)P::pk, coins, (size_t)P::ecoins, kbuf, s8, e8, epp8, pkh);
   const FrMtTask t = task_of(n, pk, coins, ct, ss, s8, e8, epp8, kbuf);
-  launch_sa_fused<P, false, AES>(st, table_of(&t, 1));
+  launch_sa_fused<P, false, AES>(st, table_of(&t, 1), false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_ss"), frodo_ss<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, kbuf, nullptr,
                (size_t)0, nullptr, ss);
 }
 
 template <class P, bool AES>
 static void dec_t(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8, uint32_t* e8,
-                  uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, uint32_t* state) {
+                  uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, uint32_t* state, int threads) {
   const FrMtTask t = task_of(n, ct, sk, ss, nullptr, s8, e8, epp8, kbuf, mubuf, fail, state, (size_t)P::sk);
   const FrMt M = table_of(&t, 1);
   launch_dec_mu<P>(st, M);
   DPKEM_LAUNCH(FR_NAME("frodo_dec_pre"), frodo_dec_pre<P>, dim3(blocks_for(n, 32)), dim3(128), st, n, ct, sk,
                (size_t)P::sk, mubuf, kbuf, s8, e8, epp8, state);
-  launch_sa_fused<P, true, AES>(st, M);
+  launch_sa_fused<P, true, AES>(st, M, false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, state, kbuf,
                sk, (size_t)P::sk, fail, ss);
 }
 
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
-                         uint32_t* e8, bool aes, int set) {
+                         uint32_t* e8, bool aes, int set, int threads) {
   if (n <= 0) return;
-  if (set == 640) keygen_t<FrodoP<640>, false>(st, n, coins, pk, sk, s8, e8);
-  else if (set == 1344) keygen_t<FrodoP<1344>, false>(st, n, coins, pk, sk, s8, e8);
-  else if (aes) keygen_t<F976, true>(st, n, coins, pk, sk, s8, e8);
-  else keygen_t<F976, false>(st, n, coins, pk, sk, s8, e8);
+  if (set == 640) keygen_t<FrodoP<640>, false>(st, n, coins, pk, sk, s8, e8, threads);
+  else if (set == 1344) keygen_t<FrodoP<1344>, false>(st, n, coins, pk, sk, s8, e8, threads);
+  else if (aes) keygen_t<F976, true>(st, n, coins, pk, sk, s8, e8, threads);
+  else keygen_t<F976, false>(st, n, coins, pk, sk, s8, e8, threads);
 }
 
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh, bool aes, int set) {
+                      uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh, bool aes, int set,
+                      int threads) {
   if (n <= 0) return;
-  if (set == 640) enc_t<FrodoP<640>, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
-  else if (set == 1344) enc_t<FrodoP<1344>, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
-  else if (aes) enc_t<F976, true>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
-  else enc_t<F976, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh);
+  if (set == 640) enc_t<FrodoP<640>, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh, threads);
+  else if (set == 1344) enc_t<FrodoP<1344>, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh, threads);
+  else if (aes) enc_t<F976, true>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh, threads);
+  else enc_t<F976, false>(st, n, pk, coins, ct, ss, s8, e8, epp8, kbuf, pkh, threads);
 }
 
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
                       uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, uint32_t* state,
-                      bool aes, int set) {
+                      bool aes, int set, int threads) {
   if (n <= 0) return;
-  if (set == 640) dec_t<FrodoP<640>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
-  else if (set == 1344) dec_t<FrodoP<1344>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
-  else if (aes) dec_t<F976, true>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
-  else dec_t<F976, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state);
+  if (set == 640) dec_t<FrodoP<640>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state, threads);
+  else if (set == 1344) dec_t<FrodoP<1344>, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state, threads);
+  else if (aes) dec_t<F976, true>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state, threads);
+  else dec_t<F976, false>(st, n, ct, sk, ss, s8, e8, epp8, kbuf, mubuf, fail, state, threads);
 }
 
 
@@ -1425,7 +1459,7 @@ size_t frodo_pipe_scratch_per_inst(int set) { return 16 * (size_t)set + 3 * 64 +
 
 template <class P, bool AES>
 static void pipe_t(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk, uint8_t* ct,
-                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch) {
+                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int threads) {
   size_t off = 0;
   auto take = [&](size_t per) {
     uint8_t* q = scratch + off;
@@ -1443,27 +1477,27 @@ static void pipe_t(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   const dim3 g(blocks_for(n, 32));
   DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, g, dim3(64), st, n, kc, pk, sk, s8, e8);
   const FrMtTask tk = task_of(n, kc, nullptr, pk, sk, s8, e8);
-  launch_as_fused<P, AES>(st, table_of(&tk, 1));
+  launch_as_fused<P, AES>(st, table_of(&tk, 1), false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_x1"), frodo_pipe_x1<P>, g, dim3(128), st, n, pk, sk, ec, kbuf_e, s8, e8, epp8);
   const FrMtTask te = task_of(n, pk, ec, ct, ss, s8, e8, epp8, kbuf_e);
-  launch_sa_fused<P, false, AES>(st, table_of(&te, 1));
+  launch_sa_fused<P, false, AES>(st, table_of(&te, 1), false, threads);
   const FrMtTask td = task_of(n, ct, sk, ss2, nullptr, s8, e8, epp8, kbuf_d, mubuf, fail, state, (size_t)P::sk);
   const FrMt Md = table_of(&td, 1);
   launch_dec_mu<P>(st, Md);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_x2"), frodo_pipe_x2<P>, g, dim3(192), st, n, ct, sk, kbuf_e, ss, mubuf, kbuf_d, s8,
                e8, epp8, state);
-  launch_sa_fused<P, true, AES>(st, Md);
+  launch_sa_fused<P, true, AES>(st, Md, false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, g, dim3(64), st, n, ct, state, kbuf_d, sk, (size_t)P::sk,
                fail, ss2);
 }
 
 void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes) {
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes, int threads) {
   if (n <= 0) return;
-  if (set == 640) pipe_t<FrodoP<640>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
-  else if (set == 1344) pipe_t<FrodoP<1344>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
-  else if (aes) pipe_t<F976, true>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
-  else pipe_t<F976, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch);
+  if (set == 640) pipe_t<FrodoP<640>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
+  else if (set == 1344) pipe_t<FrodoP<1344>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
+  else if (aes) pipe_t<F976, true>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
+  else pipe_t<F976, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
 }
 
 // Encaps / decaps where instances share public keys in groups (see
@@ -1540,7 +1574,7 @@ __global__ void __launch_bounds__(64) frodo_mt_post(const FrMt M) {
 }
 
 template <class P, bool AES>
-static void mt_t(hipStream_t st, int op, const FrodoMtDesc* d, int ntasks, bool persist) {
+static void mt_t(hipStream_t st, int op, const FrodoMtDesc* d, int ntasks, bool persist, int threads) {
   FrMtTask tasks[kFrodoMtTasks];
   for (int k = 0; k < ntasks; ++k) {
     const int n = d[k].n;
@@ -1565,26 +1599,27 @@ static void mt_t(hipStream_t st, int op, const FrodoMtDesc* d, int ntasks, bool 
   const dim3 groups(M.groups);
   if (op == 0) {
     DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<keygen>"), (frodo_mt_pre<P, 0>), groups, dim3(64), st, M);
-    launch_as_fused<P, AES>(st, M, persist);
+    launch_as_fused<P, AES>(st, M, persist, threads);
     DPKEM_LAUNCH(FR_NAME("frodo_mt_post<keygen>"), (frodo_mt_post<P, 0>), groups, dim3(64), st, M);
   } else if (op == 1) {
     DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<enc>"), (frodo_mt_pre<P, 1>), groups, dim3(64), st, M);
-    launch_sa_fused<P, false, AES>(st, M, persist);
+    launch_sa_fused<P, false, AES>(st, M, persist, threads);
     DPKEM_LAUNCH(FR_NAME("frodo_mt_post<enc>"), (frodo_mt_post<P, 1>), groups, dim3(64), st, M);
   } else {
     launch_dec_mu<P>(st, M);
     DPKEM_LAUNCH(FR_NAME("frodo_mt_dec_pre"), frodo_mt_dec_pre<P>, groups, dim3(128), st, M);
-    launch_sa_fused<P, true, AES>(st, M, persist);
+    launch_sa_fused<P, true, AES>(st, M, persist, threads);
     DPKEM_LAUNCH(FR_NAME("frodo_mt_post<dec>"), (frodo_mt_post<P, 2>), groups, dim3(64), st, M);
   }
 }
 
-void frodo_mt_launch(hipStream_t st, int op, int set, bool aes, const FrodoMtDesc* tasks, int ntasks, bool persist) {
+void frodo_mt_launch(hipStream_t st, int op, int set, bool aes, const FrodoMtDesc* tasks, int ntasks, bool persist,
+                     int threads) {
   if (ntasks <= 0 || ntasks > kFrodoMtTasks) return;
-  if (set == 640) mt_t<FrodoP<640>, false>(st, op, tasks, ntasks, persist);
-  else if (set == 1344) mt_t<FrodoP<1344>, false>(st, op, tasks, ntasks, persist);
-  else if (aes) mt_t<F976, true>(st, op, tasks, ntasks, persist);
-  else mt_t<F976, false>(st, op, tasks, ntasks, persist);
+  if (set == 640) mt_t<FrodoP<640>, false>(st, op, tasks, ntasks, persist, threads);
+  else if (set == 1344) mt_t<FrodoP<1344>, false>(st, op, tasks, ntasks, persist, threads);
+  else if (aes) mt_t<F976, true>(st, op, tasks, ntasks, persist, threads);
+  else mt_t<F976, false>(st, op, tasks, ntasks, persist, threads);
 }
 
 }  // namespace dpkem

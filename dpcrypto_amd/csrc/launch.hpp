@@ -43,17 +43,19 @@ void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch
 // per instance: s8 / e8 n * 8 bytes each (frodo_scratch_per_inst).
+// threads: workgroup shape of the fused kernels (0 = the set's default,
+// FrodoKEM-976 also 512: rows in two passes, two workgroups per CU).
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
-                         uint32_t* e8, bool aes = false, int set = 976);
+                         uint32_t* e8, bool aes = false, int set = 976, int threads = 0);
 // pkh: nullptr (hash each pk) or one sec-byte pkh row per instance
 void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
                       uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh = nullptr,
-                      bool aes = false, int set = 976);
+                      bool aes = false, int set = 976, int threads = 0);
 // Pipelined keygen -> encaps -> decaps of the same instances on one stream
 // (frodo.hip); set / aes as above.
 size_t frodo_pipe_scratch_per_inst(int set);
 void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes);
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes, int threads = 0);
 // Groups of up to kFrodoGroup instances sharing one public key expand A once;
 // groups[b] = (first, count) or nullptr for consecutive groups of
 // kFrodoGroup; key strides may be 0 (one key for all n).
@@ -67,7 +69,7 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
                               uint32_t* state, const uint2* groups, int ngroups);
 void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint32_t* s8,
                       uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, uint8_t* mubuf, uint32_t* fail, uint32_t* state,
-                      bool aes = false, int set = 976);
+                      bool aes = false, int set = 976, int threads = 0);
 
 // Independent tasks of one Frodo op (0 keygen, 1 enc, 2 dec) and parameter
 // set as ONE launch sequence (frodo.hip, multi-task launches): per task its
@@ -86,7 +88,7 @@ struct FrodoMtDesc {
 // persist: the fused kernel runs as a persistent grid of resident workgroups
 // walking all tasks' instances, instead of one workgroup per instance.
 void frodo_mt_launch(hipStream_t st, int op, int set, bool aes, const FrodoMtDesc* tasks, int ntasks,
-                     bool persist = true);
+                     bool persist = true, int threads = 0);
 
 // ---- plain comparator form (plain.hip); op 0 keygen, 1 enc, 2 dec with the
 // same output / input roles as the batched launchers.

@@ -50,7 +50,7 @@ struct Sub {
 
 struct dpkem_plan {
   dpkem_engine* e = nullptr;
-  int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED, saber_ipw = 1, frodo_persist = 1;
+  int nstreams = 1, chunks = 1, form = DPKEM_FORM_BATCHED, saber_ipw = 1, frodo_persist = 1, frodo_threads = 0;
   unsigned flags = 0;
   std::vector<Task> tasks;
   std::vector<Sub> subs;
@@ -273,7 +273,7 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
         d[k] = {(int)U.n, U.i0, U.i1, U.o0, U.o1, p->scratch + s.mt_off[k]};
       }
       frodo_mt_launch(st, T.op, frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES, d, (int)s.mt.size(),
-                      p->frodo_persist != 0);
+                      p->frodo_persist != 0, p->frodo_threads);
       ok = ok && hipGetLastError() == hipSuccess;
       ok = ok && hipEventRecord(s.done, st) == hipSuccess;
       continue;
@@ -296,7 +296,7 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
                           saber_rank_of(T.scheme), p->saber_ipw);
       else
         frodo_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
-                          frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES);
+                          frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES, p->frodo_threads);
       ok = ok && hipGetLastError() == hipSuccess;
       ok = ok && hipEventRecord(s.done, st) == hipSuccess;
       continue;
@@ -308,7 +308,7 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
     const uint8_t* i0 = in[0] ? T.i0 + s.begin * in[0] : nullptr;
     const uint8_t* i1 = in[1] ? T.i1 + s.begin * in[1] : nullptr;
     launch_chunk_on(st, p->form, T.scheme, T.op, (int)s.count, o0, o1, i0, i1, p->scratch + s.scratch_off, nullptr,
-                    p->saber_ipw);
+                    p->saber_ipw, p->frodo_threads);
     ok = ok && hipGetLastError() == hipSuccess;
     ok = ok && hipEventRecord(s.done, st) == hipSuccess;
   }
@@ -333,6 +333,7 @@ int dpkem_plan_create(dpkem_engine* e, dpkem_plan** out, int nstreams, int chunk
   p->form = e->form;
   p->saber_ipw = e->saber_ipw;
   p->frodo_persist = e->frodo_persist;
+  p->frodo_threads = e->frodo_threads;
   *out = p;
   return DPKEM_OK;
 }

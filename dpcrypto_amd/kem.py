@@ -31,6 +31,7 @@ FORM_PLAIN = 1
 # kernel tuning knobs (dpkem_engine_set_tuning)
 TUNE_SABER_POLY_IPW = 1
 TUNE_FRODO_PERSIST = 2
+TUNE_FRODO_THREADS = 3
 
 ERRORS = {
     -1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EBUDGET", -5: "EBUSY", -6: "ENOTFOUND", -7: "ECANCELED",

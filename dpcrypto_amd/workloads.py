@@ -107,20 +107,22 @@ def run_mixed(torch, eng, plain_eng, seed: int = 0x44504B454D000004, total: int 
                 p.dec(sc, o[0], ins[0], ins[1])
         return p.build()
 
-    def build(ns, ch, ipw=1, persist=1, merge=1):
+    def build(ns, ch, ipw=1, persist=1, merge=1, threads=512):
         eng.set_tuning(K.TUNE_SABER_POLY_IPW, ipw)  # the Saber product's instances per workgroup
         eng.set_tuning(K.TUNE_FRODO_PERSIST, persist)  # persistent fused kernel in merged Frodo launches
+        eng.set_tuning(K.TUNE_FRODO_THREADS, threads)  # Frodo fused kernels' workgroup size
         return add_all(eng.plan(nstreams=ns, chunks=ch, merge=bool(merge)))
 
     if progress:
         progress(f"mixed: {len(batches)} batches, autotuning the plan")
     # coordinate descent: streams x chunks at the default knobs, then each
     # knob's other values at the best schedule
-    best, tried = autotune.tune(torch, eng, f"mixed|seed={seed}|total={total}|ipw|persist|merge", build,
+    best, tried = autotune.tune(torch, eng, f"mixed|seed={seed}|total={total}|ipw|persist|merge|threads", build,
                                 use_cache=False, budget_s=tune_budget_s,
-                                candidates=[c + (1, 1, 1) for c in autotune.DEFAULT_CANDIDATES],
-                                refine=lambda b: [(b[0], b[1], 2, 1, 1), (b[0], b[1], 4, 1, 1), (b[0], b[1], 1, 0, 1),
-                                                  (b[0], b[1], 1, 1, 0)])
+                                candidates=[c + (1, 1, 1, 512) for c in autotune.DEFAULT_CANDIDATES],
+                                refine=lambda b: [(b[0], b[1], 2, 1, 1, 512), (b[0], b[1], 4, 1, 1, 512),
+                                                  (b[0], b[1], 1, 0, 1, 512), (b[0], b[1], 1, 1, 0, 512),
+                                                  (b[0], b[1], 1, 1, 1, 1024), (b[0], b[1], 1, 0, 1, 1024)])
     plan = build(*best)
     plan.launch()
     torch.cuda.synchronize()
@@ -169,7 +171,7 @@ def run_mixed(torch, eng, plain_eng, seed: int = 0x44504B454D000004, total: int 
         "batched_ops_per_s": round(total / (batched_ms / 1e3), 1),
         "batched_ms": round(batched_ms, 3),
         "schedule": {"nstreams": best[0], "chunks": best[1], "saber_poly_ipw": best[2], "frodo_persistent": best[3],
-                     "frodo_tasks_merged": best[4],
+                     "frodo_tasks_merged": best[4], "frodo_threads": best[5],
                      "tuning_ms": {"x".join(map(str, c)): round(v, 3) for c, v in tried.items()}},
         "unmerged_ms": (round(min(v for c, v in tried.items() if c[4] == 0), 3)
                         if any(c[4] == 0 for c in tried) else None),

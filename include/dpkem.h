@@ -91,9 +91,14 @@ int dpkem_engine_set_form(dpkem_engine *e, int form);
  * DPKEM_TUNE_FRODO_PERSIST: in a plan's multi-task Frodo launches
  * (DPKEM_PLAN_NO_MERGE), the fused A-generation + MFMA kernel runs as a
  * persistent grid of resident workgroups walking every task's instances (1,
- * default) or as one workgroup per instance (0). */
+ * default) or as one workgroup per instance (0).
+ * DPKEM_TUNE_FRODO_THREADS: workgroup shape of the FrodoKEM-976 fused
+ * kernels (SHAKE and AES): 512 lanes over the rows of A in two passes, two
+ * workgroups per CU (default; also 0), or 1024 lanes with one row each, one
+ * workgroup per CU; the other Frodo sets keep their shapes. */
 #define DPKEM_TUNE_SABER_POLY_IPW 1
 #define DPKEM_TUNE_FRODO_PERSIST 2
+#define DPKEM_TUNE_FRODO_THREADS 3
 int dpkem_engine_set_tuning(dpkem_engine *e, int knob, int value);
 /* Block until all work submitted on the engine's stream has finished.
  * Returns DPKEM_EDEVICE (with a message in dpkem_engine_last_error) when a

@@ -10,8 +10,9 @@ GPU's bytes with the CPU oracle on the same coins:
       and 4 instances per workgroup), three graph replays with the outputs
       cleared and unrelated work in between; every instance of every replay
   C3  FrodoKEM-976, batch 8192: the pipelined plan on the 1x1 and 8x4
-      schedules; decaps recovers every shared secret, and 64 instances
-      (0 and 8191 among them) equal the oracle on pk, sk, ct and ss
+      schedules and with the 1024-lane fused shape; decaps recovers every
+      shared secret, and 64 instances (0 and 8191 among them) equal the
+      oracle on pk, sk, ct and ss
   C4  16,384 instances in batches of U[1, 4096] (random scheme, ops 1:4:4),
       one autotuned graph: equal to the plain form everywhere, and sampled
       instances of every batch equal the oracle
@@ -142,7 +143,7 @@ def test_c2_saber_headline_plan_at_4096_every_instance(eng, oracle, torch_cuda, 
 
 
 # ------------------------------------------------------------------ C3
-@pytest.mark.parametrize("schedule", [(1, 1), (8, 4)])
+@pytest.mark.parametrize("schedule", [(1, 1, 512), (8, 4, 512), (1, 1, 1024)])
 def test_c3_frodo_plan_at_8192(eng, oracle, torch_cuda, schedule):
     import bench
     torch = torch_cuda
@@ -150,6 +151,7 @@ def test_c3_frodo_plan_at_8192(eng, oracle, torch_cuda, schedule):
     print(f"C3: FrodoKEM-976 pipelined plan, n={n}, schedule {schedule}", flush=True)
     a = bench.chain_arrays(torch, FRODO, n, 0x44504B454D000002)
     p = bench.build_chain_plan(eng, FRODO, a, *schedule)
+    eng.set_tuning(3, 0)  # later plans of this engine: default fused shape
     try:
         for rep in range(2):
             _clear(torch, a)

@@ -70,3 +70,35 @@ def test_frodo_aes_kat_and_shared_key(eng, oracle):
     oct_, oss = oracle.enc(AES, np.repeat(pk, 5, axis=0), ec, threads=5)
     assert np.array_equal(ct, oct_) and np.array_equal(ss, oss)
     assert np.array_equal(eng.dec_shared_key(AES, ct, sk), oss)
+
+
+@pytest.mark.parametrize("scheme", [0, 2])
+def test_frodo976_fused_shape_knob_keeps_every_byte(oracle, scheme):
+    """DPKEM_TUNE_FRODO_THREADS = 1024 (one row per lane, one workgroup per
+    CU, beside the default 512 x 2) changes the fused kernels' launch shape only: keygen / encaps /
+    decaps of FrodoKEM-976-SHAKE and -AES equal the oracle, a tampered
+    ciphertext included, and the knob rejects other values."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import dpcrypto_amd as D
+    n = 5
+    e = D.Engine(0)
+    try:
+        with pytest.raises(D.DpkemError):
+            e.set_tuning(D.TUNE_FRODO_THREADS, 256)
+        e.set_tuning(D.TUNE_FRODO_THREADS, 1024)
+        kc = oracle.prng_bytes(8800 + scheme, n * 64).reshape(n, 64)
+        mu = oracle.prng_bytes(8900 + scheme, n * 24).reshape(n, 24)
+        pk, sk = e.keypair(scheme, kc)
+        opk, osk = oracle.keypair(scheme, kc, threads=4)
+        assert np.array_equal(pk, opk) and np.array_equal(sk, osk)
+        ct, ss = e.enc(scheme, pk, mu)
+        oct_, oss = oracle.enc(scheme, opk, mu, threads=4)
+        assert np.array_equal(ct, oct_) and np.array_equal(ss, oss)
+        bad = oct_.copy()
+        bad[2, 100] ^= 1
+        assert np.array_equal(e.dec(scheme, bad, osk), oracle.dec(scheme, bad, osk, threads=4))
+        assert np.array_equal(e.dec(scheme, oct_, osk), oss)
+    finally:
+        e.close()

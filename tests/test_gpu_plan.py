@@ -172,10 +172,12 @@ def test_mixed_independent_plan(eng, oracle, torch_cuda):
 FRODO640, FRODO1344 = 5, 6
 
 
-@pytest.mark.parametrize("scheme,sizes,nstreams", [
-    (FRODO, (1, 31, 33, 100, 2), 2), (FRODO, (40, 7), 1), (FRODO_AES, (3, 9), 2), (FRODO640, (5, 33), 1),
-    (FRODO1344, (2, 7), 1)])
-def test_independent_frodo_tasks_merge_into_one_launch_sequence(eng, oracle, torch_cuda, scheme, sizes, nstreams):
+@pytest.mark.parametrize("scheme,sizes,nstreams,persist,threads", [
+    (FRODO, (1, 31, 33, 100, 2), 2, 1, 512), (FRODO, (40, 7), 1, 0, 512), (FRODO, (9, 70), 2, 1, 1024),
+    (FRODO_AES, (3, 9), 2, 1, 512), (FRODO_AES, (4, 2), 1, 1, 1024), (FRODO640, (5, 33), 1, 1, 0),
+    (FRODO1344, (2, 7), 1, 1, 0)])
+def test_independent_frodo_tasks_merge_into_one_launch_sequence(eng, oracle, torch_cuda, scheme, sizes, nstreams,
+                                                                 persist, threads):
     """Independent Frodo tasks of one op (uneven sizes, separate arrays) run
     as one multi-task launch sequence with a persistent fused kernel; every
     instance of every task equals the oracle on each of two replays, and the
@@ -183,7 +185,9 @@ def test_independent_frodo_tasks_merge_into_one_launch_sequence(eng, oracle, tor
     import dpcrypto_amd as D
     torch = torch_cuda
     s = D.SIZES[scheme]
-    print(f"merged Frodo tasks: scheme {scheme}, sizes {sizes}", flush=True)
+    print(f"merged Frodo tasks: scheme {scheme}, sizes {sizes}, persistent {persist}, threads {threads}", flush=True)
+    eng.set_tuning(D.TUNE_FRODO_PERSIST, persist)
+    eng.set_tuning(D.TUNE_FRODO_THREADS, threads)
     kcs = [oracle.prng_bytes(7100 + 10 * scheme + k, n * s["kcoins"]).reshape(n, -1) for k, n in enumerate(sizes)]
     ecs = [oracle.prng_bytes(7200 + 10 * scheme + k, n * s["ecoins"]).reshape(n, -1) for k, n in enumerate(sizes)]
     ref = [oracle.keypair(scheme, kc, threads=8) for kc in kcs]
@@ -237,3 +241,5 @@ def test_independent_frodo_tasks_merge_into_one_launch_sequence(eng, oracle, tor
                 eng.sync()
         finally:
             p.close()
+    eng.set_tuning(D.TUNE_FRODO_PERSIST, 1)
+    eng.set_tuning(D.TUNE_FRODO_THREADS, 0)
