@@ -136,67 +136,70 @@ KERNEL_WORK = {
 
 # HBM bytes (read, write) per launch at the profiled batch sizes (Saber family
 # 4096, Frodo 8192 instances), from two rocprofv3 --pmc passes over
-#   bench.py --schedule 1x1 --calibrate-copy --skip-mixed --skip-serving --cpu-seconds 0 --frodo-steps 2 --plain-steps 1
+#   bench.py --schedule 1x1 --calibrate-copy --skip-mixed --skip-serving --skip-per-op --cpu-seconds 0
+#            --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99969, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129246971, 256245774),
-    ('frodo_as_fused,1344', 8192): (177471539, 353364147),
-    ('frodo_as_fused,640', 8192): (85211658, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129305196, 256247376),
-    ('frodo_dec_mu', 8192): (241957658, 262144),
-    ('frodo_dec_mu,1344', 8192): (358589132, 262144),
-    ('frodo_dec_mu,640', 8192): (165536535, 262144),
-    ('frodo_encdec_pre<false>', 8192): (74999400, 128712781),
-    ('frodo_encdec_pre<true>', 8192): (1835501, 128712709),
-    ('frodo_keygen_post', 8192): (129170493, 262144),
-    ('frodo_keygen_pre', 8192): (1399965, 256983020),
-    ('frodo_keygen_pre,1344', 8192): (1517862, 353244614),
-    ('frodo_keygen_pre,640', 8192): (1296988, 168558649),
-    ('frodo_pipe_ssd', 8192): (4406739, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4531651, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5524805, 131072),
-    ('frodo_pipe_x1', 8192): (130392454, 128975015),
-    ('frodo_pipe_x1,1344', 8192): (178636755, 177340416),
-    ('frodo_pipe_x1,640', 8192): (81037311, 84934656),
-    ('frodo_pipe_x2', 8192): (131951752, 130667530),
-    ('frodo_pipe_x2,1344', 8192): (180284162, 178866707),
-    ('frodo_pipe_x2,640', 8192): (83622150, 86442009),
-    ('frodo_sa_fused<false,aes>', 8192): (259469324, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260451938, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356739152, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166973332, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388307028, 262144),
-    ('frodo_sa_fused<true>', 8192): (389415510, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533926208, 265523),
-    ('frodo_sa_fused<true>,640', 8192): (247575064, 262201),
-    ('frodo_ss', 8192): (130026764, 196608),
-    ('sab_dec_hash1', 4096): (1561519, 18877325),
-    ('sab_dec_inner', 4096): (9888743, 131072),
-    ('sab_enc_hash1', 4096): (5316983, 19051834),
-    ('sab_hash2', 4096): (5234026, 131072),
-    ('sab_keygen_hash1', 4096): (1289712, 19004992),
-    ('sab_keygen_hash2', 4096): (4251110, 131072),
-    ('sab_pipe_x0', 4096): (1307389, 49616243),
-    ('sab_pipe_x0,l2', 4096): (1297722, 23513640),
-    ('sab_pipe_x0,l4', 4096): (1305150, 85363505),
-    ('sab_pipe_x1', 4096): (36096942, 34340864),
-    ('sab_pipe_x1,l2', 4096): (17790775, 16777216),
-    ('sab_pipe_x1,l4', 4096): (62042554, 58195968),
-    ('sab_pipe_x2', 4096): (12494293, 3801088),
-    ('sab_pipe_x2,l2', 4096): (9377631, 3276800),
-    ('sab_pipe_x2,l4', 4096): (15820033, 3801088),
-    ('sab_poly<kSabEnc>', 4096): (23843366, 4456448),
-    ('sab_poly<kSabEnc>,l2', 4096): (13076875, 3014656),
-    ('sab_poly<kSabEnc>,l4', 4096): (36672064, 6029312),
-    ('sab_poly<kSabKeygen>', 4096): (18985725, 12978908),
-    ('sab_poly<kSabKeygen>,l2', 4096): (9523096, 8650816),
-    ('sab_poly<kSabKeygen>,l4', 4096): (30516615, 17332160),
-    ('sab_poly<kSabReenc>', 4096): (28973813, 131072),
-    ('sab_reenc_k', 4096): (28415734, 147456),
-    ('sab_reenc_k,l2', 4096): (16597966, 147456),
-    ('sab_reenc_k,l4', 4096): (48458933, 5128192),
+    ('frodo_as_fused', 8192): (129244331, 256248512),
+    ('frodo_as_fused,1344', 8192): (177474018, 353379321),
+    ('frodo_as_fused,640', 8192): (85209357, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129265142, 256253656),
+    ('frodo_dec_mu', 8192): (241918930, 262144),
+    ('frodo_dec_mu,1344', 8192): (358449377, 262144),
+    ('frodo_dec_mu,640', 8192): (165543326, 262144),
+    ('frodo_dec_pre', 8192): (130905019, 130400890),
+    ('frodo_encdec_pre<false>', 8192): (75005003, 128712795),
+    ('frodo_keygen_post', 8192): (129159878, 262144),
+    ('frodo_keygen_pre', 8192): (1397211, 256979842),
+    ('frodo_keygen_pre,1344', 8192): (1516030, 353240076),
+    ('frodo_keygen_pre,640', 8192): (1295699, 168559020),
+    ('frodo_pipe_ssd', 8192): (4266969, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4531530, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5524658, 131072),
+    ('frodo_pipe_x1', 8192): (130398899, 128975182),
+    ('frodo_pipe_x1,1344', 8192): (178634033, 177340416),
+    ('frodo_pipe_x1,640', 8192): (81027495, 84934656),
+    ('frodo_pipe_x2', 8192): (131948301, 130675050),
+    ('frodo_pipe_x2,1344', 8192): (180283982, 178872460),
+    ('frodo_pipe_x2,640', 8192): (83619891, 86442028),
+    ('frodo_sa_fused<false,aes>', 8192): (259422179, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260406491, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356721667, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166973098, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388252380, 262144),
+    ('frodo_sa_fused<true>', 8192): (389362240, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533915281, 265248),
+    ('frodo_sa_fused<true>,640', 8192): (247571551, 262483),
+    ('frodo_sa_grouped<false>', 8192): (129749057, 128974848),
+    ('frodo_sa_grouped<true>', 8192): (258615775, 65536),
+    ('frodo_ss', 8192): (129685987, 196608),
+    ('sab_dec_hash1', 4096): (1561486, 18876164),
+    ('sab_dec_inner', 4096): (9888394, 131072),
+    ('sab_enc_hash1', 4096): (5316841, 19049842),
+    ('sab_hash2', 4096): (5221600, 131072),
+    ('sab_keygen_hash1', 4096): (1278363, 19004650),
+    ('sab_keygen_hash2', 4096): (4242806, 131072),
+    ('sab_pipe_x0', 4096): (1310813, 49632320),
+    ('sab_pipe_x0,l2', 4096): (1296763, 23484724),
+    ('sab_pipe_x0,l4', 4096): (1305485, 85365816),
+    ('sab_pipe_x1', 4096): (36110260, 34340864),
+    ('sab_pipe_x1,l2', 4096): (17792429, 16777216),
+    ('sab_pipe_x1,l4', 4096): (62096374, 58195968),
+    ('sab_pipe_x2', 4096): (12491911, 3801088),
+    ('sab_pipe_x2,l2', 4096): (9380452, 3276800),
+    ('sab_pipe_x2,l4', 4096): (15823649, 3801088),
+    ('sab_poly<kSabEnc>', 4096): (23852916, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13076526, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (36667202, 6029312),
+    ('sab_poly<kSabKeygen>', 4096): (18985218, 12979060),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9522842, 8650876),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30515805, 17332810),
+    ('sab_poly<kSabReenc>', 4096): (28974377, 131072),
+    ('sab_reenc_k', 4096): (28411462, 147456),
+    ('sab_reenc_k,l2', 4096): (16597523, 147456),
+    ('sab_reenc_k,l4', 4096): (48449449, 5128192),
 }
 
 
@@ -207,7 +210,7 @@ KERNEL_SOURCES = {
     "saber": ("saber.hip", "saber_poly.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
     "frodo": ("frodo.hip", "frodo_sampler.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
 }
-MEASURED_HBM_SOURCE_SHA = {"saber": None, "frodo": None}
+MEASURED_HBM_SOURCE_SHA = {"saber": "e5ac61dfc0de3d99", "frodo": "d01d202a851ecbb7"}
 
 
 def kernel_family(kernel: str) -> str:
@@ -857,6 +860,17 @@ def main():
     if pe is not None:
         pe.close()
     eng.close()
+    # leave no process behind: children of this run (serving processes,
+    # compiler drivers) must be gone before the line is printed
+    try:
+        import psutil
+        left = psutil.Process().children(recursive=True)
+        for c in left:
+            c.terminate()
+        psutil.wait_procs(left, timeout=5)
+        result["child_processes_left"] = len(left)
+    except ImportError:
+        pass
     print(json.dumps(result), flush=True)
 
 
