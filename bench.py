@@ -50,8 +50,11 @@ KECCAK_LANE_OPS = 4176            # int32 VALU lane-ops per Keccak-f[1600] (gfx9
 HBM_PEAK = 8.0e12
 
 # Algorithmic work per instance for each kernel: (Keccak permutations, MACs).
-# MACs run 2 per VALU lane-op (v_dot2) for Saber; Frodo's big products run on
-# MFMA and are not counted against the VALU roofline.
+# Only the MACs of the kernels in VALU_MACS_PER_OP run on the VALU (2 per
+# v_dot2 lane-op in sab_dec_inner, 1 per multiply-add in frodo_dec_mu) and
+# count against the VALU roofline; every other product (the Saber products
+# and the chain's inner products, Frodo's A S and S' A) runs on int8 MFMA and
+# is not counted.
 KERNEL_WORK = {
     "sab_keygen_hash1": (1 + 23 + 5, 0),
     "sab_poly<kSabKeygen>": (0, 9 * 65536),
@@ -132,6 +135,9 @@ KERNEL_WORK = {
     "frodo_mt_dec_pre": (1 + 231 + 115, 0),
     "frodo_mt_post<dec>": (1, 0),
 }
+
+
+VALU_MACS_PER_OP = {"sab_dec_inner": 2.0, "frodo_dec_mu": 1.0}
 
 
 # HBM bytes (read, write) per launch at the profiled batch sizes (Saber family
@@ -237,7 +243,8 @@ def measured_hbm(kernel: str, n: int):
 
 def lane_ops(kernel: str, n: int) -> float:
     perms, macs = KERNEL_WORK.get(kernel, (0, 0))
-    return n * (perms * KECCAK_LANE_OPS + macs / 2.0)
+    per_op = VALU_MACS_PER_OP.get(kernel.split(",")[0])
+    return n * (perms * KECCAK_LANE_OPS + (macs / per_op if per_op else 0.0))
 
 
 def pct(vals, q):
