@@ -548,9 +548,25 @@ CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10,
 LANE_PAIR_PERM_FLOOR_S = 24 * 108 * 4 / 2.4e9
 
 
-def roofline_from(stats, n):
+def measure_perm_latency(eng, short=64, long=576):
+    """Measured latency of one lane-pair permutation on a lone wave (s): two
+    launches of the engine's probe kernel (one wave, back-to-back
+    permutations in the chains' unrolled form), their difference over the
+    extra permutations, so the launch overhead cancels.  Median of 3."""
+    import statistics
+    eng.probe_perm_latency(short)  # warm-up
+    vals = []
+    for _ in range(3):
+        t_short = eng.probe_perm_latency(short)
+        t_long = eng.probe_perm_latency(long)
+        vals.append((t_long - t_short) / 1e3 / (long - short))
+    return statistics.median(vals)
+
+
+def roofline_from(stats, n, perm_latency_s=None):
     """Dominant kernel by total time -> achieved VALU lane-ops/s vs peak (and,
-    for a serial-chain launch, its time against the chain's issue floor)."""
+    for a serial-chain launch, its time against the chain's issue floor:
+    nominal, and from the measured lone-wave permutation latency)."""
     if not stats:
         return None
     name, (count, total_ms) = max(stats.items(), key=lambda kv: kv[1][1])
@@ -564,6 +580,13 @@ def roofline_from(stats, n):
         "floor_ms": round(chain * LANE_PAIR_PERM_FLOOR_S * 1e3, 4),
         "frac": round(chain * LANE_PAIR_PERM_FLOOR_S / avg_s, 4),
         "what": "serial lane-pair permutations x 24 rounds x 108 VALU ops x 4 clocks at 2.4 GHz"}
+    if floor is not None and perm_latency_s:
+        floor.update({
+            "measured_perm_us": round(perm_latency_s * 1e6, 3),
+            "measured_floor_ms": round(chain * perm_latency_s * 1e3, 4),
+            "measured_frac": round(chain * perm_latency_s / avg_s, 4),
+            "measured_what": "critical permutations x the lone-wave permutation latency measured in this run "
+                             "(dpkem_probe_perm_latency)"})
     return {
         "kernel": name,
         "bound": "valu",
@@ -703,6 +726,8 @@ def main():
     sab_prof = profile_plan(torch, eng, D.SABER, args.saber_batch, args.steps, 0x44504B454D000001)
     progress("saber eager pass (op by op) with per-kernel HIP events")
     sab_eager = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, 1, 0x44504B454D000001, True)
+    perm_s = measure_perm_latency(eng)
+    progress(f"lone-wave lane-pair permutation latency: {perm_s * 1e6:.3f} us")
     result = {
         "metric": "KEM ops/s (keygen+enc+dec) Frodo-976 & Saber at 1 GPU; p99 latency",
         "value": round(sab["ops_per_s"], 1),
@@ -723,7 +748,7 @@ def main():
         "roundtrip_ok": sab["roundtrip_ok"] and sab_prof["roundtrip_ok"] and sab_eager["roundtrip_ok"],
         "schedule": sab["schedule"],
         "eager_ops_per_s": round(sab_eager["ops_per_s"], 1),
-        "roofline": roofline_from(sab_prof["kernels"], args.saber_batch),
+        "roofline": roofline_from(sab_prof["kernels"], args.saber_batch, perm_s),
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_prof["kernels"].items()},
         "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
     }
@@ -807,7 +832,7 @@ def main():
                         "ms_per_step": round(fam["wall_s"] * 1e3 / args.steps, 4),
                         "roundtrip_ok": fam["roundtrip_ok"] and fam_prof["roundtrip_ok"],
                         "schedule": {k: v for k, v in fam["schedule"].items() if k != "tuning_ms"},
-                        "roofline": roofline_from(fam_prof["kernels"], args.saber_batch),
+                        "roofline": roofline_from(fam_prof["kernels"], args.saber_batch, perm_s),
                         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fam_prof["kernels"].items()}}
     progress("saber with the chain run as three dependent tasks (no stage pipelining)")
     unpiped = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,

@@ -478,6 +478,30 @@ int dpkem_engine_set_profiling(dpkem_engine* e, int on) {
   return DPKEM_OK;
 }
 
+int dpkem_probe_perm_latency(dpkem_engine* e, int perms, float* ms) {
+  if (!e || !ms || perms < 1 || perms > 100000) return DPKEM_EINVAL;
+  std::lock_guard<std::mutex> lock(e->mu);
+  HIP_TRY(e, hipSetDevice(e->device));
+  uint32_t* out = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipError_t err = hipMalloc(&out, 64 * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipEventCreate(&t0);
+  if (err == hipSuccess) err = hipEventCreate(&t1);
+  if (err == hipSuccess) err = hipEventRecord(t0, e->stream);
+  if (err == hipSuccess) {
+    saber_probe_perm_launch(e->stream, perms, out);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipEventRecord(t1, e->stream);
+  if (err == hipSuccess) err = hipEventSynchronize(t1);
+  if (err == hipSuccess) err = hipEventElapsedTime(ms, t0, t1);
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  if (out) (void)hipFree(out);
+  if (err != hipSuccess) return record_hip_error(e, err, "dpkem_probe_perm_latency");
+  return DPKEM_OK;
+}
+
 int dpkem_engine_kernel_stats(dpkem_engine* e, char* buf, size_t len, int reset) {
   if (!e || !buf || len == 0) return DPKEM_EINVAL;
   std::lock_guard<std::mutex> lock(e->mu);

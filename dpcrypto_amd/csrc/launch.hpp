@@ -23,6 +23,9 @@ void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
 // on the current device: how many gave up since the last clear, and their
 // spin bound.
 hipError_t saber_wait_expired(uint32_t* count, bool clear);
+// One wave of `perms` back-to-back lane-pair permutations (latency probe);
+// out: 64 words.
+void saber_probe_perm_launch(hipStream_t st, int perms, uint32_t* out);
 hipError_t saber_set_wait_spin_limit(uint32_t limit);
 
 // dst[i] = src for n rows of `bytes` bytes (a multiple of 4).

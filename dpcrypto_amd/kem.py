@@ -108,6 +108,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_server_stats.argtypes = [vp, ctypes.c_char_p, sz]
         lib.dpkem_server_destroy.argtypes = [vp]
         lib.dpkem_engine_kernel_stats.argtypes = [vp, ctypes.c_char_p, sz, i]
+        lib.dpkem_probe_perm_latency.argtypes = [vp, i, ctypes.POINTER(ctypes.c_float)]
         lib.dpkem_enc_shared_key.argtypes = [vp, i, sz, vp, vp, vp, vp, i]
         lib.dpkem_dec_shared_key.argtypes = [vp, i, sz, vp, vp, vp, i]
         lib.dpkem_aes256_ecb.argtypes = [vp, vp, vp, sz]
@@ -230,6 +231,13 @@ class Engine:
 
     def set_profiling(self, on: bool):
         self._check(self.lib.dpkem_engine_set_profiling(self.h, int(on)), "set_profiling")
+
+    def probe_perm_latency(self, perms: int) -> float:
+        """ms of one launch of `perms` back-to-back lane-pair permutations on
+        one wave (the serial chains' latency floor, measured)."""
+        ms = ctypes.c_float(0.0)
+        self._check(self.lib.dpkem_probe_perm_latency(self.h, int(perms), ctypes.byref(ms)), "probe_perm_latency")
+        return ms.value
 
     def kernel_stats(self, reset: bool = True) -> dict:
         """{kernel: (launches, total_ms)} from HIP events on the engine's stream."""

@@ -117,6 +117,14 @@ const char *dpkem_engine_last_error(dpkem_engine *e);
  * at one workgroup per instance). */
 int dpkem_engine_set_profiling(dpkem_engine *e, int on);
 int dpkem_engine_kernel_stats(dpkem_engine *e, char *buf, size_t len, int reset);
+/* Latency bound of the serial hash chains, measured: one workgroup of one
+ * wave runs `perms` back-to-back lane-pair Keccak-f[1600] permutations (the
+ * form the chain kernels squeeze with, 32 states in lane pairs) on the
+ * engine's stream; *ms receives the launch's HIP-event time.  Two calls with
+ * different perms give the per-permutation latency of a lone wave, the floor
+ * of a chain launch's critical path.  No counterpart in the reference; used
+ * by the benchmark's roofline report.  perms in [1, 100000]. */
+int dpkem_probe_perm_latency(dpkem_engine *e, int perms, float *ms);
 
 /* ---------------------------------------------------------------- batched KEM
  * Explicit-coins entry points over arrays of n instances.  `mem` says where

@@ -212,6 +212,10 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
   memmove(d, s, n);
   return hipSuccess;
 }
+inline hipError_t hipEventSynchronize(hipEvent_t) {
+  if (fakehip::st().origin) fakehip::fail("event synchronized during a capture");
+  return hipSuccess;
+}
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   *ms = 0.f;
   return hipSuccess;

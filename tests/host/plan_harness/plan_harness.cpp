@@ -280,6 +280,10 @@ hipError_t saber_wait_expired(uint32_t* count, bool) {
   return hipSuccess;
 }
 hipError_t saber_set_wait_spin_limit(uint32_t) { return hipSuccess; }
+void saber_probe_perm_launch(hipStream_t st, int, uint32_t* out) {
+  launch_on(st);
+  wr(out, 64 * sizeof(uint32_t));
+}
 
 }  // namespace dpkem
 

@@ -258,3 +258,20 @@ def test_plain_form_matches_oracle(plain_eng, oracle, scheme, n):
     _eq(plain_eng.dec(scheme, oct_, osk), oss, "dec ss")
     bad = oct_.copy(); bad[:, 7] ^= 2
     _eq(plain_eng.dec(scheme, bad, osk), oracle.dec(scheme, bad, osk, threads=4), "rejected ss")
+
+
+def test_perm_latency_probe(eng):
+    """dpkem_probe_perm_latency (the serial chains' measured latency floor in
+    the bench's roofline): a longer run of back-to-back permutations takes
+    longer, the per-permutation latency is microseconds, and out-of-range
+    counts are rejected."""
+    import dpcrypto_amd as D
+    print("probe: 64 and 576 permutations on one wave", flush=True)
+    t_short = eng.probe_perm_latency(64)
+    t_long = eng.probe_perm_latency(576)
+    assert 0.0 < t_short < t_long
+    per_us = (t_long - t_short) * 1e3 / 512
+    assert 1.0 < per_us < 50.0, per_us
+    for bad in (0, 100001):
+        with pytest.raises(D.DpkemError):
+            eng.probe_perm_latency(bad)
