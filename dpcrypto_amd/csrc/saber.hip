@@ -1175,25 +1175,6 @@ hipError_t saber_wait_expired(uint32_t* count, bool clear) {
   return err;
 }
 
-// Latency probe: one wave, 32 lane-pair states, `perms` back-to-back
-// permutations in the fully unrolled form the chain kernels squeeze with; the
-// states are written out so nothing is folded away.
-__global__ void __launch_bounds__(64) sab_probe_perm(int perms, uint32_t* __restrict__ out) {
-  const uint32_t p = threadIdx.x & 1;
-  uint32_t a[25];
-#pragma unroll
-  for (int i = 0; i < 25; ++i) a[i] = (uint32_t)threadIdx.x * 0x9E3779B9u + (uint32_t)i;
-  for (int r = 0; r < perms; ++r) k2::keccakf2_unrolled(a, p);
-  uint32_t x = 0;
-#pragma unroll
-  for (int i = 0; i < 25; ++i) x ^= a[i];
-  out[threadIdx.x] = x;
-}
-
-void saber_probe_perm_launch(hipStream_t st, int perms, uint32_t* out) {
-  hipLaunchKernelGGL(sab_probe_perm, dim3(1), dim3(64), 0, st, perms, out);
-}
-
 hipError_t saber_set_wait_spin_limit(uint32_t limit) {
   return hipMemcpyToSymbol(HIP_SYMBOL(k2::wait_spin_limit), &limit, sizeof limit, 0, hipMemcpyHostToDevice);
 }
