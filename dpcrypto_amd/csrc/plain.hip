@@ -280,24 +280,57 @@ __global__ void __launch_bounds__(1024) fr_gen_a(int n, const uint8_t* pk, size_
   }
 }
 
-// B = A S + E: thread per row (S^T given as NBAR x N)
+// B = A S + E (S^T given as NBAR x N).  A is read along its rows, each row
+// once, neighbouring lanes reading neighbouring entries: a wave takes 8
+// rows at a time, lane l accumulates the 64 sums (row r0 + rr, column k)
+// over entries 4 l .. 4 l + 3 (+ 256 m) of those rows, and a butterfly over
+// the wave then leaves lane L with sum L (each step sends the half of its
+// sums a lane does not keep).
 __global__ void __launch_bounds__(1024) fr_as(int n, const uint16_t* A, const uint16_t* st, size_t sts, uint16_t* B) {
-  __shared__ uint16_t Ls[kFrodoNbar * kFrodoN];
+  __shared__ __align__(8) uint16_t Ls[kFrodoNbar * kFrodoN];
+  static_assert(kFrodoN % 8 == 0 && kFrodoN % 4 == 0, "8-row groups, 4-entry reads");
   const int i = blockIdx.x, t = threadIdx.x;
   if (i >= n) return;
   const uint16_t* e = st + kFrodoNbar * kFrodoN;  // E follows S^T
   for (int c = t; c < kFrodoNbar * kFrodoN; c += 1024) Ls[c] = st[(size_t)i * sts + c];
   __syncthreads();
-  if (t >= kFrodoN) return;
-  const uint16_t* a = A + ((size_t)i * kFrodoN + t) * kFrodoN;
-  uint16_t acc[kFrodoNbar];
-  for (int k = 0; k < kFrodoNbar; ++k) acc[k] = e[(size_t)i * sts + t * kFrodoNbar + k];
-  for (int j = 0; j < kFrodoN; ++j) {
-    const uint16_t x = a[j];
+  const int wave = t >> 6, lane = t & 63;
+  const uint16_t* Ai = A + (size_t)i * kFrodoN * kFrodoN;
+  for (int r0 = 8 * wave; r0 < kFrodoN; r0 += 8 * 16) {
+    uint32_t acc[64];
 #pragma unroll
-    for (int k = 0; k < kFrodoNbar; ++k) acc[k] = (uint16_t)(acc[k] + x * Ls[k * kFrodoN + j]);
+    for (int q = 0; q < 64; ++q) acc[q] = 0;
+    for (int j = 4 * lane; j < kFrodoN; j += 256) {
+      uint32_t s[kFrodoNbar][4];
+#pragma unroll
+      for (int k = 0; k < kFrodoNbar; ++k) {
+        const uint2 w = *reinterpret_cast<const uint2*>(Ls + k * kFrodoN + j);
+        s[k][0] = w.x & 0xFFFF, s[k][1] = w.x >> 16, s[k][2] = w.y & 0xFFFF, s[k][3] = w.y >> 16;
+      }
+#pragma unroll
+      for (int rr = 0; rr < 8; ++rr) {
+        const uint2 w = *reinterpret_cast<const uint2*>(Ai + (size_t)(r0 + rr) * kFrodoN + j);
+        const uint32_t x[4] = {w.x & 0xFFFF, w.x >> 16, w.y & 0xFFFF, w.y >> 16};
+#pragma unroll
+        for (int k = 0; k < kFrodoNbar; ++k)
+          acc[8 * rr + k] += x[0] * s[k][0] + x[1] * s[k][1] + x[2] * s[k][2] + x[3] * s[k][3];
+      }
+    }
+#pragma unroll
+    for (int m = 64; m > 1; m >>= 1) {
+      const int d = m >> 1;
+      const bool up = (lane & d) != 0;
+#pragma unroll
+      for (int q = 0; q < d; ++q) {
+        const uint32_t keep = up ? acc[q + d] : acc[q];
+        const uint32_t send = up ? acc[q] : acc[q + d];
+        acc[q] = keep + (uint32_t)__shfl_xor((int)send, d, 64);
+      }
+    }
+    const int row = r0 + (lane >> 3), k = lane & 7;
+    B[(size_t)i * kFrodoNbar * kFrodoN + row * kFrodoNbar + k] =
+        (uint16_t)(acc[0] + e[(size_t)i * sts + row * kFrodoNbar + k]);
   }
-  for (int k = 0; k < kFrodoNbar; ++k) B[(size_t)i * kFrodoNbar * kFrodoN + t * kFrodoNbar + k] = acc[k];
 }
 
 // B' = S' A + E': thread per column
