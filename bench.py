@@ -80,16 +80,16 @@ KERNEL_WORK = {
     # headline replays): stages of different ops sharing a launch
     "sab_pipe_x0": (1 + 23 + 5 + 23 + 23, 0),          # keygen seedA, GenMatrix, GenSecret; encaps' / decaps' GenMatrix
     "sab_pipe_x1": (8 + 1 + 1 + 5 + 8, 0),             # encaps' H(pk), m, G, GenSecret; keygen's H(pk)
-    "sab_pipe_x2": (10 + 9 + 1 + 5, 3 * 65536),        # encaps' K; decaps' H(ct), G, GenSecret; decaps' inner product
-    "sab_reenc_k": (1, 12 * 65536),                      # re-encryption product, then decaps' K
+    "sab_pipe_x2": (10 + 9 + 1 + 5 + 2, 3 * 65536),    # encaps' K; decaps' H(ct), G, GenSecret, both K; inner product
+    "sab_reenc": (0, 12 * 65536),                        # re-encryption product, compare, select of decaps' K
     "sab_pipe_x0,l2": (1 + 10 + 4 + 10 + 10, 0),
     "sab_pipe_x1,l2": (5 + 1 + 1 + 4 + 5, 0),
-    "sab_pipe_x2,l2": (7 + 6 + 1 + 4, 2 * 65536),
-    "sab_reenc_k,l2": (1, 6 * 65536),
+    "sab_pipe_x2,l2": (7 + 6 + 1 + 4 + 2, 2 * 65536),
+    "sab_reenc,l2": (0, 6 * 65536),
     "sab_pipe_x0,l4": (1 + 40 + 5 + 40 + 40, 0),
     "sab_pipe_x1,l4": (10 + 1 + 1 + 5 + 10, 0),
-    "sab_pipe_x2,l4": (12 + 11 + 1 + 5, 4 * 65536),
-    "sab_reenc_k,l4": (1, 20 * 65536),
+    "sab_pipe_x2,l4": (12 + 11 + 1 + 5 + 2, 4 * 65536),
+    "sab_reenc,l4": (0, 20 * 65536),
     "sab_poly<kSabKeygen>,l2": (0, 4 * 65536),
     "sab_poly<kSabEnc>,l2": (0, 6 * 65536),
     "sab_poly<kSabReenc>,l2": (0, 6 * 65536),

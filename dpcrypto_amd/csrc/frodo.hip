@@ -450,12 +450,13 @@ __device__ __forceinline__ void ss_absorb(uint32_t a[25], uint32_t p, const uint
 template <class P>
 __device__ __forceinline__ void ss_finish(uint32_t a[25], uint32_t p, int inst, const uint8_t* __restrict__ c,
                                           const uint8_t* __restrict__ kbuf, const uint8_t* __restrict__ sk,
-                                          size_t sk_stride, const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+                                          size_t sk_stride, const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss,
+                                          int force_reject = 0) {
   using Q = FrodoSsShape<P>;
 #pragma unroll
   for (int i = 0; i < Q::REM; ++i) a[i] ^= k2::deint(ld64(c + 8 * (Q::NB * Q::RW + i)), p);
   uint64_t mask = 0;
-  if (fail) mask = 0 - (uint64_t)(fail[inst] != 0);
+  if (fail) mask = 0 - (uint64_t)(fail[inst] != 0 || force_reject != 0);
 #pragma unroll
   for (int i = 0; i < Q::LS; ++i) {
     const uint64_t k = ld64(kbuf + (size_t)inst * 64 + 8 * i);
@@ -1226,21 +1227,25 @@ template <class P>
 __device__ __forceinline__ void ssd_body(int vb, int n, const uint8_t* __restrict__ ct,
                                          const uint32_t* __restrict__ state, const uint8_t* __restrict__ kbuf_d,
                                          const uint8_t* __restrict__ sk, size_t sk_stride,
-                                         const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+                                         const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss,
+                                         int force_reject) {
   FR_PAIR_SETUP
   uint32_t a[25];
 #pragma unroll
   for (int i = 0; i < 25; ++i) a[i] = state[(size_t)i * 2 * n + 2 * inst + p];
-  ss_finish<P>(a, p, inst, ct + (size_t)inst * P::ct, kbuf_d, sk, sk_stride, fail, ss);
+  ss_finish<P>(a, p, inst, ct + (size_t)inst * P::ct, kbuf_d, sk, sk_stride, fail, ss, force_reject);
 }
 
+// force_reject: test facility, every instance takes the implicit-rejection
+// branch (ss from s instead of k).
 template <class P>
 __global__ void __launch_bounds__(64) frodo_pipe_ssd(int n, const uint8_t* __restrict__ ct,
                                                      const uint32_t* __restrict__ state, const uint8_t* __restrict__ kbuf_d,
                                                      const uint8_t* __restrict__ sk, size_t sk_stride,
-                                                     const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
+                                                     const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss,
+                                                     int force_reject) {
   chain_priority();
-  ssd_body<P>(blockIdx.x, n, ct, state, kbuf_d, sk, sk_stride, fail, ss);
+  ssd_body<P>(blockIdx.x, n, ct, state, kbuf_d, sk, sk_stride, fail, ss, force_reject);
 }
 
 // decaps' chains before the re-encryption, one 128-thread block per 32
@@ -1417,7 +1422,7 @@ static void dec_t(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, u
                (size_t)P::sk, mubuf, kbuf, s8, e8, epp8, state);
   launch_sa_fused<P, true, AES>(st, M, false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, state, kbuf,
-               sk, (size_t)P::sk, fail, ss);
+               sk, (size_t)P::sk, fail, ss, 0);
 }
 
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
@@ -1459,7 +1464,7 @@ size_t frodo_pipe_scratch_per_inst(int set) { return 16 * (size_t)set + 3 * 64 +
 
 template <class P, bool AES>
 static void pipe_t(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk, uint8_t* ct,
-                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int threads) {
+                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int threads, int force_reject) {
   size_t off = 0;
   auto take = [&](size_t per) {
     uint8_t* q = scratch + off;
@@ -1488,16 +1493,17 @@ static void pipe_t(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
                e8, epp8, state);
   launch_sa_fused<P, true, AES>(st, Md, false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, g, dim3(64), st, n, ct, state, kbuf_d, sk, (size_t)P::sk,
-               fail, ss2);
+               fail, ss2, force_reject);
 }
 
 void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes, int threads) {
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes, int threads,
+                       int force_reject) {
   if (n <= 0) return;
-  if (set == 640) pipe_t<FrodoP<640>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
-  else if (set == 1344) pipe_t<FrodoP<1344>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
-  else if (aes) pipe_t<F976, true>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
-  else pipe_t<F976, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads);
+  if (set == 640) pipe_t<FrodoP<640>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads, force_reject);
+  else if (set == 1344) pipe_t<FrodoP<1344>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads, force_reject);
+  else if (aes) pipe_t<F976, true>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads, force_reject);
+  else pipe_t<F976, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads, force_reject);
 }
 
 // Encaps / decaps where instances share public keys in groups (see
@@ -1532,7 +1538,7 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
                ngroups, groups, n, sk + kFrodoSec, sk_stride, s8, reinterpret_cast<const int8_t*>(e8),
                reinterpret_cast<const int8_t*>(epp8), mubuf, (size_t)32, nullptr, ct, fail);
   DPKEM_LAUNCH("frodo_pipe_ssd", frodo_pipe_ssd<F976>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, state, kbuf, sk,
-               sk_stride, fail, ss);
+               sk_stride, fail, ss, 0);
 }
 
 // ----------------------------------------------------------------- multi-task launches
@@ -1570,7 +1576,7 @@ __global__ void __launch_bounds__(64) frodo_mt_post(const FrMt M) {
   const int vb = blockIdx.x - T.first_group;
   if constexpr (OP == 0) keygen_post_body<P>(vb, T.n, T.o0, T.o1);
   else if constexpr (OP == 1) ss_body<P>(vb, T.n, T.o0, T.kbuf, nullptr, 0, nullptr, T.o1);
-  else ssd_body<P>(vb, T.n, T.i0, T.state, T.kbuf, T.i1, (size_t)P::sk, T.fail, T.o0);
+  else ssd_body<P>(vb, T.n, T.i0, T.state, T.kbuf, T.i1, (size_t)P::sk, T.fail, T.o0, 0);
 }
 
 template <class P, bool AES>

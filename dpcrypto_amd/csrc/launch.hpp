@@ -40,8 +40,11 @@ constexpr size_t kSaberChainScratchPerInst = 3744 + 768 + 32 + 64 + 4;
 // (saber.hip): stages of different ops that read disjoint inputs share
 // launches, every op computes all of its own hashes.  Saber family, rank l.
 size_t saber_pipe_scratch_per_inst(int l);
+// force_reject (test facility): every decapsulation of the chain takes the
+// implicit-rejection branch.
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int ipw = 1);
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int ipw = 1,
+                       int force_reject = 0);
 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch
@@ -58,7 +61,8 @@ void frodo_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* c
 // (frodo.hip); set / aes as above.
 size_t frodo_pipe_scratch_per_inst(int set);
 void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes, int threads = 0);
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes, int threads = 0,
+                       int force_reject = 0);
 // Groups of up to kFrodoGroup instances sharing one public key expand A once;
 // groups[b] = (first, count) or nullptr for consecutive groups of
 // kFrodoGroup; key strides may be 0 (one key for all n).

@@ -293,10 +293,11 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
         saber_chain_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off, p->saber_ipw);
       else if (saber_rank_of(T.scheme))
         saber_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
-                          saber_rank_of(T.scheme), p->saber_ipw);
+                          saber_rank_of(T.scheme), p->saber_ipw, (p->flags & DPKEM_PLAN_FORCE_REJECT) != 0);
       else
         frodo_pipe_launch(st, (int)s.count, kc, ec, pk, sk, ct, ss, ss2, p->scratch + s.scratch_off,
-                          frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES, p->frodo_threads);
+                          frodo_set_of(T.scheme), T.scheme == DPKEM_FRODO976_AES, p->frodo_threads,
+                          (p->flags & DPKEM_PLAN_FORCE_REJECT) != 0);
       ok = ok && hipGetLastError() == hipSuccess;
       ok = ok && hipEventRecord(s.done, st) == hipSuccess;
       continue;
@@ -360,7 +361,8 @@ int dpkem_plan_add(dpkem_plan* p, int scheme, int op, size_t n, uint8_t* o0, uin
 
 int dpkem_plan_set_options(dpkem_plan* p, unsigned flags) {
   if (!p || p->built ||
-      (flags & ~(DPKEM_PLAN_SHARE_MATRIX | DPKEM_PLAN_NO_PIPELINE | DPKEM_PLAN_NO_GRAPH | DPKEM_PLAN_NO_MERGE)))
+      (flags & ~(DPKEM_PLAN_SHARE_MATRIX | DPKEM_PLAN_NO_PIPELINE | DPKEM_PLAN_NO_GRAPH | DPKEM_PLAN_NO_MERGE |
+                 DPKEM_PLAN_FORCE_REJECT)))
     return DPKEM_EINVAL;
   p->flags = flags;
   return DPKEM_OK;

@@ -180,9 +180,14 @@ __device__ __forceinline__ void load_hpk(int inst, const uint8_t* __restrict__ s
   for (int i = 0; i < 4; ++i) hpk[i] = ld64(sk + (size_t)inst * P::sk + P::sk - 64 + 8 * i);
 }
 
+// kq (optional): K-hat' halves also go to kq[i * ks + slot] in LDS, for a
+// wave of the same workgroup that hashes K; the lane of slot 0 then raises
+// the LDS count kq_ready.
 template <int L>
 __device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint64_t hpk[4], const uint8_t* m,
-                                              uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf) {
+                                              uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf,
+                                              uint32_t* kq = nullptr, int ks = 0, int slot = 0,
+                                              uint32_t* kq_ready = nullptr) {
   using P = SaberP<L>;
   uint32_t a[25];
   uint64_t buf[8];
@@ -195,6 +200,12 @@ __device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint64
   uint32_t kr[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) kr[i] = a[i];
+  if (kq) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) kq[i * ks + slot] = kr[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (slot == 0) __hip_atomic_fetch_add(kq_ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
   absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
   squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
@@ -276,38 +287,6 @@ __device__ __forceinline__ void hash2_body(int vb, int n, const uint8_t* __restr
       krbuf, sk, fail, ss);
 }
 
-// The two halves of K for the pipelined chain: H(ct) of the received
-// ciphertext (it does not depend on the re-encryption), kept as this lane's
-// interleaved halves (hc[8 inst + 2 w + p]), then K from it.
-template <int L, int G, class ABS>
-__device__ __forceinline__ void hct_core(int vb, int n, ABS&& absorb_ct, uint32_t* __restrict__ hc) {
-  SAB_PAIR_SETUP
-  uint32_t a[25];
-  absorb_ct(a, p, inst, lane);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) hc[(size_t)inst * 8 + 2 * i + p] = a[i];
-}
-
-template <int L, int G = 32>
-__device__ __forceinline__ void k_body(int vb, int n, const uint32_t* __restrict__ hc,
-                                       const uint8_t* __restrict__ krbuf, const uint8_t* __restrict__ sk,
-                                       const uint32_t* __restrict__ fail, uint8_t* __restrict__ ss) {
-  using P = SaberP<L>;
-  SAB_PAIR_SETUP
-  uint32_t a[25], m[8];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) m[4 + i] = hc[(size_t)inst * 8 + 2 * i + p];
-  const uint64_t mask = 0 - (uint64_t)(fail[inst] != 0);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint64_t k = ld64(krbuf + (size_t)inst * 64 + 8 * i);
-    const uint64_t z = ld64(sk + (size_t)inst * P::sk + P::sk - 32 + 8 * i);
-    m[i] = k2::deint((k & ~mask) | (z & mask), p);
-  }
-  absorb2_halves<kRateSha3_256, 8>(a, p, m, kDsSha3);
-  store2_words<4>(a, p, ss + (size_t)inst * kSaberSs);
-}
-
 // ----------------------------------------------------------------- hash kernels
 // hash1: blockDim = 128, wave 0 runs role 0 and wave 1 role 1 for the same 32
 // instances (role_sel >= 0: that role only, 64 threads).  hash2: 64 threads
@@ -360,9 +339,9 @@ __global__ void __launch_bounds__(64) sab_hash2(int n, const uint8_t* __restrict
 //       rewritten as plain words (plain_matrix_streams); wave 3
 //       deinterleaves pk into LDS ahead of wave 0 (stage2_message);
 //   x2  (after the encaps products, below sab_dec_inner) decaps' inner
-//       product and message chain beside encaps' K and decaps' H(ct),
-//       leaving one permutation (decaps' K) to the re-encryption launch
-//       (sab_reenc_k).
+//       product and message chain beside encaps' K and decaps' H(ct), and
+//       decaps' K for both outcomes of the re-encryption's compare, so the
+//       re-encryption launch only selects one (sab_poly<kSabReenc>).
 // Every op still computes all of its own hashes; only their order changes.
 template <int L>
 __global__ void __launch_bounds__(256) sab_pipe_x0(int n, const uint8_t* __restrict__ kc, uint8_t* __restrict__ pk,
@@ -504,7 +483,12 @@ struct SabPolyArgs {
   uint8_t* out;          // keygen: pk; enc: ct; reenc: unused
   uint8_t* sk;           // keygen: sk out
   const uint8_t* ct_in;  // reenc: received ct to compare with
-  uint32_t* fail;        // reenc: per-instance mismatch flag
+  uint32_t* fail;        // reenc: per-instance mismatch flag (or nullptr)
+  // reenc in the pipelined chain: K for both outcomes of the compare (64 B
+  // per instance, K then the implicit-rejection K), selected into ss
+  const uint8_t* kk;
+  uint8_t* ss;
+  int force_reject;      // test facility: treat every re-encryption as differing
 };
 
 constexpr int kARow = 264;  // u16 row stride of plain polynomials in the dot2 products (spreads LDS banks)
@@ -786,8 +770,14 @@ __device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L
 #pragma unroll
     for (int r = 0; r < CPT; ++r)
       if (tid + r * NT < CW) d |= cv[r] ^ ob[tid + r * NT];
-    const bool differ = __builtin_amdgcn_ballot_w64(d != 0) != 0;
-    if (tid == 0) a.fail[inst] = differ ? 1u : 0u;
+    const bool differ = __builtin_amdgcn_ballot_w64(d != 0) != 0 || a.force_reject != 0;
+    if (a.fail && tid == 0) a.fail[inst] = differ ? 1u : 0u;
+    if (a.kk && tid < kSaberSs / 4) {  // constant-time select of the precomputed K
+      const uint32_t mask = differ ? 0xFFFFFFFFu : 0u;
+      const uint32_t* kk = reinterpret_cast<const uint32_t*>(a.kk + (size_t)inst * 64);
+      reinterpret_cast<uint32_t*>(a.ss + (size_t)inst * kSaberSs)[tid] =
+          (kk[tid] & ~mask) | (kk[kSaberSs / 4 + tid] & mask);
+    }
   }
 }
 
@@ -801,24 +791,6 @@ __global__ void __launch_bounds__(64 * IPW) __attribute__((amdgpu_waves_per_eu(4
   const int inst = (int)blockIdx.x * IPW + w;
   if (inst >= a.n) return;
   sab_poly_body<MODE, L, A_PLAIN>(a, S[w], IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63), inst);
-}
-
-// The pipelined chain's re-encryption, 16 instances per workgroup (one wave
-// each, as sab_poly), then decaps' K for those 16 on wave 0 after a workgroup
-// barrier: the mismatch flags only need workgroup visibility, and the chain
-// needs no separate launch for K.  16 x 6-7 KB of LDS keeps one workgroup
-// (16 waves, 4 per SIMD) per CU.
-constexpr int kReencGroup = 16;
-template <int L>
-__global__ void __launch_bounds__(64 * kReencGroup) __attribute__((amdgpu_waves_per_eu(4)))
-sab_reenc_k(SabPolyArgs a, const uint32_t* __restrict__ hc, const uint8_t* __restrict__ krbuf,
-            const uint8_t* __restrict__ sk, uint8_t* __restrict__ ss) {
-  __shared__ __align__(16) SabPolyLds<L> S[kReencGroup];
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int inst = (int)blockIdx.x * kReencGroup + w;
-  if (inst < a.n) sab_poly_body<kSabReenc, L, true>(a, S[w], threadIdx.x & 63, inst);
-  __syncthreads();
-  if (w == 0) k_body<L, kReencGroup>(blockIdx.x, a.n, hc, krbuf, sk, a.fail, ss);
 }
 
 // Decaps inner product v = b' . s and message recovery
@@ -1014,12 +986,16 @@ __device__ __forceinline__ uint32_t dec_inner_mfma(int lane, const InnerRaw<L>& 
 // SIMD), so the roles are laid out by w mod 4 and the two long hash chains
 // each get a SIMD of their own from the start:
 //   w = 0          encaps' K = SHA3-256(kr[0:32] || H(ct));
-//   w = 1          decaps' H(ct) into hc_d (sab_reenc_k finishes K after
-//                  the re-encryption);
+//   w = 1          decaps' H(ct), handed to wave 3 through LDS;
 //   w = 3          deinterleaves both chains' copy of ct into LDS block by
 //                  block (stage2_message: lanes 0..31 for encaps' K, 32..63
 //                  for decaps' H(ct)), so the two chains absorb without
-//                  conversion work;
+//                  conversion work; then, once decaps' H(ct) (wave 1) and
+//                  K-hat' (the chain wave, after G) are posted, one
+//                  permutation on all 64 lanes for both of decaps' possible
+//                  K: SHA3-256(K-hat' || H(ct)) on lanes 0..31 and the
+//                  implicit-rejection SHA3-256(z || H(ct)) on lanes 32..63,
+//                  into kk_d (64 B per instance);
 //   w = 2, 6, 10, 7, 11  decaps' inner products on int8 MFMA, instances w',
 //                  w' + 5, ... (w' = 0 .. 4) one after the other (three waves
 //                  per SIMD hide the Toeplitz operand reads' latency); m'
@@ -1044,7 +1020,9 @@ struct X2Lds {
   InnerLds<L> w[kX2InnerWaves];
   uint32_t msg[2][kCtBlocks * kRateSha3_256 * 2 * kX2Group];  // ct halves for encaps' K / decaps' H(ct)
   alignas(8) uint8_t mq[kX2Group][32];
-  uint32_t posted, ready;
+  uint32_t hq[4][2 * kX2Group];  // decaps' H(ct) halves by lane-pair slot (wave 1 -> the K wave)
+  uint32_t kq[4][2 * kX2Group];  // decaps' K-hat' halves by slot (the chain wave -> the K wave)
+  uint32_t posted, ready, hc_ready, kq_ready;
 };
 
 template <int L>
@@ -1052,7 +1030,7 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
                                                           const uint8_t* __restrict__ ct, uint8_t* __restrict__ mbuf_d,
                                                           uint8_t* __restrict__ sraw_d, uint8_t* __restrict__ krbuf_d,
                                                           const uint8_t* __restrict__ krbuf_e, uint8_t* __restrict__ ss,
-                                                          uint32_t* __restrict__ hc_d) {
+                                                          uint8_t* __restrict__ kk_d) {
   using P = SaberP<L>;
   constexpr int CW = P::ct / 8, SL = 2 * kX2Group;
   __shared__ X2Lds<L> S;
@@ -1061,20 +1039,57 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
   if (threadIdx.x == 0) {
     S.posted = 0;
     S.ready = 0;
+    S.hc_ready = 0;
+    S.kq_ready = 0;
   }
   __syncthreads();  // the only workgroup barrier: every wave passes it before any work
   if ((wave & 3) < 2) {
     auto staged = [&](uint32_t* a, uint32_t p, int, int ln) {
       k2::absorb2_staged<kRateSha3_256, CW, SL>(a, p, S.msg[wave], &S.ready, ln, kDsSha3);
     };
-    if (wave == 0) hash2_core<L, kX2Group>(blockIdx.x, n, staged, krbuf_e, nullptr, nullptr, ss);
-    else if (wave == 1) hct_core<L, kX2Group>(blockIdx.x, n, staged, hc_d);
+    if (wave == 0) {
+      hash2_core<L, kX2Group>(blockIdx.x, n, staged, krbuf_e, nullptr, nullptr, ss);
+    } else if (wave == 1) {  // decaps' H(ct), handed to the K wave through LDS
+      const uint32_t p = lane & 1;
+      const int k = lane >> 1;
+      if (k < kX2Group && base + k < n) {
+        uint32_t a[25];
+        staged(a, p, base + k, lane);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) S.hq[i][lane] = a[i];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane == 0) __hip_atomic_fetch_add(&S.hc_ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
     return;
   }
   if (wave == kX2HelperWave) {
-    const int k = (lane & 31) >> 1, inst = min(base + k, n - 1);  // absent instances stage a copy (unused)
-    const uint64_t* w = reinterpret_cast<const uint64_t*>(ct + (size_t)inst * P::ct);
-    k2::stage2_message<kRateSha3_256, CW, SL>(S.msg[lane >> 5], &S.ready, lane & 1, lane & 31, w, lane == 0);
+    {
+      const int k = (lane & 31) >> 1, inst = min(base + k, n - 1);  // absent instances stage a copy (unused)
+      const uint64_t* w = reinterpret_cast<const uint64_t*>(ct + (size_t)inst * P::ct);
+      k2::stage2_message<kRateSha3_256, CW, SL>(S.msg[lane >> 5], &S.ready, lane & 1, lane & 31, w, lane == 0);
+    }
+    // then decaps' K for both outcomes of the re-encryption's compare, once
+    // H(ct) and K-hat' are posted: lanes 0..31 K = SHA3-256(K-hat' || H(ct)),
+    // lanes 32..63 the implicit-rejection K = SHA3-256(z || H(ct)); the
+    // re-encryption launch picks one per instance in constant time
+    const int s = lane & 31, k = s >> 1, inst = base + k;
+    const uint32_t p = lane & 1;
+    const bool rej = lane >= 32;
+    uint64_t z[4];
+    const uint8_t* zs = sk + (size_t)min(inst, n - 1) * P::sk + P::sk - 32;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) z[i] = ld64(zs + 8 * i);
+    k2::lds_wait_count(&S.hc_ready, 1u);
+    k2::lds_wait_count(&S.kq_ready, 1u);
+    uint32_t m[8], a[25];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      m[i] = rej ? k2::deint(z[i], p) : S.kq[i][s];
+      m[4 + i] = S.hq[i][s];
+    }
+    k2::absorb2_halves<kRateSha3_256, 8>(a, p, m, kDsSha3);
+    if (inst < n) store2_words<4>(a, p, kk_d + (size_t)inst * 64 + (rej ? 32 : 0));
     return;
   }
   const int iw = wave == 2 ? 0 : wave == 6 ? 1 : wave == 10 ? 2 : wave == 7 ? 3 : 4;  // inner waves 2, 6, 10, 7, 11
@@ -1108,7 +1123,7 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
   const uint32_t p = lane & 1;
   const int k = lane >> 1;
   if (k >= kX2Group || base + k >= n) return;
-  dec_msg_chain<L>(base + k, p, hpk, S.mq[k], sraw_d, krbuf_d);
+  dec_msg_chain<L>(base + k, p, hpk, S.mq[k], sraw_d, krbuf_d, &S.kq[0][0], 2 * kX2Group, lane, &S.kq_ready);
 }
 
 // ----------------------------------------------------------------- host launchers
@@ -1235,12 +1250,12 @@ void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t byt
 // K in the re-encryption launch), and ten launches become six.  scratch:
 // saber_pipe_scratch_per_inst(l) bytes per instance: three GenMatrix streams,
 // two GenSecret, two m and two kr buffers (keygen's GenSecret reuses decaps'
-// buffer, which the stream order makes safe), decaps' H(ct) and the mismatch
-// flags, plus 256-byte alignment of eleven arrays per launch.
+// buffer, which the stream order makes safe) and decaps' two candidate K,
+// plus 256-byte alignment of ten arrays per launch.
 template <int L>
 static size_t pipe_scratch() {
   using P = SaberP<L>;
-  return 3 * (size_t)P::matrix_bytes + 2 * ((size_t)P::secret_bytes + 32 + 64) + 32 + 4;
+  return 3 * (size_t)P::matrix_bytes + 2 * ((size_t)P::secret_bytes + 32 + 64) + 64;
 }
 
 size_t saber_pipe_scratch_per_inst(int l) {
@@ -1249,7 +1264,7 @@ size_t saber_pipe_scratch_per_inst(int l) {
 
 template <int L>
 static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk, uint8_t* ct,
-                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int ipw) {
+                   uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int ipw, int force_reject) {
   using P = SaberP<L>;
   size_t off = 0;
   auto take = [&](size_t per) {
@@ -1266,8 +1281,7 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint8_t* mbuf_d = take(32);
   uint8_t* krbuf_e = take(64);
   uint8_t* krbuf_d = take(64);
-  uint32_t* hc_d = reinterpret_cast<uint32_t*>(take(32));
-  uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
+  uint8_t* kk_d = take(64);
   const int nb = blocks_for(n, 32);
   const dim3 g(nb);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x0"), sab_pipe_x0<L>, g, dim3(256), st, n, kc, pk, sk, araw_k, sraw_d, araw_e,
@@ -1279,18 +1293,18 @@ static void pipe_l(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   SabPolyArgs ae{n, araw_e, sraw_e, pk, (size_t)P::pk, mbuf_e, ct, nullptr, nullptr, nullptr};
   poly_launch<kSabEnc, L, true>(SAB_NAME("sab_poly<kSabEnc>"), st, n, ae, ipw);
   DPKEM_LAUNCH(SAB_NAME("sab_pipe_x2"), sab_pipe_x2<L>, dim3(blocks_for(n, kX2Group)), dim3(kX2Threads), st, n, sk, ct,
-               mbuf_d, sraw_d, krbuf_d, krbuf_e, ss, hc_d);
-  SabPolyArgs ad{n, araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, fail};
-  DPKEM_LAUNCH(SAB_NAME("sab_reenc_k"), sab_reenc_k<L>, dim3(blocks_for(n, kReencGroup)), dim3(64 * kReencGroup), st,
-               ad, hc_d, krbuf_d, sk, ss2);
+               mbuf_d, sraw_d, krbuf_d, krbuf_e, ss, kk_d);
+  SabPolyArgs ad{n,       araw_d, sraw_d, sk + P::vecq, (size_t)P::sk, mbuf_d, nullptr, nullptr, ct, nullptr,
+                 kk_d,    ss2,    force_reject};
+  poly_launch<kSabReenc, L, true>(SAB_NAME("sab_reenc"), st, n, ad, ipw);
 }
 
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int ipw) {
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int ipw, int force_reject) {
   if (n <= 0) return;
-  if (l == 2) pipe_l<2>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw);
-  else if (l == 4) pipe_l<4>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw);
-  else pipe_l<3>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw);
+  if (l == 2) pipe_l<2>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw, force_reject);
+  else if (l == 4) pipe_l<4>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw, force_reject);
+  else pipe_l<3>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw, force_reject);
 }
 
 // Chained keygen -> encaps -> decaps of the same n Saber (l = 3) instances on

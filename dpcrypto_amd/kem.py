@@ -340,8 +340,8 @@ class Engine:
         return ss
 
     def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False, pipeline: bool = True,
-             graph: bool = True, merge: bool = True) -> "Plan":
-        return Plan(self, nstreams, chunks, share_matrix, pipeline, graph, merge)
+             graph: bool = True, merge: bool = True, force_reject: bool = False) -> "Plan":
+        return Plan(self, nstreams, chunks, share_matrix, pipeline, graph, merge, force_reject)
 
     # raw pointer entry points (device arrays preallocated by the caller; each
     # holds at least n rows of its role's width)
@@ -369,13 +369,14 @@ PLAN_SHARE_MATRIX = 1
 PLAN_NO_PIPELINE = 2
 PLAN_NO_GRAPH = 4
 PLAN_NO_MERGE = 8
+PLAN_FORCE_REJECT = 16  # test facility: pipelined chains' decapsulations take the rejection branch
 
 
 class Plan:
     """A graph-captured set of KEM tasks over device arrays (dpkem_plan_*)."""
 
     def __init__(self, engine: Engine, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False,
-                 pipeline: bool = True, graph: bool = True, merge: bool = True):
+                 pipeline: bool = True, graph: bool = True, merge: bool = True, force_reject: bool = False):
         self.eng = engine
         self.lib = engine.lib
         h = ctypes.c_void_p()
@@ -383,7 +384,8 @@ class Plan:
         self.h = h
         self._keep = []  # arrays referenced by the graph stay alive with the plan
         flags = ((PLAN_SHARE_MATRIX if share_matrix else 0) | (0 if pipeline else PLAN_NO_PIPELINE)
-                 | (0 if graph else PLAN_NO_GRAPH) | (0 if merge else PLAN_NO_MERGE))
+                 | (0 if graph else PLAN_NO_GRAPH) | (0 if merge else PLAN_NO_MERGE)
+                 | (PLAN_FORCE_REJECT if force_reject else 0))
         if flags:
             engine._check(self.lib.dpkem_plan_set_options(self.h, flags), "plan_set_options")
 

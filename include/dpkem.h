@@ -205,6 +205,13 @@ int dpkem_plan_add(dpkem_plan *p, int scheme, int op, size_t n, uint8_t *o0, uin
  * a task alone keeps the per-task path).  Results are identical.
  * DPKEM_PLAN_NO_MERGE keeps one launch sequence per task. */
 #define DPKEM_PLAN_NO_MERGE 8u
+/* DPKEM_PLAN_FORCE_REJECT (test facility): in pipelined keygen -> encaps ->
+ * decaps chains every decapsulation takes the implicit-rejection branch, as
+ * if its re-encryption had differed from the ciphertext (Saber family: ss =
+ * SHA3-256(z || H(c)); Frodo: ss = SHAKE(c || s)), so that the constant-time
+ * select of a chain that never sees a forged ciphertext can be checked.
+ * No counterpart in the reference. */
+#define DPKEM_PLAN_FORCE_REJECT 16u
 int dpkem_plan_set_options(dpkem_plan *p, unsigned flags);
 int dpkem_plan_build(dpkem_plan *p);
 int dpkem_plan_launch(dpkem_plan *p);

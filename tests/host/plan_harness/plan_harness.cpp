@@ -136,10 +136,10 @@ void saber_chain_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t*
 }
 size_t saber_pipe_scratch_per_inst(int l) {
   const size_t m = 416 * (size_t)l * l, s = 32 * (size_t)l * (l == 2 ? 10 : l == 3 ? 8 : 6);
-  return 3 * m + 2 * s + 2 * 32 + 2 * 64 + 32 + 4;
+  return 3 * m + 2 * s + 2 * 32 + 2 * 64 + 64;
 }
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int) {
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int, int) {
   const Sz& z = kSz[scheme_of_l(l)];
   const size_t m = 416 * (size_t)l * l, s = 32 * (size_t)l * (l == 2 ? 10 : l == 3 ? 8 : 6);
   launch_on(st);
@@ -150,7 +150,7 @@ void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* 
   wr(ct, n * z.ct);
   wr(ss, n * z.ss);
   wr(ss2, n * z.ss);
-  scr(scratch, carve(n, {m, m, m, s, s, 32, 32, 64, 64, 32, 4}));
+  scr(scratch, carve(n, {m, m, m, s, s, 32, 32, 64, 64, 64}));
 }
 // ---- Frodo
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
@@ -196,7 +196,7 @@ void frodo_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* s
 }
 size_t frodo_pipe_scratch_per_inst(int set) { return 16 * (size_t)set + 3 * 64 + 32 + 4 + 200; }
 void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
-                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool, int) {
+                       uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool, int, int) {
   const Sz& z = kSz[frodo_scheme(set)];
   launch_on(st);
   rd(kc, n * z.kc);
@@ -363,7 +363,8 @@ int main() {
   dpkem_engine* e = nullptr;
   check(dpkem_engine_create(&e, 0, (size_t)8 << 30), "engine_create");
   const unsigned kFlagSets[] = {0u, DPKEM_PLAN_NO_GRAPH, DPKEM_PLAN_NO_PIPELINE, DPKEM_PLAN_SHARE_MATRIX,
-                                DPKEM_PLAN_NO_MERGE, DPKEM_PLAN_NO_MERGE | DPKEM_PLAN_NO_GRAPH};
+                                DPKEM_PLAN_NO_MERGE, DPKEM_PLAN_NO_MERGE | DPKEM_PLAN_NO_GRAPH,
+                                DPKEM_PLAN_FORCE_REJECT};
   const int kSchedules[][2] = {{1, 1}, {2, 3}, {4, 4}, {8, 1}, {3, 5}, {16, 8}};
   int plans = 0;
   for (unsigned flags : kFlagSets) {

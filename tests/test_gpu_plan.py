@@ -243,3 +243,64 @@ def test_independent_frodo_tasks_merge_into_one_launch_sequence(eng, oracle, tor
             p.close()
     eng.set_tuning(D.TUNE_FRODO_PERSIST, 1)
     eng.set_tuning(D.TUNE_FRODO_THREADS, 0)
+
+
+FRODO640, FRODO1344 = 5, 6
+
+
+def _rejection_key(scheme, ct_row, sk_row):
+    """The implicit-rejection shared secret of (ct, sk), from the published
+    definitions: Saber family SHA3-256(z || SHA3-256(c)) with z the last 32
+    bytes of sk; FrodoKEM SHAKE(c || s, len ss) with s the first len-ss bytes
+    of sk (SHAKE128 for Frodo-640, SHAKE256 otherwise)."""
+    import hashlib
+    c, skb = ct_row.tobytes(), sk_row.tobytes()
+    if scheme in (SABER, LIGHTSABER, FIRESABER):
+        return hashlib.sha3_256(skb[-32:] + hashlib.sha3_256(c).digest()).digest()
+    sec = {FRODO: 24, FRODO_AES: 24, FRODO640: 16, FRODO1344: 32}[scheme]
+    h = hashlib.shake_128 if scheme == FRODO640 else hashlib.shake_256
+    return h(c + skb[:sec]).digest(sec)
+
+
+@pytest.mark.parametrize("scheme,n,nstreams,chunks", [
+    (SABER, 97, 2, 3), (SABER, 17, 1, 1), (LIGHTSABER, 33, 1, 1), (FIRESABER, 20, 1, 2),
+    (FRODO, 10, 2, 2), (FRODO640, 5, 1, 1), (FRODO1344, 3, 1, 1), (FRODO_AES, 3, 1, 1)])
+def test_forced_rejection_in_pipelined_chain(eng, oracle, torch_cuda, scheme, n, nstreams, chunks):
+    """DPKEM_PLAN_FORCE_REJECT: every decapsulation of a pipelined chain takes
+    the implicit-rejection branch of its constant-time select (a chain never
+    sees a forged ciphertext, so this branch is otherwise unreachable there).
+    keygen and encaps still equal the oracle; decaps' ss equals the
+    rejection key of (ct, sk) on every instance and every replay."""
+    import dpcrypto_amd as D
+    torch = torch_cuda
+    s = D.SIZES[scheme]
+    kc = oracle.prng_bytes(3000 + n, n * s["kcoins"]).reshape(n, -1)
+    ec = oracle.prng_bytes(4000 + n, n * s["ecoins"]).reshape(n, -1)
+    opk, osk = oracle.keypair(scheme, kc, threads=4)
+    oct_, oss = oracle.enc(scheme, opk, ec, threads=4)
+    want = np.stack([np.frombuffer(_rejection_key(scheme, oct_[i], osk[i]), np.uint8) for i in range(n)])
+    assert not np.array_equal(want, oss)
+    dkc, dec_ = _dev(torch, kc), _dev(torch, ec)
+    a = {k: torch.empty((n, s[w]), dtype=torch.uint8, device="cuda")
+         for k, w in (("pk", "pk"), ("sk", "sk"), ("ct", "ct"), ("ss", "ss"), ("ss2", "ss"))}
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    print(f"forced rejection: scheme={scheme} n={n} {nstreams}x{chunks}", flush=True)
+    plan = eng.plan(nstreams=nstreams, chunks=chunks, force_reject=True)
+    t0 = plan.keypair(scheme, a["pk"], a["sk"], dkc)
+    t1 = plan.enc(scheme, a["ct"], a["ss"], a["pk"], dec_, after=t0)
+    plan.dec(scheme, a["ss2"], a["ct"], a["sk"], after=t1)
+    plan.build()
+    assert plan.info()["subtasks"] == min(chunks, n)  # pipelined: one launch sequence per chunk
+    try:
+        for rep in range(2):
+            for t in a.values():
+                t.zero_()
+            plan.launch()
+            torch.cuda.synchronize()
+            assert np.array_equal(a["pk"].cpu().numpy(), opk), f"pk replay {rep}"
+            assert np.array_equal(a["sk"].cpu().numpy(), osk), f"sk replay {rep}"
+            assert np.array_equal(a["ct"].cpu().numpy(), oct_), f"ct replay {rep}"
+            assert np.array_equal(a["ss"].cpu().numpy(), oss), f"ss replay {rep}"
+            assert np.array_equal(a["ss2"].cpu().numpy(), want), f"rejection ss replay {rep}"
+    finally:
+        plan.close()
