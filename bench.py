@@ -146,66 +146,66 @@ VALU_MACS_PER_OP = {"sab_dec_inner": 2.0, "frodo_dec_mu": 1.0}
 #            --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99969, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129244331, 256248512),
-    ('frodo_as_fused,1344', 8192): (177474018, 353379321),
-    ('frodo_as_fused,640', 8192): (85209357, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129265142, 256253656),
-    ('frodo_dec_mu', 8192): (241918930, 262144),
-    ('frodo_dec_mu,1344', 8192): (358449377, 262144),
-    ('frodo_dec_mu,640', 8192): (165543326, 262144),
-    ('frodo_dec_pre', 8192): (130905019, 130400890),
-    ('frodo_encdec_pre<false>', 8192): (75005003, 128712795),
-    ('frodo_keygen_post', 8192): (129159878, 262144),
-    ('frodo_keygen_pre', 8192): (1397211, 256979842),
-    ('frodo_keygen_pre,1344', 8192): (1516030, 353240076),
-    ('frodo_keygen_pre,640', 8192): (1295699, 168559020),
-    ('frodo_pipe_ssd', 8192): (4266969, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4531530, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5524658, 131072),
-    ('frodo_pipe_x1', 8192): (130398899, 128975182),
-    ('frodo_pipe_x1,1344', 8192): (178634033, 177340416),
-    ('frodo_pipe_x1,640', 8192): (81027495, 84934656),
-    ('frodo_pipe_x2', 8192): (131948301, 130675050),
-    ('frodo_pipe_x2,1344', 8192): (180283982, 178872460),
-    ('frodo_pipe_x2,640', 8192): (83619891, 86442028),
-    ('frodo_sa_fused<false,aes>', 8192): (259422179, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260406491, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356721667, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166973098, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388252380, 262144),
-    ('frodo_sa_fused<true>', 8192): (389362240, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533915281, 265248),
-    ('frodo_sa_fused<true>,640', 8192): (247571551, 262483),
-    ('frodo_sa_grouped<false>', 8192): (129749057, 128974848),
-    ('frodo_sa_grouped<true>', 8192): (258615775, 65536),
-    ('frodo_ss', 8192): (129685987, 196608),
-    ('sab_dec_hash1', 4096): (1561486, 18876164),
-    ('sab_dec_inner', 4096): (9888394, 131072),
-    ('sab_enc_hash1', 4096): (5316841, 19049842),
-    ('sab_hash2', 4096): (5221600, 131072),
-    ('sab_keygen_hash1', 4096): (1278363, 19004650),
-    ('sab_keygen_hash2', 4096): (4242806, 131072),
-    ('sab_pipe_x0', 4096): (1310813, 49632320),
-    ('sab_pipe_x0,l2', 4096): (1296763, 23484724),
-    ('sab_pipe_x0,l4', 4096): (1305485, 85365816),
-    ('sab_pipe_x1', 4096): (36110260, 34340864),
-    ('sab_pipe_x1,l2', 4096): (17792429, 16777216),
-    ('sab_pipe_x1,l4', 4096): (62096374, 58195968),
-    ('sab_pipe_x2', 4096): (12491911, 3801088),
-    ('sab_pipe_x2,l2', 4096): (9380452, 3276800),
-    ('sab_pipe_x2,l4', 4096): (15823649, 3801088),
-    ('sab_poly<kSabEnc>', 4096): (23852916, 4456448),
-    ('sab_poly<kSabEnc>,l2', 4096): (13076526, 3014656),
-    ('sab_poly<kSabEnc>,l4', 4096): (36667202, 6029312),
-    ('sab_poly<kSabKeygen>', 4096): (18985218, 12979060),
-    ('sab_poly<kSabKeygen>,l2', 4096): (9522842, 8650876),
-    ('sab_poly<kSabKeygen>,l4', 4096): (30515805, 17332810),
-    ('sab_poly<kSabReenc>', 4096): (28974377, 131072),
-    ('sab_reenc_k', 4096): (28411462, 147456),
-    ('sab_reenc_k,l2', 4096): (16597523, 147456),
-    ('sab_reenc_k,l4', 4096): (48449449, 5128192),
+    ('frodo_as_fused', 8192): (129247767, 256247904),
+    ('frodo_as_fused,1344', 8192): (177479574, 353366137),
+    ('frodo_as_fused,640', 8192): (85211754, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129268746, 256251584),
+    ('frodo_dec_mu', 8192): (241946844, 262144),
+    ('frodo_dec_mu,1344', 8192): (358508990, 262144),
+    ('frodo_dec_mu,640', 8192): (165540304, 262144),
+    ('frodo_dec_pre', 8192): (130908808, 130404026),
+    ('frodo_encdec_pre<false>', 8192): (75007113, 128712731),
+    ('frodo_keygen_post', 8192): (129163511, 262144),
+    ('frodo_keygen_pre', 8192): (1397575, 256975867),
+    ('frodo_keygen_pre,1344', 8192): (1515535, 353240761),
+    ('frodo_keygen_pre,640', 8192): (1295607, 168558758),
+    ('frodo_pipe_ssd', 8192): (4268113, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4532681, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5525837, 131072),
+    ('frodo_pipe_x1', 8192): (130402538, 128975089),
+    ('frodo_pipe_x1,1344', 8192): (178639058, 177340416),
+    ('frodo_pipe_x1,640', 8192): (81029800, 84934656),
+    ('frodo_pipe_x2', 8192): (131951899, 130677080),
+    ('frodo_pipe_x2,1344', 8192): (180289079, 178865472),
+    ('frodo_pipe_x2,640', 8192): (83622269, 86442080),
+    ('frodo_sa_fused<false,aes>', 8192): (259427365, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260413119, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356731650, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166977769, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388262181, 262144),
+    ('frodo_sa_fused<true>', 8192): (389373577, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533930146, 265638),
+    ('frodo_sa_fused<true>,640', 8192): (247578618, 262342),
+    ('frodo_sa_grouped<false>', 8192): (129716018, 128974848),
+    ('frodo_sa_grouped<true>', 8192): (258633480, 65536),
+    ('frodo_ss', 8192): (129689635, 196608),
+    ('sab_dec_hash1', 4096): (1561404, 18877605),
+    ('sab_dec_inner', 4096): (9888589, 131072),
+    ('sab_enc_hash1', 4096): (5317098, 19050944),
+    ('sab_hash2', 4096): (5229938, 131072),
+    ('sab_keygen_hash1', 4096): (1286457, 19006503),
+    ('sab_keygen_hash2', 4096): (4251116, 131072),
+    ('sab_pipe_x0', 4096): (1296428, 49615576),
+    ('sab_pipe_x0,l2', 4096): (1296937, 23501244),
+    ('sab_pipe_x0,l4', 4096): (1308333, 85362325),
+    ('sab_pipe_x1', 4096): (36114150, 34340864),
+    ('sab_pipe_x1,l2', 4096): (17792953, 16777216),
+    ('sab_pipe_x1,l4', 4096): (62038381, 58195968),
+    ('sab_pipe_x2', 4096): (12584411, 3932160),
+    ('sab_pipe_x2,l2', 4096): (9468769, 3407872),
+    ('sab_pipe_x2,l4', 4096): (15915333, 3932160),
+    ('sab_poly<kSabEnc>', 4096): (23850425, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13076894, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (36676245, 6029312),
+    ('sab_poly<kSabKeygen>', 4096): (18985752, 12978871),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9535396, 8650864),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30523826, 17330126),
+    ('sab_poly<kSabReenc>', 4096): (28974609, 131072),
+    ('sab_reenc', 4096): (29487033, 131072),
+    ('sab_reenc,l2', 4096): (17015730, 131072),
+    ('sab_reenc,l4', 4096): (47703385, 4325376),
 }
 
 
@@ -216,7 +216,7 @@ KERNEL_SOURCES = {
     "saber": ("saber.hip", "saber_poly.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
     "frodo": ("frodo.hip", "frodo_sampler.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
 }
-MEASURED_HBM_SOURCE_SHA = {"saber": "e5ac61dfc0de3d99", "frodo": "d01d202a851ecbb7"}
+MEASURED_HBM_SOURCE_SHA = {"saber": "e74f421639eccaa8", "frodo": "05279686a988b988"}
 
 
 def kernel_family(kernel: str) -> str:
