@@ -644,32 +644,44 @@ def cpu_baseline(budget_s: float):
         kc = rng.integers(0, 256, (n, s["kcoins"]), dtype=np.uint8)
         ec = rng.integers(0, 256, (n, s["ecoins"]), dtype=np.uint8)
         ops = 0
+        t_op = [0.0, 0.0, 0.0]  # keygen, encaps, decaps time within the same rounds
         t0 = time.perf_counter()
         while True:
+            ta = time.perf_counter()
             pk, sk = O.keypair(scheme, kc, threads=th)
+            tb = time.perf_counter()
             ct, ss = O.enc(scheme, pk, ec, threads=th)
+            tc = time.perf_counter()
             O.dec(scheme, ct, sk, threads=th)
+            td = time.perf_counter()
+            t_op[0] += tb - ta
+            t_op[1] += tc - tb
+            t_op[2] += td - tc
             ops += 3 * n
-            if time.perf_counter() - t0 >= seconds:
+            if td - t0 >= seconds:
                 break
         dt = time.perf_counter() - t0
-        return ops / dt, n, ops // (3 * n), dt
+        rounds = ops // (3 * n)
+        per_op = {name: round(n * rounds / t, 1) for name, t in zip(("keygen", "enc", "dec"), t_op)}
+        return ops / dt, n, rounds, dt, per_op
 
     try:
-        v, n, rounds, dt = rate(O.SABER, threads, 32, budget_s * 0.45)
-        v1, _, _, dt1 = rate(O.SABER, 1, 32, budget_s * 0.1)
-        fv, fn, frounds, fdt = rate(O.FRODO, threads, 2, budget_s * 0.35)
-        fv1, _, _, fdt1 = rate(O.FRODO, 1, 2, budget_s * 0.1)
+        v, n, rounds, dt, per_op = rate(O.SABER, threads, 32, budget_s * 0.45)
+        v1, _, _, dt1, per_op1 = rate(O.SABER, 1, 32, budget_s * 0.1)
+        fv, fn, frounds, fdt, fper_op = rate(O.FRODO, threads, 2, budget_s * 0.35)
+        fv1, _, _, fdt1, fper_op1 = rate(O.FRODO, 1, 2, budget_s * 0.1)
     finally:
         O.use_library(None)
     return {"value": round(v, 1), "unit": "KEM ops/s", "cores": threads, "kind": "port",
             "sample": f"Saber l=3 keygen+encaps+decaps, {n} instances per round, {rounds} rounds "
                       f"({dt:.1f} s) on {threads} threads of the CPU oracle (oracle/, {flags})",
             "single_core_value": round(v1, 1), "single_core_seconds": round(dt1, 1),
+            "per_op": per_op, "single_core_per_op": per_op1,
             "frodo976": {"value": round(fv, 1), "unit": "KEM ops/s", "cores": threads,
                          "sample": f"FrodoKEM-976-SHAKE keygen+encaps+decaps, {fn} instances per round, {frounds} "
                                    f"rounds ({fdt:.1f} s) on {threads} threads",
-                         "single_core_value": round(fv1, 1), "single_core_seconds": round(fdt1, 1)},
+                         "single_core_value": round(fv1, 1), "single_core_seconds": round(fdt1, 1),
+                         "per_op": fper_op, "single_core_per_op": fper_op1},
             "compiler_flags": flags, "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
 
 
