@@ -297,23 +297,103 @@ __device__ __forceinline__ void shake_seed_squeeze2(uint32_t p, uint64_t dom, co
   if constexpr (REM > 0) sample_block<REM>(a, p, lut, NB * RW, f);
 }
 
+// The same squeeze split over two waves of a workgroup: the chain wave only
+// permutes and publishes each squeezed block (its lanes' halves) into one of
+// two LDS slots; a sampler wave on another SIMD rebuilds each lane's 32-bit
+// half from the pair's two halves, reads the table and runs f.  `ready`
+// counts published blocks, `done` the blocks the sampler has read (a slot is
+// refilled only once the block two before it is done).  Both waits are the
+// bounded LDS waits of keccak2.hpp.
+template <class P>
+struct FrRing {
+  uint32_t w[2][P::chain_rate][64];  // [slot][word][lane]
+  uint32_t ready, done;
+};
+
+template <class P, int CNT>
+__device__ __forceinline__ void ring_publish(FrRing<P>* ring, const uint32_t a[25], int b, int lane) {
+  if (b >= 2) k2::lds_wait_count(&ring->done, (uint32_t)(b - 1));
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) ring->w[b & 1][i][lane] = a[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if (lane == 0) __hip_atomic_fetch_add(&ring->ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <class P, int OUTW>
+__device__ __forceinline__ void shake_seed_squeeze2_ring(uint32_t p, uint64_t dom, const uint64_t seed[P::ls],
+                                                         FrRing<P>* ring, int lane) {
+  constexpr int RW = P::chain_rate, LS = P::ls;
+  uint64_t m[LS + 1];
+  m[0] = dom | (seed[0] << 8);
+#pragma unroll
+  for (int i = 1; i < LS; ++i) m[i] = (seed[i - 1] >> 56) | (seed[i] << 8);
+  m[LS] = (seed[LS - 1] >> 56) | (kDsShake << 8);
+  uint32_t a[25];
+  k2::zero2(a);
+#pragma unroll
+  for (int i = 0; i <= LS; ++i) a[i] = k2::deint(m[i], p);
+  a[RW - 1] ^= k2::half_const(kPadEnd, p);
+  k2::keccakf2(a, p);
+  constexpr int NB = OUTW / RW, REM = OUTW % RW;
+  static_assert(REM > 0, "the last block is a partial one");
+  for (int b = 0; b < NB; ++b) {
+    ring_publish<P, RW>(ring, a, b, lane);
+    k2::keccakf2_unrolled(a, p);
+  }
+  ring_publish<P, REM>(ring, a, NB, lane);
+}
+
+template <class P, int CNT, class F>
+__device__ __forceinline__ void ring_sample_block(FrRing<P>* ring, uint32_t p, const uint8_t* lut, int b, int lane,
+                                                  F&& f) {
+  k2::lds_wait_count(&ring->ready, (uint32_t)(b + 1));
+  uint32_t x[CNT], e[CNT];
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) {
+    const uint32_t mine = ring->w[b & 1][i][lane], other = ring->w[b & 1][i][lane ^ 1];
+    x[i] = p ? k2::reint_half(other, mine, 1) : k2::reint_half(mine, other, 0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // the slot's reads are done: the chain may refill it
+  if (lane == 0) __hip_atomic_fetch_add(&ring->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) e[i] = (uint32_t)lut[(x[i] >> 1) & 0x7FFF] | ((uint32_t)lut[x[i] >> 17] << 16);
+  const int w0 = b * P::chain_rate;
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) f(w0 + i, x[i], e[i]);
+}
+
+template <class P, int OUTW, class F>
+__device__ __forceinline__ void sample_ring(uint32_t p, FrRing<P>* ring, int lane, const uint8_t* lut, F&& f) {
+  constexpr int RW = P::chain_rate, NB = OUTW / RW, REM = OUTW % RW;
+  for (int b = 0; b < NB; ++b) ring_sample_block<P, RW>(ring, p, lut, b, lane, f);
+  ring_sample_block<P, REM>(ring, p, lut, NB, lane, f);
+}
+
+// Roles of a chain body: the whole chain on one wave, or split into the
+// chain wave and its sampler wave (FrRing).
+enum FrRole { kFrWhole = 0, kFrChain = 1, kFrSampler = 2 };
+
 // ----------------------------------------------------------------- keygen chains
 
 // seedA = SHAKE(z, 16); S^T || E = sample(SHAKE(0x5F || seedSE, 32 n)).
 // Writes pk[0:16], sk[0:sec] = s, sk[sec:sec+16] = seedA, the sk S^T part (LE
 // u16) and the int8 copies of S^T ([k][j]) and E ([i][k]) for the fused kernel.
-template <class P>
+template <class P, int ROLE = kFrWhole>
 __device__ __forceinline__ void keygen_pre_body(int vb, int n, const uint8_t* __restrict__ coins,
                                                 uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
-                                                uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
-  __shared__ __align__(16) uint8_t lut_lds[32768];
-  const uint8_t* lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
+                                                uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
+                                                FrRing<P>* ring = nullptr) {
+  const uint8_t* lut = nullptr;
+  if constexpr (ROLE != kFrChain) {
+    __shared__ __align__(16) uint8_t lut_lds[32768];
+    lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
+  }
   FR_PAIR_SETUP
-  constexpr int RW = P::chain_rate, LS = P::ls;
+  constexpr int RW = P::chain_rate, LS = P::ls, OUTW = 2 * P::mat / 4;
   const uint8_t* c = coins + (size_t)inst * P::kcoins;
   uint8_t* pp = pk + (size_t)inst * P::pk;
   uint8_t* q = sk + (size_t)inst * P::sk;
-  {
+  if constexpr (ROLE != kFrChain) {  // seedA (off the noise chain when a sampler wave runs it)
     const uint64_t z[2] = {ld64(c + 2 * P::sec), ld64(c + 2 * P::sec + 8)};
     uint32_t a[25];
     absorb2_regs<RW, 2>(a, p, z, kDsShake);
@@ -323,13 +403,10 @@ __device__ __forceinline__ void keygen_pre_body(int vb, int n, const uint8_t* __
 #pragma unroll
     for (int i = 0; i < LS; ++i) reinterpret_cast<uint32_t*>(q)[2 * i + p] = sc[2 * i + p];
   }
-  uint64_t seed[LS];
-#pragma unroll
-  for (int i = 0; i < LS; ++i) seed[i] = ld64(c + P::sec + 8 * i);
   uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* skS = q + P::sk_s + 4 * p;
-  shake_seed_squeeze2<P, 2 * P::mat / 4>(p, 0x5F, seed, lut, [&](int w, uint32_t x, uint32_t e) {
+  auto f = [&](int w, uint32_t x, uint32_t e) {
     uint32_t le;
     const uint32_t b = sample2(x, e, &le);
     if (w < P::mat / 4) {
@@ -338,15 +415,41 @@ __device__ __forceinline__ void keygen_pre_body(int vb, int n, const uint8_t* __
     } else {
       *reinterpret_cast<uint16_t*>(ed + 4 * (w - P::mat / 4)) = (uint16_t)b;
     }
-  });
+  };
+  if constexpr (ROLE == kFrSampler) {
+    sample_ring<P, OUTW>(p, ring, lane, lut, f);
+  } else {
+    uint64_t seed[LS];
+#pragma unroll
+    for (int i = 0; i < LS; ++i) seed[i] = ld64(c + P::sec + 8 * i);
+    if constexpr (ROLE == kFrChain) shake_seed_squeeze2_ring<P, OUTW>(p, 0x5F, seed, ring, lane);
+    else shake_seed_squeeze2<P, OUTW>(p, 0x5F, seed, lut, f);
+  }
+}
+
+// One 128-thread block per 32 instances: wave 0 runs the noise chain, wave 1
+// seedA and the sampling (FrRing).
+template <class P>
+__device__ __forceinline__ void ring_init(FrRing<P>* ring) {
+  if (threadIdx.x == 0) {
+    ring->ready = 0;
+    ring->done = 0;
+  }
+  __syncthreads();
 }
 
 template <class P>
-__global__ void __launch_bounds__(64) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
-                                                       uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
-                                                       uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
-  chain_priority();
-  keygen_pre_body<P>(blockIdx.x, n, coins, pk, sk, s8, e8);
+__global__ void __launch_bounds__(128) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
+                                                        uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                        uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
+  __shared__ FrRing<P> ring;
+  ring_init(&ring);
+  if ((threadIdx.x >> 6) == 0) {
+    chain_priority();
+    keygen_pre_body<P, kFrChain>(blockIdx.x, n, coins, pk, sk, s8, e8, &ring);
+  } else {
+    keygen_pre_body<P, kFrSampler>(blockIdx.x, n, coins, pk, sk, s8, e8, &ring);
+  }
 }
 
 // pkh = SHAKE(pk, sec); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
@@ -371,45 +474,54 @@ __global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __
 // mu = recovered mu'.
 // G2 = SHAKE(pkh || mu, 2 sec) = seedSE || k; S' || E' || E'' from
 // SHAKE(0x96 || seedSE, 32 n + 128).  k goes to kbuf[0:sec].
-template <class P, bool DEC>
+template <class P, bool DEC, int ROLE = kFrWhole>
 __device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __restrict__ keys, size_t key_stride,
                                                 const uint8_t* __restrict__ mu, size_t mu_stride,
                                                 uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
                                                 uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
-                                                const uint8_t* __restrict__ pkh) {
-  __shared__ __align__(16) uint8_t lut_lds[32768];  // staged by the one wave of a workgroup that runs this
-  const uint8_t* lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
-  FR_PAIR_SETUP
-  constexpr int RW = P::chain_rate, LS = P::ls;
-  const uint8_t* key = keys + (size_t)inst * key_stride;
-  uint32_t a[25], g[2 * LS];
-  if (DEC || pkh) {
-    const uint8_t* h = DEC ? key + P::sk_pkh : pkh + (size_t)inst * P::sec;
-#pragma unroll
-    for (int i = 0; i < LS; ++i) g[i] = k2::deint(ld64(h + 8 * i), p);
-  } else {
-    absorb2_global<RW, P::pk / 8>(a, p, key, kDsShake);
-#pragma unroll
-    for (int i = 0; i < LS; ++i) g[i] = a[i];
+                                                const uint8_t* __restrict__ pkh, FrRing<P>* ring = nullptr) {
+  const uint8_t* lut = nullptr;
+  if constexpr (ROLE != kFrChain) {
+    __shared__ __align__(16) uint8_t lut_lds[32768];  // staged by the one wave of a workgroup that samples
+    lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
   }
-  const uint8_t* m = mu + (size_t)inst * mu_stride;
-#pragma unroll
-  for (int i = 0; i < LS; ++i) g[LS + i] = k2::deint(ld64(m + 8 * i), p);
-  absorb2_halves<RW, 2 * LS>(a, p, g, kDsShake);
-  store2_words<LS>(a + LS, p, kbuf + (size_t)inst * 64);
-  uint64_t seed[LS];
-#pragma unroll
-  for (int i = 0; i < LS; ++i) seed[i] = k2::out_full(a[i], p);
+  FR_PAIR_SETUP
+  constexpr int RW = P::chain_rate, LS = P::ls, OUTW = (2 * P::mat + 64) / 4;
   uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* pd = reinterpret_cast<uint8_t*>(epp8 + (size_t)inst * 16) + 2 * p;
-  shake_seed_squeeze2<P, (2 * P::mat + 64) / 4>(p, 0x96, seed, lut, [&](int w, uint32_t x, uint32_t e) {
+  auto f = [&](int w, uint32_t x, uint32_t e) {
     uint32_t le;
     const uint16_t b = (uint16_t)sample2(x, e, &le);
     if (w < P::mat / 4) *reinterpret_cast<uint16_t*>(sd + 4 * w) = b;
     else if (w < 2 * P::mat / 4) *reinterpret_cast<uint16_t*>(ed + 4 * (w - P::mat / 4)) = b;
     else *reinterpret_cast<uint16_t*>(pd + 4 * (w - 2 * P::mat / 4)) = b;
-  });
+  };
+  if constexpr (ROLE == kFrSampler) {
+    sample_ring<P, OUTW>(p, ring, lane, lut, f);
+  } else {
+    const uint8_t* key = keys + (size_t)inst * key_stride;
+    uint32_t a[25], g[2 * LS];
+    if (DEC || pkh) {
+      const uint8_t* h = DEC ? key + P::sk_pkh : pkh + (size_t)inst * P::sec;
+#pragma unroll
+      for (int i = 0; i < LS; ++i) g[i] = k2::deint(ld64(h + 8 * i), p);
+    } else {
+      absorb2_global<RW, P::pk / 8>(a, p, key, kDsShake);
+#pragma unroll
+      for (int i = 0; i < LS; ++i) g[i] = a[i];
+    }
+    const uint8_t* m = mu + (size_t)inst * mu_stride;
+#pragma unroll
+    for (int i = 0; i < LS; ++i) g[LS + i] = k2::deint(ld64(m + 8 * i), p);
+    absorb2_halves<RW, 2 * LS>(a, p, g, kDsShake);
+    store2_words<LS>(a + LS, p, kbuf + (size_t)inst * 64);
+    uint64_t seed[LS];
+#pragma unroll
+    for (int i = 0; i < LS; ++i) seed[i] = k2::out_full(a[i], p);
+    if constexpr (ROLE == kFrChain) shake_seed_squeeze2_ring<P, OUTW>(p, 0x96, seed, ring, lane);
+    else shake_seed_squeeze2<P, OUTW>(p, 0x96, seed, lut, f);
+  }
 }
 
 template <class P, bool DEC>
@@ -1190,28 +1302,42 @@ __global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint
 //   ssd  decaps' last ss block (ct tail, kbar chosen by the FO compare).
 // Every op still computes all of its own hashes; only their order changes.
 template <class P>
-__global__ void __launch_bounds__(128) frodo_pipe_x1(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+__global__ void __launch_bounds__(192) frodo_pipe_x1(int n, const uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                      const uint8_t* __restrict__ ecoins, uint8_t* __restrict__ kbuf_e,
                                                      uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
                                                      uint32_t* __restrict__ epp8) {
-  chain_priority();
-  if ((threadIdx.x >> 6) == 0) keygen_post_body<P>(blockIdx.x, n, pk, sk);
-  else encdec_pre_body<P, false>(blockIdx.x, n, pk, (size_t)P::pk, ecoins, (size_t)P::ecoins, kbuf_e, s8, e8, epp8,
-                                 nullptr);
+  __shared__ FrRing<P> ring;
+  ring_init(&ring);
+  const int wave = threadIdx.x >> 6;
+  if (wave == 0) {
+    chain_priority();
+    keygen_post_body<P>(blockIdx.x, n, pk, sk);
+  } else if (wave == 1) {
+    chain_priority();
+    encdec_pre_body<P, false, kFrChain>(blockIdx.x, n, pk, (size_t)P::pk, ecoins, (size_t)P::ecoins, kbuf_e, s8, e8,
+                                        epp8, nullptr, &ring);
+  } else {
+    encdec_pre_body<P, false, kFrSampler>(blockIdx.x, n, pk, (size_t)P::pk, ecoins, (size_t)P::ecoins, kbuf_e, s8, e8,
+                                          epp8, nullptr, &ring);
+  }
 }
 
 template <class P>
-__global__ void __launch_bounds__(192) frodo_pipe_x2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
+__global__ void __launch_bounds__(256) frodo_pipe_x2(int n, const uint8_t* __restrict__ ct, const uint8_t* __restrict__ sk,
                                                      const uint8_t* __restrict__ kbuf_e, uint8_t* __restrict__ ss_e,
                                                      const uint8_t* __restrict__ mubuf, uint8_t* __restrict__ kbuf_d,
                                                      uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
                                                      uint32_t* __restrict__ epp8, uint32_t* __restrict__ state) {
-  chain_priority();
+  __shared__ FrRing<P> ring;
+  ring_init(&ring);
   const int wave = threadIdx.x >> 6, vb = blockIdx.x;
+  if (wave != 3) chain_priority();
   if (wave == 0) {
     ss_body<P>(vb, n, ct, kbuf_e, nullptr, 0, nullptr, ss_e);
   } else if (wave == 1) {
-    encdec_pre_body<P, true>(vb, n, sk, (size_t)P::sk, mubuf, 32, kbuf_d, s8, e8, epp8, nullptr);
+    encdec_pre_body<P, true, kFrChain>(vb, n, sk, (size_t)P::sk, mubuf, 32, kbuf_d, s8, e8, epp8, nullptr, &ring);
+  } else if (wave == 3) {
+    encdec_pre_body<P, true, kFrSampler>(vb, n, sk, (size_t)P::sk, mubuf, 32, kbuf_d, s8, e8, epp8, nullptr, &ring);
   } else {
     FR_PAIR_SETUP
     uint32_t a[25];
@@ -1394,7 +1520,7 @@ static void launch_dec_mu(hipStream_t st, const FrMt& M) {
 template <class P, bool AES>
 static void keygen_t(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8, uint32_t* e8,
                      int threads) {
-  DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, coins, pk,
+  DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, dim3(blocks_for(n, 32)), dim3(128), st, n, coins, pk,
                sk, s8, e8);
   const FrMtTask t = task_of(n, coins, nullptr, pk, sk, s8, e8);
   launch_as_fused<P, AES>(st, table_of(&t, 1), false, threads);
@@ -1480,16 +1606,16 @@ static void pipe_t(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, 
   uint32_t* fail = reinterpret_cast<uint32_t*>(take(4));
   uint32_t* state = reinterpret_cast<uint32_t*>(take(200));
   const dim3 g(blocks_for(n, 32));
-  DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, g, dim3(64), st, n, kc, pk, sk, s8, e8);
+  DPKEM_LAUNCH(FR_NAME("frodo_keygen_pre"), frodo_keygen_pre<P>, g, dim3(128), st, n, kc, pk, sk, s8, e8);
   const FrMtTask tk = task_of(n, kc, nullptr, pk, sk, s8, e8);
   launch_as_fused<P, AES>(st, table_of(&tk, 1), false, threads);
-  DPKEM_LAUNCH(FR_NAME("frodo_pipe_x1"), frodo_pipe_x1<P>, g, dim3(128), st, n, pk, sk, ec, kbuf_e, s8, e8, epp8);
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_x1"), frodo_pipe_x1<P>, g, dim3(192), st, n, pk, sk, ec, kbuf_e, s8, e8, epp8);
   const FrMtTask te = task_of(n, pk, ec, ct, ss, s8, e8, epp8, kbuf_e);
   launch_sa_fused<P, false, AES>(st, table_of(&te, 1), false, threads);
   const FrMtTask td = task_of(n, ct, sk, ss2, nullptr, s8, e8, epp8, kbuf_d, mubuf, fail, state, (size_t)P::sk);
   const FrMt Md = table_of(&td, 1);
   launch_dec_mu<P>(st, Md);
-  DPKEM_LAUNCH(FR_NAME("frodo_pipe_x2"), frodo_pipe_x2<P>, g, dim3(192), st, n, ct, sk, kbuf_e, ss, mubuf, kbuf_d, s8,
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_x2"), frodo_pipe_x2<P>, g, dim3(256), st, n, ct, sk, kbuf_e, ss, mubuf, kbuf_d, s8,
                e8, epp8, state);
   launch_sa_fused<P, true, AES>(st, Md, false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, g, dim3(64), st, n, ct, state, kbuf_d, sk, (size_t)P::sk,
