@@ -427,8 +427,6 @@ __device__ __forceinline__ void keygen_pre_body(int vb, int n, const uint8_t* __
   }
 }
 
-// One 128-thread block per 32 instances: wave 0 runs the noise chain, wave 1
-// seedA and the sampling (FrRing).
 template <class P>
 __device__ __forceinline__ void ring_init(FrRing<P>* ring) {
   if (threadIdx.x == 0) {
@@ -438,18 +436,27 @@ __device__ __forceinline__ void ring_init(FrRing<P>* ring) {
   __syncthreads();
 }
 
+// One 128-thread block per 32 instances: wave 0 runs the noise chain, wave 1
+// seedA and the sampling (FrRing).
+template <class P>
+__device__ __forceinline__ void keygen_pre_split(int vb, int n, const uint8_t* __restrict__ coins,
+                                                 uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
+                                                 uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
+  __shared__ FrRing<P> ring;
+  ring_init(&ring);
+  chain_priority();  // the sampler too: the chain waits on it for ring slots
+  if ((threadIdx.x >> 6) == 0) {
+    keygen_pre_body<P, kFrChain>(vb, n, coins, pk, sk, s8, e8, &ring);
+  } else {
+    keygen_pre_body<P, kFrSampler>(vb, n, coins, pk, sk, s8, e8, &ring);
+  }
+}
+
 template <class P>
 __global__ void __launch_bounds__(128) frodo_keygen_pre(int n, const uint8_t* __restrict__ coins,
                                                         uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                         uint32_t* __restrict__ s8, uint32_t* __restrict__ e8) {
-  __shared__ FrRing<P> ring;
-  ring_init(&ring);
-  if ((threadIdx.x >> 6) == 0) {
-    chain_priority();
-    keygen_pre_body<P, kFrChain>(blockIdx.x, n, coins, pk, sk, s8, e8, &ring);
-  } else {
-    keygen_pre_body<P, kFrSampler>(blockIdx.x, n, coins, pk, sk, s8, e8, &ring);
-  }
+  keygen_pre_split<P>(blockIdx.x, n, coins, pk, sk, s8, e8);
 }
 
 // pkh = SHAKE(pk, sec); sk = s || pk || S^T || pkh (pk already copied by the fused kernel).
@@ -524,14 +531,31 @@ __device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __
   }
 }
 
+// One 128-thread block per 32 instances: wave 0 runs G2 and the noise chain,
+// wave 1 the sampling (FrRing).
 template <class P, bool DEC>
-__global__ void __launch_bounds__(64) frodo_encdec_pre(int n, const uint8_t* __restrict__ keys, size_t key_stride,
-                                                       const uint8_t* __restrict__ mu, size_t mu_stride,
-                                                       uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
-                                                       uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
-                                                       const uint8_t* __restrict__ pkh) {
-  chain_priority();
-  encdec_pre_body<P, DEC>(blockIdx.x, n, keys, key_stride, mu, mu_stride, kbuf, s8, e8, epp8, pkh);
+__device__ __forceinline__ void encdec_pre_split(int vb, int n, const uint8_t* __restrict__ keys, size_t key_stride,
+                                                 const uint8_t* __restrict__ mu, size_t mu_stride,
+                                                 uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
+                                                 uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
+                                                 const uint8_t* __restrict__ pkh) {
+  __shared__ FrRing<P> ring;
+  ring_init(&ring);
+  chain_priority();  // the sampler too: the chain waits on it for ring slots
+  if ((threadIdx.x >> 6) == 0) {
+    encdec_pre_body<P, DEC, kFrChain>(vb, n, keys, key_stride, mu, mu_stride, kbuf, s8, e8, epp8, pkh, &ring);
+  } else {
+    encdec_pre_body<P, DEC, kFrSampler>(vb, n, keys, key_stride, mu, mu_stride, kbuf, s8, e8, epp8, pkh, &ring);
+  }
+}
+
+template <class P, bool DEC>
+__global__ void __launch_bounds__(128) frodo_encdec_pre(int n, const uint8_t* __restrict__ keys, size_t key_stride,
+                                                        const uint8_t* __restrict__ mu, size_t mu_stride,
+                                                        uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
+                                                        uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
+                                                        const uint8_t* __restrict__ pkh) {
+  encdec_pre_split<P, DEC>(blockIdx.x, n, keys, key_stride, mu, mu_stride, kbuf, s8, e8, epp8, pkh);
 }
 
 // ss = SHAKE(ct || kbar, sec); for decaps kbar = fail ? s : k'.  The full
@@ -1296,9 +1320,11 @@ __global__ void __launch_bounds__(1024) frodo_sa_grouped(int ngroups, const uint
 // For keygen -> encaps -> decaps of the same instances (a plan chain), the
 // serial chains of different ops that read disjoint inputs share launches:
 //   x1   wave 0: keygen's pkh into sk; wave 1: encaps' pkh, G2 and noise
-//        (both only need pk, complete after the keygen product);
-//   x2   wave 0: encaps' ss; wave 1: decaps' G2 and noise (after dec_mu);
-//        wave 2: decaps' SHAKE over the full ct blocks, kept in `state`;
+//        chain, wave 2 its sampling (both ops only need pk, complete
+//        after the keygen product);
+//   x2   wave 0: encaps' ss; wave 1: decaps' G2 and noise chain (after
+//        dec_mu), wave 3 its sampling; wave 2: decaps' SHAKE over the
+//        full ct blocks, kept in `state`;
 //   ssd  decaps' last ss block (ct tail, kbar chosen by the FO compare).
 // Every op still computes all of its own hashes; only their order changes.
 template <class P>
@@ -1309,11 +1335,10 @@ __global__ void __launch_bounds__(192) frodo_pipe_x1(int n, const uint8_t* __res
   __shared__ FrRing<P> ring;
   ring_init(&ring);
   const int wave = threadIdx.x >> 6;
+  chain_priority();
   if (wave == 0) {
-    chain_priority();
     keygen_post_body<P>(blockIdx.x, n, pk, sk);
   } else if (wave == 1) {
-    chain_priority();
     encdec_pre_body<P, false, kFrChain>(blockIdx.x, n, pk, (size_t)P::pk, ecoins, (size_t)P::ecoins, kbuf_e, s8, e8,
                                         epp8, nullptr, &ring);
   } else {
@@ -1331,7 +1356,7 @@ __global__ void __launch_bounds__(256) frodo_pipe_x2(int n, const uint8_t* __res
   __shared__ FrRing<P> ring;
   ring_init(&ring);
   const int wave = threadIdx.x >> 6, vb = blockIdx.x;
-  if (wave != 3) chain_priority();
+  chain_priority();
   if (wave == 0) {
     ss_body<P>(vb, n, ct, kbuf_e, nullptr, 0, nullptr, ss_e);
   } else if (wave == 1) {
@@ -1374,9 +1399,9 @@ __global__ void __launch_bounds__(64) frodo_pipe_ssd(int n, const uint8_t* __res
   ssd_body<P>(blockIdx.x, n, ct, state, kbuf_d, sk, sk_stride, fail, ss, force_reject);
 }
 
-// decaps' chains before the re-encryption, one 128-thread block per 32
+// decaps' chains before the re-encryption, one 192-thread block per 32
 // instances: wave 0 runs G2 and the S' / E' / E'' squeeze (encdec_pre_body),
-// wave 1 absorbs the full blocks of ct for ss (ss_absorb) and leaves the
+// wave 2 samples it (FrRing), wave 1 absorbs the full blocks of ct for ss (ss_absorb) and leaves the
 // sponge state in `state`, so that after the re-encryption only ss's last
 // block remains (ssd_body) instead of the whole ct absorb.
 template <class P>
@@ -1385,8 +1410,14 @@ __device__ __forceinline__ void dec_pre_body(int vb, int n, const uint8_t* __res
                                              const uint8_t* __restrict__ mubuf, uint8_t* __restrict__ kbuf,
                                              uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
                                              uint32_t* __restrict__ epp8, uint32_t* __restrict__ state) {
-  if ((threadIdx.x >> 6) == 0) {
-    encdec_pre_body<P, true>(vb, n, sk, sk_stride, mubuf, 32, kbuf, s8, e8, epp8, nullptr);
+  __shared__ FrRing<P> ring;
+  ring_init(&ring);
+  const int wave = threadIdx.x >> 6;
+  chain_priority();
+  if (wave == 0) {
+    encdec_pre_body<P, true, kFrChain>(vb, n, sk, sk_stride, mubuf, 32, kbuf, s8, e8, epp8, nullptr, &ring);
+  } else if (wave == 2) {
+    encdec_pre_body<P, true, kFrSampler>(vb, n, sk, sk_stride, mubuf, 32, kbuf, s8, e8, epp8, nullptr, &ring);
   } else {
     FR_PAIR_SETUP
     uint32_t a[25];
@@ -1397,12 +1428,11 @@ __device__ __forceinline__ void dec_pre_body(int vb, int n, const uint8_t* __res
 }
 
 template <class P>
-__global__ void __launch_bounds__(128) frodo_dec_pre(int n, const uint8_t* __restrict__ ct,
+__global__ void __launch_bounds__(192) frodo_dec_pre(int n, const uint8_t* __restrict__ ct,
                                                      const uint8_t* __restrict__ sk, size_t sk_stride,
                                                      const uint8_t* __restrict__ mubuf, uint8_t* __restrict__ kbuf,
                                                      uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
                                                      uint32_t* __restrict__ epp8, uint32_t* __restrict__ state) {
-  chain_priority();
   dec_pre_body<P>(blockIdx.x, n, ct, sk, sk_stride, mubuf, kbuf, s8, e8, epp8, state);
 }
 
@@ -1530,7 +1560,7 @@ static void keygen_t(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, u
 template <class P, bool AES>
 static void enc_t(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss, uint32_t* s8,
                   uint32_t* e8, uint32_t* epp8, uint8_t* kbuf, const uint8_t* pkh, int threads) {
-  DPKEM_LAUNCH(FR_NAME("frodo_encdec_pre<false>"), (frodo_encdec_pre<P, false>), dim3(blocks_for(n, 32)), dim3(64), st,
+  DPKEM_LAUNCH(FR_NAME("frodo_encdec_pre<false>"), (frodo_encdec_pre<P, false>), dim3(blocks_for(n, 32)), dim3(128), st,
                n, pk, (size_t)P::pk, coins, (size_t)P::ecoins, kbuf, s8, e8, epp8, pkh);
   const FrMtTask t = task_of(n, pk, coins, ct, ss, s8, e8, epp8, kbuf);
   launch_sa_fused<P, false, AES>(st, table_of(&t, 1), false, threads);
@@ -1544,7 +1574,7 @@ static void dec_t(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, u
   const FrMtTask t = task_of(n, ct, sk, ss, nullptr, s8, e8, epp8, kbuf, mubuf, fail, state, (size_t)P::sk);
   const FrMt M = table_of(&t, 1);
   launch_dec_mu<P>(st, M);
-  DPKEM_LAUNCH(FR_NAME("frodo_dec_pre"), frodo_dec_pre<P>, dim3(blocks_for(n, 32)), dim3(128), st, n, ct, sk,
+  DPKEM_LAUNCH(FR_NAME("frodo_dec_pre"), frodo_dec_pre<P>, dim3(blocks_for(n, 32)), dim3(192), st, n, ct, sk,
                (size_t)P::sk, mubuf, kbuf, s8, e8, epp8, state);
   launch_sa_fused<P, true, AES>(st, M, false, threads);
   DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, state, kbuf,
@@ -1639,7 +1669,7 @@ void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t p
                               const uint2* groups, int ngroups, const uint8_t* pkh) {
   if (n <= 0) return;
   if (!groups) ngroups = blocks_for(n, kFrodoGroup);
-  DPKEM_LAUNCH("frodo_encdec_pre<false>", (frodo_encdec_pre<F976, false>), dim3(blocks_for(n, 32)), dim3(64), st, n,
+  DPKEM_LAUNCH("frodo_encdec_pre<false>", (frodo_encdec_pre<F976, false>), dim3(blocks_for(n, 32)), dim3(128), st, n,
                pk, pk_stride, coins, (size_t)kFrodoEncCoins, kbuf, s8, e8, epp8, pkh);
   DPKEM_LAUNCH("frodo_sa_grouped<false>", (frodo_sa_grouped<false, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
                ngroups, groups, n, pk, pk_stride, s8, reinterpret_cast<const int8_t*>(e8),
@@ -1658,7 +1688,7 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
     const FrMtTask t = task_of(n, ct, sk, ss, nullptr, s8, e8, epp8, kbuf, mubuf, fail, state, sk_stride);
     launch_dec_mu<P>(st, table_of(&t, 1));
   }
-  DPKEM_LAUNCH("frodo_dec_pre", frodo_dec_pre<F976>, dim3(blocks_for(n, 32)), dim3(128), st, n, ct, sk, sk_stride,
+  DPKEM_LAUNCH("frodo_dec_pre", frodo_dec_pre<F976>, dim3(blocks_for(n, 32)), dim3(192), st, n, ct, sk, sk_stride,
                mubuf, kbuf, s8, e8, epp8, state);
   DPKEM_LAUNCH("frodo_sa_grouped<true>", (frodo_sa_grouped<true, kFrodoGroup>), dim3(ngroups), dim3(1024), st,
                ngroups, groups, n, sk + kFrodoSec, sk_stride, s8, reinterpret_cast<const int8_t*>(e8),
@@ -1676,19 +1706,17 @@ void frodo_dec_grouped_launch(hipStream_t st, int n, const uint8_t* ct, const ui
 
 // keygen: seedA, S / E chains | enc: pkh, G2, S' / E' / E'' chains
 template <class P, int OP>
-__global__ void __launch_bounds__(64) frodo_mt_pre(const FrMt M) {
-  chain_priority();
+__global__ void __launch_bounds__(128) frodo_mt_pre(const FrMt M) {
   const FrMtTask& T = M.t[mt_task_of_group(M, blockIdx.x)];
   const int vb = blockIdx.x - T.first_group;
-  if constexpr (OP == 0) keygen_pre_body<P>(vb, T.n, T.i0, T.o0, T.o1, T.s8, T.e8);
-  else encdec_pre_body<P, false>(vb, T.n, T.i0, (size_t)P::pk, T.i1, (size_t)P::ecoins, T.kbuf, T.s8, T.e8, T.epp8,
-                                 nullptr);
+  if constexpr (OP == 0) keygen_pre_split<P>(vb, T.n, T.i0, T.o0, T.o1, T.s8, T.e8);
+  else encdec_pre_split<P, false>(vb, T.n, T.i0, (size_t)P::pk, T.i1, (size_t)P::ecoins, T.kbuf, T.s8, T.e8, T.epp8,
+                                  nullptr);
 }
 
 // decaps: G2 and the S' / E' / E'' chains beside ss's ct absorb
 template <class P>
-__global__ void __launch_bounds__(128) frodo_mt_dec_pre(const FrMt M) {
-  chain_priority();
+__global__ void __launch_bounds__(192) frodo_mt_dec_pre(const FrMt M) {
   const FrMtTask& T = M.t[mt_task_of_group(M, blockIdx.x)];
   dec_pre_body<P>(blockIdx.x - T.first_group, T.n, T.i0, T.i1, (size_t)P::sk, T.mubuf, T.kbuf, T.s8, T.e8, T.epp8,
                   T.state);
@@ -1730,16 +1758,16 @@ static void mt_t(hipStream_t st, int op, const FrodoMtDesc* d, int ntasks, bool 
   if (M.total == 0) return;
   const dim3 groups(M.groups);
   if (op == 0) {
-    DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<keygen>"), (frodo_mt_pre<P, 0>), groups, dim3(64), st, M);
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<keygen>"), (frodo_mt_pre<P, 0>), groups, dim3(128), st, M);
     launch_as_fused<P, AES>(st, M, persist, threads);
     DPKEM_LAUNCH(FR_NAME("frodo_mt_post<keygen>"), (frodo_mt_post<P, 0>), groups, dim3(64), st, M);
   } else if (op == 1) {
-    DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<enc>"), (frodo_mt_pre<P, 1>), groups, dim3(64), st, M);
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_pre<enc>"), (frodo_mt_pre<P, 1>), groups, dim3(128), st, M);
     launch_sa_fused<P, false, AES>(st, M, persist, threads);
     DPKEM_LAUNCH(FR_NAME("frodo_mt_post<enc>"), (frodo_mt_post<P, 1>), groups, dim3(64), st, M);
   } else {
     launch_dec_mu<P>(st, M);
-    DPKEM_LAUNCH(FR_NAME("frodo_mt_dec_pre"), frodo_mt_dec_pre<P>, groups, dim3(128), st, M);
+    DPKEM_LAUNCH(FR_NAME("frodo_mt_dec_pre"), frodo_mt_dec_pre<P>, groups, dim3(192), st, M);
     launch_sa_fused<P, true, AES>(st, M, persist, threads);
     DPKEM_LAUNCH(FR_NAME("frodo_mt_post<dec>"), (frodo_mt_post<P, 2>), groups, dim3(64), st, M);
   }

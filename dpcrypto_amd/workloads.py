@@ -115,14 +115,15 @@ def run_mixed(torch, eng, plain_eng, seed: int = 0x44504B454D000004, total: int 
 
     if progress:
         progress(f"mixed: {len(batches)} batches, autotuning the plan")
-    # coordinate descent: streams x chunks at the default knobs, then each
-    # knob's other values at the best schedule
+    # coordinate descent: streams x chunks with the merged fused kernels
+    # persistent and single-pass (the two interact: a persistent grid holds
+    # every CU until it ends, so it decides how other streams' work
+    # overlaps), then each remaining knob's other values at the best schedule
     best, tried = autotune.tune(torch, eng, f"mixed|seed={seed}|total={total}|ipw|persist|merge|threads", build,
                                 use_cache=False, budget_s=tune_budget_s,
-                                candidates=[c + (1, 1, 1, 512) for c in autotune.DEFAULT_CANDIDATES],
-                                refine=lambda b: [(b[0], b[1], 2, 1, 1, 512), (b[0], b[1], 4, 1, 1, 512),
-                                                  (b[0], b[1], 1, 0, 1, 512), (b[0], b[1], 1, 1, 0, 512),
-                                                  (b[0], b[1], 1, 1, 1, 1024), (b[0], b[1], 1, 0, 1, 1024)])
+                                candidates=[c + (1, pe, 1, 512) for pe in (1, 0) for c in autotune.DEFAULT_CANDIDATES],
+                                refine=lambda b: [(b[0], b[1], 2, b[3], 1, 512), (b[0], b[1], 4, b[3], 1, 512),
+                                                  (b[0], b[1], 1, b[3], 0, 512), (b[0], b[1], 1, b[3], 1, 1024)])
     plan = build(*best)
     plan.launch()
     torch.cuda.synchronize()
