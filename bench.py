@@ -1,23 +1,31 @@
 #!/usr/bin/env python3
 """Batched KEM throughput on one MI355X.
 
-Headline workload (the metric's single-GPU configuration): Saber l=3, a batch
-of 4096 instances, one step = keygen + encaps + decaps of every instance,
-inputs (coins) resident in HBM before the timed region, outputs left in HBM.
-value = KEM operations per second = 3 * 4096 * steps / seconds.
+Headline workload (the metric's single-GPU configuration): Saber l=3, batches
+of 4096 instances; one step = 4096 keygens + 4096 encapsulations + 4096
+decapsulations, inputs resident in HBM before the timed region, outputs left
+in HBM.  value = KEM operations per second = 3 * 4096 * steps / seconds.
 
-Every op recomputes everything from its own inputs (encaps expands the
-public matrix from pk, decaps from sk), as separate parties would.  The step
-is a graph-captured plan; its keygen -> encaps -> decaps chain runs
-pipelined (stages of different ops that read disjoint inputs share launches,
-e.g. decaps' GenMatrix beside encaps' first hash stage).
+The steps run a rolling keygen -> encaps -> decaps pipeline over three batch
+slots: step r generates the keys of slot r mod 3, encapsulates to the keys of
+slot r - 1 (generated by the previous step) and decapsulates the ciphertexts
+of slot r - 2 (encapsulated by the previous step, with the secret keys of two
+steps before).  Each batch therefore goes through all three ops, one per
+step, and every op consumes what an earlier op produced; within a step the
+three ops are independent, so the plan runs them as one mixed launch sequence
+(all hash chains in one launch, all products in one, the closing hashes in
+one).  Every op recomputes everything from its own inputs (encaps expands the
+public matrix from pk, decaps from sk), as separate parties would.
 
 Reported beside it in the same JSON line:
+  saber_chained  the same ops with the keygen -> encaps -> decaps chain of ONE
+                 batch inside each step (pipelined launch sequence)
   frodo976       FrodoKEM-976-SHAKE, batch 8192, keygen + encaps + decaps
   frodo640 / frodo1344 / frodo976_aes / lightsaber / firesaber
-                 the other parameter sets on the same step
+                 the other parameter sets (Saber family: rolling steps)
   plain          the one-workgroup-per-KEM comparator form on the same inputs
-  latency        p50 / p99 of the per-step wall time (ms)
+  latency        p50 / p99 of the per-step wall time (ms), and of a batch's
+                 three steps from keygen to decaps
   roofline       dominant kernel of the headline step: the plan's own kernels
                  launched without a graph (same launches, streams and
                  kernels), each bracketed by HIP events on its stream;
@@ -28,9 +36,9 @@ Reported beside it in the same JSON line:
   serving        Poisson arrivals into the continuous-batching server
                  (70% Saber, ops 1:4:4) vs one request per launch
   saber_unpipelined
-                 the headline step as three dependent plan tasks
+                 the chained step as three dependent plan tasks
   saber_shared_matrix_chain
-                 the headline steps with the plan's opt-in matrix sharing
+                 the chained steps with the plan's opt-in matrix sharing
                  (encaps / decaps reuse keygen's A); not the headline value
   cpu_baseline   the CPU oracle on the host cores, on a bounded sample
 """
@@ -82,6 +90,17 @@ KERNEL_WORK = {
     "sab_pipe_x1": (8 + 1 + 1 + 5 + 8, 0),             # encaps' H(pk), m, G, GenSecret; keygen's H(pk)
     "sab_pipe_x2": (10 + 9 + 1 + 5 + 2, 3 * 65536),    # encaps' K; decaps' H(ct), G, GenSecret, both K; inner product
     "sab_reenc": (0, 12 * 65536),                        # re-encryption product, compare, select of decaps' K
+    # independent keygen, encaps and decaps tasks in one mixed launch sequence
+    # (per instance of each op): all hash chains, all products, closing hashes
+    "sab_mix_x0": ((1 + 23 + 5) + (23 + 1 + 8 + 1 + 5) + (23 + 9 + 2 + 1 + 5), 3 * 65536),
+    "sab_mix_poly": (0, (9 + 12 + 12) * 65536),
+    "sab_mix_x1": (8 + 9 + 1, 0),
+    "sab_mix_x0,l2": ((1 + 10 + 4) + (10 + 1 + 5 + 1 + 4) + (10 + 6 + 2 + 1 + 4), 2 * 65536),
+    "sab_mix_poly,l2": (0, (4 + 6 + 6) * 65536),
+    "sab_mix_x1,l2": (5 + 6 + 1, 0),
+    "sab_mix_x0,l4": ((1 + 40 + 5) + (40 + 1 + 10 + 1 + 5) + (40 + 11 + 2 + 1 + 5), 4 * 65536),
+    "sab_mix_poly,l4": (0, (16 + 20 + 20) * 65536),
+    "sab_mix_x1,l4": (10 + 11 + 1, 0),
     "sab_pipe_x0,l2": (1 + 10 + 4 + 10 + 10, 0),
     "sab_pipe_x1,l2": (5 + 1 + 1 + 4 + 5, 0),
     "sab_pipe_x2,l2": (7 + 6 + 1 + 4 + 2, 2 * 65536),
@@ -390,6 +409,112 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
                           "tuning_ms": {"x".join(map(str, c)): round(v, 4) for c, v in tried.items()}})
 
 
+def rolling_slots(torch, scheme, n, seed):
+    """Three batch slots of device-resident arrays (chain_arrays) for the
+    rolling keygen -> encaps -> decaps pipeline."""
+    return [chain_arrays(torch, scheme, n, seed + 7919 * k) for k in range(3)]
+
+
+def build_rolling_plan(eng, scheme, slots, phase, ipw=1, graph=True):
+    """Step `phase` (mod 3) of the rolling pipeline as a graph-captured plan:
+    keygen of slot phase, encaps of slot phase - 1 to the keys the previous
+    step generated, decaps of slot phase - 2 (the ciphertexts the previous
+    step encapsulated, the secret keys of two steps before).  The three tasks
+    are independent within the step, so the plan runs them as one mixed
+    launch sequence.  ipw: the Saber product's instances per workgroup."""
+    import dpcrypto_amd as D
+    eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    k, e, d = slots[phase % 3], slots[(phase - 1) % 3], slots[(phase - 2) % 3]
+    p = eng.plan(nstreams=1, chunks=1, graph=graph)
+    p.keypair(scheme, k["pk"], k["sk"], k["kc"])
+    p.enc(scheme, e["ct"], e["ss"], e["pk"], e["ec"])
+    p.dec(scheme, d["ss2"], d["ct"], d["sk"])
+    return p.build()
+
+
+def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
+    """Time `steps` steps of the rolling pipeline (each one replay of the
+    phase's graph; three phases cycle).  The product's instances per
+    workgroup (1 / 2 / 4) is picked first by timing two cycles of each.
+    Afterwards two more steps finish the batches still in flight and every
+    slot must hold ss == ss2 (encaps' and decaps' shared secrets)."""
+    import statistics
+    slots = rolling_slots(torch, scheme, n, seed)
+    stream = torch.cuda.current_stream(torch.device("cuda:0"))
+    eng.set_stream(stream.cuda_stream)
+
+    def cycle_ms(plans, reps):
+        ts = []
+        for _ in range(reps):
+            x, y = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            x.record(stream)
+            for q in plans:
+                q.launch()
+            y.record(stream)
+            y.synchronize()
+            ts.append(x.elapsed_time(y) / 3)
+        return statistics.median(ts)
+
+    tried = {}
+    for c in ([ipw] if ipw else [1, 2, 4]):
+        plans = [build_rolling_plan(eng, scheme, slots, ph, c) for ph in range(3)]
+        cycle_ms(plans, 2)
+        tried[c] = cycle_ms(plans, 4)
+        for q in plans:
+            q.close()
+    best = min(tried, key=tried.get)
+    plans = [build_rolling_plan(eng, scheme, slots, ph, best) for ph in range(3)]
+    w0 = max(warmup, 3)  # every slot through keygen, encaps and decaps before timing
+    for i in range(w0):
+        plans[i % 3].launch()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    stops = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        plans[(w0 + i) % 3].launch()
+        stops[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for i in range(2):
+        plans[(w0 + steps + i) % 3].launch()
+    torch.cuda.synchronize()
+    per_step = [x.elapsed_time(y) for x, y in zip(starts, stops)]
+    ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots)
+    for q in plans:
+        q.close()
+    batch = [sum(per_step[i:i + 3]) for i in range(max(1, len(per_step) - 2))]
+    return dict(wall_s=wall, per_step_ms=per_step, batch_ms=batch, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
+                schedule={"plan": "one mixed launch sequence per step", "saber_poly_ipw": best,
+                          "tuning_ms": {f"ipw{c}": round(v, 4) for c, v in tried.items()}})
+
+
+def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=1):
+    """Per-kernel HIP-event times of the rolling steps' kernels, launched
+    without a graph (DPKEM_PLAN_NO_GRAPH: same launches and kernels)."""
+    slots = rolling_slots(torch, scheme, n, seed)
+    eng.set_stream(torch.cuda.current_stream(torch.device("cuda:0")).cuda_stream)
+    plans = [build_rolling_plan(eng, scheme, slots, ph, ipw, graph=False) for ph in range(3)]
+    for q in plans:
+        q.launch()
+    torch.cuda.synchronize()
+    eng.kernel_stats(reset=True)
+    eng.set_profiling(True)
+    for i in range(steps):
+        plans[i % 3].launch()
+    torch.cuda.synchronize()
+    eng.set_profiling(False)
+    stats = eng.kernel_stats(reset=True)
+    for i in range(2):
+        plans[(steps + i) % 3].launch()
+    torch.cuda.synchronize()
+    ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots)
+    for q in plans:
+        q.close()
+    return dict(kernels=stats, roundtrip_ok=ok)
+
+
 # Op-level ceilings (ops/s) from the algorithmic work of one op (Keccak
 # permutations at 4,176 VALU lane-ops each plus MACs at 2 per lane-op) at the
 # VALU peak; keygen+enc+dec at 1:1:1 is their harmonic combination.
@@ -544,7 +669,9 @@ def profile_plan(torch, eng, scheme, n, steps, seed):
 # CRITICAL_PERMS x this floor, whatever its VALU throughput.
 CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10,
                   "sab_pipe_x0,l2": 11, "sab_pipe_x1,l2": 10, "sab_pipe_x2,l2": 7,
-                  "sab_pipe_x0,l4": 41, "sab_pipe_x1,l4": 16, "sab_pipe_x2,l4": 12}
+                  "sab_pipe_x0,l4": 41, "sab_pipe_x1,l4": 16, "sab_pipe_x2,l4": 12,
+                  "sab_mix_x0": 24, "sab_mix_x1": 10, "sab_mix_x0,l2": 11, "sab_mix_x1,l2": 7,
+                  "sab_mix_x0,l4": 41, "sab_mix_x1,l4": 12}
 LANE_PAIR_PERM_FLOOR_S = 24 * 108 * 4 / 2.4e9
 
 
@@ -729,13 +856,19 @@ def main():
         torch.cuda.synchronize()
         del x, y
     eng = D.Engine(0)
-    progress(f"saber batch {args.saber_batch}: {'autotune, ' if not sched else ''}"
+    ipw_fixed = sched[2] if sched and len(sched) > 2 else None
+    progress(f"saber batch {args.saber_batch}: rolling keygen -> encaps -> decaps steps, "
              f"{args.warmup} warmup + {args.steps} timed graph steps")
-    sab = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
-                            schedule=sched)
-    progress(f"saber: {sab['ops_per_s']:.0f} ops/s with schedule {sab['schedule']['nstreams']}x{sab['schedule']['chunks']}")
-    progress("saber: the headline plan's kernels launched without a graph, with per-kernel HIP events")
-    sab_prof = profile_plan(torch, eng, D.SABER, args.saber_batch, args.steps, 0x44504B454D000001)
+    sab = run_rolling_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
+                               ipw=ipw_fixed)
+    progress(f"saber: {sab['ops_per_s']:.0f} ops/s (product ipw {sab['schedule']['saber_poly_ipw']})")
+    progress("saber: the rolling steps' kernels launched without a graph, with per-kernel HIP events")
+    sab_prof = profile_rolling(torch, eng, D.SABER, args.saber_batch, max(3, args.steps), 0x44504B454D000001,
+                               sab["schedule"]["saber_poly_ipw"])
+    progress(f"saber batch {args.saber_batch}: one batch's keygen -> encaps -> decaps chained inside each step")
+    chained = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
+                                schedule=sched)
+    chained_prof = profile_plan(torch, eng, D.SABER, args.saber_batch, args.steps, 0x44504B454D000001)
     progress("saber eager pass (op by op) with per-kernel HIP events")
     sab_eager = run_workload(torch, eng, D.SABER, args.saber_batch, args.steps, 1, 0x44504B454D000001, True)
     perm_s = measure_perm_latency(eng)
@@ -753,16 +886,33 @@ def main():
         "vs_baseline": None,
         "dtype": "u16/u32 integer (mod 2^13 / 2^16)",
         "data": "synthetic seeded coins",
-        "config": {"workload": f"saber-l3 batch {args.saber_batch}: keygen+encaps+decaps per step",
+        "config": {"workload": f"saber-l3 batch {args.saber_batch}: {args.saber_batch} keygens + "
+                               f"{args.saber_batch} encaps + {args.saber_batch} decaps per step; rolling pipeline "
+                               "over three batch slots (encaps to the keys of the previous step, decaps of the "
+                               "ciphertexts of the previous step)",
                    "global_batch": args.saber_batch, "parallelism": "instances (1 GPU)"},
         "latency_ms": {"p50": round(pct(sab["per_step_ms"], 0.5), 4), "p99": round(pct(sab["per_step_ms"], 0.99), 4),
-                       "what": "per-step (batch) latency"},
-        "roundtrip_ok": sab["roundtrip_ok"] and sab_prof["roundtrip_ok"] and sab_eager["roundtrip_ok"],
+                       "what": "per-step latency (one step = the three ops over 4096 instances each)",
+                       "batch_p50": round(pct(sab["batch_ms"], 0.5), 4),
+                       "batch_p99": round(pct(sab["batch_ms"], 0.99), 4),
+                       "batch_what": "one batch from keygen to decaps: three consecutive steps"},
+        "roundtrip_ok": sab["roundtrip_ok"] and sab_prof["roundtrip_ok"] and sab_eager["roundtrip_ok"]
+                        and chained["roundtrip_ok"] and chained_prof["roundtrip_ok"],
         "schedule": sab["schedule"],
         "eager_ops_per_s": round(sab_eager["ops_per_s"], 1),
         "roofline": roofline_from(sab_prof["kernels"], args.saber_batch, perm_s),
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_prof["kernels"].items()},
         "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
+        "saber_chained": {
+            "workload": f"saber-l3 batch {args.saber_batch}: keygen -> encaps -> decaps of one batch chained "
+                        "inside each step (pipelined launch sequence, six launches)",
+            "ops_per_s": round(chained["ops_per_s"], 1),
+            "ms_per_step": round(chained["wall_s"] * 1e3 / args.steps, 4),
+            "latency_ms": {"p50": round(pct(chained["per_step_ms"], 0.5), 4),
+                           "p99": round(pct(chained["per_step_ms"], 0.99), 4)},
+            "schedule": chained["schedule"],
+            "roofline": roofline_from(chained_prof["kernels"], args.saber_batch, perm_s),
+            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in chained_prof["kernels"].items()}},
     }
     pe = None if args.skip_plain else D.Engine(0, form=D.FORM_PLAIN)
     if not args.skip_per_op:
@@ -835,17 +985,22 @@ def main():
                 "roofline": roofline_from(fs_prof["kernels"], args.frodo_batch),
                 "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fs_prof["kernels"].items()}}
     for name, sc in (("lightsaber", D.LIGHTSABER), ("firesaber", D.FIRESABER)):
-        progress(f"{name} batch {args.saber_batch}: autotuned graph steps")
-        fam = run_plan_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup, 0x44504B454D000008 + sc,
-                                tune_budget_s=20.0, schedule=sched)
-        fam_prof = profile_plan(torch, eng, sc, args.saber_batch, 3, 0x44504B454D000008 + sc)
-        result[name] = {"workload": f"{name} batch {args.saber_batch}: keygen+encaps+decaps per step",
+        progress(f"{name} batch {args.saber_batch}: rolling steps, then the chained form")
+        fam = run_rolling_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup,
+                                   0x44504B454D000008 + sc, ipw=ipw_fixed)
+        fam_prof = profile_rolling(torch, eng, sc, args.saber_batch, 3, 0x44504B454D000008 + sc,
+                                   fam["schedule"]["saber_poly_ipw"])
+        fam_ch = run_plan_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup,
+                                   0x44504B454D000008 + sc, tune_budget_s=20.0, schedule=sched)
+        result[name] = {"workload": f"{name} batch {args.saber_batch}: keygen+encaps+decaps per step (rolling)",
                         "ops_per_s": round(fam["ops_per_s"], 1),
                         "ms_per_step": round(fam["wall_s"] * 1e3 / args.steps, 4),
-                        "roundtrip_ok": fam["roundtrip_ok"] and fam_prof["roundtrip_ok"],
+                        "roundtrip_ok": fam["roundtrip_ok"] and fam_prof["roundtrip_ok"] and fam_ch["roundtrip_ok"],
                         "schedule": {k: v for k, v in fam["schedule"].items() if k != "tuning_ms"},
                         "roofline": roofline_from(fam_prof["kernels"], args.saber_batch, perm_s),
-                        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fam_prof["kernels"].items()}}
+                        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fam_prof["kernels"].items()},
+                        "chained_ops_per_s": round(fam_ch["ops_per_s"], 1),
+                        "chained_schedule": {k: v for k, v in fam_ch["schedule"].items() if k != "tuning_ms"}}
     progress("saber with the chain run as three dependent tasks (no stage pipelining)")
     unpiped = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                 tune_budget_s=20.0, schedule=sched, pipeline=False)

@@ -45,6 +45,20 @@ size_t saber_pipe_scratch_per_inst(int l);
 void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int l, int ipw = 1,
                        int force_reject = 0);
+// Independent keygen (nk instances), encaps (ne) and decaps (nd) tasks of one
+// rank l as one sequence of three launches (saber.hip, sab_mix_*); any count
+// may be 0.  Arrays in the roles of the single-op launchers.
+struct SaberMixDesc {
+  int nk, ne, nd;
+  const uint8_t* kc;
+  uint8_t *pk_k, *sk_k;
+  const uint8_t *pk_e, *ec;
+  uint8_t *ct_e, *ss_e;
+  const uint8_t *ct_d, *sk_d;
+  uint8_t* ss_d;
+};
+size_t saber_mix_scratch_bytes(int l, size_t nk, size_t ne, size_t nd);
+void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int ipw = 1);
 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <vector>
 
 #include "engine_internal.hpp"
@@ -44,6 +45,8 @@ struct Sub {
   std::vector<int> deps;   // sub indices
   std::vector<int> mt;     // non-empty: independent Frodo tasks of one op run as one multi-task launch sequence
   std::vector<size_t> mt_off;  // their scratch offsets
+  int mix[3] = {-1, -1, -1};  // independent Saber-family keygen / enc / dec tasks in one mixed launch sequence
+  bool has_mix() const { return mix[0] >= 0 || mix[1] >= 0 || mix[2] >= 0; }
 };
 
 }  // namespace
@@ -154,6 +157,40 @@ std::vector<int> merge_groups(const dpkem_plan* p, const std::vector<char>& in_c
   return group_of;
 }
 
+// Independent Saber-family tasks of one scheme (no `after`, no dependants,
+// not in a keygen -> enc -> dec chain) share one mixed launch sequence, at
+// most one task per op each (saber_mix_launch: all their hash chains in one
+// launch, all their products in one, the closing hashes in one).  A task that
+// others depend on keeps its chunked launches, so that its dependants can
+// start chunk by chunk.  Groups form in task order; returns, per task, the
+// index of its group in `mixes`, or -1.
+std::vector<int> mix_groups(const dpkem_plan* p, const std::vector<char>& in_chain,
+                            std::vector<std::array<int, 3>>* mixes) {
+  std::vector<int> mix_of(p->tasks.size(), -1);
+  if (p->form != DPKEM_FORM_BATCHED || (p->flags & DPKEM_PLAN_NO_MERGE)) return mix_of;
+  std::vector<char> has_dependant(p->tasks.size(), 0);
+  for (const Task& T : p->tasks)
+    if (T.after >= 0) has_dependant[T.after] = 1;
+  auto eligible = [&](size_t t) {
+    const Task& T = p->tasks[t];
+    return mix_of[t] < 0 && !in_chain[t] && T.after == -1 && !has_dependant[t] && saber_rank_of(T.scheme) != 0;
+  };
+  for (size_t t = 0; t < p->tasks.size(); ++t) {
+    const Task& T = p->tasks[t];
+    if (!eligible(t)) continue;
+    std::array<int, 3> g{-1, -1, -1};
+    g[T.op] = (int)t;
+    for (size_t u = t + 1; u < p->tasks.size(); ++u) {
+      const Task& U = p->tasks[u];
+      if (eligible(u) && U.scheme == T.scheme && g[U.op] < 0) g[U.op] = (int)u;
+    }
+    for (int u : g)
+      if (u >= 0) mix_of[u] = (int)mixes->size();
+    mixes->push_back(g);
+  }
+  return mix_of;
+}
+
 // Split every task into chunks and assign streams and dependencies.
 void schedule(dpkem_plan* p) {
   p->subs.clear();
@@ -168,6 +205,8 @@ void schedule(dpkem_plan* p) {
   }
   std::vector<std::vector<int>> groups;
   const std::vector<int> group_of = merge_groups(p, in_chain, &groups);
+  std::vector<std::array<int, 3>> mixes;
+  const std::vector<int> mix_of = mix_groups(p, in_chain, &mixes);
   int chains = 0;
   size_t off = 0;
   for (size_t t = 0; t < p->tasks.size(); ++t) {
@@ -177,6 +216,33 @@ void schedule(dpkem_plan* p) {
       continue;
     }
     const Task& T = p->tasks[t];
+    if (mix_of[t] >= 0) {
+      const std::array<int, 3>& g = mixes[mix_of[t]];
+      int first = (int)t;
+      for (int u : g)
+        if (u >= 0) first = std::min(first, u);
+      if (first != (int)t) {  // a later member: runs in the group's sub-task
+        task_subs[t] = task_subs[first];
+        chain_of[t] = chain_of[first];
+        continue;
+      }
+      chain_of[t] = chains++;
+      Sub s;
+      s.task = (int)t;
+      s.begin = 0;
+      s.count = T.n;
+      s.stream = chain_of[t] % p->nstreams;
+      s.scratch_off = off;
+      size_t cnt[3] = {0, 0, 0};
+      for (int k = 0; k < 3; ++k) {
+        s.mix[k] = g[k];
+        if (g[k] >= 0) cnt[k] = p->tasks[g[k]].n;
+      }
+      off += round_up(saber_mix_scratch_bytes(saber_rank_of(T.scheme), cnt[0], cnt[1], cnt[2]));
+      task_subs[t].push_back((int)p->subs.size());
+      p->subs.push_back(s);
+      continue;
+    }
     if (group_of[t] >= 0) {
       const std::vector<int>& g = groups[group_of[t]];
       if (g[0] != (int)t) {  // a later member: runs in the group's sub-task
@@ -266,6 +332,35 @@ bool issue(dpkem_plan* p, hipStream_t origin) {
     hipStream_t st = p->streams[s.stream];
     for (int d : s.deps)
       if (p->subs[d].stream != s.stream) ok = ok && hipStreamWaitEvent(st, p->subs[d].done, 0) == hipSuccess;
+    if (s.has_mix()) {
+      SaberMixDesc d{};
+      if (s.mix[0] >= 0) {
+        const Task& K = p->tasks[s.mix[0]];
+        d.nk = (int)K.n;
+        d.kc = K.i0;
+        d.pk_k = K.o0;
+        d.sk_k = K.o1;
+      }
+      if (s.mix[1] >= 0) {
+        const Task& E = p->tasks[s.mix[1]];
+        d.ne = (int)E.n;
+        d.pk_e = E.i0;
+        d.ec = E.i1;
+        d.ct_e = E.o0;
+        d.ss_e = E.o1;
+      }
+      if (s.mix[2] >= 0) {
+        const Task& Dd = p->tasks[s.mix[2]];
+        d.nd = (int)Dd.n;
+        d.ct_d = Dd.i0;
+        d.sk_d = Dd.i1;
+        d.ss_d = Dd.o0;
+      }
+      saber_mix_launch(st, saber_rank_of(T.scheme), d, p->scratch + s.scratch_off, p->saber_ipw);
+      ok = ok && hipGetLastError() == hipSuccess;
+      ok = ok && hipEventRecord(s.done, st) == hipSuccess;
+      continue;
+    }
     if (!s.mt.empty()) {
       FrodoMtDesc d[kFrodoMtTasks];
       for (size_t k = 0; k < s.mt.size(); ++k) {

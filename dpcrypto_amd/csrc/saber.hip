@@ -1126,6 +1126,167 @@ __global__ void __launch_bounds__(kX2Threads) sab_pipe_x2(int n, const uint8_t* 
   dec_msg_chain<L>(base + k, p, hpk, S.mq[k], sraw_d, krbuf_d, &S.kq[0][0], 2 * kX2Group, lane, &S.kq_ready);
 }
 
+// ----------------------------------------------------------------- independent ops in shared launches
+// Independent keygen, encaps and decaps tasks of one rank (a plan's tasks
+// with no instance-wise dependency between them) run as ONE sequence of
+// three launches instead of three sequences of three or four:
+//   mix_x0    256 threads per workgroup.  Workgroups [0, gke) serve 32
+//             keygen and 32 encaps instances each: wave 0 keygen's seedA and
+//             GenMatrix (the longest chain, 1 + 23 permutations for l = 3),
+//             wave 1 keygen's GenSecret, wave 2 encaps' GenMatrix, wave 3
+//             encaps' m, H(pk), G and GenSecret.  Workgroups [gke, gke + gd)
+//             serve 32 decaps instances: wave 0 GenMatrix; wave 1 H(ct),
+//             then decaps' K for both outcomes of the re-encryption's compare
+//             (the implicit-rejection K at once, K from K-hat' once the
+//             message chain posts it); waves 2 and 3 the inner products on
+//             int8 MFMA (16 instances each), then wave 2 decaps' message
+//             chain (G, GenSecret) for all 32 on lane pairs.
+//   mix_poly  keygen's, encaps' and the re-encryption's products in one grid;
+//             the re-encryption selects decaps' K in constant time.
+//   mix_x1    keygen's H(pk) into sk beside encaps' K.
+// Every op still computes all of its own hashes from its own inputs.  A
+// launch of 256 such workgroups places one per CU and a workgroup's four
+// waves on the CU's four SIMDs, so at 4096 instances of each op every chain
+// of mix_x0 has a SIMD to itself; launched on separate streams the same
+// chains share SIMDs and take longer.
+constexpr int kMixGroup = 32;
+
+struct SabMixArgs {
+  int nk, ne, nd, gke;  // instances per op; workgroups of keygen + encaps
+  const uint8_t* kc;
+  uint8_t *pk_k, *sk_k, *araw_k, *sraw_k;
+  const uint8_t *pk_e, *ec;
+  uint8_t *ct_e, *ss_e, *araw_e, *sraw_e, *mbuf_e, *krbuf_e;
+  const uint8_t *ct_d, *sk_d;
+  uint8_t *ss_d, *araw_d, *sraw_d, *mbuf_d, *krbuf_d, *kk_d;
+};
+
+template <int L>
+struct MixDecLds {
+  InnerLds<L> w[2];
+  alignas(8) uint8_t mq[kMixGroup][32];
+  uint32_t kq[4][2 * kMixGroup];  // decaps' K-hat' halves by lane (wave 2's chain -> wave 1)
+  uint32_t posted, kq_ready;
+};
+
+template <int L>
+__device__ __forceinline__ void mix_dec_group(const SabMixArgs& m, int g, MixDecLds<L>& S) {
+  using P = SaberP<L>;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int base = g * kMixGroup, n = m.nd;
+  if (threadIdx.x == 0) {
+    S.posted = 0;
+    S.kq_ready = 0;
+  }
+  __syncthreads();  // the only workgroup barrier
+  if (wave == 0) {
+    dec_hash1_body<L>(g, 0, n, m.sk_d, m.mbuf_d, m.araw_d, m.sraw_d, m.krbuf_d);
+    return;
+  }
+  if (wave == 1) {
+    const uint32_t p = lane & 1;
+    const int inst = base + (lane >> 1);
+    if (inst >= n) return;  // lane pairs of absent instances (lane 0 is always present)
+    uint64_t z[4];
+    const uint8_t* zs = m.sk_d + (size_t)inst * P::sk + P::sk - 32;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) z[i] = ld64(zs + 8 * i);
+    uint32_t a[25], hc[4], mm[8];
+    absorb2_global<kRateSha3_256, P::ct / 8>(a, p, m.ct_d + (size_t)inst * P::ct, kDsSha3);  // H(ct)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      hc[i] = a[i];
+      mm[i] = k2::deint(z[i], p);
+      mm[4 + i] = hc[i];
+    }
+    absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // implicit-rejection K = SHA3-256(z || H(ct))
+    store2_words<4>(a, p, m.kk_d + (size_t)inst * 64 + 32);
+    k2::lds_wait_count(&S.kq_ready, 1u);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mm[i] = S.kq[i][lane];
+    absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // K = SHA3-256(K-hat' || H(ct))
+    store2_words<4>(a, p, m.kk_d + (size_t)inst * 64);
+    return;
+  }
+  // waves 2 and 3: inner products of instances iw, iw + 2, ... (the next
+  // instance's bytes loaded while the current one is multiplied)
+  const int iw = wave - 2;
+  uint64_t hpk[4];
+  if (wave == 2) load_hpk<L>(min(base + (lane >> 1), n - 1), m.sk_d, hpk);
+  InnerRaw<L> raw;
+  {
+    const int ci = min(base + iw, n - 1);
+    load_inner_raw<L>(lane, m.ct_d + (size_t)ci * P::ct, m.sk_d + (size_t)ci * P::sk, raw);
+  }
+#pragma unroll 1
+  for (int k = iw; k < kMixGroup && base + k < n; k += 2) {
+    const int inst = base + k;
+    const InnerRaw<L> cur = raw;
+    if (k + 2 < kMixGroup) {
+      const int ci = min(inst + 2, n - 1);
+      load_inner_raw<L>(lane, m.ct_d + (size_t)ci * P::ct, m.sk_d + (size_t)ci * P::sk, raw);
+    }
+    const uint32_t byte = dec_inner_mfma<L>(lane, cur, S.w[iw]);
+    if (lane < 32) {
+      S.mq[k][lane] = (uint8_t)byte;
+      m.mbuf_d[(size_t)inst * 32 + lane] = (uint8_t)byte;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if (lane == 0) __hip_atomic_fetch_add(&S.posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (wave != 2) return;
+  k2::lds_wait_count(&S.posted, 2u);  // each inner wave posts once after bounded work
+  const uint32_t p = lane & 1;
+  const int k = lane >> 1;
+  if (base + k >= n) return;
+  dec_msg_chain<L>(base + k, p, hpk, S.mq[k], m.sraw_d, m.krbuf_d, &S.kq[0][0], 2 * kMixGroup, lane, &S.kq_ready);
+}
+
+template <int L>
+__global__ void __launch_bounds__(256) sab_mix_x0(SabMixArgs m) {
+  __shared__ MixDecLds<L> S;
+  const int b = blockIdx.x;
+  if (b >= m.gke) {
+    mix_dec_group<L>(m, b - m.gke, S);
+    return;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (wave < 2) keygen_hash1_body<L>(b, wave, m.nk, m.kc, m.pk_k, m.sk_k, m.araw_k, m.sraw_k);
+  else enc_hash1_body<L>(b, wave - 2, m.ne, m.pk_e, m.ec, m.araw_e, m.sraw_e, m.mbuf_e, m.krbuf_e);
+}
+
+// Products of the three ops: workgroups [0, bk) keygen, [bk, bk + be)
+// encaps, the rest the re-encryption; IPW one-wave instances per workgroup.
+template <int L, int IPW>
+__global__ void __launch_bounds__(64 * IPW) __attribute__((amdgpu_waves_per_eu(4)))
+sab_mix_poly(SabPolyArgs ak, SabPolyArgs ae, SabPolyArgs ad, int bk, int be) {
+  __shared__ __align__(16) SabPolyLds<L> S[IPW];
+  const int w = IPW == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tid = IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+  int b = blockIdx.x;
+  if (b < bk) {
+    const int inst = b * IPW + w;
+    if (inst < ak.n) sab_poly_body<kSabKeygen, L>(ak, S[w], tid, inst);
+    return;
+  }
+  b -= bk;
+  if (b < be) {
+    const int inst = b * IPW + w;
+    if (inst < ae.n) sab_poly_body<kSabEnc, L>(ae, S[w], tid, inst);
+    return;
+  }
+  b -= be;
+  const int inst = b * IPW + w;
+  if (inst < ad.n) sab_poly_body<kSabReenc, L>(ad, S[w], tid, inst);
+}
+
+// Closing hashes: wave 0 keygen's H(pk) into sk, wave 1 encaps' K.
+template <int L>
+__global__ void __launch_bounds__(128) sab_mix_x1(SabMixArgs m) {
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) keygen_hash2_body<L>(blockIdx.x, m.nk, m.pk_k, m.sk_k);
+  else hash2_body<L>(blockIdx.x, m.ne, m.ct_e, m.krbuf_e, nullptr, nullptr, m.ss_e);
+}
+
 // ----------------------------------------------------------------- host launchers
 // (declared in launch.hpp; all run on the given stream)
 
@@ -1305,6 +1466,87 @@ void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* 
   if (l == 2) pipe_l<2>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw, force_reject);
   else if (l == 4) pipe_l<4>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw, force_reject);
   else pipe_l<3>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, ipw, force_reject);
+}
+
+// Independent keygen / encaps / decaps tasks of one rank in one launch
+// sequence (sab_mix_x0, sab_mix_poly, sab_mix_x1).  Scratch: eleven arrays,
+// each 256-byte aligned: keygen's GenMatrix and GenSecret streams; encaps'
+// streams, m and kr; decaps' streams, m', kr and its two candidate K.
+template <int L>
+static size_t mix_scratch(size_t nk, size_t ne, size_t nd) {
+  using P = SaberP<L>;
+  auto r = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t M = P::matrix_bytes, S = P::secret_bytes;
+  return r(nk * M) + r(nk * S) + r(ne * M) + r(ne * S) + r(ne * 32) + r(ne * 64) + r(nd * M) + r(nd * S) +
+         r(nd * 32) + r(nd * 64) + r(nd * 64);
+}
+
+size_t saber_mix_scratch_bytes(int l, size_t nk, size_t ne, size_t nd) {
+  return l == 2 ? mix_scratch<2>(nk, ne, nd) : l == 4 ? mix_scratch<4>(nk, ne, nd) : mix_scratch<3>(nk, ne, nd);
+}
+
+template <int L, int IPW>
+static void mix_poly_launch(const char* name, hipStream_t st, const SabPolyArgs& ak, const SabPolyArgs& ae,
+                            const SabPolyArgs& ad) {
+  const int bk = blocks_for(ak.n, IPW), be = blocks_for(ae.n, IPW), bd = blocks_for(ad.n, IPW);
+  DPKEM_LAUNCH(name, (sab_mix_poly<L, IPW>), dim3(bk + be + bd), dim3(64 * IPW), st, ak, ae, ad, bk, be);
+}
+
+template <int L>
+static void mix_l(hipStream_t st, const SaberMixDesc& d, uint8_t* scratch, int ipw) {
+  using P = SaberP<L>;
+  const int nk = d.nk, ne = d.ne, nd = d.nd;
+  size_t off = 0;
+  auto take = [&](size_t per, int n) {
+    uint8_t* q = scratch + off;
+    off += ((size_t)n * per + 255) & ~(size_t)255;
+    return q;
+  };
+  SabMixArgs m{};
+  m.nk = nk;
+  m.ne = ne;
+  m.nd = nd;
+  m.gke = blocks_for(std::max(nk, ne), kMixGroup);
+  m.kc = d.kc;
+  m.pk_k = d.pk_k;
+  m.sk_k = d.sk_k;
+  m.araw_k = take(P::matrix_bytes, nk);
+  m.sraw_k = take(P::secret_bytes, nk);
+  m.pk_e = d.pk_e;
+  m.ec = d.ec;
+  m.ct_e = d.ct_e;
+  m.ss_e = d.ss_e;
+  m.araw_e = take(P::matrix_bytes, ne);
+  m.sraw_e = take(P::secret_bytes, ne);
+  m.mbuf_e = take(32, ne);
+  m.krbuf_e = take(64, ne);
+  m.ct_d = d.ct_d;
+  m.sk_d = d.sk_d;
+  m.ss_d = d.ss_d;
+  m.araw_d = take(P::matrix_bytes, nd);
+  m.sraw_d = take(P::secret_bytes, nd);
+  m.mbuf_d = take(32, nd);
+  m.krbuf_d = take(64, nd);
+  m.kk_d = take(64, nd);
+  const int gd = blocks_for(nd, kMixGroup);
+  if (m.gke + gd == 0) return;
+  DPKEM_LAUNCH(SAB_NAME("sab_mix_x0"), sab_mix_x0<L>, dim3(m.gke + gd), dim3(256), st, m);
+  SabPolyArgs ak{nk, m.araw_k, m.sraw_k, nullptr, 0, nullptr, d.pk_k, d.sk_k, nullptr, nullptr};
+  SabPolyArgs ae{ne, m.araw_e, m.sraw_e, d.pk_e, (size_t)P::pk, m.mbuf_e, d.ct_e, nullptr, nullptr, nullptr};
+  SabPolyArgs ad{nd,      m.araw_d, m.sraw_d, d.sk_d + P::vecq, (size_t)P::sk, m.mbuf_d, nullptr, nullptr, d.ct_d,
+                 nullptr, m.kk_d,   d.ss_d,   0};
+  if (ipw == 4) mix_poly_launch<L, 4>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad);
+  else if (ipw == 2) mix_poly_launch<L, 2>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad);
+  else mix_poly_launch<L, 1>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad);
+  if (nk > 0 || ne > 0)
+    DPKEM_LAUNCH(SAB_NAME("sab_mix_x1"), sab_mix_x1<L>, dim3(m.gke), dim3(128), st, m);
+}
+
+void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int ipw) {
+  if (d.nk < 0 || d.ne < 0 || d.nd < 0) return;
+  if (l == 2) mix_l<2>(st, d, scratch, ipw);
+  else if (l == 4) mix_l<4>(st, d, scratch, ipw);
+  else mix_l<3>(st, d, scratch, ipw);
 }
 
 // Chained keygen -> encaps -> decaps of the same n Saber (l = 3) instances on

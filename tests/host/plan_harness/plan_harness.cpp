@@ -152,6 +152,27 @@ void saber_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* 
   wr(ss2, n * z.ss);
   scr(scratch, carve(n, {m, m, m, s, s, 32, 32, 64, 64, 64}));
 }
+size_t saber_mix_scratch_bytes(int l, size_t nk, size_t ne, size_t nd) {
+  const size_t m = 416 * (size_t)l * l, s = 32 * (size_t)l * (l == 2 ? 10 : l == 3 ? 8 : 6);
+  return a256(nk * m) + a256(nk * s) + a256(ne * m) + a256(ne * s) + a256(ne * 32) + a256(ne * 64) + a256(nd * m) +
+         a256(nd * s) + a256(nd * 32) + a256(nd * 64) + a256(nd * 64);
+}
+void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int) {
+  const Sz& z = kSz[scheme_of_l(l)];
+  if (d.nk < 0 || d.ne < 0 || d.nd < 0 || d.nk + d.ne + d.nd == 0) fakehip::fail("mixed launch with no instances");
+  launch_on(st);
+  rd(d.kc, d.nk * z.kc);
+  wr(d.pk_k, d.nk * z.pk);
+  wr(d.sk_k, d.nk * z.sk);
+  rd(d.pk_e, d.ne * z.pk);
+  rd(d.ec, d.ne * z.ec);
+  wr(d.ct_e, d.ne * z.ct);
+  wr(d.ss_e, d.ne * z.ss);
+  rd(d.ct_d, d.nd * z.ct);
+  rd(d.sk_d, d.nd * z.sk);
+  wr(d.ss_d, d.nd * z.ss);
+  scr(scratch, saber_mix_scratch_bytes(l, d.nk, d.ne, d.nd));
+}
 // ---- Frodo
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
                          uint32_t* e8, bool, int set, int) {

@@ -6,9 +6,15 @@ GPU's bytes with the CPU oracle on the same coins:
 
   C1  FrodoKEM-976: 64 keypairs plus one encaps / decaps each, every byte
       (eager entry points and the pipelined plan)
-  C2  Saber l=3, batch 4096: the headline plan (one chunk, Saber product 1
-      and 4 instances per workgroup), three graph replays with the outputs
-      cleared and unrelated work in between; every instance of every replay
+  C2  Saber l=3, batch 4096: the headline's rolling steps (three batch
+      slots; each step keygen of one slot, encaps of the previous, decaps of
+      the one before, as one mixed launch sequence; product 1, 2 and 4
+      instances per workgroup): outputs cleared before each step, unrelated
+      work between steps, and every instance of every op of every step equal
+      to the oracle; and the chained plan (one batch's keygen -> encaps ->
+      decaps inside each step, one chunk, product 1 and 4 instances per
+      workgroup), three graph replays with the outputs cleared and unrelated
+      work in between; every instance of every replay
   C3  FrodoKEM-976, batch 8192: the pipelined plan on the 1x1 and 8x4
       schedules and with the 1024-lane fused shape; decaps recovers every
       shared secret, and 64 instances (0 and 8191 among them) equal the
@@ -112,8 +118,49 @@ def test_c1_frodo_64_keypairs_then_one_encaps_decaps_each(eng, oracle, torch_cud
 
 
 # ------------------------------------------------------------------ C2
+@pytest.mark.parametrize("ipw", [1, 2, 4])
+def test_c2_saber_rolling_steps_at_4096_every_instance(eng, oracle, torch_cuda, ipw):
+    import bench
+    torch = torch_cuda
+    n = 4096
+    print(f"C2: Saber rolling steps, n={n} per op, ipw={ipw}", flush=True)
+    slots = bench.rolling_slots(torch, SABER, n, 0x44504B454D000001)
+    plans = [bench.build_rolling_plan(eng, SABER, slots, ph, ipw) for ph in range(3)]
+    ref = []
+    for sl in slots:
+        kc, ec = _host(sl["kc"]), _host(sl["ec"])
+        opk, osk = oracle.keypair(SABER, kc, threads=THREADS)
+        oct_, oss = oracle.enc(SABER, opk, ec, threads=THREADS)
+        ref.append((opk, osk, oct_, oss))
+    junk = torch.zeros((n, 96), dtype=torch.uint8, device="cuda")
+    try:
+        for step in range(6):
+            k, e, d = step % 3, (step - 1) % 3, (step - 2) % 3
+            # clear exactly what this step writes: keygen's pk / sk, encaps'
+            # ct / ss, decaps' ss
+            slots[k]["pk"].zero_(); slots[k]["sk"].zero_()
+            if step >= 1:
+                slots[e]["ct"].zero_(); slots[e]["ss"].zero_()
+            if step >= 2:
+                slots[d]["ss2"].zero_()
+            plans[step % 3].launch()
+            torch.cuda.synchronize()
+            _eq(_host(slots[k]["pk"]), ref[k][0], f"keygen pk (step {step}, slot {k})")
+            _eq(_host(slots[k]["sk"]), ref[k][1], f"keygen sk (step {step}, slot {k})")
+            if step >= 1:
+                _eq(_host(slots[e]["ct"]), ref[e][2], f"encaps ct (step {step}, slot {e})")
+                _eq(_host(slots[e]["ss"]), ref[e][3], f"encaps ss (step {step}, slot {e})")
+            if step >= 2:
+                _eq(_host(slots[d]["ss2"]), ref[d][3], f"decaps ss (step {step}, slot {d})")
+            eng.keypair(SABER, junk[:100])
+            eng.sync()
+    finally:
+        for q in plans:
+            q.close()
+
+
 @pytest.mark.parametrize("ipw", [1, 4])
-def test_c2_saber_headline_plan_at_4096_every_instance(eng, oracle, torch_cuda, ipw):
+def test_c2_saber_chained_plan_at_4096_every_instance(eng, oracle, torch_cuda, ipw):
     import bench
     torch = torch_cuda
     n = 4096

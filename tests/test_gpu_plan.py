@@ -304,3 +304,76 @@ def test_forced_rejection_in_pipelined_chain(eng, oracle, torch_cuda, scheme, n,
             assert np.array_equal(a["ss2"].cpu().numpy(), want), f"rejection ss replay {rep}"
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("scheme,sizes,ipw,merge", [
+    (SABER, (300, 257, 130), 1, True),
+    (SABER, (300, 257, 130), 4, True),
+    (SABER, (300, 257, 130), 1, False),
+    (LIGHTSABER, (65, 33, 97), 2, True),
+    (FIRESABER, (31, 70, 45), 4, True),
+    (SABER, (0, 0, 77), 1, True),
+    (SABER, (40, 0, 0), 1, True),
+    (SABER, (0, 33, 0), 4, True),
+    (FIRESABER, (1, 1, 1), 1, True),
+])
+def test_independent_saber_ops_share_one_mixed_launch_sequence(eng, oracle, torch_cuda, scheme, sizes, ipw, merge):
+    """Independent keygen / encaps / decaps tasks of one Saber-family rank
+    (unequal sizes, any of them absent) run as one mixed launch sequence
+    (merge on) or one sequence per task (merge off); every output equals the
+    oracle, decapsulations of tampered ciphertexts included (every third
+    ciphertext has one bit flipped: implicit rejection)."""
+    import dpcrypto_amd as D
+    torch = torch_cuda
+    nk, ne, nd = sizes
+    s = D.SIZES[scheme]
+    print(f"mixed Saber ops: scheme {scheme}, sizes {sizes}, ipw {ipw}, merge {merge}", flush=True)
+    kc = oracle.prng_bytes(900 + nk, max(nk, 1) * s["kcoins"]).reshape(-1, s["kcoins"])[:nk]
+    # keys for encaps and decaps from other coins
+    m = max(ne, nd, 1)
+    kc2 = oracle.prng_bytes(901 + m, m * s["kcoins"]).reshape(m, s["kcoins"])
+    pk2, sk2 = oracle.keypair(scheme, kc2, threads=8)
+    ec = oracle.prng_bytes(902 + m, m * s["ecoins"]).reshape(m, s["ecoins"])
+    ct2, ss2 = oracle.enc(scheme, pk2, ec, threads=8)
+    ct_t = ct2.copy()
+    ct_t[::3, 7] ^= 0x10
+    exp_dec = ss2.copy()
+    for i in range(0, m, 3):
+        exp_dec[i] = np.frombuffer(_rejection_key(scheme, ct_t[i], sk2[i]), dtype=np.uint8)
+    okp, osk = oracle.keypair(scheme, kc, threads=8) if nk else (None, None)
+    dev = {k: _dev(torch, v) for k, v in dict(kc=kc if nk else np.zeros((1, s["kcoins"]), np.uint8), pk2=pk2,
+                                             ec=ec, ct=ct_t, sk2=sk2).items()}
+    out = {"pk": torch.empty((max(nk, 1), s["pk"]), dtype=torch.uint8, device="cuda"),
+           "sk": torch.empty((max(nk, 1), s["sk"]), dtype=torch.uint8, device="cuda"),
+           "ct": torch.empty((m, s["ct"]), dtype=torch.uint8, device="cuda"),
+           "ss": torch.empty((m, s["ss"]), dtype=torch.uint8, device="cuda"),
+           "ss2": torch.empty((m, s["ss"]), dtype=torch.uint8, device="cuda")}
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    plan = eng.plan(nstreams=2, chunks=1, merge=merge)
+    if nk:
+        plan.keypair(scheme, out["pk"][:nk], out["sk"][:nk], dev["kc"][:nk])
+    if ne:
+        plan.enc(scheme, out["ct"][:ne], out["ss"][:ne], dev["pk2"][:ne], dev["ec"][:ne])
+    if nd:
+        plan.dec(scheme, out["ss2"][:nd], dev["ct"][:nd], dev["sk2"][:nd])
+    plan.build()
+    try:
+        for rep in range(2):
+            for t in out.values():
+                t.zero_()
+            torch.cuda.synchronize()
+            plan.launch()
+            torch.cuda.synchronize()
+            if nk:
+                np.testing.assert_array_equal(out["pk"][:nk].cpu().numpy(), okp, err_msg=f"pk, replay {rep}")
+                np.testing.assert_array_equal(out["sk"][:nk].cpu().numpy(), osk, err_msg=f"sk, replay {rep}")
+            if ne:
+                np.testing.assert_array_equal(out["ct"][:ne].cpu().numpy(), ct2[:ne], err_msg=f"ct, replay {rep}")
+                np.testing.assert_array_equal(out["ss"][:ne].cpu().numpy(), ss2[:ne], err_msg=f"ss, replay {rep}")
+            if nd:
+                np.testing.assert_array_equal(out["ss2"][:nd].cpu().numpy(), exp_dec[:nd],
+                                              err_msg=f"decaps ss, replay {rep}")
+    finally:
+        plan.close()
+        eng.set_tuning(D.TUNE_SABER_POLY_IPW, 1)
