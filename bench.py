@@ -12,9 +12,9 @@ slot r - 1 (generated by the previous step) and decapsulates the ciphertexts
 of slot r - 2 (encapsulated by the previous step, with the secret keys of two
 steps before).  Each batch therefore goes through all three ops, one per
 step, and every op consumes what an earlier op produced; within a step the
-three ops are independent, so the plan runs them as one mixed launch sequence
-(all hash chains in one launch, all products in one, the closing hashes in
-one).  Every op recomputes everything from its own inputs (encaps expands the
+three ops are independent and run as two launches (dpkem_roll_*): all hash
+chains of the three ops plus the closing hashes of the previous steps'
+batches (keygen's H(pk), encaps' K) in one, all products in the other.  Every op recomputes everything from its own inputs (encaps expands the
 public matrix from pk, decaps from sk), as separate parties would.
 
 Reported beside it in the same JSON line:
@@ -101,6 +101,11 @@ KERNEL_WORK = {
     "sab_mix_x0,l4": ((1 + 40 + 5) + (40 + 1 + 10 + 1 + 5) + (40 + 11 + 2 + 1 + 5), 4 * 65536),
     "sab_mix_poly,l4": (0, (16 + 20 + 20) * 65536),
     "sab_mix_x1,l4": (10 + 11 + 1, 0),
+    # rolling pipeline step: the mixed hash chains plus the previous batches'
+    # closing hashes (keygen's H(pk), encaps' H(ct) and K); products as mixed
+    "sab_roll_x0": ((1 + 23 + 5) + (23 + 1 + 8 + 1 + 5) + (23 + 9 + 2 + 1 + 5) + (8 + 9 + 1), 3 * 65536),
+    "sab_roll_x0,l2": ((1 + 10 + 4) + (10 + 1 + 5 + 1 + 4) + (10 + 6 + 2 + 1 + 4) + (5 + 6 + 1), 2 * 65536),
+    "sab_roll_x0,l4": ((1 + 40 + 5) + (40 + 1 + 10 + 1 + 5) + (40 + 11 + 2 + 1 + 5) + (10 + 11 + 1), 4 * 65536),
     "sab_pipe_x0,l2": (1 + 10 + 4 + 10 + 10, 0),
     "sab_pipe_x1,l2": (5 + 1 + 1 + 4 + 5, 0),
     "sab_pipe_x2,l2": (7 + 6 + 1 + 4 + 2, 2 * 65536),
@@ -415,41 +420,39 @@ def rolling_slots(torch, scheme, n, seed):
     return [chain_arrays(torch, scheme, n, seed + 7919 * k) for k in range(3)]
 
 
-def build_rolling_plan(eng, scheme, slots, phase, ipw=1, graph=True):
-    """Step `phase` (mod 3) of the rolling pipeline as a graph-captured plan:
-    keygen of slot phase, encaps of slot phase - 1 to the keys the previous
-    step generated, decaps of slot phase - 2 (the ciphertexts the previous
-    step encapsulated, the secret keys of two steps before).  The three tasks
-    are independent within the step, so the plan runs them as one mixed
-    launch sequence.  ipw: the Saber product's instances per workgroup."""
+def build_roll(eng, scheme, slots, ipw=1, graph=True):
+    """The rolling pipeline (dpkem_roll_*) over the three slots: step k
+    generates the keys of slot k % 3, encapsulates to the keys of slot
+    (k - 1) % 3 (generated by step k - 1) and decapsulates the ciphertexts of
+    slot (k - 2) % 3 (encapsulated by step k - 1), and finishes the previous
+    steps' batches (keygen's H(pk), encaps' K) in the same two launches.
+    ipw: the Saber product's instances per workgroup."""
     import dpcrypto_amd as D
     eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
-    k, e, d = slots[phase % 3], slots[(phase - 1) % 3], slots[(phase - 2) % 3]
-    p = eng.plan(nstreams=1, chunks=1, graph=graph)
-    p.keypair(scheme, k["pk"], k["sk"], k["kc"])
-    p.enc(scheme, e["ct"], e["ss"], e["pk"], e["ec"])
-    p.dec(scheme, d["ss2"], d["ct"], d["sk"])
-    return p.build()
+    r = eng.roll(scheme, slots[0]["kc"].shape[0])
+    for k, sl in enumerate(slots):
+        r.bind(k, sl["kc"], sl["ec"], sl["pk"], sl["sk"], sl["ct"], sl["ss"], sl["ss2"])
+    return r.build(graph=graph)
 
 
 def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
-    """Time `steps` steps of the rolling pipeline (each one replay of the
-    phase's graph; three phases cycle).  The product's instances per
-    workgroup (1 / 2 / 4) is picked first by timing two cycles of each.
-    Afterwards two more steps finish the batches still in flight and every
-    slot must hold ss == ss2 (encaps' and decaps' shared secrets)."""
+    """Time `steps` steps of the rolling pipeline (graph replays).  The
+    product's instances per workgroup (1 / 2 / 4) is picked first by timing
+    two cycles of three steps each.  Afterwards two more steps finish the
+    batches still in flight and every slot must hold ss == ss2 (encaps' and
+    decaps' shared secrets)."""
     import statistics
     slots = rolling_slots(torch, scheme, n, seed)
     stream = torch.cuda.current_stream(torch.device("cuda:0"))
     eng.set_stream(stream.cuda_stream)
 
-    def cycle_ms(plans, reps):
+    def cycle_ms(r, reps):
         ts = []
         for _ in range(reps):
             x, y = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             x.record(stream)
-            for q in plans:
-                q.launch()
+            for _ in range(3):
+                r.step()
             y.record(stream)
             y.synchronize()
             ts.append(x.elapsed_time(y) / 3)
@@ -457,61 +460,57 @@ def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
 
     tried = {}
     for c in ([ipw] if ipw else [1, 2, 4]):
-        plans = [build_rolling_plan(eng, scheme, slots, ph, c) for ph in range(3)]
-        cycle_ms(plans, 2)
-        tried[c] = cycle_ms(plans, 4)
-        for q in plans:
-            q.close()
+        r = build_roll(eng, scheme, slots, c)
+        cycle_ms(r, 2)
+        tried[c] = cycle_ms(r, 4)
+        r.close()
     best = min(tried, key=tried.get)
-    plans = [build_rolling_plan(eng, scheme, slots, ph, best) for ph in range(3)]
-    w0 = max(warmup, 3)  # every slot through keygen, encaps and decaps before timing
-    for i in range(w0):
-        plans[i % 3].launch()
+    r = build_roll(eng, scheme, slots, best)
+    for _ in range(max(warmup, 3)):  # every slot through keygen, encaps and decaps before timing
+        r.step()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     stops = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
         starts[i].record(stream)
-        plans[(w0 + i) % 3].launch()
+        r.step()
         stops[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    for i in range(2):
-        plans[(w0 + steps + i) % 3].launch()
+    for _ in range(2):
+        r.step()
     torch.cuda.synchronize()
     per_step = [x.elapsed_time(y) for x, y in zip(starts, stops)]
     ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots)
-    for q in plans:
-        q.close()
+    r.close()
     batch = [sum(per_step[i:i + 3]) for i in range(max(1, len(per_step) - 2))]
     return dict(wall_s=wall, per_step_ms=per_step, batch_ms=batch, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
-                schedule={"plan": "one mixed launch sequence per step", "saber_poly_ipw": best,
+                schedule={"pipeline": "dpkem_roll: two launches per step", "saber_poly_ipw": best,
                           "tuning_ms": {f"ipw{c}": round(v, 4) for c, v in tried.items()}})
 
 
 def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=1):
     """Per-kernel HIP-event times of the rolling steps' kernels, launched
-    without a graph (DPKEM_PLAN_NO_GRAPH: same launches and kernels)."""
+    without a graph (same launches and kernels)."""
     slots = rolling_slots(torch, scheme, n, seed)
     eng.set_stream(torch.cuda.current_stream(torch.device("cuda:0")).cuda_stream)
-    plans = [build_rolling_plan(eng, scheme, slots, ph, ipw, graph=False) for ph in range(3)]
-    for q in plans:
-        q.launch()
+    r = build_roll(eng, scheme, slots, ipw, graph=False)
+    for _ in range(3):
+        r.step()
     torch.cuda.synchronize()
     eng.kernel_stats(reset=True)
     eng.set_profiling(True)
-    for i in range(steps):
-        plans[i % 3].launch()
+    for _ in range(steps):
+        r.step()
     torch.cuda.synchronize()
     eng.set_profiling(False)
     stats = eng.kernel_stats(reset=True)
-    for i in range(2):
-        plans[(steps + i) % 3].launch()
+    for _ in range(2):
+        r.step()
     torch.cuda.synchronize()
     ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots)
-    for q in plans:
-        q.close()
+    r.close()
     return dict(kernels=stats, roundtrip_ok=ok)
 
 
@@ -671,7 +670,8 @@ CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10,
                   "sab_pipe_x0,l2": 11, "sab_pipe_x1,l2": 10, "sab_pipe_x2,l2": 7,
                   "sab_pipe_x0,l4": 41, "sab_pipe_x1,l4": 16, "sab_pipe_x2,l4": 12,
                   "sab_mix_x0": 24, "sab_mix_x1": 10, "sab_mix_x0,l2": 11, "sab_mix_x1,l2": 7,
-                  "sab_mix_x0,l4": 41, "sab_mix_x1,l4": 12}
+                  "sab_mix_x0,l4": 41, "sab_mix_x1,l4": 12,
+                  "sab_roll_x0": 24, "sab_roll_x0,l2": 11, "sab_roll_x0,l4": 41}
 LANE_PAIR_PERM_FLOOR_S = 24 * 108 * 4 / 2.4e9
 
 
@@ -889,7 +889,7 @@ def main():
         "config": {"workload": f"saber-l3 batch {args.saber_batch}: {args.saber_batch} keygens + "
                                f"{args.saber_batch} encaps + {args.saber_batch} decaps per step; rolling pipeline "
                                "over three batch slots (encaps to the keys of the previous step, decaps of the "
-                               "ciphertexts of the previous step)",
+                               "ciphertexts of the previous step; a batch is complete two steps after its keygen)",
                    "global_batch": args.saber_batch, "parallelism": "instances (1 GPU)"},
         "latency_ms": {"p50": round(pct(sab["per_step_ms"], 0.5), 4), "p99": round(pct(sab["per_step_ms"], 0.99), 4),
                        "what": "per-step latency (one step = the three ops over 4096 instances each)",

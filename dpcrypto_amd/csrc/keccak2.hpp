@@ -427,6 +427,46 @@ __device__ __forceinline__ void squeeze2_sliced(uint32_t a[25], uint32_t p, uint
   for (int i = 0; i < REM; ++i) dst[2 * (NB * RW + i) + p] = a[i];
 }
 
+// squeeze2_sliced with a wave-uniform run-time length: kernels whose waves
+// squeeze streams of different lengths share this one call site, i.e. one
+// copy of the unrolled permutation in the instruction cache instead of one
+// per stream.
+template <int RW>
+__device__ __forceinline__ void squeeze2_sliced_rt(uint32_t a[25], uint32_t p, uint32_t* __restrict__ dst, int outw) {
+  const int nb = outw / RW, rem = outw - nb * RW;
+  for (int b = 0; b < nb; ++b) {
+#pragma unroll
+    for (int i = 0; i < RW; ++i) dst[2 * (b * RW + i) + p] = a[i];
+    keccakf2_unrolled(a, p);
+  }
+#pragma unroll
+  for (int i = 0; i < RW; ++i)
+    if (i < rem) dst[2 * (nb * RW + i) + p] = a[i];
+}
+
+// absorb2_staged with a wave-uniform run-time message length (nw words),
+// for the same reason.
+template <int RW, int SLOTS>
+__device__ __forceinline__ void absorb2_staged_rt(uint32_t a[25], uint32_t p, const uint32_t* msg,
+                                                  const uint32_t* ready, int s, int nw, uint64_t ds) {
+  const int nb = nw / RW, rem = nw - nb * RW;
+  zero2(a);
+  for (int b = 0; b < nb; ++b) {
+    lds_wait_count(ready, (uint32_t)(b + 1));
+#pragma unroll
+    for (int i = 0; i < RW; ++i) a[i] ^= msg[(b * RW + i) * SLOTS + s];
+    keccakf2_unrolled(a, p);
+  }
+  if (rem > 0) lds_wait_count(ready, (uint32_t)(nb + 1));
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    if (i < rem) a[i] ^= msg[(nb * RW + i) * SLOTS + s];
+    if (i == rem) a[i] ^= half_const(ds, p);
+  }
+  a[RW - 1] ^= half_const(0x8000000000000000ULL, p);
+  keccakf2(a, p);
+}
+
 // Store full 64-bit output words: the even lane writes the low 32 bits and the
 // odd lane the high 32 bits of each word.
 template <int NW>

@@ -59,6 +59,25 @@ struct SaberMixDesc {
 };
 size_t saber_mix_scratch_bytes(int l, size_t nk, size_t ne, size_t nd);
 void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int ipw = 1);
+// One step of a rolling keygen -> encaps -> decaps pipeline over batch slots
+// of n instances (saber.hip: sab_roll_x0, then the mixed products): keygen
+// of slot r (its product too), encaps of slot r - 1 (to the keys keygen wrote
+// one step earlier; its K follows one step later), decaps of slot r - 2, and
+// the closing hashes of the previous steps: keygen's H(pk) into slot r - 1's
+// sk, encaps' K into slot r - 2's ss.  Each slot has its own scratch of
+// saber_mix_scratch_bytes(l, n, n, n) bytes: scratch_k / _e / _d are those of
+// slots r, r - 1, r - 2.
+struct SaberRollDesc {
+  int n;
+  const uint8_t* kc;  // slot r
+  uint8_t *pk_k, *sk_k;
+  const uint8_t *pk_e, *ec;  // slot r - 1
+  uint8_t *ct_e, *sk_e;
+  const uint8_t *ct_d, *sk_d;  // slot r - 2
+  uint8_t *ss_d, *ss_e_prev;   // decaps' ss, and encaps' ss of the same slot
+  uint8_t *scratch_k, *scratch_e, *scratch_d;
+};
+void saber_roll_launch(hipStream_t st, int l, const SaberRollDesc& d, int ipw = 1);
 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch

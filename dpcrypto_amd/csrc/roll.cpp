@@ -1,0 +1,199 @@
+// Rolling keygen -> encaps -> decaps pipelines (dpkem_roll_*): three batch
+// slots of one Saber-family scheme, one step = sab_roll_x0 + the mixed
+// products (saber_roll_launch), captured once per step mod 3 into a HIP graph
+// and replayed on the engine's stream.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+
+#include <mutex>
+#include <new>
+
+#include "engine_internal.hpp"
+
+using namespace dpkem;
+
+struct dpkem_roll {
+  dpkem_engine* e = nullptr;
+  int scheme = 0, l = 3, ipw = 1;
+  size_t n = 0;
+  struct Slot {
+    const uint8_t *kc = nullptr, *ec = nullptr;
+    uint8_t *pk = nullptr, *sk = nullptr, *ct = nullptr, *ss = nullptr, *ss2 = nullptr;
+  } slot[3];
+  bool bound[3] = {false, false, false};
+  uint8_t* scratch = nullptr;
+  size_t per_slot = 0;
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+  uint64_t steps = 0;
+  unsigned flags = 0;
+  bool built = false;
+};
+
+namespace {
+
+int rank_of(int scheme) {
+  return scheme == DPKEM_SABER ? 3 : scheme == DPKEM_LIGHTSABER ? 2 : scheme == DPKEM_FIRESABER ? 4 : 0;
+}
+
+void release(dpkem_roll* r) {
+  r->built = false;
+  for (int k = 0; k < 3; ++k) {
+    if (r->exec[k]) (void)hipGraphExecDestroy(r->exec[k]);
+    if (r->graph[k]) (void)hipGraphDestroy(r->graph[k]);
+    r->exec[k] = nullptr;
+    r->graph[k] = nullptr;
+  }
+  if (r->scratch) (void)hipFree(r->scratch);
+  r->scratch = nullptr;
+}
+
+// Step `phase` (mod 3): keygen of slot phase, encaps of slot phase - 1,
+// decaps of slot phase - 2.
+void issue(dpkem_roll* r, hipStream_t st, int phase) {
+  const int ks = phase, es = (phase + 2) % 3, ds = (phase + 1) % 3;
+  const auto &K = r->slot[ks], &E = r->slot[es], &Dd = r->slot[ds];
+  SaberRollDesc d{};
+  d.n = (int)r->n;
+  d.kc = K.kc;
+  d.pk_k = K.pk;
+  d.sk_k = K.sk;
+  d.pk_e = E.pk;
+  d.ec = E.ec;
+  d.ct_e = E.ct;
+  d.sk_e = E.sk;
+  d.ct_d = Dd.ct;
+  d.sk_d = Dd.sk;
+  d.ss_d = Dd.ss2;
+  d.ss_e_prev = Dd.ss;
+  d.scratch_k = r->scratch + (size_t)ks * r->per_slot;
+  d.scratch_e = r->scratch + (size_t)es * r->per_slot;
+  d.scratch_d = r->scratch + (size_t)ds * r->per_slot;
+  saber_roll_launch(st, r->l, d, r->ipw);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpkem_roll_create(dpkem_engine* e, int scheme, size_t n, dpkem_roll** out) {
+  if (!e || !out || !rank_of(scheme) || n == 0 || n > (size_t)(1 << 24) || e->form != DPKEM_FORM_BATCHED)
+    return DPKEM_EINVAL;
+  dpkem_roll* r = new (std::nothrow) dpkem_roll();
+  if (!r) return DPKEM_ENOMEM;
+  r->e = e;
+  r->scheme = scheme;
+  r->l = rank_of(scheme);
+  r->n = n;
+  *out = r;
+  return DPKEM_OK;
+}
+
+int dpkem_roll_bind(dpkem_roll* r, int slot, const uint8_t* kcoins, const uint8_t* ecoins, uint8_t* pk, uint8_t* sk,
+                    uint8_t* ct, uint8_t* ss, uint8_t* ss2) {
+  if (!r || r->built || slot < 0 || slot > 2) return DPKEM_EINVAL;
+  const void* ptrs[7] = {kcoins, ecoins, pk, sk, ct, ss, ss2};
+  for (const void* q : ptrs)
+    if (!q || ((uintptr_t)q & 7)) return DPKEM_EINVAL;
+  r->slot[slot] = {kcoins, ecoins, pk, sk, ct, ss, ss2};
+  r->bound[slot] = true;
+  return DPKEM_OK;
+}
+
+int dpkem_roll_build(dpkem_roll* r, unsigned flags) {
+  if (!r || (flags & ~DPKEM_PLAN_NO_GRAPH) || !r->bound[0] || !r->bound[1] || !r->bound[2]) return DPKEM_EINVAL;
+  dpkem_engine* e = r->e;
+  std::lock_guard<std::mutex> lock(e->mu);
+  release(r);
+  r->flags = flags;
+  r->ipw = e->saber_ipw;
+  r->per_slot = (saber_mix_scratch_bytes(r->l, r->n, r->n, r->n) + 255) & ~(size_t)255;
+  if (3 * r->per_slot > e->budget) {
+    snprintf(e->last_err, sizeof e->last_err, "rolling pipeline scratch %zu B exceeds HBM budget %zu B",
+             3 * r->per_slot, e->budget);
+    return DPKEM_EBUDGET;
+  }
+  hipError_t err;
+#define ROLL_TRY(call)                          \
+  do {                                          \
+    err = (call);                               \
+    if (err != hipSuccess) {                    \
+      int rc = record_hip_error(e, err, #call); \
+      release(r);                               \
+      return rc;                                \
+    }                                           \
+  } while (0)
+  ROLL_TRY(hipSetDevice(e->device));
+  ROLL_TRY(hipMalloc(&r->scratch, 3 * r->per_slot));
+  if (!(flags & DPKEM_PLAN_NO_GRAPH)) {
+    for (int ph = 0; ph < 3; ++ph) {
+      hipStream_t cap;
+      ROLL_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+      if (err != hipSuccess) {
+        (void)hipStreamDestroy(cap);
+        int rc = record_hip_error(e, err, "hipStreamBeginCapture");
+        release(r);
+        return rc;
+      }
+      Profiler* saved = tls_profiler;
+      tls_profiler = nullptr;  // no timing events inside the graph
+      issue(r, cap, ph);
+      tls_profiler = saved;
+      const hipError_t lerr = hipGetLastError();
+      err = hipStreamEndCapture(cap, &r->graph[ph]);
+      (void)hipStreamDestroy(cap);
+      if (err != hipSuccess || lerr != hipSuccess) {
+        int rc = record_hip_error(e, err != hipSuccess ? err : lerr, "rolling step capture");
+        release(r);
+        return rc;
+      }
+      ROLL_TRY(hipGraphInstantiate(&r->exec[ph], r->graph[ph], nullptr, nullptr, 0));
+    }
+  }
+#undef ROLL_TRY
+  r->steps = 0;
+  r->built = true;
+  return DPKEM_OK;
+}
+
+int dpkem_roll_step(dpkem_roll* r) {
+  if (!r || !r->built) return DPKEM_EINVAL;
+  dpkem_engine* e = r->e;
+  std::lock_guard<std::mutex> lock(e->mu);
+  hipError_t err = hipSetDevice(e->device);
+  if (err != hipSuccess) return record_hip_error(e, err, "hipSetDevice");
+  const int phase = (int)(r->steps % 3);
+  if (r->flags & DPKEM_PLAN_NO_GRAPH) {
+    tls_profiler = e->profiling ? &e->prof : nullptr;
+    issue(r, e->stream, phase);
+    tls_profiler = nullptr;
+    err = hipGetLastError();
+  } else {
+    err = hipGraphLaunch(r->exec[phase], e->stream);
+  }
+  if (err != hipSuccess) return record_hip_error(e, err, "rolling step");
+  ++r->steps;
+  return DPKEM_OK;
+}
+
+int dpkem_roll_info(dpkem_roll* r, uint64_t* steps, size_t* scratch_bytes) {
+  if (!r) return DPKEM_EINVAL;
+  if (steps) *steps = r->steps;
+  if (scratch_bytes) *scratch_bytes = 3 * r->per_slot;
+  return DPKEM_OK;
+}
+
+int dpkem_roll_destroy(dpkem_roll* r) {
+  if (!r) return DPKEM_EINVAL;
+  {
+    std::lock_guard<std::mutex> lock(r->e->mu);
+    (void)hipSetDevice(r->e->device);
+    (void)hipStreamSynchronize(r->e->stream);
+    release(r);
+  }
+  delete r;
+  return DPKEM_OK;
+}
+
+}  // extern "C"

@@ -125,19 +125,25 @@ __device__ __forceinline__ void enc_m(int inst, uint32_t p, const uint8_t* __res
   store2_words<4>(h, p, mbuf + (size_t)inst * 32);
 }
 
-// Encaps' kr = G(m || H(pk)) from the halves g[0..7] (m, H(pk)), then the
-// GenSecret stream from kr[32:64].
-template <int L>
-__device__ __forceinline__ void enc_g_chain(int inst, uint32_t p, const uint32_t g[8], uint8_t* __restrict__ sraw,
-                                            uint8_t* __restrict__ krbuf) {
-  using P = SaberP<L>;
-  uint32_t a[25];
+// Encaps' kr = G(m || H(pk)) from the halves g[0..7] (m, H(pk)) into krbuf;
+// a then holds the GenSecret sponge of kr[32:64], its first block ready.
+__device__ __forceinline__ void enc_g_absorb(int inst, uint32_t p, const uint32_t g[8], uint8_t* __restrict__ krbuf,
+                                             uint32_t a[25]) {
   absorb2_halves<kRateSha3_512, 8>(a, p, g, kDsSha3);
   uint32_t kr[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) kr[i] = a[i];
   store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
   absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
+}
+
+// enc_g_absorb, then the GenSecret stream.
+template <int L>
+__device__ __forceinline__ void enc_g_chain(int inst, uint32_t p, const uint32_t g[8], uint8_t* __restrict__ sraw,
+                                            uint8_t* __restrict__ krbuf) {
+  using P = SaberP<L>;
+  uint32_t a[25];
+  enc_g_absorb(inst, p, g, krbuf, a);
   squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
       a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
 }
@@ -180,16 +186,14 @@ __device__ __forceinline__ void load_hpk(int inst, const uint8_t* __restrict__ s
   for (int i = 0; i < 4; ++i) hpk[i] = ld64(sk + (size_t)inst * P::sk + P::sk - 64 + 8 * i);
 }
 
-// kq (optional): K-hat' halves also go to kq[i * ks + slot] in LDS, for a
-// wave of the same workgroup that hashes K; the lane of slot 0 then raises
-// the LDS count kq_ready.
-template <int L>
-__device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint64_t hpk[4], const uint8_t* m,
-                                              uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf,
-                                              uint32_t* kq = nullptr, int ks = 0, int slot = 0,
-                                              uint32_t* kq_ready = nullptr) {
-  using P = SaberP<L>;
-  uint32_t a[25];
+// dec_msg_absorb leaves a holding the GenSecret sponge of kr[32:64] (first
+// block ready); dec_msg_chain also squeezes the stream.  kq (optional):
+// K-hat' halves also go to kq[i * ks + slot] in LDS, for a wave of the same
+// workgroup that hashes K; the lane of slot 0 then raises the LDS count
+// kq_ready.
+__device__ __forceinline__ void dec_msg_absorb(int inst, uint32_t p, const uint64_t hpk[4], const uint8_t* m,
+                                               uint8_t* __restrict__ krbuf, uint32_t a[25], uint32_t* kq = nullptr,
+                                               int ks = 0, int slot = 0, uint32_t* kq_ready = nullptr) {
   uint64_t buf[8];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -208,6 +212,16 @@ __device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint64
   }
   store2_words<8>(kr, p, krbuf + (size_t)inst * 64);
   absorb2_halves<kRateShake128, 4>(a, p, kr + 4, kDsShake);
+}
+
+template <int L>
+__device__ __forceinline__ void dec_msg_chain(int inst, uint32_t p, const uint64_t hpk[4], const uint8_t* m,
+                                              uint8_t* __restrict__ sraw, uint8_t* __restrict__ krbuf,
+                                              uint32_t* kq = nullptr, int ks = 0, int slot = 0,
+                                              uint32_t* kq_ready = nullptr) {
+  using P = SaberP<L>;
+  uint32_t a[25];
+  dec_msg_absorb(inst, p, hpk, m, krbuf, a, kq, ks, slot, kq_ready);
   squeeze2_sliced<kRateShake128, P::secret_bytes / 8>(
       a, p, reinterpret_cast<uint32_t*>(sraw + (size_t)inst * P::secret_bytes));
 }
@@ -1159,6 +1173,14 @@ struct SabMixArgs {
   uint8_t *ct_e, *ss_e, *araw_e, *sraw_e, *mbuf_e, *krbuf_e;
   const uint8_t *ct_d, *sk_d;
   uint8_t *ss_d, *araw_d, *sraw_d, *mbuf_d, *krbuf_d, *kk_d;
+  // rolling pipeline only (sab_roll_x0): the closing hashes of the two
+  // previous steps' batches -- keygen's H(pk) into sk_t for the keys pk_t of
+  // one step earlier, encaps' K into ss_t for the ciphertexts ct_t (and kr
+  // in krbuf_t) of two steps earlier; all n = nk = ne = nd instances
+  const uint8_t* pk_t;
+  uint8_t* sk_t;
+  const uint8_t *ct_t, *krbuf_t;
+  uint8_t* ss_t;
 };
 
 template <int L>
@@ -1169,90 +1191,198 @@ struct MixDecLds {
   uint32_t posted, kq_ready;
 };
 
-template <int L>
-__device__ __forceinline__ void mix_dec_group(const SabMixArgs& m, int g, MixDecLds<L>& S) {
+// Every squeeze of mix_x0 (keygen's, encaps' and decaps' GenMatrix, keygen's,
+// encaps' and decaps' GenSecret) goes through one call site with a run-time
+// length, so the kernel holds one copy of the unrolled permutation: with a
+// copy per stream the four waves of a CU ran four ~20 KB loops through the
+// instruction cache.  Each wave first brings its sponge to its first output
+// block (absorb phase, role by role), then squeezes.
+//
+// ROLL (sab_roll_x0, rolling pipeline): keygen's GenSecret moves to wave 3 of
+// the decaps workgroup, after its inner products (about 14 + 6 of the 24
+// permutation-times of the longest chain), and wave 1 of the keygen / encaps
+// workgroup runs the previous batches' closing hashes instead (keygen's
+// H(pk), 8 permutations, then encaps' H(ct) and K, 10): measured per-role
+// times leave every wave within the keygen GenMatrix chain's length, so the
+// closing hashes need no launch of their own.
+template <int L, bool ROLL>
+__device__ __forceinline__ void x0_body(const SabMixArgs& m, MixDecLds<L>& S) {
   using P = SaberP<L>;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int base = g * kMixGroup, n = m.nd;
-  if (threadIdx.x == 0) {
-    S.posted = 0;
-    S.kq_ready = 0;
-  }
-  __syncthreads();  // the only workgroup barrier
-  if (wave == 0) {
-    dec_hash1_body<L>(g, 0, n, m.sk_d, m.mbuf_d, m.araw_d, m.sraw_d, m.krbuf_d);
-    return;
-  }
-  if (wave == 1) {
-    const uint32_t p = lane & 1;
-    const int inst = base + (lane >> 1);
-    if (inst >= n) return;  // lane pairs of absent instances (lane 0 is always present)
-    uint64_t z[4];
-    const uint8_t* zs = m.sk_d + (size_t)inst * P::sk + P::sk - 32;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) z[i] = ld64(zs + 8 * i);
-    uint32_t a[25], hc[4], mm[8];
-    absorb2_global<kRateSha3_256, P::ct / 8>(a, p, m.ct_d + (size_t)inst * P::ct, kDsSha3);  // H(ct)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      hc[i] = a[i];
-      mm[i] = k2::deint(z[i], p);
-      mm[4 + i] = hc[i];
-    }
-    absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // implicit-rejection K = SHA3-256(z || H(ct))
-    store2_words<4>(a, p, m.kk_d + (size_t)inst * 64 + 32);
-    k2::lds_wait_count(&S.kq_ready, 1u);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) mm[i] = S.kq[i][lane];
-    absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // K = SHA3-256(K-hat' || H(ct))
-    store2_words<4>(a, p, m.kk_d + (size_t)inst * 64);
-    return;
-  }
-  // waves 2 and 3: inner products of instances iw, iw + 2, ... (the next
-  // instance's bytes loaded while the current one is multiplied)
-  const int iw = wave - 2;
-  uint64_t hpk[4];
-  if (wave == 2) load_hpk<L>(min(base + (lane >> 1), n - 1), m.sk_d, hpk);
-  InnerRaw<L> raw;
-  {
-    const int ci = min(base + iw, n - 1);
-    load_inner_raw<L>(lane, m.ct_d + (size_t)ci * P::ct, m.sk_d + (size_t)ci * P::sk, raw);
-  }
-#pragma unroll 1
-  for (int k = iw; k < kMixGroup && base + k < n; k += 2) {
-    const int inst = base + k;
-    const InnerRaw<L> cur = raw;
-    if (k + 2 < kMixGroup) {
-      const int ci = min(inst + 2, n - 1);
-      load_inner_raw<L>(lane, m.ct_d + (size_t)ci * P::ct, m.sk_d + (size_t)ci * P::sk, raw);
-    }
-    const uint32_t byte = dec_inner_mfma<L>(lane, cur, S.w[iw]);
-    if (lane < 32) {
-      S.mq[k][lane] = (uint8_t)byte;
-      m.mbuf_d[(size_t)inst * 32 + lane] = (uint8_t)byte;
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  if (lane == 0) __hip_atomic_fetch_add(&S.posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-  if (wave != 2) return;
-  k2::lds_wait_count(&S.posted, 2u);  // each inner wave posts once after bounded work
   const uint32_t p = lane & 1;
-  const int k = lane >> 1;
-  if (base + k >= n) return;
-  dec_msg_chain<L>(base + k, p, hpk, S.mq[k], m.sraw_d, m.krbuf_d, &S.kq[0][0], 2 * kMixGroup, lane, &S.kq_ready);
+  const bool dgrp = (int)blockIdx.x >= m.gke;
+  const int g = dgrp ? (int)blockIdx.x - m.gke : (int)blockIdx.x;
+  const int inst = g * kMixGroup + (lane >> 1);
+  uint32_t a[25];
+  uint8_t* dst;  // this wave's stream (sliced layout)
+  int outw;      // its length in 64-bit words
+  if (!dgrp) {
+    if (wave == 0) {  // keygen: seedA = SHAKE128(seedA_raw, 32) into pk and sk, then GenMatrix
+      if (inst >= m.nk) return;
+      const uint8_t* c = m.kc + (size_t)inst * kSaberKeygenCoins;
+      uint64_t w[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) w[i] = ld64(c + 8 * i);
+      absorb2_regs<kRateShake128, 4>(a, p, w, kDsShake);
+      uint32_t h[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[i] = a[i];
+      store2_words<4>(h, p, m.pk_k + (size_t)inst * P::pk + P::vecp);
+      store2_words<4>(h, p, m.sk_k + (size_t)inst * P::sk + P::vecq + P::vecp);
+      absorb2_halves<kRateShake128, 4>(a, p, h, kDsShake);
+      dst = m.araw_k + (size_t)inst * P::matrix_bytes;
+      outw = P::matrix_bytes / 8;
+    } else if (ROLL && wave == 1) {  // the closing hashes of the previous batches
+      if (inst >= m.nk) return;
+      absorb2_global<kRateSha3_256, P::pk / 8>(a, p, m.pk_t + (size_t)inst * P::pk, kDsSha3);  // keygen's H(pk)
+      store2_words<4>(a, p, m.sk_t + (size_t)inst * P::sk + P::sk - 64);
+      uint64_t kw[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) kw[i] = ld64(m.krbuf_t + (size_t)inst * 64 + 8 * i);
+      absorb2_global<kRateSha3_256, P::ct / 8>(a, p, m.ct_t + (size_t)inst * P::ct, kDsSha3);  // encaps' H(ct)
+      uint32_t mm[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        mm[i] = k2::deint(kw[i], p);
+        mm[4 + i] = a[i];
+      }
+      absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // K = SHA3-256(kr[0:32] || H(ct))
+      store2_words<4>(a, p, m.ss_t + (size_t)inst * kSaberSs);
+      return;
+    } else if (wave == 1) {  // keygen: GenSecret stream, z into sk
+      if (inst >= m.nk) return;
+      const uint8_t* c = m.kc + (size_t)inst * kSaberKeygenCoins;
+      uint64_t w[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) w[i] = ld64(c + 32 + 8 * i);
+      absorb2_regs<kRateShake128, 4>(a, p, w, kDsShake);
+      uint32_t* z = reinterpret_cast<uint32_t*>(m.sk_k + (size_t)inst * P::sk + P::sk - 32);
+      const uint32_t* zc = reinterpret_cast<const uint32_t*>(c + 64);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) z[2 * i + p] = zc[2 * i + p];
+      dst = m.sraw_k + (size_t)inst * P::secret_bytes;
+      outw = P::secret_bytes / 8;
+    } else if (wave == 2) {  // encaps: GenMatrix from the seedA bytes of pk
+      if (inst >= m.ne) return;
+      uint64_t w[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) w[i] = ld64(m.pk_e + (size_t)inst * P::pk + P::vecp + 8 * i);
+      absorb2_regs<kRateShake128, 4>(a, p, w, kDsShake);
+      dst = m.araw_e + (size_t)inst * P::matrix_bytes;
+      outw = P::matrix_bytes / 8;
+    } else {  // encaps: m, H(pk), G, then the GenSecret stream
+      if (inst >= m.ne) return;
+      uint32_t gw[8];
+      enc_m(inst, p, m.ec, m.mbuf_e, gw);
+      absorb2_global<kRateSha3_256, P::pk / 8>(a, p, m.pk_e + (size_t)inst * P::pk, kDsSha3);  // H(pk)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) gw[4 + i] = a[i];
+      enc_g_absorb(inst, p, gw, m.krbuf_e, a);
+      dst = m.sraw_e + (size_t)inst * P::secret_bytes;
+      outw = P::secret_bytes / 8;
+    }
+  } else {
+    const int base = g * kMixGroup, n = m.nd;
+    if (threadIdx.x == 0) {
+      S.posted = 0;
+      S.kq_ready = 0;
+    }
+    __syncthreads();  // the only workgroup barrier (decaps workgroups)
+    if (wave == 1) {  // H(ct), then decaps' K for both outcomes of the re-encryption's compare
+      if (inst >= n) return;  // lane pairs of absent instances (lane 0 is always present)
+      uint64_t z[4];
+      const uint8_t* zs = m.sk_d + (size_t)inst * P::sk + P::sk - 32;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) z[i] = ld64(zs + 8 * i);
+      uint32_t hc[4], mm[8];
+      absorb2_global<kRateSha3_256, P::ct / 8>(a, p, m.ct_d + (size_t)inst * P::ct, kDsSha3);  // H(ct)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        hc[i] = a[i];
+        mm[i] = k2::deint(z[i], p);
+        mm[4 + i] = hc[i];
+      }
+      absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // implicit-rejection K = SHA3-256(z || H(ct))
+      store2_words<4>(a, p, m.kk_d + (size_t)inst * 64 + 32);
+      k2::lds_wait_count(&S.kq_ready, 1u);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) mm[i] = S.kq[i][lane];
+      absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // K = SHA3-256(K-hat' || H(ct))
+      store2_words<4>(a, p, m.kk_d + (size_t)inst * 64);
+      return;
+    }
+    if (wave == 0) {  // decaps: GenMatrix from the seedA bytes of sk
+      if (inst >= n) return;
+      uint64_t w[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) w[i] = ld64(m.sk_d + (size_t)inst * P::sk + P::vecq + P::vecp + 8 * i);
+      absorb2_regs<kRateShake128, 4>(a, p, w, kDsShake);
+      dst = m.araw_d + (size_t)inst * P::matrix_bytes;
+      outw = P::matrix_bytes / 8;
+    } else {
+      // waves 2 and 3: inner products of instances iw, iw + 2, ... (the next
+      // instance's bytes loaded while the current one is multiplied); then
+      // wave 2 runs decaps' message chain (G, GenSecret) for all 32
+      const int iw = wave - 2;
+      uint64_t hpk[4];
+      if (wave == 2) load_hpk<L>(min(inst, n - 1), m.sk_d, hpk);
+      InnerRaw<L> raw;
+      {
+        const int ci = min(base + iw, n - 1);
+        load_inner_raw<L>(lane, m.ct_d + (size_t)ci * P::ct, m.sk_d + (size_t)ci * P::sk, raw);
+      }
+#pragma unroll 1
+      for (int k = iw; k < kMixGroup && base + k < n; k += 2) {
+        const int ki = base + k;
+        const InnerRaw<L> cur = raw;
+        if (k + 2 < kMixGroup) {
+          const int ci = min(ki + 2, n - 1);
+          load_inner_raw<L>(lane, m.ct_d + (size_t)ci * P::ct, m.sk_d + (size_t)ci * P::sk, raw);
+        }
+        const uint32_t byte = dec_inner_mfma<L>(lane, cur, S.w[iw]);
+        if (lane < 32) {
+          S.mq[k][lane] = (uint8_t)byte;
+          m.mbuf_d[(size_t)ki * 32 + lane] = (uint8_t)byte;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane == 0) __hip_atomic_fetch_add(&S.posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (ROLL && wave == 3) {  // keygen: GenSecret stream, z into sk
+        if (inst >= m.nk) return;
+        const uint8_t* c = m.kc + (size_t)inst * kSaberKeygenCoins;
+        uint64_t w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = ld64(c + 32 + 8 * i);
+        absorb2_regs<kRateShake128, 4>(a, p, w, kDsShake);
+        uint32_t* z = reinterpret_cast<uint32_t*>(m.sk_k + (size_t)inst * P::sk + P::sk - 32);
+        const uint32_t* zc = reinterpret_cast<const uint32_t*>(c + 64);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[2 * i + p] = zc[2 * i + p];
+        dst = m.sraw_k + (size_t)inst * P::secret_bytes;
+        outw = P::secret_bytes / 8;
+      } else {
+        if (wave != 2) return;
+        k2::lds_wait_count(&S.posted, 2u);  // each inner wave posts once after bounded work
+        if (inst >= n) return;
+        dec_msg_absorb(inst, p, hpk, S.mq[lane >> 1], m.krbuf_d, a, &S.kq[0][0], 2 * kMixGroup, lane, &S.kq_ready);
+        dst = m.sraw_d + (size_t)inst * P::secret_bytes;
+        outw = P::secret_bytes / 8;
+      }
+    }
+  }
+  k2::squeeze2_sliced_rt<kRateShake128>(a, p, reinterpret_cast<uint32_t*>(dst), outw);
 }
 
 template <int L>
 __global__ void __launch_bounds__(256) sab_mix_x0(SabMixArgs m) {
   __shared__ MixDecLds<L> S;
-  const int b = blockIdx.x;
-  if (b >= m.gke) {
-    mix_dec_group<L>(m, b - m.gke, S);
-    return;
-  }
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (wave < 2) keygen_hash1_body<L>(b, wave, m.nk, m.kc, m.pk_k, m.sk_k, m.araw_k, m.sraw_k);
-  else enc_hash1_body<L>(b, wave - 2, m.ne, m.pk_e, m.ec, m.araw_e, m.sraw_e, m.mbuf_e, m.krbuf_e);
+  x0_body<L, false>(m, S);
+}
+
+template <int L>
+__global__ void __launch_bounds__(256) sab_roll_x0(SabMixArgs m) {
+  __shared__ MixDecLds<L> S;
+  x0_body<L, true>(m, S);
 }
 
 // Products of the three ops: workgroups [0, bk) keygen, [bk, bk + be)
@@ -1280,11 +1410,67 @@ sab_mix_poly(SabPolyArgs ak, SabPolyArgs ae, SabPolyArgs ad, int bk, int be) {
   if (inst < ad.n) sab_poly_body<kSabReenc, L>(ad, S[w], tid, inst);
 }
 
-// Closing hashes: wave 0 keygen's H(pk) into sk, wave 1 encaps' K.
+// Closing hashes of 32 keygen and 32 encaps instances per workgroup: wave 0
+// keygen's H(pk) into sk, wave 1 encaps' H(ct) and K; waves 2 and 3
+// deinterleave pk and ct into LDS block by block ahead of them
+// (stage2_message), so the two chains absorb without conversion work, through
+// one shared absorb call site (one copy of the unrolled permutation).
 template <int L>
-__global__ void __launch_bounds__(128) sab_mix_x1(SabMixArgs m) {
-  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) keygen_hash2_body<L>(blockIdx.x, m.nk, m.pk_k, m.sk_k);
-  else hash2_body<L>(blockIdx.x, m.ne, m.ct_e, m.krbuf_e, nullptr, nullptr, m.ss_e);
+struct MixX1Lds {
+  static constexpr int kPkBlocks = (SaberP<L>::pk / 8 + kRateSha3_256 - 1) / kRateSha3_256;
+  static constexpr int kCtBlocks = (SaberP<L>::ct / 8 + kRateSha3_256 - 1) / kRateSha3_256;
+  uint32_t pkm[kPkBlocks * kRateSha3_256 * 2 * kMixGroup];
+  uint32_t ctm[kCtBlocks * kRateSha3_256 * 2 * kMixGroup];
+  uint32_t pk_ready, ct_ready;
+};
+
+template <int L>
+__global__ void __launch_bounds__(256) sab_mix_x1(SabMixArgs m) {
+  using P = SaberP<L>;
+  __shared__ MixX1Lds<L> S;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const uint32_t p = lane & 1;
+  const int base = (int)blockIdx.x * kMixGroup, inst = base + (lane >> 1);
+  if (threadIdx.x == 0) {
+    S.pk_ready = 0;
+    S.ct_ready = 0;
+  }
+  __syncthreads();  // the only workgroup barrier
+  if (wave >= 2) {  // helpers; absent instances stage a copy of the last one (unused)
+    const bool kg = wave == 2;
+    const int n = kg ? m.nk : m.ne;
+    if (base >= n) return;
+    const uint8_t* src = kg ? m.pk_k + (size_t)min(inst, n - 1) * P::pk : m.ct_e + (size_t)min(inst, n - 1) * P::ct;
+    if (kg)
+      k2::stage2_message<kRateSha3_256, P::pk / 8, 2 * kMixGroup>(S.pkm, &S.pk_ready, p, lane,
+                                                                  reinterpret_cast<const uint64_t*>(src), lane == 0);
+    else
+      k2::stage2_message<kRateSha3_256, P::ct / 8, 2 * kMixGroup>(S.ctm, &S.ct_ready, p, lane,
+                                                                  reinterpret_cast<const uint64_t*>(src), lane == 0);
+    return;
+  }
+  const bool kg = wave == 0;
+  if (inst >= (kg ? m.nk : m.ne)) return;
+  uint32_t a[25];
+  uint64_t kw[4];  // encaps' kr[0:32], loaded before H(ct) so that its latency is off the chain
+  if (!kg) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) kw[i] = ld64(m.krbuf_e + (size_t)inst * 64 + 8 * i);
+  }
+  k2::absorb2_staged_rt<kRateSha3_256, 2 * kMixGroup>(a, p, kg ? S.pkm : S.ctm, kg ? &S.pk_ready : &S.ct_ready, lane,
+                                                      kg ? P::pk / 8 : P::ct / 8, kDsSha3);
+  if (kg) {
+    store2_words<4>(a, p, m.sk_k + (size_t)inst * P::sk + P::sk - 64);  // H(pk)
+    return;
+  }
+  uint32_t mm[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    mm[i] = k2::deint(kw[i], p);
+    mm[4 + i] = a[i];
+  }
+  absorb2_halves<kRateSha3_256, 8>(a, p, mm, kDsSha3);  // K = SHA3-256(kr[0:32] || H(ct))
+  store2_words<4>(a, p, m.ss_e + (size_t)inst * kSaberSs);
 }
 
 // ----------------------------------------------------------------- host launchers
@@ -1492,16 +1678,58 @@ static void mix_poly_launch(const char* name, hipStream_t st, const SabPolyArgs&
   DPKEM_LAUNCH(name, (sab_mix_poly<L, IPW>), dim3(bk + be + bd), dim3(64 * IPW), st, ak, ae, ad, bk, be);
 }
 
+// The eleven scratch arrays of a mixed sequence, in mix_scratch order.
+struct MixScratch {
+  uint8_t *araw_k, *sraw_k, *araw_e, *sraw_e, *mbuf_e, *krbuf_e, *araw_d, *sraw_d, *mbuf_d, *krbuf_d, *kk_d;
+};
+
 template <int L>
-static void mix_l(hipStream_t st, const SaberMixDesc& d, uint8_t* scratch, int ipw) {
+static MixScratch mix_carve(uint8_t* scratch, int nk, int ne, int nd) {
   using P = SaberP<L>;
-  const int nk = d.nk, ne = d.ne, nd = d.nd;
   size_t off = 0;
   auto take = [&](size_t per, int n) {
     uint8_t* q = scratch + off;
     off += ((size_t)n * per + 255) & ~(size_t)255;
     return q;
   };
+  MixScratch x;
+  x.araw_k = take(P::matrix_bytes, nk);
+  x.sraw_k = take(P::secret_bytes, nk);
+  x.araw_e = take(P::matrix_bytes, ne);
+  x.sraw_e = take(P::secret_bytes, ne);
+  x.mbuf_e = take(32, ne);
+  x.krbuf_e = take(64, ne);
+  x.araw_d = take(P::matrix_bytes, nd);
+  x.sraw_d = take(P::secret_bytes, nd);
+  x.mbuf_d = take(32, nd);
+  x.krbuf_d = take(64, nd);
+  x.kk_d = take(64, nd);
+  return x;
+}
+
+template <int L>
+static void mix_poly_any(const char* name, hipStream_t st, const SabPolyArgs& ak, const SabPolyArgs& ae,
+                         const SabPolyArgs& ad, int ipw) {
+  if (ipw == 4) mix_poly_launch<L, 4>(name, st, ak, ae, ad);
+  else if (ipw == 2) mix_poly_launch<L, 2>(name, st, ak, ae, ad);
+  else mix_poly_launch<L, 1>(name, st, ak, ae, ad);
+}
+
+// The products of a mixed sequence over the arrays of m.
+template <int L>
+static void mix_products(hipStream_t st, const SabMixArgs& m, int ipw) {
+  using P = SaberP<L>;
+  SabPolyArgs ak{m.nk, m.araw_k, m.sraw_k, nullptr, 0, nullptr, m.pk_k, m.sk_k, nullptr, nullptr};
+  SabPolyArgs ae{m.ne, m.araw_e, m.sraw_e, m.pk_e, (size_t)P::pk, m.mbuf_e, m.ct_e, nullptr, nullptr, nullptr};
+  SabPolyArgs ad{m.nd,    m.araw_d, m.sraw_d, m.sk_d + P::vecq, (size_t)P::sk, m.mbuf_d, nullptr, nullptr, m.ct_d,
+                 nullptr, m.kk_d,   m.ss_d,   0};
+  mix_poly_any<L>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad, ipw);
+}
+
+template <int L>
+static void mix_l(hipStream_t st, const SaberMixDesc& d, uint8_t* scratch, int ipw) {
+  const int nk = d.nk, ne = d.ne, nd = d.nd;
+  const MixScratch x = mix_carve<L>(scratch, nk, ne, nd);
   SabMixArgs m{};
   m.nk = nk;
   m.ne = ne;
@@ -1510,36 +1738,77 @@ static void mix_l(hipStream_t st, const SaberMixDesc& d, uint8_t* scratch, int i
   m.kc = d.kc;
   m.pk_k = d.pk_k;
   m.sk_k = d.sk_k;
-  m.araw_k = take(P::matrix_bytes, nk);
-  m.sraw_k = take(P::secret_bytes, nk);
+  m.araw_k = x.araw_k;
+  m.sraw_k = x.sraw_k;
   m.pk_e = d.pk_e;
   m.ec = d.ec;
   m.ct_e = d.ct_e;
   m.ss_e = d.ss_e;
-  m.araw_e = take(P::matrix_bytes, ne);
-  m.sraw_e = take(P::secret_bytes, ne);
-  m.mbuf_e = take(32, ne);
-  m.krbuf_e = take(64, ne);
+  m.araw_e = x.araw_e;
+  m.sraw_e = x.sraw_e;
+  m.mbuf_e = x.mbuf_e;
+  m.krbuf_e = x.krbuf_e;
   m.ct_d = d.ct_d;
   m.sk_d = d.sk_d;
   m.ss_d = d.ss_d;
-  m.araw_d = take(P::matrix_bytes, nd);
-  m.sraw_d = take(P::secret_bytes, nd);
-  m.mbuf_d = take(32, nd);
-  m.krbuf_d = take(64, nd);
-  m.kk_d = take(64, nd);
+  m.araw_d = x.araw_d;
+  m.sraw_d = x.sraw_d;
+  m.mbuf_d = x.mbuf_d;
+  m.krbuf_d = x.krbuf_d;
+  m.kk_d = x.kk_d;
   const int gd = blocks_for(nd, kMixGroup);
   if (m.gke + gd == 0) return;
   DPKEM_LAUNCH(SAB_NAME("sab_mix_x0"), sab_mix_x0<L>, dim3(m.gke + gd), dim3(256), st, m);
-  SabPolyArgs ak{nk, m.araw_k, m.sraw_k, nullptr, 0, nullptr, d.pk_k, d.sk_k, nullptr, nullptr};
-  SabPolyArgs ae{ne, m.araw_e, m.sraw_e, d.pk_e, (size_t)P::pk, m.mbuf_e, d.ct_e, nullptr, nullptr, nullptr};
-  SabPolyArgs ad{nd,      m.araw_d, m.sraw_d, d.sk_d + P::vecq, (size_t)P::sk, m.mbuf_d, nullptr, nullptr, d.ct_d,
-                 nullptr, m.kk_d,   d.ss_d,   0};
-  if (ipw == 4) mix_poly_launch<L, 4>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad);
-  else if (ipw == 2) mix_poly_launch<L, 2>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad);
-  else mix_poly_launch<L, 1>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad);
+  mix_products<L>(st, m, ipw);
   if (nk > 0 || ne > 0)
-    DPKEM_LAUNCH(SAB_NAME("sab_mix_x1"), sab_mix_x1<L>, dim3(m.gke), dim3(128), st, m);
+    DPKEM_LAUNCH(SAB_NAME("sab_mix_x1"), sab_mix_x1<L>, dim3(m.gke), dim3(256), st, m);
+}
+
+// One step of the rolling pipeline (see saber_roll_launch in launch.hpp):
+// sab_roll_x0, then the mixed products.
+template <int L>
+static void roll_l(hipStream_t st, const SaberRollDesc& d, int ipw) {
+  const int n = d.n;
+  const MixScratch xk = mix_carve<L>(d.scratch_k, n, n, n);
+  const MixScratch xe = mix_carve<L>(d.scratch_e, n, n, n);
+  const MixScratch xd = mix_carve<L>(d.scratch_d, n, n, n);
+  SabMixArgs m{};
+  m.nk = m.ne = m.nd = n;
+  m.gke = blocks_for(n, kMixGroup);
+  m.kc = d.kc;
+  m.pk_k = d.pk_k;
+  m.sk_k = d.sk_k;
+  m.araw_k = xk.araw_k;
+  m.sraw_k = xk.sraw_k;
+  m.pk_e = d.pk_e;
+  m.ec = d.ec;
+  m.ct_e = d.ct_e;
+  m.araw_e = xe.araw_e;
+  m.sraw_e = xe.sraw_e;
+  m.mbuf_e = xe.mbuf_e;
+  m.krbuf_e = xe.krbuf_e;
+  m.ct_d = d.ct_d;
+  m.sk_d = d.sk_d;
+  m.ss_d = d.ss_d;
+  m.araw_d = xd.araw_d;
+  m.sraw_d = xd.sraw_d;
+  m.mbuf_d = xd.mbuf_d;
+  m.krbuf_d = xd.krbuf_d;
+  m.kk_d = xd.kk_d;
+  m.pk_t = d.pk_e;         // keygen of one step earlier: the keys encaps uses now
+  m.sk_t = d.sk_e;
+  m.ct_t = d.ct_d;         // encaps of two steps earlier: the ciphertexts decaps uses now
+  m.krbuf_t = xd.krbuf_e;  // its kr, in that slot's encaps scratch
+  m.ss_t = d.ss_e_prev;
+  DPKEM_LAUNCH(SAB_NAME("sab_roll_x0"), sab_roll_x0<L>, dim3(2 * m.gke), dim3(256), st, m);
+  mix_products<L>(st, m, ipw);
+}
+
+void saber_roll_launch(hipStream_t st, int l, const SaberRollDesc& d, int ipw) {
+  if (d.n <= 0) return;
+  if (l == 2) roll_l<2>(st, d, ipw);
+  else if (l == 4) roll_l<4>(st, d, ipw);
+  else roll_l<3>(st, d, ipw);
 }
 
 void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int ipw) {

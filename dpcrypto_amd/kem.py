@@ -97,6 +97,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.dpkem_plan_launch.argtypes = [vp]
         lib.dpkem_plan_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         lib.dpkem_plan_destroy.argtypes = [vp]
+        lib.dpkem_roll_create.argtypes = [vp, i, sz, ctypes.POINTER(vp)]
+        lib.dpkem_roll_bind.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp]
+        lib.dpkem_roll_build.argtypes = [vp, ctypes.c_uint]
+        lib.dpkem_roll_step.argtypes = [vp]
+        lib.dpkem_roll_info.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(sz)]
+        lib.dpkem_roll_destroy.argtypes = [vp]
         lib.dpkem_now_us.restype = ctypes.c_double
         lib.dpkem_server_create.argtypes = [ctypes.POINTER(vp), vp]
         lib.dpkem_server_submit.argtypes = [vp, vp, sz]
@@ -339,6 +345,11 @@ class Engine:
                     "dec_shared_key")
         return ss
 
+    def roll(self, scheme: int, n: int) -> "Roll":
+        """A rolling keygen -> encaps -> decaps pipeline over three batch
+        slots of n instances (dpkem_roll_*; Saber family)."""
+        return Roll(self, scheme, n)
+
     def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False, pipeline: bool = True,
              graph: bool = True, merge: bool = True, force_reject: bool = False) -> "Plan":
         return Plan(self, nstreams, chunks, share_matrix, pipeline, graph, merge, force_reject)
@@ -528,3 +539,58 @@ class Server:
 
 def make_requests(n: int) -> np.ndarray:
     return np.zeros(n, REQ_DTYPE)
+
+
+class Roll:
+    """A rolling keygen -> encaps -> decaps pipeline (dpkem_roll_*): three
+    batch slots of n instances of one Saber-family scheme.  Step k generates
+    the keys of slot k % 3, encapsulates to those of slot (k - 1) % 3 and
+    decapsulates the ciphertexts of slot (k - 2) % 3; after step k >= 2 every
+    output of slot (k - 2) % 3 is complete until step k + 1."""
+
+    def __init__(self, engine: Engine, scheme: int, n: int):
+        self.eng = engine
+        self.lib = engine.lib
+        self.scheme = scheme
+        self.n = n
+        h = ctypes.c_void_p()
+        engine._check(self.lib.dpkem_roll_create(engine.h, scheme, n, ctypes.byref(h)), "roll_create")
+        self.h = h
+        self._keep = {}
+
+    def bind(self, slot: int, kc, ec, pk, sk, ct, ss, ss2):
+        """Device arrays of one slot: keygen coins, encaps coins, pk, sk, ct,
+        encaps' ss, decaps' ss (torch CUDA tensors on the engine's GPU, at
+        least n rows of their role's width)."""
+        n = self.n
+        check_role_arrays(self.scheme, OP_KEYGEN, n, (pk, sk, kc, None), self.eng.device)
+        check_role_arrays(self.scheme, OP_ENC, n, (ct, ss, pk, ec), self.eng.device)
+        check_role_arrays(self.scheme, OP_DEC, n, (ss2, None, ct, sk), self.eng.device)
+        self._keep[slot] = (kc, ec, pk, sk, ct, ss, ss2)
+        self.eng._check(self.lib.dpkem_roll_bind(self.h, slot, _ptr(kc), _ptr(ec), _ptr(pk), _ptr(sk), _ptr(ct),
+                                                 _ptr(ss), _ptr(ss2)), "roll_bind")
+        return self
+
+    def build(self, graph: bool = True):
+        self.eng._check(self.lib.dpkem_roll_build(self.h, 0 if graph else PLAN_NO_GRAPH), "roll_build")
+        return self
+
+    def step(self):
+        self.eng._check(self.lib.dpkem_roll_step(self.h), "roll_step")
+
+    def info(self) -> dict:
+        steps, scr = ctypes.c_uint64(), ctypes.c_size_t()
+        self.eng._check(self.lib.dpkem_roll_info(self.h, ctypes.byref(steps), ctypes.byref(scr)), "roll_info")
+        return {"steps": steps.value, "scratch_bytes": scr.value}
+
+    def close(self):
+        if self.h:
+            self.eng._check(self.lib.dpkem_roll_destroy(self.h), "roll_destroy")
+            self.h = None
+            self._keep = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -173,6 +173,30 @@ void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scr
   wr(d.ss_d, d.nd * z.ss);
   scr(scratch, saber_mix_scratch_bytes(l, d.nk, d.ne, d.nd));
 }
+void saber_roll_launch(hipStream_t st, int l, const SaberRollDesc& d, int) {
+  const Sz& z = kSz[scheme_of_l(l)];
+  const size_t n = (size_t)d.n;
+  if (d.n <= 0) fakehip::fail("rolling step with no instances");
+  launch_on(st);
+  rd(d.kc, n * z.kc);
+  wr(d.pk_k, n * z.pk);
+  wr(d.sk_k, n * z.sk);
+  rd(d.pk_e, n * z.pk);
+  rd(d.ec, n * z.ec);
+  wr(d.ct_e, n * z.ct);
+  wr(d.sk_e, n * z.sk);
+  rd(d.ct_d, n * z.ct);
+  rd(d.sk_d, n * z.sk);
+  wr(d.ss_d, n * z.ss);
+  wr(d.ss_e_prev, n * z.ss);
+  // the three slots' scratch: distinct regions, each written in full (a step
+  // is one launch sequence of its own: only its own regions are compared)
+  g_regions.clear();
+  const size_t per = saber_mix_scratch_bytes(l, n, n, n);
+  scr(d.scratch_k, per);
+  scr(d.scratch_e, per);
+  scr(d.scratch_d, per);
+}
 // ---- Frodo
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
                          uint32_t* e8, bool, int set, int) {
@@ -418,6 +442,31 @@ int main() {
         add_op(p, A, DPKEM_SABER, DPKEM_OP_DEC, 40, u);
       });
       plans += 2;
+    }
+  }
+  // rolling pipelines of every Saber-family rank, with and without graphs,
+  // several steps and a rebuild
+  for (int scheme : {DPKEM_SABER, DPKEM_LIGHTSABER, DPKEM_FIRESABER}) {
+    for (unsigned flags : {0u, DPKEM_PLAN_NO_GRAPH}) {
+      for (size_t n : {(size_t)1, (size_t)37, (size_t)300}) {
+        Arr A;
+        const Sz& z = kSz[scheme];
+        dpkem_roll* r = nullptr;
+        check(dpkem_roll_create(e, scheme, n, &r), "roll_create");
+        for (int k = 0; k < 3; ++k)
+          check(dpkem_roll_bind(r, k, A.get(n * z.kc), A.get(n * z.ec), A.get(n * z.pk), A.get(n * z.sk),
+                                A.get(n * z.ct), A.get(n * z.ss), A.get(n * z.ss)),
+                "roll_bind");
+        for (int b = 0; b < 2; ++b) {
+          check(dpkem_roll_build(r, flags), "roll_build");
+          for (int st = 0; st < 5; ++st) {
+            g_regions.clear();
+            check(dpkem_roll_step(r), "roll_step");
+          }
+        }
+        check(dpkem_roll_destroy(r), "roll_destroy");
+        ++plans;
+      }
     }
   }
   // the plain form (no chains, no merging)

@@ -6,12 +6,12 @@ GPU's bytes with the CPU oracle on the same coins:
 
   C1  FrodoKEM-976: 64 keypairs plus one encaps / decaps each, every byte
       (eager entry points and the pipelined plan)
-  C2  Saber l=3, batch 4096: the headline's rolling steps (three batch
+  C2  Saber l=3, batch 4096: the headline's rolling pipeline (three batch
       slots; each step keygen of one slot, encaps of the previous, decaps of
-      the one before, as one mixed launch sequence; product 1, 2 and 4
-      instances per workgroup): outputs cleared before each step, unrelated
-      work between steps, and every instance of every op of every step equal
-      to the oracle; and the chained plan (one batch's keygen -> encaps ->
+      the one before; product 1, 2 and 4 instances per workgroup): a slot's
+      outputs cleared when its batch starts, unrelated work between steps,
+      and every output of every instance of every batch equal to the oracle
+      two steps later; and the chained plan (one batch's keygen -> encaps ->
       decaps inside each step, one chunk, product 1 and 4 instances per
       workgroup), three graph replays with the outputs cleared and unrelated
       work in between; every instance of every replay
@@ -123,40 +123,36 @@ def test_c2_saber_rolling_steps_at_4096_every_instance(eng, oracle, torch_cuda, 
     import bench
     torch = torch_cuda
     n = 4096
-    print(f"C2: Saber rolling steps, n={n} per op, ipw={ipw}", flush=True)
+    print(f"C2: Saber rolling pipeline, n={n} per slot, ipw={ipw}", flush=True)
     slots = bench.rolling_slots(torch, SABER, n, 0x44504B454D000001)
-    plans = [bench.build_rolling_plan(eng, SABER, slots, ph, ipw) for ph in range(3)]
     ref = []
     for sl in slots:
         kc, ec = _host(sl["kc"]), _host(sl["ec"])
         opk, osk = oracle.keypair(SABER, kc, threads=THREADS)
         oct_, oss = oracle.enc(SABER, opk, ec, threads=THREADS)
         ref.append((opk, osk, oct_, oss))
+    r = bench.build_roll(eng, SABER, slots, ipw)
     junk = torch.zeros((n, 96), dtype=torch.uint8, device="cuda")
     try:
-        for step in range(6):
-            k, e, d = step % 3, (step - 1) % 3, (step - 2) % 3
-            # clear exactly what this step writes: keygen's pk / sk, encaps'
-            # ct / ss, decaps' ss
-            slots[k]["pk"].zero_(); slots[k]["sk"].zero_()
-            if step >= 1:
-                slots[e]["ct"].zero_(); slots[e]["ss"].zero_()
-            if step >= 2:
-                slots[d]["ss2"].zero_()
-            plans[step % 3].launch()
+        for step in range(8):
+            # a slot's outputs are cleared when its batch starts (its keygen
+            # step) and must all be complete two steps later
+            _clear(torch, slots[step % 3])
+            r.step()
             torch.cuda.synchronize()
-            _eq(_host(slots[k]["pk"]), ref[k][0], f"keygen pk (step {step}, slot {k})")
-            _eq(_host(slots[k]["sk"]), ref[k][1], f"keygen sk (step {step}, slot {k})")
-            if step >= 1:
-                _eq(_host(slots[e]["ct"]), ref[e][2], f"encaps ct (step {step}, slot {e})")
-                _eq(_host(slots[e]["ss"]), ref[e][3], f"encaps ss (step {step}, slot {e})")
             if step >= 2:
-                _eq(_host(slots[d]["ss2"]), ref[d][3], f"decaps ss (step {step}, slot {d})")
+                c = (step - 2) % 3
+                sl, (opk, osk, oct_, oss) = slots[c], ref[c]
+                _eq(_host(sl["pk"]), opk, f"pk (step {step}, slot {c})")
+                _eq(_host(sl["sk"]), osk, f"sk (step {step}, slot {c})")
+                _eq(_host(sl["ct"]), oct_, f"ct (step {step}, slot {c})")
+                _eq(_host(sl["ss"]), oss, f"encaps ss (step {step}, slot {c})")
+                _eq(_host(sl["ss2"]), oss, f"decaps ss (step {step}, slot {c})")
+            # unrelated eager work between steps, on the same engine
             eng.keypair(SABER, junk[:100])
             eng.sync()
     finally:
-        for q in plans:
-            q.close()
+        r.close()
 
 
 @pytest.mark.parametrize("ipw", [1, 4])

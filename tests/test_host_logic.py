@@ -159,11 +159,12 @@ def test_bench_roofline_arithmetic():
 
 @pytest.mark.skipif(CXX is None, reason="no host C++ compiler")
 def test_plan_bookkeeping_under_asan_with_a_capture_checking_fake_runtime(tmp_path):
-    """plan.cpp and engine.cpp built for the CPU against a fake HIP runtime
+    """plan.cpp, roll.cpp and engine.cpp built for the CPU against a fake HIP runtime
     (tests/host/plan_harness) with AddressSanitizer and UBSan: every plan
     shape -- chains of every scheme, uneven mixes with merged Frodo groups
     and dependants, every option flag, several schedules, rebuilds, the
-    plain form -- builds, replays and is destroyed without a memory error,
+    plain form -- and every rolling pipeline shape builds, replays and is
+    destroyed without a memory error,
     without two launches of a plan sharing scratch, and within the rules of
     stream capture (no unjoined work, no waits on events from outside the
     capture, no use of destroyed streams or events)."""
@@ -172,7 +173,8 @@ def test_plan_bookkeeping_under_asan_with_a_capture_checking_fake_runtime(tmp_pa
     csrc = os.path.join(ROOT, "dpcrypto_amd", "csrc")
     r = subprocess.run([CXX, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
                         "-D__HIP_PLATFORM_AMD__", "-I", hdir, os.path.join(hdir, "plan_harness.cpp"),
-                        os.path.join(csrc, "plan.cpp"), os.path.join(csrc, "engine.cpp"), "-o", exe],
+                        os.path.join(csrc, "plan.cpp"), os.path.join(csrc, "roll.cpp"), os.path.join(csrc, "engine.cpp"),
+                        "-o", exe],
                        capture_output=True, text=True)
     if r.returncode != 0 and "sanitize" in r.stderr:
         pytest.skip("sanitizer runtime unavailable: " + r.stderr[-300:])
