@@ -627,8 +627,8 @@ __device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L
     wave_sync();
 
     // 2. CBD of s_j (mu bits per coefficient: HW(low mu/2) - HW(high mu/2))
-    //    into the Toeplitz copies; the matrix polynomials (groups of 8 13-bit
-    //    values) and b_j (groups of 4 10-bit values) into the limb slots
+    //    into the Toeplitz copies; the matrix polynomials and b_j (groups of 4
+    //    13- or 10-bit values, one group per lane per polynomial) into the limb slots
     {
       const int g = tid;  // coefficients 4g .. 4g + 3 from mu / 2 bytes
       const uint64_t v = lds_bytes64(reinterpret_cast<const uint8_t*>(S.u.k.sstage), g * (P::mu / 2));
@@ -643,29 +643,25 @@ __device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L
       }
       toep8_put_wave(S.u.k.V, g, pv, nv);
     }
-    const uint8_t* mb = reinterpret_cast<const uint8_t*>(S.u.k.mstage);
+    {
+      // coefficients 4q .. 4q + 3 (q = lane) of each matrix polynomial: the
+      // 52 bits at bit 52 q of its 416 bytes, i.e. byte 6.5 q, shift 0 or 4
+      // (both fixed per lane), read as three dwords and aligned
+      const uint8_t* mb = reinterpret_cast<const uint8_t*>(S.u.k.mstage);
+      const int q = tid, off = (52 * q) >> 3, sh = off & 3;
+      const uint32_t bs = (52 * q) & 7;
+      const uint32_t* d = reinterpret_cast<const uint32_t*>(mb + (off & ~3));
 #pragma unroll
-    for (int r = 0; r < (32 * L + NT - 1) / NT; ++r) {
-      const int g = tid + r * NT;
-      if (g < 32 * L) {
-        // bytes 13 g .. 13 g + 12 as four dwords w0..w3 (little-endian), then
-        // the eight 13-bit fields at bits 13 i
-        const int off = 13 * g;
-        const uint32_t* d = reinterpret_cast<const uint32_t*>(mb + (off & ~3));
-        const uint32_t d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], sh = off & 3;
+      for (int col = 0; col < L; ++col) {
+        const uint32_t d0 = d[104 * col], d1 = d[104 * col + 1], d2 = d[104 * col + 2];
         const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, sh), w1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
-        const uint32_t w2 = __builtin_amdgcn_alignbyte(d3, d2, sh), w3 = __builtin_amdgcn_alignbyte(0u, d3, sh);
-        const uint32_t x0 = w0 & 0x1FFF, x1 = (w0 >> 13) & 0x1FFF;
-        const uint32_t x2 = __builtin_amdgcn_alignbit(w1, w0, 26) & 0x1FFF, x3 = (w1 >> 7) & 0x1FFF;
-        const uint32_t x4 = __builtin_amdgcn_alignbit(w2, w1, 20) & 0x1FFF, x5 = (w2 >> 1) & 0x1FFF;
-        const uint32_t x6 = (w2 >> 14) & 0x1FFF, x7 = __builtin_amdgcn_alignbit(w3, w2, 27) & 0x1FFF;
-        uint32_t lo0, hi0, lo1, hi1;
-        limbs8x4(x0 | (x1 << 16), x2 | (x3 << 16), &lo0, &hi0);
-        limbs8x4(x4 | (x5 << 16), x6 | (x7 << 16), &lo1, &hi1);
-        const int col = g >> 5;
-        int8_t* blk = S.u.k.Lb + 8 * (g & 31);
-        *reinterpret_cast<uint2*>(blk + col * kLbSlot) = make_uint2(lo0, lo1);
-        *reinterpret_cast<uint2*>(blk + (L + 1 + col) * kLbSlot) = make_uint2(hi0, hi1);
+        const uint32_t lo = __builtin_amdgcn_alignbit(w1, w0, bs), hi = w1 >> bs;
+        const uint32_t x0 = lo & 0x1FFF, x1 = (lo >> 13) & 0x1FFF;
+        const uint32_t x2 = __builtin_amdgcn_alignbit(hi, lo, 26) & 0x1FFF, x3 = (hi >> 7) & 0x1FFF;
+        uint32_t l4, h4;
+        limbs8x4(x0 | (x1 << 16), x2 | (x3 << 16), &l4, &h4);
+        *reinterpret_cast<uint32_t*>(S.u.k.Lb + col * kLbSlot + 4 * q) = l4;
+        *reinterpret_cast<uint32_t*>(S.u.k.Lb + (L + 1 + col) * kLbSlot + 4 * q) = h4;
       }
     }
     if (HAS_B) {
@@ -723,47 +719,51 @@ __device__ __forceinline__ void sab_poly_body(const SabPolyArgs& a, SabPolyLds<L
   wave_sync();
 
   // 4. rounding + packing into the LDS output image: rows 0..L-1 are b' (or
-  //    keygen's b), row L the inner product v'; 8 coefficients per item
-  constexpr int ROWS = HAS_B ? L + 1 : L;
+  //    keygen's b), row L the inner product v'
   uint8_t* outb = S.u.out.outb;
-#pragma unroll 1
-  for (int x = tid; x < 32 * ROWS; x += NT) {
-    const int row = x >> 5, grp = x & 31;
-    uint32_t acc8[8];
-    const uint16_t* plo = S.u.out.part + row * kPartSlot + 8 * grp;
-    const uint16_t* phi = S.u.out.part + (L + 1 + row) * kPartSlot + 8 * grp;
+  {
+    // rows 0..L-1: b' (keygen's b) = (x + h1) >> 3 in 10 bits, where
+    // x = lo + 256 hi (the limb sums): only bits 3..12 of x + 4 are kept, so
+    // the sums are combined and rounded mod 2^16 two coefficients per
+    // instruction (16-bit lanes); lane q packs coefficients 4q .. 4q + 3 of
+    // each row into the 5 bytes at 5q
+    const int q = tid;
+    const u16x2 k4 = {4, 4}, k3 = {3, 3}, k8 = {8, 8};
 #pragma unroll
-    for (int d = 0; d < 8; ++d) acc8[d] = (uint32_t)plo[d] + 256u * (uint32_t)phi[d];
-    if (row < L) {
-      // b' = (x + h1) >> 3 -> 10 bits, 8 coefficients -> 10 bytes
-      uint64_t lo = 0;
-      uint32_t hi = 0;
-#pragma unroll
-      for (int d = 0; d < 8; ++d) {
-        const uint64_t v = ((acc8[d] + 4u) >> 3) & 0x3FF;
-        const int bit = 10 * d;
-        if (bit < 64) lo |= v << bit;
-        if (bit + 10 > 64) hi |= (uint32_t)(bit >= 64 ? v << (bit - 64) : v >> (64 - bit));
-      }
-      uint8_t* o = outb + row * 320 + 10 * grp;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) o[i] = (uint8_t)(lo >> (8 * i));
-      o[8] = (uint8_t)hi;
-      o[9] = (uint8_t)(hi >> 8);
-    } else {
-      // c_m = (v' - m 2^9 + h1) >> (10 - eT) -> eT bits; byte grp of m holds the bits of 8 grp .. 8 grp + 7
-      const uint32_t mbyte = a.mbuf[(size_t)inst * 32 + grp];
-      uint64_t packed = 0;
-#pragma unroll
-      for (int d = 0; d < 8; ++d) {
-        const uint32_t mb = (mbyte >> d) & 1;
-        const uint64_t c = ((acc8[d] - (mb << 9) + 4u) >> (10 - P::et)) & ((1u << P::et) - 1);
-        packed |= c << (P::et * d);
-      }
-      uint8_t* o = outb + P::vecp + P::et * grp;
-#pragma unroll
-      for (int i = 0; i < P::et; ++i) o[i] = (uint8_t)(packed >> (8 * i));
+    for (int row = 0; row < L; ++row) {
+      const uint2 lo = *reinterpret_cast<const uint2*>(S.u.out.part + row * kPartSlot + 4 * q);
+      const uint2 hi = *reinterpret_cast<const uint2*>(S.u.out.part + (L + 1 + row) * kPartSlot + 4 * q);
+      const u16x2 x01 = __builtin_bit_cast(u16x2, lo.x) + (__builtin_bit_cast(u16x2, hi.x) << k8);
+      const u16x2 x23 = __builtin_bit_cast(u16x2, lo.y) + (__builtin_bit_cast(u16x2, hi.y) << k8);
+      const uint32_t v01 = __builtin_bit_cast(uint32_t, (x01 + k4) >> k3) & 0x03FF03FFu;
+      const uint32_t v23 = __builtin_bit_cast(uint32_t, (x23 + k4) >> k3) & 0x03FF03FFu;
+      const uint32_t w = (v01 & 0x3FF) | ((v01 >> 16) << 10) | ((v23 & 0x3FF) << 20) | (v23 >> 16 << 30);
+      uint8_t* o = outb + row * 320 + 5 * q;
+      o[0] = (uint8_t)w;
+      o[1] = (uint8_t)(w >> 8);
+      o[2] = (uint8_t)(w >> 16);
+      o[3] = (uint8_t)(w >> 24);
+      o[4] = (uint8_t)(v23 >> 18);
     }
+  }
+  if (HAS_B && tid < 32) {
+    // row L: c_m = (v' - m 2^9 + h1) >> (10 - eT) -> eT bits, 8 coefficients
+    // per lane; byte grp of m holds the bits of coefficients 8 grp .. 8 grp + 7
+    const int grp = tid;
+    const uint16_t* plo = S.u.out.part + L * kPartSlot + 8 * grp;
+    const uint16_t* phi = S.u.out.part + (2 * L + 1) * kPartSlot + 8 * grp;
+    const uint32_t mbyte = a.mbuf[(size_t)inst * 32 + grp];
+    uint64_t packed = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const uint32_t v = (uint32_t)plo[d] + 256u * (uint32_t)phi[d];
+      const uint32_t mb = (mbyte >> d) & 1;
+      const uint64_t c = ((v - (mb << 9) + 4u) >> (10 - P::et)) & ((1u << P::et) - 1);
+      packed |= c << (P::et * d);
+    }
+    uint8_t* o = outb + P::vecp + P::et * grp;
+#pragma unroll
+    for (int i = 0; i < P::et; ++i) o[i] = (uint8_t)(packed >> (8 * i));
   }
   wave_sync();
 
