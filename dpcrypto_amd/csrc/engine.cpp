@@ -142,16 +142,29 @@ void launch_chunk_on(hipStream_t st, int form, int scheme, int op, int n, uint8_
     return;
   }
   if (saber_family(scheme)) {
-    const int l = saber_rank(scheme);
-    const int mu = l == 2 ? 10 : l == 3 ? 8 : 6;
-    uint8_t* araw = c.take((size_t)n * 416 * l * l);  // GenMatrix streams
-    uint8_t* sraw = c.take((size_t)n * 32 * l * mu);  // GenSecret streams
-    uint8_t* mbuf = c.take((size_t)n * 32);
-    uint8_t* krbuf = c.take((size_t)n * 64);
-    uint32_t* fail = reinterpret_cast<uint32_t*>(c.take((size_t)n * 4));
-    if (op == kKeygen) saber_keygen_launch(st, n, i0, o0, o1, araw, sraw, l, saber_ipw);
-    else if (op == kEnc) saber_enc_launch(st, n, i0, i1, o0, o1, araw, sraw, mbuf, krbuf, l, saber_ipw);
-    else saber_dec_launch(st, n, i0, i1, o0, araw, sraw, mbuf, krbuf, fail, l, saber_ipw);
+    // one op as a mixed launch sequence (saber_mix_launch): its hash chains
+    // in one launch, its products in one, keygen's / encaps' closing hashes
+    // in one (decaps: two launches; its K for both compare outcomes is
+    // hashed in the first, the re-encryption selects one)
+    SaberMixDesc d{};
+    if (op == kKeygen) {
+      d.nk = n;
+      d.kc = i0;
+      d.pk_k = o0;
+      d.sk_k = o1;
+    } else if (op == kEnc) {
+      d.ne = n;
+      d.pk_e = i0;
+      d.ec = i1;
+      d.ct_e = o0;
+      d.ss_e = o1;
+    } else {
+      d.nd = n;
+      d.ct_d = i0;
+      d.sk_d = i1;
+      d.ss_d = o0;
+    }
+    saber_mix_launch(st, saber_rank(scheme), d, scratch, saber_ipw);
   } else {
     const int set = frodo_set(scheme);
     uint32_t* s8 = reinterpret_cast<uint32_t*>(c.take((size_t)n * 8 * set));

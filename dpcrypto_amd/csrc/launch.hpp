@@ -7,17 +7,10 @@
 namespace dpkem {
 
 // ---- batched form (saber.hip, frodo.hip)
-// Saber family: l = 2 (LightSaber), 3 (Saber), 4 (FireSaber); araw / sraw
-// hold saber_scratch_per_inst(l) bytes per instance with the rest.  ipw:
-// product instances per workgroup (1, 2 or 4; a tuning knob, results are
-// identical).
+// Saber family: l = 2 (LightSaber), 3 (Saber), 4 (FireSaber).  One op of n
+// instances runs as a mixed sequence (saber_mix_launch below) over
+// saber_scratch_per_inst(l) bytes of scratch per instance.
 size_t saber_scratch_per_inst(int l);
-void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
-                         uint8_t* sraw, int l = 3, int ipw = 1);
-void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l = 3, int ipw = 1);
-void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
-                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l = 3, int ipw = 1);
 
 // Bounded LDS waits of the Saber-family kernels (keccak2.hpp lds_wait_count)
 // on the current device: how many gave up since the last clear, and their
@@ -138,8 +131,7 @@ void plain_saber_launch(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1,
 void plain_frodo_launch(hipStream_t st, int op, int n, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1,
                         uint8_t* scratch);
 
-// Scratch bytes per instance for each (scheme, op) of the batched form.
-constexpr size_t kSaberScratchPerInst = 3744 + 768 + 32 + 64 + 16;
+// Scratch bytes per instance of a FrodoKEM op in the batched form:
 // s8 (8 n) | e8 (8 n) | epp8 64 | kbuf 64 | mubuf 32 | fail 4 | state 200 (+ padding)
 constexpr size_t frodo_scratch_per_inst(int set) { return 16 * (size_t)set + 64 + 64 + 32 + 16 + 200; }
 // scratch of one multi-task entry of n instances (seven arrays, each 256-aligned)

@@ -1493,41 +1493,6 @@ static void poly_launch(const char* name, hipStream_t st, int n, const SabPolyAr
   else DPKEM_LAUNCH(name, (sab_poly<MODE, L, 1, A_PLAIN>), dim3(n), dim3(64), st, a);
 }
 
-template <int L>
-void keygen_l(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw, uint8_t* sraw,
-              int ipw) {
-  DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash1"), sab_keygen_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n,
-               coins, pk, sk, araw, sraw);
-  SabPolyArgs a{n, araw, sraw, nullptr, 0, nullptr, pk, sk, nullptr, nullptr};
-  poly_launch<kSabKeygen, L>(SAB_NAME("sab_poly<kSabKeygen>"), st, n, a, ipw);
-  DPKEM_LAUNCH(SAB_NAME("sab_keygen_hash2"), sab_keygen_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, pk, sk);
-}
-
-template <int L>
-void enc_l(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss, uint8_t* araw,
-           uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int ipw) {
-  using P = SaberP<L>;
-  DPKEM_LAUNCH(SAB_NAME("sab_enc_hash1"), sab_enc_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, pk, coins,
-               araw, sraw, mbuf, krbuf);
-  SabPolyArgs a{n, araw, sraw, pk, (size_t)P::pk, mbuf, ct, nullptr, nullptr, nullptr};
-  poly_launch<kSabEnc, L>(SAB_NAME("sab_poly<kSabEnc>"), st, n, a, ipw);
-  DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, nullptr,
-               nullptr, ss);
-}
-
-template <int L>
-void dec_l(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw, uint8_t* sraw,
-           uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int ipw) {
-  using P = SaberP<L>;
-  DPKEM_LAUNCH(SAB_NAME("sab_dec_inner"), sab_dec_inner<L>, dim3(blocks_for(n, 2)), dim3(64), st, n, ct, sk, mbuf);
-  DPKEM_LAUNCH(SAB_NAME("sab_dec_hash1"), sab_dec_hash1<L>, dim3(blocks_for(n, 32)), dim3(128), st, -1, n, sk, mbuf,
-               araw, sraw, krbuf);
-  SabPolyArgs a{n, araw, sraw, sk + P::vecq, (size_t)P::sk, mbuf, nullptr, nullptr, ct, fail};
-  poly_launch<kSabReenc, L>(SAB_NAME("sab_poly<kSabReenc>"), st, n, a, ipw);
-  DPKEM_LAUNCH(SAB_NAME("sab_hash2"), sab_hash2<L>, dim3(blocks_for(n, 32)), dim3(64), st, n, ct, krbuf, sk, fail,
-               ss);
-}
-
 hipError_t saber_wait_expired(uint32_t* count, bool clear) {
   hipError_t err = hipMemcpyFromSymbol(count, HIP_SYMBOL(k2::wait_expired), sizeof(uint32_t), 0, hipMemcpyDeviceToHost);
   if (err == hipSuccess && clear && *count) {
@@ -1541,34 +1506,12 @@ hipError_t saber_set_wait_spin_limit(uint32_t limit) {
   return hipMemcpyToSymbol(HIP_SYMBOL(k2::wait_spin_limit), &limit, sizeof limit, 0, hipMemcpyHostToDevice);
 }
 
+// Scratch per instance of one op run as a mixed sequence (the largest,
+// decaps: GenMatrix and GenSecret streams, m', kr, both candidate K).
 size_t saber_scratch_per_inst(int l) {
   const size_t m = l == 2 ? SaberP<2>::matrix_bytes : l == 3 ? SaberP<3>::matrix_bytes : SaberP<4>::matrix_bytes;
   const size_t s = l == 2 ? SaberP<2>::secret_bytes : l == 3 ? SaberP<3>::secret_bytes : SaberP<4>::secret_bytes;
-  return m + s + 32 + 64 + 16;
-}
-
-void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
-                         uint8_t* sraw, int l, int ipw) {
-  if (n <= 0) return;
-  if (l == 2) keygen_l<2>(st, n, coins, pk, sk, araw, sraw, ipw);
-  else if (l == 4) keygen_l<4>(st, n, coins, pk, sk, araw, sraw, ipw);
-  else keygen_l<3>(st, n, coins, pk, sk, araw, sraw, ipw);
-}
-
-void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l, int ipw) {
-  if (n <= 0) return;
-  if (l == 2) enc_l<2>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf, ipw);
-  else if (l == 4) enc_l<4>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf, ipw);
-  else enc_l<3>(st, n, pk, coins, ct, ss, araw, sraw, mbuf, krbuf, ipw);
-}
-
-void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
-                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l, int ipw) {
-  if (n <= 0) return;
-  if (l == 2) dec_l<2>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail, ipw);
-  else if (l == 4) dec_l<4>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail, ipw);
-  else dec_l<3>(st, n, ct, sk, ss, araw, sraw, mbuf, krbuf, fail, ipw);
+  return m + s + 32 + 64 + 64;
 }
 
 // dst[i] = src for i < n (bytes per row a multiple of 4).

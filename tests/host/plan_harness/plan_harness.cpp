@@ -77,43 +77,7 @@ namespace dpkem {
 // ---- batched Saber family (per-array pointers carved by the engine)
 size_t saber_scratch_per_inst(int l) {
   const size_t m = 416 * (size_t)l * l, s = 32 * (size_t)l * (l == 2 ? 10 : l == 3 ? 8 : 6);
-  return m + s + 32 + 64 + 16;
-}
-void saber_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint8_t* araw,
-                         uint8_t* sraw, int l, int) {
-  const Sz& z = kSz[scheme_of_l(l)];
-  launch_on(st);
-  rd(coins, n * z.kc);
-  wr(pk, n * z.pk);
-  wr(sk, n * z.sk);
-  scr(araw, (size_t)n * 416 * l * l);
-  scr(sraw, (size_t)n * 32 * l * (l == 2 ? 10 : l == 3 ? 8 : 6));
-}
-void saber_enc_launch(hipStream_t st, int n, const uint8_t* pk, const uint8_t* coins, uint8_t* ct, uint8_t* ss,
-                      uint8_t* araw, uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, int l, int) {
-  const Sz& z = kSz[scheme_of_l(l)];
-  launch_on(st);
-  rd(pk, n * z.pk);
-  rd(coins, n * z.ec);
-  wr(ct, n * z.ct);
-  wr(ss, n * z.ss);
-  scr(araw, (size_t)n * 416 * l * l);
-  scr(sraw, (size_t)n * 32 * l * (l == 2 ? 10 : l == 3 ? 8 : 6));
-  scr(mbuf, (size_t)n * 32);
-  scr(krbuf, (size_t)n * 64);
-}
-void saber_dec_launch(hipStream_t st, int n, const uint8_t* ct, const uint8_t* sk, uint8_t* ss, uint8_t* araw,
-                      uint8_t* sraw, uint8_t* mbuf, uint8_t* krbuf, uint32_t* fail, int l, int) {
-  const Sz& z = kSz[scheme_of_l(l)];
-  launch_on(st);
-  rd(ct, n * z.ct);
-  rd(sk, n * z.sk);
-  wr(ss, n * z.ss);
-  scr(araw, (size_t)n * 416 * l * l);
-  scr(sraw, (size_t)n * 32 * l * (l == 2 ? 10 : l == 3 ? 8 : 6));
-  scr(mbuf, (size_t)n * 32);
-  scr(krbuf, (size_t)n * 64);
-  scr(fail, (size_t)n * 4);
+  return m + s + 32 + 64 + 64;
 }
 void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t bytes, uint8_t* dst) {
   launch_on(st);
