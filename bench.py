@@ -170,99 +170,101 @@ VALU_MACS_PER_OP = {"sab_dec_inner": 2.0, "frodo_dec_mu": 1.0}
 #            --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99969, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-('frodo_as_fused', 8192): (129267144, 256247836),
-    ('frodo_as_fused,1344', 8192): (177476791, 353388441),
-    ('frodo_as_fused,640', 8192): (85214542, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129283856, 256251592),
-    ('frodo_dec_mu', 8192): (241876120, 262144),
-    ('frodo_dec_mu,1344', 8192): (358489887, 262144),
-    ('frodo_dec_mu,640', 8192): (165535445, 262144),
-    ('frodo_dec_pre', 8192): (130937417, 130375680),
-    ('frodo_encdec_pre<false>', 8192): (75037802, 128713197),
-    ('frodo_keygen_post', 8192): (129167106, 262144),
-    ('frodo_keygen_pre', 8192): (1435980, 256980164),
-    ('frodo_keygen_pre,1344', 8192): (1553938, 353242444),
-    ('frodo_keygen_pre,640', 8192): (1336372, 168558592),
-    ('frodo_pipe_ssd', 8192): (4272090, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4532556, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5536947, 131072),
-    ('frodo_pipe_x1', 8192): (130433601, 128975029),
-    ('frodo_pipe_x1,1344', 8192): (178657667, 177340454),
-    ('frodo_pipe_x1,640', 8192): (81066926, 84934656),
-    ('frodo_pipe_x2', 8192): (131987910, 130662033),
-    ('frodo_pipe_x2,1344', 8192): (180324868, 178850547),
-    ('frodo_pipe_x2,640', 8192): (83656073, 86442438),
-    ('frodo_sa_fused<false,aes>', 8192): (259421567, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260402690, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356736017, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166966011, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388235224, 262144),
-    ('frodo_sa_fused<true>', 8192): (389362397, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533915894, 265555),
-    ('frodo_sa_fused<true>,640', 8192): (247571721, 262259),
-    ('frodo_sa_grouped<false>', 8192): (129716996, 128974848),
-    ('frodo_sa_grouped<true>', 8192): (258626264, 65536),
-    ('frodo_ss', 8192): (129680929, 196608),
-    ('plain::copy_k', 8192): (55915911, 55462596),
-    ('plain::fr_as', 8192): (16571025126, 127926272),
-    ('plain::fr_bs_decode', 8192): (270734881, 393216),
-    ('plain::fr_gen_a', 8192): (1384364, 19990052085),
-    ('plain::fr_le_load', 8192): (128978335, 127926272),
-    ('plain::fr_le_store', 8192): (127929279, 128189312),
-    ('plain::fr_pack', 8192): (77187582, 77201913),
-    ('plain::fr_sa', 8192): (18533444916, 131461456),
-    ('plain::fr_sample', 8192): (256537282, 256551594),
-    ('plain::fr_sb', 8192): (265734044, 1048576),
-    ('plain::fr_select', 8192): (1487645, 393216),
-    ('plain::fr_unpack', 8192): (96706267, 96206848),
-    ('plain::sab_cbd', 8192): (3166748, 6291456),
-    ('plain::sab_cm', 8192): (2642540, 524288),
-    ('plain::sab_kdf', 8192): (724881, 131072),
-    ('plain::sab_msg', 8192): (2900549, 131072),
-    ('plain::sab_pack13', 8192): (6299710, 5111808),
-    ('plain::sab_polymul', 8192): (18953523, 4194304),
-    ('plain::sab_round_pack10', 8192): (6314279, 3932160),
-    ('plain::sab_unpack10', 8192): (4443823, 6291456),
-    ('plain::sab_unpack13', 8192): (13141034, 15728640),
-    ('plain::sha3_256_k', 8192): (4186497, 131072),
-    ('plain::sha3_512_64_k', 8192): (327630, 262144),
-    ('plain::shake128_seed32', 8192): (470008, 131072),
-    ('plain::shake256_dom_seed', 8192): (664858, 255852704),
-    ('plain::shake256_k', 8192): (612642, 262144),
-    ('plain::verify_k', 8192): (257935554, 262144),
-    ('sab_dec_hash1', 4096): (1560430, 18879705),
-    ('sab_dec_inner', 4096): (9888313, 131072),
-    ('sab_enc_hash1', 4096): (5316867, 19048532),
-    ('sab_hash2', 4096): (5234913, 131072),
-    ('sab_keygen_hash1', 4096): (1289408, 19006061),
-    ('sab_keygen_hash2', 4096): (4250999, 131072),
-    ('sab_mix_poly', 4096): (72300832, 17575864),
-    ('sab_mix_poly,l2', 4096): (39609539, 11800248),
-    ('sab_mix_poly,l4', 4096): (114894084, 27445207),
-    ('sab_pipe_x0', 4096): (1287550, 49632404),
-    ('sab_pipe_x0,l2', 4096): (1282225, 23488119),
-    ('sab_pipe_x0,l4', 4096): (1304777, 85365894),
-    ('sab_pipe_x1', 4096): (36083963, 34340864),
-    ('sab_pipe_x1,l2', 4096): (17764807, 16777216),
-    ('sab_pipe_x1,l4', 4096): (62093988, 58195968),
-    ('sab_pipe_x2', 4096): (12584063, 3932160),
-    ('sab_pipe_x2,l2', 4096): (9457244, 3407872),
-    ('sab_pipe_x2,l4', 4096): (15914627, 3932160),
-    ('sab_poly<kSabEnc>', 4096): (23842808, 4456448),
-    ('sab_poly<kSabEnc>,l2', 4096): (13076532, 3014656),
-    ('sab_poly<kSabEnc>,l4', 4096): (36667273, 6029312),
-    ('sab_poly<kSabKeygen>', 4096): (18979084, 12979426),
-    ('sab_poly<kSabKeygen>,l2', 4096): (9537104, 8650818),
-    ('sab_poly<kSabKeygen>,l4', 4096): (30522830, 17331243),
-    ('sab_poly<kSabReenc>', 4096): (28968842, 131072),
-    ('sab_reenc', 4096): (29486837, 131072),
-    ('sab_reenc,l2', 4096): (17001949, 131072),
-    ('sab_reenc,l4', 4096): (47702038, 4325376),
-    ('sab_roll_x0', 4096): (18694849, 57338452),
-    ('sab_roll_x0,l2', 4096): (13106665, 30141640),
-    ('sab_roll_x0,l4', 4096): (26740738, 93257069),
+    ('frodo_as_fused', 8192): (129270534, 256248206),
+    ('frodo_as_fused,1344', 8192): (177481982, 353402502),
+    ('frodo_as_fused,640', 8192): (85216777, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129287566, 256251008),
+    ('frodo_dec_mu', 8192): (241917232, 262144),
+    ('frodo_dec_mu,1344', 8192): (358499544, 262144),
+    ('frodo_dec_mu,640', 8192): (165549308, 262144),
+    ('frodo_dec_pre', 8192): (130941427, 130374629),
+    ('frodo_encdec_pre<false>', 8192): (75039770, 128713088),
+    ('frodo_keygen_post', 8192): (129170493, 262144),
+    ('frodo_keygen_pre', 8192): (1435811, 256975241),
+    ('frodo_keygen_pre,1344', 8192): (1553365, 353241804),
+    ('frodo_keygen_pre,640', 8192): (1336099, 168558617),
+    ('frodo_pipe_ssd', 8192): (4272202, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4532675, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5537092, 131072),
+    ('frodo_pipe_x1', 8192): (130437021, 128975121),
+    ('frodo_pipe_x1,1344', 8192): (178662351, 177340582),
+    ('frodo_pipe_x1,640', 8192): (81069051, 84934656),
+    ('frodo_pipe_x2', 8192): (131990503, 130650055),
+    ('frodo_pipe_x2,1344', 8192): (180329315, 178851187),
+    ('frodo_pipe_x2,640', 8192): (83658471, 86442528),
+    ('frodo_sa_fused<false,aes>', 8192): (259429809, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260409533, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356745295, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166970440, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388246684, 262144),
+    ('frodo_sa_fused<true>', 8192): (389372394, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533929536, 265465),
+    ('frodo_sa_fused<true>,640', 8192): (247578238, 262188),
+    ('frodo_sa_grouped<false>', 8192): (129720952, 128974848),
+    ('frodo_sa_grouped<true>', 8192): (258621336, 65536),
+    ('frodo_ss', 8192): (129684330, 196608),
+    ('plain::copy_k', 8192): (55917377, 55462683),
+    ('plain::fr_as', 8192): (16571766952, 127926272),
+    ('plain::fr_bs_decode', 8192): (270778839, 393216),
+    ('plain::fr_gen_a', 8192): (1381244, 19995041898),
+    ('plain::fr_le_load', 8192): (128982613, 127926272),
+    ('plain::fr_le_store', 8192): (127932634, 128188608),
+    ('plain::fr_pack', 8192): (77189734, 77202099),
+    ('plain::fr_sa', 8192): (18560851224, 131442160),
+    ('plain::fr_sample', 8192): (256544009, 256551594),
+    ('plain::fr_sb', 8192): (265737300, 1048576),
+    ('plain::fr_select', 8192): (1487684, 393216),
+    ('plain::fr_unpack', 8192): (96708771, 96206848),
+    ('plain::sab_cbd', 8192): (3166831, 6291456),
+    ('plain::sab_cm', 8192): (2642610, 524288),
+    ('plain::sab_kdf', 8192): (724900, 131072),
+    ('plain::sab_msg', 8192): (2900625, 131072),
+    ('plain::sab_pack13', 8192): (6299875, 5111808),
+    ('plain::sab_polymul', 8192): (18953988, 4194304),
+    ('plain::sab_round_pack10', 8192): (6314423, 3932160),
+    ('plain::sab_unpack10', 8192): (4443939, 6291456),
+    ('plain::sab_unpack13', 8192): (13141379, 15728640),
+    ('plain::sha3_256_k', 8192): (4186606, 131072),
+    ('plain::sha3_512_64_k', 8192): (327638, 262144),
+    ('plain::shake128_seed32', 8192): (469956, 131072),
+    ('plain::shake256_dom_seed', 8192): (664875, 255852608),
+    ('plain::shake256_k', 8192): (612658, 262144),
+    ('plain::verify_k', 8192): (257941805, 262144),
+    ('sab_dec_hash1', 4096): (1177545, 3407872),
+    ('sab_dec_inner', 4096): (9888542, 131072),
+    ('sab_enc_hash1', 4096): (4919698, 3538944),
+    ('sab_hash2', 4096): (5235050, 131072),
+    ('sab_keygen_hash1', 4096): (1289320, 19006102),
+    ('sab_keygen_hash2', 4096): (4251110, 131072),
+    ('sab_mix_poly', 4096): (72279648, 17589173),
+    ('sab_mix_poly,l2', 4096): (39603408, 11801130),
+    ('sab_mix_poly,l4', 4096): (115969978, 28767319),
+    ('sab_mix_x0', 4096): (5775214, 19114686),
+    ('sab_mix_x1', 4096): (4949949, 131072),
+    ('sab_pipe_x0', 4096): (1284783, 49613132),
+    ('sab_pipe_x0,l2', 4096): (1296581, 23500143),
+    ('sab_pipe_x0,l4', 4096): (1304155, 85363404),
+    ('sab_pipe_x1', 4096): (36086029, 34340864),
+    ('sab_pipe_x1,l2', 4096): (17773470, 16777216),
+    ('sab_pipe_x1,l4', 4096): (62019377, 58195968),
+    ('sab_pipe_x2', 4096): (12584591, 3932160),
+    ('sab_pipe_x2,l2', 4096): (9465683, 3407872),
+    ('sab_pipe_x2,l4', 4096): (15915044, 3932160),
+    ('sab_poly<kSabEnc>', 4096): (23842741, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13075851, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (36688389, 6029312),
+    ('sab_poly<kSabKeygen>', 4096): (18983585, 12979343),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9527116, 8650838),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30525678, 17328969),
+    ('sab_poly<kSabReenc>', 4096): (28980251, 131072),
+    ('sab_reenc', 4096): (29471234, 131072),
+    ('sab_reenc,l2', 4096): (17010586, 131072),
+    ('sab_reenc,l4', 4096): (48756830, 5111808),
+    ('sab_roll_x0', 4096): (18666587, 57352466),
+    ('sab_roll_x0,l2', 4096): (13093350, 30139294),
+    ('sab_roll_x0,l4', 4096): (26753228, 93253281),
 }
 
 
@@ -273,7 +275,7 @@ KERNEL_SOURCES = {
     "saber": ("saber.hip", "saber_poly.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
     "frodo": ("frodo.hip", "frodo_sampler.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
 }
-MEASURED_HBM_SOURCE_SHA = {"saber": "52c7144445911721", "frodo": "873a4511a2ed22f5"}
+MEASURED_HBM_SOURCE_SHA = {"saber": "4e19566d8b5a3b55", "frodo": "873a4511a2ed22f5"}
 
 
 def kernel_family(kernel: str) -> str:
