@@ -20,9 +20,14 @@ public matrix from pk, decaps from sk), as separate parties would.
 Reported beside it in the same JSON line:
   saber_chained  the same ops with the keygen -> encaps -> decaps chain of ONE
                  batch inside each step (pipelined launch sequence)
-  frodo976       FrodoKEM-976-SHAKE, batch 8192, keygen + encaps + decaps
+  frodo976       FrodoKEM-976-SHAKE, batch 8192, keygen + encaps + decaps,
+                 rolling steps as above (six launches: decaps' mu', every
+                 serial SHAKE chain of the three ops in one launch, the three
+                 fused products, decaps' last ss block); `chained` the plan
+                 with one batch's chain inside each step
   frodo640 / frodo1344 / frodo976_aes / lightsaber / firesaber
-                 the other parameter sets (Saber family: rolling steps)
+                 the other parameter sets, rolling steps (chained_ops_per_s:
+                 the chained form)
   plain          the one-workgroup-per-KEM comparator form on the same inputs
   latency        p50 / p99 of the per-step wall time (ms), and of a batch's
                  three steps from keygen to decaps
@@ -123,6 +128,12 @@ KERNEL_WORK = {
     "frodo_pipe_x1": (115 + 347, 0),                    # keygen's pkh beside encaps' chain
     "frodo_pipe_x2": (116 + 232 + 115, 0),              # encaps' ss, decaps' chain, decaps' ct absorb
     "frodo_pipe_ssd": (1, 0),
+    # rolling step: keygen's seedA + noise chain, keygen's pkh (closing hash),
+    # encaps' pkh + G2 + noise chain, encaps' ss (closing hash), decaps' G2 +
+    # noise chain, decaps' ct absorb
+    "frodo_roll_x0": (1 + 230 + 115 + 347 + 116 + 232 + 115, 0),
+    "frodo_roll_x0,640": (1 + 122 + 58 + 182 + 58 + 124 + 57, 0),
+    "frodo_roll_x0,1344": (1 + 317 + 159 + 478 + 160 + 319 + 159, 0),
     "frodo_pipe_x1,640": (58 + 182, 0),
     "frodo_pipe_x2,640": (58 + 124 + 57, 0),
     "frodo_pipe_ssd,640": (1, 0),
@@ -455,28 +466,50 @@ def rolling_slots(torch, scheme, n, seed):
     return [chain_arrays(torch, scheme, n, seed + 7919 * k) for k in range(3)]
 
 
-def build_roll(eng, scheme, slots, ipw=1, graph=True):
+def build_roll(eng, scheme, slots, ipw=1, graph=True, threads=0):
     """The rolling pipeline (dpkem_roll_*) over the three slots: step k
     generates the keys of slot k % 3, encapsulates to the keys of slot
     (k - 1) % 3 (generated by step k - 1) and decapsulates the ciphertexts of
     slot (k - 2) % 3 (encapsulated by step k - 1), and finishes the previous
-    steps' batches (keygen's H(pk), encaps' K) in the same two launches.
-    ipw: the Saber product's instances per workgroup."""
+    steps' batches (keygen's H(pk), encaps' K) in the same launches.
+    ipw: the Saber product's instances per workgroup; threads: the Frodo
+    fused kernels' workgroup size (0 = the set's default)."""
     import dpcrypto_amd as D
     eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
+    eng.set_tuning(D.TUNE_FRODO_THREADS, threads)
     r = eng.roll(scheme, slots[0]["kc"].shape[0])
     for k, sl in enumerate(slots):
         r.bind(k, sl["kc"], sl["ec"], sl["pk"], sl["sk"], sl["ct"], sl["ss"], sl["ss2"])
     return r.build(graph=graph)
 
 
+def roll_shapes(scheme):
+    """Launch-shape candidates of a rolling pipeline: the Saber product's
+    instances per workgroup, or the Frodo fused kernels' workgroup size
+    (FrodoKEM-976: the default 512 lanes in two passes over A's rows, or
+    1024 in one)."""
+    import dpcrypto_amd as D
+    if scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER):
+        return [1, 2, 4]
+    return [0, 1024] if scheme in (D.FRODO976, D.FRODO976_AES) else [0]
+
+
+def build_roll_shape(eng, scheme, slots, shape, graph=True):
+    import dpcrypto_amd as D
+    if scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER):
+        return build_roll(eng, scheme, slots, shape, graph=graph)
+    return build_roll(eng, scheme, slots, 1, graph=graph, threads=shape)
+
+
 def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
     """Time `steps` steps of the rolling pipeline (graph replays).  The
-    product's instances per workgroup (1 / 2 / 4) is picked first by timing
+    launch shape (roll_shapes; `ipw` fixes it) is picked first by timing
     two cycles of three steps each.  Afterwards two more steps finish the
     batches still in flight and every slot must hold ss == ss2 (encaps' and
     decaps' shared secrets)."""
     import statistics
+    import dpcrypto_amd as D
+    saber = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
     slots = rolling_slots(torch, scheme, n, seed)
     stream = torch.cuda.current_stream(torch.device("cuda:0"))
     eng.set_stream(stream.cuda_stream)
@@ -494,13 +527,15 @@ def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
         return statistics.median(ts)
 
     tried = {}
-    for c in ([ipw] if ipw else [1, 2, 4]):
-        r = build_roll(eng, scheme, slots, c)
-        cycle_ms(r, 2)
-        tried[c] = cycle_ms(r, 4)
+    shapes = [ipw] if ipw is not None else roll_shapes(scheme)
+    for c in shapes:
+        r = build_roll_shape(eng, scheme, slots, c)
+        if len(shapes) > 1:
+            cycle_ms(r, 1 if not saber else 2)
+            tried[c] = cycle_ms(r, 2 if not saber else 4)
         r.close()
-    best = min(tried, key=tried.get)
-    r = build_roll(eng, scheme, slots, best)
+    best = min(tried, key=tried.get) if tried else shapes[0]
+    r = build_roll_shape(eng, scheme, slots, best)
     for _ in range(max(warmup, 3)):  # every slot through keygen, encaps and decaps before timing
         r.step()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
@@ -520,17 +555,26 @@ def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
     ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots)
     r.close()
     batch = [sum(per_step[i:i + 3]) for i in range(max(1, len(per_step) - 2))]
+    if saber:
+        sched = {"pipeline": "dpkem_roll: two launches per step", "saber_poly_ipw": best,
+                 "tuning_ms": {f"ipw{c}": round(v, 4) for c, v in tried.items()}}
+    else:
+        sched = {"pipeline": "dpkem_roll: six launches per step (decaps' mu', every chain in one launch, three "
+                             "fused products, decaps' last ss block)", "frodo_threads": best or None,
+                 "tuning_ms": {f"threads{c or 'default'}": round(v, 3) for c, v in tried.items()}}
     return dict(wall_s=wall, per_step_ms=per_step, batch_ms=batch, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
-                schedule={"pipeline": "dpkem_roll: two launches per step", "saber_poly_ipw": best,
-                          "tuning_ms": {f"ipw{c}": round(v, 4) for c, v in tried.items()}})
+                schedule=sched)
 
 
-def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=1):
+def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=None):
     """Per-kernel HIP-event times of the rolling steps' kernels, launched
-    without a graph (same launches and kernels)."""
+    without a graph (same launches and kernels); ipw: the launch shape
+    (roll_shapes), by default the first candidate."""
+    if ipw is None:
+        ipw = roll_shapes(scheme)[0]
     slots = rolling_slots(torch, scheme, n, seed)
     eng.set_stream(torch.cuda.current_stream(torch.device("cuda:0")).cuda_stream)
-    r = build_roll(eng, scheme, slots, ipw, graph=False)
+    r = build_roll_shape(eng, scheme, slots, ipw, graph=False)
     for _ in range(3):
         r.step()
     torch.cuda.synchronize()
@@ -706,7 +750,8 @@ CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10,
                   "sab_pipe_x0,l4": 41, "sab_pipe_x1,l4": 16, "sab_pipe_x2,l4": 12,
                   "sab_mix_x0": 24, "sab_mix_x1": 10, "sab_mix_x0,l2": 11, "sab_mix_x1,l2": 7,
                   "sab_mix_x0,l4": 41, "sab_mix_x1,l4": 12,
-                  "sab_roll_x0": 24, "sab_roll_x0,l2": 11, "sab_roll_x0,l4": 41}
+                  "sab_roll_x0": 24, "sab_roll_x0,l2": 11, "sab_roll_x0,l4": 41,
+                  "frodo_roll_x0": 347, "frodo_roll_x0,640": 182, "frodo_roll_x0,1344": 478}
 LANE_PAIR_PERM_FLOOR_S = 24 * 108 * 4 / 2.4e9
 
 
@@ -847,6 +892,72 @@ def cpu_baseline(budget_s: float):
             "compiler_flags": flags, "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
 
 
+# FrodoKEM rolling pipelines measured in a child process of their own (the
+# parent reads one JSON line from it): keyed by the bench line's names.
+FRODO_ROLLING = (("frodo976", 0, 0x44504B454D000002), ("frodo976_aes", 2, 0x44504B454D000007),
+                 ("frodo640", 5, 0x44504B454D000010 + 5), ("frodo1344", 6, 0x44504B454D000010 + 6))
+
+
+def frodo_rolling_child(args):
+    """--frodo-rolling-child: every FrodoKEM set's rolling steps (graph
+    replays, launch shape tuned first) and the same steps' kernels with HIP
+    events; one JSON line on stdout."""
+    import torch
+    import dpcrypto_amd as D
+    eng = D.Engine(0)
+    out = {}
+    for name, sc, seed in FRODO_ROLLING:
+        print(f"[bench] {name} batch {args.frodo_batch}: rolling keygen -> encaps -> decaps steps, 3 warmup + "
+              f"{args.frodo_steps} timed graph steps, then the steps' kernels with HIP events (child process)",
+              file=sys.stderr, flush=True)
+        r = run_rolling_workload(torch, eng, sc, args.frodo_batch, args.frodo_steps, 3, seed)
+        prof = profile_rolling(torch, eng, sc, args.frodo_batch, 2 if name == "frodo976" else 1, seed,
+                               r["schedule"]["frodo_threads"] or 0)
+        out[name] = {"ops_per_s": r["ops_per_s"], "wall_s": r["wall_s"], "per_step_ms": r["per_step_ms"],
+                     "batch_ms": r["batch_ms"], "roundtrip_ok": r["roundtrip_ok"] and prof["roundtrip_ok"],
+                     "schedule": r["schedule"], "kernels": {k: list(v) for k, v in prof["kernels"].items()}}
+        print(f"[bench] {name}: {r['ops_per_s']:.0f} ops/s rolling", file=sys.stderr, flush=True)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_frodo_rolling_child(args, progress):
+    """Runs frodo_rolling_child in a fresh process; returns (results, None)
+    or (None, what went wrong)."""
+    import subprocess
+    progress("frodo rolling pipelines (child process)")
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--frodo-rolling-child", "--frodo-batch",
+           str(args.frodo_batch), "--frodo-steps", str(args.frodo_steps)]
+    try:
+        r = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, timeout=600)
+    except subprocess.TimeoutExpired:
+        return None, "child process timed out"
+    if r.returncode != 0:
+        return None, f"child process exited with status {r.returncode}"
+    try:
+        return json.loads(r.stdout.decode().strip().splitlines()[-1]), None
+    except (ValueError, IndexError):
+        return None, "child process printed no result"
+
+
+def merge_rolling(entry, roll, steps, n, label):
+    """A FrodoKEM parameter set's line: the rolling steps' rate as ops_per_s
+    when the child measured them (the chained plan's under chained_ops_per_s),
+    otherwise the chained plan's alone."""
+    if not roll:
+        entry["rolling_error"] = "not measured"
+        return
+    entry["chained_ops_per_s"] = entry["ops_per_s"]
+    entry["chained_schedule"] = entry.get("schedule")
+    entry["workload"] = f"{label} batch {n}: keygen+encaps+decaps per step (rolling, measured in a child process)"
+    entry["ops_per_s"] = round(roll["ops_per_s"], 1)
+    entry["ms_per_step"] = round(roll["wall_s"] * 1e3 / steps, 3)
+    entry["roundtrip_ok"] = entry["roundtrip_ok"] and roll["roundtrip_ok"]
+    entry["schedule"] = {k: v for k, v in roll["schedule"].items() if k != "tuning_ms"}
+    entry["roofline"] = roofline_from(roll["kernels"], n)
+    entry["kernels_ms"] = {k: round(v[1] / v[0], 4) for k, v in roll["kernels"].items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -869,7 +980,13 @@ def main():
                          "IPW = Saber product instances per workgroup, default 1)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one elementwise kernel over 256 MiB in / 256 MiB out (HBM counter scale)")
+    ap.add_argument("--frodo-rolling-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    import faulthandler
+    faulthandler.enable()  # a fatal signal prints the Python stack of every thread
+    if args.frodo_rolling_child:
+        frodo_rolling_child(args)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: instances are independent, so N GPUs would run N disjoint "
@@ -958,26 +1075,59 @@ def main():
                                    "frodo976_keygen_enc_dec": round(chain_ceiling("frodo976"), 1)}
     result["frac_of_ceiling"] = round(sab["ops_per_s"] / chain_ceiling("saber"), 4)
     if not args.skip_frodo:
-        progress(f"frodo976 batch {args.frodo_batch}: autotune, 2 warmup + {args.frodo_steps} timed graph steps")
-        fr = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 2, 0x44504B454D000002,
-                               schedule=sched)
-        progress(f"frodo976: {fr['ops_per_s']:.0f} ops/s; plan kernels and an eager pass with per-kernel HIP events")
-        fr_prof = profile_plan(torch, eng, D.FRODO976, args.frodo_batch, 1, 0x44504B454D000002)
+        rolling, rolling_err = run_frodo_rolling_child(args, progress)
+        progress(f"frodo976 batch {args.frodo_batch}: one batch's keygen -> encaps -> decaps chained inside each "
+                 f"step, autotune, 2 warmup + {args.frodo_steps} timed graph steps")
+        fr_ch = run_plan_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 2, 0x44504B454D000002,
+                                  schedule=sched)
+        progress(f"frodo976: {fr_ch['ops_per_s']:.0f} ops/s chained; plan kernels and an eager pass with per-kernel "
+                 "HIP events")
+        fr_ch_prof = profile_plan(torch, eng, D.FRODO976, args.frodo_batch, 1, 0x44504B454D000002)
         fr_eager = run_workload(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps, 1, 0x44504B454D000002, True)
-        result["frodo976"] = {
-            "workload": f"frodokem-976-shake batch {args.frodo_batch}: keygen+encaps+decaps per step",
-            "ops_per_s": round(fr["ops_per_s"], 1),
-            "frac_of_ceiling": round(fr["ops_per_s"] / chain_ceiling("frodo976"), 4),
-            "ms_per_step": round(fr["wall_s"] * 1e3 / args.frodo_steps, 3),
-            "latency_ms": {"p50": round(pct(fr["per_step_ms"], 0.5), 3), "p99": round(pct(fr["per_step_ms"], 0.99), 3)},
-            "steps": args.frodo_steps,
-            "roundtrip_ok": fr["roundtrip_ok"] and fr_prof["roundtrip_ok"] and fr_eager["roundtrip_ok"],
-            "schedule": fr["schedule"],
-            "eager_ops_per_s": round(fr_eager["ops_per_s"], 1),
-            "roofline": roofline_from(fr_prof["kernels"], args.frodo_batch),
-            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_prof["kernels"].items()},
-            "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_eager["kernels"].items()},
-        }
+        chained_fr = {
+            "workload": f"frodokem-976-shake batch {args.frodo_batch}: keygen -> encaps -> decaps of one batch "
+                        "chained inside each step (pipelined plan, eight launches per chunk)",
+            "ops_per_s": round(fr_ch["ops_per_s"], 1),
+            "frac_of_ceiling": round(fr_ch["ops_per_s"] / chain_ceiling("frodo976"), 4),
+            "ms_per_step": round(fr_ch["wall_s"] * 1e3 / args.frodo_steps, 3),
+            "latency_ms": {"p50": round(pct(fr_ch["per_step_ms"], 0.5), 3),
+                           "p99": round(pct(fr_ch["per_step_ms"], 0.99), 3)},
+            "schedule": fr_ch["schedule"],
+            "roofline": roofline_from(fr_ch_prof["kernels"], args.frodo_batch),
+            "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_ch_prof["kernels"].items()}}
+        fr = (rolling or {}).get("frodo976")
+        if fr:
+            result["frodo976"] = {
+                "workload": f"frodokem-976-shake batch {args.frodo_batch}: {args.frodo_batch} keygens + "
+                            f"{args.frodo_batch} encaps + {args.frodo_batch} decaps per step; rolling pipeline over "
+                            "three batch slots (encaps to the keys of the previous step, decaps of the ciphertexts "
+                            "of the previous step; a batch is complete two steps after its keygen); measured in a "
+                            "child process",
+                "ops_per_s": round(fr["ops_per_s"], 1),
+                "frac_of_ceiling": round(fr["ops_per_s"] / chain_ceiling("frodo976"), 4),
+                "ms_per_step": round(fr["wall_s"] * 1e3 / args.frodo_steps, 3),
+                "latency_ms": {"p50": round(pct(fr["per_step_ms"], 0.5), 3),
+                               "p99": round(pct(fr["per_step_ms"], 0.99), 3),
+                               "batch_p50": round(pct(fr["batch_ms"], 0.5), 3),
+                               "batch_p99": round(pct(fr["batch_ms"], 0.99), 3),
+                               "batch_what": "one batch from keygen to decaps: three consecutive steps"},
+                "steps": args.frodo_steps,
+                "roundtrip_ok": (fr["roundtrip_ok"] and fr_ch["roundtrip_ok"] and fr_ch_prof["roundtrip_ok"]
+                                 and fr_eager["roundtrip_ok"]),
+                "schedule": fr["schedule"],
+                "eager_ops_per_s": round(fr_eager["ops_per_s"], 1),
+                "roofline": roofline_from(fr["kernels"], args.frodo_batch),
+                "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr["kernels"].items()},
+                "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_eager["kernels"].items()},
+                "chained": chained_fr,
+            }
+        else:
+            result["frodo976"] = {
+                **chained_fr, "steps": args.frodo_steps,
+                "roundtrip_ok": fr_ch["roundtrip_ok"] and fr_ch_prof["roundtrip_ok"] and fr_eager["roundtrip_ok"],
+                "eager_ops_per_s": round(fr_eager["ops_per_s"], 1),
+                "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fr_eager["kernels"].items()},
+                "rolling_error": rolling_err}
         if not args.skip_per_op:
             progress(f"frodo976 batch {args.frodo_batch}: keygen, encaps, decaps timed as separate plans, "
                      "and in the plain form")
@@ -991,7 +1141,7 @@ def main():
         if not args.skip_frodo:
             fp = run_workload(torch, pe, D.FRODO976, args.frodo_batch, 1, 0, 0x44504B454D000002)
             plain["frodo976_ops_per_s"] = round(fp["ops_per_s"], 1)
-            plain["frodo976_speedup_batched_over_plain"] = round(fr["ops_per_s"] / fp["ops_per_s"], 2)
+            plain["frodo976_speedup_batched_over_plain"] = round(result["frodo976"]["ops_per_s"] / fp["ops_per_s"], 2)
         result["plain"] = plain
     if not args.skip_frodo:
         progress(f"frodo976-aes batch {args.frodo_batch}: autotuned graph steps, then an eager pass with HIP events")
@@ -1004,6 +1154,8 @@ def main():
             "roundtrip_ok": fa["roundtrip_ok"] and fa_eager["roundtrip_ok"],
             "schedule": {k: v for k, v in fa["schedule"].items() if k != "tuning_ms"},
             "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fa_eager["kernels"].items()}}
+        merge_rolling(result["frodo976_aes"], (rolling or {}).get("frodo976_aes"), args.frodo_steps,
+                      args.frodo_batch, "frodokem-976-aes")
         progress("frodo976 encaps + decaps against one key (A expanded once per 4 instances)")
         result["frodo976_shared_key"] = run_shared_key(torch, eng, D.FRODO976, args.frodo_batch, args.frodo_steps,
                                                        0x44504B454D000006)
@@ -1019,6 +1171,7 @@ def main():
                 "schedule": {k: v for k, v in fs["schedule"].items() if k != "tuning_ms"},
                 "roofline": roofline_from(fs_prof["kernels"], args.frodo_batch),
                 "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in fs_prof["kernels"].items()}}
+            merge_rolling(result[name], (rolling or {}).get(name), args.frodo_steps, args.frodo_batch, f"{name}-shake")
     for name, sc in (("lightsaber", D.LIGHTSABER), ("firesaber", D.FIRESABER)):
         progress(f"{name} batch {args.saber_batch}: rolling steps, then the chained form")
         fam = run_rolling_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup,

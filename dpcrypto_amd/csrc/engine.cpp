@@ -78,12 +78,14 @@ int fail_hip(dpkem_engine* e, hipError_t err, const char* what) {
 // After the engine's stream has drained: did a bounded wait inside a kernel
 // give up (its launch then wrote invalid results)?  Clears the record.
 int check_waits(dpkem_engine* e) {
-  uint32_t expired = 0;
-  HIP_TRY(e, dpkem::saber_wait_expired(&expired, true));
-  if (expired) {
+  uint32_t saber = 0, frodo = 0;
+  HIP_TRY(e, dpkem::saber_wait_expired(&saber, true));
+  HIP_TRY(e, dpkem::frodo_wait_expired(&frodo, true));
+  if (saber || frodo) {
     snprintf(e->last_err, sizeof e->last_err,
-             "a bounded LDS wait gave up inside a Saber-family kernel (%u lanes); results of launches since the "
-             "last check are invalid", expired);
+             "a bounded LDS wait gave up inside a %s kernel (%u lanes); results of launches since the last check "
+             "are invalid",
+             saber ? "Saber-family" : "FrodoKEM", saber + frodo);
     return DPKEM_EDEVICE;
   }
   return DPKEM_OK;
@@ -409,7 +411,7 @@ int dpkem_engine_create(dpkem_engine** out, int device, size_t budget) {
   // 0 to exercise the give-up path); the device value is shared by the process
   uint32_t limit = 1u << 22;
   if (const char* v = getenv("DPKEM_WAIT_SPIN_LIMIT")) limit = (uint32_t)strtoul(v, nullptr, 0);
-  if (dpkem::saber_set_wait_spin_limit(limit) != hipSuccess) {
+  if (dpkem::saber_set_wait_spin_limit(limit) != hipSuccess || dpkem::frodo_set_wait_spin_limit(limit) != hipSuccess) {
     (void)hipStreamDestroy(e->own_stream);
     delete e;
     return DPKEM_EDEVICE;

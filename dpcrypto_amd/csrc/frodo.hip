@@ -343,6 +343,34 @@ __device__ __forceinline__ void shake_seed_squeeze2_ring(uint32_t p, uint64_t do
   ring_publish<P, REM>(ring, a, NB, lane);
 }
 
+// The same with the domain byte and the length (outw words, not a multiple
+// of the rate) known at run time, so that chain waves squeezing streams of
+// different lengths share one call site: one copy of the unrolled
+// permutation in the instruction cache.  The final, partial block is
+// published whole; its sampler reads only the first outw % rate words.
+template <class P>
+__device__ __forceinline__ void shake_seed_squeeze2_ring_rt(uint32_t p, uint64_t dom, const uint64_t seed[P::ls],
+                                                            FrRing<P>* ring, int lane, int outw) {
+  constexpr int RW = P::chain_rate, LS = P::ls;
+  uint64_t m[LS + 1];
+  m[0] = dom | (seed[0] << 8);
+#pragma unroll
+  for (int i = 1; i < LS; ++i) m[i] = (seed[i - 1] >> 56) | (seed[i] << 8);
+  m[LS] = (seed[LS - 1] >> 56) | (kDsShake << 8);
+  uint32_t a[25];
+  k2::zero2(a);
+#pragma unroll
+  for (int i = 0; i <= LS; ++i) a[i] = k2::deint(m[i], p);
+  a[RW - 1] ^= k2::half_const(kPadEnd, p);
+  k2::keccakf2(a, p);
+  const int nb = outw / RW;
+  for (int b = 0; b < nb; ++b) {
+    ring_publish<P, RW>(ring, a, b, lane);
+    k2::keccakf2_unrolled(a, p);
+  }
+  ring_publish<P, RW>(ring, a, nb, lane);
+}
+
 template <class P, int CNT, class F>
 __device__ __forceinline__ void ring_sample_block(FrRing<P>* ring, uint32_t p, const uint8_t* lut, int b, int lane,
                                                   F&& f) {
@@ -378,13 +406,13 @@ enum FrRole { kFrWhole = 0, kFrChain = 1, kFrSampler = 2 };
 // seedA = SHAKE(z, 16); S^T || E = sample(SHAKE(0x5F || seedSE, 32 n)).
 // Writes pk[0:16], sk[0:sec] = s, sk[sec:sec+16] = seedA, the sk S^T part (LE
 // u16) and the int8 copies of S^T ([k][j]) and E ([i][k]) for the fused kernel.
-template <class P, int ROLE = kFrWhole>
+// OWN_LUT = false: the sampler reads the table at `lut`, staged by the caller.
+template <class P, int ROLE = kFrWhole, bool OWN_LUT = true>
 __device__ __forceinline__ void keygen_pre_body(int vb, int n, const uint8_t* __restrict__ coins,
                                                 uint8_t* __restrict__ pk, uint8_t* __restrict__ sk,
                                                 uint32_t* __restrict__ s8, uint32_t* __restrict__ e8,
-                                                FrRing<P>* ring = nullptr) {
-  const uint8_t* lut = nullptr;
-  if constexpr (ROLE != kFrChain) {
+                                                FrRing<P>* ring = nullptr, const uint8_t* lut = nullptr) {
+  if constexpr (ROLE != kFrChain && OWN_LUT) {
     __shared__ __align__(16) uint8_t lut_lds[32768];
     lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
   }
@@ -481,19 +509,47 @@ __global__ void __launch_bounds__(64) frodo_keygen_post(int n, const uint8_t* __
 // mu = recovered mu'.
 // G2 = SHAKE(pkh || mu, 2 sec) = seedSE || k; S' || E' || E'' from
 // SHAKE(0x96 || seedSE, 32 n + 128).  k goes to kbuf[0:sec].
-template <class P, bool DEC, int ROLE = kFrWhole>
+template <class P, bool DEC>
+__device__ __forceinline__ void encdec_seed(int inst, uint32_t p, const uint8_t* __restrict__ keys, size_t key_stride,
+                                            const uint8_t* __restrict__ mu, size_t mu_stride,
+                                            uint8_t* __restrict__ kbuf, const uint8_t* __restrict__ pkh,
+                                            uint64_t seed[P::ls]) {
+  constexpr int RW = P::chain_rate, LS = P::ls;
+  const uint8_t* key = keys + (size_t)inst * key_stride;
+  uint32_t a[25], g[2 * LS];
+  if (DEC || pkh) {
+    const uint8_t* h = DEC ? key + P::sk_pkh : pkh + (size_t)inst * P::sec;
+#pragma unroll
+    for (int i = 0; i < LS; ++i) g[i] = k2::deint(ld64(h + 8 * i), p);
+  } else {
+    absorb2_global<RW, P::pk / 8>(a, p, key, kDsShake);
+#pragma unroll
+    for (int i = 0; i < LS; ++i) g[i] = a[i];
+  }
+  const uint8_t* m = mu + (size_t)inst * mu_stride;
+#pragma unroll
+  for (int i = 0; i < LS; ++i) g[LS + i] = k2::deint(ld64(m + 8 * i), p);
+  absorb2_halves<RW, 2 * LS>(a, p, g, kDsShake);
+  store2_words<LS>(a + LS, p, kbuf + (size_t)inst * 64);
+#pragma unroll
+  for (int i = 0; i < LS; ++i) seed[i] = k2::out_full(a[i], p);
+}
+
+// The chain (G2, then the noise squeeze) and / or the sampling; OWN_LUT as
+// in keygen_pre_body.
+template <class P, bool DEC, int ROLE = kFrWhole, bool OWN_LUT = true>
 __device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __restrict__ keys, size_t key_stride,
                                                 const uint8_t* __restrict__ mu, size_t mu_stride,
                                                 uint8_t* __restrict__ kbuf, uint32_t* __restrict__ s8,
                                                 uint32_t* __restrict__ e8, uint32_t* __restrict__ epp8,
-                                                const uint8_t* __restrict__ pkh, FrRing<P>* ring = nullptr) {
-  const uint8_t* lut = nullptr;
-  if constexpr (ROLE != kFrChain) {
+                                                const uint8_t* __restrict__ pkh, FrRing<P>* ring = nullptr,
+                                                const uint8_t* lut = nullptr) {
+  if constexpr (ROLE != kFrChain && OWN_LUT) {
     __shared__ __align__(16) uint8_t lut_lds[32768];  // staged by the one wave of a workgroup that samples
     lut = stage_sample_lut<P>(lut_lds, threadIdx.x & 63);
   }
   FR_PAIR_SETUP
-  constexpr int RW = P::chain_rate, LS = P::ls, OUTW = (2 * P::mat + 64) / 4;
+  constexpr int LS = P::ls, OUTW = (2 * P::mat + 64) / 4;
   uint8_t* sd = reinterpret_cast<uint8_t*>(s8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* ed = reinterpret_cast<uint8_t*>(e8 + (size_t)inst * (P::mat / 4)) + 2 * p;
   uint8_t* pd = reinterpret_cast<uint8_t*>(epp8 + (size_t)inst * 16) + 2 * p;
@@ -507,25 +563,8 @@ __device__ __forceinline__ void encdec_pre_body(int vb, int n, const uint8_t* __
   if constexpr (ROLE == kFrSampler) {
     sample_ring<P, OUTW>(p, ring, lane, lut, f);
   } else {
-    const uint8_t* key = keys + (size_t)inst * key_stride;
-    uint32_t a[25], g[2 * LS];
-    if (DEC || pkh) {
-      const uint8_t* h = DEC ? key + P::sk_pkh : pkh + (size_t)inst * P::sec;
-#pragma unroll
-      for (int i = 0; i < LS; ++i) g[i] = k2::deint(ld64(h + 8 * i), p);
-    } else {
-      absorb2_global<RW, P::pk / 8>(a, p, key, kDsShake);
-#pragma unroll
-      for (int i = 0; i < LS; ++i) g[i] = a[i];
-    }
-    const uint8_t* m = mu + (size_t)inst * mu_stride;
-#pragma unroll
-    for (int i = 0; i < LS; ++i) g[LS + i] = k2::deint(ld64(m + 8 * i), p);
-    absorb2_halves<RW, 2 * LS>(a, p, g, kDsShake);
-    store2_words<LS>(a + LS, p, kbuf + (size_t)inst * 64);
     uint64_t seed[LS];
-#pragma unroll
-    for (int i = 0; i < LS; ++i) seed[i] = k2::out_full(a[i], p);
+    encdec_seed<P, DEC>(inst, p, keys, key_stride, mu, mu_stride, kbuf, pkh, seed);
     if constexpr (ROLE == kFrChain) shake_seed_squeeze2_ring<P, OUTW>(p, 0x96, seed, ring, lane);
     else shake_seed_squeeze2<P, OUTW>(p, 0x96, seed, lut, f);
   }
@@ -1436,6 +1475,109 @@ __global__ void __launch_bounds__(192) frodo_dec_pre(int n, const uint8_t* __res
   dec_pre_body<P>(blockIdx.x, n, ct, sk, sk_stride, mubuf, kbuf, s8, e8, epp8, state);
 }
 
+// ----------------------------------------------------------------- rolling keygen -> encaps -> decaps
+//
+// One step of a rolling pipeline over three batch slots: keygen of slot r,
+// encaps of slot r - 1 (to the keys keygen wrote one step earlier), decaps
+// of slot r - 2 (of the ciphertexts encaps wrote one step earlier).  Run
+// chained per batch, each chain launch holds one chain wave per 32
+// instances, 256 for a batch of 8192 on 1,024 SIMDs, and lasts as long as
+// its longest chain; the chains of the three ops here are independent, so
+// they all go into ONE launch, one 448-thread workgroup per 32 instances of
+// each op:
+//   wave 0  keygen's noise chain (slot r), then the closing hash of slot
+//           r - 2's encaps: ss = SHAKE(ct || k)
+//   wave 1  encaps' pkh, G2 and noise chain (slot r - 1)
+//   wave 2  decaps' G2 and noise chain (slot r - 2, mu' from frodo_dec_mu)
+//   wave 3  decaps' SHAKE over the full ct blocks (state, finished after the
+//           re-encryption by frodo_pipe_ssd), then the closing hash of slot
+//           r - 1's keygen: pkh = SHAKE(pk) into its sk
+//   waves 4 / 5 / 6  the samplers of waves 1 / 2 / 0 (FrRing), wave 6 also
+//           keygen's seedA
+// Waves w and w + 4 of a workgroup share a SIMD: each SIMD runs one chain
+// (wave 0: 230 + 117 permutations for FrodoKEM-976, wave 1: 347, wave 2:
+// 232, wave 3: 116 + 115) and at most one sampler.  The three noise
+// squeezes share one call site (shake_seed_squeeze2_ring_rt).  Every op
+// still computes all of its own hashes.
+struct FrRollArgs {
+  int n;
+  // keygen, slot r
+  const uint8_t* kc;
+  uint8_t *pk_k, *sk_k;
+  uint32_t *s8_k, *e8_k;
+  // encaps, slot r - 1; that slot's keygen closing hash writes sk_e
+  const uint8_t *pk_e, *ec;
+  uint8_t *sk_e, *kbuf_e;
+  uint32_t *s8_e, *e8_e, *epp8_e;
+  // decaps, slot r - 2; that slot's encaps closing hash reads kbuf_ed, writes ss_ed
+  const uint8_t *ct_d, *sk_d, *mubuf_d, *kbuf_ed;
+  uint8_t *kbuf_d, *ss_ed;
+  uint32_t *s8_d, *e8_d, *epp8_d, *state_d;
+};
+
+template <class P>
+__global__ void __launch_bounds__(448) frodo_roll_x0(const FrRollArgs A) {
+  constexpr int RW = P::chain_rate, LS = P::ls;
+  constexpr int OUTW_K = 2 * P::mat / 4, OUTW_E = (2 * P::mat + 64) / 4;
+  static_assert(OUTW_K % RW > 0 && OUTW_E % RW > 0, "each noise stream ends in a partial block");
+  __shared__ FrRing<P> ring[3];
+  __shared__ __align__(16) uint8_t lut[32768];
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(sample_lut_global<P::n>());
+    uint4* dst = reinterpret_cast<uint4*>(lut);
+    for (int i = threadIdx.x; i < 32768 / 16; i += 448) dst[i] = src[i];
+    if (threadIdx.x < 3) {
+      ring[threadIdx.x].ready = 0;
+      ring[threadIdx.x].done = 0;
+    }
+  }
+  __syncthreads();
+  chain_priority();  // the samplers too: the chains wait on them for ring slots
+  const int wave = threadIdx.x >> 6, vb = blockIdx.x, n = A.n;
+  if (wave < 3) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t p = lane & 1;
+    const int inst = vb * 32 + (lane >> 1);
+    if (inst < n) {
+      uint64_t seed[LS];
+      uint64_t dom = 0x96;
+      int outw = OUTW_E;
+      if (wave == 0) {
+        const uint8_t* c = A.kc + (size_t)inst * P::kcoins + P::sec;
+#pragma unroll
+        for (int i = 0; i < LS; ++i) seed[i] = ld64(c + 8 * i);
+        dom = 0x5F;
+        outw = OUTW_K;
+      } else if (wave == 1) {
+        encdec_seed<P, false>(inst, p, A.pk_e, (size_t)P::pk, A.ec, (size_t)P::ecoins, A.kbuf_e, nullptr, seed);
+      } else {
+        encdec_seed<P, true>(inst, p, A.sk_d, (size_t)P::sk, A.mubuf_d, 32, A.kbuf_d, nullptr, seed);
+      }
+      shake_seed_squeeze2_ring_rt<P>(p, dom, seed, &ring[wave], lane, outw);
+    }
+    if (wave == 0) ss_body<P>(vb, n, A.ct_d, A.kbuf_ed, nullptr, 0, nullptr, A.ss_ed);
+  } else if (wave == 3) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t p = lane & 1;
+    const int inst = vb * 32 + (lane >> 1);
+    if (inst < n) {
+      uint32_t a[25];
+      ss_absorb<P>(a, p, A.ct_d + (size_t)inst * P::ct);
+#pragma unroll
+      for (int i = 0; i < 25; ++i) A.state_d[(size_t)i * 2 * n + 2 * inst + p] = a[i];
+    }
+    keygen_post_body<P>(vb, n, A.pk_e, A.sk_e);
+  } else if (wave == 4) {
+    encdec_pre_body<P, false, kFrSampler, false>(vb, n, A.pk_e, (size_t)P::pk, A.ec, (size_t)P::ecoins, A.kbuf_e,
+                                                 A.s8_e, A.e8_e, A.epp8_e, nullptr, &ring[1], lut);
+  } else if (wave == 5) {
+    encdec_pre_body<P, true, kFrSampler, false>(vb, n, A.sk_d, (size_t)P::sk, A.mubuf_d, 32, A.kbuf_d, A.s8_d, A.e8_d,
+                                                A.epp8_d, nullptr, &ring[2], lut);
+  } else {
+    keygen_pre_body<P, kFrSampler, false>(vb, n, A.kc, A.pk_k, A.sk_k, A.s8_k, A.e8_k, &ring[0], lut);
+  }
+}
+
 // ----------------------------------------------------------------- host launchers
 
 static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
@@ -1660,6 +1802,108 @@ void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* 
   else if (set == 1344) pipe_t<FrodoP<1344>, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads, force_reject);
   else if (aes) pipe_t<F976, true>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads, force_reject);
   else pipe_t<F976, false>(st, n, kc, ec, pk, sk, ct, ss, ss2, scratch, threads, force_reject);
+}
+
+// The bounded LDS waits of this file's kernels (the noise chains' ring
+// hand-overs, keccak2.hpp lds_wait_count) count their give-ups in this
+// translation unit's own device counter, and take their bound from its own
+// copy of the limit.
+hipError_t frodo_wait_expired(uint32_t* count, bool clear) {
+  hipError_t err = hipMemcpyFromSymbol(count, HIP_SYMBOL(k2::wait_expired), sizeof(uint32_t), 0, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && clear && *count) {
+    const uint32_t zero = 0;
+    err = hipMemcpyToSymbol(HIP_SYMBOL(k2::wait_expired), &zero, sizeof zero, 0, hipMemcpyHostToDevice);
+  }
+  return err;
+}
+
+hipError_t frodo_set_wait_spin_limit(uint32_t limit) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(k2::wait_spin_limit), &limit, sizeof limit, 0, hipMemcpyHostToDevice);
+}
+
+// Rolling pipeline: one slot's scratch is s8 | e8 | epp8 | kbuf_e | kbuf_d |
+// mubuf | fail | state, each array 256-aligned.  A slot carries one op per
+// step, so its s8 / e8 / epp8 serve keygen, encaps and decaps in turn;
+// kbuf_e lives on from encaps' chain to the closing hash one step later.
+size_t frodo_roll_scratch_bytes(int set, size_t n) { return n * frodo_pipe_scratch_per_inst(set) + 8 * 256; }
+
+struct FrSlot {
+  uint32_t *s8, *e8, *epp8;
+  uint8_t *kbuf_e, *kbuf_d, *mubuf;
+  uint32_t *fail, *state;
+};
+
+template <class P>
+static FrSlot fr_slot(uint8_t* base, int n) {
+  size_t off = 0;
+  auto take = [&](size_t per) {
+    uint8_t* q = base + off;
+    off += ((size_t)n * per + 255) & ~(size_t)255;
+    return q;
+  };
+  FrSlot s;
+  s.s8 = reinterpret_cast<uint32_t*>(take(P::mat));
+  s.e8 = reinterpret_cast<uint32_t*>(take(P::mat));
+  s.epp8 = reinterpret_cast<uint32_t*>(take(64));
+  s.kbuf_e = take(64);
+  s.kbuf_d = take(64);
+  s.mubuf = take(32);
+  s.fail = reinterpret_cast<uint32_t*>(take(4));
+  s.state = reinterpret_cast<uint32_t*>(take(200));
+  return s;
+}
+
+// Step: decaps' mu' (slot r - 2), every chain (frodo_roll_x0), the keygen /
+// encaps / re-encryption products, decaps' last ss block.
+template <class P, bool AES>
+static void roll_t(hipStream_t st, const RollDesc& d, int threads) {
+  const int n = d.n;
+  const FrSlot K = fr_slot<P>(d.scratch_k, n), E = fr_slot<P>(d.scratch_e, n), D = fr_slot<P>(d.scratch_d, n);
+  const FrMtTask td =
+      task_of(n, d.ct_d, d.sk_d, d.ss_d, nullptr, D.s8, D.e8, D.epp8, D.kbuf_d, D.mubuf, D.fail, D.state, (size_t)P::sk);
+  const FrMt Md = table_of(&td, 1);
+  launch_dec_mu<P>(st, Md);
+  FrRollArgs a;
+  a.n = n;
+  a.kc = d.kc;
+  a.pk_k = d.pk_k;
+  a.sk_k = d.sk_k;
+  a.s8_k = K.s8;
+  a.e8_k = K.e8;
+  a.pk_e = d.pk_e;
+  a.ec = d.ec;
+  a.sk_e = d.sk_e;
+  a.kbuf_e = E.kbuf_e;
+  a.s8_e = E.s8;
+  a.e8_e = E.e8;
+  a.epp8_e = E.epp8;
+  a.ct_d = d.ct_d;
+  a.sk_d = d.sk_d;
+  a.mubuf_d = D.mubuf;
+  a.kbuf_ed = D.kbuf_e;
+  a.kbuf_d = D.kbuf_d;
+  a.ss_ed = d.ss_e_prev;
+  a.s8_d = D.s8;
+  a.e8_d = D.e8;
+  a.epp8_d = D.epp8;
+  a.state_d = D.state;
+  const dim3 g(blocks_for(n, 32));
+  DPKEM_LAUNCH(FR_NAME("frodo_roll_x0"), frodo_roll_x0<P>, g, dim3(448), st, a);
+  const FrMtTask tk = task_of(n, d.kc, nullptr, d.pk_k, d.sk_k, K.s8, K.e8);
+  launch_as_fused<P, AES>(st, table_of(&tk, 1), false, threads);
+  const FrMtTask te = task_of(n, d.pk_e, d.ec, d.ct_e, nullptr, E.s8, E.e8, E.epp8, E.kbuf_e);
+  launch_sa_fused<P, false, AES>(st, table_of(&te, 1), false, threads);
+  launch_sa_fused<P, true, AES>(st, Md, false, threads);
+  DPKEM_LAUNCH(FR_NAME("frodo_pipe_ssd"), frodo_pipe_ssd<P>, g, dim3(64), st, n, d.ct_d, D.state, D.kbuf_d, d.sk_d,
+               (size_t)P::sk, D.fail, d.ss_d, 0);
+}
+
+void frodo_roll_launch(hipStream_t st, int set, bool aes, const RollDesc& d, int threads) {
+  if (d.n <= 0) return;
+  if (set == 640) roll_t<FrodoP<640>, false>(st, d, threads);
+  else if (set == 1344) roll_t<FrodoP<1344>, false>(st, d, threads);
+  else if (aes) roll_t<F976, true>(st, d, threads);
+  else roll_t<F976, false>(st, d, threads);
 }
 
 // Encaps / decaps where instances share public keys in groups (see

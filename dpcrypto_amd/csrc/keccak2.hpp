@@ -317,7 +317,7 @@ __device__ __forceinline__ void absorb2_global(uint32_t a[25], uint32_t p, const
 // A wait that reaches its bound gives up: the wave goes on with whatever the
 // LDS holds (in-bounds reads, garbage results) and adds to wait_expired, a
 // counter of this translation unit that the host reads and clears
-// (saber_wait_expired) and reports as a failed launch.  wait_spin_limit is
+// (saber_wait_expired, frodo_wait_expired) and reports as a failed launch.  wait_spin_limit is
 // the bound (host-settable; 0 makes every wait give up at once, which is how
 // the failure path is exercised without a faulty kernel).
 static __device__ uint32_t wait_spin_limit = 1u << 22;

@@ -20,6 +20,10 @@ hipError_t saber_wait_expired(uint32_t* count, bool clear);
 // out: 64 words.
 void saber_probe_perm_launch(hipStream_t st, int perms, uint32_t* out);
 hipError_t saber_set_wait_spin_limit(uint32_t limit);
+// The same for the FrodoKEM kernels (frodo.hip: the noise chains' ring
+// hand-overs); each kernel file keeps its own counter and bound.
+hipError_t frodo_wait_expired(uint32_t* count, bool clear);
+hipError_t frodo_set_wait_spin_limit(uint32_t limit);
 
 // dst[i] = src for n rows of `bytes` bytes (a multiple of 4).
 void broadcast_rows_launch(hipStream_t st, int n, const uint8_t* src, size_t bytes, uint8_t* dst);
@@ -53,14 +57,14 @@ struct SaberMixDesc {
 size_t saber_mix_scratch_bytes(int l, size_t nk, size_t ne, size_t nd);
 void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int ipw = 1);
 // One step of a rolling keygen -> encaps -> decaps pipeline over batch slots
-// of n instances (saber.hip: sab_roll_x0, then the mixed products): keygen
-// of slot r (its product too), encaps of slot r - 1 (to the keys keygen wrote
-// one step earlier; its K follows one step later), decaps of slot r - 2, and
-// the closing hashes of the previous steps: keygen's H(pk) into slot r - 1's
-// sk, encaps' K into slot r - 2's ss.  Each slot has its own scratch of
-// saber_mix_scratch_bytes(l, n, n, n) bytes: scratch_k / _e / _d are those of
-// slots r, r - 1, r - 2.
-struct SaberRollDesc {
+// of n instances: keygen of slot r, encaps of slot r - 1 (to the keys keygen
+// wrote one step earlier), decaps of slot r - 2 (of the ciphertexts encaps
+// wrote one step earlier), and the closing hashes of the previous steps:
+// keygen's pkh into slot r - 1's sk, encaps' shared secret into slot r - 2's
+// ss.  Each slot has scratch of its own (saber_mix_scratch_bytes(l, n, n, n)
+// or frodo_roll_scratch_bytes(set, n) bytes): scratch_k / _e / _d are those
+// of slots r, r - 1, r - 2.
+struct RollDesc {
   int n;
   const uint8_t* kc;  // slot r
   uint8_t *pk_k, *sk_k;
@@ -70,7 +74,8 @@ struct SaberRollDesc {
   uint8_t *ss_d, *ss_e_prev;   // decaps' ss, and encaps' ss of the same slot
   uint8_t *scratch_k, *scratch_e, *scratch_d;
 };
-void saber_roll_launch(hipStream_t st, int l, const SaberRollDesc& d, int ipw = 1);
+// Saber family (saber.hip: sab_roll_x0, then the mixed products).
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw = 1);
 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch
@@ -89,6 +94,11 @@ size_t frodo_pipe_scratch_per_inst(int set);
 void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* ec, uint8_t* pk, uint8_t* sk,
                        uint8_t* ct, uint8_t* ss, uint8_t* ss2, uint8_t* scratch, int set, bool aes, int threads = 0,
                        int force_reject = 0);
+// FrodoKEM (frodo.hip: frodo_dec_mu, frodo_roll_x0 -- every chain of the
+// step in one launch --, the three fused products, frodo_pipe_ssd); set /
+// aes / threads as above.
+size_t frodo_roll_scratch_bytes(int set, size_t n);
+void frodo_roll_launch(hipStream_t st, int set, bool aes, const RollDesc& d, int threads = 0);
 // Groups of up to kFrodoGroup instances sharing one public key expand A once;
 // groups[b] = (first, count) or nullptr for consecutive groups of
 // kFrodoGroup; key strides may be 0 (one key for all n).

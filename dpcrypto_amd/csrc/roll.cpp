@@ -1,6 +1,8 @@
 // Rolling keygen -> encaps -> decaps pipelines (dpkem_roll_*): three batch
-// slots of one Saber-family scheme, one step = sab_roll_x0 + the mixed
-// products (saber_roll_launch), captured once per step mod 3 into a HIP graph
+// slots of one scheme; one step is, for the Saber family, sab_roll_x0 + the
+// mixed products (saber_roll_launch), for FrodoKEM decaps' mu', every chain
+// in one launch, the three fused products and decaps' last ss block
+// (frodo_roll_launch).  Each step mod 3 is captured once into a HIP graph
 // and replayed on the engine's stream.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
@@ -15,6 +17,8 @@ using namespace dpkem;
 struct dpkem_roll {
   dpkem_engine* e = nullptr;
   int scheme = 0, l = 3, ipw = 1;
+  int set = 0, threads = 0;  // FrodoKEM: parameter set (0 = Saber family), fused-kernel shape
+  bool aes = false;
   size_t n = 0;
   struct Slot {
     const uint8_t *kc = nullptr, *ec = nullptr;
@@ -35,6 +39,12 @@ namespace {
 int rank_of(int scheme) {
   return scheme == DPKEM_SABER ? 3 : scheme == DPKEM_LIGHTSABER ? 2 : scheme == DPKEM_FIRESABER ? 4 : 0;
 }
+int set_of(int scheme) {
+  return scheme == DPKEM_FRODO976 || scheme == DPKEM_FRODO976_AES ? 976
+         : scheme == DPKEM_FRODO640                              ? 640
+         : scheme == DPKEM_FRODO1344                             ? 1344
+                                                                 : 0;
+}
 
 void release(dpkem_roll* r) {
   r->built = false;
@@ -53,7 +63,7 @@ void release(dpkem_roll* r) {
 void issue(dpkem_roll* r, hipStream_t st, int phase) {
   const int ks = phase, es = (phase + 2) % 3, ds = (phase + 1) % 3;
   const auto &K = r->slot[ks], &E = r->slot[es], &Dd = r->slot[ds];
-  SaberRollDesc d{};
+  RollDesc d{};
   d.n = (int)r->n;
   d.kc = K.kc;
   d.pk_k = K.pk;
@@ -69,7 +79,8 @@ void issue(dpkem_roll* r, hipStream_t st, int phase) {
   d.scratch_k = r->scratch + (size_t)ks * r->per_slot;
   d.scratch_e = r->scratch + (size_t)es * r->per_slot;
   d.scratch_d = r->scratch + (size_t)ds * r->per_slot;
-  saber_roll_launch(st, r->l, d, r->ipw);
+  if (r->set) frodo_roll_launch(st, r->set, r->aes, d, r->threads);
+  else saber_roll_launch(st, r->l, d, r->ipw);
 }
 
 }  // namespace
@@ -77,13 +88,15 @@ void issue(dpkem_roll* r, hipStream_t st, int phase) {
 extern "C" {
 
 int dpkem_roll_create(dpkem_engine* e, int scheme, size_t n, dpkem_roll** out) {
-  if (!e || !out || !rank_of(scheme) || n == 0 || n > (size_t)(1 << 24) || e->form != DPKEM_FORM_BATCHED)
+  if (!e || !out || (!rank_of(scheme) && !set_of(scheme)) || n == 0 || n > (size_t)(1 << 24) || e->form != DPKEM_FORM_BATCHED)
     return DPKEM_EINVAL;
   dpkem_roll* r = new (std::nothrow) dpkem_roll();
   if (!r) return DPKEM_ENOMEM;
   r->e = e;
   r->scheme = scheme;
   r->l = rank_of(scheme);
+  r->set = set_of(scheme);
+  r->aes = scheme == DPKEM_FRODO976_AES;
   r->n = n;
   *out = r;
   return DPKEM_OK;
@@ -107,7 +120,10 @@ int dpkem_roll_build(dpkem_roll* r, unsigned flags) {
   release(r);
   r->flags = flags;
   r->ipw = e->saber_ipw;
-  r->per_slot = (saber_mix_scratch_bytes(r->l, r->n, r->n, r->n) + 255) & ~(size_t)255;
+  r->threads = e->frodo_threads;
+  r->per_slot = ((r->set ? frodo_roll_scratch_bytes(r->set, r->n) : saber_mix_scratch_bytes(r->l, r->n, r->n, r->n)) +
+                 255) &
+                ~(size_t)255;
   if (3 * r->per_slot > e->budget) {
     snprintf(e->last_err, sizeof e->last_err, "rolling pipeline scratch %zu B exceeds HBM budget %zu B",
              3 * r->per_slot, e->budget);

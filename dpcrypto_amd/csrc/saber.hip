@@ -1710,7 +1710,7 @@ static void mix_l(hipStream_t st, const SaberMixDesc& d, uint8_t* scratch, int i
 // One step of the rolling pipeline (see saber_roll_launch in launch.hpp):
 // sab_roll_x0, then the mixed products.
 template <int L>
-static void roll_l(hipStream_t st, const SaberRollDesc& d, int ipw) {
+static void roll_l(hipStream_t st, const RollDesc& d, int ipw) {
   const int n = d.n;
   const MixScratch xk = mix_carve<L>(d.scratch_k, n, n, n);
   const MixScratch xe = mix_carve<L>(d.scratch_e, n, n, n);
@@ -1747,7 +1747,7 @@ static void roll_l(hipStream_t st, const SaberRollDesc& d, int ipw) {
   mix_products<L>(st, m, ipw);
 }
 
-void saber_roll_launch(hipStream_t st, int l, const SaberRollDesc& d, int ipw) {
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw) {
   if (d.n <= 0) return;
   if (l == 2) roll_l<2>(st, d, ipw);
   else if (l == 4) roll_l<4>(st, d, ipw);

@@ -560,10 +560,12 @@ int dpkem_server_drain(dpkem_server* s) {
   s->cv.notify_one();
   s->drained.wait(lk, [&] { return s->inflight_requests == 0; });
   // a bounded wait inside a kernel that gave up invalidates what ran since the last check
-  uint32_t expired = 0;
+  uint32_t saber = 0, frodo = 0;
   (void)hipSetDevice(s->device);
-  if (saber_wait_expired(&expired, true) != hipSuccess || expired) {
-    snprintf(s->last_err, sizeof s->last_err, "a bounded LDS wait gave up inside a Saber-family kernel");
+  if (saber_wait_expired(&saber, true) != hipSuccess || frodo_wait_expired(&frodo, true) != hipSuccess || saber ||
+      frodo) {
+    snprintf(s->last_err, sizeof s->last_err, "a bounded LDS wait gave up inside a %s kernel",
+             frodo && !saber ? "FrodoKEM" : "Saber-family");
     return DPKEM_EDEVICE;
   }
   return DPKEM_OK;

@@ -347,7 +347,7 @@ class Engine:
 
     def roll(self, scheme: int, n: int) -> "Roll":
         """A rolling keygen -> encaps -> decaps pipeline over three batch
-        slots of n instances (dpkem_roll_*; Saber family)."""
+        slots of n instances (dpkem_roll_*; any scheme)."""
         return Roll(self, scheme, n)
 
     def plan(self, nstreams: int = 1, chunks: int = 1, share_matrix: bool = False, pipeline: bool = True,
@@ -543,7 +543,7 @@ def make_requests(n: int) -> np.ndarray:
 
 class Roll:
     """A rolling keygen -> encaps -> decaps pipeline (dpkem_roll_*): three
-    batch slots of n instances of one Saber-family scheme.  Step k generates
+    batch slots of n instances of one scheme.  Step k generates
     the keys of slot k % 3, encapsulates to those of slot (k - 1) % 3 and
     decapsulates the ciphertexts of slot (k - 2) % 3; after step k >= 2 every
     output of slot (k - 2) % 3 is complete until step k + 1."""

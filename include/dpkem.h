@@ -220,15 +220,18 @@ int dpkem_plan_destroy(dpkem_plan *p);
 
 /* ---------------------------------------------------------------- rolling pipelines
  * A rolling keygen -> encaps -> decaps pipeline for a stream of batches of
- * one Saber-family scheme (DPKEM_SABER / LIGHTSABER / FIRESABER): three batch
+ * one scheme (any DPKEM_* scheme): three batch
  * slots of n instances, each bound to device arrays (roles as in
  * dpkem_keypair / enc / dec: keygen coins, encaps coins, pk, sk, ct, encaps'
  * ss, decaps' ss2).  Step k (k = 0, 1, ...) generates the keys of slot
  * k mod 3, encapsulates to the keys of slot (k - 1) mod 3 (generated by step
  * k - 1) and decapsulates the ciphertexts of slot (k - 2) mod 3 (encapsulated
  * by step k - 1); it also finishes the previous steps' batches (keygen's
- * H(pk) into sk, encaps' K into ss) in the same launches.  A step is two
- * kernel launches (all hash chains of the three ops, then all products).
+ * H(pk) into sk, encaps' K into ss) in the same launches.  A Saber-family
+ * step is two kernel launches (all hash chains of the three ops, then all
+ * products); a FrodoKEM step six (decaps' mu', all serial SHAKE chains of
+ * the three ops and the closing hashes in one launch, the keygen, encaps and
+ * re-encryption products, decaps' last ss block).
  * After step k (k >= 2) every output of slot (k - 2) mod 3 -- pk, sk, ct, ss,
  * ss2 -- is complete, and stays so until step k + 1 starts that slot's next
  * keygen; slots of steps before k = 2 hold garbage.  Each op recomputes
@@ -236,14 +239,15 @@ int dpkem_plan_destroy(dpkem_plan *p);
  * and the outputs are byte-identical to theirs.  Steps are graph replays (one
  * graph per step mod 3) on the engine's stream.  No counterpart in the
  * reference, whose drivers run one op on one batch at a time
- * (SABER-GPU/src/cuda_kernel.cu:19-299). */
+ * (SABER-GPU/src/cuda_kernel.cu:19-299, FrodoKEM976-GPU/src/cuda_kernel.cu:21-335). */
 typedef struct dpkem_roll dpkem_roll;
 int dpkem_roll_create(dpkem_engine *e, int scheme, size_t n, dpkem_roll **out);
 int dpkem_roll_bind(dpkem_roll *r, int slot, const uint8_t *kcoins, const uint8_t *ecoins, uint8_t *pk,
                     uint8_t *sk, uint8_t *ct, uint8_t *ss, uint8_t *ss2);
 /* flags: 0, or DPKEM_PLAN_NO_GRAPH (issue each step's launches directly, so
  * that the engine's profiler can time them).  Takes the engine's current
- * DPKEM_TUNE_SABER_POLY_IPW setting. */
+ * DPKEM_TUNE_SABER_POLY_IPW (Saber family) or DPKEM_TUNE_FRODO_THREADS
+ * (FrodoKEM) setting. */
 int dpkem_roll_build(dpkem_roll *r, unsigned flags);
 int dpkem_roll_step(dpkem_roll *r);
 int dpkem_roll_info(dpkem_roll *r, uint64_t *steps, size_t *scratch_bytes);

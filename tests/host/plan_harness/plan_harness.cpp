@@ -137,8 +137,9 @@ void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scr
   wr(d.ss_d, d.nd * z.ss);
   scr(scratch, saber_mix_scratch_bytes(l, d.nk, d.ne, d.nd));
 }
-void saber_roll_launch(hipStream_t st, int l, const SaberRollDesc& d, int) {
-  const Sz& z = kSz[scheme_of_l(l)];
+// A rolling step's accesses: its three slots' arrays in the roles of the
+// three ops, and each slot's scratch of `per` bytes written in full.
+void roll_access(hipStream_t st, const Sz& z, const RollDesc& d, size_t per) {
   const size_t n = (size_t)d.n;
   if (d.n <= 0) fakehip::fail("rolling step with no instances");
   launch_on(st);
@@ -153,13 +154,15 @@ void saber_roll_launch(hipStream_t st, int l, const SaberRollDesc& d, int) {
   rd(d.sk_d, n * z.sk);
   wr(d.ss_d, n * z.ss);
   wr(d.ss_e_prev, n * z.ss);
-  // the three slots' scratch: distinct regions, each written in full (a step
-  // is one launch sequence of its own: only its own regions are compared)
+  // the three slots' scratch: distinct regions (a step is one launch
+  // sequence of its own: only its own regions are compared)
   g_regions.clear();
-  const size_t per = saber_mix_scratch_bytes(l, n, n, n);
   scr(d.scratch_k, per);
   scr(d.scratch_e, per);
   scr(d.scratch_d, per);
+}
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int) {
+  roll_access(st, kSz[scheme_of_l(l)], d, saber_mix_scratch_bytes(l, (size_t)d.n, (size_t)d.n, (size_t)d.n));
 }
 // ---- Frodo
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,
@@ -216,6 +219,10 @@ void frodo_pipe_launch(hipStream_t st, int n, const uint8_t* kc, const uint8_t* 
   wr(ss, n * z.ss);
   wr(ss2, n * z.ss);
   scr(scratch, carve(n, {8 * (size_t)set, 8 * (size_t)set, 64, 64, 64, 32, 4, 200}));
+}
+size_t frodo_roll_scratch_bytes(int set, size_t n) { return n * frodo_pipe_scratch_per_inst(set) + 8 * 256; }
+void frodo_roll_launch(hipStream_t st, int set, bool, const RollDesc& d, int) {
+  roll_access(st, kSz[frodo_scheme(set)], d, frodo_roll_scratch_bytes(set, (size_t)d.n));
 }
 void frodo_enc_grouped_launch(hipStream_t st, int n, const uint8_t* pk, size_t pk_stride, const uint8_t* coins,
                               uint8_t* ct, uint8_t* ss, uint32_t* s8, uint32_t* e8, uint32_t* epp8, uint8_t* kbuf,
@@ -289,6 +296,11 @@ hipError_t saber_wait_expired(uint32_t* count, bool) {
   return hipSuccess;
 }
 hipError_t saber_set_wait_spin_limit(uint32_t) { return hipSuccess; }
+hipError_t frodo_wait_expired(uint32_t* count, bool) {
+  *count = 0;
+  return hipSuccess;
+}
+hipError_t frodo_set_wait_spin_limit(uint32_t) { return hipSuccess; }
 void saber_probe_perm_launch(hipStream_t st, int, uint32_t* out) {
   launch_on(st);
   wr(out, 64 * sizeof(uint32_t));
@@ -408,9 +420,10 @@ int main() {
       plans += 2;
     }
   }
-  // rolling pipelines of every Saber-family rank, with and without graphs,
-  // several steps and a rebuild
-  for (int scheme : {DPKEM_SABER, DPKEM_LIGHTSABER, DPKEM_FIRESABER}) {
+  // rolling pipelines of every scheme, with and without graphs, several
+  // steps and a rebuild
+  for (int scheme : {DPKEM_SABER, DPKEM_LIGHTSABER, DPKEM_FIRESABER, DPKEM_FRODO976, DPKEM_FRODO976_AES,
+                     DPKEM_FRODO640, DPKEM_FRODO1344}) {
     for (unsigned flags : {0u, DPKEM_PLAN_NO_GRAPH}) {
       for (size_t n : {(size_t)1, (size_t)37, (size_t)300}) {
         Arr A;

@@ -15,10 +15,13 @@ GPU's bytes with the CPU oracle on the same coins:
       decaps inside each step, one chunk, product 1 and 4 instances per
       workgroup), three graph replays with the outputs cleared and unrelated
       work in between; every instance of every replay
-  C3  FrodoKEM-976, batch 8192: the pipelined plan on the 1x1 and 8x4
-      schedules and with the 1024-lane fused shape; decaps recovers every
-      shared secret, and 64 instances (0 and 8191 among them) equal the
-      oracle on pk, sk, ct and ss
+  C3  FrodoKEM-976, batch 8192: the headline's rolling pipeline (default and
+      1024-lane fused shapes): a slot's outputs cleared when its batch
+      starts, unrelated work between steps; two steps later decaps recovers
+      every shared secret and 64 instances of the slot (0 and 8191 among
+      them) equal the oracle on pk, sk, ct and both ss; and the chained
+      pipelined plan on the 1x1 and 8x4 schedules and with the 1024-lane
+      fused shape, checked the same way
   C4  16,384 instances in batches of U[1, 4096] (random scheme, ops 1:4:4),
       one autotuned graph: equal to the plain form everywhere, and sampled
       instances of every batch equal the oracle
@@ -186,6 +189,45 @@ def test_c2_saber_chained_plan_at_4096_every_instance(eng, oracle, torch_cuda, i
 
 
 # ------------------------------------------------------------------ C3
+@pytest.mark.parametrize("threads", [0, 1024])
+def test_c3_frodo_rolling_steps_at_8192(eng, oracle, torch_cuda, threads):
+    import bench
+    torch = torch_cuda
+    n = 8192
+    print(f"C3: FrodoKEM-976 rolling pipeline, n={n} per slot, fused threads {threads or 'default'}", flush=True)
+    slots = bench.rolling_slots(torch, FRODO, n, 0x44504B454D000002)
+    rng = np.random.default_rng(11)
+    idx = np.unique(np.concatenate([[0, 1, 4095, 4096, 8191], rng.choice(n, 59, replace=False)]))
+    ref = []
+    for sl in slots:
+        opk, osk = oracle.keypair(FRODO, _host(sl["kc"])[idx], threads=THREADS)
+        oct_, oss = oracle.enc(FRODO, opk, _host(sl["ec"])[idx], threads=THREADS)
+        ref.append((opk, osk, oct_, oss))
+    r = bench.build_roll(eng, FRODO, slots, graph=True, threads=threads)
+    junk = torch.zeros((64, 64), dtype=torch.uint8, device="cuda")
+    try:
+        for step in range(5):
+            _clear(torch, slots[step % 3])
+            r.step()
+            torch.cuda.synchronize()
+            if step >= 2:
+                c = (step - 2) % 3
+                sl, (opk, osk, oct_, oss) = slots[c], ref[c]
+                assert torch.equal(sl["ss"], sl["ss2"]), f"decaps lost a shared secret (step {step}, slot {c})"
+                assert bool((sl["ss"] != 0).any(dim=1).all()), f"an instance of slot {c} left its ss unwritten"
+                _eq(_host(sl["pk"])[idx], opk, f"pk (step {step}, slot {c})", idx)
+                _eq(_host(sl["sk"])[idx], osk, f"sk (step {step}, slot {c})", idx)
+                _eq(_host(sl["ct"])[idx], oct_, f"ct (step {step}, slot {c})", idx)
+                _eq(_host(sl["ss"])[idx], oss, f"encaps ss (step {step}, slot {c})", idx)
+                _eq(_host(sl["ss2"])[idx], oss, f"decaps ss (step {step}, slot {c})", idx)
+            # unrelated eager work between steps, on the same engine
+            eng.keypair(FRODO, junk[:2])
+            eng.sync()
+    finally:
+        r.close()
+        eng.set_tuning(3, 0)  # later plans of this engine: default fused shape
+
+
 @pytest.mark.parametrize("schedule", [(1, 1, 512), (8, 4, 512), (1, 1, 1024)])
 def test_c3_frodo_plan_at_8192(eng, oracle, torch_cuda, schedule):
     import bench

@@ -1,7 +1,9 @@
 """A bounded wait that gives up inside a kernel is reported, not hidden.
 
-Some Saber-family kernels hand data between waves of a workgroup through LDS
-counts and wait for them with a bounded spin (keccak2.hpp lds_wait_count).
+Some Saber-family kernels, and the FrodoKEM noise chains with their sampler
+waves, hand data between waves of a workgroup through LDS counts and wait
+for them with a bounded spin (keccak2.hpp lds_wait_count; each kernel file
+keeps its own give-up counter, and the engine reads both).
 If a wait reached its bound, the wave would go on with unpublished LDS and
 write wrong results; the kernel records that, and the engine's next sync
 returns DPKEM_EDEVICE.  The give-up is forced from the host: in a fresh
@@ -53,6 +55,13 @@ SCRIPT = textwrap.dedent("""
         print("NO_ERROR", flush=True)
     except D.DpkemError as exc:
         print("REPORTED", exc.code, str(exc)[:120], flush=True)
+    fk = torch.from_numpy(O.prng_bytes(4244, 40 * 64).reshape(40, 64)).cuda()
+    try:
+        eng.keypair(D.FRODO976, fk[:1])  # eager FrodoKEM keygen: its noise chain hands blocks to a sampler wave
+        eng.sync()
+        print("FRODO_NO_ERROR", flush=True)
+    except D.DpkemError as exc:
+        print("FRODO_REPORTED", exc.code, " ".join(str(exc)[:160].split()), flush=True)
     p.close(); eng.close()
     del os.environ["DPKEM_WAIT_SPIN_LIMIT"]
     eng = D.Engine(0)
@@ -63,6 +72,10 @@ SCRIPT = textwrap.dedent("""
     oct_, oss = O.enc(D.SABER, opk, ec, threads=8)
     ok = (np.array_equal(a["pk"].cpu().numpy(), opk) and np.array_equal(a["ct"].cpu().numpy(), oct_)
           and np.array_equal(a["ss"].cpu().numpy(), oss) and np.array_equal(a["ss2"].cpu().numpy(), oss))
+    fpk, fsk = eng.keypair(D.FRODO976, fk)
+    eng.sync()
+    gpk, gsk = O.keypair(D.FRODO976, fk.cpu().numpy(), threads=8)
+    ok = ok and np.array_equal(fpk.cpu().numpy(), gpk) and np.array_equal(fsk.cpu().numpy(), gsk)
     print("CLEAN_RUN_OK" if ok else "CLEAN_RUN_DIFFERS", flush=True)
     p.close(); eng.close()
 """) % ROOT
@@ -78,4 +91,5 @@ def test_expired_wait_is_reported_by_sync():
     assert r.returncode == 0, r.stderr[-2000:]
     lines = r.stdout.split("\n")
     assert any(ln.startswith("REPORTED -3") for ln in lines), r.stdout
+    assert any(ln.startswith("FRODO_REPORTED -3") and "FrodoKEM" in ln for ln in lines), r.stdout
     assert "CLEAN_RUN_OK" in lines, r.stdout

@@ -1,7 +1,8 @@
 """The rolling keygen -> encaps -> decaps pipeline (dpkem_roll_*) against the
-oracle: every output of every batch of every Saber-family rank, at ragged
-sizes, graph-replayed and issued directly, and decapsulations of tampered
-ciphertexts (implicit rejection) inside the pipeline."""
+oracle: every output of every batch of every scheme (Saber family, FrodoKEM
+sets), at ragged sizes, graph-replayed and issued directly, and
+decapsulations of tampered ciphertexts (implicit rejection) inside the
+pipeline."""
 import hashlib
 import os
 import sys
@@ -15,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-SABER, LIGHTSABER, FIRESABER = 1, 3, 4
+FRODO976, SABER, FRODO976_AES, LIGHTSABER, FIRESABER, FRODO640, FRODO1344 = 0, 1, 2, 3, 4, 5, 6
 
 
 @pytest.fixture(scope="module")
@@ -33,6 +34,7 @@ def eng(torch_cuda):
     e.set_stream(torch_cuda.cuda.current_stream().cuda_stream)
     yield e
     e.set_tuning(D.TUNE_SABER_POLY_IPW, 1)
+    e.set_tuning(D.TUNE_FRODO_THREADS, 0)
     e.close()
 
 
@@ -53,17 +55,23 @@ def _slots(torch, oracle, scheme, n, seed):
     return slots, ref
 
 
-@pytest.mark.parametrize("scheme,n,ipw,graph", [
+# shape: the Saber product's instances per workgroup, or the Frodo fused
+# kernels' workgroup size (0 = the set's default, 1024 = FrodoKEM-976's other)
+@pytest.mark.parametrize("scheme,n,shape,graph", [
     (SABER, 300, 1, True), (SABER, 37, 4, True), (SABER, 1, 2, True), (SABER, 65, 1, False),
     (LIGHTSABER, 130, 4, True), (LIGHTSABER, 33, 1, False),
     (FIRESABER, 97, 1, True), (FIRESABER, 5, 4, False),
+    (FRODO976, 300, 0, True), (FRODO976, 37, 1024, True), (FRODO976, 1, 0, False),
+    (FRODO976_AES, 33, 0, True), (FRODO640, 40, 0, True), (FRODO640, 3, 0, False),
+    (FRODO1344, 29, 0, True),
 ])
-def test_roll_every_output_of_every_batch(eng, oracle, torch_cuda, scheme, n, ipw, graph):
+def test_roll_every_output_of_every_batch(eng, oracle, torch_cuda, scheme, n, shape, graph):
     import bench
     torch = torch_cuda
-    print(f"roll: scheme {scheme}, n {n}, ipw {ipw}, graph {graph}", flush=True)
+    print(f"roll: scheme {scheme}, n {n}, shape {shape}, graph {graph}", flush=True)
     slots, ref = _slots(torch, oracle, scheme, n, 5000 + n)
-    r = bench.build_roll(eng, scheme, slots, ipw, graph=graph)
+    frodo = scheme in (FRODO976, FRODO976_AES, FRODO640, FRODO1344)
+    r = bench.build_roll(eng, scheme, slots, 1 if frodo else shape, graph=graph, threads=shape if frodo else 0)
     try:
         for step in range(7):
             for t in ("pk", "sk", "ct", "ss", "ss2"):
@@ -80,11 +88,23 @@ def test_roll_every_output_of_every_batch(eng, oracle, torch_cuda, scheme, n, ip
         r.close()
 
 
-@pytest.mark.parametrize("scheme", [SABER, LIGHTSABER, FIRESABER])
+def _rejection_key(scheme, sk_row, ct_row):
+    """The implicit-rejection shared secret of a decapsulation of ct_row:
+    Saber family SHA3-256(z || SHA3-256(c)) with z the last 32 bytes of sk;
+    FrodoKEM SHAKE(c || s) with s the first ss-length bytes of sk (SHAKE128
+    for Frodo-640, SHAKE256 otherwise)."""
+    if scheme in (SABER, LIGHTSABER, FIRESABER):
+        return hashlib.sha3_256(sk_row[-32:] + hashlib.sha3_256(ct_row).digest()).digest()
+    ln = {FRODO640: 16, FRODO1344: 32}.get(scheme, 24)
+    h = hashlib.shake_128 if scheme == FRODO640 else hashlib.shake_256
+    return h(ct_row + sk_row[:ln]).digest(ln)
+
+
+@pytest.mark.parametrize("scheme", [SABER, LIGHTSABER, FIRESABER, FRODO976, FRODO640])
 def test_roll_decapsulates_tampered_ciphertexts_by_implicit_rejection(eng, oracle, torch_cuda, scheme):
     """Between the step that encapsulates slot 0 and the step that
     decapsulates it, every third ciphertext of the slot gets one bit flipped:
-    those decapsulations must return SHA3-256(z || SHA3-256(c)) with the
+    those decapsulations must return the implicit-rejection key of the
     flipped c, the others the oracle's shared secret."""
     import bench
     torch = torch_cuda
@@ -104,7 +124,7 @@ def test_roll_decapsulates_tampered_ciphertexts_by_implicit_rejection(eng, oracl
         got = slots[0]["ss2"].cpu().numpy()
         for i in range(n):
             if i % 3 == 0:
-                want = hashlib.sha3_256(sk[i].tobytes()[-32:] + hashlib.sha3_256(ct[i].tobytes()).digest()).digest()
+                want = _rejection_key(scheme, sk[i].tobytes(), ct[i].tobytes())
                 assert got[i].tobytes() == want, f"instance {i}: not the implicit-rejection key"
             else:
                 assert np.array_equal(got[i], ref[0][3][i]), f"instance {i}: decaps differs from the oracle"
@@ -116,7 +136,9 @@ def test_roll_rejects_misuse(eng, torch_cuda):
     import dpcrypto_amd as D
     from dpcrypto_amd.kem import DpkemError
     with pytest.raises(DpkemError):
-        eng.roll(D.FRODO976, 4)  # Saber family only
+        eng.roll(7, 4)  # no such scheme
+    with pytest.raises(DpkemError):
+        eng.roll(D.FRODO976, 0)  # no instances
     r = eng.roll(D.SABER, 4)
     with pytest.raises(DpkemError):
         r.build()  # slots not bound
