@@ -14,8 +14,13 @@ steps before).  Each batch therefore goes through all three ops, one per
 step, and every op consumes what an earlier op produced; within a step the
 three ops are independent and run as two launches (dpkem_roll_*): all hash
 chains of the three ops plus the closing hashes of the previous steps'
-batches (keygen's H(pk), encaps' K) in one, all products in the other.  Every op recomputes everything from its own inputs (encaps expands the
-public matrix from pk, decaps from sk), as separate parties would.
+batches (keygen's H(pk), encaps' K) in one, all products in the other.
+The autotuner also tries the overlapped form over six slots (encaps to the
+keys of two steps earlier, decaps of the ciphertexts of two steps earlier),
+in which step r + 1's hash launch runs on one stream beside step r's
+products on another.  Every op recomputes everything from its own inputs
+(encaps expands the public matrix from pk, decaps from sk), as separate
+parties would.
 
 Reported beside it in the same JSON line:
   saber_chained  the same ops with the keygen -> encaps -> decaps chain of ONE
@@ -460,83 +465,92 @@ def run_plan_workload(torch, eng, scheme, n, steps, warmup, seed, tune_budget_s=
                           "tuning_ms": {"x".join(map(str, c)): round(v, 4) for c, v in tried.items()}})
 
 
-def rolling_slots(torch, scheme, n, seed):
-    """Three batch slots of device-resident arrays (chain_arrays) for the
-    rolling keygen -> encaps -> decaps pipeline."""
-    return [chain_arrays(torch, scheme, n, seed + 7919 * k) for k in range(3)]
+def rolling_slots(torch, scheme, n, seed, count=3):
+    """`count` batch slots of device-resident arrays (chain_arrays) for the
+    rolling keygen -> encaps -> decaps pipeline (3, or 6 for the overlapped
+    form)."""
+    return [chain_arrays(torch, scheme, n, seed + 7919 * k) for k in range(count)]
 
 
-def build_roll(eng, scheme, slots, ipw=1, graph=True, threads=0):
-    """The rolling pipeline (dpkem_roll_*) over the three slots: step k
-    generates the keys of slot k % 3, encapsulates to the keys of slot
-    (k - 1) % 3 (generated by step k - 1) and decapsulates the ciphertexts of
-    slot (k - 2) % 3 (encapsulated by step k - 1), and finishes the previous
+def build_roll(eng, scheme, slots, ipw=1, graph=True, threads=0, overlap=False):
+    """The rolling pipeline (dpkem_roll_*) over the slots: step k generates
+    the keys of slot k % 3, encapsulates to the keys of slot (k - 1) % 3
+    (generated by step k - 1) and decapsulates the ciphertexts of slot
+    (k - 2) % 3 (encapsulated by step k - 1), and finishes the previous
     steps' batches (keygen's H(pk), encaps' K) in the same launches.
-    ipw: the Saber product's instances per workgroup; threads: the Frodo
-    fused kernels' workgroup size (0 = the set's default)."""
+    overlap: the six-slot form (slots k, k - 2, k - 4 mod 6; hash launches
+    beside the previous step's products).  ipw: the Saber product's
+    instances per workgroup; threads: the Frodo fused kernels' workgroup
+    size (0 = the set's default)."""
     import dpcrypto_amd as D
     eng.set_tuning(D.TUNE_SABER_POLY_IPW, ipw)
     eng.set_tuning(D.TUNE_FRODO_THREADS, threads)
     r = eng.roll(scheme, slots[0]["kc"].shape[0])
-    for k, sl in enumerate(slots):
+    for k, sl in enumerate(slots[:6 if overlap else 3]):
         r.bind(k, sl["kc"], sl["ec"], sl["pk"], sl["sk"], sl["ct"], sl["ss"], sl["ss2"])
-    return r.build(graph=graph)
+    return r.build(graph=graph and not overlap, overlap=overlap)
 
 
 def roll_shapes(scheme):
-    """Launch-shape candidates of a rolling pipeline: the Saber product's
-    instances per workgroup, or the Frodo fused kernels' workgroup size
-    (FrodoKEM-976: the default 512 lanes in two passes over A's rows, or
-    1024 in one)."""
+    """Launch-shape candidates of a rolling pipeline: for the Saber family
+    (overlapped form, product instances per workgroup), for FrodoKEM the
+    fused kernels' workgroup size (FrodoKEM-976: the default 512 lanes in
+    two passes over A's rows, or 1024 in one)."""
     import dpcrypto_amd as D
     if scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER):
-        return [1, 2, 4]
+        return [(o, c) for o in (False, True) for c in (1, 2, 4)]
     return [0, 1024] if scheme in (D.FRODO976, D.FRODO976_AES) else [0]
 
 
 def build_roll_shape(eng, scheme, slots, shape, graph=True):
     import dpcrypto_amd as D
     if scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER):
-        return build_roll(eng, scheme, slots, shape, graph=graph)
+        overlap, ipw = shape
+        return build_roll(eng, scheme, slots, ipw, graph=graph, overlap=overlap)
     return build_roll(eng, scheme, slots, 1, graph=graph, threads=shape)
 
 
 def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
-    """Time `steps` steps of the rolling pipeline (graph replays).  The
-    launch shape (roll_shapes; `ipw` fixes it) is picked first by timing
-    two cycles of three steps each.  Afterwards two more steps finish the
-    batches still in flight and every slot must hold ss == ss2 (encaps' and
-    decaps' shared secrets)."""
+    """Time `steps` steps of the rolling pipeline (graph replays, or direct
+    launches on two streams in the overlapped form).  The launch shape
+    (roll_shapes; `ipw` fixes the Saber product's and keeps the three-slot
+    form) is picked first by timing two cycles of steps.  Afterwards the
+    batches still in flight are finished and every slot must hold ss == ss2
+    (encaps' and decaps' shared secrets)."""
     import statistics
     import dpcrypto_amd as D
     saber = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
-    slots = rolling_slots(torch, scheme, n, seed)
+    shapes = roll_shapes(scheme) if ipw is None else [(False, ipw)]
+    slots = rolling_slots(torch, scheme, n, seed, 6 if any(isinstance(s, tuple) and s[0] for s in shapes) else 3)
     stream = torch.cuda.current_stream(torch.device("cuda:0"))
     eng.set_stream(stream.cuda_stream)
 
-    def cycle_ms(r, reps):
+    def cycle_ms(r, reps, per):
         ts = []
         for _ in range(reps):
             x, y = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             x.record(stream)
-            for _ in range(3):
+            for _ in range(per):
                 r.step()
             y.record(stream)
-            y.synchronize()
-            ts.append(x.elapsed_time(y) / 3)
+            torch.cuda.synchronize()
+            ts.append(x.elapsed_time(y) / per)
         return statistics.median(ts)
 
+    def lag(shape):  # steps from a batch's keygen to its last output
+        return 4 if isinstance(shape, tuple) and shape[0] else 2
+
     tried = {}
-    shapes = [ipw] if ipw is not None else roll_shapes(scheme)
     for c in shapes:
         r = build_roll_shape(eng, scheme, slots, c)
         if len(shapes) > 1:
-            cycle_ms(r, 1 if not saber else 2)
-            tried[c] = cycle_ms(r, 2 if not saber else 4)
+            per = 6 if lag(c) == 4 else 3
+            cycle_ms(r, 1 if not saber else 2, per)
+            tried[c] = cycle_ms(r, 2 if not saber else 4, per)
         r.close()
     best = min(tried, key=tried.get) if tried else shapes[0]
     r = build_roll_shape(eng, scheme, slots, best)
-    for _ in range(max(warmup, 3)):  # every slot through keygen, encaps and decaps before timing
+    for _ in range(max(warmup, lag(best) + 1)):  # every slot through keygen, encaps and decaps before timing
         r.step()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     stops = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
@@ -548,30 +562,37 @@ def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
         stops[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    for _ in range(2):
+    for _ in range(lag(best) + 1):
         r.step()
     torch.cuda.synchronize()
     per_step = [x.elapsed_time(y) for x, y in zip(starts, stops)]
-    ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots)
+    ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots[:6 if lag(best) == 4 else 3])
     r.close()
-    batch = [sum(per_step[i:i + 3]) for i in range(max(1, len(per_step) - 2))]
+    span = lag(best) + 1
+    batch = [sum(per_step[i:i + span]) for i in range(max(1, len(per_step) - span + 1))]
     if saber:
-        sched = {"pipeline": "dpkem_roll: two launches per step", "saber_poly_ipw": best,
-                 "tuning_ms": {f"ipw{c}": round(v, 4) for c, v in tried.items()}}
+        overlap, ipw_best = best
+        sched = {"pipeline": ("dpkem_roll overlapped: six slots, hash launches on one stream beside the previous "
+                              "step's products on another" if overlap else "dpkem_roll: two launches per step"),
+                 "saber_poly_ipw": ipw_best, "overlap": overlap,
+                 "tuning_ms": {f"{'overlap' if o else 'plain'}-ipw{c}": round(v, 4) for (o, c), v in tried.items()}}
     else:
         sched = {"pipeline": "dpkem_roll: six launches per step (decaps' mu', every chain in one launch, three "
                              "fused products, decaps' last ss block)", "frodo_threads": best or None,
                  "tuning_ms": {f"threads{c or 'default'}": round(v, 3) for c, v in tried.items()}}
-    return dict(wall_s=wall, per_step_ms=per_step, batch_ms=batch, roundtrip_ok=ok, ops_per_s=3 * n * steps / wall,
-                schedule=sched)
+    return dict(wall_s=wall, per_step_ms=per_step, batch_ms=batch, batch_steps=span, roundtrip_ok=ok,
+                ops_per_s=3 * n * steps / wall, schedule=sched)
 
 
 def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=None):
     """Per-kernel HIP-event times of the rolling steps' kernels, launched
-    without a graph (same launches and kernels); ipw: the launch shape
-    (roll_shapes), by default the first candidate."""
+    without a graph in the three-slot form (same kernels; each runs alone);
+    ipw: the launch shape (roll_shapes), by default the first candidate."""
     if ipw is None:
         ipw = roll_shapes(scheme)[0]
+    import dpcrypto_amd as D
+    if scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER):
+        ipw = (False, ipw[1] if isinstance(ipw, tuple) else ipw)
     slots = rolling_slots(torch, scheme, n, seed)
     eng.set_stream(torch.cuda.current_stream(torch.device("cuda:0")).cuda_stream)
     r = build_roll_shape(eng, scheme, slots, ipw, graph=False)
@@ -596,14 +617,31 @@ def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=None):
 # Op-level ceilings (ops/s) from the algorithmic work of one op (Keccak
 # permutations at 4,176 VALU lane-ops each plus MACs at 2 per lane-op) at the
 # VALU peak; keygen+enc+dec at 1:1:1 is their harmonic combination.
-OP_CEILING = {
+# The serial-chain Keccak work sets them: permutations per op at 4,176 VALU
+# lane-ops each over the VALU peak.  The polynomial and matrix products run
+# on int8 MFMA beside the VALU (for Saber 9 / 12 / 12 x 65,536 13-bit MACs
+# per op, two int8 limb MACs each: under 1% of a keygen's Keccak time at
+# the 5 POP/s int8 peak), so they add no term.  Saber keygen: seedA 1,
+# GenMatrix 23, GenSecret 5, H(pk) 8; encaps: GenMatrix 23, m 1, H(pk) 8,
+# G 1, GenSecret 5, H(ct) 9, K 1; decaps: GenMatrix 23, H(ct) 9, both K 2,
+# G 1, GenSecret 5.  FrodoKEM-976: A's 976 rows x 12 SHAKE128 blocks plus
+# the SHAKE256 chains (keygen 1 + 230 + 115, encaps 115 + 1 + 231 + 116,
+# decaps 1 + 231 + 116).
+OP_PERMS = {
+    ("saber", "keygen"): 37, ("saber", "enc"): 48, ("saber", "dec"): 40,
+    ("frodo976", "keygen"): 976 * 12 + 346, ("frodo976", "enc"): 976 * 12 + 463, ("frodo976", "dec"): 976 * 12 + 348,
+}
+OP_CEILING = {k: VALU_PEAK / (v * KECCAK_LANE_OPS) for k, v in OP_PERMS.items()}
+# The same ceilings with the Saber products' MACs also counted on the VALU
+# (2 MACs per lane-op), as first estimated before the products moved to MFMA.
+OP_CEILING_PRODUCTS_ON_VALU = {
     ("saber", "keygen"): 87.5e6, ("saber", "enc"): 66.2e6, ("saber", "dec"): 60.1e6,
-    ("frodo976", "keygen"): 780900.0, ("frodo976", "enc"): 773396.0, ("frodo976", "dec"): 780770.0,
 }
 
 
-def chain_ceiling(name):
-    return 3.0 / sum(1.0 / OP_CEILING[(name, op)] for op in ("keygen", "enc", "dec"))
+def chain_ceiling(name, table=None):
+    table = OP_CEILING if table is None else table
+    return 3.0 / sum(1.0 / table[(name, op)] for op in ("keygen", "enc", "dec"))
 
 
 def run_per_op(torch, eng, plain_eng, scheme, name, n, steps, seed, tune_budget_s=10.0):
@@ -1039,15 +1077,20 @@ def main():
         "dtype": "u16/u32 integer (mod 2^13 / 2^16)",
         "data": "synthetic seeded coins",
         "config": {"workload": f"saber-l3 batch {args.saber_batch}: {args.saber_batch} keygens + "
-                               f"{args.saber_batch} encaps + {args.saber_batch} decaps per step; rolling pipeline "
-                               "over three batch slots (encaps to the keys of the previous step, decaps of the "
-                               "ciphertexts of the previous step; a batch is complete two steps after its keygen)",
+                               f"{args.saber_batch} encaps + {args.saber_batch} decaps per step; "
+                               + ("rolling pipeline over six batch slots (encaps to the keys of two steps earlier, "
+                                  "decaps of the ciphertexts of two steps earlier; each step's hash launch runs "
+                                  "beside the previous step's products; a batch is complete four steps after its "
+                                  "keygen)" if sab["schedule"].get("overlap") else
+                                  "rolling pipeline over three batch slots (encaps to the keys of the previous "
+                                  "step, decaps of the ciphertexts of the previous step; a batch is complete two "
+                                  "steps after its keygen)"),
                    "global_batch": args.saber_batch, "parallelism": "instances (1 GPU)"},
         "latency_ms": {"p50": round(pct(sab["per_step_ms"], 0.5), 4), "p99": round(pct(sab["per_step_ms"], 0.99), 4),
                        "what": "per-step latency (one step = the three ops over 4096 instances each)",
                        "batch_p50": round(pct(sab["batch_ms"], 0.5), 4),
                        "batch_p99": round(pct(sab["batch_ms"], 0.99), 4),
-                       "batch_what": "one batch from keygen to decaps: three consecutive steps"},
+                       "batch_what": f"one batch from keygen to decaps: {sab['batch_steps']} consecutive steps"},
         "roundtrip_ok": sab["roundtrip_ok"] and sab_prof["roundtrip_ok"] and sab_eager["roundtrip_ok"]
                         and chained["roundtrip_ok"] and chained_prof["roundtrip_ok"],
         "schedule": sab["schedule"],
@@ -1072,7 +1115,11 @@ def main():
         result["per_op"] = {"saber": run_per_op(torch, eng, pe, D.SABER, "saber", args.saber_batch, args.steps,
                                                 0x44504B454D000001)}
     result["ceiling_ops_per_s"] = {"saber_keygen_enc_dec": round(chain_ceiling("saber"), 1),
-                                   "frodo976_keygen_enc_dec": round(chain_ceiling("frodo976"), 1)}
+                                   "frodo976_keygen_enc_dec": round(chain_ceiling("frodo976"), 1),
+                                   "basis": "serial-chain Keccak permutations per op at 4,176 VALU lane-ops each "
+                                            "over the VALU peak; products on int8 MFMA alongside",
+                                   "saber_with_products_on_valu": round(
+                                       chain_ceiling("saber", OP_CEILING_PRODUCTS_ON_VALU), 1)}
     result["frac_of_ceiling"] = round(sab["ops_per_s"] / chain_ceiling("saber"), 4)
     if not args.skip_frodo:
         rolling, rolling_err = run_frodo_rolling_child(args, progress)

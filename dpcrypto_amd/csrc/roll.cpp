@@ -4,6 +4,17 @@
 // in one launch, the three fused products and decaps' last ss block
 // (frodo_roll_launch).  Each step mod 3 is captured once into a HIP graph
 // and replayed on the engine's stream.
+//
+// Overlapped form (Saber family, DPKEM_ROLL_OVERLAP, six slots): step k
+// generates the keys of slot k, encapsulates to slot k - 2 and decapsulates
+// slot k - 4 (mod 6).  Step k + 1's hash launch then reads nothing that
+// step k's products write (keygen's pk of slot k - 1 came from the products
+// of step k - 1, the ciphertexts of slot k - 3 from those of step k - 1 too),
+// so the hash launches run on one stream and the products on another: the
+// chain waves of step k + 1 (one per SIMD, latency-bound) share the chip with
+// step k's products.  Per step, with events: hashes(k) wait for the engine
+// stream's work so far (which has joined products(k - 2)); products(k) wait
+// for hashes(k); the engine stream then joins products(k - 1).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 
@@ -23,8 +34,12 @@ struct dpkem_roll {
   struct Slot {
     const uint8_t *kc = nullptr, *ec = nullptr;
     uint8_t *pk = nullptr, *sk = nullptr, *ct = nullptr, *ss = nullptr, *ss2 = nullptr;
-  } slot[3];
-  bool bound[3] = {false, false, false};
+  } slot[6];
+  bool bound[6] = {false, false, false, false, false, false};
+  int nslots = 3;  // 6 in the overlapped form
+  // overlapped form: the hash and product streams, and their events
+  hipStream_t hs = nullptr, ps = nullptr;
+  hipEvent_t ev_user = nullptr, ev_hash = nullptr, ev_prod[2] = {nullptr, nullptr};
   uint8_t* scratch = nullptr;
   size_t per_slot = 0;
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
@@ -48,6 +63,16 @@ int set_of(int scheme) {
 
 void release(dpkem_roll* r) {
   r->built = false;
+  if (r->hs) (void)hipStreamSynchronize(r->hs);
+  if (r->ps) (void)hipStreamSynchronize(r->ps);
+  for (hipEvent_t* ev : {&r->ev_user, &r->ev_hash, &r->ev_prod[0], &r->ev_prod[1]}) {
+    if (*ev) (void)hipEventDestroy(*ev);
+    *ev = nullptr;
+  }
+  for (hipStream_t* st : {&r->hs, &r->ps}) {
+    if (*st) (void)hipStreamDestroy(*st);
+    *st = nullptr;
+  }
   for (int k = 0; k < 3; ++k) {
     if (r->exec[k]) (void)hipGraphExecDestroy(r->exec[k]);
     if (r->graph[k]) (void)hipGraphDestroy(r->graph[k]);
@@ -58,10 +83,13 @@ void release(dpkem_roll* r) {
   r->scratch = nullptr;
 }
 
-// Step `phase` (mod 3): keygen of slot phase, encaps of slot phase - 1,
-// decaps of slot phase - 2.
-void issue(dpkem_roll* r, hipStream_t st, int phase) {
-  const int ks = phase, es = (phase + 2) % 3, ds = (phase + 1) % 3;
+// Step `phase`: keygen of slot phase, encaps of slot phase - 1 and decaps of
+// slot phase - 2 (mod 3), or in the overlapped form slots phase, phase - 2,
+// phase - 4 (mod 6); part: which launches (kRollBoth / kRollChains /
+// kRollProducts).
+void issue(dpkem_roll* r, hipStream_t st, int phase, int part = kRollBoth) {
+  const int S = r->nslots, lag = S == 6 ? 2 : 1;
+  const int ks = phase, es = (phase + S - lag) % S, ds = (phase + S - 2 * lag) % S;
   const auto &K = r->slot[ks], &E = r->slot[es], &Dd = r->slot[ds];
   RollDesc d{};
   d.n = (int)r->n;
@@ -80,7 +108,7 @@ void issue(dpkem_roll* r, hipStream_t st, int phase) {
   d.scratch_e = r->scratch + (size_t)es * r->per_slot;
   d.scratch_d = r->scratch + (size_t)ds * r->per_slot;
   if (r->set) frodo_roll_launch(st, r->set, r->aes, d, r->threads);
-  else saber_roll_launch(st, r->l, d, r->ipw);
+  else saber_roll_launch(st, r->l, d, r->ipw, part);
 }
 
 }  // namespace
@@ -104,7 +132,7 @@ int dpkem_roll_create(dpkem_engine* e, int scheme, size_t n, dpkem_roll** out) {
 
 int dpkem_roll_bind(dpkem_roll* r, int slot, const uint8_t* kcoins, const uint8_t* ecoins, uint8_t* pk, uint8_t* sk,
                     uint8_t* ct, uint8_t* ss, uint8_t* ss2) {
-  if (!r || r->built || slot < 0 || slot > 2) return DPKEM_EINVAL;
+  if (!r || r->built || slot < 0 || slot > 5) return DPKEM_EINVAL;
   const void* ptrs[7] = {kcoins, ecoins, pk, sk, ct, ss, ss2};
   for (const void* q : ptrs)
     if (!q || ((uintptr_t)q & 7)) return DPKEM_EINVAL;
@@ -114,19 +142,24 @@ int dpkem_roll_bind(dpkem_roll* r, int slot, const uint8_t* kcoins, const uint8_
 }
 
 int dpkem_roll_build(dpkem_roll* r, unsigned flags) {
-  if (!r || (flags & ~DPKEM_PLAN_NO_GRAPH) || !r->bound[0] || !r->bound[1] || !r->bound[2]) return DPKEM_EINVAL;
+  if (!r || (flags & ~(DPKEM_PLAN_NO_GRAPH | DPKEM_ROLL_OVERLAP))) return DPKEM_EINVAL;
+  const bool overlap = (flags & DPKEM_ROLL_OVERLAP) != 0;
+  if (overlap && r->set) return DPKEM_EINVAL;  // Saber family only
+  for (int k = 0; k < (overlap ? 6 : 3); ++k)
+    if (!r->bound[k]) return DPKEM_EINVAL;
   dpkem_engine* e = r->e;
   std::lock_guard<std::mutex> lock(e->mu);
   release(r);
   r->flags = flags;
+  r->nslots = overlap ? 6 : 3;
   r->ipw = e->saber_ipw;
   r->threads = e->frodo_threads;
   r->per_slot = ((r->set ? frodo_roll_scratch_bytes(r->set, r->n) : saber_mix_scratch_bytes(r->l, r->n, r->n, r->n)) +
                  255) &
                 ~(size_t)255;
-  if (3 * r->per_slot > e->budget) {
+  if (r->nslots * r->per_slot > e->budget) {
     snprintf(e->last_err, sizeof e->last_err, "rolling pipeline scratch %zu B exceeds HBM budget %zu B",
-             3 * r->per_slot, e->budget);
+             r->nslots * r->per_slot, e->budget);
     return DPKEM_EBUDGET;
   }
   hipError_t err;
@@ -140,8 +173,13 @@ int dpkem_roll_build(dpkem_roll* r, unsigned flags) {
     }                                           \
   } while (0)
   ROLL_TRY(hipSetDevice(e->device));
-  ROLL_TRY(hipMalloc(&r->scratch, 3 * r->per_slot));
-  if (!(flags & DPKEM_PLAN_NO_GRAPH)) {
+  ROLL_TRY(hipMalloc(&r->scratch, r->nslots * r->per_slot));
+  if (overlap) {  // launches issued directly on two streams of its own
+    ROLL_TRY(hipStreamCreateWithFlags(&r->hs, hipStreamNonBlocking));
+    ROLL_TRY(hipStreamCreateWithFlags(&r->ps, hipStreamNonBlocking));
+    for (hipEvent_t* ev : {&r->ev_user, &r->ev_hash, &r->ev_prod[0], &r->ev_prod[1]})
+      ROLL_TRY(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  } else if (!(flags & DPKEM_PLAN_NO_GRAPH)) {
     for (int ph = 0; ph < 3; ++ph) {
       hipStream_t cap;
       ROLL_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -179,8 +217,22 @@ int dpkem_roll_step(dpkem_roll* r) {
   std::lock_guard<std::mutex> lock(e->mu);
   hipError_t err = hipSetDevice(e->device);
   if (err != hipSuccess) return record_hip_error(e, err, "hipSetDevice");
-  const int phase = (int)(r->steps % 3);
-  if (r->flags & DPKEM_PLAN_NO_GRAPH) {
+  const int phase = (int)(r->steps % (uint64_t)r->nslots);
+  if (r->nslots == 6) {
+    const uint64_t k = r->steps;
+    tls_profiler = e->profiling ? &e->prof : nullptr;
+    bool ok = hipEventRecord(r->ev_user, e->stream) == hipSuccess &&
+              hipStreamWaitEvent(r->hs, r->ev_user, 0) == hipSuccess;
+    if (ok) issue(r, r->hs, phase, kRollChains);
+    ok = ok && hipEventRecord(r->ev_hash, r->hs) == hipSuccess &&
+         hipStreamWaitEvent(r->ps, r->ev_hash, 0) == hipSuccess;
+    if (ok) issue(r, r->ps, phase, kRollProducts);
+    ok = ok && hipEventRecord(r->ev_prod[k & 1], r->ps) == hipSuccess;
+    if (ok && k > 0) ok = hipStreamWaitEvent(e->stream, r->ev_prod[(k - 1) & 1], 0) == hipSuccess;
+    tls_profiler = nullptr;
+    err = hipGetLastError();
+    if (err == hipSuccess && !ok) err = hipErrorUnknown;
+  } else if (r->flags & DPKEM_PLAN_NO_GRAPH) {
     tls_profiler = e->profiling ? &e->prof : nullptr;
     issue(r, e->stream, phase);
     tls_profiler = nullptr;
@@ -196,7 +248,7 @@ int dpkem_roll_step(dpkem_roll* r) {
 int dpkem_roll_info(dpkem_roll* r, uint64_t* steps, size_t* scratch_bytes) {
   if (!r) return DPKEM_EINVAL;
   if (steps) *steps = r->steps;
-  if (scratch_bytes) *scratch_bytes = 3 * r->per_slot;
+  if (scratch_bytes) *scratch_bytes = r->nslots * r->per_slot;
   return DPKEM_OK;
 }
 

@@ -1710,7 +1710,7 @@ static void mix_l(hipStream_t st, const SaberMixDesc& d, uint8_t* scratch, int i
 // One step of the rolling pipeline (see saber_roll_launch in launch.hpp):
 // sab_roll_x0, then the mixed products.
 template <int L>
-static void roll_l(hipStream_t st, const RollDesc& d, int ipw) {
+static void roll_l(hipStream_t st, const RollDesc& d, int ipw, int part) {
   const int n = d.n;
   const MixScratch xk = mix_carve<L>(d.scratch_k, n, n, n);
   const MixScratch xe = mix_carve<L>(d.scratch_e, n, n, n);
@@ -1743,15 +1743,15 @@ static void roll_l(hipStream_t st, const RollDesc& d, int ipw) {
   m.ct_t = d.ct_d;         // encaps of two steps earlier: the ciphertexts decaps uses now
   m.krbuf_t = xd.krbuf_e;  // its kr, in that slot's encaps scratch
   m.ss_t = d.ss_e_prev;
-  DPKEM_LAUNCH(SAB_NAME("sab_roll_x0"), sab_roll_x0<L>, dim3(2 * m.gke), dim3(256), st, m);
-  mix_products<L>(st, m, ipw);
+  if (part != kRollProducts) DPKEM_LAUNCH(SAB_NAME("sab_roll_x0"), sab_roll_x0<L>, dim3(2 * m.gke), dim3(256), st, m);
+  if (part != kRollChains) mix_products<L>(st, m, ipw);
 }
 
-void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw) {
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw, int part) {
   if (d.n <= 0) return;
-  if (l == 2) roll_l<2>(st, d, ipw);
-  else if (l == 4) roll_l<4>(st, d, ipw);
-  else roll_l<3>(st, d, ipw);
+  if (l == 2) roll_l<2>(st, d, ipw, part);
+  else if (l == 4) roll_l<4>(st, d, ipw, part);
+  else roll_l<3>(st, d, ipw, part);
 }
 
 void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int ipw) {

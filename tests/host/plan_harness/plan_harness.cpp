@@ -161,7 +161,7 @@ void roll_access(hipStream_t st, const Sz& z, const RollDesc& d, size_t per) {
   scr(d.scratch_e, per);
   scr(d.scratch_d, per);
 }
-void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int) {
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int, int) {
   roll_access(st, kSz[scheme_of_l(l)], d, saber_mix_scratch_bytes(l, (size_t)d.n, (size_t)d.n, (size_t)d.n));
 }
 // ---- Frodo
@@ -444,6 +444,26 @@ int main() {
         check(dpkem_roll_destroy(r), "roll_destroy");
         ++plans;
       }
+    }
+  }
+  // the overlapped form: six slots, hash launches and products on two
+  // streams of the pipeline, every step joined by the engine's stream
+  for (int scheme : {DPKEM_SABER, DPKEM_LIGHTSABER, DPKEM_FIRESABER}) {
+    for (size_t n : {(size_t)1, (size_t)37, (size_t)300}) {
+      Arr A;
+      const Sz& z = kSz[scheme];
+      dpkem_roll* r = nullptr;
+      check(dpkem_roll_create(e, scheme, n, &r), "roll_create");
+      for (int k = 0; k < 6; ++k)
+        check(dpkem_roll_bind(r, k, A.get(n * z.kc), A.get(n * z.ec), A.get(n * z.pk), A.get(n * z.sk),
+                              A.get(n * z.ct), A.get(n * z.ss), A.get(n * z.ss)),
+              "roll_bind");
+      for (int b = 0; b < 2; ++b) {
+        check(dpkem_roll_build(r, DPKEM_ROLL_OVERLAP), "roll_build overlap");
+        for (int st = 0; st < 13; ++st) check(dpkem_roll_step(r), "roll_step overlap");
+      }
+      check(dpkem_roll_destroy(r), "roll_destroy");
+      ++plans;
     }
   }
   // the plain form (no chains, no merging)

@@ -8,10 +8,12 @@ GPU's bytes with the CPU oracle on the same coins:
       (eager entry points and the pipelined plan)
   C2  Saber l=3, batch 4096: the headline's rolling pipeline (three batch
       slots; each step keygen of one slot, encaps of the previous, decaps of
-      the one before; product 1, 2 and 4 instances per workgroup): a slot's
-      outputs cleared when its batch starts, unrelated work between steps,
-      and every output of every instance of every batch equal to the oracle
-      two steps later; and the chained plan (one batch's keygen -> encaps ->
+      the one before; product 1, 2 and 4 instances per workgroup; and the
+      overlapped six-slot form, hash launches beside the previous step's
+      products on a second stream): a slot's outputs cleared when its batch
+      starts, unrelated work between steps, and every output of every
+      instance of every batch equal to the oracle once complete (two steps
+      later, five in the overlapped form); and the chained plan (one batch's keygen -> encaps ->
       decaps inside each step, one chunk, product 1 and 4 instances per
       workgroup), three graph replays with the outputs cleared and unrelated
       work in between; every instance of every replay
@@ -121,30 +123,31 @@ def test_c1_frodo_64_keypairs_then_one_encaps_decaps_each(eng, oracle, torch_cud
 
 
 # ------------------------------------------------------------------ C2
-@pytest.mark.parametrize("ipw", [1, 2, 4])
-def test_c2_saber_rolling_steps_at_4096_every_instance(eng, oracle, torch_cuda, ipw):
+@pytest.mark.parametrize("overlap,ipw", [(False, 1), (False, 2), (False, 4), (True, 1), (True, 4)])
+def test_c2_saber_rolling_steps_at_4096_every_instance(eng, oracle, torch_cuda, overlap, ipw):
     import bench
     torch = torch_cuda
     n = 4096
-    print(f"C2: Saber rolling pipeline, n={n} per slot, ipw={ipw}", flush=True)
-    slots = bench.rolling_slots(torch, SABER, n, 0x44504B454D000001)
+    S, lag = (6, 5) if overlap else (3, 2)  # slots; steps from a batch's keygen step to the one it is complete after
+    print(f"C2: Saber rolling pipeline, n={n} per slot, {S} slots, ipw={ipw}", flush=True)
+    slots = bench.rolling_slots(torch, SABER, n, 0x44504B454D000001, S)
     ref = []
     for sl in slots:
         kc, ec = _host(sl["kc"]), _host(sl["ec"])
         opk, osk = oracle.keypair(SABER, kc, threads=THREADS)
         oct_, oss = oracle.enc(SABER, opk, ec, threads=THREADS)
         ref.append((opk, osk, oct_, oss))
-    r = bench.build_roll(eng, SABER, slots, ipw)
+    r = bench.build_roll(eng, SABER, slots, ipw, overlap=overlap)
     junk = torch.zeros((n, 96), dtype=torch.uint8, device="cuda")
     try:
-        for step in range(8):
+        for step in range(lag + 7):
             # a slot's outputs are cleared when its batch starts (its keygen
-            # step) and must all be complete two steps later
-            _clear(torch, slots[step % 3])
+            # step) and must all be complete `lag` steps later
+            _clear(torch, slots[step % S])
             r.step()
             torch.cuda.synchronize()
-            if step >= 2:
-                c = (step - 2) % 3
+            if step >= lag:
+                c = (step - lag) % S
                 sl, (opk, osk, oct_, oss) = slots[c], ref[c]
                 _eq(_host(sl["pk"]), opk, f"pk (step {step}, slot {c})")
                 _eq(_host(sl["sk"]), osk, f"sk (step {step}, slot {c})")

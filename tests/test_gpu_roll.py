@@ -38,11 +38,11 @@ def eng(torch_cuda):
     e.close()
 
 
-def _slots(torch, oracle, scheme, n, seed):
+def _slots(torch, oracle, scheme, n, seed, count=3):
     import dpcrypto_amd as D
     s = D.SIZES[scheme]
     slots, ref = [], []
-    for k in range(3):
+    for k in range(count):
         kc = oracle.prng_bytes(seed + 10 * k, n * s["kcoins"]).reshape(n, -1)
         ec = oracle.prng_bytes(seed + 10 * k + 1, n * s["ecoins"]).reshape(n, -1)
         opk, osk = oracle.keypair(scheme, kc, threads=8)
@@ -100,6 +100,34 @@ def _rejection_key(scheme, sk_row, ct_row):
     return h(ct_row + sk_row[:ln]).digest(ln)
 
 
+@pytest.mark.parametrize("scheme,n,ipw", [(SABER, 300, 1), (SABER, 37, 4), (SABER, 1, 2), (LIGHTSABER, 65, 2),
+                                           (FIRESABER, 33, 1), (FIRESABER, 97, 4)])
+def test_roll_overlapped_every_output_of_every_batch(eng, oracle, torch_cuda, scheme, n, ipw):
+    """The six-slot overlapped form (hash launches on one stream beside the
+    previous step's products on another): a slot's outputs are cleared
+    before the step of its keygen, and after step k >= 5 every output of
+    slot (k - 5) % 6 equals the oracle."""
+    import bench
+    torch = torch_cuda
+    print(f"roll overlapped: scheme {scheme}, n {n}, ipw {ipw}", flush=True)
+    slots, ref = _slots(torch, oracle, scheme, n, 6000 + n, count=6)
+    r = bench.build_roll(eng, scheme, slots, ipw, overlap=True)
+    try:
+        for step in range(13):
+            for t in ("pk", "sk", "ct", "ss", "ss2"):
+                slots[step % 6][t].zero_()
+            r.step()
+            torch.cuda.synchronize()
+            if step >= 5:
+                c = (step - 5) % 6
+                for t, v in zip(("pk", "sk", "ct", "ss", "ss2"), ref[c] + (ref[c][3],)):
+                    np.testing.assert_array_equal(slots[c][t].cpu().numpy(), v,
+                                                  err_msg=f"{t} of slot {c} after step {step}")
+        assert r.info()["steps"] == 13
+    finally:
+        r.close()
+
+
 @pytest.mark.parametrize("scheme", [SABER, LIGHTSABER, FIRESABER, FRODO976, FRODO640])
 def test_roll_decapsulates_tampered_ciphertexts_by_implicit_rejection(eng, oracle, torch_cuda, scheme):
     """Between the step that encapsulates slot 0 and the step that
@@ -142,4 +170,19 @@ def test_roll_rejects_misuse(eng, torch_cuda):
     r = eng.roll(D.SABER, 4)
     with pytest.raises(DpkemError):
         r.build()  # slots not bound
+    def slot(scheme, n):
+        s = D.SIZES[scheme]
+        return [torch_cuda.zeros((n, s[k]), dtype=torch_cuda.uint8, device="cuda")
+                for k in ("kcoins", "ecoins", "pk", "sk", "ct", "ss", "ss")]
+
+    for k in range(3):
+        r.bind(k, *slot(D.SABER, 4))
+    with pytest.raises(DpkemError):
+        r.build(overlap=True)  # the overlapped form needs six slots
     r.close()
+    f = eng.roll(D.FRODO976, 1)
+    for k in range(6):
+        f.bind(k, *slot(D.FRODO976, 1))
+    with pytest.raises(DpkemError):
+        f.build(overlap=True)  # Saber family only
+    f.close()
