@@ -186,101 +186,104 @@ VALU_MACS_PER_OP = {"sab_dec_inner": 2.0, "frodo_dec_mu": 1.0}
 #            --frodo-steps 2 --plain-steps 1
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
-# FETCH_SIZE x 1.99975, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# FETCH_SIZE x 1.99970, WRITE_SIZE x 1.00000.  Counters cannot be read live.
 MEASURED_HBM_BYTES = {
-    ('frodo_as_fused', 8192): (129270534, 256248206),
-    ('frodo_as_fused,1344', 8192): (177481982, 353402502),
-    ('frodo_as_fused,640', 8192): (85216777, 157548544),
-    ('frodo_as_fused<aes>', 8192): (129287566, 256251008),
-    ('frodo_dec_mu', 8192): (241917232, 262144),
-    ('frodo_dec_mu,1344', 8192): (358499544, 262144),
-    ('frodo_dec_mu,640', 8192): (165549308, 262144),
-    ('frodo_dec_pre', 8192): (130941427, 130374629),
-    ('frodo_encdec_pre<false>', 8192): (75039770, 128713088),
-    ('frodo_keygen_post', 8192): (129170493, 262144),
-    ('frodo_keygen_pre', 8192): (1435811, 256975241),
-    ('frodo_keygen_pre,1344', 8192): (1553365, 353241804),
-    ('frodo_keygen_pre,640', 8192): (1336099, 168558617),
-    ('frodo_pipe_ssd', 8192): (4272202, 196608),
-    ('frodo_pipe_ssd,1344', 8192): (4532675, 262144),
-    ('frodo_pipe_ssd,640', 8192): (5537092, 131072),
-    ('frodo_pipe_x1', 8192): (130437021, 128975121),
-    ('frodo_pipe_x1,1344', 8192): (178662351, 177340582),
-    ('frodo_pipe_x1,640', 8192): (81069051, 84934656),
-    ('frodo_pipe_x2', 8192): (131990503, 130650055),
-    ('frodo_pipe_x2,1344', 8192): (180329315, 178851187),
-    ('frodo_pipe_x2,640', 8192): (83658471, 86442528),
-    ('frodo_sa_fused<false,aes>', 8192): (259429809, 128974848),
-    ('frodo_sa_fused<false>', 8192): (260409533, 128974848),
-    ('frodo_sa_fused<false>,1344', 8192): (356745295, 177209344),
-    ('frodo_sa_fused<false>,640', 8192): (166970440, 79822848),
-    ('frodo_sa_fused<true,aes>', 8192): (388246684, 262144),
-    ('frodo_sa_fused<true>', 8192): (389372394, 262144),
-    ('frodo_sa_fused<true>,1344', 8192): (533929536, 265465),
-    ('frodo_sa_fused<true>,640', 8192): (247578238, 262188),
-    ('frodo_sa_grouped<false>', 8192): (129720952, 128974848),
-    ('frodo_sa_grouped<true>', 8192): (258621336, 65536),
-    ('frodo_ss', 8192): (129684330, 196608),
-    ('plain::copy_k', 8192): (55917377, 55462683),
-    ('plain::fr_as', 8192): (16571766952, 127926272),
-    ('plain::fr_bs_decode', 8192): (270778839, 393216),
-    ('plain::fr_gen_a', 8192): (1381244, 19995041898),
-    ('plain::fr_le_load', 8192): (128982613, 127926272),
-    ('plain::fr_le_store', 8192): (127932634, 128188608),
-    ('plain::fr_pack', 8192): (77189734, 77202099),
-    ('plain::fr_sa', 8192): (18560851224, 131442160),
-    ('plain::fr_sample', 8192): (256544009, 256551594),
-    ('plain::fr_sb', 8192): (265737300, 1048576),
-    ('plain::fr_select', 8192): (1487684, 393216),
-    ('plain::fr_unpack', 8192): (96708771, 96206848),
-    ('plain::sab_cbd', 8192): (3166831, 6291456),
-    ('plain::sab_cm', 8192): (2642610, 524288),
-    ('plain::sab_kdf', 8192): (724900, 131072),
-    ('plain::sab_msg', 8192): (2900625, 131072),
-    ('plain::sab_pack13', 8192): (6299875, 5111808),
-    ('plain::sab_polymul', 8192): (18953988, 4194304),
-    ('plain::sab_round_pack10', 8192): (6314423, 3932160),
-    ('plain::sab_unpack10', 8192): (4443939, 6291456),
-    ('plain::sab_unpack13', 8192): (13141379, 15728640),
-    ('plain::sha3_256_k', 8192): (4186606, 131072),
-    ('plain::sha3_512_64_k', 8192): (327638, 262144),
-    ('plain::shake128_seed32', 8192): (469956, 131072),
-    ('plain::shake256_dom_seed', 8192): (664875, 255852608),
-    ('plain::shake256_k', 8192): (612658, 262144),
-    ('plain::verify_k', 8192): (257941805, 262144),
-    ('sab_dec_hash1', 4096): (1177545, 3407872),
-    ('sab_dec_inner', 4096): (9888542, 131072),
-    ('sab_enc_hash1', 4096): (4919698, 3538944),
-    ('sab_hash2', 4096): (5235050, 131072),
-    ('sab_keygen_hash1', 4096): (1289320, 19006102),
-    ('sab_keygen_hash2', 4096): (4251110, 131072),
-    ('sab_mix_poly', 4096): (72279648, 17589173),
-    ('sab_mix_poly,l2', 4096): (39603408, 11801130),
-    ('sab_mix_poly,l4', 4096): (115969978, 28767319),
-    ('sab_mix_x0', 4096): (5775214, 19114686),
-    ('sab_mix_x1', 4096): (4949949, 131072),
-    ('sab_pipe_x0', 4096): (1284783, 49613132),
-    ('sab_pipe_x0,l2', 4096): (1296581, 23500143),
-    ('sab_pipe_x0,l4', 4096): (1304155, 85363404),
-    ('sab_pipe_x1', 4096): (36086029, 34340864),
-    ('sab_pipe_x1,l2', 4096): (17773470, 16777216),
-    ('sab_pipe_x1,l4', 4096): (62019377, 58195968),
-    ('sab_pipe_x2', 4096): (12584591, 3932160),
-    ('sab_pipe_x2,l2', 4096): (9465683, 3407872),
-    ('sab_pipe_x2,l4', 4096): (15915044, 3932160),
-    ('sab_poly<kSabEnc>', 4096): (23842741, 4456448),
-    ('sab_poly<kSabEnc>,l2', 4096): (13075851, 3014656),
-    ('sab_poly<kSabEnc>,l4', 4096): (36688389, 6029312),
-    ('sab_poly<kSabKeygen>', 4096): (18983585, 12979343),
-    ('sab_poly<kSabKeygen>,l2', 4096): (9527116, 8650838),
-    ('sab_poly<kSabKeygen>,l4', 4096): (30525678, 17328969),
-    ('sab_poly<kSabReenc>', 4096): (28980251, 131072),
-    ('sab_reenc', 4096): (29471234, 131072),
-    ('sab_reenc,l2', 4096): (17010586, 131072),
-    ('sab_reenc,l4', 4096): (48756830, 5111808),
-    ('sab_roll_x0', 4096): (18666587, 57352466),
-    ('sab_roll_x0,l2', 4096): (13093350, 30139294),
-    ('sab_roll_x0,l4', 4096): (26753228, 93253281),
+    ('frodo_as_fused', 8192): (129247208, 256245767),
+    ('frodo_as_fused,1344', 8192): (177458279, 353382438),
+    ('frodo_as_fused,640', 8192): (85214814, 157548544),
+    ('frodo_as_fused<aes>', 8192): (129303548, 256245941),
+    ('frodo_dec_mu', 8192): (256060732, 262144),
+    ('frodo_dec_mu,1344', 8192): (358477781, 262144),
+    ('frodo_dec_mu,640', 8192): (165536658, 262144),
+    ('frodo_dec_pre', 8192): (130915869, 130395792),
+    ('frodo_encdec_pre<false>', 8192): (75028256, 128713549),
+    ('frodo_keygen_post', 8192): (129167291, 262144),
+    ('frodo_keygen_pre', 8192): (1430538, 256964056),
+    ('frodo_keygen_pre,1344', 8192): (1547183, 353243891),
+    ('frodo_keygen_pre,640', 8192): (1336732, 168558668),
+    ('frodo_pipe_ssd', 8192): (4392128, 196608),
+    ('frodo_pipe_ssd,1344', 8192): (4518336, 262144),
+    ('frodo_pipe_ssd,640', 8192): (5536955, 131072),
+    ('frodo_pipe_x1', 8192): (130434257, 128975605),
+    ('frodo_pipe_x1,1344', 8192): (178671232, 177341228),
+    ('frodo_pipe_x1,640', 8192): (81071137, 84934656),
+    ('frodo_pipe_x2', 8192): (131991284, 130652792),
+    ('frodo_pipe_x2,1344', 8192): (180311381, 178919302),
+    ('frodo_pipe_x2,640', 8192): (83639043, 86442995),
+    ('frodo_roll_x0', 8192): (520002953, 523797063),
+    ('frodo_roll_x0,1344', 8192): (714049588, 714828370),
+    ('frodo_roll_x0,640', 8192): (324252262, 341495698),
+    ('frodo_sa_fused<false,aes>', 8192): (259476028, 128974848),
+    ('frodo_sa_fused<false>', 8192): (260449633, 128974848),
+    ('frodo_sa_fused<false>,1344', 8192): (356734065, 177209344),
+    ('frodo_sa_fused<false>,640', 8192): (166974056, 79822848),
+    ('frodo_sa_fused<true,aes>', 8192): (388306616, 262144),
+    ('frodo_sa_fused<true>', 8192): (389398035, 262144),
+    ('frodo_sa_fused<true>,1344', 8192): (533916936, 265416),
+    ('frodo_sa_fused<true>,640', 8192): (247572911, 262285),
+    ('frodo_sa_grouped<false>', 8192): (129748323, 128974848),
+    ('frodo_sa_grouped<true>', 8192): (258629834, 65536),
+    ('frodo_ss', 8192): (129686234, 196608),
+    ('plain::copy_k', 8192): (55916302, 55462532),
+    ('plain::fr_as', 8192): (16571141997, 127926272),
+    ('plain::fr_bs_decode', 8192): (270769823, 393216),
+    ('plain::fr_gen_a', 8192): (1385475, 19994088352),
+    ('plain::fr_le_load', 8192): (128979031, 127926272),
+    ('plain::fr_le_store', 8192): (127929462, 128189376),
+    ('plain::fr_pack', 8192): (77188230, 77202233),
+    ('plain::fr_sa', 8192): (18570791927, 131428000),
+    ('plain::fr_sample', 8192): (256537649, 256551594),
+    ('plain::fr_sb', 8192): (265728025, 1048576),
+    ('plain::fr_select', 8192): (1487647, 393216),
+    ('plain::fr_unpack', 8192): (96706278, 96206848),
+    ('plain::sab_cbd', 8192): (3166753, 6291456),
+    ('plain::sab_cm', 8192): (2642544, 524288),
+    ('plain::sab_kdf', 8192): (724882, 131072),
+    ('plain::sab_msg', 8192): (2900553, 131072),
+    ('plain::sab_pack13', 8192): (6299719, 5111808),
+    ('plain::sab_polymul', 8192): (18953423, 4194304),
+    ('plain::sab_round_pack10', 8192): (6314053, 3932160),
+    ('plain::sab_unpack10', 8192): (4443829, 6291456),
+    ('plain::sab_unpack13', 8192): (13141261, 15728640),
+    ('plain::sha3_256_k', 8192): (4186503, 131072),
+    ('plain::sha3_512_64_k', 8192): (327630, 262144),
+    ('plain::shake128_seed32', 8192): (469944, 131072),
+    ('plain::shake256_dom_seed', 8192): (664859, 255852896),
+    ('plain::shake256_k', 8192): (612643, 262144),
+    ('plain::verify_k', 8192): (257935283, 262144),
+    ('sab_dec_hash1', 4096): (1177522, 3407872),
+    ('sab_dec_inner', 4096): (9888297, 131072),
+    ('sab_enc_hash1', 4096): (4932886, 3538944),
+    ('sab_hash2', 4096): (5234920, 131072),
+    ('sab_keygen_hash1', 4096): (1289299, 19007307),
+    ('sab_keygen_hash2', 4096): (4251005, 131072),
+    ('sab_mix_poly', 4096): (72292904, 17590032),
+    ('sab_mix_poly,l2', 4096): (39602419, 11801380),
+    ('sab_mix_poly,l4', 4096): (115965620, 28766883),
+    ('sab_mix_x0', 4096): (5775069, 19113673),
+    ('sab_mix_x1', 4096): (4965184, 131072),
+    ('sab_pipe_x0', 4096): (1288494, 49623287),
+    ('sab_pipe_x0,l2', 4096): (1296772, 23486277),
+    ('sab_pipe_x0,l4', 4096): (1304083, 85387969),
+    ('sab_pipe_x1', 4096): (36087353, 34340864),
+    ('sab_pipe_x1,l2', 4096): (17773024, 16777216),
+    ('sab_pipe_x1,l4', 4096): (62046729, 58195968),
+    ('sab_pipe_x2', 4096): (12584081, 3932160),
+    ('sab_pipe_x2,l2', 4096): (9465449, 3407872),
+    ('sab_pipe_x2,l4', 4096): (15914906, 3932160),
+    ('sab_poly<kSabEnc>', 4096): (23843648, 4456448),
+    ('sab_poly<kSabEnc>,l2', 4096): (13075527, 3014656),
+    ('sab_poly<kSabEnc>,l4', 4096): (36688325, 6029312),
+    ('sab_poly<kSabKeygen>', 4096): (18983120, 12979696),
+    ('sab_poly<kSabKeygen>,l2', 4096): (9526879, 8650863),
+    ('sab_poly<kSabKeygen>,l4', 4096): (30524921, 17328898),
+    ('sab_poly<kSabReenc>', 4096): (28974514, 131072),
+    ('sab_reenc', 4096): (29471737, 131072),
+    ('sab_reenc,l2', 4096): (17010164, 131072),
+    ('sab_reenc,l4', 4096): (48755660, 5111808),
+    ('sab_roll_x0', 4096): (18634235, 57374980),
+    ('sab_roll_x0,l2', 4096): (13111599, 30149308),
+    ('sab_roll_x0,l4', 4096): (26774250, 93216428),
 }
 
 
@@ -291,7 +294,7 @@ KERNEL_SOURCES = {
     "saber": ("saber.hip", "saber_poly.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
     "frodo": ("frodo.hip", "frodo_sampler.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
 }
-MEASURED_HBM_SOURCE_SHA = {"saber": "4e19566d8b5a3b55", "frodo": "873a4511a2ed22f5"}
+MEASURED_HBM_SOURCE_SHA = {"saber": "34424a36e100d27a", "frodo": "7a4099279b76c56d"}
 
 
 def kernel_family(kernel: str) -> str:

@@ -1382,6 +1382,10 @@ __global__ void __launch_bounds__(256) sab_mix_x0(SabMixArgs m) {
 template <int L>
 __global__ void __launch_bounds__(256) sab_roll_x0(SabMixArgs m) {
   __shared__ MixDecLds<L> S;
+  // In the overlapped pipeline this launch shares its SIMDs with the
+  // previous step's products: its waves (chains, and the helpers the chains
+  // wait on) take issue priority, and the product waves fill the gaps.
+  __builtin_amdgcn_s_setprio(3);
   x0_body<L, true>(m, S);
 }
 
