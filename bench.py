@@ -481,8 +481,8 @@ def build_roll(eng, scheme, slots, ipw=1, graph=True, threads=0, overlap=False):
     (generated by step k - 1) and decapsulates the ciphertexts of slot
     (k - 2) % 3 (encapsulated by step k - 1), and finishes the previous
     steps' batches (keygen's H(pk), encaps' K) in the same launches.
-    overlap: the six-slot form (slots k, k - 2, k - 4 mod 6; hash launches
-    beside the previous step's products).  ipw: the Saber product's
+    overlap: the six-slot form (slots k, k - 2, k - 4 mod 6; one launch per
+    step with its chains and the previous step's products).  ipw: the Saber product's
     instances per workgroup; threads: the Frodo fused kernels' workgroup
     size (0 = the set's default)."""
     import dpcrypto_amd as D
@@ -501,7 +501,7 @@ def roll_shapes(scheme):
     two passes over A's rows, or 1024 in one)."""
     import dpcrypto_amd as D
     if scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER):
-        return [(o, c) for o in (False, True) for c in (1, 2, 4)]
+        return [(False, c) for c in (1, 2, 4)] + [(True, 4)]  # the overlapped form's products: 4 per workgroup
     return [0, 1024] if scheme in (D.FRODO976, D.FRODO976_AES) else [0]
 
 
@@ -515,7 +515,7 @@ def build_roll_shape(eng, scheme, slots, shape, graph=True):
 
 def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
     """Time `steps` steps of the rolling pipeline (graph replays, or direct
-    launches on two streams in the overlapped form).  The launch shape
+    launches, one per step in the overlapped form).  The launch shape
     (roll_shapes; `ipw` fixes the Saber product's and keeps the three-slot
     form) is picked first by timing two cycles of steps.  Afterwards the
     batches still in flight are finished and every slot must hold ss == ss2
@@ -575,8 +575,8 @@ def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
     batch = [sum(per_step[i:i + span]) for i in range(max(1, len(per_step) - span + 1))]
     if saber:
         overlap, ipw_best = best
-        sched = {"pipeline": ("dpkem_roll overlapped: six slots, hash launches on one stream beside the previous "
-                              "step's products on another" if overlap else "dpkem_roll: two launches per step"),
+        sched = {"pipeline": ("dpkem_roll overlapped: six slots, one launch per step holding its chains and the "
+                              "previous step's products" if overlap else "dpkem_roll: two launches per step"),
                  "saber_poly_ipw": ipw_best, "overlap": overlap,
                  "tuning_ms": {f"{'overlap' if o else 'plain'}-ipw{c}": round(v, 4) for (o, c), v in tried.items()}}
     else:

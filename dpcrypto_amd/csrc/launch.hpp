@@ -74,11 +74,12 @@ struct RollDesc {
   uint8_t *ss_d, *ss_e_prev;   // decaps' ss, and encaps' ss of the same slot
   uint8_t *scratch_k, *scratch_e, *scratch_d;
 };
-// Saber family (saber.hip: sab_roll_x0, then the mixed products); part:
-// both launches, or only the hash launch / only the products (the
-// overlapped pipeline issues them on two streams).
-enum { kRollBoth = 0, kRollChains = 1, kRollProducts = 2 };
-void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw = 1, int part = kRollBoth);
+// Saber family (saber.hip: sab_roll_x0, then the mixed products).
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw = 1);
+// Overlapped rolling step in one launch (saber.hip: sab_roll_fused): the
+// chains of step `hash` and the products of step `prod` (either may be
+// null), the products four instances per workgroup.
+void saber_roll_fused_launch(hipStream_t st, int l, const RollDesc* hash, const RollDesc* prod);
 
 // set: 976 (FrodoKEM-976), 640 or 1344 (Frodo-640 / Frodo-1344, SHAKE);
 // aes: FrodoKEM-976-AES (A rows from AES-128) instead of -SHAKE.  Scratch

@@ -7,14 +7,13 @@
 //
 // Overlapped form (Saber family, DPKEM_ROLL_OVERLAP, six slots): step k
 // generates the keys of slot k, encapsulates to slot k - 2 and decapsulates
-// slot k - 4 (mod 6).  Step k + 1's hash launch then reads nothing that
-// step k's products write (keygen's pk of slot k - 1 came from the products
-// of step k - 1, the ciphertexts of slot k - 3 from those of step k - 1 too),
-// so the hash launches run on one stream and the products on another: the
-// chain waves of step k + 1 (one per SIMD, latency-bound) share the chip with
-// step k's products.  Per step, with events: hashes(k) wait for the engine
-// stream's work so far (which has joined products(k - 2)); products(k) wait
-// for hashes(k); the engine stream then joins products(k - 1).
+// slot k - 4 (mod 6).  Step k's chains then read nothing that step k - 1's
+// products write and write nothing they read (keygen's pk of slot k - 2 and
+// the ciphertexts of slot k - 4 came from the products of step k - 2; step
+// k - 1's products touch slots k - 1, k - 3 and k - 5), so one launch per
+// step (sab_roll_fused) carries step k's chains and step k - 1's products:
+// the chain waves (one per SIMD, latency-bound) share the chip with the
+// product waves, on the engine's stream, with no second stream or event.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 
@@ -37,9 +36,6 @@ struct dpkem_roll {
   } slot[6];
   bool bound[6] = {false, false, false, false, false, false};
   int nslots = 3;  // 6 in the overlapped form
-  // overlapped form: the hash and product streams, and their events
-  hipStream_t hs = nullptr, ps = nullptr;
-  hipEvent_t ev_user = nullptr, ev_hash = nullptr, ev_prod[2] = {nullptr, nullptr};
   uint8_t* scratch = nullptr;
   size_t per_slot = 0;
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
@@ -63,16 +59,6 @@ int set_of(int scheme) {
 
 void release(dpkem_roll* r) {
   r->built = false;
-  if (r->hs) (void)hipStreamSynchronize(r->hs);
-  if (r->ps) (void)hipStreamSynchronize(r->ps);
-  for (hipEvent_t* ev : {&r->ev_user, &r->ev_hash, &r->ev_prod[0], &r->ev_prod[1]}) {
-    if (*ev) (void)hipEventDestroy(*ev);
-    *ev = nullptr;
-  }
-  for (hipStream_t* st : {&r->hs, &r->ps}) {
-    if (*st) (void)hipStreamDestroy(*st);
-    *st = nullptr;
-  }
   for (int k = 0; k < 3; ++k) {
     if (r->exec[k]) (void)hipGraphExecDestroy(r->exec[k]);
     if (r->graph[k]) (void)hipGraphDestroy(r->graph[k]);
@@ -83,11 +69,10 @@ void release(dpkem_roll* r) {
   r->scratch = nullptr;
 }
 
-// Step `phase`: keygen of slot phase, encaps of slot phase - 1 and decaps of
-// slot phase - 2 (mod 3), or in the overlapped form slots phase, phase - 2,
-// phase - 4 (mod 6); part: which launches (kRollBoth / kRollChains /
-// kRollProducts).
-void issue(dpkem_roll* r, hipStream_t st, int phase, int part = kRollBoth) {
+// Arrays of step `phase`: keygen of slot phase, encaps of slot phase - 1 and
+// decaps of slot phase - 2 (mod 3), or in the overlapped form slots phase,
+// phase - 2, phase - 4 (mod 6).
+RollDesc step_desc(const dpkem_roll* r, int phase) {
   const int S = r->nslots, lag = S == 6 ? 2 : 1;
   const int ks = phase, es = (phase + S - lag) % S, ds = (phase + S - 2 * lag) % S;
   const auto &K = r->slot[ks], &E = r->slot[es], &Dd = r->slot[ds];
@@ -107,8 +92,13 @@ void issue(dpkem_roll* r, hipStream_t st, int phase, int part = kRollBoth) {
   d.scratch_k = r->scratch + (size_t)ks * r->per_slot;
   d.scratch_e = r->scratch + (size_t)es * r->per_slot;
   d.scratch_d = r->scratch + (size_t)ds * r->per_slot;
+  return d;
+}
+
+void issue(dpkem_roll* r, hipStream_t st, int phase) {
+  const RollDesc d = step_desc(r, phase);
   if (r->set) frodo_roll_launch(st, r->set, r->aes, d, r->threads);
-  else saber_roll_launch(st, r->l, d, r->ipw, part);
+  else saber_roll_launch(st, r->l, d, r->ipw);
 }
 
 }  // namespace
@@ -174,11 +164,7 @@ int dpkem_roll_build(dpkem_roll* r, unsigned flags) {
   } while (0)
   ROLL_TRY(hipSetDevice(e->device));
   ROLL_TRY(hipMalloc(&r->scratch, r->nslots * r->per_slot));
-  if (overlap) {  // launches issued directly on two streams of its own
-    ROLL_TRY(hipStreamCreateWithFlags(&r->hs, hipStreamNonBlocking));
-    ROLL_TRY(hipStreamCreateWithFlags(&r->ps, hipStreamNonBlocking));
-    for (hipEvent_t* ev : {&r->ev_user, &r->ev_hash, &r->ev_prod[0], &r->ev_prod[1]})
-      ROLL_TRY(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  if (overlap) {  // one launch per step, issued directly on the engine's stream
   } else if (!(flags & DPKEM_PLAN_NO_GRAPH)) {
     for (int ph = 0; ph < 3; ++ph) {
       hipStream_t cap;
@@ -218,20 +204,12 @@ int dpkem_roll_step(dpkem_roll* r) {
   hipError_t err = hipSetDevice(e->device);
   if (err != hipSuccess) return record_hip_error(e, err, "hipSetDevice");
   const int phase = (int)(r->steps % (uint64_t)r->nslots);
-  if (r->nslots == 6) {
-    const uint64_t k = r->steps;
+  if (r->nslots == 6) {  // this step's chains beside the previous step's products
+    const RollDesc cur = step_desc(r, phase), prev = step_desc(r, (phase + 5) % 6);
     tls_profiler = e->profiling ? &e->prof : nullptr;
-    bool ok = hipEventRecord(r->ev_user, e->stream) == hipSuccess &&
-              hipStreamWaitEvent(r->hs, r->ev_user, 0) == hipSuccess;
-    if (ok) issue(r, r->hs, phase, kRollChains);
-    ok = ok && hipEventRecord(r->ev_hash, r->hs) == hipSuccess &&
-         hipStreamWaitEvent(r->ps, r->ev_hash, 0) == hipSuccess;
-    if (ok) issue(r, r->ps, phase, kRollProducts);
-    ok = ok && hipEventRecord(r->ev_prod[k & 1], r->ps) == hipSuccess;
-    if (ok && k > 0) ok = hipStreamWaitEvent(e->stream, r->ev_prod[(k - 1) & 1], 0) == hipSuccess;
+    saber_roll_fused_launch(e->stream, r->l, &cur, r->steps > 0 ? &prev : nullptr);
     tls_profiler = nullptr;
     err = hipGetLastError();
-    if (err == hipSuccess && !ok) err = hipErrorUnknown;
   } else if (r->flags & DPKEM_PLAN_NO_GRAPH) {
     tls_profiler = e->profiling ? &e->prof : nullptr;
     issue(r, e->stream, phase);

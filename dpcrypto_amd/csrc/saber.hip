@@ -1391,13 +1391,11 @@ __global__ void __launch_bounds__(256) sab_roll_x0(SabMixArgs m) {
 
 // Products of the three ops: workgroups [0, bk) keygen, [bk, bk + be)
 // encaps, the rest the re-encryption; IPW one-wave instances per workgroup.
+// Workgroup b of the products grid (IPW one-wave instances; w this wave's
+// instance slot, tid its lane).
 template <int L, int IPW>
-__global__ void __launch_bounds__(64 * IPW) __attribute__((amdgpu_waves_per_eu(4)))
-sab_mix_poly(SabPolyArgs ak, SabPolyArgs ae, SabPolyArgs ad, int bk, int be) {
-  __shared__ __align__(16) SabPolyLds<L> S[IPW];
-  const int w = IPW == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int tid = IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63);
-  int b = blockIdx.x;
+__device__ __forceinline__ void mix_poly_wg(const SabPolyArgs& ak, const SabPolyArgs& ae, const SabPolyArgs& ad,
+                                            int bk, int be, SabPolyLds<L>* S, int b, int w, int tid) {
   if (b < bk) {
     const int inst = b * IPW + w;
     if (inst < ak.n) sab_poly_body<kSabKeygen, L>(ak, S[w], tid, inst);
@@ -1412,6 +1410,37 @@ sab_mix_poly(SabPolyArgs ak, SabPolyArgs ae, SabPolyArgs ad, int bk, int be) {
   b -= be;
   const int inst = b * IPW + w;
   if (inst < ad.n) sab_poly_body<kSabReenc, L>(ad, S[w], tid, inst);
+}
+
+template <int L, int IPW>
+__global__ void __launch_bounds__(64 * IPW) __attribute__((amdgpu_waves_per_eu(4)))
+sab_mix_poly(SabPolyArgs ak, SabPolyArgs ae, SabPolyArgs ad, int bk, int be) {
+  __shared__ __align__(16) SabPolyLds<L> S[IPW];
+  const int w = IPW == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tid = IPW == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+  mix_poly_wg<L, IPW>(ak, ae, ad, bk, be, S, blockIdx.x, w, tid);
+}
+
+// One overlapped rolling step in one launch: workgroups [0, hash_wgs) are
+// sab_roll_x0's (this step's chains and closing hashes, at issue priority),
+// the rest the previous step's products, four one-wave instances each.  The
+// dispatcher places the chain workgroups first, one per CU, and the product
+// workgroups fill the remaining wave slots, so the products of step k - 1
+// run beside the chains of step k without a second stream or an event.
+template <int L>
+__global__ void __launch_bounds__(256) sab_roll_fused(SabMixArgs m, int hash_wgs, SabPolyArgs ak, SabPolyArgs ae,
+                                                      SabPolyArgs ad, int bk, int be) {
+  __shared__ __align__(16) union FusedLds {
+    MixDecLds<L> h;
+    SabPolyLds<L> p[4];
+  } U;
+  if ((int)blockIdx.x < hash_wgs) {
+    __builtin_amdgcn_s_setprio(3);
+    x0_body<L, true>(m, U.h);
+    return;
+  }
+  mix_poly_wg<L, 4>(ak, ae, ad, bk, be, U.p, (int)blockIdx.x - hash_wgs,
+                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), (int)(threadIdx.x & 63));
 }
 
 // Closing hashes of 32 keygen and 32 encaps instances per workgroup: wave 0
@@ -1664,12 +1693,18 @@ static void mix_poly_any(const char* name, hipStream_t st, const SabPolyArgs& ak
 
 // The products of a mixed sequence over the arrays of m.
 template <int L>
-static void mix_products(hipStream_t st, const SabMixArgs& m, int ipw) {
+static void mix_poly_args(const SabMixArgs& m, SabPolyArgs& ak, SabPolyArgs& ae, SabPolyArgs& ad) {
   using P = SaberP<L>;
-  SabPolyArgs ak{m.nk, m.araw_k, m.sraw_k, nullptr, 0, nullptr, m.pk_k, m.sk_k, nullptr, nullptr};
-  SabPolyArgs ae{m.ne, m.araw_e, m.sraw_e, m.pk_e, (size_t)P::pk, m.mbuf_e, m.ct_e, nullptr, nullptr, nullptr};
-  SabPolyArgs ad{m.nd,    m.araw_d, m.sraw_d, m.sk_d + P::vecq, (size_t)P::sk, m.mbuf_d, nullptr, nullptr, m.ct_d,
-                 nullptr, m.kk_d,   m.ss_d,   0};
+  ak = SabPolyArgs{m.nk, m.araw_k, m.sraw_k, nullptr, 0, nullptr, m.pk_k, m.sk_k, nullptr, nullptr};
+  ae = SabPolyArgs{m.ne, m.araw_e, m.sraw_e, m.pk_e, (size_t)P::pk, m.mbuf_e, m.ct_e, nullptr, nullptr, nullptr};
+  ad = SabPolyArgs{m.nd,    m.araw_d, m.sraw_d, m.sk_d + P::vecq, (size_t)P::sk, m.mbuf_d, nullptr, nullptr, m.ct_d,
+                   nullptr, m.kk_d,   m.ss_d,   0};
+}
+
+template <int L>
+static void mix_products(hipStream_t st, const SabMixArgs& m, int ipw) {
+  SabPolyArgs ak, ae, ad;
+  mix_poly_args<L>(m, ak, ae, ad);
   mix_poly_any<L>(SAB_NAME("sab_mix_poly"), st, ak, ae, ad, ipw);
 }
 
@@ -1714,7 +1749,7 @@ static void mix_l(hipStream_t st, const SaberMixDesc& d, uint8_t* scratch, int i
 // One step of the rolling pipeline (see saber_roll_launch in launch.hpp):
 // sab_roll_x0, then the mixed products.
 template <int L>
-static void roll_l(hipStream_t st, const RollDesc& d, int ipw, int part) {
+static SabMixArgs roll_args(const RollDesc& d) {
   const int n = d.n;
   const MixScratch xk = mix_carve<L>(d.scratch_k, n, n, n);
   const MixScratch xe = mix_carve<L>(d.scratch_e, n, n, n);
@@ -1747,15 +1782,48 @@ static void roll_l(hipStream_t st, const RollDesc& d, int ipw, int part) {
   m.ct_t = d.ct_d;         // encaps of two steps earlier: the ciphertexts decaps uses now
   m.krbuf_t = xd.krbuf_e;  // its kr, in that slot's encaps scratch
   m.ss_t = d.ss_e_prev;
-  if (part != kRollProducts) DPKEM_LAUNCH(SAB_NAME("sab_roll_x0"), sab_roll_x0<L>, dim3(2 * m.gke), dim3(256), st, m);
-  if (part != kRollChains) mix_products<L>(st, m, ipw);
+  return m;
 }
 
-void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw, int part) {
+template <int L>
+static void roll_l(hipStream_t st, const RollDesc& d, int ipw) {
+  const SabMixArgs m = roll_args<L>(d);
+  DPKEM_LAUNCH(SAB_NAME("sab_roll_x0"), sab_roll_x0<L>, dim3(2 * m.gke), dim3(256), st, m);
+  mix_products<L>(st, m, ipw);
+}
+
+// hash: this step's chains (or none); prod: the step whose products run
+// beside them (or none).
+template <int L>
+static void roll_fused_l(hipStream_t st, const RollDesc* hash, const RollDesc* prod) {
+  const SabMixArgs m = hash ? roll_args<L>(*hash) : SabMixArgs{};
+  const int hash_wgs = hash ? 2 * m.gke : 0;
+  SabPolyArgs ak{}, ae{}, ad{};
+  int bk = 0, be = 0, bd = 0;
+  if (prod) {
+    mix_poly_args<L>(roll_args<L>(*prod), ak, ae, ad);
+    bk = blocks_for(ak.n, 4);
+    be = blocks_for(ae.n, 4);
+    bd = blocks_for(ad.n, 4);
+  }
+  if (hash_wgs + bk + be + bd == 0) return;
+  DPKEM_LAUNCH(SAB_NAME("sab_roll_fused"), sab_roll_fused<L>, dim3(hash_wgs + bk + be + bd), dim3(256), st, m,
+               hash_wgs, ak, ae, ad, bk, be);
+}
+
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int ipw) {
   if (d.n <= 0) return;
-  if (l == 2) roll_l<2>(st, d, ipw, part);
-  else if (l == 4) roll_l<4>(st, d, ipw, part);
-  else roll_l<3>(st, d, ipw, part);
+  if (l == 2) roll_l<2>(st, d, ipw);
+  else if (l == 4) roll_l<4>(st, d, ipw);
+  else roll_l<3>(st, d, ipw);
+}
+
+void saber_roll_fused_launch(hipStream_t st, int l, const RollDesc* hash, const RollDesc* prod) {
+  if (hash && hash->n <= 0) hash = nullptr;
+  if (prod && prod->n <= 0) prod = nullptr;
+  if (l == 2) roll_fused_l<2>(st, hash, prod);
+  else if (l == 4) roll_fused_l<4>(st, hash, prod);
+  else roll_fused_l<3>(st, hash, prod);
 }
 
 void saber_mix_launch(hipStream_t st, int l, const SaberMixDesc& d, uint8_t* scratch, int ipw) {

@@ -381,7 +381,7 @@ PLAN_NO_PIPELINE = 2
 PLAN_NO_GRAPH = 4
 PLAN_NO_MERGE = 8
 PLAN_FORCE_REJECT = 16  # test facility: pipelined chains' decapsulations take the rejection branch
-ROLL_OVERLAP = 256  # rolling pipeline over six slots, hash launches beside the previous step's products
+ROLL_OVERLAP = 256  # rolling pipeline over six slots, each step's chains beside the previous step's products in one launch
 
 
 class Plan:
@@ -574,9 +574,9 @@ class Roll:
 
     def build(self, graph: bool = True, overlap: bool = False):
         """overlap: the six-slot form (slots 0-5 bound; step k: keygen of
-        slot k, encaps of slot k - 2, decaps of slot k - 4, mod 6), its hash
-        launches on one stream beside the previous step's products on
-        another; after step k every output of slot (k - 5) % 6 is complete."""
+        slot k, encaps of slot k - 2, decaps of slot k - 4, mod 6), one
+        launch per step holding its chains and the previous step's products;
+        after step k every output of slot (k - 5) % 6 is complete."""
         flags = (0 if graph else PLAN_NO_GRAPH) | (ROLL_OVERLAP if overlap else 0)
         self.eng._check(self.lib.dpkem_roll_build(self.h, flags), "roll_build")
         return self

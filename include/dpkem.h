@@ -246,15 +246,13 @@ int dpkem_roll_bind(dpkem_roll *r, int slot, const uint8_t *kcoins, const uint8_
                     uint8_t *sk, uint8_t *ct, uint8_t *ss, uint8_t *ss2);
 /* Overlapped form (DPKEM_ROLL_OVERLAP; Saber family; slots 0-5 bound): step
  * k generates the keys of slot k mod 6, encapsulates to slot (k - 2) mod 6
- * and decapsulates slot (k - 4) mod 6, so that step k + 1's hash launch
- * depends on nothing step k's products write; the hash launches run on one
- * stream of the pipeline and the products on another, step k + 1's chains
- * beside step k's products.  Each step orders its hash launch after the work
- * already on the engine's stream and makes the engine's stream wait for the
- * previous step's products: after step k (k >= 5) every output of slot
- * (k - 5) mod 6 is complete on the engine's stream, until step k + 1 starts
- * that slot's next keygen.  Launches are issued directly (no graph);
- * dpkem_roll_destroy waits for both streams. */
+ * and decapsulates slot (k - 4) mod 6, so that step k's hash chains depend on
+ * nothing step k - 1's products write; each step is one launch on the
+ * engine's stream holding step k's chains and step k - 1's products (four
+ * instances per workgroup; DPKEM_TUNE_SABER_POLY_IPW does not apply).  After
+ * step k (k >= 5) every output of slot (k - 5) mod 6 is complete on the
+ * engine's stream, until step k + 1 starts that slot's next keygen.
+ * Launches are issued directly (no graph). */
 #define DPKEM_ROLL_OVERLAP 256u
 /* flags: 0, DPKEM_PLAN_NO_GRAPH (issue each step's launches directly, so
  * that the engine's profiler can time them), or DPKEM_ROLL_OVERLAP.  Takes the engine's current

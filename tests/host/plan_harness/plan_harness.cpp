@@ -161,8 +161,26 @@ void roll_access(hipStream_t st, const Sz& z, const RollDesc& d, size_t per) {
   scr(d.scratch_e, per);
   scr(d.scratch_d, per);
 }
-void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int, int) {
+void saber_roll_launch(hipStream_t st, int l, const RollDesc& d, int) {
   roll_access(st, kSz[scheme_of_l(l)], d, saber_mix_scratch_bytes(l, (size_t)d.n, (size_t)d.n, (size_t)d.n));
+}
+// One launch with a step's chains and another step's products: both steps'
+// arrays and scratch, which must not overlap (they run at the same time).
+void saber_roll_fused_launch(hipStream_t st, int l, const RollDesc* hash, const RollDesc* prod) {
+  if (!hash && !prod) fakehip::fail("fused rolling launch with nothing to do");
+  const size_t per = saber_mix_scratch_bytes(l, (size_t)(hash ? hash->n : prod->n), (size_t)(hash ? hash->n : prod->n),
+                                             (size_t)(hash ? hash->n : prod->n));
+  std::vector<std::pair<uintptr_t, uintptr_t>> mine;
+  if (hash) {
+    roll_access(st, kSz[scheme_of_l(l)], *hash, per);
+    mine = g_regions;
+  }
+  if (prod) {
+    roll_access(st, kSz[scheme_of_l(l)], *prod, per);
+    for (auto& a : mine)
+      for (auto& b : g_regions)
+        if (a.first < b.second && b.first < a.second) fakehip::fail("a fused step's chains and products share scratch");
+  }
 }
 // ---- Frodo
 void frodo_keygen_launch(hipStream_t st, int n, const uint8_t* coins, uint8_t* pk, uint8_t* sk, uint32_t* s8,

@@ -9,8 +9,8 @@ GPU's bytes with the CPU oracle on the same coins:
   C2  Saber l=3, batch 4096: the headline's rolling pipeline (three batch
       slots; each step keygen of one slot, encaps of the previous, decaps of
       the one before; product 1, 2 and 4 instances per workgroup; and the
-      overlapped six-slot form, hash launches beside the previous step's
-      products on a second stream): a slot's outputs cleared when its batch
+      overlapped six-slot form, each step's chains beside the previous
+      step's products in one launch): a slot's outputs cleared when its batch
       starts, unrelated work between steps, and every output of every
       instance of every batch equal to the oracle once complete (two steps
       later, five in the overlapped form); and the chained plan (one batch's keygen -> encaps ->

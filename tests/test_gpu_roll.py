@@ -103,8 +103,8 @@ def _rejection_key(scheme, sk_row, ct_row):
 @pytest.mark.parametrize("scheme,n,ipw", [(SABER, 300, 1), (SABER, 37, 4), (SABER, 1, 2), (LIGHTSABER, 65, 2),
                                            (FIRESABER, 33, 1), (FIRESABER, 97, 4)])
 def test_roll_overlapped_every_output_of_every_batch(eng, oracle, torch_cuda, scheme, n, ipw):
-    """The six-slot overlapped form (hash launches on one stream beside the
-    previous step's products on another): a slot's outputs are cleared
+    """The six-slot overlapped form (each step's chains beside the previous
+    step's products in one launch): a slot's outputs are cleared
     before the step of its keygen, and after step k >= 5 every output of
     slot (k - 5) % 6 equals the oracle."""
     import bench
