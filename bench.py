@@ -17,8 +17,8 @@ chains of the three ops plus the closing hashes of the previous steps'
 batches (keygen's H(pk), encaps' K) in one, all products in the other.
 The autotuner also tries the overlapped form over six slots (encaps to the
 keys of two steps earlier, decaps of the ciphertexts of two steps earlier),
-in which step r + 1's hash launch runs on one stream beside step r's
-products on another.  Every op recomputes everything from its own inputs
+in which each step is one launch holding its hash chains and the previous
+step's products.  Every op recomputes everything from its own inputs
 (encaps expands the public matrix from pk, decaps from sk), as separate
 parties would.
 
@@ -116,6 +116,14 @@ KERNEL_WORK = {
     "sab_roll_x0": ((1 + 23 + 5) + (23 + 1 + 8 + 1 + 5) + (23 + 9 + 2 + 1 + 5) + (8 + 9 + 1), 3 * 65536),
     "sab_roll_x0,l2": ((1 + 10 + 4) + (10 + 1 + 5 + 1 + 4) + (10 + 6 + 2 + 1 + 4) + (5 + 6 + 1), 2 * 65536),
     "sab_roll_x0,l4": ((1 + 40 + 5) + (40 + 1 + 10 + 1 + 5) + (40 + 11 + 2 + 1 + 5) + (10 + 11 + 1), 4 * 65536),
+    # overlapped rolling step in one launch: this step's chains (as
+    # sab_roll_x0) and the previous step's products (as sab_mix_poly)
+    "sab_roll_fused": ((1 + 23 + 5) + (23 + 1 + 8 + 1 + 5) + (23 + 9 + 2 + 1 + 5) + (8 + 9 + 1),
+                       (3 + 9 + 12 + 12) * 65536),
+    "sab_roll_fused,l2": ((1 + 10 + 4) + (10 + 1 + 5 + 1 + 4) + (10 + 6 + 2 + 1 + 4) + (5 + 6 + 1),
+                          (2 + 4 + 6 + 6) * 65536),
+    "sab_roll_fused,l4": ((1 + 40 + 5) + (40 + 1 + 10 + 1 + 5) + (40 + 11 + 2 + 1 + 5) + (10 + 11 + 1),
+                          (4 + 16 + 20 + 20) * 65536),
     "sab_pipe_x0,l2": (1 + 10 + 4 + 10 + 10, 0),
     "sab_pipe_x1,l2": (5 + 1 + 1 + 4 + 5, 0),
     "sab_pipe_x2,l2": (7 + 6 + 1 + 4 + 2, 2 * 65536),
@@ -187,6 +195,10 @@ VALU_MACS_PER_OP = {"sab_dec_inner": 2.0, "frodo_dec_mu": 1.0}
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
 # counter reads for an elementwise kernel over 256 MiB in / 256 MiB out:
 # FETCH_SIZE x 1.99970, WRITE_SIZE x 1.00000.  Counters cannot be read live.
+# The sab_roll_fused rows come from the same two passes over the Saber-family
+# part of that command (--skip-frodo; scale 1.99975 / 1.00000), in which the
+# unchanged Saber kernels (sab_roll_x0, sab_pipe_x0, sab_poly, ...) read
+# within 4% of their rows here.
 MEASURED_HBM_BYTES = {
     ('frodo_as_fused', 8192): (129247208, 256245767),
     ('frodo_as_fused,1344', 8192): (177458279, 353382438),
@@ -284,6 +296,9 @@ MEASURED_HBM_BYTES = {
     ('sab_roll_x0', 4096): (18634235, 57374980),
     ('sab_roll_x0,l2', 4096): (13111599, 30149308),
     ('sab_roll_x0,l4', 4096): (26774250, 93216428),
+    ('sab_roll_fused', 4096): (92342679, 76169056),
+    ('sab_roll_fused,l2', 4096): (53018414, 42397696),
+    ('sab_roll_fused,l4', 4096): (140606620, 118882368),
 }
 
 
@@ -294,7 +309,7 @@ KERNEL_SOURCES = {
     "saber": ("saber.hip", "saber_poly.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
     "frodo": ("frodo.hip", "frodo_sampler.hpp", "keccak2.hpp", "keccak.hpp", "sponge.hpp", "common.hpp"),
 }
-MEASURED_HBM_SOURCE_SHA = {"saber": "34424a36e100d27a", "frodo": "7a4099279b76c56d"}
+MEASURED_HBM_SOURCE_SHA = {"saber": "a3bbf5e3e97a1972", "frodo": "7a4099279b76c56d"}
 
 
 def kernel_family(kernel: str) -> str:
@@ -540,8 +555,8 @@ def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
             ts.append(x.elapsed_time(y) / per)
         return statistics.median(ts)
 
-    def lag(shape):  # steps from a batch's keygen to its last output
-        return 4 if isinstance(shape, tuple) and shape[0] else 2
+    def lag(shape):  # steps from a batch's keygen step to the step that writes its last output
+        return 5 if isinstance(shape, tuple) and shape[0] else 2
 
     tried = {}
     for c in shapes:
@@ -587,19 +602,23 @@ def run_rolling_workload(torch, eng, scheme, n, steps, warmup, seed, ipw=None):
                 ops_per_s=3 * n * steps / wall, schedule=sched)
 
 
-def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=None):
+def profile_rolling(torch, eng, scheme, n, steps, seed, shape=None):
     """Per-kernel HIP-event times of the rolling steps' kernels, launched
-    without a graph in the three-slot form (same kernels; each runs alone);
-    ipw: the launch shape (roll_shapes), by default the first candidate."""
-    if ipw is None:
-        ipw = roll_shapes(scheme)[0]
+    without a graph.  shape: a roll_shapes candidate (default the first);
+    for the Saber family an int is the three-slot form's product instances
+    per workgroup.  The overlapped shape's step is one launch
+    (sab_roll_fused); the three-slot form's launches each run alone."""
     import dpcrypto_amd as D
-    if scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER):
-        ipw = (False, ipw[1] if isinstance(ipw, tuple) else ipw)
-    slots = rolling_slots(torch, scheme, n, seed)
+    saber = scheme in (D.SABER, D.LIGHTSABER, D.FIRESABER)
+    if shape is None:
+        shape = roll_shapes(scheme)[0]
+    if saber and not isinstance(shape, tuple):
+        shape = (False, shape)
+    overlap = saber and bool(shape[0])
+    slots = rolling_slots(torch, scheme, n, seed, 6 if overlap else 3)
     eng.set_stream(torch.cuda.current_stream(torch.device("cuda:0")).cuda_stream)
-    r = build_roll_shape(eng, scheme, slots, ipw, graph=False)
-    for _ in range(3):
+    r = build_roll_shape(eng, scheme, slots, shape, graph=False)
+    for _ in range(6 if overlap else 3):
         r.step()
     torch.cuda.synchronize()
     eng.kernel_stats(reset=True)
@@ -609,7 +628,7 @@ def profile_rolling(torch, eng, scheme, n, steps, seed, ipw=None):
     torch.cuda.synchronize()
     eng.set_profiling(False)
     stats = eng.kernel_stats(reset=True)
-    for _ in range(2):
+    for _ in range(6 if overlap else 2):  # every batch in flight finished
         r.step()
     torch.cuda.synchronize()
     ok = all(bool(torch.equal(sl["ss"], sl["ss2"])) for sl in slots)
@@ -792,6 +811,7 @@ CRITICAL_PERMS = {"sab_pipe_x0": 24, "sab_pipe_x1": 14, "sab_pipe_x2": 10,
                   "sab_mix_x0": 24, "sab_mix_x1": 10, "sab_mix_x0,l2": 11, "sab_mix_x1,l2": 7,
                   "sab_mix_x0,l4": 41, "sab_mix_x1,l4": 12,
                   "sab_roll_x0": 24, "sab_roll_x0,l2": 11, "sab_roll_x0,l4": 41,
+                  "sab_roll_fused": 24, "sab_roll_fused,l2": 11, "sab_roll_fused,l4": 41,
                   "frodo_roll_x0": 347, "frodo_roll_x0,640": 182, "frodo_roll_x0,1344": 478}
 LANE_PAIR_PERM_FLOOR_S = 24 * 108 * 4 / 2.4e9
 
@@ -1056,8 +1076,12 @@ def main():
                                ipw=ipw_fixed)
     progress(f"saber: {sab['ops_per_s']:.0f} ops/s (product ipw {sab['schedule']['saber_poly_ipw']})")
     progress("saber: the rolling steps' kernels launched without a graph, with per-kernel HIP events")
+    sab_shape = (sab["schedule"]["overlap"], sab["schedule"]["saber_poly_ipw"])
     sab_prof = profile_rolling(torch, eng, D.SABER, args.saber_batch, max(3, args.steps), 0x44504B454D000001,
-                               sab["schedule"]["saber_poly_ipw"])
+                               sab_shape)
+    # the same step's launches alone (three-slot form, chains then products)
+    sab_alone = sab_prof if not sab_shape[0] else profile_rolling(
+        torch, eng, D.SABER, args.saber_batch, max(3, args.steps), 0x44504B454D000001, (False, sab_shape[1]))
     progress(f"saber batch {args.saber_batch}: one batch's keygen -> encaps -> decaps chained inside each step")
     chained = run_plan_workload(torch, eng, D.SABER, args.saber_batch, args.steps, args.warmup, 0x44504B454D000001,
                                 schedule=sched)
@@ -1082,9 +1106,9 @@ def main():
         "config": {"workload": f"saber-l3 batch {args.saber_batch}: {args.saber_batch} keygens + "
                                f"{args.saber_batch} encaps + {args.saber_batch} decaps per step; "
                                + ("rolling pipeline over six batch slots (encaps to the keys of two steps earlier, "
-                                  "decaps of the ciphertexts of two steps earlier; each step's hash launch runs "
-                                  "beside the previous step's products; a batch is complete four steps after its "
-                                  "keygen)" if sab["schedule"].get("overlap") else
+                                  "decaps of the ciphertexts of two steps earlier; each step is one launch holding "
+                                  "its hash chains and the previous step's products; a batch is complete five "
+                                  "steps after its keygen step)" if sab["schedule"].get("overlap") else
                                   "rolling pipeline over three batch slots (encaps to the keys of the previous "
                                   "step, decaps of the ciphertexts of the previous step; a batch is complete two "
                                   "steps after its keygen)"),
@@ -1094,12 +1118,14 @@ def main():
                        "batch_p50": round(pct(sab["batch_ms"], 0.5), 4),
                        "batch_p99": round(pct(sab["batch_ms"], 0.99), 4),
                        "batch_what": f"one batch from keygen to decaps: {sab['batch_steps']} consecutive steps"},
-        "roundtrip_ok": sab["roundtrip_ok"] and sab_prof["roundtrip_ok"] and sab_eager["roundtrip_ok"]
+        "roundtrip_ok": sab["roundtrip_ok"] and sab_prof["roundtrip_ok"] and sab_alone["roundtrip_ok"]
+                        and sab_eager["roundtrip_ok"]
                         and chained["roundtrip_ok"] and chained_prof["roundtrip_ok"],
         "schedule": sab["schedule"],
         "eager_ops_per_s": round(sab_eager["ops_per_s"], 1),
         "roofline": roofline_from(sab_prof["kernels"], args.saber_batch, perm_s),
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_prof["kernels"].items()},
+        "alone_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_alone["kernels"].items()},
         "eager_kernels_ms": {k: round(v[1] / v[0], 4) for k, v in sab_eager["kernels"].items()},
         "saber_chained": {
             "workload": f"saber-l3 batch {args.saber_batch}: keygen -> encaps -> decaps of one batch chained "
@@ -1227,7 +1253,7 @@ def main():
         fam = run_rolling_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup,
                                    0x44504B454D000008 + sc, ipw=ipw_fixed)
         fam_prof = profile_rolling(torch, eng, sc, args.saber_batch, 3, 0x44504B454D000008 + sc,
-                                   fam["schedule"]["saber_poly_ipw"])
+                                   (fam["schedule"]["overlap"], fam["schedule"]["saber_poly_ipw"]))
         fam_ch = run_plan_workload(torch, eng, sc, args.saber_batch, args.steps, args.warmup,
                                    0x44504B454D000008 + sc, tune_budget_s=20.0, schedule=sched)
         result[name] = {"workload": f"{name} batch {args.saber_batch}: keygen+encaps+decaps per step (rolling)",
